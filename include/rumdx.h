@@ -1,0 +1,190 @@
+/*
+ * rumdx.h -- C ABI of the MI355X (gfx950) implementation of RUMD's per-timestep
+ * hot path: LJ 12-6 pair forces, neighbour lists with cell sorting, the
+ * Nose-Hoover leapfrog integrator and the FP64 block reductions.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative RUMDX_E* code on
+ *    failure; rumdx_last_error() then returns a thread-local message.  This
+ *    replaces RUMD's RUMD_Error exceptions (include/rumd/RUMD_Error.h:6-31),
+ *    which the C++ layer above re-raises.
+ *  - Pointers named *_dev are device pointers; everything else is host memory.
+ *  - Per-particle arrays use RUMD's float4 layout (include/rumd/ParticleData.h:41-62)
+ *    and are indexed by storage slot (the cell-sorted order after a sorted
+ *    rebuild); unsorted_index[slot] is the original particle id
+ *    (src/ParticleData.cu:383-393).
+ *  - `stream` is a hipStream_t (NULL = the default stream).  No entry point
+ *    synchronises, allocates or frees memory; scratch comes from buffers the
+ *    caller sized with the *_bytes() helpers.
+ *  - The box is RUMD's float[8] {cubic, Lx, Ly, Lz, 1/Lx, 1/Ly, 1/Lz, shift}
+ *    (include/rumd/SimulationBox.h:165-178), passed by value from host memory.
+ */
+#ifndef RUMDX_H
+#define RUMDX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RUMDX_VERSION 1
+
+#define RUMDX_OK 0
+#define RUMDX_E_INVALID -1   /* bad argument (RUMD_Error) */
+#define RUMDX_E_RESOURCE -2  /* launch resources exceeded (RUMD_cudaResourceError) */
+#define RUMDX_E_HIP -3       /* HIP runtime error */
+
+/* PairPotential::CutoffMethod (include/rumd/PairPotential.h:35) */
+#define RUMDX_CUTOFF_NS 0
+#define RUMDX_CUTOFF_SP 1
+#define RUMDX_CUTOFF_SF 2
+
+/* Number of floats per type pair in the parameter table (include/rumd/rumd_base.h:28). */
+#define RUMDX_NUM_PARAM 12
+/* Largest number of particle types a launch accepts (thermostat mask is 32 bits,
+ * src/IntegratorNVT.cu:157-174). */
+#define RUMDX_MAX_TYPES 32
+
+/* Slots of rumdx_nblist.flags_dev (device uint32[8]). */
+#define RUMDX_FLAG_REBUILD 0   /* != 0: the next build call rebuilds (updateRequired[0]) */
+#define RUMDX_FLAG_OVERFLOW 1  /* sticky: some list hit max_nbrs (updateRequired[1]) */
+#define RUMDX_FLAG_BUILDS 2    /* number of completed builds */
+#define RUMDX_FLAG_MAXLEN 3    /* longest list seen since the flag was cleared */
+#define RUMDX_FLAG_DONE 4      /* internal completion counter, zero between calls */
+
+/* Device-resident particle arrays (ParticleData, include/rumd/ParticleData.h:41-62). */
+typedef struct rumdx_particles {
+  float* r_dev;     /* nvp x 4: x, y, z, type (int bits in the float) */
+  float* v_dev;     /* nvp x 4: vx, vy, vz, 1/m */
+  float* f_dev;     /* nvp x 4: fx, fy, fz, u_i (pair energy, already halved) */
+  float* w_dev;     /* nvp x 4: w_mol, sxz, sxy, w_i (sum_j r_ij^2 s_ij) */
+  float* im_dev;    /* nvp x 4: image counts stored as floats */
+  float* sts_dev;   /* nvp x 4: sxx, syy, szz, syz; may be NULL when stresses are never requested */
+  float* rlo_dev;   /* nvp x 4: low-order position parts (FP64 position bookkeeping); NULL = plain FP32 positions */
+  uint32_t* unsorted_index_dev; /* nvp: original particle id of each slot */
+  uint32_t n;       /* real particles */
+  uint32_t nvp;     /* allocated (padded) particles, >= n */
+} rumdx_particles;
+
+/* Neighbour-list state (NeighborList, include/rumd/NeighborList.h).  The list is
+ * column-major like RUMD's: entry k of slot i is nbl_dev[k*nvp + i]
+ * (src/PairPotential.cu:360-367).  num_nbrs_dev[i] == max_nbrs marks a list that
+ * overflowed; the force kernel then rescans the cells for that particle. */
+typedef struct rumdx_nblist {
+  uint32_t* nbl_dev;        /* max_nbrs x nvp */
+  uint32_t* num_nbrs_dev;   /* nvp */
+  float* last_r_dev;        /* nvp x 4: positions at the last build */
+  uint32_t* flags_dev;      /* 8 x uint32, see RUMDX_FLAG_* */
+  const float* cut_skin2_dev; /* ntypes^2: (rc_ij + skin)^2 formed in FP32 (src/NeighborList.cu:1239-1240) */
+  uint32_t max_nbrs;
+  uint32_t ntypes;
+  float skin;
+  /* Cell machinery for the sorted method (NULL/0 for the N2 method). */
+  int32_t nc[3];            /* cells per axis, from rumdx_cell_grid() */
+  const uint32_t* cell_rank_dev; /* nc0*nc1*nc2: linear cell -> storage rank (Morton order) */
+  uint32_t* cell_start_dev; /* ncells + 1, by rank: first slot of each cell */
+  uint32_t* cell_count_dev; /* ncells, by rank: zero between builds */
+  uint32_t* slot_cell_dev;  /* nvp: linear cell index of each slot at the last build */
+  void* sort_scratch_dev;   /* rumdx_sort_scratch_bytes(nvp, ncells) bytes */
+} rumdx_nblist;
+
+/* Device-resident Nose-Hoover state (IntegratorNVT, src/IntegratorNVT.cu:82-92, 426-437). */
+typedef struct rumdx_nh_state {
+  double ps;          /* thermostat state Ps (d_thermostatState) */
+  double target_T;    /* target temperature, advanced by dT_per_step after each update */
+  double dT_per_step; /* SetTemperatureChangePerTimeStep */
+  double ke2_sum;     /* FP64 sum of m v^2(t+h/2) over thermostatted particles, last step */
+} rumdx_nh_state;
+
+/* ---------------------------------------------------------------- misc */
+int rumdx_version(void);
+const char* rumdx_last_error(void);
+/* Bytes of device scratch the reduction/integrator entry points need for n particles. */
+size_t rumdx_workspace_bytes(uint32_t n);
+/* Bytes of sort scratch a sorted build needs. */
+size_t rumdx_sort_scratch_bytes(uint32_t nvp, uint32_t ncells);
+
+/* ---------------------------------------------------------------- cells
+ * Cell grid of NeighborList::CalculateNBL_Sorting (src/NeighborList.cu:717-726):
+ * nc = max(5, floor(2 L / (max_cutoff + skin))) per axis, computed in FP32. */
+int rumdx_cell_grid(const float box[8], float max_cutoff, float skin, int32_t nc_out[3]);
+/* Storage rank of every linear cell (x fastest, src/NeighborList.cu:1136-1140):
+ * cells are stored in Morton (Z-order) order of their coordinates.  Host output. */
+int rumdx_cell_rank_table(const int32_t nc[3], uint32_t* rank_out);
+
+/* ---------------------------------------------------------------- neighbour list
+ * NeighborList::CheckRebuildRequired / displacement_since_nb_build
+ * (src/NeighborList.cu:569-581, 970-997): sets flags[REBUILD] if any particle
+ * moved more than skin/2 (FP32 minimum image, strict >). */
+int rumdx_nblist_check(const rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
+
+/* NeighborList::ResetNeighborList (src/NeighborList.cu:504-509): marks the list
+ * stale so that the next build call rebuilds it (a one-thread kernel, so it is
+ * safe inside captured graphs). */
+int rumdx_nblist_request_rebuild(rumdx_nblist* nb, void* stream);
+
+/* NeighborList::UpdateNBlist, N2 method (calculateNBL_N2, src/NeighborList.cu:1001-1099).
+ * Does nothing on the device unless flags[REBUILD] is set. */
+int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
+
+/* NeighborList::UpdateNBlist, sort method: cell binning, spatial sort of every
+ * particle array and the cell-stencil build (CalculateNBL_Sorting,
+ * src/NeighborList.cu:717-781, 1142-1319; Sample::SortParticlesByKey,
+ * include/rumd/Sample.h:174-188; ParticleData::UpdateAfterSorting,
+ * src/ParticleData.cu:493-528).  Particles are stored by cell in Morton order of
+ * the cells and by original slot inside a cell (a stable sort).  Does nothing
+ * on the device unless flags[REBUILD] is set. */
+int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
+
+/* ---------------------------------------------------------------- pair forces
+ * Pot_LJ_12_6 / Pot_IPL_12_6 force, energy and virial (PairPotential::CalcF,
+ * kernels Calcf_NBL_tp_equal_one / Calcf_NBL_tp, src/PairPotential.cu:330-515;
+ * fij, include/rumd/PairPotential.h:1198-1244).  params_dev is RUMD's table
+ * ntypes*ntypes*12 floats (param 0 = Rcut, 1 = A12, 2 = A6, 10 = s(rc)*rc,
+ * 11 = u(rc); src/PairPotential.cu:82-125).  nb == NULL sums over all pairs
+ * (NB method "none").  initialize != 0 stores f/w (and sts), otherwise adds.
+ * Pair arithmetic is FP32; the minimum-image displacement is formed in FP64. */
+int rumdx_pair_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* params_dev,
+                  uint32_t ntypes, const float box[8], int cutoff_method, int initialize,
+                  int calc_stresses, void* stream);
+
+/* ---------------------------------------------------------------- integrator
+ * IntegratorNVT::Integrate (integrateNVTAlgorithm, src/IntegratorNVT.cu:64-79,
+ * 364-418) followed, when thermostat_on_mask != 0, by UpdateThermostatState
+ * (src/IntegratorNVT.cu:82-92, 426-437) with the kinetic-energy sum in FP64 and
+ * Ps, T in FP64.  omega2 = 4 pi^2 / tau^2, dof = degrees of freedom.  When
+ * p->rlo_dev != NULL positions are advanced and wrapped in FP64 and kept as
+ * hi/lo float pairs.  When nb != NULL the skin check of rumdx_nblist_check is
+ * fused in (flags[REBUILD] set for the next build). */
+int rumdx_nvt_step(rumdx_particles* p, rumdx_nh_state* nh_dev, float dt, uint32_t thermostat_on_mask,
+                   float omega2, uint32_t dof, const float box[8], rumdx_nblist* nb,
+                   void* workspace_dev, void* stream);
+
+/* IntegratorNVT::SetMomentumToZero (src/IntegratorNVT.cu:176-183, 444-472):
+ * subtracts the centre-of-mass velocity, with the momentum sum in FP64. */
+int rumdx_zero_momentum(rumdx_particles* p, void* workspace_dev, void* stream);
+
+/* IntegratorNVT::GetKineticEnergy (src/IntegratorNVT.cu:316-356): the leapfrog
+ * average 1/2 sum m (v^2(t-h/2) + v^2(t+h/2))/2 with v(t+h/2) predicted from the
+ * current forces and Ps.  Writes one double to out_dev. */
+int rumdx_kinetic_energy(const rumdx_particles* p, const rumdx_nh_state* nh_dev, float dt,
+                         uint32_t thermostat_on_mask, double* out_dev, void* workspace_dev,
+                         void* stream);
+
+/* ---------------------------------------------------------------- reductions
+ * FP64 sums of float4 components (the sums behind Sample::GetPotentialEnergy,
+ * GetVirial, src/Sample.cu:353-391, and sumIdenticalArrays,
+ * src/rumd_algorithms.cu:162-250).  out_dev[c] = sum_i arr[i].c for the
+ * components whose bit is set in comp_mask (bit 0 = x ... bit 3 = w); other
+ * outputs are written as 0.  The summation order is fixed, so results are
+ * bitwise reproducible. */
+int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_mask, double* out_dev,
+                    void* workspace_dev, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RUMDX_H */

@@ -1,0 +1,642 @@
+/*
+ * rumd_oracle.c -- CPU restatement of RUMD's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path and the
+ * CPU baseline timed beside it; nothing in the product (rumd_amd/) links,
+ * loads or calls it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it (through oracle/oracle.py).
+ *
+ * Two kinds of function live here:
+ *   - FP64 restatements of the physics (pair force/energy/virial, leapfrog with
+ *     Nose-Hoover, kinetic energy, sums), used to bound the FP32 kernels' error;
+ *   - FP32-exact restatements (source order, no FMA contraction: the file is
+ *     compiled with -ffp-contract=off) of the arithmetic whose results must
+ *     match bit for bit: the neighbour-list criterion and cell binning, and the
+ *     FP32 leapfrog update.
+ *
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *   orc_forces_brute / orc_forces_nl / orc_forces_cells: RUMD-written golden
+ *     configurations (Conf/KobAndersen/KA_N1000_T0.466, KA_N256_L6.0_T0.50,
+ *     Conf/KobAndersenAlt/KA_N864_L9.0_T0.50AltMass; forces, pe, vir columns),
+ *     LJ_N1000_rho0.80_T0.80 at r - v dt, closed forms for SF (F(rc)=0,
+ *     u(rc)=0, -du/dr = F by finite differences), Newton's third law.
+ *   orc_nblist_*: brute force vs cell scan, symmetry, superset of the rc sphere.
+ *   orc_nvt_step: closed form for one particle, Ps = 0 => NVE, time
+ *     reversibility, the r(t+dt) = r + v(t+dt/2) dt relation held by the
+ *     LJ_N1000 fixture.
+ *   orc_kinetic_energy, orc_thermostat_update, orc_sum_f4: closed forms,
+ *     integer-valued inputs.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NUM_PARAM 12
+
+/* ------------------------------------------------------------------ helpers */
+
+static inline uint32_t type_of(const float* r4) {
+  uint32_t t;
+  memcpy(&t, r4 + 3, sizeof t);
+  return t;
+}
+
+/* FP64 minimum image (RectangularSimulationBox::calculateDistance,
+ * include/rumd/SimulationBox.h:111-125, in double). */
+static inline double min_image_f64(const float* a, const float* b, const double L[3], double d[3]) {
+  double s = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double x = (double)a[k] - (double)b[k];
+    x -= L[k] * rint(x / L[k]);
+    d[k] = x;
+    s += x * x;
+  }
+  return s;
+}
+
+/* FP32 minimum image exactly as the reference writes it (uncontracted). */
+static inline float min_image_d2_f32(const float* a, const float* b, const float L[3], const float iL[3]) {
+  float dx = a[0] - b[0];
+  float dy = a[1] - b[1];
+  float dz = a[2] - b[2];
+  dx -= L[0] * rintf(dx * iL[0]);
+  dy -= L[1] * rintf(dy * iL[1]);
+  dz -= L[2] * rintf(dz * iL[2]);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+/* ------------------------------------------------------------------ pair term
+ * Pot_IPL_12_6::ComputeInteraction (include/rumd/PairPotential.h:161-168) and
+ * fij (include/rumd/PairPotential.h:1198-1244) in FP64.  param: RUMD row
+ * (0 = Rcut, 1 = A12, 2 = A6).  The shifts u(rc) and s(rc) rc are formed here
+ * in FP64 from the same float A12, A6, Rcut (CopyParamsToGPU,
+ * src/PairPotential.cu:82-125, does this in float). */
+typedef struct {
+  double rc, rc2, A12, A6, ucut, src; /* src = s(rc) * rc */
+} pair_par;
+
+static pair_par make_par(const float* param) {
+  pair_par p;
+  p.rc = (double)param[0];
+  p.rc2 = p.rc * p.rc;
+  p.A12 = (double)param[1];
+  p.A6 = (double)param[2];
+  if (p.rc > 0.0) {
+    const double i2 = 1.0 / p.rc2, i6 = i2 * i2 * i2;
+    p.ucut = i6 * (p.A12 * i6 + p.A6);
+    p.src = i6 * i2 * (12.0 * p.A12 * i6 + 6.0 * p.A6) * p.rc;
+  } else {
+    p.ucut = 0.0;
+    p.src = 0.0;
+  }
+  return p;
+}
+
+/* Adds the (i,j) term to fi[0..2], u, vir.  Returns 1 if inside the cut-off. */
+static inline int pair_add(const pair_par* p, int cutoff, const double d[3], double d2, double* fi, double* u,
+                           double* vir) {
+  if (!(d2 < p->rc2)) return 0;
+  const double i2 = 1.0 / d2, i6 = i2 * i2 * i2, t = p->A12 * i6;
+  double s = i6 * i2 * (12.0 * t + 6.0 * p->A6);
+  double e = i6 * (t + p->A6);
+  if (cutoff == 1) {        /* SP */
+    e -= p->ucut;
+  } else if (cutoff == 2) { /* SF */
+    const double r = sqrt(d2);
+    s -= p->src / r;
+    e += p->src * (r - p->rc) - p->ucut;
+  }
+  fi[0] += d[0] * s;
+  fi[1] += d[1] * s;
+  fi[2] += d[2] * s;
+  *u += e;
+  *vir += d2 * s;
+  return 1;
+}
+
+static pair_par* make_table(int nt, const float* params) {
+  pair_par* t = (pair_par*)malloc(sizeof(pair_par) * (size_t)nt * nt);
+  for (int k = 0; k < nt * nt; ++k) t[k] = make_par(params + (size_t)k * NUM_PARAM);
+  return t;
+}
+
+/* ------------------------------------------------------------------ forces
+ * Output per particle: f4[4i..4i+3] = (Fx, Fy, Fz, 1/2 sum_j u_ij),
+ * vir[i] = sum_j r_ij^2 s_ij (RUMD's w.w; the xyz writer stores vir/2). */
+
+/* O(N^2) over all pairs (the nbl == 0 branch of Calcf_NBL_tp). */
+void orc_forces_brute(int n, const float* pos4, const double L[3], int nt, const float* params, int cutoff,
+                      double* f4, double* vir) {
+  pair_par* tab = make_table(nt, params);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    double fi[3] = {0, 0, 0}, u = 0, w = 0, d[3];
+    const uint32_t ti = type_of(pos4 + 4 * (size_t)i);
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double d2 = min_image_f64(pos4 + 4 * (size_t)i, pos4 + 4 * (size_t)j, L, d);
+      pair_add(&tab[ti * nt + type_of(pos4 + 4 * (size_t)j)], cutoff, d, d2, fi, &u, &w);
+    }
+    f4[4 * (size_t)i] = fi[0];
+    f4[4 * (size_t)i + 1] = fi[1];
+    f4[4 * (size_t)i + 2] = fi[2];
+    f4[4 * (size_t)i + 3] = 0.5 * u;
+    vir[i] = w;
+  }
+  free(tab);
+}
+
+/* Over a given neighbour list in CSR form (offsets[n+1], idx). */
+void orc_forces_nl(int n, const float* pos4, const double L[3], int nt, const float* params, int cutoff,
+                   const int64_t* offsets, const int32_t* idx, double* f4, double* vir) {
+  pair_par* tab = make_table(nt, params);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    double fi[3] = {0, 0, 0}, u = 0, w = 0, d[3];
+    const uint32_t ti = type_of(pos4 + 4 * (size_t)i);
+    for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+      const int j = idx[k];
+      const double d2 = min_image_f64(pos4 + 4 * (size_t)i, pos4 + 4 * (size_t)j, L, d);
+      pair_add(&tab[ti * nt + type_of(pos4 + 4 * (size_t)j)], cutoff, d, d2, fi, &u, &w);
+    }
+    f4[4 * (size_t)i] = fi[0];
+    f4[4 * (size_t)i + 1] = fi[1];
+    f4[4 * (size_t)i + 2] = fi[2];
+    f4[4 * (size_t)i + 3] = 0.5 * u;
+    vir[i] = w;
+  }
+  free(tab);
+}
+
+/* ------------------------------------------------------------------ cell lists (FP64)
+ * Linked-cell bins of width >= rcut_max (27-cell stencil).  Used for FP64 forces
+ * at large N and by the CPU baseline; a plain acceleration of the all-pairs sum
+ * (requires at least 3 cells per axis). */
+typedef struct {
+  int nc[3];
+  int* head; /* per cell, first particle or -1 */
+  int* next; /* per particle */
+} cell_list;
+
+static int cell_index_f64(const double x[3], const double L[3], const int nc[3]) {
+  int c[3];
+  for (int k = 0; k < 3; ++k) {
+    double s = x[k] / L[k] + 0.5;
+    s -= floor(s);
+    int q = (int)(s * nc[k]);
+    if (q >= nc[k]) q = nc[k] - 1;
+    if (q < 0) q = 0;
+    c[k] = q;
+  }
+  return c[0] + nc[0] * (c[1] + nc[1] * c[2]);
+}
+
+static int build_cells(cell_list* cl, int n, const float* pos4, const double L[3], double width) {
+  for (int k = 0; k < 3; ++k) {
+    cl->nc[k] = (int)floor(L[k] / width);
+    if (cl->nc[k] < 3) return -1;
+  }
+  const size_t nc = (size_t)cl->nc[0] * cl->nc[1] * cl->nc[2];
+  cl->head = (int*)malloc(sizeof(int) * nc);
+  cl->next = (int*)malloc(sizeof(int) * (size_t)n);
+  for (size_t c = 0; c < nc; ++c) cl->head[c] = -1;
+  for (int i = n - 1; i >= 0; --i) {
+    const double x[3] = {pos4[4 * (size_t)i], pos4[4 * (size_t)i + 1], pos4[4 * (size_t)i + 2]};
+    const int c = cell_index_f64(x, L, cl->nc);
+    cl->next[i] = cl->head[c];
+    cl->head[c] = i;
+  }
+  return 0;
+}
+
+static void free_cells(cell_list* cl) {
+  free(cl->head);
+  free(cl->next);
+}
+
+/* FP64 forces through a linked-cell list; returns -1 if the box is too small. */
+int orc_forces_cells(int n, const float* pos4, const double L[3], int nt, const float* params, int cutoff,
+                     double* f4, double* vir) {
+  pair_par* tab = make_table(nt, params);
+  double rmax = 0.0;
+  for (int k = 0; k < nt * nt; ++k)
+    if (tab[k].rc > rmax) rmax = tab[k].rc;
+  cell_list cl;
+  if (build_cells(&cl, n, pos4, L, rmax) != 0) {
+    free(tab);
+    return -1;
+  }
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    double fi[3] = {0, 0, 0}, u = 0, w = 0, d[3];
+    const float* ri = pos4 + 4 * (size_t)i;
+    const uint32_t ti = type_of(ri);
+    const double x[3] = {ri[0], ri[1], ri[2]};
+    const int c = cell_index_f64(x, L, cl.nc);
+    const int cx = c % cl.nc[0], cy = (c / cl.nc[0]) % cl.nc[1], cz = c / (cl.nc[0] * cl.nc[1]);
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int ox = (cx + dx + cl.nc[0]) % cl.nc[0];
+          const int oy = (cy + dy + cl.nc[1]) % cl.nc[1];
+          const int oz = (cz + dz + cl.nc[2]) % cl.nc[2];
+          for (int j = cl.head[ox + cl.nc[0] * (oy + cl.nc[1] * oz)]; j >= 0; j = cl.next[j]) {
+            if (j == i) continue;
+            const double d2 = min_image_f64(ri, pos4 + 4 * (size_t)j, L, d);
+            pair_add(&tab[ti * nt + type_of(pos4 + 4 * (size_t)j)], cutoff, d, d2, fi, &u, &w);
+          }
+        }
+    f4[4 * (size_t)i] = fi[0];
+    f4[4 * (size_t)i + 1] = fi[1];
+    f4[4 * (size_t)i + 2] = fi[2];
+    f4[4 * (size_t)i + 3] = 0.5 * u;
+    vir[i] = w;
+  }
+  free_cells(&cl);
+  free(tab);
+  return 0;
+}
+
+/* ------------------------------------------------------------------ neighbour lists (FP32 exact)
+ * box8: RUMD float[8] box.  cut_skin2[ti*nt+tj] = (rc_ij + skin)^2 formed in
+ * float (src/NeighborList.cu:1239-1240).  Results are CSR over the input order:
+ * counts[n] then, when idx != NULL, offsets/idx.  Entries inside a row are in
+ * ascending j for the brute list and in RUMD's stencil order for the cell list. */
+
+static void box_f32(const float box8[8], float L[3], float iL[3]) {
+  const int cubic = ((unsigned)box8[0]) != 0u;
+  for (int k = 0; k < 3; ++k) {
+    L[k] = cubic ? box8[1] : box8[1 + k];
+    iL[k] = cubic ? box8[4] : box8[4 + k];
+  }
+}
+
+/* calculateNBL_N2 (src/NeighborList.cu:1001-1099): all j != i with
+ * d^2 < (rc_ij + skin)^2, FP32. */
+void orc_nblist_brute_f32(int n, const float* pos4, const float box8[8], int nt, const float* cut_skin2,
+                          int32_t* counts, const int64_t* offsets, int32_t* idx) {
+  float L[3], iL[3];
+  box_f32(box8, L, iL);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    const uint32_t ti = type_of(pos4 + 4 * (size_t)i);
+    int c = 0;
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const float d2 = min_image_d2_f32(pos4 + 4 * (size_t)i, pos4 + 4 * (size_t)j, L, iL);
+      if (d2 < cut_skin2[ti * nt + type_of(pos4 + 4 * (size_t)j)]) {
+        if (idx) idx[offsets[i] + c] = j;
+        ++c;
+      }
+    }
+    counts[i] = c;
+  }
+}
+
+/* Cell grid of CalculateNBL_Sorting (src/NeighborList.cu:717-726), FP32. */
+void orc_cell_grid(const float box8[8], float max_cutoff, float skin, int32_t nc[3]) {
+  float L[3], iL[3];
+  box_f32(box8, L, iL);
+  for (int k = 0; k < 3; ++k) {
+    int q = (int)floorf(2.0f * L[k] / (max_cutoff + skin));
+    nc[k] = q < 5 ? 5 : q;
+  }
+}
+
+/* calculateCellCoordinates (src/NeighborList.cu:1106-1133), FP32. */
+static int cell_coord_f32(float x, float iL, int nc) {
+  const float s = x * iL + 0.5f;
+  int c = (int)((float)nc * s);
+  if (c >= nc) c = nc - 1;
+  if (c < 0) c = 0;
+  return c;
+}
+
+void orc_cell_of(int n, const float* pos4, const float box8[8], const int32_t nc[3], int32_t* lin) {
+  float L[3], iL[3];
+  box_f32(box8, L, iL);
+  for (int i = 0; i < n; ++i) {
+    const float* r = pos4 + 4 * (size_t)i;
+    const int cx = cell_coord_f32(r[0], iL[0], nc[0]);
+    const int cy = cell_coord_f32(r[1], iL[1], nc[1]);
+    const int cz = cell_coord_f32(r[2], iL[2], nc[2]);
+    lin[i] = cx + cy * nc[0] + cz * nc[0] * nc[1];
+  }
+}
+
+/* calculateNBL_CellsSorted (src/NeighborList.cu:1204-1319): bin by cell, scan
+ * the 5x5x5 stencil in (dZ, dY, dX) order, particles of a cell in ascending
+ * index, FP32 criterion. */
+void orc_nblist_cells_f32(int n, const float* pos4, const float box8[8], int nt, const float* cut_skin2,
+                          const int32_t nc[3], int32_t* counts, const int64_t* offsets, int32_t* idx) {
+  float L[3], iL[3];
+  box_f32(box8, L, iL);
+  const int ncell = nc[0] * nc[1] * nc[2];
+  int32_t* lin = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  int32_t* start = (int32_t*)calloc((size_t)ncell + 1, sizeof(int32_t));
+  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+  orc_cell_of(n, pos4, box8, nc, lin);
+  for (int i = 0; i < n; ++i) start[lin[i] + 1]++;
+  for (int c = 0; c < ncell; ++c) start[c + 1] += start[c];
+  {
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * (size_t)ncell);
+    memcpy(cur, start, sizeof(int32_t) * (size_t)ncell);
+    for (int i = 0; i < n; ++i) order[cur[lin[i]]++] = i; /* stable: ascending i inside a cell */
+    free(cur);
+  }
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; ++i) {
+    const float* ri = pos4 + 4 * (size_t)i;
+    const uint32_t ti = type_of(ri);
+    const int cx = cell_coord_f32(ri[0], iL[0], nc[0]);
+    const int cy = cell_coord_f32(ri[1], iL[1], nc[1]);
+    const int cz = cell_coord_f32(ri[2], iL[2], nc[2]);
+    int cnt = 0;
+    for (int dz = -2; dz <= 2; ++dz) {
+      const int oz = (cz + dz + nc[2]) % nc[2];
+      for (int dy = -2; dy <= 2; ++dy) {
+        const int oy = (cy + dy + nc[1]) % nc[1];
+        for (int dx = -2; dx <= 2; ++dx) {
+          const int ox = (cx + dx + nc[0]) % nc[0];
+          const int c = ox + oy * nc[0] + oz * nc[0] * nc[1];
+          for (int k = start[c]; k < start[c + 1]; ++k) {
+            const int j = order[k];
+            if (j == i) continue;
+            const float d2 = min_image_d2_f32(ri, pos4 + 4 * (size_t)j, L, iL);
+            if (d2 < cut_skin2[ti * nt + type_of(pos4 + 4 * (size_t)j)]) {
+              if (idx) idx[offsets[i] + cnt] = j;
+              ++cnt;
+            }
+          }
+        }
+      }
+    }
+    counts[i] = cnt;
+  }
+  free(lin);
+  free(start);
+  free(order);
+}
+
+/* displacement_since_nb_build (src/NeighborList.cu:970-997): 1 if any particle
+ * moved more than skin/2 since last_r (strict >, FP32). */
+int orc_skin_exceeded(int n, const float* pos4, const float* last4, const float box8[8], float skin) {
+  float L[3], iL[3];
+  box_f32(box8, L, iL);
+  const float s24 = (float)(0.25 * (double)skin * (double)skin);
+  int any = 0;
+  for (int i = 0; i < n; ++i)
+    if (min_image_d2_f32(pos4 + 4 * (size_t)i, last4 + 4 * (size_t)i, L, iL) > s24) any = 1;
+  return any;
+}
+
+/* ------------------------------------------------------------------ integrator
+ * integrateNVTAlgorithm (src/IntegratorNVT.cu:364-418), FP32 in source order.
+ * rlo4 != NULL: positions advanced and wrapped in FP64 from hi + lo and stored
+ * back as a hi/lo float pair (the FP64 position bookkeeping variant).
+ * ke2_out (may be NULL) receives sum over thermostatted particles of
+ * (v.v)/v.w in FP64 (the thermostat's kinetic-energy sum). */
+void orc_nvt_step(int n, float* pos4, float* vel4, const float* f4, float* im4, float* rlo4, const float box8[8],
+                  float dt, double ps, uint32_t on_mask, double* ke2_out) {
+  float L[3], iL[3];
+  box_f32(box8, L, iL);
+  const float ps_f = (float)ps;
+  long double ke2 = 0.0L;
+  for (int i = 0; i < n; ++i) {
+    float* r = pos4 + 4 * (size_t)i;
+    float* v = vel4 + 4 * (size_t)i;
+    const float* f = f4 + 4 * (size_t)i;
+    float* im = im4 + 4 * (size_t)i;
+    const int apply = on_mask != 0u && ((on_mask >> (type_of(r) & 31u)) & 1u);
+    const float lps = apply ? ps_f : 0.f;
+    const float factor = 0.5f * lps * dt;
+    const float plus = 1.f / (1.f + factor);
+    const float minus = 1.f - factor;
+    for (int k = 0; k < 3; ++k) v[k] = plus * (minus * v[k] + v[3] * f[k] * dt);
+    float img[3];
+    if (rlo4 == NULL) {
+      for (int k = 0; k < 3; ++k) r[k] += v[k] * dt;
+      for (int k = 0; k < 3; ++k) img[k] = rintf(r[k] * iL[k]);
+      for (int k = 0; k < 3; ++k) r[k] -= L[k] * img[k];
+    } else {
+      float* lo = rlo4 + 4 * (size_t)i;
+      for (int k = 0; k < 3; ++k) {
+        const double Ld = (double)L[k];
+        double x = (double)r[k] + (double)lo[k] + (double)v[k] * (double)dt;
+        const double q = rint(x * (1.0 / Ld));
+        x -= Ld * q;
+        r[k] = (float)x;
+        lo[k] = (float)(x - (double)r[k]);
+        img[k] = (float)q;
+      }
+    }
+    for (int k = 0; k < 3; ++k) im[k] += img[k];
+    if (apply) ke2 += (long double)((v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / v[3]);
+  }
+  if (ke2_out) *ke2_out = (double)ke2;
+}
+
+/* updateNVTThermostatState (src/IntegratorNVT.cu:426-437) in FP64, then the
+ * target-temperature ramp (src/IntegratorNVT.cu:91). */
+void orc_thermostat_update(double* ps, double* target_T, double dT_per_step, double ke2, uint32_t dof, float dt,
+                           float omega2) {
+  double dev = ke2 / ((double)dof * *target_T) - 1.0;
+  if (dev > 10.0) dev = 10.0;
+  *ps += (double)dt * (double)omega2 * dev;
+  *target_T += dT_per_step;
+}
+
+/* IntegratorNVT::GetKineticEnergy (src/IntegratorNVT.cu:316-356). */
+double orc_kinetic_energy(int n, const float* pos4, const float* vel4, const float* f4, float dt, double ps,
+                          uint32_t on_mask) {
+  long double tot = 0.0L;
+  for (int i = 0; i < n; ++i) {
+    const float* r = pos4 + 4 * (size_t)i;
+    const float* v = vel4 + 4 * (size_t)i;
+    const float* f = f4 + 4 * (size_t)i;
+    const double mass = (double)(1.f / v[3]);
+    const float tsim = v[3] * dt;
+    const int apply = on_mask != 0u && ((on_mask >> (type_of(r) & 31u)) & 1u);
+    const float factor = apply ? (float)(0.5f * dt * (on_mask ? ps : 0.0)) : 0.f;
+    const float plus = 1.f / (1.f + factor);
+    const float minus = 1.f - factor;
+    const float vf0 = plus * (minus * v[0] + tsim * f[0]);
+    const float vf1 = plus * (minus * v[1] + tsim * f[1]);
+    const float vf2 = plus * (minus * v[2] + tsim * f[2]);
+    const double vb = (double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double vfs = (double)(vf0 * vf0 + vf1 * vf1 + vf2 * vf2);
+    tot += (long double)(mass * ((vb + vfs) / 2.0));
+  }
+  return (double)(0.5L * tot);
+}
+
+/* Sum of one float4 component in extended precision (Sample::GetPotentialEnergy,
+ * GetVirial, src/Sample.cu:353-391). */
+double orc_sum_f4(int n, const float* a4, int comp) {
+  long double s = 0.0L;
+  for (int i = 0; i < n; ++i) s += (long double)a4[4 * (size_t)i + comp];
+  return (double)s;
+}
+
+/* ------------------------------------------------------------------ CPU baseline MD
+ * The reference's step loop (Simulation.Run, Python/rumd/Simulation.py:579-597)
+ * on the CPU in FP64: Verlet list with skin rebuilt when any particle moved
+ * more than skin/2 (linked cells of width >= rc_max + skin), pair forces over
+ * the list, Toxvaerd Nose-Hoover leapfrog, FP64 thermostat.  pos/vel are
+ * 3*n doubles, types/inv_mass per particle.  Returns the number of list
+ * builds; *pe_out gets the last potential energy. */
+int orc_md_run(int n, double* pos, double* vel, const int32_t* types, const double* inv_mass, const double L[3],
+               int nt, const float* params, int cutoff, double skin, double dt, int nsteps, int thermostat,
+               double target_T, double omega2, double* ps_io, double* pe_out) {
+  pair_par* tab = make_table(nt, params);
+  double rmax = 0.0;
+  for (int k = 0; k < nt * nt; ++k)
+    if (tab[k].rc > rmax) rmax = tab[k].rc;
+  const double rl = rmax + skin;
+  double* cut2 = (double*)malloc(sizeof(double) * (size_t)nt * nt);
+  for (int k = 0; k < nt * nt; ++k) cut2[k] = (tab[k].rc + skin) * (tab[k].rc + skin);
+  float* pos4 = (float*)malloc(sizeof(float) * 4 * (size_t)n);
+  double* last = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+  double* frc = (double*)malloc(sizeof(double) * 3 * (size_t)n);
+  double* pe = (double*)malloc(sizeof(double) * (size_t)n);
+  int* cnt = (int*)malloc(sizeof(int) * (size_t)n);
+  int cap = (int)(1.5 * 4.0 / 3.0 * M_PI * rl * rl * rl * n / (L[0] * L[1] * L[2])) + 64;
+  int32_t* nbl = (int32_t*)malloc(sizeof(int32_t) * (size_t)n * cap);
+  const int dof = 3 * n - 3;
+  int builds = 0, need = 1;
+  double ps = *ps_io;
+  for (int step = 0; step <= nsteps; ++step) {
+    if (!need) {
+      const double h2 = 0.25 * skin * skin;
+      for (int i = 0; i < n && !need; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          double x = pos[3 * (size_t)i + k] - last[3 * (size_t)i + k];
+          x -= L[k] * rint(x / L[k]);
+          s += x * x;
+        }
+        if (s > h2) need = 1;
+      }
+    }
+    if (need) {
+      for (int i = 0; i < n; ++i) {
+        for (int k = 0; k < 3; ++k) pos4[4 * (size_t)i + k] = (float)pos[3 * (size_t)i + k];
+        memcpy(pos4 + 4 * (size_t)i + 3, &types[i], sizeof(int32_t));
+        memcpy(last + 3 * (size_t)i, pos + 3 * (size_t)i, 3 * sizeof(double));
+      }
+      cell_list cl;
+      if (build_cells(&cl, n, pos4, L, rl) != 0) {
+        free(tab); free(cut2); free(pos4); free(last); free(frc); free(pe); free(cnt); free(nbl);
+        return -1;
+      }
+      int overflow = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(| : overflow)
+      for (int i = 0; i < n; ++i) {
+        const double* ri = pos + 3 * (size_t)i;
+        const double x[3] = {ri[0], ri[1], ri[2]};
+        const int c = cell_index_f64(x, L, cl.nc);
+        const int cx = c % cl.nc[0], cy = (c / cl.nc[0]) % cl.nc[1], cz = c / (cl.nc[0] * cl.nc[1]);
+        int m = 0;
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+              const int ox = (cx + dx + cl.nc[0]) % cl.nc[0];
+              const int oy = (cy + dy + cl.nc[1]) % cl.nc[1];
+              const int oz = (cz + dz + cl.nc[2]) % cl.nc[2];
+              for (int j = cl.head[ox + cl.nc[0] * (oy + cl.nc[1] * oz)]; j >= 0; j = cl.next[j]) {
+                if (j == i) continue;
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                  double d = ri[k] - pos[3 * (size_t)j + k];
+                  d -= L[k] * rint(d / L[k]);
+                  s += d * d;
+                }
+                if (s < cut2[types[i] * nt + types[j]]) {
+                  if (m < cap) nbl[(size_t)i * cap + m] = j;
+                  ++m;
+                }
+              }
+            }
+        if (m > cap) overflow = 1;
+        cnt[i] = m < cap ? m : cap;
+      }
+      free_cells(&cl);
+      if (overflow) {
+        free(tab); free(cut2); free(pos4); free(last); free(frc); free(pe); free(cnt); free(nbl);
+        return -2;
+      }
+      ++builds;
+      need = 0;
+    }
+    /* forces */
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; ++i) {
+      double fi[3] = {0, 0, 0}, u = 0, w = 0;
+      const double* ri = pos + 3 * (size_t)i;
+      for (int m = 0; m < cnt[i]; ++m) {
+        const int j = nbl[(size_t)i * cap + m];
+        double d[3], s2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+          double x = ri[k] - pos[3 * (size_t)j + k];
+          x -= L[k] * rint(x / L[k]);
+          d[k] = x;
+          s2 += x * x;
+        }
+        pair_add(&tab[types[i] * nt + types[j]], cutoff, d, s2, fi, &u, &w);
+      }
+      frc[3 * (size_t)i] = fi[0];
+      frc[3 * (size_t)i + 1] = fi[1];
+      frc[3 * (size_t)i + 2] = fi[2];
+      pe[i] = 0.5 * u;
+    }
+    if (step == nsteps) break;
+    /* leapfrog + Nose-Hoover */
+    const double factor = thermostat ? 0.5 * ps * dt : 0.0;
+    const double plus = 1.0 / (1.0 + factor), minus = 1.0 - factor;
+    double ke2 = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : ke2)
+    for (int i = 0; i < n; ++i) {
+      double vv = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        double* v = vel + 3 * (size_t)i + k;
+        *v = plus * (minus * *v + inv_mass[i] * frc[3 * (size_t)i + k] * dt);
+        double* x = pos + 3 * (size_t)i + k;
+        *x += *v * dt;
+        *x -= L[k] * rint(*x / L[k]);
+        vv += *v * *v;
+      }
+      ke2 += vv / inv_mass[i];
+    }
+    if (thermostat) {
+      double dev = ke2 / ((double)dof * target_T) - 1.0;
+      if (dev > 10.0) dev = 10.0;
+      ps += dt * omega2 * dev;
+    }
+  }
+  double pe_tot = 0.0;
+  for (int i = 0; i < n; ++i) pe_tot += pe[i];
+  *pe_out = pe_tot;
+  *ps_io = ps;
+  free(tab); free(cut2); free(pos4); free(last); free(frc); free(pe); free(cnt); free(nbl);
+  return builds;
+}
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_num_threads(int k) {
+#ifdef _OPENMP
+  if (k > 0) omp_set_num_threads(k);
+#else
+  (void)k;
+#endif
+}

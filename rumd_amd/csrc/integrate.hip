@@ -1,0 +1,346 @@
+// Integrator and FP64 reduction kernels (gfx950).
+//
+// Reference: IntegratorNVT::Integrate / integrateNVTAlgorithm
+// (src/IntegratorNVT.cu:64-79, 364-418), UpdateThermostatState /
+// updateNVTThermostatState (:82-92, 426-437), SetMomentumToZero with
+// calculateMomentum / zeroTotalMomentum (:176-183, 444-472), GetKineticEnergy
+// (:316-356); sumIdenticalArrays / reduce (src/rumd_algorithms.cu:162-250).
+//
+// The leapfrog update is one streaming pass.  The thermostat's kinetic-energy
+// sum is fused into it as an FP64 block reduction; the workgroup that finishes
+// last adds the block partials in a fixed order and advances Ps in FP64, so a
+// step is one launch and the sum is bitwise reproducible.  The reference
+// instead stores m v^2 per particle and reduces it in FP32 over ~4 launches.
+#include "rumdx_common.h"
+
+namespace rumdx {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  set_error(msg);
+  return code;
+}
+
+int check_launch(const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return RUMDX_OK;
+  const int code = (e == hipErrorLaunchOutOfResources || e == hipErrorInvalidConfiguration) ? RUMDX_E_RESOURCE : RUMDX_E_HIP;
+  return fail(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+// One particle of integrateNVTAlgorithm, in source order without FMA
+// contraction (reproducible bit for bit on the host).  With HILO the position
+// is advanced and wrapped in FP64 from the hi/lo pair and stored back as a pair.
+template <bool HILO>
+__device__ __forceinline__ float nvt_particle(float4& r, float4& v, const float4 f, float4& im, float4& lo, float ps,
+                                              float dt, const Box& bx) {
+#pragma clang fp contract(off)
+  const float factor = 0.5f * ps * dt;
+  const float plus = 1.f / (1.f + factor);
+  const float minus = 1.f - factor;
+  v.x = plus * (minus * v.x + v.w * f.x * dt);
+  v.y = plus * (minus * v.y + v.w * f.y * dt);
+  v.z = plus * (minus * v.z + v.w * f.z * dt);
+  float ix, iy, iz;
+  if (!HILO) {
+    r.x += v.x * dt;
+    r.y += v.y * dt;
+    r.z += v.z * dt;
+    ix = rintf(r.x * bx.iL[0]);
+    iy = rintf(r.y * bx.iL[1]);
+    iz = rintf(r.z * bx.iL[2]);
+    r.x -= bx.L[0] * ix;
+    r.y -= bx.L[1] * iy;
+    r.z -= bx.L[2] * iz;
+  } else {
+    const double dtd = dt;
+    double x = static_cast<double>(r.x) + static_cast<double>(lo.x) + static_cast<double>(v.x) * dtd;
+    double y = static_cast<double>(r.y) + static_cast<double>(lo.y) + static_cast<double>(v.y) * dtd;
+    double z = static_cast<double>(r.z) + static_cast<double>(lo.z) + static_cast<double>(v.z) * dtd;
+    const double kx = rint(x * bx.iLd[0]), ky = rint(y * bx.iLd[1]), kz = rint(z * bx.iLd[2]);
+    x -= bx.Ld[0] * kx;
+    y -= bx.Ld[1] * ky;
+    z -= bx.Ld[2] * kz;
+    r.x = static_cast<float>(x);
+    r.y = static_cast<float>(y);
+    r.z = static_cast<float>(z);
+    lo.x = static_cast<float>(x - static_cast<double>(r.x));
+    lo.y = static_cast<float>(y - static_cast<double>(r.y));
+    lo.z = static_cast<float>(z - static_cast<double>(r.z));
+    ix = static_cast<float>(kx);
+    iy = static_cast<float>(ky);
+    iz = static_cast<float>(kz);
+  }
+  im.x += ix;
+  im.y += iy;
+  im.z += iz;
+  return (v.x * v.x + v.y * v.y + v.z * v.z) / v.w;
+}
+
+struct NvtArgs {
+  float4* r;
+  float4* v;
+  const float4* f;
+  float4* im;
+  float4* rlo;
+  rumdx_nh_state* nh;
+  double* partials;
+  uint32_t* counter;
+  const float4* last_r;  // skin check fused when non-null
+  uint32_t* nb_flags;
+  float skin2_over_4;
+  float dt;
+  float omega2;
+  uint32_t on_mask;
+  uint32_t dof;
+  uint32_t n;
+};
+
+template <bool THERMO, bool HILO, bool SKIN>
+__global__ __launch_bounds__(kBlock) void k_nvt(NvtArgs a, Box bx) {
+  __shared__ double s_red[kBlock / 64];
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t i = b * kBlock + threadIdx.x;
+  float ke2 = 0.f;
+  // Every block reads Ps before it arrives at the counter, so the last block
+  // may advance it: all reads of this step's Ps have happened by then.
+  const float ps_all = THERMO ? static_cast<float>(a.nh->ps) : 0.f;
+  if (i < a.n) {
+    float4 r = a.r[i], v = a.v[i], im = a.im[i];
+    const float4 f = a.f[i];
+    float4 lo = HILO ? a.rlo[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool apply = THERMO && ((a.on_mask >> (__float_as_uint(r.w) & 31u)) & 1u);
+    const float ke = nvt_particle<HILO>(r, v, f, im, lo, apply ? ps_all : 0.f, a.dt, bx);
+    if (apply) ke2 = ke;
+    a.r[i] = r;
+    a.v[i] = v;
+    a.im[i] = im;
+    if (HILO) a.rlo[i] = lo;
+    if (SKIN) {
+      if (min_image_d2_f32(r, a.last_r[i], bx) > a.skin2_over_4) a.nb_flags[RUMDX_FLAG_REBUILD] = 1u;
+    }
+  }
+  if (THERMO) {
+    const double s = block_sum(static_cast<double>(ke2), s_red);
+    if (threadIdx.x == 0) a.partials[b] = s;
+    if (arrive_last(a.counter, gridDim.x)) {
+      const double tot = sum_partials(a.partials, gridDim.x, 1, s_red);
+      if (threadIdx.x == 0) {
+        rumdx_nh_state* nh = a.nh;
+        double dev = tot / (static_cast<double>(a.dof) * nh->target_T) - 1.0;
+        if (dev > 10.0) dev = 10.0;  // runaway clamp (src/IntegratorNVT.cu:432-434)
+        nh->ps += static_cast<double>(a.dt) * static_cast<double>(a.omega2) * dev;
+        nh->ke2_sum = tot;
+        nh->target_T += nh->dT_per_step;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- momentum
+__global__ __launch_bounds__(kBlock) void k_momentum_sum(const float4* __restrict__ v, double* partials,
+                                                         uint32_t* counter, double* out4, uint32_t n) {
+  __shared__ double s_red[kBlock / 64];
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t i = b * kBlock + threadIdx.x;
+  double p[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < n) {
+    const float4 vi = v[i];
+    const float m = 1.f / vi.w;  // calculateMomentum (src/IntegratorNVT.cu:462-470)
+    p[0] = static_cast<double>(vi.x * m);
+    p[1] = static_cast<double>(vi.y * m);
+    p[2] = static_cast<double>(vi.z * m);
+    p[3] = static_cast<double>(m);
+  }
+  for (int c = 0; c < 4; ++c) {
+    const double s = block_sum(p[c], s_red);
+    if (threadIdx.x == 0) partials[static_cast<size_t>(b) * 4 + c] = s;
+  }
+  if (arrive_last(counter, gridDim.x)) {
+    for (int c = 0; c < 4; ++c) {
+      const double s = sum_partials(partials + c, gridDim.x, 4, s_red);
+      if (threadIdx.x == 0) out4[c] = s;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_momentum_apply(float4* __restrict__ v, const double* __restrict__ tot4,
+                                                           uint32_t n) {
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double inv_m = 1.0 / tot4[3];
+  float4 vi = v[i];
+  vi.x -= static_cast<float>(tot4[0] * inv_m);
+  vi.y -= static_cast<float>(tot4[1] * inv_m);
+  vi.z -= static_cast<float>(tot4[2] * inv_m);
+  v[i] = vi;
+}
+
+// ---------------------------------------------------------------- kinetic energy
+// IntegratorNVT::GetKineticEnergy (src/IntegratorNVT.cu:316-356), FP64 sum.
+__global__ __launch_bounds__(kBlock) void k_kinetic(const float4* __restrict__ r, const float4* __restrict__ v,
+                                                    const float4* __restrict__ f, const rumdx_nh_state* nh,
+                                                    float dt, uint32_t on_mask, double* partials, uint32_t* counter,
+                                                    double* out, uint32_t n) {
+#pragma clang fp contract(off)
+  __shared__ double s_red[kBlock / 64];
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t i = b * kBlock + threadIdx.x;
+  double e = 0.0;
+  if (i < n) {
+    const float4 vi = v[i], fi = f[i];
+    const double mass = static_cast<double>(1.f / vi.w);
+    const float tsim = vi.w * dt;
+    const bool apply = ((on_mask >> (__float_as_uint(r[i].w) & 31u)) & 1u) != 0u;
+    const double ps = (on_mask != 0u && nh != nullptr) ? nh->ps : 0.0;
+    const float factor = apply ? static_cast<float>(0.5f * dt * ps) : 0.f;
+    const float plus = 1.f / (1.f + factor);
+    const float minus = 1.f - factor;
+    const float vfx = plus * (minus * vi.x + tsim * fi.x);
+    const float vfy = plus * (minus * vi.y + tsim * fi.y);
+    const float vfz = plus * (minus * vi.z + tsim * fi.z);
+    const double vb = static_cast<double>(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
+    const double vf = static_cast<double>(vfx * vfx + vfy * vfy + vfz * vfz);
+    e = mass * ((vb + vf) / 2.0);
+  }
+  const double s = block_sum(e, s_red);
+  if (threadIdx.x == 0) partials[b] = s;
+  if (arrive_last(counter, gridDim.x)) {
+    const double tot = sum_partials(partials, gridDim.x, 1, s_red);
+    if (threadIdx.x == 0) *out = 0.5 * tot;
+  }
+}
+
+// ---------------------------------------------------------------- float4 -> FP64 sums
+__global__ __launch_bounds__(kBlock) void k_reduce_f4(const float4* __restrict__ a, uint32_t mask, double* partials,
+                                                      uint32_t* counter, double* out4, uint32_t n) {
+  __shared__ double s_red[kBlock / 64];
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t i = b * kBlock + threadIdx.x;
+  float4 x = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (i < n) x = a[i];
+  const float c4[4] = {x.x, x.y, x.z, x.w};
+  for (int c = 0; c < 4; ++c) {
+    if (!((mask >> c) & 1u)) continue;
+    const double s = block_sum(static_cast<double>(c4[c]), s_red);
+    if (threadIdx.x == 0) partials[static_cast<size_t>(b) * 4 + c] = s;
+  }
+  if (arrive_last(counter, gridDim.x)) {
+    for (int c = 0; c < 4; ++c) {
+      double s = 0.0;
+      if ((mask >> c) & 1u) s = sum_partials(partials + c, gridDim.x, 4, s_red);
+      if (threadIdx.x == 0) out4[c] = s;
+    }
+  }
+}
+
+template <bool THERMO, bool HILO, bool SKIN>
+void launch_nvt(const NvtArgs& a, const Box& bx, hipStream_t st) {
+  hipLaunchKernelGGL((k_nvt<THERMO, HILO, SKIN>), dim3(blocks_for(a.n)), dim3(kBlock), 0, st, a, bx);
+}
+
+}  // namespace
+}  // namespace rumdx
+
+using namespace rumdx;
+
+extern "C" int rumdx_version(void) { return RUMDX_VERSION; }
+
+extern "C" const char* rumdx_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" size_t rumdx_workspace_bytes(uint32_t n) {
+  return Workspace::kPartials + sizeof(double) * 4 * static_cast<size_t>(blocks_for(n > 0 ? n : 1));
+}
+
+extern "C" int rumdx_nvt_step(rumdx_particles* p, rumdx_nh_state* nh_dev, float dt, uint32_t thermostat_on_mask,
+                              float omega2, uint32_t dof, const float box[8], rumdx_nblist* nb, void* workspace_dev,
+                              void* stream) {
+  if (p == nullptr || box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nvt_step: null argument");
+  if (p->r_dev == nullptr || p->v_dev == nullptr || p->f_dev == nullptr || p->im_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_nvt_step: particle arrays not allocated");
+  const bool thermo = thermostat_on_mask != 0u;
+  if (thermo && (nh_dev == nullptr || workspace_dev == nullptr))
+    return fail(RUMDX_E_INVALID, "rumdx_nvt_step: thermostat state and workspace required");
+  if (thermo && dof == 0) return fail(RUMDX_E_INVALID, "rumdx_nvt_step: zero degrees of freedom");
+  if (nb != nullptr && (nb->flags_dev == nullptr || nb->last_r_dev == nullptr))
+    return fail(RUMDX_E_INVALID, "rumdx_nvt_step: neighbour list not allocated");
+  if (p->n == 0) return RUMDX_OK;
+  NvtArgs a{};
+  a.r = reinterpret_cast<float4*>(p->r_dev);
+  a.v = reinterpret_cast<float4*>(p->v_dev);
+  a.f = reinterpret_cast<const float4*>(p->f_dev);
+  a.im = reinterpret_cast<float4*>(p->im_dev);
+  a.rlo = reinterpret_cast<float4*>(p->rlo_dev);
+  a.nh = nh_dev;
+  a.partials = workspace_dev ? Workspace::partials(workspace_dev) : nullptr;
+  a.counter = workspace_dev ? Workspace::counters(workspace_dev) + kCtrNvt : nullptr;
+  a.dt = dt;
+  a.omega2 = omega2;
+  a.on_mask = thermostat_on_mask;
+  a.dof = dof;
+  a.n = p->n;
+  if (nb != nullptr) {
+    a.last_r = reinterpret_cast<const float4*>(nb->last_r_dev);
+    a.nb_flags = nb->flags_dev;
+    a.skin2_over_4 = static_cast<float>(0.25 * static_cast<double>(nb->skin) * static_cast<double>(nb->skin));
+  }
+  const Box bx = make_box(box);
+  const hipStream_t st = as_stream(stream);
+  const bool hilo = p->rlo_dev != nullptr, skin = nb != nullptr;
+  const int sel = (thermo ? 4 : 0) | (hilo ? 2 : 0) | (skin ? 1 : 0);
+  switch (sel) {
+    case 0: launch_nvt<false, false, false>(a, bx, st); break;
+    case 1: launch_nvt<false, false, true>(a, bx, st); break;
+    case 2: launch_nvt<false, true, false>(a, bx, st); break;
+    case 3: launch_nvt<false, true, true>(a, bx, st); break;
+    case 4: launch_nvt<true, false, false>(a, bx, st); break;
+    case 5: launch_nvt<true, false, true>(a, bx, st); break;
+    case 6: launch_nvt<true, true, false>(a, bx, st); break;
+    default: launch_nvt<true, true, true>(a, bx, st); break;
+  }
+  return check_launch("rumdx_nvt_step");
+}
+
+extern "C" int rumdx_zero_momentum(rumdx_particles* p, void* workspace_dev, void* stream) {
+  if (p == nullptr || p->v_dev == nullptr || workspace_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_zero_momentum: null argument");
+  if (p->n == 0) return RUMDX_OK;
+  const hipStream_t st = as_stream(stream);
+  const uint32_t nb = blocks_for(p->n);
+  double* tot = Workspace::scalars(workspace_dev);
+  hipLaunchKernelGGL(k_momentum_sum, dim3(nb), dim3(kBlock), 0, st, reinterpret_cast<const float4*>(p->v_dev),
+                     Workspace::partials(workspace_dev), Workspace::counters(workspace_dev) + kCtrMomentum, tot, p->n);
+  hipLaunchKernelGGL(k_momentum_apply, dim3(nb), dim3(kBlock), 0, st, reinterpret_cast<float4*>(p->v_dev), tot, p->n);
+  return check_launch("rumdx_zero_momentum");
+}
+
+extern "C" int rumdx_kinetic_energy(const rumdx_particles* p, const rumdx_nh_state* nh_dev, float dt,
+                                    uint32_t thermostat_on_mask, double* out_dev, void* workspace_dev, void* stream) {
+  if (p == nullptr || out_dev == nullptr || workspace_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_kinetic_energy: null argument");
+  if (thermostat_on_mask != 0u && nh_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_kinetic_energy: thermostat state required");
+  if (p->n == 0) return fail(RUMDX_E_INVALID, "rumdx_kinetic_energy: sample has no particles");
+  hipLaunchKernelGGL(k_kinetic, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
+                     reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<const float4*>(p->v_dev),
+                     reinterpret_cast<const float4*>(p->f_dev), nh_dev, dt, thermostat_on_mask,
+                     Workspace::partials(workspace_dev), Workspace::counters(workspace_dev) + kCtrKinetic, out_dev,
+                     p->n);
+  return check_launch("rumdx_kinetic_energy");
+}
+
+extern "C" int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_mask, double* out_dev,
+                               void* workspace_dev, void* stream) {
+  if (arr4_dev == nullptr || out_dev == nullptr || workspace_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_reduce_f4: null argument");
+  if (n == 0) return fail(RUMDX_E_INVALID, "rumdx_reduce_f4: empty array");
+  hipLaunchKernelGGL(k_reduce_f4, dim3(blocks_for(n)), dim3(kBlock), 0, as_stream(stream),
+                     reinterpret_cast<const float4*>(arr4_dev), comp_mask & 15u, Workspace::partials(workspace_dev),
+                     Workspace::counters(workspace_dev) + kCtrReduce, out_dev, n);
+  return check_launch("rumdx_reduce_f4");
+}

@@ -1,0 +1,504 @@
+// Neighbour-list kernels (gfx950): skin check, N2 build and the sorted
+// (cell-binned, Morton-ordered) build.
+//
+// Reference: NeighborList::UpdateNBlist / CheckRebuildRequired
+// (src/NeighborList.cu:569-690), displacement_since_nb_build (:970-997),
+// calculateNBL_N2 (:1001-1099), CalculateNBL_Sorting (:717-781) with
+// calculateCellIndices / calculateCellsSorted / calculateNBL_CellsSorted
+// (:1142-1319), Sample::SortParticlesByKey (include/rumd/Sample.h:174-188) and
+// ParticleData::UpdateAfterSorting (src/ParticleData.cu:493-528).
+//
+// Every build kernel first reads flags[REBUILD] and exits when it is clear, so a
+// step can launch the whole chain unconditionally (no host round trip, capturable
+// in a graph).  The last workgroup of the final kernel clears the flag.
+//
+// The sorted build is a deterministic counting sort: histogram of cell ranks,
+// two-level exclusive scan, scatter, then each cell's slots are put back in
+// ascending old-slot order (what a stable key sort gives).  The gather writes
+// into scratch; the list kernel reads the sorted positions from there and copies
+// every array back to its home buffer for its own slot, so buffer addresses
+// never change (captured graphs stay valid).
+#include <algorithm>
+#include <utility>
+#include <vector>
+
+#include "rumdx_common.h"
+
+namespace rumdx {
+namespace {
+
+constexpr uint32_t kScanChunk = 1024;  // cells per scan block (256 threads x 4)
+
+struct SortScratch {
+  uint32_t* key_lin;  // nvp: linear cell of each old slot
+  uint32_t* perm;     // nvp: old slot of each new slot
+  uint32_t* parts;    // nparts + 1
+  float4* t_r;        // nvp each
+  float4* t_v;
+  float4* t_f;
+  float4* t_w;
+  float4* t_im;
+  float4* t_rlo;
+  uint32_t* t_idx;    // nvp
+};
+
+size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+SortScratch carve(void* base, uint32_t nvp, uint32_t ncells) {
+  char* p = static_cast<char*>(base);
+  SortScratch s;
+  const size_t u32n = align256(sizeof(uint32_t) * nvp);
+  const size_t f4n = align256(sizeof(float4) * nvp);
+  const uint32_t nparts = blocks_for(ncells, kScanChunk);
+  s.key_lin = reinterpret_cast<uint32_t*>(p); p += u32n;
+  s.perm = reinterpret_cast<uint32_t*>(p); p += u32n;
+  s.parts = reinterpret_cast<uint32_t*>(p); p += align256(sizeof(uint32_t) * (nparts + 1));
+  s.t_r = reinterpret_cast<float4*>(p); p += f4n;
+  s.t_v = reinterpret_cast<float4*>(p); p += f4n;
+  s.t_f = reinterpret_cast<float4*>(p); p += f4n;
+  s.t_w = reinterpret_cast<float4*>(p); p += f4n;
+  s.t_im = reinterpret_cast<float4*>(p); p += f4n;
+  s.t_rlo = reinterpret_cast<float4*>(p); p += f4n;
+  s.t_idx = reinterpret_cast<uint32_t*>(p);
+  return s;
+}
+
+size_t scratch_bytes(uint32_t nvp, uint32_t ncells) {
+  const size_t u32n = align256(sizeof(uint32_t) * nvp);
+  const size_t f4n = align256(sizeof(float4) * nvp);
+  const uint32_t nparts = blocks_for(ncells, kScanChunk);
+  return 3 * u32n + align256(sizeof(uint32_t) * (nparts + 1)) + 6 * f4n;
+}
+
+__device__ __forceinline__ bool rebuild_pending(const uint32_t* flags) {
+  return __hip_atomic_load(flags + RUMDX_FLAG_REBUILD, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+
+// Called by every block of the final build kernel after its work: the block
+// that finishes last clears the rebuild request and counts the build.
+__device__ __forceinline__ void finish_build(uint32_t* flags) {
+  if (arrive_last(flags + RUMDX_FLAG_DONE, gridDim.x) && threadIdx.x == 0) {
+    flags[RUMDX_FLAG_REBUILD] = 0u;
+    flags[RUMDX_FLAG_BUILDS] += 1u;
+  }
+}
+
+// ------------------------------------------------------------------ skin check
+__global__ __launch_bounds__(kBlock) void k_skin_check(const float4* __restrict__ r, const float4* __restrict__ last_r,
+                                                       uint32_t* __restrict__ flags, float skin2_over_4, Box bx,
+                                                       uint32_t n) {
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float d2 = min_image_d2_f32(r[i], last_r[i], bx);
+  if (d2 > skin2_over_4) flags[RUMDX_FLAG_REBUILD] = 1u;
+}
+
+__global__ void k_request_rebuild(uint32_t* flags) {
+  if (threadIdx.x == 0) flags[RUMDX_FLAG_REBUILD] = 1u;
+}
+
+// ------------------------------------------------------------------ N2 build
+// All pairs in slot order, positions staged through LDS.  List entries come out
+// in ascending j (RUMD's atomicInc order is unspecified; only sets compare).
+__global__ __launch_bounds__(kBlock) void k_build_n2(const float4* __restrict__ r, float4* __restrict__ last_r,
+                                                     uint32_t* __restrict__ nbl, uint32_t* __restrict__ num_nbrs,
+                                                     uint32_t* __restrict__ flags, const float* __restrict__ cut_skin2,
+                                                     uint32_t nt, Box bx, uint32_t n, uint32_t nvp,
+                                                     uint32_t max_nbrs) {
+  if (!rebuild_pending(flags)) return;
+  extern __shared__ float4 s_dyn[];
+  float4* s_r = s_dyn;
+  float* s_cut = reinterpret_cast<float*>(s_dyn + kBlock);
+  for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_skin2[k];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool live = i < n;
+  const float4 ri = live ? r[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t ti = __float_as_uint(ri.w);
+  uint32_t cnt = 0;
+  for (uint32_t base = 0; base < n; base += kBlock) {
+    __syncthreads();
+    if (base + threadIdx.x < n) s_r[threadIdx.x] = r[base + threadIdx.x];
+    __syncthreads();
+    const uint32_t m = (n - base) < kBlock ? (n - base) : kBlock;
+    if (live) {
+      for (uint32_t k = 0; k < m; ++k) {
+        const uint32_t j = base + k;
+        if (j == i) continue;
+        const float4 rj = s_r[k];
+        const float d2 = min_image_d2_f32(ri, rj, bx);
+        if (d2 < s_cut[ti * nt + __float_as_uint(rj.w)]) {
+          if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * nvp + i] = j;
+          cnt += (cnt < max_nbrs) ? 1u : 0u;
+        }
+      }
+    }
+  }
+  if (live) {
+    last_r[i] = ri;
+    num_nbrs[i] = cnt;
+    if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
+    atomicMax(flags + RUMDX_FLAG_MAXLEN, cnt);
+  }
+  finish_build(flags);
+}
+
+// ------------------------------------------------------------------ sorted build
+struct CellGeom {
+  int nc0, nc1, nc2;
+  uint32_t ncells;
+};
+
+__device__ __forceinline__ uint32_t cell_of(float4 rr, const Box& bx, const CellGeom& g) {
+  const int cx = cell_coord(rr.x, bx.iL[0], g.nc0);
+  const int cy = cell_coord(rr.y, bx.iL[1], g.nc1);
+  const int cz = cell_coord(rr.z, bx.iL[2], g.nc2);
+  return static_cast<uint32_t>(cx + cy * g.nc0 + cz * g.nc0 * g.nc1);
+}
+
+// calculateCellIndices (src/NeighborList.cu:1142-1172) plus the histogram.
+__global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__ r, const uint32_t* __restrict__ flags,
+                                                      const uint32_t* __restrict__ cell_rank,
+                                                      uint32_t* __restrict__ key_lin,
+                                                      uint32_t* __restrict__ cell_count, Box bx, CellGeom g,
+                                                      uint32_t n) {
+  if (!rebuild_pending(flags)) return;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t lin = cell_of(r[i], bx, g);
+  key_lin[i] = lin;
+  atomicAdd(cell_count + cell_rank[lin], 1u);
+}
+
+// Block-wide exclusive scan of 4 values per thread (kScanChunk per block).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
+  // inclusive scan within the wave
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= static_cast<uint32_t>(o)) x += y;
+  }
+  if (lane == 63) lds[wave] = x;
+  __syncthreads();
+  uint32_t wave_off = 0, tot = 0;
+  for (uint32_t k = 0; k < kBlock / 64; ++k) {
+    if (k < wave) wave_off += lds[k];
+    tot += lds[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return wave_off + x - v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_parts(const uint32_t* __restrict__ cell_count,
+                                                       const uint32_t* __restrict__ flags,
+                                                       uint32_t* __restrict__ parts, uint32_t ncells) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ uint32_t lds[kBlock / 64];
+  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < 4; ++k)
+    if (base + k < ncells) s += cell_count[base + k];
+  uint32_t tot;
+  block_excl_scan(s, lds, &tot);
+  if (threadIdx.x == 0) parts[blockIdx.x] = tot;
+}
+
+// One workgroup: exclusive scan of the per-chunk totals, in place.
+__global__ __launch_bounds__(kBlock) void k_scan_top(const uint32_t* __restrict__ flags, uint32_t* __restrict__ parts,
+                                                     uint32_t nparts) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ uint32_t lds[kBlock / 64];
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nparts; base += kBlock) {
+    const uint32_t k = base + threadIdx.x;
+    const uint32_t v = k < nparts ? parts[k] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(v, lds, &tot);
+    if (k < nparts) parts[k] = carry + ex;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) parts[nparts] = carry;
+}
+
+// cell_start[c] = exclusive prefix of the counts; counts are zeroed so the
+// scatter can use them as per-cell cursors.
+__global__ __launch_bounds__(kBlock) void k_scan_final(uint32_t* __restrict__ cell_count,
+                                                       const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ parts,
+                                                       uint32_t* __restrict__ cell_start, uint32_t ncells,
+                                                       uint32_t nparts) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ uint32_t lds[kBlock / 64];
+  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4;
+  uint32_t c[4];
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < 4; ++k) {
+    c[k] = base + k < ncells ? cell_count[base + k] : 0u;
+    s += c[k];
+  }
+  uint32_t tot;
+  uint32_t off = block_excl_scan(s, lds, &tot) + parts[blockIdx.x];
+  for (uint32_t k = 0; k < 4; ++k) {
+    if (base + k < ncells) {
+      cell_start[base + k] = off;
+      cell_count[base + k] = 0u;
+    }
+    off += c[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) cell_start[ncells] = parts[nparts];
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ key_lin,
+                                                    const uint32_t* __restrict__ flags,
+                                                    const uint32_t* __restrict__ cell_rank,
+                                                    const uint32_t* __restrict__ cell_start,
+                                                    uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
+                                                    uint32_t n) {
+  if (!rebuild_pending(flags)) return;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t rank = cell_rank[key_lin[i]];
+  const uint32_t slot = cell_start[rank] + atomicAdd(cell_count + rank, 1u);
+  if (slot < n) perm[slot] = i;  // always true when the counts started at zero
+}
+
+// Restores ascending old-slot order inside each cell (the scatter's order is
+// unspecified) and re-zeroes the counts for the next build.
+__global__ __launch_bounds__(kBlock) void k_cell_order(const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ cell_start,
+                                                       uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
+                                                       uint32_t ncells) {
+  if (!rebuild_pending(flags)) return;
+  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncells) return;
+  const uint32_t s = cell_start[c], e = cell_start[c + 1];
+  cell_count[c] = 0u;
+  for (uint32_t a = s + 1; a < e; ++a) {
+    const uint32_t v = perm[a];
+    uint32_t b = a;
+    while (b > s && perm[b - 1] > v) {
+      perm[b] = perm[b - 1];
+      --b;
+    }
+    perm[b] = v;
+  }
+}
+
+// Permutes every per-particle array into scratch (new slot s <- old slot perm[s]).
+__global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ flags,
+                                                   const uint32_t* __restrict__ perm, const float4* __restrict__ r,
+                                                   const float4* __restrict__ v, const float4* __restrict__ f,
+                                                   const float4* __restrict__ w, const float4* __restrict__ im,
+                                                   const float4* __restrict__ rlo,
+                                                   const uint32_t* __restrict__ idx, SortScratch s, uint32_t n) {
+  if (!rebuild_pending(flags)) return;
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t o = perm[i];
+  s.t_r[i] = r[o];
+  s.t_v[i] = v[o];
+  s.t_f[i] = f[o];
+  s.t_w[i] = w[o];
+  s.t_im[i] = im[o];
+  if (rlo != nullptr) s.t_rlo[i] = rlo[o];
+  s.t_idx[i] = idx[o];
+}
+
+// calculateNBL_CellsSorted (src/NeighborList.cu:1204-1319): scan the 5x5x5 cell
+// stencil in RUMD's (dZ, dY, dX) order, ascending slot inside each cell; accept
+// j != i with d^2 < (rc_ij + skin)^2.  Also copies this slot's sorted data back
+// to the home arrays and records last_r and the slot's cell.
+__global__ __launch_bounds__(kBlock) void k_build_cells(
+    uint32_t* __restrict__ flags, SortScratch s, const uint32_t* __restrict__ cell_rank,
+    const uint32_t* __restrict__ cell_start, const float* __restrict__ cut_skin2, uint32_t nt, Box bx, CellGeom g,
+    float4* __restrict__ r, float4* __restrict__ v, float4* __restrict__ f, float4* __restrict__ w,
+    float4* __restrict__ im, float4* __restrict__ rlo, uint32_t* __restrict__ idx, float4* __restrict__ last_r,
+    uint32_t* __restrict__ slot_cell, uint32_t* __restrict__ nbl, uint32_t* __restrict__ num_nbrs, uint32_t n,
+    uint32_t nvp, uint32_t max_nbrs) {
+  if (!rebuild_pending(flags)) return;
+  extern __shared__ float s_cut[];
+  for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_skin2[k];
+  __syncthreads();
+  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i < n) {
+    const float4 ri = s.t_r[i];
+    const float* crow = s_cut + __float_as_uint(ri.w) * nt;
+    const int cx = cell_coord(ri.x, bx.iL[0], g.nc0);
+    const int cy = cell_coord(ri.y, bx.iL[1], g.nc1);
+    const int cz = cell_coord(ri.z, bx.iL[2], g.nc2);
+    uint32_t cnt = 0;
+    for (int dz = -2; dz <= 2; ++dz) {
+      const int oz = (cz + dz + g.nc2) % g.nc2;
+      for (int dy = -2; dy <= 2; ++dy) {
+        const int oy = (cy + dy + g.nc1) % g.nc1;
+        const int rowbase = oy * g.nc0 + oz * g.nc0 * g.nc1;
+        for (int dx = -2; dx <= 2; ++dx) {
+          const int ox = (cx + dx + g.nc0) % g.nc0;
+          const uint32_t rank = cell_rank[ox + rowbase];
+          const uint32_t js = cell_start[rank], je = cell_start[rank + 1];
+          for (uint32_t j = js; j < je; ++j) {
+            if (j == i) continue;
+            const float4 rj = s.t_r[j];
+            const float d2 = min_image_d2_f32(ri, rj, bx);
+            if (d2 < crow[__float_as_uint(rj.w)]) {
+              if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * nvp + i] = j;
+              cnt += (cnt < max_nbrs) ? 1u : 0u;
+            }
+          }
+        }
+      }
+    }
+    num_nbrs[i] = cnt;
+    if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
+    atomicMax(flags + RUMDX_FLAG_MAXLEN, cnt);
+    slot_cell[i] = static_cast<uint32_t>(cx + cy * g.nc0 + cz * g.nc0 * g.nc1);
+    r[i] = ri;
+    last_r[i] = ri;
+    v[i] = s.t_v[i];
+    f[i] = s.t_f[i];
+    w[i] = s.t_w[i];
+    im[i] = s.t_im[i];
+    if (rlo != nullptr) rlo[i] = s.t_rlo[i];
+    idx[i] = s.t_idx[i];
+  }
+  finish_build(flags);
+}
+
+int check_nb(const rumdx_particles* p, const rumdx_nblist* nb, const char* who) {
+  if (p == nullptr || nb == nullptr) return fail(RUMDX_E_INVALID, std::string(who) + ": null argument");
+  if (nb->flags_dev == nullptr || nb->nbl_dev == nullptr || nb->num_nbrs_dev == nullptr ||
+      nb->last_r_dev == nullptr || nb->cut_skin2_dev == nullptr)
+    return fail(RUMDX_E_INVALID, std::string(who) + ": neighbour list not allocated");
+  if (nb->max_nbrs == 0) return fail(RUMDX_E_INVALID, std::string(who) + ": max_nbrs is zero");
+  if (static_cast<uint64_t>(nb->max_nbrs) * p->nvp >= (1ull << 32))
+    return fail(RUMDX_E_INVALID, std::string(who) + ": max_nbrs * nvp exceeds 2^32 entries");
+  if (nb->ntypes == 0 || nb->ntypes > RUMDX_MAX_TYPES) return fail(RUMDX_E_INVALID, std::string(who) + ": ntypes out of range");
+  return RUMDX_OK;
+}
+
+}  // namespace
+}  // namespace rumdx
+
+using namespace rumdx;
+
+extern "C" size_t rumdx_sort_scratch_bytes(uint32_t nvp, uint32_t ncells) { return scratch_bytes(nvp, ncells); }
+
+extern "C" int rumdx_cell_grid(const float box[8], float max_cutoff, float skin, int32_t nc_out[3]) {
+  if (box == nullptr || nc_out == nullptr) return fail(RUMDX_E_INVALID, "rumdx_cell_grid: null argument");
+  if (!(max_cutoff + skin > 0.f)) return fail(RUMDX_E_INVALID, "rumdx_cell_grid: cutoff + skin must be positive");
+  const Box bx = make_box(box);
+  for (int a = 0; a < 3; ++a) {
+    // (int)floor(n_Cells * L / (maxCutoff + skin)) with n_Cells = 2, all in float
+    const float q = 2.0f * bx.L[a] / (max_cutoff + skin);
+    int nc = static_cast<int>(floorf(q));
+    if (nc < 5) nc = 5;
+    nc_out[a] = nc;
+  }
+  return RUMDX_OK;
+}
+
+extern "C" int rumdx_cell_rank_table(const int32_t nc[3], uint32_t* rank_out) {
+  if (nc == nullptr || rank_out == nullptr) return fail(RUMDX_E_INVALID, "rumdx_cell_rank_table: null argument");
+  for (int a = 0; a < 3; ++a)
+    if (nc[a] < 1 || nc[a] > 2048) return fail(RUMDX_E_INVALID, "rumdx_cell_rank_table: cells per axis out of range");
+  // Rank = position of the cell's Morton code (x in bit 0, y in bit 1, z in
+  // bit 2 of each triple) among all cells of the grid.
+  const uint32_t n0 = nc[0], n1 = nc[1], n2 = nc[2];
+  const uint64_t ncells = static_cast<uint64_t>(n0) * n1 * n2;
+  std::vector<std::pair<uint64_t, uint32_t>> code(ncells);
+  for (uint32_t z = 0; z < n2; ++z)
+    for (uint32_t y = 0; y < n1; ++y)
+      for (uint32_t x = 0; x < n0; ++x) {
+        uint64_t m = 0;
+        for (uint32_t b = 0; b < 11; ++b)
+          m |= (static_cast<uint64_t>((x >> b) & 1u) << (3 * b)) | (static_cast<uint64_t>((y >> b) & 1u) << (3 * b + 1)) |
+               (static_cast<uint64_t>((z >> b) & 1u) << (3 * b + 2));
+        const uint32_t lin = x + y * n0 + z * n0 * n1;
+        code[lin] = {m, lin};
+      }
+  std::sort(code.begin(), code.end());
+  for (uint64_t k = 0; k < ncells; ++k) rank_out[code[k].second] = static_cast<uint32_t>(k);
+  return RUMDX_OK;
+}
+
+extern "C" int rumdx_nblist_check(const rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream) {
+  if (p == nullptr || nb == nullptr || box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_check: null argument");
+  if (nb->flags_dev == nullptr || nb->last_r_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_nblist_check: neighbour list not allocated");
+  if (p->n == 0) return RUMDX_OK;
+  // skin2_over_4 = 0.25*skin*skin formed in double, passed as float (src/NeighborList.cu:578-579)
+  const float s24 = static_cast<float>(0.25 * static_cast<double>(nb->skin) * static_cast<double>(nb->skin));
+  hipLaunchKernelGGL(k_skin_check, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
+                     reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<const float4*>(nb->last_r_dev),
+                     nb->flags_dev, s24, make_box(box), p->n);
+  return check_launch("rumdx_nblist_check");
+}
+
+extern "C" int rumdx_nblist_request_rebuild(rumdx_nblist* nb, void* stream) {
+  if (nb == nullptr || nb->flags_dev == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_request_rebuild: null argument");
+  hipLaunchKernelGGL(k_request_rebuild, dim3(1), dim3(64), 0, as_stream(stream), nb->flags_dev);
+  return check_launch("rumdx_nblist_request_rebuild");
+}
+
+extern "C" int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream) {
+  int rc = check_nb(p, nb, "rumdx_nblist_build_n2");
+  if (rc != RUMDX_OK) return rc;
+  if (box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_n2: null box");
+  if (p->n == 0) return RUMDX_OK;
+  const size_t lds = sizeof(float4) * kBlock + sizeof(float) * nb->ntypes * nb->ntypes;
+  hipLaunchKernelGGL(k_build_n2, dim3(blocks_for(p->n)), dim3(kBlock), lds, as_stream(stream),
+                     reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<float4*>(nb->last_r_dev),
+                     nb->nbl_dev, nb->num_nbrs_dev, nb->flags_dev, nb->cut_skin2_dev, nb->ntypes, make_box(box), p->n,
+                     p->nvp, nb->max_nbrs);
+  return check_launch("rumdx_nblist_build_n2");
+}
+
+extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream) {
+  int rc = check_nb(p, nb, "rumdx_nblist_build_sorted");
+  if (rc != RUMDX_OK) return rc;
+  if (box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: null box");
+  if (nb->cell_rank_dev == nullptr || nb->cell_start_dev == nullptr || nb->cell_count_dev == nullptr ||
+      nb->slot_cell_dev == nullptr || nb->sort_scratch_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: cell arrays not allocated");
+  if (p->v_dev == nullptr || p->im_dev == nullptr || p->unsorted_index_dev == nullptr || p->f_dev == nullptr ||
+      p->w_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: particle arrays not allocated");
+  for (int a = 0; a < 3; ++a)
+    if (nb->nc[a] < 5) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: fewer than 5 cells per axis");
+  if (p->n == 0) return RUMDX_OK;
+  const CellGeom g{nb->nc[0], nb->nc[1], nb->nc[2],
+                   static_cast<uint32_t>(nb->nc[0]) * static_cast<uint32_t>(nb->nc[1]) * static_cast<uint32_t>(nb->nc[2])};
+  const Box bx = make_box(box);
+  const hipStream_t st = as_stream(stream);
+  SortScratch s = carve(nb->sort_scratch_dev, p->nvp, g.ncells);
+  const uint32_t nparts = blocks_for(g.ncells, kScanChunk);
+  const uint32_t nblk = blocks_for(p->n);
+  const float4* r = reinterpret_cast<const float4*>(p->r_dev);
+
+  hipLaunchKernelGGL(k_cell_hist, dim3(nblk), dim3(kBlock), 0, st, r, nb->flags_dev, nb->cell_rank_dev, s.key_lin,
+                     nb->cell_count_dev, bx, g, p->n);
+  hipLaunchKernelGGL(k_scan_parts, dim3(nparts), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.parts,
+                     g.ncells);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, st, nb->flags_dev, s.parts, nparts);
+  hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.parts,
+                     nb->cell_start_dev, g.ncells, nparts);
+  hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBlock), 0, st, s.key_lin, nb->flags_dev, nb->cell_rank_dev,
+                     nb->cell_start_dev, nb->cell_count_dev, s.perm, p->n);
+  hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(g.ncells)), dim3(kBlock), 0, st, nb->flags_dev,
+                     nb->cell_start_dev, nb->cell_count_dev, s.perm, g.ncells);
+  hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlock), 0, st, nb->flags_dev, s.perm, r,
+                     reinterpret_cast<const float4*>(p->v_dev), reinterpret_cast<const float4*>(p->f_dev),
+                     reinterpret_cast<const float4*>(p->w_dev), reinterpret_cast<const float4*>(p->im_dev),
+                     reinterpret_cast<const float4*>(p->rlo_dev), p->unsorted_index_dev, s, p->n);
+  hipLaunchKernelGGL(k_build_cells, dim3(nblk), dim3(kBlock), sizeof(float) * nb->ntypes * nb->ntypes, st,
+                     nb->flags_dev, s, nb->cell_rank_dev, nb->cell_start_dev, nb->cut_skin2_dev, nb->ntypes, bx, g,
+                     reinterpret_cast<float4*>(p->r_dev), reinterpret_cast<float4*>(p->v_dev),
+                     reinterpret_cast<float4*>(p->f_dev), reinterpret_cast<float4*>(p->w_dev),
+                     reinterpret_cast<float4*>(p->im_dev), reinterpret_cast<float4*>(p->rlo_dev),
+                     p->unsorted_index_dev, reinterpret_cast<float4*>(nb->last_r_dev), nb->slot_cell_dev, nb->nbl_dev,
+                     nb->num_nbrs_dev, p->n, p->nvp, nb->max_nbrs);
+  return check_launch("rumdx_nblist_build_sorted");
+}

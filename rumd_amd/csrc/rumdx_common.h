@@ -1,0 +1,149 @@
+// Device-side helpers shared by the rumdx kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "rumdx.h"
+
+namespace rumdx {
+
+// Threads per workgroup for the per-particle kernels: four wave64s.
+constexpr uint32_t kBlock = 256;
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs.
+constexpr uint32_t kNumXcd = 8;
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_launch(const char* what);
+
+// ------------------------------------------------------------------ launch geometry
+inline uint32_t blocks_for(uint32_t n, uint32_t per = kBlock) { return (n + per - 1) / per; }
+
+// Maps the hardware workgroup id to a logical block so that each XCD receives
+// one contiguous range of logical blocks (hence of cell-sorted particles): the
+// neighbour positions a block gathers then sit in that XCD's own L2.  Bijective
+// on [0, nb) for any nb.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t hw, uint32_t nb) {
+  const uint32_t q = nb / kNumXcd, r = nb % kNumXcd;
+  const uint32_t x = hw % kNumXcd, k = hw / kNumXcd;
+  const uint32_t base = (x < r) ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return base + k;
+}
+
+// ------------------------------------------------------------------ box
+// RUMD's float[8] box, expanded on the host (loadMemory semantics,
+// include/rumd/SimulationBox.h:165-178): a cubic box uses L[0] and 1/L[0]
+// for all three axes.
+struct Box {
+  float L[3];
+  float iL[3];
+  double Ld[3];
+  double iLd[3];
+};
+
+inline Box make_box(const float box[8]) {
+  Box b;
+  const bool cubic = static_cast<unsigned>(box[0]) != 0u;
+  for (int a = 0; a < 3; ++a) {
+    b.L[a] = cubic ? box[1] : box[1 + a];
+    b.iL[a] = cubic ? box[4] : box[4 + a];
+    b.Ld[a] = static_cast<double>(b.L[a]);
+    b.iLd[a] = 1.0 / b.Ld[a];
+  }
+  return b;
+}
+
+// RUMD's FP32 minimum-image squared distance (RectangularSimulationBox::
+// calculateDistance, include/rumd/SimulationBox.h:111-125), evaluated in source
+// order without FMA contraction so that it is reproducible bit for bit on the
+// host.  Used by every neighbour-list criterion and by the skin check.
+__device__ __forceinline__ float min_image_d2_f32(float4 a, float4 b, const Box& bx) {
+#pragma clang fp contract(off)
+  float dx = a.x - b.x;
+  float dy = a.y - b.y;
+  float dz = a.z - b.z;
+  dx -= bx.L[0] * rintf(dx * bx.iL[0]);
+  dy -= bx.L[1] * rintf(dy * bx.iL[1]);
+  dz -= bx.L[2] * rintf(dz * bx.iL[2]);
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Cell coordinate of one axis (calculateCellCoordinates,
+// src/NeighborList.cu:1106-1133), uncontracted; out-of-range values clamp.
+__device__ __forceinline__ int cell_coord(float x, float iL, int nc) {
+#pragma clang fp contract(off)
+  const float s = x * iL + 0.5f;
+  int c = static_cast<int>(static_cast<float>(nc) * s);
+  c = c >= nc ? nc - 1 : c;
+  c = c < 0 ? 0 : c;
+  return c;
+}
+
+// ------------------------------------------------------------------ reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Fixed-order sum over a kBlock-thread workgroup: lane tree inside each wave,
+// then the four wave totals in wave order.  Result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds4) {
+  v = wave_sum(v);
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 0) lds4[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (uint32_t k = 0; k < kBlock / 64; ++k) s += lds4[k];
+  }
+  __syncthreads();
+  return s;
+}
+
+// Last-block-done handshake.  Each block publishes its partial before calling;
+// returns true in every thread of the block that arrives last, after an
+// acquire fence so that all partials are visible to it.  The counter is left
+// at zero for the next launch.
+__device__ __forceinline__ bool arrive_last(uint32_t* counter, uint32_t nblocks) {
+  __shared__ uint32_t s_last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (t == nblocks - 1) ? 1u : 0u;
+    if (s_last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  if (s_last) __threadfence();
+  return s_last != 0u;
+}
+
+// Fixed-order sum of partials[0..np) by one workgroup; valid in thread 0.
+__device__ __forceinline__ double sum_partials(const double* partials, uint32_t np, uint32_t stride,
+                                               double* lds4) {
+  double v = 0.0;
+  for (uint32_t k = threadIdx.x; k < np; k += kBlock) v += partials[static_cast<size_t>(k) * stride];
+  return block_sum(v, lds4);
+}
+
+// ------------------------------------------------------------------ workspace
+// Layout of the caller's reduction workspace.
+struct Workspace {
+  static constexpr size_t kCounters = 0;    // uint32[16]
+  static constexpr size_t kScalars = 64;    // double[8] scratch results
+  static constexpr size_t kPartials = 256;  // double[4 * nblocks]
+  static uint32_t* counters(void* ws) { return reinterpret_cast<uint32_t*>(static_cast<char*>(ws) + kCounters); }
+  static double* scalars(void* ws) { return reinterpret_cast<double*>(static_cast<char*>(ws) + kScalars); }
+  static double* partials(void* ws) { return reinterpret_cast<double*>(static_cast<char*>(ws) + kPartials); }
+};
+// Counter slots, one per kernel family.
+enum : uint32_t { kCtrNvt = 0, kCtrMomentum = 1, kCtrKinetic = 2, kCtrReduce = 3 };
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+}  // namespace rumdx

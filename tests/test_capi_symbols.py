@@ -1,0 +1,73 @@
+"""The C-ABI library loads on a machine without a GPU and exports every entry
+point include/rumdx.h declares.  No compute call is made here."""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+
+import numpy as np
+
+import capi
+
+
+def test_header_declares_entry_points():
+    names = capi.declared_symbols()
+    assert "rumdx_pair_lj" in names and "rumdx_nvt_step" in names and len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(capi.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = [s for s in capi.declared_symbols() if s not in exported]
+    assert not missing, missing
+    lib = capi.lib()
+    for s in capi.declared_symbols():
+        assert getattr(lib, s) is not None
+
+
+def test_host_side_helpers_without_gpu():
+    """Entry points that only do host arithmetic: version, sizes, cell grid,
+    Morton rank table; argument validation reports errors."""
+    lib = capi.lib()
+    assert lib.rumdx_version() == 1
+    assert lib.rumdx_workspace_bytes(1 << 20) >= 256 + 8 * 4 * 4096
+    nc = np.zeros(3, dtype=np.int32)
+    from rumd_amd import configs
+    assert lib.rumdx_cell_grid(configs.box8(95.6033), 2.5, 0.3, nc) == 0
+    assert list(nc) == [68, 68, 68]
+    rank = np.zeros(int(nc.prod()), dtype=np.uint32)
+    assert lib.rumdx_cell_rank_table(nc, rank) == 0
+    assert np.array_equal(np.sort(rank), np.arange(rank.size, dtype=np.uint32))
+    # Morton order: the first 8 ranks are the 2x2x2 corner block
+    first = np.nonzero(rank < 8)[0]
+    assert sorted(first.tolist()) == [0, 1, 68, 69, 68 * 68, 68 * 68 + 1, 68 * 68 + 68, 68 * 68 + 69]
+    assert lib.rumdx_pair_lj(None, None, None, 2, configs.box8(10.0), 1, 1, 0, None) == -1
+    assert b"null" in lib.rumdx_last_error()
+    bad = np.zeros(3, dtype=np.int32)
+    assert lib.rumdx_cell_rank_table(bad, rank) == -1
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of the ABI structs have the C compiler's sizes and offsets."""
+    src = tmp_path / "layout.c"
+    fields = {
+        "rumdx_particles": capi.Particles,
+        "rumdx_nblist": capi.NbList,
+        "rumdx_nh_state": capi.NhState,
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "rumdx.h"', "int main(void) {"]
+    for cname, py in fields.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", f"-I{capi.ROOT / 'include'}", str(src), "-o", str(exe)], check=True)
+    got = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                   check=True).stdout.splitlines())
+    for cname, py in fields.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(py, fname).offset, (cname, fname)

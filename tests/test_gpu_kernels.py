@@ -1,0 +1,238 @@
+"""Kernel-level parity of the HIP path, called directly through the C ABI
+(librumdx.so), against the CPU oracle and the RUMD-written golden fixtures."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from capi import GpuSystem, gpu_available
+from helpers import (FORCE_TOL_GUARDED, force_errors, gpu_sets_original_ids, csr_to_sets, ka_table, lj_table,
+                     load_golden, unsort)
+from rumd_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+SP, SF, NS = 1, 2, 0
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.fail("no GPU visible: the gpu tests must run on an MI355X")
+
+
+def _golden_system(golden_dir, name, table, method, **kw):
+    c = load_golden(golden_dir / name)
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    vel4 = configs.pack_vel4(c.velocities, c.types, c.masses)
+    g = GpuSystem(pos4, vel4, configs.box8(c.box), c.num_types, table, method=method, **kw)
+    return c, pos4, vel4, g
+
+
+@pytest.mark.parametrize("method", ["n2", "sorted", "none"])
+def test_forces_golden_ka1000(golden_dir, method):
+    table = ka_table()
+    c, pos4, _, g = _golden_system(golden_dir, "KA_N1000_T0.466.xyz.gz", table, method)
+    print(f"[case] KA_N1000 SP method={method}", flush=True)
+    g.calc_f(SP)
+    uidx = g.unsorted_index()
+    f = unsort(g.get("f"), uidx)
+    w = unsort(g.get("w"), uidx)
+    f_ref, vir_ref = oracle.forces_brute(pos4, c.box, 2, table, SP)
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+    np.testing.assert_allclose(f[:, 3], f_ref[:, 3], rtol=0, atol=2e-5)
+    np.testing.assert_allclose(w[:, 3], vir_ref, rtol=0, atol=5e-4)
+    # and against the forces RUMD itself wrote into the file
+    e_file = force_errors(f[:, :3], c.forces.astype(np.float64))
+    assert e_file["unguarded_median"] < 5e-6 and e_file["guarded_max"] < 1e-4, e_file
+
+
+@pytest.mark.parametrize("name", ["KA_N256_L6.0_T0.50.xyz.gz", "KA_N864_L9.0_T0.50AltMass.xyz.gz"])
+def test_forces_golden_small(golden_dir, name):
+    table = ka_table()
+    c, pos4, _, g = _golden_system(golden_dir, name, table, "n2")
+    print(f"[case] {name} SP n2", flush=True)
+    g.calc_f(SP)
+    f = unsort(g.get("f"), g.unsorted_index())
+    f_ref, _ = oracle.forces_brute(pos4, c.box, 2, table, SP)
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+
+
+def test_forces_golden_lj_at_previous_positions(golden_dir):
+    """LJ_N1000 stores forces at r - v dt (they were written before the last
+    position update); the GPU force there must reproduce the file's forces."""
+    c = load_golden(golden_dir / "LJ_N1000_rho0.80_T0.80.xyz.gz")
+    dt = float(c.meta["dt"])
+    prev = (c.positions.astype(np.float64) - c.velocities.astype(np.float64) * dt)
+    prev -= c.box * np.rint(prev / c.box)
+    pos4 = configs.pack_pos4(prev.astype(np.float32), c.types)
+    g = GpuSystem(pos4, configs.pack_vel4(c.velocities, c.types, c.masses), configs.box8(c.box), 1, lj_table(),
+                  method="n2")
+    g.calc_f(SP)
+    f = unsort(g.get("f"), g.unsorted_index())
+    e = force_errors(f[:, :3], c.forces.astype(np.float64))
+    assert e["unguarded_median"] < 1e-5 and e["guarded_max"] < 2e-4, e
+
+
+@pytest.mark.parametrize("cutoff", [NS, SP, SF])
+def test_forces_ka_sorted_65536(cutoff):
+    """C2-sized KA system through the sorted build (N > 8000), all cut-off methods."""
+    sysm = configs.ka_random_lattice(65536, density=1.2, temperature=0.45, seed=2001)
+    table = ka_table()
+    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
+    g = GpuSystem(pos4, configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses), configs.box8(sysm.box), 2,
+                  table, method="sorted", max_nbrs=160)
+    print(f"[case] KA_N65536 cutoff={cutoff} sorted", flush=True)
+    g.calc_f(cutoff)
+    uidx = g.unsorted_index()
+    assert np.array_equal(np.sort(uidx), np.arange(g.n, dtype=np.uint32))
+    f = unsort(g.get("f"), uidx)
+    w = unsort(g.get("w"), uidx)
+    f_ref, vir_ref = oracle.forces_cells(pos4, sysm.box, 2, table, cutoff)
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+    pe_gpu, pe_ref = float(np.sum(f[:, 3], dtype=np.float64)), float(f_ref[:, 3].sum())
+    assert abs(pe_gpu - pe_ref) <= 1e-5 * abs(pe_ref) + 1e-3
+    w_gpu, w_ref = float(np.sum(w[:, 3], dtype=np.float64)), float(vir_ref.sum())
+    assert abs(w_gpu - w_ref) <= 1e-5 * abs(w_ref) + 1e-3
+
+
+def test_nblist_sets_bit_exact_n2(golden_dir):
+    table = ka_table()
+    c, pos4, _, g = _golden_system(golden_dir, "KA_N1000_T0.466.xyz.gz", table, "n2")
+    g.update_nblist()
+    cnt, nbl = g.neighbor_lists()
+    sets_gpu = gpu_sets_original_ids(cnt, nbl, g.unsorted_index())
+    off, idx = oracle.nblist_brute_f32(pos4, configs.box8(c.box), 2, configs.cut_skin2_table(table, 0.3))
+    assert sets_gpu == csr_to_sets(off, idx)
+
+
+def test_nblist_sets_bit_exact_sorted():
+    sysm = configs.ka_random_lattice(20000, density=1.2, seed=77)
+    table = ka_table()
+    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
+    b8 = configs.box8(sysm.box)
+    g = GpuSystem(pos4, configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses), b8, 2, table, method="sorted",
+                  max_nbrs=200)
+    g.update_nblist()
+    cnt, nbl = g.neighbor_lists()
+    uidx = g.unsorted_index()
+    sets_gpu = gpu_sets_original_ids(cnt, nbl, uidx)
+    nc = oracle.cell_grid(b8, 2.5, 0.3)
+    assert np.array_equal(nc, g.nc)
+    off, idx = oracle.nblist_cells_f32(pos4, b8, 2, configs.cut_skin2_table(table, 0.3), nc)
+    assert sets_gpu == csr_to_sets(off, idx)
+    # the sorted slots hold the same particles, in Morton order of their cells,
+    # ascending original id inside a cell (stable sort from the identity order)
+    r_sorted = g.get("r")
+    assert np.array_equal(r_sorted, pos4[uidx])
+    lin = oracle.cell_of(pos4[uidx], b8, nc)
+    rank = g.cell_rank.download(np.uint32, int(nc.prod()))[lin]
+    assert np.all(np.diff(rank.astype(np.int64)) >= 0)
+    same = np.diff(rank.astype(np.int64)) == 0
+    assert np.all(np.diff(uidx.astype(np.int64))[same] > 0)
+    f = g.flags_host()
+    assert f[1] == 0 and f[0] == 0 and f[2] == 1
+
+
+def test_nblist_overflow_falls_back_exactly():
+    """A list capacity far below the real neighbour count: the overflow flag is
+    raised and the forces are still exact (cell rescan for overflowed lists)."""
+    sysm = configs.ka_random_lattice(12000, density=1.2, seed=5)
+    table = ka_table()
+    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
+    g = GpuSystem(pos4, configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses), configs.box8(sysm.box), 2,
+                  table, method="sorted", max_nbrs=40)
+    g.calc_f(SF)
+    assert g.flags_host()[1] == 1
+    f = unsort(g.get("f"), g.unsorted_index())
+    f_ref, _ = oracle.forces_cells(pos4, sysm.box, 2, table, SF)
+    assert force_errors(f[:, :3], f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
+
+
+def test_skin_trigger_and_no_rebuild():
+    sysm = configs.ka_random_lattice(10000, density=1.2, seed=9)
+    table = ka_table()
+    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
+    g = GpuSystem(pos4, configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses), configs.box8(sysm.box), 2,
+                  table, method="sorted", max_nbrs=200)
+    g.update_nblist()
+    assert g.flags_host()[2] == 1
+    g.update_nblist()  # nothing moved: no rebuild
+    assert g.flags_host()[2] == 1
+    r = g.get("r").copy()
+    r[17, 0] += np.float32(0.151)  # beyond skin/2 = 0.15
+    g.r.upload(np.vstack([r, np.zeros((g.nvp - g.n, 4), np.float32)]))
+    g.update_nblist()
+    assert g.flags_host()[2] == 2
+
+
+@pytest.mark.parametrize("hilo", [False, True])
+@pytest.mark.parametrize("thermo", [False, True])
+def test_integrator_bit_exact(golden_dir, hilo, thermo):
+    table = ka_table()
+    c, pos4, vel4, g = _golden_system(golden_dir, "KA_N1000_T0.466.xyz.gz", table, "n2", rlo=hilo)
+    g.calc_f(SP)
+    dt, ps0, T, tau = 0.004, float(c.meta["Nose-Hoover-Ps"]), 0.466, 0.2
+    omega2 = np.float32(4.0 * np.pi ** 2 / tau ** 2)
+    on = 0xFFFFFFFF if thermo else 0
+    g.set_nh(ps=ps0, target_T=T)
+    f_slot = g.get("f")
+    r_h, v_h, im_h = g.get("r").copy(), g.get("v").copy(), g.get("im").copy()
+    rlo_h = np.zeros_like(r_h) if hilo else None
+    g.nvt_step(dt, on_mask=on, omega2=float(omega2))
+    ke2 = oracle.nvt_step(r_h, v_h, f_slot, im_h, configs.box8(c.box), dt, ps0, on, rlo4=rlo_h)
+    assert np.array_equal(g.get("r"), r_h)
+    assert np.array_equal(g.get("v"), v_h)
+    assert np.array_equal(g.get("im"), im_h)
+    if hilo:
+        assert np.array_equal(g.get("rlo"), rlo_h)
+    nh = g.get_nh()
+    if thermo:
+        ps_ref, t_ref = oracle.thermostat_update(ps0, T, 0.0, ke2, 3 * g.n - 3, dt, float(omega2))
+        assert nh.ke2_sum == pytest.approx(ke2, rel=1e-13)
+        assert nh.ps == pytest.approx(ps_ref, rel=1e-12, abs=1e-15)
+    else:
+        assert nh.ps == ps0
+
+
+def test_reduce_f4_integer_exact_and_fp64():
+    rng = np.random.default_rng(3)
+    n = 300001
+    a = rng.integers(-1000, 1000, size=(n, 4)).astype(np.float32)
+    g = GpuSystem(np.zeros((n, 4), np.float32), np.ones((n, 4), np.float32), configs.box8(100.0), 1, lj_table(),
+                  method="none")
+    g.f.upload(np.vstack([a, np.zeros((g.nvp - n, 4), np.float32)]))
+    s = g.reduce_f4(g.f, 15)
+    assert np.array_equal(s, a.astype(np.int64).sum(axis=0).astype(np.float64))
+    b = rng.normal(size=(n, 4)).astype(np.float32)
+    g.f.upload(np.vstack([b, np.zeros((g.nvp - n, 4), np.float32)]))
+    s = g.reduce_f4(g.f, 0b1010)
+    assert s[0] == 0.0 and s[2] == 0.0
+    for c in (1, 3):
+        assert s[c] == pytest.approx(oracle.sum_f4(b, c), rel=1e-12, abs=1e-9)
+    s2 = g.reduce_f4(g.f, 0b1010)
+    assert np.array_equal(s, s2)  # fixed order: bitwise reproducible
+
+
+def test_momentum_and_kinetic_energy(golden_dir):
+    table = ka_table()
+    c, pos4, vel4, g = _golden_system(golden_dir, "KA_N864_L9.0_T0.50AltMass.xyz.gz", table, "n2")
+    g.calc_f(SP)
+    ps = float(c.meta["Nose-Hoover-Ps"])
+    g.set_nh(ps=ps, target_T=0.5)
+    import ctypes as C
+    from capi import lib, check, NhState
+    check(lib().rumdx_kinetic_energy(C.byref(g.p), C.cast(g.nh.value, C.POINTER(NhState)), 0.005, 0xFFFFFFFF,
+                                     g.out.value, g.ws.value, None))
+    ke = g.out.download(np.float64, 1)[0]
+    ke_ref = oracle.kinetic_energy(g.get("r"), g.get("v"), g.get("f"), 0.005, ps, 0xFFFFFFFF)
+    assert ke == pytest.approx(ke_ref, rel=1e-12)
+    check(lib().rumdx_zero_momentum(C.byref(g.p), g.ws.value, None))
+    v = g.get("v")
+    m = 1.0 / v[:, 3].astype(np.float64)
+    p = (v[:, :3].astype(np.float64) * m[:, None]).sum(axis=0)
+    assert np.all(np.abs(p) < 1e-4)

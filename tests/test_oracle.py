@@ -1,0 +1,273 @@
+"""Pins the CPU oracle (oracle/rumd_oracle.c) before it is trusted as the
+checker: RUMD-written golden configurations, closed forms, invariants and
+brute force on small inputs.  CPU only."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import csr_to_sets, force_errors, ka_table, lj_table, load_golden
+from rumd_amd import configs
+
+NS, SP, SF = 0, 1, 2
+
+
+# ------------------------------------------------------------------ golden vectors
+@pytest.mark.parametrize("name,median,p99,mx", [
+    ("KA_N1000_T0.466.xyz.gz", 3e-6, 3e-5, 1e-4),
+    ("KA_N256_L6.0_T0.50.xyz.gz", 3e-6, 3e-5, 1e-4),
+    ("KA_N864_L9.0_T0.50AltMass.xyz.gz", 3e-6, 3e-5, 1e-4),
+])
+def test_forces_match_rumd_written_fixtures(golden_dir, name, median, p99, mx):
+    """Kob-Andersen, shifted potential, rc = 2.5 sigma_ab: the FP64 oracle
+    reproduces the forces, per-particle energies and virials RUMD wrote."""
+    c = load_golden(golden_dir / name)
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    f4, vir = oracle.forces_brute(pos4, c.box, 2, ka_table(), SP)
+    e = force_errors(c.forces.astype(np.float64), f4[:, :3])
+    assert e["unguarded_median"] < median and e["unguarded_p99"] < p99 and e["unguarded_max"] < mx, e
+    np.testing.assert_allclose(f4[:, 3], c.pe, atol=2e-5)
+    np.testing.assert_allclose(vir / 2.0, c.vir, atol=5e-4)
+
+
+def test_fixture_rejects_plausible_mistakes(golden_dir):
+    """The fixture check above fails for a dropped shift, the SF method, the
+    wrong cut-off or a swapped A/B parameter set."""
+    c = load_golden(golden_dir / "KA_N1000_T0.466.xyz.gz")
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    good, _ = oracle.forces_brute(pos4, c.box, 2, ka_table(), SP)
+    ns, _ = oracle.forces_brute(pos4, c.box, 2, ka_table(), NS)
+    assert np.abs(ns[:, 3] - c.pe).max() > 1e-2          # dropped energy shift
+    sf, _ = oracle.forces_brute(pos4, c.box, 2, ka_table(), SF)
+    assert force_errors(c.forces.astype(np.float64), sf[:, :3])["unguarded_median"] > 1e-4
+    rc2, _ = oracle.forces_brute(pos4, c.box, 2, ka_table(2.0), SP)
+    assert force_errors(c.forces.astype(np.float64), rc2[:, :3])["unguarded_median"] > 1e-4
+    swapped = ka_table()[[3, 1, 2, 0]]
+    sw, _ = oracle.forces_brute(pos4, c.box, 2, swapped, SP)
+    assert force_errors(c.forces.astype(np.float64), sw[:, :3])["unguarded_median"] > 1e-2
+    assert force_errors(c.forces.astype(np.float64), good[:, :3])["unguarded_median"] < 3e-6
+
+
+def test_lj_fixture_pins_leapfrog_position_update(golden_dir):
+    """LJ_N1000_rho0.80_T0.80 stores forces computed before the last position
+    update r(t+dt) = r(t) + v(t+dt/2) dt: forces at r - v dt match, at r not."""
+    c = load_golden(golden_dir / "LJ_N1000_rho0.80_T0.80.xyz.gz")
+    dt = float(c.meta["dt"])
+    prev = c.positions.astype(np.float64) - c.velocities.astype(np.float64) * dt
+    prev -= c.box * np.rint(prev / c.box)
+    f_prev, _ = oracle.forces_brute(configs.pack_pos4(prev.astype(np.float32), c.types), c.box, 1, lj_table(), SP)
+    f_now, _ = oracle.forces_brute(configs.pack_pos4(c.positions, c.types), c.box, 1, lj_table(), SP)
+    e_prev = force_errors(c.forces.astype(np.float64), f_prev[:, :3])
+    e_now = force_errors(c.forces.astype(np.float64), f_now[:, :3])
+    assert e_prev["unguarded_median"] < 1e-5, e_prev
+    assert e_now["unguarded_median"] > 1e-3, e_now
+    # and the oracle's own leapfrog reproduces positions from r - v dt
+    pos4 = configs.pack_pos4(prev.astype(np.float32), c.types)
+    vel4 = configs.pack_vel4(c.velocities, c.types, c.masses)
+    vel4_before = vel4.copy()
+    # back out v(t - dt/2) from v(t + dt/2) with the file's forces (NVE part only)
+    vel4_before[:, :3] = (c.velocities.astype(np.float64) - c.forces.astype(np.float64) * dt).astype(np.float32)
+    im = np.zeros_like(pos4)
+    f4 = np.zeros_like(pos4)
+    f4[:, :3] = c.forces
+    oracle.nvt_step(pos4, vel4_before, f4, im, configs.box8(c.box), dt, 0.0, 0)
+    d = pos4[:, :3].astype(np.float64) - c.positions
+    d -= c.box * np.rint(d / c.box)
+    assert np.abs(d).max() < 1e-4
+
+
+# ------------------------------------------------------------------ closed forms
+def _pair_system(r, L=20.0):
+    pos4 = configs.pack_pos4(np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0]], np.float32), np.zeros(2, np.int32))
+    return pos4, np.array([L, L, L])
+
+
+@pytest.mark.parametrize("cutoff", [NS, SP, SF])
+def test_pair_closed_form(cutoff):
+    tab = lj_table(2.5)
+    for r in (0.95, 1.12, 1.5, 2.2, 2.49):
+        pos4, L = _pair_system(r)
+        f4, vir = oracle.forces_brute(pos4, L, 1, tab, cutoff)
+        r = float(np.float32(r))
+        u = 4.0 * (r ** -12 - r ** -6)
+        fmag = 24.0 * (2.0 * r ** -13 - r ** -7)  # -du/dr
+        rc = 2.5
+        uc, fc = 4.0 * (rc ** -12 - rc ** -6), 24.0 * (2.0 * rc ** -13 - rc ** -7)
+        if cutoff == SP:
+            u -= uc
+        elif cutoff == SF:
+            u = u - uc + fc * (r - rc)
+            fmag -= fc
+        # particle 0 at origin, 1 at +x: force on 0 points to -x when repulsive
+        assert f4[0, 0] == pytest.approx(-fmag, rel=1e-10)
+        assert f4[1, 0] == pytest.approx(fmag, rel=1e-10)
+        assert f4[0, 3] + f4[1, 3] == pytest.approx(u, rel=1e-10, abs=1e-14)
+        assert vir[0] == pytest.approx(r * fmag, rel=1e-10)
+
+
+def test_sf_continuity_and_finite_difference():
+    tab = lj_table(2.5)
+    pos4, L = _pair_system(2.5 - 1e-7)
+    f4, _ = oracle.forces_brute(pos4, L, 1, tab, SF)
+    assert abs(f4[0, 0]) < 1e-6 and abs(f4[0, 3] + f4[1, 3]) < 1e-7
+    for r in (1.05, 1.7, 2.3):
+        h = 1e-4
+        up = [oracle.forces_brute(_pair_system(x)[0], _pair_system(x)[1], 1, tab, SF)[0][:, 3].sum()
+              for x in (r - h, r + h)]
+        f4, _ = oracle.forces_brute(_pair_system(r)[0], _pair_system(r)[1], 1, tab, SF)
+        r32 = [float(np.float32(x)) for x in (r - h, r + h)]
+        dudr = (up[1] - up[0]) / (r32[1] - r32[0])
+        assert f4[1, 0] == pytest.approx(-dudr, rel=2e-4)
+
+
+def test_newton3_and_cells_vs_brute():
+    s = configs.ka_random_lattice(4000, density=1.2, seed=11)
+    pos4 = configs.pack_pos4(s.positions, s.types)
+    for cut in (SP, SF):
+        fb, vb = oracle.forces_brute(pos4, s.box, 2, ka_table(), cut)
+        fc, vc = oracle.forces_cells(pos4, s.box, 2, ka_table(), cut)
+        np.testing.assert_allclose(fc, fb, rtol=1e-12, atol=1e-10)
+        assert np.abs(fb[:, :3].sum(axis=0)).max() < 1e-8
+
+
+# ------------------------------------------------------------------ neighbour lists
+def test_nblist_cells_equal_brute_and_symmetric():
+    s = configs.ka_random_lattice(6000, density=1.2, seed=21)
+    pos4 = configs.pack_pos4(s.positions, s.types)
+    b8 = configs.box8(s.box)
+    cut2 = configs.cut_skin2_table(ka_table(), 0.3)
+    ob, ib = oracle.nblist_brute_f32(pos4, b8, 2, cut2)
+    nc = oracle.cell_grid(b8, 2.5, 0.3)
+    oc, ic = oracle.nblist_cells_f32(pos4, b8, 2, cut2, nc)
+    sb, sc = csr_to_sets(ob, ib), csr_to_sets(oc, ic)
+    assert sb == sc
+    for i, nb in enumerate(sb):
+        for j in nb:
+            assert i in sb[j]
+    # superset of the rc sphere (FP64 distances)
+    d = pos4[:, None, :3].astype(np.float64) - pos4[None, :200, :3]
+    d -= s.box * np.rint(d / s.box)
+    r = np.sqrt((d ** 2).sum(-1))
+    sig = np.array([[1.0, 0.8], [0.8, 0.88]])
+    t = s.types
+    inside = r < 2.5 * sig[t[:, None], t[None, :200]]
+    for j in range(200):
+        for i in np.nonzero(inside[:, j])[0]:
+            if i != j:
+                assert i in sb[j]
+    mean = (ob[-1]) / s.n
+    assert 85 < mean < 100  # n-bar ~ 92 for KA at rho=1.2, skin 0.3
+
+
+def test_cell_grid_matches_reference_formula():
+    b8 = configs.box8(95.6033)
+    assert list(oracle.cell_grid(b8, 2.5, 0.3)) == [68, 68, 68]
+    assert list(oracle.cell_grid(configs.box8(6.0), 2.5, 0.3)) == [5, 5, 5]
+
+
+def test_skin_exceeded_threshold():
+    pos4 = configs.pack_pos4(np.zeros((4, 3), np.float32), np.zeros(4, np.int32))
+    last = pos4.copy()
+    b8 = configs.box8(10.0)
+    assert not oracle.skin_exceeded(pos4, last, b8, 0.3)
+    pos4[2, 0] = np.float32(0.149)
+    assert not oracle.skin_exceeded(pos4, last, b8, 0.3)
+    pos4[2, 0] = np.float32(0.151)
+    assert oracle.skin_exceeded(pos4, last, b8, 0.3)
+    # minimum image: moving across the boundary is a small displacement
+    last[1, 0] = np.float32(4.99)
+    pos4[1, 0] = np.float32(-4.99)
+    pos4[2, 0] = 0.0
+    assert not oracle.skin_exceeded(pos4, last, b8, 0.3)
+
+
+# ------------------------------------------------------------------ integrator / thermostat / sums
+def test_nvt_step_closed_form_one_particle():
+    pos4 = configs.pack_pos4(np.array([[1.0, -2.0, 3.0]], np.float32), np.zeros(1, np.int32))
+    vel4 = np.array([[0.5, -0.25, 0.125, 0.5]], np.float32)  # 1/m = 0.5
+    f4 = np.array([[2.0, 4.0, -8.0, 0.0]], np.float32)
+    im = np.zeros_like(pos4)
+    dt, ps = np.float32(0.01), 0.3
+    v0 = vel4[0, :3].astype(np.float64)
+    fac = 0.5 * np.float32(ps) * dt
+    ke2 = oracle.nvt_step(pos4, vel4, f4, im, configs.box8(10.0), float(dt), ps, 0xFFFFFFFF)
+    v1 = (v0 * (1 - fac) + 0.5 * f4[0, :3] * dt) / (1 + fac)
+    np.testing.assert_allclose(vel4[0, :3], v1, rtol=1e-6)
+    np.testing.assert_allclose(pos4[0, :3], np.array([1.0, -2.0, 3.0]) + v1 * dt, rtol=1e-6)
+    assert ke2 == pytest.approx((v1 ** 2).sum() / 0.5, rel=1e-6)
+
+
+def test_nvt_wrap_and_images():
+    pos4 = configs.pack_pos4(np.array([[4.999, 0.0, -4.999]], np.float32), np.zeros(1, np.int32))
+    vel4 = np.array([[1.0, 0.0, -1.0, 1.0]], np.float32)
+    im = np.zeros_like(pos4)
+    oracle.nvt_step(pos4, vel4, np.zeros_like(pos4), im, configs.box8(10.0), 0.01, 0.0, 0)
+    assert pos4[0, 0] < -4.9 and pos4[0, 2] > 4.9
+    assert im[0, 0] == 1.0 and im[0, 2] == -1.0
+
+
+def test_nve_time_reversible_and_ps_zero_is_nve(golden_dir):
+    c = load_golden(golden_dir / "KA_N256_L6.0_T0.50.xyz.gz")
+    tab = ka_table()
+    b8 = configs.box8(c.box)
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    vel4 = configs.pack_vel4(c.velocities, c.types, c.masses)
+    im = np.zeros_like(pos4)
+    rlo = np.zeros_like(pos4)
+    p0, v0 = pos4.copy(), vel4.copy()
+    for _ in range(50):
+        f4, _ = oracle.forces_brute(pos4, c.box, 2, tab, SP)
+        oracle.nvt_step(pos4, vel4, f4.astype(np.float32), im, b8, 0.005, 0.0, 0, rlo4=rlo)
+    # leapfrog reversal: (r_N, -v_{N+1/2}) retraces the trajectory back to r_0
+    f4, _ = oracle.forces_brute(pos4, c.box, 2, tab, SP)
+    vel4[:, :3] = -(vel4[:, :3] + vel4[:, 3:4] * f4[:, :3].astype(np.float32) * np.float32(0.005))
+    for _ in range(50):
+        f4, _ = oracle.forces_brute(pos4, c.box, 2, tab, SP)
+        oracle.nvt_step(pos4, vel4, f4.astype(np.float32), im, b8, 0.005, 0.0, 0, rlo4=rlo)
+    d = pos4[:, :3].astype(np.float64) + rlo[:, :3] - p0[:, :3]
+    d -= c.box * np.rint(d / c.box)
+    assert np.abs(d).max() < 1e-4
+    # Ps = 0 with the thermostat on equals NVE bit for bit
+    a, b = p0.copy(), v0.copy()
+    a2, b2 = p0.copy(), v0.copy()
+    f4, _ = oracle.forces_brute(a, c.box, 2, tab, SP)
+    oracle.nvt_step(a, b, f4.astype(np.float32), np.zeros_like(a), b8, 0.005, 0.0, 0xFFFFFFFF)
+    oracle.nvt_step(a2, b2, f4.astype(np.float32), np.zeros_like(a), b8, 0.005, 0.0, 0)
+    assert np.array_equal(a, a2) and np.array_equal(b, b2)
+
+
+def test_thermostat_update_closed_form():
+    ps, T = oracle.thermostat_update(0.1, 0.5, 0.001, 3.0 * 999 * 0.6, 3 * 1000 - 3, 0.004, 987.0)
+    assert ps == pytest.approx(0.1 + 0.004 * 987.0 * (0.6 / 0.5 - 1.0), rel=1e-6)
+    assert T == pytest.approx(0.501)
+    ps, _ = oracle.thermostat_update(0.0, 0.1, 0.0, 1e6, 100, 0.004, 987.0)
+    assert ps == pytest.approx(0.004 * 987.0 * 10.0, rel=1e-6)  # runaway clamp
+
+
+def test_kinetic_energy_closed_form():
+    pos4 = configs.pack_pos4(np.zeros((2, 3), np.float32), np.array([0, 1], np.int32))
+    vel4 = np.array([[1.0, 0.0, 0.0, 1.0], [0.0, 2.0, 0.0, 0.5]], np.float32)
+    f4 = np.zeros_like(vel4)
+    ke = oracle.kinetic_energy(pos4, vel4, f4, 0.005, 0.0, 0)
+    assert ke == pytest.approx(0.5 * (1.0 * 1.0 + 2.0 * 4.0))
+
+
+def test_sum_integer_exact():
+    a = np.arange(40000, dtype=np.float32).reshape(10000, 4)
+    assert oracle.sum_f4(a, 1) == float(np.arange(1, 40000, 4, dtype=np.int64).sum())
+
+
+def test_md_run_nve_energy_conserved():
+    """The FP64 CPU MD (the CPU baseline) conserves energy in NVE to O(dt^2)."""
+    s = configs.lj_liquid(8, density=0.8442, temperature=0.72)
+    tab = lj_table()
+    pos = s.positions.astype(np.float64).copy()
+    vel = s.velocities.astype(np.float64).copy()
+    inv_m = np.ones(s.n)
+    b, pe0, _ = oracle.md_run(pos, vel, s.types, inv_m, s.box, 1, tab, SP, 0.3, 0.002, 0, False, 0.72, 0.0, 0.0)
+    e0 = pe0 + 0.5 * (vel ** 2).sum()
+    b, pe1, _ = oracle.md_run(pos, vel, s.types, inv_m, s.box, 1, tab, SP, 0.3, 0.002, 500, False, 0.72, 0.0, 0.0)
+    e1 = pe1 + 0.5 * (vel ** 2).sum()
+    assert b >= 2
+    assert abs(e1 - e0) / s.n < 5e-3
