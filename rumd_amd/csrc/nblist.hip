@@ -128,14 +128,14 @@ __global__ __launch_bounds__(kBlock) void k_build_n2(const float4* __restrict__ 
         const float d2 = min_image_d2_f32(ri, rj, bx);
         if (d2 < s_cut[ti * nt + __float_as_uint(rj.w)]) {
           if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * nvp + i] = j;
-          cnt += (cnt < max_nbrs) ? 1u : 0u;
+          ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
         }
       }
     }
   }
   if (live) {
     last_r[i] = ri;
-    num_nbrs[i] = cnt;
+    num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;  // == max_nbrs marks an overflowed list
     if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
     atomicMax(flags + RUMDX_FLAG_MAXLEN, cnt);
   }
@@ -345,13 +345,13 @@ __global__ __launch_bounds__(kBlock) void k_build_cells(
             const float d2 = min_image_d2_f32(ri, rj, bx);
             if (d2 < crow[__float_as_uint(rj.w)]) {
               if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * nvp + i] = j;
-              cnt += (cnt < max_nbrs) ? 1u : 0u;
+              ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
             }
           }
         }
       }
     }
-    num_nbrs[i] = cnt;
+    num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;  // == max_nbrs marks an overflowed list
     if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
     atomicMax(flags + RUMDX_FLAG_MAXLEN, cnt);
     slot_cell[i] = static_cast<uint32_t>(cx + cy * g.nc0 + cz * g.nc0 * g.nc1);

@@ -1,0 +1,221 @@
+// Python surface of the host classes, with the names, keyword arguments and
+// numpy shapes of RUMD's SWIG module (Swig/Sample.i, Swig/Potentials.i,
+// Swig/Integrators.i, Swig/SimulationBox.i, Swig/rumd.i).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "rumd_host.h"
+
+namespace py = pybind11;
+using f32arr = py::array_t<float, py::array::c_style | py::array::forcecast>;
+using i32arr = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+
+namespace {
+
+// Nx3 float32 in original particle order (Sample.i GetPositions etc.)
+py::array_t<float> get_xyz(Sample& s, const char* what) {
+  const unsigned n = s.GetNumberOfParticles();
+  if (!n) throw RUMD_Error("Sample", "Get", "Sample has no particles");
+  s.Synchronize();
+  std::vector<float> a(4ull * n);
+  s.GetParticleData().Get(what, a.data(), s.GetStream());
+  py::array_t<float> out({static_cast<py::ssize_t>(n), static_cast<py::ssize_t>(3)});
+  auto o = out.mutable_unchecked<2>();
+  for (unsigned i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) o(i, k) = a[4ull * i + k];
+  return out;
+}
+
+// Nx4 float32 (f: fx,fy,fz,u_i; w: w_mol,sxz,sxy,w_i; ...)
+py::array_t<float> get_x4(Sample& s, const char* what) {
+  const unsigned n = s.GetNumberOfParticles();
+  if (!n) throw RUMD_Error("Sample", "Get", "Sample has no particles");
+  s.Synchronize();
+  py::array_t<float> out({static_cast<py::ssize_t>(n), static_cast<py::ssize_t>(4)});
+  s.GetParticleData().Get(what, out.mutable_data(), s.GetStream());
+  return out;
+}
+
+const float* check_nx3(Sample& s, const f32arr& a) {
+  if (a.ndim() != 2 || a.shape(0) != s.GetNumberOfParticles() || a.shape(1) != 3)
+    throw py::value_error("Wrong array shape, must be Nx3");
+  return a.data();
+}
+
+void reset_lists(Sample& s) {
+  for (Potential* p : s.GetPotentials()) p->ResetInternalData();
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_rumd, m) {
+  m.doc() = "C++ host classes of rumd_amd (RUMD's Sample/Potential/Integrator API) over the gfx950 C ABI";
+
+  py::register_exception_translator([](std::exception_ptr p) {
+    try {
+      if (p) std::rethrow_exception(p);
+    } catch (const RUMD_cudaResourceError& e) {
+      PyErr_SetString(PyExc_ValueError, e.what());
+    } catch (const RUMD_Error& e) {
+      PyErr_SetString(PyExc_RuntimeError, e.what());
+    }
+  });
+
+  m.attr("VERSION") = "3.5";
+  m.def("GetVersion", []() { return std::string("3.5"); });
+  m.def("NumberOfPairPotentialParameters", []() { return RUMDX_NUM_PARAM; });
+
+  py::class_<Device, std::unique_ptr<Device, py::nodelete>>(m, "Device")
+      .def_static("GetDevice", &Device::GetDevice, py::return_value_policy::reference)
+      .def("GetDeviceName", &Device::GetDeviceName)
+      .def("GetDeviceReport", &Device::GetDeviceReport)
+      .def("GetDeviceMemory", &Device::GetDeviceMemory)
+      .def("Synchronize", &Device::Synchronize)
+      .def("Time", &Device::Time)
+      .def("CheckErrors", &Device::CheckErrors);
+
+  py::class_<SimulationBox>(m, "SimulationBox")
+      .def("GetVolume", &SimulationBox::GetVolume)
+      .def("GetLength", &SimulationBox::GetLength, py::arg("dir"))
+      .def("GetInfoString", &SimulationBox::GetInfoString, py::arg("precision") = 9);
+  py::class_<RectangularSimulationBox, SimulationBox>(m, "RectangularSimulationBox")
+      .def(py::init<float, float, float>(), py::arg("Lx"), py::arg("Ly"), py::arg("Lz"))
+      .def("SetLengths", &RectangularSimulationBox::SetLengths);
+
+  py::class_<Potential>(m, "Potential")
+      .def("SetVerbose", &Potential::SetVerbose, py::arg("vb"))
+      .def("SetID_String", &Potential::SetID_String, py::arg("set_ID_String"))
+      .def("GetID_String", &Potential::GetID_String)
+      .def("ResetInternalData", &Potential::ResetInternalData);
+
+  py::class_<PairPotential, Potential> pp(m, "PairPotential");
+  py::enum_<PairPotential::CutoffMethod>(pp, "CutoffMethod")
+      .value("NS", PairPotential::NS)
+      .value("SP", PairPotential::SP)
+      .value("SF", PairPotential::SF)
+      .export_values();
+  pp.def("GetNbListSkin", &PairPotential::GetNbListSkin)
+      .def("GetMaxCutoff", &PairPotential::GetMaxCutoff)
+      .def("GetNbListRebuildRequired", &PairPotential::GetNbListRebuildRequired)
+      .def("GetNumberOfNbListBuilds", &PairPotential::GetNumberOfNbListBuilds)
+      .def("SetNbListSkin", &PairPotential::SetNbListSkin, py::arg("skin"))
+      .def("SetNB_Method", &PairPotential::SetNB_Method, py::arg("method"))
+      .def("GetNB_Method", &PairPotential::GetNB_Method)
+      .def("SetNbListAllocateMax", &PairPotential::SetNbListAllocateMax, py::arg("set_allocate_max"))
+      .def("SetNbMaxNumNbrs", &PairPotential::SetNbMaxNumNbrs, py::arg("maxNumNbrs"))
+      .def("SetAssumeNewton3", &PairPotential::SetAssumeNewton3, py::arg("assume_N3"))
+      .def("GetPotentialParameter", &PairPotential::GetPotentialParameter, py::arg("i"), py::arg("j"), py::arg("index"))
+      .def("GetParameterList", &PairPotential::GetPotentialParameters, py::arg("num_types"))
+      .def("GetPotentialEnergy", &PairPotential::GetPotentialEnergy)
+      .def("GetActualMaxNumNbrs", &PairPotential::GetActualMaxNumNbrs)
+      .def("GetCutoffMethod", &PairPotential::GetCutoffMethod);
+
+  py::class_<Pot_IPL_12_6, PairPotential>(m, "Pot_IPL_12_6")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_IPL_12_6::SetParams, py::arg("i"), py::arg("j"), py::arg("A12"), py::arg("A6"),
+           py::arg("Rcut"));
+  py::class_<Pot_LJ_12_6, Pot_IPL_12_6>(m, "Pot_LJ_12_6")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_LJ_12_6::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  m.attr("NoShift") = PairPotential::NS;
+  m.attr("ShiftedPotential") = PairPotential::SP;
+  m.attr("ShiftedForce") = PairPotential::SF;
+
+  py::class_<Integrator>(m, "Integrator")
+      .def("Integrate", &Integrator::Integrate)
+      .def("SetTimeStep", &Integrator::SetTimeStep, py::arg("dt"))
+      .def("GetTimeStep", &Integrator::GetTimeStep)
+      .def("GetKineticEnergy", &Integrator::GetKineticEnergy, py::arg("copy") = true)
+      .def("SetMomentumToZero", &Integrator::SetMomentumToZero)
+      .def("GetInfoString", &Integrator::GetInfoString, py::arg("precision") = 9)
+      .def("InitializeFromInfoString", &Integrator::InitializeFromInfoString, py::arg("infoStr"),
+           py::arg("verbose") = true)
+      .def("GetTotalMomentum", &Integrator::GetTotalMomentum);
+  py::class_<IntegratorNVT, Integrator>(m, "IntegratorNVT")
+      .def(py::init<float, double, double>(), py::arg("timeStep"), py::arg("targetTemperature"),
+           py::arg("thermostatRelaxationTime") = 0.2)
+      .def(py::init<float>(), py::arg("timeStep"))
+      .def("SetRelaxationTime", &IntegratorNVT::SetRelaxationTime, py::arg("tau"))
+      .def("GetRelaxationTime", &IntegratorNVT::GetRelaxationTime)
+      .def("SetTargetTemperature", &IntegratorNVT::SetTargetTemperature, py::arg("T"), py::arg("reset") = true)
+      .def("GetTargetTemperature", &IntegratorNVT::GetTargetTemperature)
+      .def("SetThermostatState", &IntegratorNVT::SetThermostatState, py::arg("Ps"))
+      .def("GetThermostatState", &IntegratorNVT::GetThermostatState)
+      .def("GetThermostatKineticEnergy", &IntegratorNVT::GetThermostatKineticEnergy)
+      .def("SetThermostatOn", &IntegratorNVT::SetThermostatOn, py::arg("on") = true)
+      .def("SetThermostatOnParticleType", &IntegratorNVT::SetThermostatOnParticleType, py::arg("type"),
+           py::arg("on"))
+      .def("SetTemperatureChangePerTimeStep", &IntegratorNVT::SetTemperatureChangePerTimeStep,
+           py::arg("temperatureChangePerTimeStep"));
+  // %rename (IntegratorNVE) IntegratorNVT(float timeStep) (Swig/Integrators.i:16)
+  m.def("IntegratorNVE", [](float timeStep) { return new IntegratorNVT(timeStep); }, py::arg("timeStep"),
+        py::return_value_policy::take_ownership);
+
+  py::class_<Sample>(m, "Sample")
+      .def(py::init<unsigned, unsigned>(), py::arg("pb") = 0, py::arg("tp") = 0)
+      .def("SetNumberOfParticles", &Sample::SetNumberOfParticles, py::arg("num_part"))
+      .def("GetNumberOfParticles", &Sample::GetNumberOfParticles)
+      .def("GetNumberOfVirtualParticles", &Sample::GetNumberOfVirtualParticles)
+      .def("GetNumberOfTypes", &Sample::GetNumberOfTypes)
+      .def("GetNumberThisType", &Sample::GetNumberThisType, py::arg("type"))
+      .def("SetVerbose", &Sample::SetVerbose, py::arg("vb"))
+      .def("SetSimulationBox", &Sample::SetSimulationBox, py::arg("simBox"), py::keep_alive<1, 2>())
+      .def("GetSimulationBox", &Sample::GetSimulationBox, py::return_value_policy::reference_internal)
+      .def("SetPotential", &Sample::SetPotential, py::arg("potential"), py::keep_alive<1, 2>())
+      .def("AddPotential", &Sample::AddPotential, py::arg("potential"), py::keep_alive<1, 2>())
+      .def("SetIntegrator", &Sample::SetIntegrator, py::arg("newItg"), py::keep_alive<1, 2>())
+      .def("GetIntegrator", &Sample::GetIntegrator, py::return_value_policy::reference_internal)
+      .def("SetPB_TP", &Sample::SetPB_TP, py::arg("pb"), py::arg("tp"))
+      .def("GetParticlesPerBlock", &Sample::GetParticlesPerBlock)
+      .def("GetThreadsPerParticle", &Sample::GetThreadsPerParticle)
+      .def("SetMixedPrecisionPositions", &Sample::SetMixedPrecisionPositions, py::arg("on"))
+      .def("GetMixedPrecisionPositions", &Sample::GetMixedPrecisionPositions)
+      .def("CalcF", &Sample::CalcF, py::arg("calc_stresses") = false)
+      .def("GetPotentialEnergy", &Sample::GetPotentialEnergy)
+      .def("GetVirial", &Sample::GetVirial)
+      .def("GetStress", &Sample::GetStress)
+      .def("GetNumberOfDOFs", &Sample::GetNumberOfDOFs)
+      .def("SetNumberOfDOFs", &Sample::SetNumberOfDOFs, py::arg("DOFs"))
+      .def("SetMass", &Sample::SetMass, py::arg("type"), py::arg("mass"))
+      .def("GetMass", &Sample::GetMass, py::arg("type"))
+      .def("GetMeanMass", &Sample::GetMeanMass)
+      .def("SortParticles", &Sample::SortParticles)
+      .def("Synchronize", &Sample::Synchronize)
+      .def("RunSteps", &Sample::RunSteps, py::arg("first_step"), py::arg("nsteps"), py::arg("block") = 10,
+           py::arg("momentum_interval") = 100, py::call_guard<py::gil_scoped_release>())
+      .def("GetPositions", [](Sample& s) { return get_xyz(s, "r"); })
+      .def("GetVelocities", [](Sample& s) { return get_xyz(s, "v"); })
+      .def("GetForces", [](Sample& s) { return get_xyz(s, "f"); })
+      .def("GetImages", [](Sample& s) { return get_xyz(s, "im"); })
+      .def("GetForcesAndEnergies", [](Sample& s) { return get_x4(s, "f"); })
+      .def("GetVirials", [](Sample& s) { return get_x4(s, "w"); })
+      .def("GetStresses", [](Sample& s) { return get_x4(s, "sts"); })
+      .def("GetTypes", [](Sample& s) {
+        py::array_t<int32_t> out(s.GetNumberOfParticles());
+        s.GetParticleData().GetTypes(out.mutable_data(), s.GetStream());
+        return out;
+      })
+      .def("SetPositions", [](Sample& s, const f32arr& a) {
+        s.Synchronize();
+        s.GetParticleData().SetPositions(check_nx3(s, a), s.GetStream());
+        reset_lists(s);
+      }, py::arg("positions"))
+      .def("SetVelocities", [](Sample& s, const f32arr& a) {
+        s.Synchronize();
+        s.GetParticleData().SetVelocities(check_nx3(s, a), s.GetStream());
+      }, py::arg("velocities"))
+      .def("SetImages", [](Sample& s, const f32arr& a) {
+        s.Synchronize();
+        s.GetParticleData().SetImages(check_nx3(s, a), s.GetStream());
+      }, py::arg("images"))
+      .def("SetTypes", [](Sample& s, const i32arr& t) {
+        if (t.ndim() != 1 || t.shape(0) != s.GetNumberOfParticles())
+          throw py::value_error("Wrong vector length, must be N (number of particles)");
+        s.Synchronize();
+        s.GetParticleData().SetTypes(t.data(), s.GetStream());
+        for (Potential* p : s.GetPotentials()) p->SetSample(&s);
+        s.InvalidateGraphs();
+      }, py::arg("types"));
+}

@@ -1,0 +1,352 @@
+// PairPotential / Pot_IPL_12_6 / Pot_LJ_12_6 and NeighborList host logic
+// (include/rumd/PairPotential.h, src/PairPotential.cu, src/NeighborList.cu).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "rumd_host.h"
+
+namespace {
+constexpr unsigned kNumParam = RUMDX_NUM_PARAM;
+constexpr unsigned kMinSizeMemoryOpt = 10000;   // src/NeighborList.cu:27
+constexpr unsigned kDefaultMaxNumNbrs = 50;     // src/NeighborList.cu:28
+constexpr unsigned kSortThreshold = 8000;       // src/NeighborList.cu:31
+}  // namespace
+
+// ================================================================== NeighborList
+NeighborList::NeighborList() = default;
+
+void NeighborList::Initialize(Sample* s) {
+  sample_ = s;
+  allocated_ = false;
+  fused_check_pending_ = false;
+}
+
+void NeighborList::SetCutoffs(const PairPotential* pot) {
+  if (!sample_) return;
+  ntypes_ = sample_->GetNumberOfTypes();
+  cutoffs_.assign(ntypes_ * ntypes_, 0.f);
+  max_cutoff_ = 0.f;
+  for (unsigned i = 0; i < ntypes_; ++i)
+    for (unsigned j = 0; j < ntypes_; ++j) {
+      const float rc = pot->GetPotentialParameter(i, j, 0);
+      cutoffs_[i * ntypes_ + j] = rc;
+      max_cutoff_ = std::max(max_cutoff_, rc);
+    }
+  allocated_ = false;  // cut-offs (and possibly the cell grid) changed
+}
+
+void NeighborList::SetSkin(float skin) {
+  if (!(skin >= 0.f)) throw RUMD_Error("NeighborList", "SetSkin", "skin must be non-negative");
+  skin_ = skin;
+  allocated_ = false;
+  if (sample_) sample_->InvalidateGraphs();
+}
+
+void NeighborList::SetNB_Method(const std::string& m) {
+  if (m == "default") method_ = DEFAULT;
+  else if (m == "none") method_ = NONE;
+  else if (m == "n2") method_ = N2;
+  else if (m == "sort") method_ = SORT;
+  else throw RUMD_Error("NeighborList", "SetNB_Method", "unknown method '" + m + "' (default, none, n2, sort)");
+  allocated_ = false;
+  if (sample_) sample_->InvalidateGraphs();
+}
+
+std::string NeighborList::GetNB_Method() const {
+  switch (method_) {
+    case NONE: return "none";
+    case N2: return "n2";
+    case SORT: return "sort";
+    default: return "default";
+  }
+}
+
+NeighborList::Method NeighborList::GetResolvedMethod() const {
+  if (method_ != DEFAULT) return method_;
+  const unsigned n = sample_ ? sample_->GetNumberOfParticles() : 0;
+  return n > kSortThreshold ? SORT : N2;  // src/NeighborList.cu:586-587
+}
+
+void NeighborList::SetMaxNumNbrs(unsigned m) {
+  user_max_nbrs_ = m;
+  allocated_ = false;
+  if (sample_) sample_->InvalidateGraphs();
+}
+
+void NeighborList::ResetNeighborList() {
+  if (allocated_) rumdx_check(rumdx_nblist_request_rebuild(&desc_, sample_->GetStream()), "NeighborList", "ResetNeighborList");
+  fused_check_pending_ = false;
+}
+
+void NeighborList::Allocate() {
+  const unsigned n = sample_->GetNumberOfParticles();
+  const unsigned nvp = sample_->GetNumberOfVirtualParticles();
+  if (ntypes_ == 0 || cutoffs_.empty()) throw RUMD_Error("NeighborList", "Allocate", "cut-offs not set");
+  if (!sample_->GetSimulationBox()) throw RUMD_Error("NeighborList", "Allocate", "no simulation box");
+  const Method method = GetResolvedMethod();
+  // list capacity: NeighborList::UpdateNBlist (src/NeighborList.cu:595-610)
+  if (user_max_nbrs_ > 0) {
+    max_nbrs_ = std::min(user_max_nbrs_, n);
+  } else if (grown_max_nbrs_ > 0) {
+    max_nbrs_ = std::min(grown_max_nbrs_, n);
+  } else {
+    if (n < kMinSizeMemoryOpt || allocate_max_) {
+      max_nbrs_ = n;
+    } else {
+      const double rs = static_cast<double>(max_cutoff_) + skin_;
+      unsigned est = static_cast<unsigned>(1.5 * (4. / 3.) * M_PI * rs * rs * rs * n /
+                                           sample_->GetSimulationBox()->GetVolume());
+      max_nbrs_ = std::max(est, kDefaultMaxNumNbrs);
+      max_nbrs_ = std::min(max_nbrs_, n);
+    }
+  }
+  if (max_nbrs_ == 0) max_nbrs_ = 1;
+  if (static_cast<uint64_t>(max_nbrs_) * nvp >= (1ull << 32))
+    throw RUMD_cudaResourceError("NeighborList", "Allocate", "neighbour list exceeds 2^32 entries");
+  const hipStream_t st = sample_->GetStream();
+  nbl_.resize(static_cast<size_t>(max_nbrs_) * nvp);
+  num_nbrs_.resize(nvp);
+  last_r_.resize(4ull * nvp);
+  flags_.resize(8);
+  std::vector<float> cs2(ntypes_ * ntypes_);
+  for (size_t k = 0; k < cs2.size(); ++k) {
+    const float c = cutoffs_[k] + skin_;  // squared in float (src/NeighborList.cu:1239-1240)
+    cs2[k] = c * c;
+  }
+  cut_skin2_.resize(cs2.size());
+  cut_skin2_.upload(cs2.data(), cs2.size(), st);
+  std::memset(&desc_, 0, sizeof(desc_));
+  desc_.nbl_dev = nbl_.get();
+  desc_.num_nbrs_dev = num_nbrs_.get();
+  desc_.last_r_dev = last_r_.get();
+  desc_.flags_dev = flags_.get();
+  desc_.cut_skin2_dev = cut_skin2_.get();
+  desc_.max_nbrs = max_nbrs_;
+  desc_.ntypes = ntypes_;
+  desc_.skin = skin_;
+  if (method == SORT) {
+    int pair_pots = 0;
+    for (Potential* p : sample_->GetPotentials())
+      if (dynamic_cast<PairPotential*>(p)) ++pair_pots;
+    if (pair_pots > 1)
+      throw RUMD_Error("NeighborList", "Allocate",
+                       "the sort method reorders particles and serves one pair potential per sample; "
+                       "use SetNB_Method('n2') for the others");
+    int32_t nc[3];
+    rumdx_check(rumdx_cell_grid(sample_->GetSimulationBox()->GetBox8(), max_cutoff_, skin_, nc), "NeighborList",
+                "Allocate");
+    const size_t ncells = static_cast<size_t>(nc[0]) * nc[1] * nc[2];
+    std::vector<uint32_t> rank(ncells);
+    rumdx_check(rumdx_cell_rank_table(nc, rank.data()), "NeighborList", "Allocate");
+    cell_rank_.resize(ncells);
+    cell_rank_.upload(rank.data(), ncells, st);
+    cell_start_.resize(ncells + 1);
+    cell_count_.resize(ncells);
+    slot_cell_.resize(nvp);
+    sort_scratch_.resize(rumdx_sort_scratch_bytes(nvp, static_cast<uint32_t>(ncells)));
+    std::memcpy(desc_.nc, nc, sizeof(nc));
+    desc_.cell_rank_dev = cell_rank_.get();
+    desc_.cell_start_dev = cell_start_.get();
+    desc_.cell_count_dev = cell_count_.get();
+    desc_.slot_cell_dev = slot_cell_.get();
+    desc_.sort_scratch_dev = sort_scratch_.get();
+  } else {
+    cell_rank_.release();
+    cell_start_.release();
+    cell_count_.release();
+    slot_cell_.release();
+    sort_scratch_.release();
+  }
+  allocated_nvp_ = nvp;
+  allocated_ = true;
+  fused_check_pending_ = false;
+  rumdx_check(rumdx_nblist_request_rebuild(&desc_, st), "NeighborList", "Allocate");
+  sample_->InvalidateGraphs();
+}
+
+void NeighborList::EnsureAllocated() {
+  if (!sample_ || GetResolvedMethod() == NONE) return;
+  if (!allocated_ || allocated_nvp_ != sample_->GetNumberOfVirtualParticles()) Allocate();
+}
+
+void NeighborList::UpdateNBlist() {
+  if (!sample_) throw RUMD_Error("NeighborList", "UpdateNBlist", "no sample");
+  const Method method = GetResolvedMethod();
+  if (method == NONE) return;
+  EnsureAllocated();
+  rumdx_particles* p = sample_->GetParticleData().Descriptor();
+  const float* box = sample_->GetSimulationBox()->GetBox8();
+  const hipStream_t st = sample_->GetStream();
+  if (!fused_check_pending_) rumdx_check(rumdx_nblist_check(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
+  fused_check_pending_ = false;
+  if (method == SORT)
+    rumdx_check(rumdx_nblist_build_sorted(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
+  else
+    rumdx_check(rumdx_nblist_build_n2(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
+}
+
+bool NeighborList::HandleOverflow() {
+  if (!allocated_) return false;
+  uint32_t f[8];
+  flags_.download(f, 8, sample_->GetStream());
+  if (!f[RUMDX_FLAG_OVERFLOW]) return false;
+  const unsigned n = sample_->GetNumberOfParticles();
+  // the build counts past capacity, so MAXLEN is the true longest list
+  unsigned grown = std::min(std::max(max_nbrs_ + max_nbrs_ / 2, f[RUMDX_FLAG_MAXLEN] + f[RUMDX_FLAG_MAXLEN] / 4 + 8), n);
+  if (grown <= max_nbrs_) grown = n;
+  if (sample_->GetVerbose())
+    fprintf(stderr, "[Info] NeighborList: list overflowed (longest %u); growing capacity %u -> %u\n",
+            f[RUMDX_FLAG_MAXLEN], max_nbrs_, grown);
+  if (user_max_nbrs_) user_max_nbrs_ = grown;
+  grown_max_nbrs_ = grown;
+  allocated_ = false;
+  Allocate();
+  return true;
+}
+
+unsigned NeighborList::GetActualMaxNumNbrs() {
+  if (!allocated_) return 0;
+  uint32_t f[8];
+  flags_.download(f, 8, sample_->GetStream());
+  return f[RUMDX_FLAG_MAXLEN];
+}
+
+int NeighborList::GetRebuildRequired() {
+  if (!allocated_) return 1;
+  uint32_t f[8];
+  flags_.download(f, 8, sample_->GetStream());
+  return f[RUMDX_FLAG_REBUILD] ? 1 : 0;
+}
+
+unsigned NeighborList::GetNumberOfBuilds() {
+  if (!allocated_) return 0;
+  uint32_t f[8];
+  flags_.download(f, 8, sample_->GetStream());
+  return f[RUMDX_FLAG_BUILDS];
+}
+
+// ================================================================== PairPotential
+PairPotential::PairPotential(CutoffMethod m) : cutoff_method_(m) {
+  if (m < NS || m > SF) throw RUMD_Error("PairPotential", "PairPotential", "Invalid cutoffMethod");
+  SetID_String("base_pair_pot");
+}
+
+void PairPotential::SetParamsRow(unsigned i, unsigned j, const std::vector<float>& row) {
+  if (i >= RUMDX_MAX_TYPES || j >= RUMDX_MAX_TYPES) throw RUMD_Error(GetID_String(), "SetParams", "type out of range");
+  if (row.size() + 2 > kNumParam)
+    throw RUMD_Error(GetID_String(), "CopyParamsToGPU", "more parameters than NumParam allows");
+  std::vector<float> p(row);
+  p.resize(kNumParam, 0.f);
+  ComputeShifts(p);
+  params_map_[{i, j}] = p;
+  if (i != j && assume_newton3_) params_map_[{j, i}] = p;
+  if (sample) {
+    UploadParams();
+    neighborList.SetCutoffs(this);
+    sample->InvalidateGraphs();
+  }
+}
+
+float PairPotential::GetPotentialParameter(unsigned i, unsigned j, unsigned idx) const {
+  auto it = params_map_.find({i, j});
+  if (it == params_map_.end() || idx >= it->second.size()) return 0.f;
+  return it->second[idx];
+}
+
+std::vector<float> PairPotential::GetPotentialParameters(unsigned num_types) const {
+  std::vector<float> v;
+  for (unsigned i = 0; i < num_types; ++i)
+    for (unsigned j = 0; j < num_types; ++j)
+      for (unsigned p = 0; p < kNumParam - 2; ++p) v.push_back(GetPotentialParameter(i, j, p));
+  return v;
+}
+
+void PairPotential::UploadParams() {
+  const unsigned nt = sample->GetNumberOfTypes();
+  std::vector<float> tab(static_cast<size_t>(nt) * nt * kNumParam, 0.f);
+  for (unsigned i = 0; i < nt; ++i)
+    for (unsigned j = 0; j < nt; ++j) {
+      auto it = params_map_.find({i, j});
+      if (it != params_map_.end())
+        std::copy(it->second.begin(), it->second.end(), tab.begin() + (static_cast<size_t>(i) * nt + j) * kNumParam);
+    }
+  if (d_params_.size() != tab.size()) d_params_.resize(tab.size());
+  d_params_.upload(tab.data(), tab.size(), sample->GetStream());
+  table_types_ = nt;
+}
+
+void PairPotential::Initialize() {
+  UploadParams();
+  neighborList.Initialize(sample);
+  neighborList.SetCutoffs(this);
+}
+
+void PairPotential::CalcF(bool initialize, bool calc_stresses) {
+  if (!sample) throw RUMD_Error(GetID_String(), "CalcF", "no sample");
+  if (table_types_ != sample->GetNumberOfTypes()) Initialize();
+  neighborList.UpdateNBlist();
+  rumdx_nblist* nb = neighborList.GetResolvedMethod() == NeighborList::NONE ? nullptr : neighborList.Descriptor();
+  rumdx_check(rumdx_pair_lj(sample->GetParticleData().Descriptor(), nb, d_params_.get(), table_types_,
+                            sample->GetSimulationBox()->GetBox8(), cutoff_method_, initialize ? 1 : 0,
+                            calc_stresses ? 1 : 0, sample->GetStream()),
+              GetID_String().c_str(), "CalcF");
+}
+
+// PairPotential::GetPotentialEnergy (src/PairPotential.cu:252-267): this
+// potential's energy alone, computed into private arrays, summed in FP64.
+double PairPotential::GetPotentialEnergy() {
+  if (!sample) throw RUMD_Error(GetID_String(), "GetPotentialEnergy", "no sample");
+  if (table_types_ != sample->GetNumberOfTypes()) Initialize();
+  neighborList.UpdateNBlist();
+  const unsigned nvp = sample->GetNumberOfVirtualParticles();
+  if (pe_f_.size() != 4ull * nvp) {
+    pe_f_.resize(4ull * nvp);
+    pe_w_.resize(4ull * nvp);
+  }
+  rumdx_particles p = *sample->GetParticleData().Descriptor();
+  p.f_dev = pe_f_.get();
+  p.w_dev = pe_w_.get();
+  rumdx_nblist* nb = neighborList.GetResolvedMethod() == NeighborList::NONE ? nullptr : neighborList.Descriptor();
+  const hipStream_t st = sample->GetStream();
+  rumdx_check(rumdx_pair_lj(&p, nb, d_params_.get(), table_types_, sample->GetSimulationBox()->GetBox8(),
+                            cutoff_method_, 1, 0, st),
+              GetID_String().c_str(), "GetPotentialEnergy");
+  rumdx_check(rumdx_reduce_f4(pe_f_.get(), p.n, 8u, sample->ScalarOut(), sample->Workspace(), st),
+              GetID_String().c_str(), "GetPotentialEnergy");
+  double out[4];
+  hip_check(hipMemcpyAsync(out, sample->ScalarOut(), sizeof(out), hipMemcpyDeviceToHost, st), "PairPotential",
+            "GetPotentialEnergy");
+  hip_check(hipStreamSynchronize(st), "PairPotential", "GetPotentialEnergy");
+  return out[3];
+}
+
+// ================================================================== Pot_IPL_12_6 / Pot_LJ_12_6
+void Pot_IPL_12_6::SetParams(unsigned i, unsigned j, float A12, float A6, float Rcut) {
+  if (verbose && sample && sample->GetVerbose())
+    printf("%s: A12 = %g, A6 = %g, Rcut = %g.\n", GetID_String().c_str(), A12, A6, Rcut);
+  SetParamsRow(i, j, {Rcut, A12, A6});
+}
+
+// u(rc) and s(rc) rc for the shifted potential / shifted force cut-offs
+// (CopyParamsToGPU, src/PairPotential.cu:82-125), formed in FP64 from the
+// stored float parameters.
+void Pot_IPL_12_6::ComputeShifts(std::vector<float>& row) const {
+  const double rc = row[0], a12 = row[1], a6 = row[2];
+  double u = 0.0, s = 0.0;
+  if (rc > 0.0) {
+    const double i2 = 1.0 / (rc * rc), i6 = i2 * i2 * i2;
+    u = i6 * (a12 * i6 + a6);
+    s = i6 * i2 * (12.0 * a12 * i6 + 6.0 * a6);
+  }
+  row[kNumParam - 1] = static_cast<float>(u);
+  row[kNumParam - 2] = static_cast<float>(s * rc);
+}
+
+void Pot_LJ_12_6::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  if (verbose && sample && sample->GetVerbose())
+    printf("%s: Sigma = %g, Epsilon = %g, Rcut = %g.\n", GetID_String().c_str(), Sigma, Epsilon, Rcut);
+  // include/rumd/PairPotential.h:202-207
+  Pot_IPL_12_6::SetParams(i, j, static_cast<float>(4. * Epsilon * std::pow(static_cast<double>(Sigma), 12.)),
+                          static_cast<float>(-4. * Epsilon * std::pow(static_cast<double>(Sigma), 6.)), Rcut * Sigma);
+}

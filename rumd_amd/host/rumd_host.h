@@ -1,0 +1,421 @@
+// Host-side C++ classes mirroring RUMD's Sample / Potential / Integrator API
+// (include/rumd/Sample.h, Potential.h, PairPotential.h, NeighborList.h,
+// Integrator.h, IntegratorNVT.h, SimulationBox.h, ParticleData.h,
+// RUMD_Error.h, Device.h).  Every computation on particles runs on the GPU
+// through the C ABI in include/rumdx.h; this layer owns device memory, the
+// stream, parameter tables and the step schedule.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "rumdx.h"
+
+// ------------------------------------------------------------------ errors
+// RUMD_Error (include/rumd/RUMD_Error.h:6-31): class, method and message.
+class RUMD_Error : public std::runtime_error {
+ public:
+  RUMD_Error(const std::string& className, const std::string& methodName, const std::string& errorStr)
+      : std::runtime_error(className + "::" + methodName + ": " + errorStr),
+        className(className),
+        methodName(methodName),
+        errorStr(errorStr) {}
+  std::string className, methodName, errorStr;
+};
+
+// Resource errors map to ValueError in Python (Swig/rumd.i:41-53).
+class RUMD_cudaResourceError : public RUMD_Error {
+ public:
+  using RUMD_Error::RUMD_Error;
+};
+
+void hip_check(hipError_t e, const char* cls, const char* method);
+void rumdx_check(int rc, const char* cls, const char* method);
+
+// ------------------------------------------------------------------ device memory
+template <class T>
+class DeviceArray {
+ public:
+  DeviceArray() = default;
+  explicit DeviceArray(size_t n) { resize(n); }
+  ~DeviceArray() { release(); }
+  DeviceArray(const DeviceArray&) = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+  DeviceArray(DeviceArray&& o) noexcept : ptr_(o.ptr_), n_(o.n_) { o.ptr_ = nullptr; o.n_ = 0; }
+  DeviceArray& operator=(DeviceArray&& o) noexcept {
+    if (this != &o) {
+      release();
+      ptr_ = o.ptr_;
+      n_ = o.n_;
+      o.ptr_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  // Allocates n elements, zero-filled (synchronous; never on a hot path).
+  void resize(size_t n);
+  void release();
+  T* get() const { return ptr_; }
+  size_t size() const { return n_; }
+  void upload(const T* h, size_t n, hipStream_t st);
+  void download(T* h, size_t n, hipStream_t st) const;
+
+ private:
+  T* ptr_ = nullptr;
+  size_t n_ = 0;
+};
+
+// ------------------------------------------------------------------ Device
+// Device singleton (src/Device.cu:143-222): name, memory, sync, timing.
+class Device {
+ public:
+  static Device& GetDevice();
+  void Init();
+  std::string GetDeviceName();
+  std::string GetDeviceReport();
+  size_t GetDeviceMemory();
+  void Synchronize();
+  float Time();  // seconds since the first call, measured with hipEvents
+  void CheckErrors();
+
+ private:
+  Device() = default;
+  bool initialized_ = false;
+  hipDeviceProp_t prop_{};
+  hipEvent_t start_ = nullptr;
+};
+
+// ------------------------------------------------------------------ boxes
+class SimulationBox {
+ public:
+  virtual ~SimulationBox() = default;
+  virtual float GetVolume() const = 0;
+  virtual float GetLength(int dir) const = 0;
+  virtual std::string GetInfoString(unsigned precision) const = 0;
+  // RUMD's float[8] {cubic, Lx, Ly, Lz, 1/Lx, 1/Ly, 1/Lz, shift}
+  const float* GetBox8() const { return box8_; }
+
+ protected:
+  float box8_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+// RectangularSimulationBox (include/rumd/SimulationBox.h:96-190).
+class RectangularSimulationBox : public SimulationBox {
+ public:
+  RectangularSimulationBox(float Lx, float Ly, float Lz);
+  float GetVolume() const override;
+  float GetLength(int dir) const override;
+  std::string GetInfoString(unsigned precision) const override;
+  void SetLengths(float Lx, float Ly, float Lz);
+};
+
+class Sample;
+class ParticleData;
+
+// ------------------------------------------------------------------ potentials
+// Potential (include/rumd/Potential.h:16-63).
+class Potential {
+ public:
+  virtual ~Potential() = default;
+  virtual void CalcF(bool initialize, bool calc_stresses) = 0;
+  virtual bool EnergyIncludedInParticleSum() const = 0;
+  virtual void SetSample(Sample* s);
+  virtual void Initialize() = 0;
+  virtual void ResetInternalData() {}
+  virtual double GetPotentialEnergy() { return 0.0; }
+  virtual double GetVirial() { return 0.0; }
+  void SetVerbose(bool vb) { verbose = vb; }
+  void SetID_String(const std::string& s) { ID_String = s; }
+  std::string GetID_String() const { return ID_String; }
+
+ protected:
+  Sample* sample = nullptr;
+  bool verbose = true;
+  std::string ID_String;
+};
+
+class PairPotential;
+
+// NeighborList (include/rumd/NeighborList.h, src/NeighborList.cu).  Owns the
+// list, its rebuild flags and the cell arrays of the sorted method.
+class NeighborList {
+ public:
+  enum Method { DEFAULT, NONE, N2, SORT };
+  NeighborList();
+  void Initialize(Sample* s);
+  void SetCutoffs(const PairPotential* pot);
+  void SetSkin(float skin);
+  float GetSkin() const { return skin_; }
+  float GetMaxCutoff() const { return max_cutoff_; }
+  void SetNB_Method(const std::string& method);
+  std::string GetNB_Method() const;
+  Method GetResolvedMethod() const;
+  void SetMaxNumNbrs(unsigned m);
+  void SetAllocateMax(bool b) { allocate_max_ = b; }
+  unsigned GetActualMaxNumNbrs();
+  int GetRebuildRequired();
+  unsigned GetNumberOfBuilds();
+  // Marks the list stale: the next UpdateNBlist rebuilds (ResetNeighborList).
+  void ResetNeighborList();
+  // Launches the device-side check and build chain (UpdateNBlist).
+  void UpdateNBlist();
+  // Allocates buffers now (before a graph capture, which may not allocate).
+  void EnsureAllocated();
+  // Called after an integrator step that ran this list's skin check fused.
+  void NoteFusedCheck() { fused_check_pending_ = true; }
+  void ResetFusedCheck() { fused_check_pending_ = false; }
+  // Reads the flags; grows the list after an overflow.  Synchronises.
+  bool HandleOverflow();
+  bool IsSorted() const { return GetResolvedMethod() == SORT; }
+  rumdx_nblist* Descriptor() { return &desc_; }
+
+ private:
+  void Allocate();
+  Sample* sample_ = nullptr;
+  Method method_ = DEFAULT;
+  float skin_ = 0.5f;  // reference default (src/NeighborList.cu:50)
+  float max_cutoff_ = 0.f;
+  unsigned max_nbrs_ = 0;
+  unsigned user_max_nbrs_ = 0;
+  unsigned grown_max_nbrs_ = 0;  // capacity after an overflow
+  bool allocate_max_ = false;
+  bool fused_check_pending_ = false;
+  bool allocated_ = false;
+  unsigned ntypes_ = 0;
+  std::vector<float> cutoffs_;  // rc_ij
+  DeviceArray<uint32_t> nbl_, num_nbrs_, flags_, cell_rank_, cell_start_, cell_count_, slot_cell_;
+  DeviceArray<float> last_r_, cut_skin2_;
+  DeviceArray<char> sort_scratch_;
+  rumdx_nblist desc_{};
+  uint32_t allocated_nvp_ = 0;
+};
+
+// PairPotential (include/rumd/PairPotential.h:33-128, src/PairPotential.cu).
+class PairPotential : public Potential {
+ public:
+  enum CutoffMethod { NS = 0, SP = 1, SF = 2 };
+  explicit PairPotential(CutoffMethod m);
+  void CalcF(bool initialize, bool calc_stresses) override;
+  bool EnergyIncludedInParticleSum() const override { return true; }
+  void Initialize() override;
+  void ResetInternalData() override { neighborList.ResetNeighborList(); }
+  double GetPotentialEnergy() override;
+  // Parameter rows: (Rcut, A12, A6, ...); shifts appended (CopyParamsToGPU).
+  void SetParamsRow(unsigned i, unsigned j, const std::vector<float>& row);
+  float GetPotentialParameter(unsigned i, unsigned j, unsigned idx) const;
+  std::vector<float> GetPotentialParameters(unsigned num_types) const;
+  void SetNbListSkin(float skin) { neighborList.SetSkin(skin); }
+  float GetNbListSkin() const { return neighborList.GetSkin(); }
+  float GetMaxCutoff() const { return neighborList.GetMaxCutoff(); }
+  void SetNB_Method(const std::string& m) { neighborList.SetNB_Method(m); }
+  std::string GetNB_Method() const { return neighborList.GetNB_Method(); }
+  void SetNbListAllocateMax(bool b) { neighborList.SetAllocateMax(b); }
+  void SetNbMaxNumNbrs(unsigned m) { neighborList.SetMaxNumNbrs(m); }
+  unsigned GetActualMaxNumNbrs() { return neighborList.GetActualMaxNumNbrs(); }
+  int GetNbListRebuildRequired() { return neighborList.GetRebuildRequired(); }
+  unsigned GetNumberOfNbListBuilds() { return neighborList.GetNumberOfBuilds(); }
+  int GetCutoffMethod() const { return cutoff_method_; }
+  void SetAssumeNewton3(bool b) { assume_newton3_ = b; }
+  NeighborList& GetNeighborList() { return neighborList; }
+  // Shift constants u(rc), s(rc) rc of one parameter row.
+  virtual void ComputeShifts(std::vector<float>& row) const = 0;
+  unsigned NumTypesInTable() const { return table_types_; }
+
+ protected:
+  void UploadParams();
+  NeighborList neighborList;
+  CutoffMethod cutoff_method_;
+  bool assume_newton3_ = true;
+  std::map<std::pair<unsigned, unsigned>, std::vector<float>> params_map_;
+  DeviceArray<float> d_params_;
+  unsigned table_types_ = 0;
+  DeviceArray<float> pe_f_, pe_w_;  // private arrays for GetPotentialEnergy
+};
+
+// Pot_IPL_12_6 (include/rumd/PairPotential.h:134-169).
+class Pot_IPL_12_6 : public PairPotential {
+ public:
+  explicit Pot_IPL_12_6(CutoffMethod m) : PairPotential(m) { SetID_String("potIPL_12_6"); }
+  void SetParams(unsigned i, unsigned j, float A12, float A6, float Rcut);
+  void ComputeShifts(std::vector<float>& row) const override;
+};
+
+// Pot_LJ_12_6 (include/rumd/PairPotential.h:184-208).
+class Pot_LJ_12_6 : public Pot_IPL_12_6 {
+ public:
+  explicit Pot_LJ_12_6(CutoffMethod m) : Pot_IPL_12_6(m) { SetID_String("potLJ_12_6"); }
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+};
+
+// ------------------------------------------------------------------ integrators
+// Integrator (include/rumd/Integrator.h:18-70).
+class Integrator {
+ public:
+  virtual ~Integrator() = default;
+  virtual void SetSample(Sample* s) { S = s; }
+  virtual void Integrate() = 0;
+  virtual std::string GetInfoString(unsigned precision) const = 0;
+  virtual void InitializeFromInfoString(const std::string& info, bool verbose = true) = 0;
+  virtual void SetMomentumToZero() {}
+  virtual double GetKineticEnergy(bool copy = true) = 0;
+  virtual void SetTimeStep(float dt) { timeStep = dt; }
+  virtual float GetTimeStep() const { return timeStep; }
+  std::vector<double> GetTotalMomentum();
+
+ protected:
+  Sample* S = nullptr;
+  float timeStep = 1.f;
+};
+
+// IntegratorNVT (include/rumd/IntegratorNVT.h, src/IntegratorNVT.cu): NVE
+// leapfrog when built with only a time step, Nose-Hoover otherwise.
+class IntegratorNVT : public Integrator {
+ public:
+  explicit IntegratorNVT(float timeStep);  // NVE
+  IntegratorNVT(float timeStep, double targetTemperature, double thermostatRelaxationTime);
+  void SetSample(Sample* s) override;
+  void Integrate() override;
+  std::string GetInfoString(unsigned precision) const override;
+  void InitializeFromInfoString(const std::string& info, bool verbose = true) override;
+  void SetMomentumToZero() override;
+  double GetKineticEnergy(bool copy = true) override;
+  void SetTimeStep(float dt) override;
+  void SetRelaxationTime(float tau);
+  float GetRelaxationTime() const { return relaxation_time_; }
+  void SetTargetTemperature(double T, bool reset = true);
+  double GetTargetTemperature();
+  void SetThermostatState(double ps);
+  double GetThermostatState();
+  double GetThermostatKineticEnergy();  // last FP64 sum of m v^2 / 2
+  void SetThermostatOn(bool on);
+  void SetThermostatOnParticleType(unsigned type, bool on);
+  unsigned GetThermostatOnMask() const { return thermostat_on_; }
+  void SetTemperatureChangePerTimeStep(double dT);
+
+ private:
+  void PushState();
+  void PullState();
+  unsigned thermostat_on_ = 0;
+  float relaxation_time_ = 0.2f;
+  rumdx_nh_state host_state_{0.0, 0.0, 0.0, 0.0};
+  DeviceArray<rumdx_nh_state> d_state_;
+};
+
+// ------------------------------------------------------------------ particle data
+// ParticleData (include/rumd/ParticleData.h): device arrays in slot order,
+// host mirrors in original particle order.
+class ParticleData {
+ public:
+  void SetNumberOfParticles(unsigned n, unsigned pb, bool mixed_positions);
+  unsigned GetNumberOfParticles() const { return n_; }
+  unsigned GetNumberOfVirtualParticles() const { return nvp_; }
+  unsigned GetNumberOfTypes() const { return static_cast<unsigned>(masses_.size()); }
+  unsigned GetNumberThisType(unsigned t) const;
+  void SetNumberOfTypes(unsigned nt);
+  void SetMass(unsigned type, float mass);
+  float GetMass(unsigned type) const;
+  bool MixedPositions() const { return mixed_; }
+  // host <-> device in original order (the unsorted index is applied here)
+  void SetPositions(const float* xyz, hipStream_t st);
+  void SetTypes(const int32_t* types, hipStream_t st);
+  void SetVelocities(const float* xyz, hipStream_t st);
+  void SetImages(const float* xyz, hipStream_t st);
+  void SetForces(const float* f4, hipStream_t st);
+  void Get(const char* what, float* out4, hipStream_t st);  // "r","v","f","w","im","sts": n x 4, original order
+  void GetTypes(int32_t* out, hipStream_t st);
+  void ResetSorting(hipStream_t st);
+  rumdx_particles* Descriptor() { return &desc_; }
+  std::vector<uint32_t> UnsortedIndex(hipStream_t st);
+
+  DeviceArray<float> d_r, d_v, d_f, d_w, d_im, d_sts, d_rlo;
+  DeviceArray<uint32_t> d_unsorted_index;
+
+ private:
+  void UpdateMassesOnDevice(hipStream_t st);
+  void Scatter(const std::vector<float>& slot4, float* out4, hipStream_t st);
+  std::vector<float> GatherToSlots(const float* orig4, hipStream_t st);
+  unsigned n_ = 0, nvp_ = 0;
+  bool mixed_ = false;
+  std::vector<float> masses_;
+  std::vector<int32_t> types_;  // original order
+  rumdx_particles desc_{};
+};
+
+// ------------------------------------------------------------------ Sample
+// Sample (include/rumd/Sample.h, src/Sample.cu): particle data, box, potential
+// list and integrator; CalcF and the FP64 sums.  One HIP stream per sample.
+class Sample {
+ public:
+  Sample(unsigned pb, unsigned tp);
+  ~Sample();
+  Sample(const Sample&) = delete;
+  Sample& operator=(const Sample&) = delete;
+
+  void SetNumberOfParticles(unsigned n);
+  void SetSimulationBox(SimulationBox* box);
+  SimulationBox* GetSimulationBox() const { return box_; }
+  void SetPotential(Potential* p);
+  void AddPotential(Potential* p);
+  void SetIntegrator(Integrator* itg);
+  Integrator* GetIntegrator() const { return itg_; }
+  void SetVerbose(bool vb) { verbose_ = vb; }
+  bool GetVerbose() const { return verbose_; }
+  void SetPB_TP(unsigned pb, unsigned tp);
+  unsigned GetParticlesPerBlock() const { return pb_; }
+  unsigned GetThreadsPerParticle() const { return tp_; }
+  void SetMixedPrecisionPositions(bool on);  // FP64 position bookkeeping
+  bool GetMixedPrecisionPositions() const { return mixed_; }
+
+  void CalcF(bool calc_stresses = false);
+  double GetPotentialEnergy();
+  double GetVirial();
+  std::vector<double> GetStress();
+  unsigned GetNumberOfDOFs() const;
+  void SetNumberOfDOFs(unsigned dofs) { user_dofs_ = dofs; }
+  unsigned GetNumberOfParticles() const { return pd_.GetNumberOfParticles(); }
+  unsigned GetNumberOfVirtualParticles() const { return pd_.GetNumberOfVirtualParticles(); }
+  unsigned GetNumberOfTypes() const { return pd_.GetNumberOfTypes(); }
+  unsigned GetNumberThisType(unsigned t) const { return pd_.GetNumberThisType(t); }
+  void SetMass(unsigned type, float mass);
+  float GetMass(unsigned type) const { return pd_.GetMass(type); }
+  float GetMeanMass() const;
+  void SortParticles();
+
+  // Runs nsteps of [CalcF, Integrate] with blocks of `block` steps captured as
+  // HIP graphs; momentum is reset before every step whose index is a
+  // multiple of momentum_interval (0 = never), as Simulation.Run does.
+  void RunSteps(unsigned long first_step, unsigned nsteps, unsigned block, unsigned momentum_interval);
+  void InvalidateGraphs();
+
+  ParticleData& GetParticleData() { return pd_; }
+  hipStream_t GetStream() const { return stream_; }
+  void* Workspace() { return workspace_.get(); }
+  double* ScalarOut() { return scalars_.get(); }
+  const std::vector<Potential*>& GetPotentials() const { return potentials_; }
+  NeighborList* FusedCheckList();
+  void Synchronize();
+
+ private:
+  void PrepareRun();
+  void StepOnce(bool zero_momentum);
+  unsigned pb_ = 0, tp_ = 0;
+  unsigned user_dofs_ = 0;
+  bool verbose_ = true;
+  bool mixed_ = false;
+  SimulationBox* box_ = nullptr;
+  std::vector<Potential*> potentials_;
+  Integrator* itg_ = nullptr;
+  ParticleData pd_;
+  hipStream_t stream_ = nullptr;
+  DeviceArray<char> workspace_;
+  DeviceArray<double> scalars_;
+  // captured step blocks: key = (block size, momentum reset at first step)
+  std::map<std::pair<unsigned, bool>, hipGraphExec_t> graphs_;
+};
