@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""Benchmark: atom-timesteps/s of the RUMD hot path on one MI355X.
+
+Workload (one GPU): Kob-Andersen 80:20 binary LJ, N = 1,048,576 (FCC 64^3
+start, density 1.2, box L = 95.6033), shifted-force cut-off rc = 2.5 sigma_ab,
+Nose-Hoover NVT T = 0.45, tau = 0.2, dt = 0.0025, neighbour-list skin 0.3,
+cell-sorted (Morton) list rebuilt by the skin criterion.  Precision: FP32 pair
+forces; FP64 energy/virial/kinetic sums, thermostat state and position
+bookkeeping ("mixed").
+
+A step is CalcF + Integrate (with the device-side skin check and, when it
+fires, the sort + list rebuild), run as HIP-graph-captured blocks of 10 steps;
+momentum is reset every 100 steps as RUMD's Simulation.Run does.  W warm-up
+steps are untimed; the K timed steps are bracketed by stream synchronisation.
+
+Also reported:
+  roofline     the force kernel's algorithmic HBM bytes per launch
+               (16 + 4 + 4 nbar + 16 + 16 bytes per particle) over its average
+               duration, timed with HIP events on the sample's stream;
+  cpu_baseline the FP64 CPU MD of the oracle (same algorithm: Verlet list with
+               skin, cells, Nose-Hoover leapfrog) on the same start, on all host
+               cores, over a bounded number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def build_system(n_cells: int, seed: int):
+    from rumd_amd import configs, conf_io
+    s = configs.ka_random_lattice(4 * n_cells ** 3, density=1.2, temperature=0.45, seed=seed, lattice="fcc")
+    conf = conf_io.Configuration(n=s.n, box=s.box, types=s.types, positions=s.positions, masses=s.masses,
+                                 velocities=s.velocities, images=np.zeros_like(s.positions))
+    return s, conf
+
+
+def make_sample(conf, mixed: bool):
+    import rumd_amd.rumd as rumd
+    from rumd_amd import configs
+    sample = rumd.Sample(0, 0)
+    sample.SetVerbose(False)
+    sample.SetMixedPrecisionPositions(mixed)
+    sample.SetConfiguration(conf)
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedForce)
+    pot.SetVerbose(False)
+    for (i, j, sg, ep, rc) in configs.ka_params(2.5):
+        pot.SetParams(i, j, Sigma=sg, Epsilon=ep, Rcut=rc)
+    pot.SetNbListSkin(0.3)
+    sample.SetPotential(pot)
+    itg = rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.45, thermostatRelaxationTime=0.2)
+    sample.SetIntegrator(itg)
+    return sample, pot, itg
+
+
+def cpu_baseline(system, budget_s: float, max_steps: int):
+    """FP64 CPU MD (oracle) on the same start; steps bounded by a time budget."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    from rumd_amd import configs
+    table = configs.lj_param_table(configs.ka_params(2.5), 2)
+    threads = oracle.num_threads()
+    pos = system.positions.astype(np.float64).copy()
+    vel = system.velocities.astype(np.float64).copy()
+    inv_m = np.ones(system.n)
+    omega2 = float(np.float32(4 * np.pi ** 2 / 0.2 ** 2))
+    # one short run to size the sample (includes a list build)
+    t0 = time.perf_counter()
+    _, _, ps = oracle.md_run(pos, vel, system.types, inv_m, system.box, 2, table, 2, 0.3, 0.0025, 2, True, 0.45,
+                             omega2, 0.0)
+    t_probe = time.perf_counter() - t0
+    steps = int(max(3, min(max_steps, budget_s / max(t_probe / 2.0, 1e-6))))
+    t0 = time.perf_counter()
+    builds, _, _ = oracle.md_run(pos, vel, system.types, inv_m, system.box, 2, table, 2, 0.3, 0.0025, steps, True,
+                                 0.45, omega2, ps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": system.n * steps / dt,
+        "unit": "atom-timesteps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"KA N={system.n}, {steps} NVT steps (FP64, Verlet list skin 0.3, {builds} list builds incl. "
+                  f"the first), same start as the GPU run, {threads} OpenMP threads",
+    }
+
+
+def read_traffic(profile_dir: Path):
+    """HBM bytes per force-kernel launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)."""
+    f = profile_dir / "force_kernel_ka1m_pmc.json"
+    if not f.exists():
+        return None
+    try:
+        return float(json.loads(f.read_text())["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--cells", type=int, default=64, help="FCC cells per axis (N = 4 cells^3)")
+    ap.add_argument("--precision", choices=["mixed", "fp32"], default="mixed")
+    ap.add_argument("--block", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=3001)
+    ap.add_argument("--force-reps", type=int, default=50)
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the single large system would shard by spatial domains, see the design notes).", file=sys.stderr)
+        return 2
+
+    system, conf = build_system(args.cells, args.seed)
+    sample, pot, itg = make_sample(conf, args.precision == "mixed")
+    n = sample.GetNumberOfParticles()
+    print(f"[bench] KA N={n} L={system.box[0]:.5f} precision={args.precision}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    sample.RunSteps(0, args.warmup, args.block, 100)
+    print(f"[bench] warm-up {args.warmup} steps in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
+
+    builds0 = pot.GetNumberOfNbListBuilds()
+    sample.Synchronize()
+    t0 = time.perf_counter()
+    sample.RunSteps(args.warmup, args.steps, args.block, 100)  # synchronises at the end
+    elapsed = time.perf_counter() - t0
+    builds = pot.GetNumberOfNbListBuilds() - builds0
+    value = n * args.steps / elapsed
+    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s -> {value / 1e9:.3f} G atom-steps/s "
+          f"({builds} list builds)", file=sys.stderr, flush=True)
+
+    nbar, nmax = pot.GetNeighborListStats()
+    force_ms = pot.TimeForceKernel(args.force_reps)
+    bytes_per_atom = 16 + 4 + 4 * nbar + 16 + 16
+    achieved = bytes_per_atom * n / (force_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": read_traffic(ROOT / "profiles"),
+        "kernel": "k_lj_nbl (LJ SF force over the neighbour list)",
+        "kernel_ms": round(force_ms, 5),
+        "bytes_per_atom": round(bytes_per_atom, 2),
+    }
+    kbar = args.steps / builds if builds else None
+    step_bytes = bytes_per_atom + 64 + 48 + 16 + (4 * nbar + 260) / (kbar if kbar else float("inf"))
+    pe = sample.GetPotentialEnergy() / n
+    ke = itg.GetKineticEnergy() / n
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(system, args.cpu_budget, 400)
+
+    line = {
+        "metric": "atom-timesteps/s (KA LJ N=1M) mixed vs FP32; batched replica atom-steps/s",
+        "value": round(value, 1),
+        "unit": "atom-timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 forces, f64 sums/thermostat/positions" if args.precision == "mixed" else "f32",
+        "data": "synthetic (seeded FCC start, Maxwell velocities)",
+        "config": {
+            "workload": f"Kob-Andersen 80:20 LJ N={n} rho=1.2 shifted-force rc=2.5sigma NVT T=0.45 tau=0.2 "
+                        f"dt=0.0025 skin=0.3, Morton cell-sorted list",
+            "n": n,
+            "precision": args.precision,
+            "graph_block_steps": args.block,
+            "nbar": round(nbar, 2),
+            "nmax": int(nmax),
+            "list_builds": builds,
+            "kbar": round(kbar, 2) if kbar else None,
+            "model_bytes_per_atom_step": round(step_bytes, 1),
+            "model_hbm_fraction_step": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
+            "pe_per_particle": round(pe, 6),
+            "ke_per_particle": round(ke, 6),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

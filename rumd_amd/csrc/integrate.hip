@@ -6,11 +6,11 @@
 // calculateMomentum / zeroTotalMomentum (:176-183, 444-472), GetKineticEnergy
 // (:316-356); sumIdenticalArrays / reduce (src/rumd_algorithms.cu:162-250).
 //
-// The leapfrog update is one streaming pass.  The thermostat's kinetic-energy
-// sum is fused into it as an FP64 block reduction; the workgroup that finishes
-// last adds the block partials in a fixed order and advances Ps in FP64, so a
-// step is one launch and the sum is bitwise reproducible.  The reference
-// instead stores m v^2 per particle and reduces it in FP32 over ~4 launches.
+// The leapfrog update is one streaming pass with the thermostat's
+// kinetic-energy sum fused in as an FP64 block reduction; a one-workgroup
+// kernel then adds the block partials in a fixed order and advances Ps in FP64
+// (bitwise reproducible).  The reference instead stores m v^2 per particle and
+// reduces it in FP32 over ~4 launches.
 #include "rumdx_common.h"
 
 namespace rumdx {
@@ -90,7 +90,6 @@ struct NvtArgs {
   float4* rlo;
   rumdx_nh_state* nh;
   double* partials;
-  uint32_t* counter;
   const float4* last_r;  // skin check fused when non-null
   uint32_t* nb_flags;
   float skin2_over_4;
@@ -107,8 +106,6 @@ __global__ __launch_bounds__(kBlock) void k_nvt(NvtArgs a, Box bx) {
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t i = b * kBlock + threadIdx.x;
   float ke2 = 0.f;
-  // Every block reads Ps before it arrives at the counter, so the last block
-  // may advance it: all reads of this step's Ps have happened by then.
   const float ps_all = THERMO ? static_cast<float>(a.nh->ps) : 0.f;
   if (i < a.n) {
     float4 r = a.r[i], v = a.v[i], im = a.im[i];
@@ -128,23 +125,39 @@ __global__ __launch_bounds__(kBlock) void k_nvt(NvtArgs a, Box bx) {
   if (THERMO) {
     const double s = block_sum(static_cast<double>(ke2), s_red);
     if (threadIdx.x == 0) a.partials[b] = s;
-    if (arrive_last(a.counter, gridDim.x)) {
-      const double tot = sum_partials(a.partials, gridDim.x, 1, s_red);
-      if (threadIdx.x == 0) {
-        rumdx_nh_state* nh = a.nh;
-        double dev = tot / (static_cast<double>(a.dof) * nh->target_T) - 1.0;
-        if (dev > 10.0) dev = 10.0;  // runaway clamp (src/IntegratorNVT.cu:432-434)
-        nh->ps += static_cast<double>(a.dt) * static_cast<double>(a.omega2) * dev;
-        nh->ke2_sum = tot;
-        nh->target_T += nh->dT_per_step;
-      }
-    }
+  }
+}
+
+// UpdateThermostatState in FP64: fixed-order sum of the block partials of
+// k_nvt, then Ps += dt omega^2 min(KE2 / (dof T) - 1, 10) and the T ramp.
+// One workgroup; the kernel boundary makes the partials visible.
+__global__ __launch_bounds__(kBlock) void k_nvt_finish(const double* __restrict__ partials, uint32_t np,
+                                                       rumdx_nh_state* nh, float dt, float omega2, uint32_t dof) {
+  __shared__ double s_red[kBlock / 64];
+  const double tot = sum_partials(partials, np, 1, s_red);
+  if (threadIdx.x == 0) {
+    double dev = tot / (static_cast<double>(dof) * nh->target_T) - 1.0;
+    if (dev > 10.0) dev = 10.0;  // runaway clamp (src/IntegratorNVT.cu:432-434)
+    nh->ps += static_cast<double>(dt) * static_cast<double>(omega2) * dev;
+    nh->ke2_sum = tot;
+    nh->target_T += nh->dT_per_step;
+  }
+}
+
+// Fixed-order sums of `ncomp`-strided block partials into out[c] * scale.
+__global__ __launch_bounds__(kBlock) void k_finish_sums(const double* __restrict__ partials, uint32_t np,
+                                                        uint32_t ncomp, uint32_t mask, double scale,
+                                                        double* __restrict__ out) {
+  __shared__ double s_red[kBlock / 64];
+  for (uint32_t c = 0; c < ncomp; ++c) {
+    double s = 0.0;
+    if ((mask >> c) & 1u) s = sum_partials(partials + c, np, ncomp, s_red);
+    if (threadIdx.x == 0) out[c] = s * scale;
   }
 }
 
 // ---------------------------------------------------------------- momentum
-__global__ __launch_bounds__(kBlock) void k_momentum_sum(const float4* __restrict__ v, double* partials,
-                                                         uint32_t* counter, double* out4, uint32_t n) {
+__global__ __launch_bounds__(kBlock) void k_momentum_sum(const float4* __restrict__ v, double* partials, uint32_t n) {
   __shared__ double s_red[kBlock / 64];
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t i = b * kBlock + threadIdx.x;
@@ -160,12 +173,6 @@ __global__ __launch_bounds__(kBlock) void k_momentum_sum(const float4* __restric
   for (int c = 0; c < 4; ++c) {
     const double s = block_sum(p[c], s_red);
     if (threadIdx.x == 0) partials[static_cast<size_t>(b) * 4 + c] = s;
-  }
-  if (arrive_last(counter, gridDim.x)) {
-    for (int c = 0; c < 4; ++c) {
-      const double s = sum_partials(partials + c, gridDim.x, 4, s_red);
-      if (threadIdx.x == 0) out4[c] = s;
-    }
   }
 }
 
@@ -185,8 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_momentum_apply(float4* __restrict__ 
 // IntegratorNVT::GetKineticEnergy (src/IntegratorNVT.cu:316-356), FP64 sum.
 __global__ __launch_bounds__(kBlock) void k_kinetic(const float4* __restrict__ r, const float4* __restrict__ v,
                                                     const float4* __restrict__ f, const rumdx_nh_state* nh,
-                                                    float dt, uint32_t on_mask, double* partials, uint32_t* counter,
-                                                    double* out, uint32_t n) {
+                                                    float dt, uint32_t on_mask, double* partials, uint32_t n) {
 #pragma clang fp contract(off)
   __shared__ double s_red[kBlock / 64];
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
@@ -210,15 +216,11 @@ __global__ __launch_bounds__(kBlock) void k_kinetic(const float4* __restrict__ r
   }
   const double s = block_sum(e, s_red);
   if (threadIdx.x == 0) partials[b] = s;
-  if (arrive_last(counter, gridDim.x)) {
-    const double tot = sum_partials(partials, gridDim.x, 1, s_red);
-    if (threadIdx.x == 0) *out = 0.5 * tot;
-  }
 }
 
 // ---------------------------------------------------------------- float4 -> FP64 sums
 __global__ __launch_bounds__(kBlock) void k_reduce_f4(const float4* __restrict__ a, uint32_t mask, double* partials,
-                                                      uint32_t* counter, double* out4, uint32_t n) {
+                                                      uint32_t n) {
   __shared__ double s_red[kBlock / 64];
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t i = b * kBlock + threadIdx.x;
@@ -230,18 +232,14 @@ __global__ __launch_bounds__(kBlock) void k_reduce_f4(const float4* __restrict__
     const double s = block_sum(static_cast<double>(c4[c]), s_red);
     if (threadIdx.x == 0) partials[static_cast<size_t>(b) * 4 + c] = s;
   }
-  if (arrive_last(counter, gridDim.x)) {
-    for (int c = 0; c < 4; ++c) {
-      double s = 0.0;
-      if ((mask >> c) & 1u) s = sum_partials(partials + c, gridDim.x, 4, s_red);
-      if (threadIdx.x == 0) out4[c] = s;
-    }
-  }
 }
 
 template <bool THERMO, bool HILO, bool SKIN>
 void launch_nvt(const NvtArgs& a, const Box& bx, hipStream_t st) {
-  hipLaunchKernelGGL((k_nvt<THERMO, HILO, SKIN>), dim3(blocks_for(a.n)), dim3(kBlock), 0, st, a, bx);
+  const uint32_t nb = blocks_for(a.n);
+  hipLaunchKernelGGL((k_nvt<THERMO, HILO, SKIN>), dim3(nb), dim3(kBlock), 0, st, a, bx);
+  if (THERMO)
+    hipLaunchKernelGGL(k_nvt_finish, dim3(1), dim3(kBlock), 0, st, a.partials, nb, a.nh, a.dt, a.omega2, a.dof);
 }
 
 }  // namespace
@@ -278,7 +276,6 @@ extern "C" int rumdx_nvt_step(rumdx_particles* p, rumdx_nh_state* nh_dev, float 
   a.rlo = reinterpret_cast<float4*>(p->rlo_dev);
   a.nh = nh_dev;
   a.partials = workspace_dev ? Workspace::partials(workspace_dev) : nullptr;
-  a.counter = workspace_dev ? Workspace::counters(workspace_dev) + kCtrNvt : nullptr;
   a.dt = dt;
   a.omega2 = omega2;
   a.on_mask = thermostat_on_mask;
@@ -314,7 +311,9 @@ extern "C" int rumdx_zero_momentum(rumdx_particles* p, void* workspace_dev, void
   const uint32_t nb = blocks_for(p->n);
   double* tot = Workspace::scalars(workspace_dev);
   hipLaunchKernelGGL(k_momentum_sum, dim3(nb), dim3(kBlock), 0, st, reinterpret_cast<const float4*>(p->v_dev),
-                     Workspace::partials(workspace_dev), Workspace::counters(workspace_dev) + kCtrMomentum, tot, p->n);
+                     Workspace::partials(workspace_dev), p->n);
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, st, Workspace::partials(workspace_dev), nb, 4u, 15u, 1.0,
+                     tot);
   hipLaunchKernelGGL(k_momentum_apply, dim3(nb), dim3(kBlock), 0, st, reinterpret_cast<float4*>(p->v_dev), tot, p->n);
   return check_launch("rumdx_zero_momentum");
 }
@@ -326,11 +325,12 @@ extern "C" int rumdx_kinetic_energy(const rumdx_particles* p, const rumdx_nh_sta
   if (thermostat_on_mask != 0u && nh_dev == nullptr)
     return fail(RUMDX_E_INVALID, "rumdx_kinetic_energy: thermostat state required");
   if (p->n == 0) return fail(RUMDX_E_INVALID, "rumdx_kinetic_energy: sample has no particles");
-  hipLaunchKernelGGL(k_kinetic, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
-                     reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<const float4*>(p->v_dev),
-                     reinterpret_cast<const float4*>(p->f_dev), nh_dev, dt, thermostat_on_mask,
-                     Workspace::partials(workspace_dev), Workspace::counters(workspace_dev) + kCtrKinetic, out_dev,
-                     p->n);
+  const uint32_t nb = blocks_for(p->n);
+  hipLaunchKernelGGL(k_kinetic, dim3(nb), dim3(kBlock), 0, as_stream(stream), reinterpret_cast<const float4*>(p->r_dev),
+                     reinterpret_cast<const float4*>(p->v_dev), reinterpret_cast<const float4*>(p->f_dev), nh_dev, dt,
+                     thermostat_on_mask, Workspace::partials(workspace_dev), p->n);
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, as_stream(stream), Workspace::partials(workspace_dev), nb,
+                     1u, 1u, 0.5, out_dev);
   return check_launch("rumdx_kinetic_energy");
 }
 
@@ -339,8 +339,10 @@ extern "C" int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_
   if (arr4_dev == nullptr || out_dev == nullptr || workspace_dev == nullptr)
     return fail(RUMDX_E_INVALID, "rumdx_reduce_f4: null argument");
   if (n == 0) return fail(RUMDX_E_INVALID, "rumdx_reduce_f4: empty array");
-  hipLaunchKernelGGL(k_reduce_f4, dim3(blocks_for(n)), dim3(kBlock), 0, as_stream(stream),
-                     reinterpret_cast<const float4*>(arr4_dev), comp_mask & 15u, Workspace::partials(workspace_dev),
-                     Workspace::counters(workspace_dev) + kCtrReduce, out_dev, n);
+  const uint32_t nb = blocks_for(n);
+  hipLaunchKernelGGL(k_reduce_f4, dim3(nb), dim3(kBlock), 0, as_stream(stream), reinterpret_cast<const float4*>(arr4_dev),
+                     comp_mask & 15u, Workspace::partials(workspace_dev), n);
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, as_stream(stream), Workspace::partials(workspace_dev), nb,
+                     4u, comp_mask & 15u, 1.0, out_dev);
   return check_launch("rumdx_reduce_f4");
 }

@@ -13,8 +13,8 @@
 // in a graph).  The last workgroup of the final kernel clears the flag.
 //
 // The sorted build is a deterministic counting sort: histogram of cell ranks,
-// two-level exclusive scan, scatter, then each cell's slots are put back in
-// ascending old-slot order (what a stable key sort gives).  The gather writes
+// one-workgroup exclusive scan, scatter, then each cell's slots are put back in
+// ascending old-slot order (what a stable key sort gives) and gathered.  The gather writes
 // into scratch; the list kernel reads the sorted positions from there and copies
 // every array back to its home buffer for its own slot, so buffer addresses
 // never change (captured graphs stay valid).
@@ -27,12 +27,9 @@
 namespace rumdx {
 namespace {
 
-constexpr uint32_t kScanChunk = 1024;  // cells per scan block (256 threads x 4)
-
 struct SortScratch {
   uint32_t* key_lin;  // nvp: linear cell of each old slot
   uint32_t* perm;     // nvp: old slot of each new slot
-  uint32_t* parts;    // nparts + 1
   float4* t_r;        // nvp each
   float4* t_v;
   float4* t_f;
@@ -49,10 +46,8 @@ SortScratch carve(void* base, uint32_t nvp, uint32_t ncells) {
   SortScratch s;
   const size_t u32n = align256(sizeof(uint32_t) * nvp);
   const size_t f4n = align256(sizeof(float4) * nvp);
-  const uint32_t nparts = blocks_for(ncells, kScanChunk);
   s.key_lin = reinterpret_cast<uint32_t*>(p); p += u32n;
   s.perm = reinterpret_cast<uint32_t*>(p); p += u32n;
-  s.parts = reinterpret_cast<uint32_t*>(p); p += align256(sizeof(uint32_t) * (nparts + 1));
   s.t_r = reinterpret_cast<float4*>(p); p += f4n;
   s.t_v = reinterpret_cast<float4*>(p); p += f4n;
   s.t_f = reinterpret_cast<float4*>(p); p += f4n;
@@ -66,8 +61,8 @@ SortScratch carve(void* base, uint32_t nvp, uint32_t ncells) {
 size_t scratch_bytes(uint32_t nvp, uint32_t ncells) {
   const size_t u32n = align256(sizeof(uint32_t) * nvp);
   const size_t f4n = align256(sizeof(float4) * nvp);
-  const uint32_t nparts = blocks_for(ncells, kScanChunk);
-  return 3 * u32n + align256(sizeof(uint32_t) * (nparts + 1)) + 6 * f4n;
+  (void)ncells;
+  return 3 * u32n + 6 * f4n;
 }
 
 __device__ __forceinline__ bool rebuild_pending(const uint32_t* flags) {
@@ -77,7 +72,7 @@ __device__ __forceinline__ bool rebuild_pending(const uint32_t* flags) {
 // Called by every block of the final build kernel after its work: the block
 // that finishes last clears the rebuild request and counts the build.
 __device__ __forceinline__ void finish_build(uint32_t* flags) {
-  if (arrive_last(flags + RUMDX_FLAG_DONE, gridDim.x) && threadIdx.x == 0) {
+  if (last_to_arrive(flags + RUMDX_FLAG_DONE, gridDim.x) && threadIdx.x == 0) {
     flags[RUMDX_FLAG_REBUILD] = 0u;
     flags[RUMDX_FLAG_BUILDS] += 1u;
   }
@@ -169,86 +164,52 @@ __global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__
   atomicAdd(cell_count + cell_rank[lin], 1u);
 }
 
-// Block-wide exclusive scan of 4 values per thread (kScanChunk per block).
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* lds, uint32_t* total) {
-  // inclusive scan within the wave
+// Exclusive scan of the cell counts by one 1024-thread workgroup (the counts
+// are a few MiB at most, and a single pass saves two launches per step):
+// cell_start[c] = sum of counts before c, cell_start[ncells] = n; the counts
+// are zeroed so the scatter can use them as per-cell cursors.
+constexpr uint32_t kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_cells(uint32_t* __restrict__ cell_count,
+                                                             const uint32_t* __restrict__ flags,
+                                                             uint32_t* __restrict__ cell_start, uint32_t ncells) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ uint32_t s_wave[kScanThreads / 64];
   const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= static_cast<uint32_t>(o)) x += y;
-  }
-  if (lane == 63) lds[wave] = x;
-  __syncthreads();
-  uint32_t wave_off = 0, tot = 0;
-  for (uint32_t k = 0; k < kBlock / 64; ++k) {
-    if (k < wave) wave_off += lds[k];
-    tot += lds[k];
-  }
-  __syncthreads();
-  *total = tot;
-  return wave_off + x - v;
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_parts(const uint32_t* __restrict__ cell_count,
-                                                       const uint32_t* __restrict__ flags,
-                                                       uint32_t* __restrict__ parts, uint32_t ncells) {
-  if (!rebuild_pending(flags)) return;
-  __shared__ uint32_t lds[kBlock / 64];
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4;
-  uint32_t s = 0;
-  for (uint32_t k = 0; k < 4; ++k)
-    if (base + k < ncells) s += cell_count[base + k];
-  uint32_t tot;
-  block_excl_scan(s, lds, &tot);
-  if (threadIdx.x == 0) parts[blockIdx.x] = tot;
-}
-
-// One workgroup: exclusive scan of the per-chunk totals, in place.
-__global__ __launch_bounds__(kBlock) void k_scan_top(const uint32_t* __restrict__ flags, uint32_t* __restrict__ parts,
-                                                     uint32_t nparts) {
-  if (!rebuild_pending(flags)) return;
-  __shared__ uint32_t lds[kBlock / 64];
   uint32_t carry = 0;
-  for (uint32_t base = 0; base < nparts; base += kBlock) {
-    const uint32_t k = base + threadIdx.x;
-    const uint32_t v = k < nparts ? parts[k] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan(v, lds, &tot);
-    if (k < nparts) parts[k] = carry + ex;
+  for (uint32_t base = 0; base < ncells; base += 4 * kScanThreads) {
+    const uint32_t c0 = base + 4 * threadIdx.x;
+    uint32_t c[4];
+    uint32_t v = 0;
+    for (uint32_t k = 0; k < 4; ++k) {
+      c[k] = c0 + k < ncells ? cell_count[c0 + k] : 0u;
+      v += c[k];
+    }
+    uint32_t x = v;  // inclusive scan inside the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= static_cast<uint32_t>(o)) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    uint32_t off = carry, tot = 0;
+    for (uint32_t k = 0; k < kScanThreads / 64; ++k) {
+      if (k < wave) off += s_wave[k];
+      tot += s_wave[k];
+    }
+    off += x - v;
+    for (uint32_t k = 0; k < 4; ++k) {
+      if (c0 + k < ncells) {
+        cell_start[c0 + k] = off;
+        cell_count[c0 + k] = 0u;
+      }
+      off += c[k];
+    }
     carry += tot;
     __syncthreads();
   }
-  if (threadIdx.x == 0) parts[nparts] = carry;
-}
-
-// cell_start[c] = exclusive prefix of the counts; counts are zeroed so the
-// scatter can use them as per-cell cursors.
-__global__ __launch_bounds__(kBlock) void k_scan_final(uint32_t* __restrict__ cell_count,
-                                                       const uint32_t* __restrict__ flags,
-                                                       const uint32_t* __restrict__ parts,
-                                                       uint32_t* __restrict__ cell_start, uint32_t ncells,
-                                                       uint32_t nparts) {
-  if (!rebuild_pending(flags)) return;
-  __shared__ uint32_t lds[kBlock / 64];
-  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * 4;
-  uint32_t c[4];
-  uint32_t s = 0;
-  for (uint32_t k = 0; k < 4; ++k) {
-    c[k] = base + k < ncells ? cell_count[base + k] : 0u;
-    s += c[k];
-  }
-  uint32_t tot;
-  uint32_t off = block_excl_scan(s, lds, &tot) + parts[blockIdx.x];
-  for (uint32_t k = 0; k < 4; ++k) {
-    if (base + k < ncells) {
-      cell_start[base + k] = off;
-      cell_count[base + k] = 0u;
-    }
-    off += c[k];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) cell_start[ncells] = parts[nparts];
+  if (threadIdx.x == 0) cell_start[ncells] = carry;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ key_lin,
@@ -265,46 +226,41 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__
   if (slot < n) perm[slot] = i;  // always true when the counts started at zero
 }
 
-// Restores ascending old-slot order inside each cell (the scatter's order is
-// unspecified) and re-zeroes the counts for the next build.
-__global__ __launch_bounds__(kBlock) void k_cell_order(const uint32_t* __restrict__ flags,
-                                                       const uint32_t* __restrict__ cell_start,
-                                                       uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
-                                                       uint32_t ncells) {
+// One thread per cell: restore ascending old-slot order inside the cell (the
+// scatter's order is unspecified), re-zero its count for the next build, and
+// gather its particles' arrays into scratch (new slot <- old slot).
+__global__ __launch_bounds__(kBlock) void k_order_gather(const uint32_t* __restrict__ flags,
+                                                         const uint32_t* __restrict__ cell_start,
+                                                         uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
+                                                         const float4* __restrict__ r, const float4* __restrict__ v,
+                                                         const float4* __restrict__ f, const float4* __restrict__ w,
+                                                         const float4* __restrict__ im, const float4* __restrict__ rlo,
+                                                         const uint32_t* __restrict__ idx, SortScratch s,
+                                                         uint32_t ncells) {
   if (!rebuild_pending(flags)) return;
-  const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (c >= ncells) return;
-  const uint32_t s = cell_start[c], e = cell_start[c + 1];
+  const uint32_t b0 = cell_start[c], e0 = cell_start[c + 1];
   cell_count[c] = 0u;
-  for (uint32_t a = s + 1; a < e; ++a) {
-    const uint32_t v = perm[a];
+  for (uint32_t a = b0 + 1; a < e0; ++a) {
+    const uint32_t x = perm[a];
     uint32_t b = a;
-    while (b > s && perm[b - 1] > v) {
+    while (b > b0 && perm[b - 1] > x) {
       perm[b] = perm[b - 1];
       --b;
     }
-    perm[b] = v;
+    perm[b] = x;
   }
-}
-
-// Permutes every per-particle array into scratch (new slot s <- old slot perm[s]).
-__global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ flags,
-                                                   const uint32_t* __restrict__ perm, const float4* __restrict__ r,
-                                                   const float4* __restrict__ v, const float4* __restrict__ f,
-                                                   const float4* __restrict__ w, const float4* __restrict__ im,
-                                                   const float4* __restrict__ rlo,
-                                                   const uint32_t* __restrict__ idx, SortScratch s, uint32_t n) {
-  if (!rebuild_pending(flags)) return;
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t o = perm[i];
-  s.t_r[i] = r[o];
-  s.t_v[i] = v[o];
-  s.t_f[i] = f[o];
-  s.t_w[i] = w[o];
-  s.t_im[i] = im[o];
-  if (rlo != nullptr) s.t_rlo[i] = rlo[o];
-  s.t_idx[i] = idx[o];
+  for (uint32_t k = b0; k < e0; ++k) {
+    const uint32_t o = perm[k];
+    s.t_r[k] = r[o];
+    s.t_v[k] = v[o];
+    s.t_f[k] = f[o];
+    s.t_w[k] = w[o];
+    s.t_im[k] = im[o];
+    if (rlo != nullptr) s.t_rlo[k] = rlo[o];
+    s.t_idx[k] = idx[o];
+  }
 }
 
 // calculateNBL_CellsSorted (src/NeighborList.cu:1204-1319): scan the 5x5x5 cell
@@ -474,25 +430,20 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
   const Box bx = make_box(box);
   const hipStream_t st = as_stream(stream);
   SortScratch s = carve(nb->sort_scratch_dev, p->nvp, g.ncells);
-  const uint32_t nparts = blocks_for(g.ncells, kScanChunk);
   const uint32_t nblk = blocks_for(p->n);
   const float4* r = reinterpret_cast<const float4*>(p->r_dev);
 
   hipLaunchKernelGGL(k_cell_hist, dim3(nblk), dim3(kBlock), 0, st, r, nb->flags_dev, nb->cell_rank_dev, s.key_lin,
                      nb->cell_count_dev, bx, g, p->n);
-  hipLaunchKernelGGL(k_scan_parts, dim3(nparts), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.parts,
-                     g.ncells);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, st, nb->flags_dev, s.parts, nparts);
-  hipLaunchKernelGGL(k_scan_final, dim3(nparts), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.parts,
-                     nb->cell_start_dev, g.ncells, nparts);
+  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(kScanThreads), 0, st, nb->cell_count_dev, nb->flags_dev,
+                     nb->cell_start_dev, g.ncells);
   hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBlock), 0, st, s.key_lin, nb->flags_dev, nb->cell_rank_dev,
                      nb->cell_start_dev, nb->cell_count_dev, s.perm, p->n);
-  hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(g.ncells)), dim3(kBlock), 0, st, nb->flags_dev,
-                     nb->cell_start_dev, nb->cell_count_dev, s.perm, g.ncells);
-  hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlock), 0, st, nb->flags_dev, s.perm, r,
-                     reinterpret_cast<const float4*>(p->v_dev), reinterpret_cast<const float4*>(p->f_dev),
-                     reinterpret_cast<const float4*>(p->w_dev), reinterpret_cast<const float4*>(p->im_dev),
-                     reinterpret_cast<const float4*>(p->rlo_dev), p->unsorted_index_dev, s, p->n);
+  hipLaunchKernelGGL(k_order_gather, dim3(blocks_for(g.ncells)), dim3(kBlock), 0, st, nb->flags_dev, nb->cell_start_dev,
+                     nb->cell_count_dev, s.perm, r, reinterpret_cast<const float4*>(p->v_dev),
+                     reinterpret_cast<const float4*>(p->f_dev), reinterpret_cast<const float4*>(p->w_dev),
+                     reinterpret_cast<const float4*>(p->im_dev), reinterpret_cast<const float4*>(p->rlo_dev),
+                     p->unsorted_index_dev, s, g.ncells);
   hipLaunchKernelGGL(k_build_cells, dim3(nblk), dim3(kBlock), sizeof(float) * nb->ntypes * nb->ntypes, st,
                      nb->flags_dev, s, nb->cell_rank_dev, nb->cell_start_dev, nb->cut_skin2_dev, nb->ntypes, bx, g,
                      reinterpret_cast<float4*>(p->r_dev), reinterpret_cast<float4*>(p->v_dev),

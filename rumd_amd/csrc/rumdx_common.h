@@ -105,21 +105,21 @@ __device__ __forceinline__ double block_sum(double v, double* lds4) {
   return s;
 }
 
-// Last-block-done handshake.  Each block publishes its partial before calling;
-// returns true in every thread of the block that arrives last, after an
-// acquire fence so that all partials are visible to it.  The counter is left
-// at zero for the next launch.
-__device__ __forceinline__ bool arrive_last(uint32_t* counter, uint32_t nblocks) {
+// Returns true (in every thread) for the block that arrives last at `counter`,
+// which is left at zero for the next launch.  No memory fence is issued: this
+// orders nothing but the arrivals themselves, which is enough for a block to
+// clear a flag every other block has already read.  (An agent-scope fence on
+// gfx950 writes back the whole L2 of the XCD, so data handed between blocks
+// goes through a separate finishing kernel instead.)
+__device__ __forceinline__ bool last_to_arrive(uint32_t* counter, uint32_t nblocks) {
   __shared__ uint32_t s_last;
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = (t == nblocks - 1) ? 1u : 0u;
     if (s_last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
-  if (s_last) __threadfence();
   return s_last != 0u;
 }
 
@@ -141,8 +141,6 @@ struct Workspace {
   static double* scalars(void* ws) { return reinterpret_cast<double*>(static_cast<char*>(ws) + kScalars); }
   static double* partials(void* ws) { return reinterpret_cast<double*>(static_cast<char*>(ws) + kPartials); }
 };
-// Counter slots, one per kernel family.
-enum : uint32_t { kCtrNvt = 0, kCtrMomentum = 1, kCtrKinetic = 2, kCtrReduce = 3 };
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 

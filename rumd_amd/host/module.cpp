@@ -109,7 +109,10 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetParameterList", &PairPotential::GetPotentialParameters, py::arg("num_types"))
       .def("GetPotentialEnergy", &PairPotential::GetPotentialEnergy)
       .def("GetActualMaxNumNbrs", &PairPotential::GetActualMaxNumNbrs)
-      .def("GetCutoffMethod", &PairPotential::GetCutoffMethod);
+      .def("GetCutoffMethod", &PairPotential::GetCutoffMethod)
+      .def("TimeForceKernel", &PairPotential::TimeForceKernel, py::arg("reps"),
+           py::call_guard<py::gil_scoped_release>())
+      .def("GetNeighborListStats", &PairPotential::GetNeighborListStats);
 
   py::class_<Pot_IPL_12_6, PairPotential>(m, "Pot_IPL_12_6")
       .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))

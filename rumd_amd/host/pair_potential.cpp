@@ -205,6 +205,10 @@ bool NeighborList::HandleOverflow() {
   return true;
 }
 
+void NeighborList::DownloadCounts(std::vector<uint32_t>& out) {
+  if (allocated_) num_nbrs_.download(out.data(), std::min(out.size(), num_nbrs_.size()), sample_->GetStream());
+}
+
 unsigned NeighborList::GetActualMaxNumNbrs() {
   if (!allocated_) return 0;
   uint32_t f[8];
@@ -319,6 +323,42 @@ double PairPotential::GetPotentialEnergy() {
             "GetPotentialEnergy");
   hip_check(hipStreamSynchronize(st), "PairPotential", "GetPotentialEnergy");
   return out[3];
+}
+
+float PairPotential::TimeForceKernel(unsigned reps) {
+  if (!sample) throw RUMD_Error(GetID_String(), "TimeForceKernel", "no sample");
+  if (reps == 0) reps = 1;
+  neighborList.UpdateNBlist();
+  rumdx_nblist* nb = neighborList.GetResolvedMethod() == NeighborList::NONE ? nullptr : neighborList.Descriptor();
+  const hipStream_t st = sample->GetStream();
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "PairPotential", "TimeForceKernel");
+  hip_check(hipEventCreate(&b), "PairPotential", "TimeForceKernel");
+  hip_check(hipEventRecord(a, st), "PairPotential", "TimeForceKernel");
+  for (unsigned k = 0; k < reps; ++k)
+    rumdx_check(rumdx_pair_lj(sample->GetParticleData().Descriptor(), nb, d_params_.get(), table_types_,
+                              sample->GetSimulationBox()->GetBox8(), cutoff_method_, 1, 0, st),
+                GetID_String().c_str(), "TimeForceKernel");
+  hip_check(hipEventRecord(b, st), "PairPotential", "TimeForceKernel");
+  hip_check(hipEventSynchronize(b), "PairPotential", "TimeForceKernel");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, a, b), "PairPotential", "TimeForceKernel");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+std::vector<double> PairPotential::GetNeighborListStats() {
+  if (!sample) throw RUMD_Error(GetID_String(), "GetNeighborListStats", "no sample");
+  const unsigned n = sample->GetNumberOfParticles();
+  std::vector<uint32_t> cnt(sample->GetNumberOfVirtualParticles());
+  neighborList.DownloadCounts(cnt);
+  double sum = 0.0, mx = 0.0;
+  for (unsigned i = 0; i < n; ++i) {
+    sum += cnt[i];
+    mx = std::max(mx, static_cast<double>(cnt[i]));
+  }
+  return {sum / n, mx};
 }
 
 // ================================================================== Pot_IPL_12_6 / Pot_LJ_12_6

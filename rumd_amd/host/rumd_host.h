@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -174,6 +175,7 @@ class NeighborList {
   // Reads the flags; grows the list after an overflow.  Synchronises.
   bool HandleOverflow();
   bool IsSorted() const { return GetResolvedMethod() == SORT; }
+  void DownloadCounts(std::vector<uint32_t>& out);
   rumdx_nblist* Descriptor() { return &desc_; }
 
  private:
@@ -223,6 +225,12 @@ class PairPotential : public Potential {
   unsigned GetNumberOfNbListBuilds() { return neighborList.GetNumberOfBuilds(); }
   int GetCutoffMethod() const { return cutoff_method_; }
   void SetAssumeNewton3(bool b) { assume_newton3_ = b; }
+  // Average duration (ms) of the force kernel alone over `reps` launches on
+  // the sample's stream, bracketed by HIP events (the list is brought up to
+  // date first and not rebuilt in between).
+  float TimeForceKernel(unsigned reps);
+  // Mean and maximum current list length (synchronises).
+  std::vector<double> GetNeighborListStats();
   NeighborList& GetNeighborList() { return neighborList; }
   // Shift constants u(rc), s(rc) rc of one parameter row.
   virtual void ComputeShifts(std::vector<float>& row) const = 0;
