@@ -107,8 +107,49 @@ def read_traffic(profile_dir: Path):
         return None
 
 
+def counter_calibration(n: int, reps: int) -> dict:
+    """Kernels of known HBM traffic for scaling the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE counters to bytes: k_reduce_f4 reads 16 n bytes, and
+    k_momentum_apply reads and writes 16 n bytes (n float4 far larger than the
+    caches).  Run under rocprofv3 --pmc together with the force kernel."""
+    import ctypes as C
+    from rumd_amd import _native
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    lib = _native.librumdx()
+
+    class Particles(C.Structure):
+        _fields_ = [(f, C.c_void_p) for f in ("r", "v", "f", "w", "im", "sts", "rlo", "idx")] + \
+                   [("n", C.c_uint32), ("nvp", C.c_uint32)]
+
+    lib.rumdx_workspace_bytes.restype = C.c_size_t
+    lib.rumdx_workspace_bytes.argtypes = [C.c_uint32]
+    lib.rumdx_reduce_f4.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.rumdx_zero_momentum.argtypes = [C.POINTER(Particles), C.c_void_p, C.c_void_p]
+    arr, ws, out = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    nbytes = 16 * n
+    for ptr, size in ((arr, nbytes), (ws, lib.rumdx_workspace_bytes(n)), (out, 64)):
+        if hip.hipMalloc(C.byref(ptr), size):
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(arr, 0, nbytes)
+    p = Particles()
+    p.v, p.n, p.nvp = arr.value, n, n
+    for _ in range(reps):
+        if lib.rumdx_reduce_f4(arr, n, 15, out, ws, None) or lib.rumdx_zero_momentum(C.byref(p), ws, None):
+            raise RuntimeError("calibration kernel failed")
+    hip.hipDeviceSynchronize()
+    for ptr in (arr, ws, out):
+        hip.hipFree(ptr)
+    return {"n": n, "k_reduce_f4_read_bytes": nbytes, "k_momentum_apply_read_bytes": nbytes,
+            "k_momentum_apply_write_bytes": nbytes}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-counters", type=int, default=0, metavar="N",
+                    help="first run kernels of known traffic over N float4 (for rocprofv3 --pmc scaling)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=1000)
@@ -125,6 +166,9 @@ def main() -> int:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(the single large system would shard by spatial domains, see the design notes).", file=sys.stderr)
         return 2
+
+    if args.calibrate_counters > 0:
+        print(json.dumps({"calibration": counter_calibration(args.calibrate_counters, 3)}), file=sys.stderr, flush=True)
 
     system, conf = build_system(args.cells, args.seed)
     sample, pot, itg = make_sample(conf, args.precision == "mixed")

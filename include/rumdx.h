@@ -53,6 +53,8 @@ extern "C" {
 #define RUMDX_FLAG_BUILDS 2    /* number of completed builds */
 #define RUMDX_FLAG_MAXLEN 3    /* longest list seen since the flag was cleared */
 #define RUMDX_FLAG_DONE 4      /* internal completion counter, zero between calls */
+#define RUMDX_FLAG_MAXHALO 5   /* largest tile halo (particles) of the last sorted build */
+#define RUMDX_FLAG_ERROR 6     /* nonzero: a tile halo exceeded 65535 particles (list unusable) */
 
 /* Device-resident particle arrays (ParticleData, include/rumd/ParticleData.h:41-62). */
 typedef struct rumdx_particles {
@@ -68,12 +70,27 @@ typedef struct rumdx_particles {
   uint32_t nvp;     /* allocated (padded) particles, >= n */
 } rumdx_particles;
 
-/* Neighbour-list state (NeighborList, include/rumd/NeighborList.h).  The list is
- * column-major like RUMD's: entry k of slot i is nbl_dev[k*nvp + i]
- * (src/PairPotential.cu:360-367).  num_nbrs_dev[i] == max_nbrs marks a list that
- * overflowed; the force kernel then rescans the cells for that particle. */
+/* Neighbour-list state (NeighborList, include/rumd/NeighborList.h).  Two
+ * entry formats:
+ *   N2 method:     nbl_dev, u32 slot of the neighbour, column-major like
+ *                  RUMD's: entry k of slot i at [k*nvp + i]
+ *                  (src/PairPotential.cu:360-367);
+ *   sorted method: nbl16_dev, u16 position of the neighbour inside the halo
+ *                  of the particle's cell tile (see rumdx_tile_table): the
+ *                  LDS byte offset 16*h of halo particle h when the halo holds
+ *                  at most 4096 particles, else h itself; in groups of 8
+ *                  consecutive entries: entry k of slot i at
+ *                  [((k/8)*nvp + i)*8 + k%8], so one 16-byte load serves 8
+ *                  pairs and a wave reads 1 KiB contiguous per group.  The
+ *                  halo layout is rebuilt from the cell arrays, and
+ *                  rumdx_nblist_export() converts to slots.  lane_order_dev
+ *                  holds, per tile, the tile's particles (local index) by
+ *                  descending list length, so the force kernel gives each
+ *                  wave lists of similar length.
+ * num_nbrs_dev[i] == max_nbrs marks a list that overflowed; the force kernel
+ * then rescans the cell stencil for that particle. */
 typedef struct rumdx_nblist {
-  uint32_t* nbl_dev;        /* max_nbrs x nvp */
+  uint32_t* nbl_dev;        /* N2 method: max_nbrs x nvp */
   uint32_t* num_nbrs_dev;   /* nvp */
   float* last_r_dev;        /* nvp x 4: positions at the last build */
   uint32_t* flags_dev;      /* 8 x uint32, see RUMDX_FLAG_* */
@@ -88,7 +105,17 @@ typedef struct rumdx_nblist {
   uint32_t* cell_count_dev; /* ncells, by rank: zero between builds */
   uint32_t* slot_cell_dev;  /* nvp: linear cell index of each slot at the last build */
   void* sort_scratch_dev;   /* rumdx_sort_scratch_bytes(nvp, ncells) bytes */
+  uint16_t* nbl16_dev;      /* sorted method: (max_nbrs rounded up to 8) x nvp halo offsets */
+  uint32_t ntiles;          /* from rumdx_tile_table() */
+  const uint32_t* tile_first_rank_dev; /* ntiles + 1 */
+  const uint32_t* tile_origin_dev;     /* ntiles: cx0 | cy0 << 10 | cz0 << 20 */
+  uint32_t halo_cap;        /* halo particles a tile stages in LDS (<= RUMDX_HALO_CAP_MAX) */
+  uint16_t* lane_order_dev; /* sorted method: nvp, written by the build */
 } rumdx_nblist;
+
+/* Cells per tile edge and the largest LDS halo a launch may request. */
+#define RUMDX_TILE_EDGE 4
+#define RUMDX_HALO_CAP_MAX 3072
 
 /* Device-resident Nose-Hoover state (IntegratorNVT, src/IntegratorNVT.cu:82-92, 426-437). */
 typedef struct rumdx_nh_state {
@@ -113,6 +140,17 @@ int rumdx_cell_grid(const float box[8], float max_cutoff, float skin, int32_t nc
 /* Storage rank of every linear cell (x fastest, src/NeighborList.cu:1136-1140):
  * cells are stored in Morton (Z-order) order of their coordinates.  Host output. */
 int rumdx_cell_rank_table(const int32_t nc[3], uint32_t* rank_out);
+/* Tiles: aligned RUMDX_TILE_EDGE^3 bricks of cells in Morton order, i.e.
+ * contiguous rank (and slot) ranges.  Returns the tile count; with non-null
+ * outputs also fills first_rank_out[ntiles+1] and origin_out[ntiles] (brick
+ * origin cell, packed cx | cy << 10 | cz << 20).  rank is the table of
+ * rumdx_cell_rank_table.  Host only; returns a negative code on error. */
+int rumdx_tile_table(const int32_t nc[3], const uint32_t* rank, uint32_t* first_rank_out, uint32_t* origin_out);
+/* Halo capacity (particles) to request for a grid of nc cells holding n
+ * particles: the mean 8^3-cell halo with margin, rounded up to 64, at most
+ * RUMDX_HALO_CAP_MAX.  Tiles whose halo is larger still compute exactly,
+ * reading positions from global memory. */
+uint32_t rumdx_halo_cap_estimate(const int32_t nc[3], uint32_t n);
 
 /* ---------------------------------------------------------------- neighbour list
  * NeighborList::CheckRebuildRequired / displacement_since_nb_build
@@ -137,6 +175,11 @@ int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb, const floa
  * the cells and by original slot inside a cell (a stable sort).  Does nothing
  * on the device unless flags[REBUILD] is set. */
 int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
+
+/* Writes the current list in RUMD's format -- u32 neighbour slots, column
+ * major, out_dev[k*nvp + i] for k < num_nbrs[i] -- whatever the internal
+ * format (for tools and checks that read lists; not on the step path). */
+int rumdx_nblist_export(const rumdx_particles* p, const rumdx_nblist* nb, uint32_t* out_dev, void* stream);
 
 /* ---------------------------------------------------------------- pair forces
  * Pot_LJ_12_6 / Pot_IPL_12_6 force, energy and virial (PairPotential::CalcF,

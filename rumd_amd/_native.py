@@ -18,4 +18,11 @@ try:
 except ImportError as e:  # pragma: no cover - reported to the user
     raise ImportError(f"the rumd_amd host extension is not built ({e}); run __graft_entry__.build()") from e
 
-__all__ = ["_rumd", "LIBRUMDX"]
+
+
+def librumdx() -> ctypes.CDLL:
+    """The loaded C-ABI kernel library (include/rumdx.h)."""
+    return _librumdx
+
+
+__all__ = ["_rumd", "LIBRUMDX", "librumdx"]

@@ -4,24 +4,27 @@
 // fij (include/rumd/PairPotential.h:1198-1244), Pot_IPL_12_6::ComputeInteraction
 // (include/rumd/PairPotential.h:161-168).
 //
-// Design: one lane per particle, streaming the column-major neighbour list
-// (a wave reads 64 consecutive u32 per list row), neighbour positions gathered
-// through L2 (particles are cell-sorted in Morton order and blocks are mapped
-// XCD-contiguous).  Pair arithmetic is FP32; the minimum-image displacement is
-// formed in FP64 so pairs that cross the periodic boundary of a large box keep
-// full FP32 accuracy.  The per-particle sums stay in registers (one lane owns a
-// particle, no cross-lane traffic).
+// Design.  Lists of the sorted method are served by the tile kernel: one
+// workgroup per cell tile (a brick of 4^3 cells, a contiguous slot range)
+// stages the positions of the tile's halo in LDS, and list entries are LDS
+// byte offsets in 16-byte groups of 8, so a pair costs one LDS read and no
+// address arithmetic.  The pair arithmetic is VALU-bound, so the table holds
+// pre-multiplied constants and lanes are given lists of similar length.
+// Lists of the N2 method (u32 slots, column-major like RUMD's) are served by
+// a lane-per-particle kernel gathering positions through L2.  Pair arithmetic
+// is FP32; where a pair may cross the periodic boundary the minimum-image
+// displacement is formed in FP64, so large boxes keep full FP32 accuracy.
+// Per-particle sums stay in registers; no atomics, so results are
+// reproducible bit for bit.
 #include "rumdx_common.h"
+#include "tiles.h"
 
 namespace rumdx {
 namespace {
 
-// Per type pair, two float4 in LDS: {rc^2, A12, A6, s(rc) rc} and {u(rc), rc, -, -}.
-struct PairTable {
-  const float4* p;
-  uint32_t nt;
-};
-
+// Per type pair, two float4 in LDS: {rc^2, A12, A6, s(rc) rc} and
+// {12 A12, 6 A6, -u(rc), -(u(rc) + s(rc) rc^2)}: the last two are the energy
+// shifts of the shifted-potential and shifted-force cut-offs.
 struct Acc {
   float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f, vir = 0.f;
   float sxx = 0.f, syy = 0.f, szz = 0.f, syz = 0.f, sxz = 0.f, sxy = 0.f;
@@ -38,42 +41,41 @@ struct CellFallback {
 __device__ __forceinline__ void load_table(float4* s_tab, const float* __restrict__ params, uint32_t nt) {
   for (uint32_t k = threadIdx.x; k < nt * nt; k += blockDim.x) {
     const float* q = params + static_cast<size_t>(k) * RUMDX_NUM_PARAM;
-    const float rc = q[0];
-    s_tab[2 * k] = make_float4(rc * rc, q[1], q[2], q[RUMDX_NUM_PARAM - 2]);
-    s_tab[2 * k + 1] = make_float4(q[RUMDX_NUM_PARAM - 1], rc, 0.f, 0.f);
+    const float rc = q[0], a12 = q[1], a6 = q[2];
+    const float src = q[RUMDX_NUM_PARAM - 2], ucut = q[RUMDX_NUM_PARAM - 1];
+    s_tab[2 * k] = make_float4(rc * rc, a12, a6, src);
+    s_tab[2 * k + 1] = make_float4(12.f * a12, 6.f * a6, -ucut, -fmaf(src, rc, ucut));
   }
 }
 
-// One pair term (fij with ComputeInteraction inlined).  The cut-off test and
-// the shift follow the reference; evaluation is branch-free and masked.
+// The LJ term of one pair given its displacement (fij with ComputeInteraction
+// inlined): u = A12 r^-12 + A6 r^-6, s = -u'(r)/r, shifted as the cut-off
+// method asks (SP: u - u(rc); SF: s - s(rc) rc/r and u - u(rc) + s(rc) rc (r - rc)).
+// Evaluation is branch-free; pairs at or beyond rc are masked.
 template <int CUT, bool STR>
-__device__ __forceinline__ void pair_term(double xi, double yi, double zi, float4 rj, const float4* prow,
-                                          const Box& bx, Acc& a) {
-  double x = xi - static_cast<double>(rj.x);
-  double y = yi - static_cast<double>(rj.y);
-  double z = zi - static_cast<double>(rj.z);
-  x -= bx.Ld[0] * rint(x * bx.iLd[0]);
-  y -= bx.Ld[1] * rint(y * bx.iLd[1]);
-  z -= bx.Ld[2] * rint(z * bx.iLd[2]);
-  const float dx = static_cast<float>(x), dy = static_cast<float>(y), dz = static_cast<float>(z);
-  const float d2 = static_cast<float>(x * x + y * y + z * z);
-
-  const uint32_t tj = __float_as_uint(rj.w);
+__device__ __forceinline__ void pair_core(float dx, float dy, float dz, float d2, uint32_t tj, const float4* prow,
+                                          Acc& a) {
   const float4 pa = prow[2 * tj];
   const float4 pb = prow[2 * tj + 1];
-
-  const float inv2 = __builtin_amdgcn_rcpf(d2);
+  // SF needs 1/r anyway: one rsq gives both 1/r and 1/r^2
+  float rinv = 0.f, inv2;
+  if (CUT == RUMDX_CUTOFF_SF) {
+    rinv = __builtin_amdgcn_rsqf(d2);
+    inv2 = rinv * rinv;
+  } else {
+    inv2 = __builtin_amdgcn_rcpf(d2);
+  }
   const float inv6 = inv2 * inv2 * inv2;
-  const float t = pa.y * inv6;
-  float s = inv6 * inv2 * (12.f * t + 6.f * pa.z);
-  float u = inv6 * (t + pa.z);
+  float s = (inv6 * inv2) * fmaf(pb.x, inv6, pb.y);
+  float u;
   if (CUT == RUMDX_CUTOFF_SP) {
-    u -= pb.x;
+    u = fmaf(inv6, fmaf(pa.y, inv6, pa.z), pb.z);
   } else if (CUT == RUMDX_CUTOFF_SF) {
-    const float rinv = __builtin_amdgcn_rsqf(d2);
-    const float rr = d2 * rinv;
-    s -= pa.w * rinv;
-    u += pa.w * (rr - pb.y) - pb.x;
+    u = fmaf(inv6, fmaf(pa.y, inv6, pa.z), pb.w);
+    s = fmaf(-pa.w, rinv, s);
+    u = fmaf(pa.w, d2 * rinv, u);
+  } else {
+    u = inv6 * fmaf(pa.y, inv6, pa.z);
   }
   const bool in = d2 < pa.x;
   s = in ? s : 0.f;
@@ -91,6 +93,29 @@ __device__ __forceinline__ void pair_term(double xi, double yi, double zi, float
     a.sxz -= dx * dz * s;
     a.sxy -= dx * dy * s;
   }
+}
+
+// Pair term with the minimum-image displacement formed in FP64 and rounded
+// to FP32 (any pair, including those across the periodic boundary).
+template <int CUT, bool STR>
+__device__ __forceinline__ void pair_term(double xi, double yi, double zi, float4 rj, const float4* prow,
+                                          const Box& bx, Acc& a) {
+  double x = xi - static_cast<double>(rj.x);
+  double y = yi - static_cast<double>(rj.y);
+  double z = zi - static_cast<double>(rj.z);
+  x -= bx.Ld[0] * rint(x * bx.iLd[0]);
+  y -= bx.Ld[1] * rint(y * bx.iLd[1]);
+  z -= bx.Ld[2] * rint(z * bx.iLd[2]);
+  pair_core<CUT, STR>(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z),
+                      static_cast<float>(x * x + y * y + z * z), __float_as_uint(rj.w), prow, a);
+}
+
+// Pair term for a pair that cannot wrap: the FP32 difference of two floats is
+// correctly rounded, i.e. the same number the FP64 path produces.
+template <int CUT, bool STR>
+__device__ __forceinline__ void pair_term_direct(float4 ri, float4 rj, const float4* prow, Acc& a) {
+  const float dx = ri.x - rj.x, dy = ri.y - rj.y, dz = ri.z - rj.z;
+  pair_core<CUT, STR>(dx, dy, dz, dx * dx + dy * dy + dz * dz, __float_as_uint(rj.w), prow, a);
 }
 
 template <bool STR, bool INIT>
@@ -227,6 +252,136 @@ __global__ __launch_bounds__(kBlock) void k_lj_all(const float4* __restrict__ r,
   if (live) store_result<STR, INIT>(i, a, f, w, sts);
 }
 
+// Sums a particle's partial accumulators over the 2^lg lanes that share it
+// (a fixed butterfly, so the result does not depend on timing).  Called by
+// every lane of the wave.
+template <bool STR>
+__device__ __forceinline__ void combine_lanes(Acc& a, uint32_t lg) {
+  for (uint32_t o = 1; o < (1u << lg); o <<= 1) {
+    a.fx += __shfl_xor(a.fx, o, 64);
+    a.fy += __shfl_xor(a.fy, o, 64);
+    a.fz += __shfl_xor(a.fz, o, 64);
+    a.u += __shfl_xor(a.u, o, 64);
+    a.vir += __shfl_xor(a.vir, o, 64);
+    if (STR) {
+      a.sxx += __shfl_xor(a.sxx, o, 64);
+      a.syy += __shfl_xor(a.syy, o, 64);
+      a.szz += __shfl_xor(a.szz, o, 64);
+      a.syz += __shfl_xor(a.syz, o, 64);
+      a.sxz += __shfl_xor(a.sxz, o, 64);
+      a.sxy += __shfl_xor(a.sxy, o, 64);
+    }
+  }
+}
+
+// Neighbour position of a byte entry: the halo positions start the dynamic LDS.
+__device__ __forceinline__ float4 halo_pos(const float4* s_pos, uint32_t entry) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pos) + entry);
+}
+
+// Tile force kernel for lists of the sorted method: one workgroup per cell
+// tile stages the tile's halo positions in LDS; list entries are LDS byte
+// offsets, so neighbour positions come from LDS.  Waves take the tile's
+// particles 64 at a time in the build's lane order (descending list length,
+// so a wave's lists have similar lengths); the last, partial chunk spreads
+// each list over 2^lg lanes (groups part, part + 2^lg, ...), combined by a
+// butterfly.  A tile whose halo does not wrap uses the direct FP32
+// difference; a wrapping tile forms displacements in FP64.  Overflowed lists,
+// and tiles whose halo exceeds the LDS capacity, rescan the particle's
+// build-time cell stencil (positions from LDS or global memory).
+template <int CUT, bool STR, bool INIT>
+__global__ __launch_bounds__(kBlock) void k_lj_tiles(const float4* __restrict__ r, float4* __restrict__ f,
+                                                     float4* __restrict__ w, float4* __restrict__ sts,
+                                                     const uint16_t* __restrict__ nbl16,
+                                                     const uint32_t* __restrict__ num_nbrs,
+                                                     const uint32_t* __restrict__ slot_cell,
+                                                     const uint16_t* __restrict__ lane_order,
+                                                     const float* __restrict__ params, uint32_t nt, Box bx, TileGeom tg,
+                                                     uint32_t nvp, uint32_t max_nbrs) {
+  __shared__ HaloLds L;
+  extern __shared__ float4 s_dyn[];
+  float4* s_pos = s_dyn;
+  float4* s_tab = s_dyn + tg.halo_cap;
+  load_table(s_tab, params, nt);
+  const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+  stage_halo(t, tg, r, s_pos, L);
+  const Halo& h = L.h;
+  const bool staged = h.npart <= tg.halo_cap;
+  const uint32_t np = h.slot1 - h.slot0;
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t c0 = (threadIdx.x >> 6) * 64u; c0 < np; c0 += kBlock) {
+    const uint32_t cnt = (np - c0) < 64u ? (np - c0) : 64u;
+    const uint32_t lg = cnt > 32u ? 0u : cnt > 16u ? 1u : cnt > 8u ? 2u : 3u;
+    const uint32_t part = lane & ((1u << lg) - 1u);
+    const uint32_t pl = lane >> lg;
+    const bool live = pl < cnt;
+    Acc a;
+    uint32_t i = 0;
+    if (live) {
+      i = h.slot0 + lane_order[h.slot0 + c0 + pl];
+      const float4 ri = r[i];
+      const float4* prow = s_tab + 2 * __float_as_uint(ri.w) * nt;
+      const uint32_t nn = num_nbrs[i];
+      if (staged && nn < max_nbrs) {
+        const uint32_t nfull = nn >> 3, rem = nn & 7u;
+        const uint32_t step = 1u << lg;
+        const size_t gstride = static_cast<size_t>(nvp) << lg;
+        const u32x4* grp = reinterpret_cast<const u32x4*>(nbl16) + static_cast<size_t>(part) * nvp + i;
+        const bool tail = rem != 0u && (nfull & (step - 1u)) == part;  // this lane owns group nfull
+        if (!h.wraps) {
+          for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
+            const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
+          }
+          if (tail) {
+            const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+            for (int e = 0; e < 7; ++e)
+              if (static_cast<uint32_t>(e) < rem)
+                pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
+          }
+        } else {
+          const double xi = ri.x, yi = ri.y, zi = ri.z;
+          for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
+            const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
+          }
+          if (tail) {
+            const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+            for (int e = 0; e < 7; ++e)
+              if (static_cast<uint32_t>(e) < rem)
+                pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
+          }
+        }
+      } else if (part == 0) {
+        const double xi = ri.x, yi = ri.y, zi = ri.z;
+        const uint32_t lin = slot_cell[i];
+        const int cx = static_cast<int>(lin % tg.nc0);
+        const int cy = static_cast<int>((lin / tg.nc0) % tg.nc1);
+        const int cz = static_cast<int>(lin / (static_cast<uint32_t>(tg.nc0) * tg.nc1));
+        const uint32_t hc = halo_cell(h, tg, cx, cy, cz);
+        const uint32_t my_off = L.off[hc] + (i - L.start[hc]);
+        for (int dz = -2; dz <= 2; ++dz)
+          for (int dy = -2; dy <= 2; ++dy)
+            for (int dx = -2; dx <= 2; ++dx) {
+              const uint32_t hcell = halo_cell(h, tg, cx + dx, cy + dy, cz + dz);
+              const uint32_t o0 = L.off[hcell], o1 = L.off[hcell + 1], g0 = L.start[hcell];
+              for (uint32_t oo = o0; oo < o1; ++oo) {
+                if (oo == my_off) continue;
+                pair_term<CUT, STR>(xi, yi, zi, staged ? s_pos[oo] : r[g0 + (oo - o0)], prow, bx, a);
+              }
+            }
+      }
+    }
+    combine_lanes<STR>(a, lg);
+    if (live && part == 0) store_result<STR, INIT>(i, a, f, w, sts);
+  }
+}
+
 template <int CUT, bool STR, bool INIT>
 int launch_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* params, uint32_t nt, const Box& bx,
               hipStream_t st) {
@@ -236,7 +391,13 @@ int launch_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* par
   float4* w = reinterpret_cast<float4*>(p->w_dev);
   float4* sts = reinterpret_cast<float4*>(p->sts_dev);
   const size_t tab_bytes = sizeof(float4) * 2 * nt * nt;
-  if (nb != nullptr) {
+  if (nb != nullptr && nb->nbl16_dev != nullptr) {
+    const TileGeom tg{nb->nc[0], nb->nc[1], nb->nc[2], nb->cell_rank_dev, nb->cell_start_dev,
+                      nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, nb->halo_cap};
+    hipLaunchKernelGGL((k_lj_tiles<CUT, STR, INIT>), dim3(nb->ntiles), dim3(kBlock),
+                       tab_bytes + sizeof(float4) * nb->halo_cap, st, r, f, w, sts, nb->nbl16_dev, nb->num_nbrs_dev,
+                       nb->slot_cell_dev, nb->lane_order_dev, params, nt, bx, tg, p->nvp, nb->max_nbrs);
+  } else if (nb != nullptr) {
     CellFallback fb{};
     if (nb->cell_rank_dev != nullptr) {
       fb.cell_rank = nb->cell_rank_dev;
@@ -277,8 +438,13 @@ extern "C" int rumdx_pair_lj(const rumdx_particles* p, const rumdx_nblist* nb, c
   if (calc_stresses && p->sts_dev == nullptr) return fail(RUMDX_E_INVALID, "rumdx_pair_lj: stresses requested but sts is null");
   if (ntypes == 0 || ntypes > RUMDX_MAX_TYPES) return fail(RUMDX_E_INVALID, "rumdx_pair_lj: ntypes out of range");
   if (p->n == 0) return RUMDX_OK;
-  if (nb != nullptr && (nb->nbl_dev == nullptr || nb->num_nbrs_dev == nullptr || nb->max_nbrs == 0))
+  if (nb != nullptr && ((nb->nbl_dev == nullptr && nb->nbl16_dev == nullptr) || nb->num_nbrs_dev == nullptr ||
+                        nb->max_nbrs == 0))
     return fail(RUMDX_E_INVALID, "rumdx_pair_lj: neighbour list not allocated");
+  if (nb != nullptr && nb->nbl16_dev != nullptr &&
+      (nb->ntiles == 0 || nb->tile_first_rank_dev == nullptr || nb->tile_origin_dev == nullptr ||
+       nb->slot_cell_dev == nullptr || nb->lane_order_dev == nullptr || nb->halo_cap > RUMDX_HALO_CAP_MAX))
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj: tile arrays not allocated");
   const Box bx = make_box(box);
   const hipStream_t st = as_stream(stream);
   const bool init = initialize != 0, str = calc_stresses != 0;

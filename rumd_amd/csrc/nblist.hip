@@ -13,7 +13,7 @@
 // in a graph).  The last workgroup of the final kernel clears the flag.
 //
 // The sorted build is a deterministic counting sort: histogram of cell ranks,
-// one-workgroup exclusive scan, scatter, then each cell's slots are put back in
+// two-pass exclusive scan, scatter, then each cell's slots are put back in
 // ascending old-slot order (what a stable key sort gives) and gathered.  The gather writes
 // into scratch; the list kernel reads the sorted positions from there and copies
 // every array back to its home buffer for its own slot, so buffer addresses
@@ -23,23 +23,31 @@
 #include <vector>
 
 #include "rumdx_common.h"
+#include "tiles.h"
 
 namespace rumdx {
 namespace {
 
 struct SortScratch {
-  uint32_t* key_lin;  // nvp: linear cell of each old slot
-  uint32_t* perm;     // nvp: old slot of each new slot
-  float4* t_r;        // nvp each
+  uint32_t* key_lin;    // nvp: linear cell of each old slot
+  uint32_t* perm;       // nvp: old slot of each new slot
+  float4* t_r;          // nvp each
   float4* t_v;
   float4* t_f;
   float4* t_w;
   float4* t_im;
   float4* t_rlo;
-  uint32_t* t_idx;    // nvp
+  uint32_t* t_idx;      // nvp
+  uint32_t* chunk_sum;  // one per kScanChunk cells
 };
 
+// Cells per workgroup of the two-pass count scan.
+constexpr uint32_t kScanPerThread = 16;
+constexpr uint32_t kScanChunk = kBlock * kScanPerThread;
+
 size_t align256(size_t x) { return (x + 255) & ~static_cast<size_t>(255); }
+
+uint32_t scan_chunks(uint32_t ncells) { return (ncells + kScanChunk - 1) / kScanChunk; }
 
 SortScratch carve(void* base, uint32_t nvp, uint32_t ncells) {
   char* p = static_cast<char*>(base);
@@ -54,15 +62,16 @@ SortScratch carve(void* base, uint32_t nvp, uint32_t ncells) {
   s.t_w = reinterpret_cast<float4*>(p); p += f4n;
   s.t_im = reinterpret_cast<float4*>(p); p += f4n;
   s.t_rlo = reinterpret_cast<float4*>(p); p += f4n;
-  s.t_idx = reinterpret_cast<uint32_t*>(p);
+  s.t_idx = reinterpret_cast<uint32_t*>(p); p += u32n;
+  s.chunk_sum = reinterpret_cast<uint32_t*>(p);
+  (void)ncells;
   return s;
 }
 
 size_t scratch_bytes(uint32_t nvp, uint32_t ncells) {
   const size_t u32n = align256(sizeof(uint32_t) * nvp);
   const size_t f4n = align256(sizeof(float4) * nvp);
-  (void)ncells;
-  return 3 * u32n + 6 * f4n;
+  return 3 * u32n + 6 * f4n + align256(sizeof(uint32_t) * scan_chunks(ncells));
 }
 
 __device__ __forceinline__ bool rebuild_pending(const uint32_t* flags) {
@@ -164,52 +173,107 @@ __global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__
   atomicAdd(cell_count + cell_rank[lin], 1u);
 }
 
-// Exclusive scan of the cell counts by one 1024-thread workgroup (the counts
-// are a few MiB at most, and a single pass saves two launches per step):
-// cell_start[c] = sum of counts before c, cell_start[ncells] = n; the counts
-// are zeroed so the scatter can use them as per-cell cursors.
-constexpr uint32_t kScanThreads = 1024;
-
-__global__ __launch_bounds__(kScanThreads) void k_scan_cells(uint32_t* __restrict__ cell_count,
-                                                             const uint32_t* __restrict__ flags,
-                                                             uint32_t* __restrict__ cell_start, uint32_t ncells) {
-  if (!rebuild_pending(flags)) return;
-  __shared__ uint32_t s_wave[kScanThreads / 64];
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < ncells; base += 4 * kScanThreads) {
-    const uint32_t c0 = base + 4 * threadIdx.x;
-    uint32_t c[4];
-    uint32_t v = 0;
-    for (uint32_t k = 0; k < 4; ++k) {
-      c[k] = c0 + k < ncells ? cell_count[c0 + k] : 0u;
-      v += c[k];
-    }
-    uint32_t x = v;  // inclusive scan inside the wave
+// Total of v over a kBlock-thread workgroup, returned in every thread.
+__device__ __forceinline__ uint32_t block_total_u32(uint32_t v, uint32_t* lds4) {
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o, 64);
-      if (lane >= static_cast<uint32_t>(o)) x += y;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63u) == 0) lds4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  for (uint32_t k = 0; k < kBlock / 64; ++k) t += lds4[k];
+  __syncthreads();
+  return t;
+}
+
+// Exclusive scan of the cell counts in two passes over kScanChunk-cell
+// chunks: chunk totals, then each chunk adds the totals before it and scans
+// itself.  cell_start[c] = sum of counts before c, cell_start[ncells] = n; the
+// counts are zeroed so the scatter can use them as per-cell cursors.
+__global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ cell_count,
+                                                        const uint32_t* __restrict__ flags,
+                                                        uint32_t* __restrict__ chunk_sum, uint32_t ncells) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ uint32_t s_w[kBlock / 64];
+  const uint32_t base = blockIdx.x * kScanChunk;
+  uint32_t v = 0;
+  for (uint32_t k = threadIdx.x; k < kScanChunk; k += kBlock) {
+    const uint32_t c = base + k;
+    v += c < ncells ? cell_count[c] : 0u;
+  }
+  v = block_total_u32(v, s_w);
+  if (threadIdx.x == 0) chunk_sum[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ cell_count,
+                                                       const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ chunk_sum,
+                                                       uint32_t* __restrict__ cell_start, uint32_t ncells) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ uint32_t s_w[kBlock / 64];
+  uint32_t carry = 0;
+  for (uint32_t k = threadIdx.x; k < blockIdx.x; k += kBlock) carry += chunk_sum[k];
+  carry = block_total_u32(carry, s_w);
+
+  const uint32_t c0 = blockIdx.x * kScanChunk + threadIdx.x * kScanPerThread;
+  uint32_t c[kScanPerThread];
+  const bool full = c0 + kScanPerThread <= ncells;
+  if (full) {
+    const uint4* src = reinterpret_cast<const uint4*>(cell_count + c0);
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPerThread / 4; ++q) {
+      const uint4 x = src[q];
+      c[4 * q] = x.x;
+      c[4 * q + 1] = x.y;
+      c[4 * q + 2] = x.z;
+      c[4 * q + 3] = x.w;
     }
-    if (lane == 63) s_wave[wave] = x;
-    __syncthreads();
-    uint32_t off = carry, tot = 0;
-    for (uint32_t k = 0; k < kScanThreads / 64; ++k) {
-      if (k < wave) off += s_wave[k];
-      tot += s_wave[k];
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPerThread; ++k) c[k] = c0 + k < ncells ? cell_count[c0 + k] : 0u;
+  }
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPerThread; ++k) v += c[k];
+  // exclusive scan of the per-thread sums: inside each wave, then wave totals
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= static_cast<uint32_t>(o)) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint32_t off = carry, tot = 0;
+  for (uint32_t k = 0; k < kBlock / 64; ++k) {
+    off += (k < wave) ? s_w[k] : 0u;
+    tot += s_w[k];
+  }
+  off += x - v;
+  uint32_t st[kScanPerThread];
+#pragma unroll
+  for (uint32_t k = 0; k < kScanPerThread; ++k) {
+    st[k] = off;
+    off += c[k];
+  }
+  if (full) {
+    uint4* ds = reinterpret_cast<uint4*>(cell_start + c0);
+    uint4* dc = reinterpret_cast<uint4*>(cell_count + c0);
+#pragma unroll
+    for (uint32_t q = 0; q < kScanPerThread / 4; ++q) {
+      ds[q] = make_uint4(st[4 * q], st[4 * q + 1], st[4 * q + 2], st[4 * q + 3]);
+      dc[q] = make_uint4(0u, 0u, 0u, 0u);
     }
-    off += x - v;
-    for (uint32_t k = 0; k < 4; ++k) {
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPerThread; ++k) {
       if (c0 + k < ncells) {
-        cell_start[c0 + k] = off;
+        cell_start[c0 + k] = st[k];
         cell_count[c0 + k] = 0u;
       }
-      off += c[k];
     }
-    carry += tot;
-    __syncthreads();
   }
-  if (threadIdx.x == 0) cell_start[ncells] = carry;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) cell_start[ncells] = carry + tot;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ key_lin,
@@ -227,16 +291,11 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__
 }
 
 // One thread per cell: restore ascending old-slot order inside the cell (the
-// scatter's order is unspecified), re-zero its count for the next build, and
-// gather its particles' arrays into scratch (new slot <- old slot).
-__global__ __launch_bounds__(kBlock) void k_order_gather(const uint32_t* __restrict__ flags,
-                                                         const uint32_t* __restrict__ cell_start,
-                                                         uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
-                                                         const float4* __restrict__ r, const float4* __restrict__ v,
-                                                         const float4* __restrict__ f, const float4* __restrict__ w,
-                                                         const float4* __restrict__ im, const float4* __restrict__ rlo,
-                                                         const uint32_t* __restrict__ idx, SortScratch s,
-                                                         uint32_t ncells) {
+// scatter's order is unspecified) and re-zero its count for the next build.
+__global__ __launch_bounds__(kBlock) void k_cell_order(const uint32_t* __restrict__ flags,
+                                                       const uint32_t* __restrict__ cell_start,
+                                                       uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
+                                                       uint32_t ncells) {
   if (!rebuild_pending(flags)) return;
   const uint32_t c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (c >= ncells) return;
@@ -251,82 +310,180 @@ __global__ __launch_bounds__(kBlock) void k_order_gather(const uint32_t* __restr
     }
     perm[b] = x;
   }
-  for (uint32_t k = b0; k < e0; ++k) {
-    const uint32_t o = perm[k];
-    s.t_r[k] = r[o];
-    s.t_v[k] = v[o];
-    s.t_f[k] = f[o];
-    s.t_w[k] = w[o];
-    s.t_im[k] = im[o];
-    if (rlo != nullptr) s.t_rlo[k] = rlo[o];
-    s.t_idx[k] = idx[o];
-  }
 }
 
-// calculateNBL_CellsSorted (src/NeighborList.cu:1204-1319): scan the 5x5x5 cell
-// stencil in RUMD's (dZ, dY, dX) order, ascending slot inside each cell; accept
-// j != i with d^2 < (rc_ij + skin)^2.  Also copies this slot's sorted data back
-// to the home arrays and records last_r and the slot's cell.
-__global__ __launch_bounds__(kBlock) void k_build_cells(
-    uint32_t* __restrict__ flags, SortScratch s, const uint32_t* __restrict__ cell_rank,
-    const uint32_t* __restrict__ cell_start, const float* __restrict__ cut_skin2, uint32_t nt, Box bx, CellGeom g,
-    float4* __restrict__ r, float4* __restrict__ v, float4* __restrict__ f, float4* __restrict__ w,
-    float4* __restrict__ im, float4* __restrict__ rlo, uint32_t* __restrict__ idx, float4* __restrict__ last_r,
-    uint32_t* __restrict__ slot_cell, uint32_t* __restrict__ nbl, uint32_t* __restrict__ num_nbrs, uint32_t n,
-    uint32_t nvp, uint32_t max_nbrs) {
+// One thread per new slot: gather the particle arrays into scratch.
+__global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ flags,
+                                                   const uint32_t* __restrict__ perm, const float4* __restrict__ r,
+                                                   const float4* __restrict__ v, const float4* __restrict__ f,
+                                                   const float4* __restrict__ w, const float4* __restrict__ im,
+                                                   const float4* __restrict__ rlo, const uint32_t* __restrict__ idx,
+                                                   SortScratch s, uint32_t n) {
   if (!rebuild_pending(flags)) return;
-  extern __shared__ float s_cut[];
+  const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t o = perm[k];
+  s.t_r[k] = r[o];
+  s.t_v[k] = v[o];
+  s.t_f[k] = f[o];
+  s.t_w[k] = w[o];
+  s.t_im[k] = im[o];
+  if (rlo != nullptr) s.t_rlo[k] = rlo[o];
+  s.t_idx[k] = idx[o];
+}
+
+struct HomeArrays {
+  float4* r;
+  float4* v;
+  float4* f;
+  float4* w;
+  float4* im;
+  float4* rlo;
+  uint32_t* idx;
+  float4* last_r;
+  uint32_t* slot_cell;
+  uint16_t* lane_order;
+};
+
+// calculateNBL_CellsSorted (src/NeighborList.cu:1204-1319), one workgroup per
+// cell tile with the tile's halo staged in LDS: for each particle, scan the
+// 5x5x5 cell stencil in RUMD's (dZ, dY, dX) order, ascending slot inside each
+// cell, and accept j != i with d^2 < (rc_ij + skin)^2 (FP32, the reference's
+// minimum-image arithmetic; a tile whose halo does not wrap needs no image
+// correction, and the plain difference is then the same number).  Entries are
+// halo offsets.  Each particle's sorted data is copied back to its home slot.
+__global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ flags, SortScratch s, TileGeom tg,
+                                                        const float* __restrict__ cut_skin2, uint32_t nt, Box bx,
+                                                        HomeArrays o, uint16_t* __restrict__ nbl16,
+                                                        uint32_t* __restrict__ num_nbrs, uint32_t nvp,
+                                                        uint32_t max_nbrs) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ HaloLds L;
+  __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
+  __shared__ uint32_t s_key[kOrderMax];
+  extern __shared__ float4 s_pos[];
   for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_skin2[k];
-  __syncthreads();
-  const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i < n) {
+  const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+  stage_halo(t, tg, s.t_r, s_pos, L);
+  const Halo& h = L.h;
+  const bool staged = h.npart <= tg.halo_cap;
+  const bool bytes = h.npart <= kByteEntryHalo;
+  const uint32_t np = h.slot1 - h.slot0;
+  for (uint32_t i = h.slot0 + threadIdx.x; i < h.slot1; i += kBlock) {
     const float4 ri = s.t_r[i];
     const float* crow = s_cut + __float_as_uint(ri.w) * nt;
-    const int cx = cell_coord(ri.x, bx.iL[0], g.nc0);
-    const int cy = cell_coord(ri.y, bx.iL[1], g.nc1);
-    const int cz = cell_coord(ri.z, bx.iL[2], g.nc2);
+    const int cx = cell_coord(ri.x, bx.iL[0], tg.nc0);
+    const int cy = cell_coord(ri.y, bx.iL[1], tg.nc1);
+    const int cz = cell_coord(ri.z, bx.iL[2], tg.nc2);
+    const uint32_t hc = halo_cell(h, tg, cx, cy, cz);
+    const uint32_t my_off = L.off[hc] + (i - L.start[hc]);
+    u32x4* grp = reinterpret_cast<u32x4*>(nbl16);
+    u32x4 w = {0u, 0u, 0u, 0u};
     uint32_t cnt = 0;
+    auto consider = [&](uint32_t oo, float4 rj) {
+      const float d2 = h.wraps ? min_image_d2_f32(ri, rj, bx) : plain_d2_f32(ri, rj);
+      if (d2 < crow[__float_as_uint(rj.w)]) {
+        if (cnt < max_nbrs) group_append(w, cnt, encode_entry(oo, bytes), grp, nvp, i);
+        ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
+      }
+    };
+    // The five cells (cx-2..cx+2, y, z) are consecutive halo cells, hence one
+    // contiguous run of LDS offsets in ascending dX order, unless the halo
+    // covers the whole x axis and the stencil wraps inside it.
+    const int hx = halo_axis(cx, h.base[0], tg.nc0);
+    const bool rows = staged && hx >= 2 && hx + 2 < h.ext[0];
     for (int dz = -2; dz <= 2; ++dz) {
-      const int oz = (cz + dz + g.nc2) % g.nc2;
+      const int hz = halo_axis(cz + dz, h.base[2], tg.nc2);
       for (int dy = -2; dy <= 2; ++dy) {
-        const int oy = (cy + dy + g.nc1) % g.nc1;
-        const int rowbase = oy * g.nc0 + oz * g.nc0 * g.nc1;
-        for (int dx = -2; dx <= 2; ++dx) {
-          const int ox = (cx + dx + g.nc0) % g.nc0;
-          const uint32_t rank = cell_rank[ox + rowbase];
-          const uint32_t js = cell_start[rank], je = cell_start[rank + 1];
-          for (uint32_t j = js; j < je; ++j) {
-            if (j == i) continue;
-            const float4 rj = s.t_r[j];
-            const float d2 = min_image_d2_f32(ri, rj, bx);
-            if (d2 < crow[__float_as_uint(rj.w)]) {
-              if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * nvp + i] = j;
-              ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
-            }
+        const int hy = halo_axis(cy + dy, h.base[1], tg.nc1);
+        if (rows) {
+          const uint32_t hc0 = static_cast<uint32_t>(hx - 2 + h.ext[0] * (hy + h.ext[1] * hz));
+          const uint32_t o1 = L.off[hc0 + 5];
+          for (uint32_t oo = L.off[hc0]; oo < o1; ++oo)
+            if (oo != my_off) consider(oo, s_pos[oo]);
+        } else {
+          for (int dx = -2; dx <= 2; ++dx) {
+            const uint32_t hcell = static_cast<uint32_t>(halo_axis(cx + dx, h.base[0], tg.nc0) +
+                                                         h.ext[0] * (hy + h.ext[1] * hz));
+            const uint32_t o0 = L.off[hcell], o1 = L.off[hcell + 1], g0 = L.start[hcell];
+            for (uint32_t oo = o0; oo < o1; ++oo)
+              if (oo != my_off) consider(oo, staged ? s_pos[oo] : s.t_r[g0 + (oo - o0)]);
           }
         }
       }
     }
-    num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;  // == max_nbrs marks an overflowed list
+    const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
+    if (stored & 7u) grp[static_cast<size_t>(stored >> 3) * nvp + i] = w;  // partial last group
+    num_nbrs[i] = stored;  // == max_nbrs marks an overflowed list
+    if (np <= kOrderMax) s_key[i - h.slot0] = ((0xffffu - min(stored, 0xffffu)) << 16) | (i - h.slot0);
     if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
     atomicMax(flags + RUMDX_FLAG_MAXLEN, cnt);
-    slot_cell[i] = static_cast<uint32_t>(cx + cy * g.nc0 + cz * g.nc0 * g.nc1);
-    r[i] = ri;
-    last_r[i] = ri;
-    v[i] = s.t_v[i];
-    f[i] = s.t_f[i];
-    w[i] = s.t_w[i];
-    im[i] = s.t_im[i];
-    if (rlo != nullptr) rlo[i] = s.t_rlo[i];
-    idx[i] = s.t_idx[i];
+    o.slot_cell[i] = static_cast<uint32_t>(cx + cy * tg.nc0 + cz * tg.nc0 * tg.nc1);
+    o.r[i] = ri;
+    o.last_r[i] = ri;
+    o.v[i] = s.t_v[i];
+    o.f[i] = s.t_f[i];
+    o.w[i] = s.t_w[i];
+    o.im[i] = s.t_im[i];
+    if (o.rlo != nullptr) o.rlo[i] = s.t_rlo[i];
+    o.idx[i] = s.t_idx[i];
+  }
+  // Lane order of the force kernel: the tile's particles by descending list
+  // length, ties by slot (a rank count over unique keys; deterministic).
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < np; k += kBlock) {
+    uint32_t rk = k;
+    if (np <= kOrderMax) {
+      const uint32_t key = s_key[k];
+      rk = 0;
+      for (uint32_t j = 0; j < np; ++j) rk += s_key[j] < key ? 1u : 0u;
+    }
+    o.lane_order[h.slot0 + rk] = static_cast<uint16_t>(k);
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(flags + RUMDX_FLAG_MAXHALO, h.npart);
+    if (h.npart > 65535u) flags[RUMDX_FLAG_ERROR] = 1u;
   }
   finish_build(flags);
 }
 
+// Converts tile-format entries (halo offsets) back to slots.
+__global__ __launch_bounds__(kBlock) void k_export_tiles(TileGeom tg, const uint16_t* __restrict__ nbl16,
+                                                         const uint32_t* __restrict__ num_nbrs,
+                                                         uint32_t* __restrict__ out, uint32_t nvp) {
+  __shared__ HaloLds L;
+  TileGeom g = tg;
+  g.halo_cap = 0;  // table only, no positions
+  stage_halo(blockIdx.x, g, nullptr, nullptr, L);
+  const Halo& h = L.h;
+  const bool bytes = h.npart <= kByteEntryHalo;
+  for (uint32_t i = h.slot0 + threadIdx.x; i < h.slot1; i += kBlock) {
+    const uint32_t nn = num_nbrs[i];
+    for (uint32_t k = 0; k < nn; ++k) {
+      const uint32_t oo = decode_entry(nbl16[(static_cast<size_t>(k >> 3) * nvp + i) * 8 + (k & 7u)], bytes);
+      uint32_t lo = 0, hi = h.ncell;  // last cell with off <= oo
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (L.off[mid] <= oo) lo = mid; else hi = mid;
+      }
+      out[static_cast<size_t>(k) * nvp + i] = L.start[lo] + (oo - L.off[lo]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_export_slots(const uint32_t* __restrict__ nbl,
+                                                         const uint32_t* __restrict__ num_nbrs,
+                                                         uint32_t* __restrict__ out, uint32_t n, uint32_t nvp) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t nn = num_nbrs[i];
+  for (uint32_t k = 0; k < nn; ++k) out[static_cast<size_t>(k) * nvp + i] = nbl[static_cast<size_t>(k) * nvp + i];
+}
+
 int check_nb(const rumdx_particles* p, const rumdx_nblist* nb, const char* who) {
   if (p == nullptr || nb == nullptr) return fail(RUMDX_E_INVALID, std::string(who) + ": null argument");
-  if (nb->flags_dev == nullptr || nb->nbl_dev == nullptr || nb->num_nbrs_dev == nullptr ||
-      nb->last_r_dev == nullptr || nb->cut_skin2_dev == nullptr)
+  if (nb->flags_dev == nullptr || nb->num_nbrs_dev == nullptr || nb->last_r_dev == nullptr ||
+      nb->cut_skin2_dev == nullptr)
     return fail(RUMDX_E_INVALID, std::string(who) + ": neighbour list not allocated");
   if (nb->max_nbrs == 0) return fail(RUMDX_E_INVALID, std::string(who) + ": max_nbrs is zero");
   if (static_cast<uint64_t>(nb->max_nbrs) * p->nvp >= (1ull << 32))
@@ -403,6 +560,7 @@ extern "C" int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb,
   int rc = check_nb(p, nb, "rumdx_nblist_build_n2");
   if (rc != RUMDX_OK) return rc;
   if (box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_n2: null box");
+  if (nb->nbl_dev == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_n2: slot list not allocated");
   if (p->n == 0) return RUMDX_OK;
   const size_t lds = sizeof(float4) * kBlock + sizeof(float) * nb->ntypes * nb->ntypes;
   hipLaunchKernelGGL(k_build_n2, dim3(blocks_for(p->n)), dim3(kBlock), lds, as_stream(stream),
@@ -417,8 +575,11 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
   if (rc != RUMDX_OK) return rc;
   if (box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: null box");
   if (nb->cell_rank_dev == nullptr || nb->cell_start_dev == nullptr || nb->cell_count_dev == nullptr ||
-      nb->slot_cell_dev == nullptr || nb->sort_scratch_dev == nullptr)
-    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: cell arrays not allocated");
+      nb->slot_cell_dev == nullptr || nb->sort_scratch_dev == nullptr || nb->nbl16_dev == nullptr ||
+      nb->lane_order_dev == nullptr ||
+      nb->tile_first_rank_dev == nullptr || nb->tile_origin_dev == nullptr || nb->ntiles == 0)
+    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: cell or tile arrays not allocated");
+  if (nb->halo_cap > RUMDX_HALO_CAP_MAX) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: halo_cap too large");
   if (p->v_dev == nullptr || p->im_dev == nullptr || p->unsorted_index_dev == nullptr || p->f_dev == nullptr ||
       p->w_dev == nullptr)
     return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: particle arrays not allocated");
@@ -435,21 +596,91 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
 
   hipLaunchKernelGGL(k_cell_hist, dim3(nblk), dim3(kBlock), 0, st, r, nb->flags_dev, nb->cell_rank_dev, s.key_lin,
                      nb->cell_count_dev, bx, g, p->n);
-  hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(kScanThreads), 0, st, nb->cell_count_dev, nb->flags_dev,
+  const uint32_t nchunk = scan_chunks(g.ncells);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nchunk), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.chunk_sum,
+                     g.ncells);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nchunk), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.chunk_sum,
                      nb->cell_start_dev, g.ncells);
   hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBlock), 0, st, s.key_lin, nb->flags_dev, nb->cell_rank_dev,
                      nb->cell_start_dev, nb->cell_count_dev, s.perm, p->n);
-  hipLaunchKernelGGL(k_order_gather, dim3(blocks_for(g.ncells)), dim3(kBlock), 0, st, nb->flags_dev, nb->cell_start_dev,
-                     nb->cell_count_dev, s.perm, r, reinterpret_cast<const float4*>(p->v_dev),
-                     reinterpret_cast<const float4*>(p->f_dev), reinterpret_cast<const float4*>(p->w_dev),
-                     reinterpret_cast<const float4*>(p->im_dev), reinterpret_cast<const float4*>(p->rlo_dev),
-                     p->unsorted_index_dev, s, g.ncells);
-  hipLaunchKernelGGL(k_build_cells, dim3(nblk), dim3(kBlock), sizeof(float) * nb->ntypes * nb->ntypes, st,
-                     nb->flags_dev, s, nb->cell_rank_dev, nb->cell_start_dev, nb->cut_skin2_dev, nb->ntypes, bx, g,
-                     reinterpret_cast<float4*>(p->r_dev), reinterpret_cast<float4*>(p->v_dev),
-                     reinterpret_cast<float4*>(p->f_dev), reinterpret_cast<float4*>(p->w_dev),
-                     reinterpret_cast<float4*>(p->im_dev), reinterpret_cast<float4*>(p->rlo_dev),
-                     p->unsorted_index_dev, reinterpret_cast<float4*>(nb->last_r_dev), nb->slot_cell_dev, nb->nbl_dev,
-                     nb->num_nbrs_dev, p->n, p->nvp, nb->max_nbrs);
+  hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(g.ncells)), dim3(kBlock), 0, st, nb->flags_dev, nb->cell_start_dev,
+                     nb->cell_count_dev, s.perm, g.ncells);
+  hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlock), 0, st, nb->flags_dev, s.perm, r,
+                     reinterpret_cast<const float4*>(p->v_dev), reinterpret_cast<const float4*>(p->f_dev),
+                     reinterpret_cast<const float4*>(p->w_dev), reinterpret_cast<const float4*>(p->im_dev),
+                     reinterpret_cast<const float4*>(p->rlo_dev), p->unsorted_index_dev, s, p->n);
+  const TileGeom tg{nb->nc[0], nb->nc[1], nb->nc[2], nb->cell_rank_dev, nb->cell_start_dev,
+                    nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, nb->halo_cap};
+  const HomeArrays home{reinterpret_cast<float4*>(p->r_dev), reinterpret_cast<float4*>(p->v_dev),
+                        reinterpret_cast<float4*>(p->f_dev), reinterpret_cast<float4*>(p->w_dev),
+                        reinterpret_cast<float4*>(p->im_dev), reinterpret_cast<float4*>(p->rlo_dev),
+                        p->unsorted_index_dev, reinterpret_cast<float4*>(nb->last_r_dev), nb->slot_cell_dev,
+                        nb->lane_order_dev};
+  hipLaunchKernelGGL(k_build_tiles, dim3(nb->ntiles), dim3(kBlock), sizeof(float4) * nb->halo_cap, st, nb->flags_dev,
+                     s, tg, nb->cut_skin2_dev, nb->ntypes, bx, home, nb->nbl16_dev, nb->num_nbrs_dev, p->nvp,
+                     nb->max_nbrs);
   return check_launch("rumdx_nblist_build_sorted");
+}
+
+extern "C" int rumdx_tile_table(const int32_t nc[3], const uint32_t* rank, uint32_t* first_rank_out,
+                                uint32_t* origin_out) {
+  if (nc == nullptr) return fail(RUMDX_E_INVALID, "rumdx_tile_table: null argument");
+  int nb[3];
+  for (int a = 0; a < 3; ++a) {
+    if (nc[a] < 1 || nc[a] > 1023) return fail(RUMDX_E_INVALID, "rumdx_tile_table: cells per axis out of range");
+    nb[a] = (nc[a] + kTileEdge - 1) / kTileEdge;
+  }
+  const uint64_t ntiles = static_cast<uint64_t>(nb[0]) * nb[1] * nb[2];
+  if (first_rank_out == nullptr && origin_out == nullptr) return static_cast<int>(ntiles);
+  if (rank == nullptr || first_rank_out == nullptr || origin_out == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_tile_table: null output");
+  // The origin cell of an aligned brick has the smallest Morton code (hence
+  // rank) of the brick, and bricks occupy disjoint code ranges.
+  std::vector<std::pair<uint32_t, uint32_t>> t;
+  t.reserve(ntiles);
+  for (int z = 0; z < nb[2]; ++z)
+    for (int y = 0; y < nb[1]; ++y)
+      for (int x = 0; x < nb[0]; ++x) {
+        const uint32_t cx = x * kTileEdge, cy = y * kTileEdge, cz = z * kTileEdge;
+        const uint32_t lin = cx + cy * nc[0] + cz * static_cast<uint32_t>(nc[0]) * nc[1];
+        t.emplace_back(rank[lin], cx | (cy << 10) | (cz << 20));
+      }
+  std::sort(t.begin(), t.end());
+  for (uint64_t k = 0; k < ntiles; ++k) {
+    first_rank_out[k] = t[k].first;
+    origin_out[k] = t[k].second;
+  }
+  first_rank_out[ntiles] = static_cast<uint32_t>(static_cast<uint64_t>(nc[0]) * nc[1] * nc[2]);
+  return static_cast<int>(ntiles);
+}
+
+extern "C" uint32_t rumdx_halo_cap_estimate(const int32_t nc[3], uint32_t n) {
+  if (nc == nullptr) return 0;
+  const double ncells = static_cast<double>(nc[0]) * nc[1] * nc[2];
+  const double halo = static_cast<double>(kHaloCellsMax) * n / ncells;
+  uint32_t cap = static_cast<uint32_t>(1.25 * halo + 64.0);
+  cap = (cap + 63u) & ~63u;
+  return cap > RUMDX_HALO_CAP_MAX ? RUMDX_HALO_CAP_MAX : cap;
+}
+
+extern "C" int rumdx_nblist_export(const rumdx_particles* p, const rumdx_nblist* nb, uint32_t* out_dev,
+                                   void* stream) {
+  if (p == nullptr || nb == nullptr || out_dev == nullptr || nb->num_nbrs_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_nblist_export: null argument");
+  if (p->n == 0) return RUMDX_OK;
+  const hipStream_t st = as_stream(stream);
+  if (nb->nbl16_dev != nullptr) {
+    if (nb->tile_first_rank_dev == nullptr || nb->tile_origin_dev == nullptr || nb->cell_rank_dev == nullptr ||
+        nb->cell_start_dev == nullptr)
+      return fail(RUMDX_E_INVALID, "rumdx_nblist_export: tile arrays not allocated");
+    const TileGeom tg{nb->nc[0], nb->nc[1], nb->nc[2], nb->cell_rank_dev, nb->cell_start_dev,
+                      nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, 0u};
+    hipLaunchKernelGGL(k_export_tiles, dim3(nb->ntiles), dim3(kBlock), 0, st, tg, nb->nbl16_dev, nb->num_nbrs_dev,
+                       out_dev, p->nvp);
+  } else {
+    if (nb->nbl_dev == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_export: list not allocated");
+    hipLaunchKernelGGL(k_export_slots, dim3(blocks_for(p->n)), dim3(kBlock), 0, st, nb->nbl_dev, nb->num_nbrs_dev,
+                       out_dev, p->n, p->nvp);
+  }
+  return check_launch("rumdx_nblist_export");
 }

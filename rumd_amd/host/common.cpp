@@ -56,6 +56,7 @@ void DeviceArray<T>::download(T* h, size_t n, hipStream_t st) const {
 template class DeviceArray<float>;
 template class DeviceArray<double>;
 template class DeviceArray<uint32_t>;
+template class DeviceArray<uint16_t>;
 template class DeviceArray<char>;
 template class DeviceArray<rumdx_nh_state>;
 

@@ -112,7 +112,31 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetCutoffMethod", &PairPotential::GetCutoffMethod)
       .def("TimeForceKernel", &PairPotential::TimeForceKernel, py::arg("reps"),
            py::call_guard<py::gil_scoped_release>())
-      .def("GetNeighborListStats", &PairPotential::GetNeighborListStats);
+      .def("GetNeighborListStats", &PairPotential::GetNeighborListStats)
+      .def("SetNbHaloCapacity", [](PairPotential& p, unsigned cap) { p.GetNeighborList().SetHaloCapacity(cap); },
+           py::arg("cap"))
+      .def("GetNbHaloCapacity", [](PairPotential& p) { return p.GetNeighborList().GetHaloCapacity(); })
+      .def("GetNeighborListCSR", [](PairPotential& p, Sample& s) {
+        // neighbour lists in original particle ids, CSR (offsets[n+1], ids)
+        s.Synchronize();
+        std::vector<uint32_t> cnt;
+        const std::vector<uint32_t> lst = p.GetNeighborList().ExportList(cnt);
+        const unsigned n = s.GetNumberOfParticles(), nvp = s.GetNumberOfVirtualParticles();
+        const std::vector<uint32_t> idx = s.GetParticleData().UnsortedIndex(s.GetStream());
+        std::vector<uint32_t> per(n, 0u);
+        for (unsigned slot = 0; slot < n; ++slot) per[idx[slot]] = cnt[slot];
+        py::array_t<int64_t> off(n + 1);
+        auto o = off.mutable_unchecked<1>();
+        o(0) = 0;
+        for (unsigned i = 0; i < n; ++i) o(i + 1) = o(i) + per[i];
+        py::array_t<int32_t> ids(static_cast<py::ssize_t>(o(n)));
+        auto d = ids.mutable_unchecked<1>();
+        for (unsigned slot = 0; slot < n; ++slot) {
+          const uint32_t id = idx[slot];
+          for (uint32_t k = 0; k < cnt[slot]; ++k) d(o(id) + k) = static_cast<int32_t>(idx[lst[static_cast<size_t>(k) * nvp + slot]]);
+        }
+        return py::make_tuple(off, ids);
+      }, py::arg("sample"));
 
   py::class_<Pot_IPL_12_6, PairPotential>(m, "Pot_IPL_12_6")
       .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))

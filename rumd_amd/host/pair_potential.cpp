@@ -34,12 +34,14 @@ void NeighborList::SetCutoffs(const PairPotential* pot) {
       max_cutoff_ = std::max(max_cutoff_, rc);
     }
   allocated_ = false;  // cut-offs (and possibly the cell grid) changed
+  halo_cap_ = 0;
 }
 
 void NeighborList::SetSkin(float skin) {
   if (!(skin >= 0.f)) throw RUMD_Error("NeighborList", "SetSkin", "skin must be non-negative");
   skin_ = skin;
   allocated_ = false;
+  halo_cap_ = 0;
   if (sample_) sample_->InvalidateGraphs();
 }
 
@@ -105,7 +107,14 @@ void NeighborList::Allocate() {
   if (static_cast<uint64_t>(max_nbrs_) * nvp >= (1ull << 32))
     throw RUMD_cudaResourceError("NeighborList", "Allocate", "neighbour list exceeds 2^32 entries");
   const hipStream_t st = sample_->GetStream();
-  nbl_.resize(static_cast<size_t>(max_nbrs_) * nvp);
+  if (method == SORT) {
+    // entries are stored in groups of 8 (16 bytes) per slot
+    nbl16_.resize(static_cast<size_t>((max_nbrs_ + 7u) & ~7u) * nvp);
+    nbl_.release();
+  } else {
+    nbl_.resize(static_cast<size_t>(max_nbrs_) * nvp);
+    nbl16_.release();
+  }
   num_nbrs_.resize(nvp);
   last_r_.resize(4ull * nvp);
   flags_.resize(8);
@@ -144,6 +153,7 @@ void NeighborList::Allocate() {
     cell_start_.resize(ncells + 1);
     cell_count_.resize(ncells);
     slot_cell_.resize(nvp);
+    lane_order_.resize(nvp);
     sort_scratch_.resize(rumdx_sort_scratch_bytes(nvp, static_cast<uint32_t>(ncells)));
     std::memcpy(desc_.nc, nc, sizeof(nc));
     desc_.cell_rank_dev = cell_rank_.get();
@@ -151,12 +161,32 @@ void NeighborList::Allocate() {
     desc_.cell_count_dev = cell_count_.get();
     desc_.slot_cell_dev = slot_cell_.get();
     desc_.sort_scratch_dev = sort_scratch_.get();
+    desc_.lane_order_dev = lane_order_.get();
+    // cell tiles of the LDS-staged build and force kernels
+    const int ntiles = rumdx_tile_table(nc, rank.data(), nullptr, nullptr);
+    if (ntiles <= 0) rumdx_check(ntiles == 0 ? RUMDX_E_INVALID : ntiles, "NeighborList", "Allocate");
+    std::vector<uint32_t> first(ntiles + 1), origin(ntiles);
+    const int nt2 = rumdx_tile_table(nc, rank.data(), first.data(), origin.data());
+    if (nt2 != ntiles) rumdx_check(RUMDX_E_INVALID, "NeighborList", "Allocate");
+    tile_first_rank_.resize(first.size());
+    tile_first_rank_.upload(first.data(), first.size(), st);
+    tile_origin_.resize(origin.size());
+    tile_origin_.upload(origin.data(), origin.size(), st);
+    if (halo_cap_ == 0) halo_cap_ = rumdx_halo_cap_estimate(nc, n);
+    desc_.nbl16_dev = nbl16_.get();
+    desc_.ntiles = static_cast<uint32_t>(ntiles);
+    desc_.tile_first_rank_dev = tile_first_rank_.get();
+    desc_.tile_origin_dev = tile_origin_.get();
+    desc_.halo_cap = halo_cap_;
   } else {
     cell_rank_.release();
     cell_start_.release();
     cell_count_.release();
     slot_cell_.release();
+    lane_order_.release();
     sort_scratch_.release();
+    tile_first_rank_.release();
+    tile_origin_.release();
   }
   allocated_nvp_ = nvp;
   allocated_ = true;
@@ -190,6 +220,16 @@ bool NeighborList::HandleOverflow() {
   if (!allocated_) return false;
   uint32_t f[8];
   flags_.download(f, 8, sample_->GetStream());
+  if (f[RUMDX_FLAG_ERROR])
+    throw RUMD_Error("NeighborList", "UpdateNBlist", "a cell tile's halo holds more than 65535 particles");
+  if (desc_.nbl16_dev != nullptr && f[RUMDX_FLAG_MAXHALO] > halo_cap_ && halo_cap_ < RUMDX_HALO_CAP_MAX) {
+    // tiles beyond the LDS capacity computed exactly from global memory; give
+    // the next launches room for them
+    halo_cap_ = std::min<uint32_t>(RUMDX_HALO_CAP_MAX, ((f[RUMDX_FLAG_MAXHALO] + f[RUMDX_FLAG_MAXHALO] / 8) + 63u) & ~63u);
+    desc_.halo_cap = halo_cap_;
+    sample_->InvalidateGraphs();
+    if (!f[RUMDX_FLAG_OVERFLOW]) return true;
+  }
   if (!f[RUMDX_FLAG_OVERFLOW]) return false;
   const unsigned n = sample_->GetNumberOfParticles();
   // the build counts past capacity, so MAXLEN is the true longest list
@@ -203,6 +243,26 @@ bool NeighborList::HandleOverflow() {
   allocated_ = false;
   Allocate();
   return true;
+}
+
+void NeighborList::SetHaloCapacity(unsigned cap) {
+  if (cap > RUMDX_HALO_CAP_MAX) throw RUMD_Error("NeighborList", "SetHaloCapacity", "capacity above RUMDX_HALO_CAP_MAX");
+  halo_cap_ = cap;
+  desc_.halo_cap = cap;
+  if (sample_) sample_->InvalidateGraphs();
+}
+
+std::vector<uint32_t> NeighborList::ExportList(std::vector<uint32_t>& counts) {
+  const unsigned nvp = sample_->GetNumberOfVirtualParticles();
+  counts.assign(nvp, 0u);
+  if (!allocated_) return {};
+  DeviceArray<uint32_t> out(static_cast<size_t>(max_nbrs_) * nvp);
+  rumdx_check(rumdx_nblist_export(sample_->GetParticleData().Descriptor(), &desc_, out.get(), sample_->GetStream()),
+              "NeighborList", "ExportList");
+  std::vector<uint32_t> h(static_cast<size_t>(max_nbrs_) * nvp);
+  out.download(h.data(), h.size(), sample_->GetStream());
+  num_nbrs_.download(counts.data(), nvp, sample_->GetStream());
+  return h;
 }
 
 void NeighborList::DownloadCounts(std::vector<uint32_t>& out) {

@@ -176,6 +176,12 @@ class NeighborList {
   bool HandleOverflow();
   bool IsSorted() const { return GetResolvedMethod() == SORT; }
   void DownloadCounts(std::vector<uint32_t>& out);
+  // LDS halo capacity of the tile kernels, in particles (0 = estimate from
+  // the density); tiles with larger halos compute from global memory.
+  void SetHaloCapacity(unsigned cap);
+  unsigned GetHaloCapacity() const { return halo_cap_; }
+  // The list in RUMD's format (u32 slots, column-major), for checks and tools.
+  std::vector<uint32_t> ExportList(std::vector<uint32_t>& counts);
   rumdx_nblist* Descriptor() { return &desc_; }
 
  private:
@@ -193,8 +199,12 @@ class NeighborList {
   unsigned ntypes_ = 0;
   std::vector<float> cutoffs_;  // rc_ij
   DeviceArray<uint32_t> nbl_, num_nbrs_, flags_, cell_rank_, cell_start_, cell_count_, slot_cell_;
+  DeviceArray<uint32_t> tile_first_rank_, tile_origin_;
+  DeviceArray<uint16_t> nbl16_;
+  uint32_t halo_cap_ = 0;  // LDS halo capacity of the tile kernels (particles); 0 = estimate
   DeviceArray<float> last_r_, cut_skin2_;
   DeviceArray<char> sort_scratch_;
+  DeviceArray<uint16_t> lane_order_;
   rumdx_nblist desc_{};
   uint32_t allocated_nvp_ = 0;
 };

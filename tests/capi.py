@@ -50,6 +50,12 @@ class NbList(C.Structure):
         ("cell_count_dev", C.c_void_p),
         ("slot_cell_dev", C.c_void_p),
         ("sort_scratch_dev", C.c_void_p),
+        ("nbl16_dev", C.c_void_p),
+        ("ntiles", C.c_uint32),
+        ("tile_first_rank_dev", C.c_void_p),
+        ("tile_origin_dev", C.c_void_p),
+        ("halo_cap", C.c_uint32),
+        ("lane_order_dev", C.c_void_p),
     ]
 
 
@@ -95,9 +101,14 @@ def lib() -> C.CDLL:
     L.rumdx_zero_momentum.argtypes = [PP, vp, vp]
     L.rumdx_kinetic_energy.argtypes = [PP, NHP, f32, u32, vp, vp, vp]
     L.rumdx_reduce_f4.argtypes = [vp, u32, u32, vp, vp, vp]
+    L.rumdx_tile_table.argtypes = [i32a, vp, vp, vp]
+    L.rumdx_halo_cap_estimate.argtypes = [i32a, u32]
+    L.rumdx_nblist_export.argtypes = [PP, NP, vp, vp]
     for name in declared_symbols():
-        if name not in ("rumdx_version", "rumdx_last_error", "rumdx_workspace_bytes", "rumdx_sort_scratch_bytes"):
+        if name not in ("rumdx_version", "rumdx_last_error", "rumdx_workspace_bytes", "rumdx_sort_scratch_bytes",
+                        "rumdx_halo_cap_estimate"):
             getattr(L, name).restype = C.c_int
+    L.rumdx_halo_cap_estimate.restype = u32
     _lib = L
     return L
 
@@ -190,7 +201,8 @@ class GpuSystem:
     directly through the C ABI (no host C++ layer)."""
 
     def __init__(self, pos4, vel4, box8, nt, param_table, skin=0.3, max_nbrs=None, method="sorted", rlo=False,
-                 with_sts=False, pad=64):
+                 with_sts=False, pad=64, halo_cap=None):
+        self.halo_cap_request = halo_cap
         self.n = int(pos4.shape[0])
         self.nvp = ((self.n + pad - 1) // pad) * pad
         self.nt = nt
@@ -231,12 +243,16 @@ class GpuSystem:
             max_nbrs = min(self.n, 512)
         self.max_nbrs = max_nbrs
         self.d_cut2 = DeviceBuffer.from_array(cut2)
-        self.nbl = DeviceBuffer(4 * self.nvp * max_nbrs)
+        self.nbl = DeviceBuffer(4 * self.nvp * max_nbrs) if self.method == "n2" else None
+        # tile-format entries come in 16-byte groups of 8 per slot
+        self.nbl16 = DeviceBuffer(2 * self.nvp * ((max_nbrs + 7) // 8 * 8)) if self.method == "sorted" else None
+        self.export_buf = DeviceBuffer(4 * self.nvp * max_nbrs)
         self.num_nbrs = DeviceBuffer(4 * self.nvp)
         self.last_r = DeviceBuffer(16 * self.nvp)
         self.flags = DeviceBuffer(32)
         nb = NbList()
-        nb.nbl_dev, nb.num_nbrs_dev, nb.last_r_dev = self.nbl.value, self.num_nbrs.value, self.last_r.value
+        nb.nbl_dev = self.nbl.value if self.nbl else None
+        nb.num_nbrs_dev, nb.last_r_dev = self.num_nbrs.value, self.last_r.value
         nb.flags_dev, nb.cut_skin2_dev = self.flags.value, self.d_cut2.value
         nb.max_nbrs, nb.ntypes, nb.skin = max_nbrs, self.nt, skin
         if self.method == "sorted":
@@ -250,11 +266,29 @@ class GpuSystem:
             self.cell_start = DeviceBuffer(4 * (ncells + 1))
             self.cell_count = DeviceBuffer(4 * ncells)
             self.slot_cell = DeviceBuffer(4 * self.nvp)
+            self.lane_order = DeviceBuffer(2 * self.nvp)
             self.sort_scratch = DeviceBuffer(lib().rumdx_sort_scratch_bytes(self.nvp, ncells))
             nb.nc[:] = [int(x) for x in nc]
             nb.cell_rank_dev, nb.cell_start_dev = self.cell_rank.value, self.cell_start.value
             nb.cell_count_dev, nb.slot_cell_dev = self.cell_count.value, self.slot_cell.value
             nb.sort_scratch_dev = self.sort_scratch.value
+            nb.lane_order_dev = self.lane_order.value
+            ntiles = lib().rumdx_tile_table(nc, None, None, None)
+            assert ntiles > 0
+            first = np.zeros(ntiles + 1, dtype=np.uint32)
+            origin = np.zeros(ntiles, dtype=np.uint32)
+            check(0 if lib().rumdx_tile_table(nc, rank.ctypes.data, first.ctypes.data, origin.ctypes.data) == ntiles
+                  else -1)
+            self.tile_first = DeviceBuffer.from_array(first)
+            self.tile_origin = DeviceBuffer.from_array(origin)
+            self.ntiles = ntiles
+            cap = self.halo_cap_request
+            if cap is None:
+                cap = lib().rumdx_halo_cap_estimate(nc, self.n)
+            nb.nbl16_dev = self.nbl16.value
+            nb.ntiles = ntiles
+            nb.tile_first_rank_dev, nb.tile_origin_dev = self.tile_first.value, self.tile_origin.value
+            nb.halo_cap = cap
         self.nb = nb
         check(lib().rumdx_nblist_request_rebuild(C.byref(self.nb), None))
 
@@ -303,7 +337,8 @@ class GpuSystem:
         return self.flags.download(np.uint32, 8)
 
     def neighbor_lists(self):
-        """List of neighbour slot arrays per slot (num_nbrs entries each)."""
+        """(counts, slots) in RUMD's column-major u32 format, via rumdx_nblist_export."""
+        check(lib().rumdx_nblist_export(C.byref(self.p), C.byref(self.nb), self.export_buf.value, None))
         cnt = self.num_nbrs.download(np.uint32, self.nvp)[: self.n]
-        nbl = self.nbl.download(np.uint32, (self.max_nbrs, self.nvp))
+        nbl = self.export_buf.download(np.uint32, (self.max_nbrs, self.nvp))
         return cnt, nbl

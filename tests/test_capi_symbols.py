@@ -48,6 +48,34 @@ def test_host_side_helpers_without_gpu():
     assert lib.rumdx_cell_rank_table(bad, rank) == -1
 
 
+def test_tile_table_bricks_are_contiguous_rank_ranges():
+    """Aligned 4^3 bricks of Morton-ranked cells are contiguous rank ranges,
+    ordered by rank; edge bricks of grids that are not multiples of 4 too."""
+    lib = capi.lib()
+    for nc_list in ([68, 68, 68], [27, 27, 27], [6, 7, 5], [9, 12, 10]):
+        nc = np.array(nc_list, dtype=np.int32)
+        ncells = int(nc.prod())
+        rank = np.zeros(ncells, dtype=np.uint32)
+        assert lib.rumdx_cell_rank_table(nc, rank) == 0
+        nt = lib.rumdx_tile_table(nc, None, None, None)
+        assert nt == int(np.prod((nc + 3) // 4))
+        first = np.zeros(nt + 1, dtype=np.uint32)
+        origin = np.zeros(nt, dtype=np.uint32)
+        assert lib.rumdx_tile_table(nc, rank.ctypes.data, first.ctypes.data, origin.ctypes.data) == nt
+        assert first[0] == 0 and first[-1] == ncells and np.all(np.diff(first.astype(np.int64)) > 0)
+        inv = np.argsort(rank)  # rank -> linear cell
+        for t in range(nt):
+            cells = inv[first[t]:first[t + 1]]
+            x, y, z = cells % nc[0], (cells // nc[0]) % nc[1], cells // (nc[0] * nc[1])
+            ox, oy, oz = origin[t] & 1023, (origin[t] >> 10) & 1023, (origin[t] >> 20) & 1023
+            assert x.min() == ox and y.min() == oy and z.min() == oz
+            bw = [min(4, nc[a] - o) for a, o in enumerate((ox, oy, oz))]
+            assert cells.size == bw[0] * bw[1] * bw[2]
+            assert x.max() < ox + 4 and y.max() < oy + 4 and z.max() < oz + 4
+    cap = lib.rumdx_halo_cap_estimate(np.array([68, 68, 68], np.int32), 1 << 20)
+    assert cap % 64 == 0 and 512 * (1 << 20) / 68 ** 3 < cap <= 3072
+
+
 def test_struct_layouts_match_header(tmp_path):
     """ctypes mirrors of the ABI structs have the C compiler's sizes and offsets."""
     src = tmp_path / "layout.c"

@@ -74,6 +74,14 @@ def test_sorted_api_order_and_forces():
     assert force_errors(s.GetForces(), f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
     assert pot.GetNumberOfNbListBuilds() == 1
     assert 100 < pot.GetActualMaxNumNbrs() < 200
+    # neighbour sets in original ids, bit-exact against the FP32 restatement
+    off, ids = pot.GetNeighborListCSR(s)
+    b8 = configs.box8(sysm.box)
+    o2, i2 = oracle.nblist_cells_f32(configs.pack_pos4(sysm.positions, sysm.types), b8, 2,
+                                     configs.cut_skin2_table(ka_table(), 0.3), oracle.cell_grid(b8, 2.5, 0.3))
+    assert np.array_equal(off, o2)
+    for i in range(0, sysm.n, 97):
+        assert set(ids[off[i]:off[i + 1]].tolist()) == set(i2[o2[i]:o2[i + 1]].tolist())
 
 
 def _run_api_loop(s, itg, n):

@@ -138,6 +138,47 @@ def test_nblist_sets_bit_exact_sorted():
     assert f[1] == 0 and f[0] == 0 and f[2] == 1
 
 
+@pytest.mark.parametrize("halo_cap", [64, None])
+def test_tile_halo_capacity_fallback_exact(halo_cap):
+    """Tiles whose halo exceeds the LDS capacity (here: every tile, cap 64)
+    build the same list sets and compute the same forces from global memory."""
+    sysm = configs.ka_random_lattice(30000, density=1.2, seed=12)
+    table = ka_table()
+    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
+    b8 = configs.box8(sysm.box)
+    g = GpuSystem(pos4, configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses), b8, 2, table, method="sorted",
+                  max_nbrs=200, halo_cap=halo_cap)
+    print(f"[case] tile halo cap={halo_cap}", flush=True)
+    g.calc_f(SF)
+    uidx = g.unsorted_index()
+    flags = g.flags_host()
+    assert flags[6] == 0 and flags[5] > 0
+    if halo_cap == 64:
+        assert flags[5] > 64
+    cnt, nbl = g.neighbor_lists()
+    nc = oracle.cell_grid(b8, 2.5, 0.3)
+    off, idx = oracle.nblist_cells_f32(pos4, b8, 2, configs.cut_skin2_table(table, 0.3), nc)
+    assert gpu_sets_original_ids(cnt, nbl, uidx) == csr_to_sets(off, idx)
+    f = unsort(g.get("f"), uidx)
+    f_ref, _ = oracle.forces_cells(pos4, sysm.box, 2, table, SF)
+    assert force_errors(f[:, :3], f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
+
+
+@pytest.mark.parametrize("name", ["KA_N256_L6.0_T0.50.xyz.gz", "KA_N1000_T0.466.xyz.gz"])
+def test_tiles_small_boxes_full_wrap(golden_dir, name):
+    """Boxes of 5-6 cells per axis: every tile's halo wraps onto itself; the
+    sorted build must still give RUMD's sets (each cell once)."""
+    table = ka_table()
+    c, pos4, _, g = _golden_system(golden_dir, name, table, "sorted")
+    g.update_nblist()
+    cnt, nbl = g.neighbor_lists()
+    b8 = configs.box8(c.box)
+    nc = oracle.cell_grid(b8, 2.5, 0.3)
+    assert int(nc.min()) <= 6
+    off, idx = oracle.nblist_cells_f32(pos4, b8, 2, configs.cut_skin2_table(table, 0.3), nc)
+    assert gpu_sets_original_ids(cnt, nbl, g.unsorted_index()) == csr_to_sets(off, idx)
+
+
 def test_nblist_overflow_falls_back_exactly():
     """A list capacity far below the real neighbour count: the overflow flag is
     raised and the forces are still exact (cell rescan for overflowed lists)."""
