@@ -13,8 +13,8 @@
 // in a graph).  The last workgroup of the final kernel clears the flag.
 //
 // The sorted build is a deterministic counting sort: histogram of cell ranks,
-// two-pass exclusive scan, scatter, then each cell's slots are put back in
-// ascending old-slot order (what a stable key sort gives) and gathered.  The gather writes
+// two-pass exclusive scan, scatter, then a gather that puts each cell's
+// particles in ascending old-slot order (what a stable key sort gives).  The gather writes
 // into scratch; the list kernel reads the sorted positions from there and copies
 // every array back to its home buffer for its own slot, so buffer addresses
 // never change (captured graphs stay valid).
@@ -30,7 +30,7 @@ namespace {
 
 struct SortScratch {
   uint32_t* key_lin;    // nvp: linear cell of each old slot
-  uint32_t* perm;       // nvp: old slot of each new slot
+  uint32_t* perm;       // nvp: old slots grouped by cell (scatter output)
   float4* t_r;          // nvp each
   float4* t_v;
   float4* t_f;
@@ -40,6 +40,9 @@ struct SortScratch {
   uint32_t* t_idx;      // nvp
   uint32_t* chunk_sum;  // one per kScanChunk cells
 };
+
+// Grid cap of the particle-parallel rebuild kernels (they loop over the rest).
+constexpr uint32_t kChainGridCap = 1024;
 
 // Cells per workgroup of the two-pass count scan.
 constexpr uint32_t kScanPerThread = 16;
@@ -166,11 +169,11 @@ __global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__
                                                       uint32_t* __restrict__ cell_count, Box bx, CellGeom g,
                                                       uint32_t n) {
   if (!rebuild_pending(flags)) return;
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t lin = cell_of(r[i], bx, g);
-  key_lin[i] = lin;
-  atomicAdd(cell_count + cell_rank[lin], 1u);
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const uint32_t lin = cell_of(r[i], bx, g);
+    key_lin[i] = lin;
+    atomicAdd(cell_count + cell_rank[lin], 1u);
+  }
 }
 
 // Total of v over a kBlock-thread workgroup, returned in every thread.
@@ -188,7 +191,7 @@ __device__ __forceinline__ uint32_t block_total_u32(uint32_t v, uint32_t* lds4) 
 // Exclusive scan of the cell counts in two passes over kScanChunk-cell
 // chunks: chunk totals, then each chunk adds the totals before it and scans
 // itself.  cell_start[c] = sum of counts before c, cell_start[ncells] = n; the
-// counts are zeroed so the scatter can use them as per-cell cursors.
+// counts stay, as the scatter's per-cell cursors.
 __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restrict__ cell_count,
                                                         const uint32_t* __restrict__ flags,
                                                         uint32_t* __restrict__ chunk_sum, uint32_t ncells) {
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_reduce(const uint32_t* __restri
   if (threadIdx.x == 0) chunk_sum[blockIdx.x] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ cell_count,
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* __restrict__ cell_count,
                                                        const uint32_t* __restrict__ flags,
                                                        const uint32_t* __restrict__ chunk_sum,
                                                        uint32_t* __restrict__ cell_start, uint32_t ncells) {
@@ -258,24 +261,20 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(uint32_t* __restrict__ ce
   }
   if (full) {
     uint4* ds = reinterpret_cast<uint4*>(cell_start + c0);
-    uint4* dc = reinterpret_cast<uint4*>(cell_count + c0);
 #pragma unroll
-    for (uint32_t q = 0; q < kScanPerThread / 4; ++q) {
+    for (uint32_t q = 0; q < kScanPerThread / 4; ++q)
       ds[q] = make_uint4(st[4 * q], st[4 * q + 1], st[4 * q + 2], st[4 * q + 3]);
-      dc[q] = make_uint4(0u, 0u, 0u, 0u);
-    }
   } else {
 #pragma unroll
-    for (uint32_t k = 0; k < kScanPerThread; ++k) {
-      if (c0 + k < ncells) {
-        cell_start[c0 + k] = st[k];
-        cell_count[c0 + k] = 0u;
-      }
-    }
+    for (uint32_t k = 0; k < kScanPerThread; ++k)
+      if (c0 + k < ncells) cell_start[c0 + k] = st[k];
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) cell_start[ncells] = carry + tot;
 }
 
+// Places every old slot in its cell's slot range, counting the cell's cursor
+// down so the counts are zero again for the next build.  The order inside a
+// cell is whatever the atomics give; the gather restores it.
 __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__ key_lin,
                                                     const uint32_t* __restrict__ flags,
                                                     const uint32_t* __restrict__ cell_rank,
@@ -283,53 +282,42 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
                                                     uint32_t n) {
   if (!rebuild_pending(flags)) return;
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t rank = cell_rank[key_lin[i]];
-  const uint32_t slot = cell_start[rank] + atomicAdd(cell_count + rank, 1u);
-  if (slot < n) perm[slot] = i;  // always true when the counts started at zero
-}
-
-// One thread per cell: restore ascending old-slot order inside the cell (the
-// scatter's order is unspecified) and re-zero its count for the next build.
-__global__ __launch_bounds__(kBlock) void k_cell_order(const uint32_t* __restrict__ flags,
-                                                       const uint32_t* __restrict__ cell_start,
-                                                       uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
-                                                       uint32_t ncells) {
-  if (!rebuild_pending(flags)) return;
-  const uint32_t c = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (c >= ncells) return;
-  const uint32_t b0 = cell_start[c], e0 = cell_start[c + 1];
-  cell_count[c] = 0u;
-  for (uint32_t a = b0 + 1; a < e0; ++a) {
-    const uint32_t x = perm[a];
-    uint32_t b = a;
-    while (b > b0 && perm[b - 1] > x) {
-      perm[b] = perm[b - 1];
-      --b;
-    }
-    perm[b] = x;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const uint32_t rank = cell_rank[key_lin[i]];
+    const uint32_t slot = cell_start[rank] + atomicSub(cell_count + rank, 1u) - 1u;
+    if (slot < n) perm[slot] = i;  // always true when hist and scatter saw the same keys
   }
 }
 
-// One thread per new slot: gather the particle arrays into scratch.
+// Gathers the particle arrays into scratch in sorted order.  Thread k takes
+// the old slot x the scatter put at k; its new slot is the cell's first slot
+// plus the number of cell members with a smaller old slot, i.e. cells keep
+// ascending old-slot order (what a stable key sort gives).
 __global__ __launch_bounds__(kBlock) void k_gather(const uint32_t* __restrict__ flags,
-                                                   const uint32_t* __restrict__ perm, const float4* __restrict__ r,
+                                                   const uint32_t* __restrict__ perm,
+                                                   const uint32_t* __restrict__ key_lin,
+                                                   const uint32_t* __restrict__ cell_rank,
+                                                   const uint32_t* __restrict__ cell_start,
+                                                   const float4* __restrict__ r,
                                                    const float4* __restrict__ v, const float4* __restrict__ f,
                                                    const float4* __restrict__ w, const float4* __restrict__ im,
                                                    const float4* __restrict__ rlo, const uint32_t* __restrict__ idx,
                                                    SortScratch s, uint32_t n) {
   if (!rebuild_pending(flags)) return;
-  const uint32_t k = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t o = perm[k];
-  s.t_r[k] = r[o];
-  s.t_v[k] = v[o];
-  s.t_f[k] = f[o];
-  s.t_w[k] = w[o];
-  s.t_im[k] = im[o];
-  if (rlo != nullptr) s.t_rlo[k] = rlo[o];
-  s.t_idx[k] = idx[o];
+  for (uint32_t k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+    const uint32_t x = perm[k];
+    const uint32_t rank = cell_rank[key_lin[x]];
+    const uint32_t b0 = cell_start[rank], e0 = cell_start[rank + 1];
+    uint32_t d = b0;
+    for (uint32_t j = b0; j < e0; ++j) d += perm[j] < x ? 1u : 0u;
+    s.t_r[d] = r[x];
+    s.t_v[d] = v[x];
+    s.t_f[d] = f[x];
+    s.t_w[d] = w[x];
+    s.t_im[d] = im[x];
+    if (rlo != nullptr) s.t_rlo[d] = rlo[x];
+    s.t_idx[d] = idx[x];
+  }
 }
 
 struct HomeArrays {
@@ -345,13 +333,177 @@ struct HomeArrays {
   uint16_t* lane_order;
 };
 
+// Per-wave LDS of the cooperative list build: the stencil segments of the
+// cell being built and a window of its candidates (LDS offsets, in stencil
+// order).
+constexpr uint32_t kCandWindow = 512;
+struct WaveBuild {
+  uint16_t cand[kCandWindow];
+  uint16_t seg_start[128];  // LDS offset of the segment's first particle
+  uint16_t seg_pre[128];    // candidates before the segment
+  uint16_t seg_len[128];
+};
+
+// Orders the LDS accesses of one wave (lanes read what other lanes wrote).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Appends the accepted candidates of one 64-lane chunk to the list of slot i,
+// which holds cnt entries so far; returns the new count (uniform).
+__device__ __forceinline__ uint32_t append_chunk(bool acc, uint32_t entry, uint32_t cnt, uint16_t* __restrict__ nbl16,
+                                                 uint32_t i, uint32_t nvp, uint32_t max_nbrs) {
+  const uint64_t mask = __ballot(acc);
+  const uint32_t k = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+  if (acc && k < max_nbrs) nbl16[(static_cast<size_t>(k >> 3) * nvp + i) * 8 + (k & 7u)] = static_cast<uint16_t>(entry);
+  return cnt + static_cast<uint32_t>(__popcll(mask));
+}
+
+// Lists of one cell of a staged tile, built by one wave: lanes enumerate the
+// cell's stencil candidates (the 125 cells, or 25 rows of 5 consecutive halo
+// cells, in RUMD's (dZ, dY, dX) order, ascending slot inside each cell) and
+// test them against one particle at a time; a ballot places the accepted
+// entries in candidate order.  Writes the lists and leaves each particle's
+// full count (which may exceed max_nbrs) in s_cnt[slot - slot0].
+template <bool WRAPS>
+__device__ __forceinline__ void build_cell_wave(const Halo& h, const HaloLds& L, const TileGeom& tg,
+                                                const float4* s_pos, const float* s_cut, uint32_t nt, const Box& bx,
+                                                int cx, int cy, int cz, WaveBuild& wb, uint32_t* s_cnt,
+                                                uint16_t* __restrict__ nbl16, uint32_t nvp, uint32_t max_nbrs,
+                                                bool bytes) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t hc = halo_cell(h, tg, cx, cy, cz);
+  // wave-uniform values, made scalar so the loops below stay on full waves
+  const uint32_t coff = __builtin_amdgcn_readfirstlane(L.off[hc]);
+  const uint32_t m = __builtin_amdgcn_readfirstlane(L.off[hc + 1]) - coff;
+  const uint32_t gstart = __builtin_amdgcn_readfirstlane(L.start[hc]);
+  if (m == 0) return;
+  wave_lds_sync();  // the previous cell's tables have been read
+  const int hx = halo_axis(cx, h.base[0], tg.nc0);
+  const bool rows = hx >= 2 && hx + 2 < h.ext[0];
+  const uint32_t nseg = rows ? 25u : 125u;
+  // segment table and its exclusive prefix (two 64-lane passes)
+  uint32_t carry = 0;
+  for (uint32_t s0 = 0; s0 < nseg; s0 += 64) {
+    const uint32_t sg = s0 + lane;
+    uint32_t st = 0, len = 0;
+    if (sg < nseg) {
+      uint32_t hcell;
+      if (rows) {
+        const int hz = halo_axis(cz + static_cast<int>(sg / 5) - 2, h.base[2], tg.nc2);
+        const int hy = halo_axis(cy + static_cast<int>(sg % 5) - 2, h.base[1], tg.nc1);
+        hcell = static_cast<uint32_t>(hx - 2 + h.ext[0] * (hy + h.ext[1] * hz));
+        st = L.off[hcell];
+        len = L.off[hcell + 5] - st;
+      } else {
+        hcell = halo_cell(h, tg, cx + static_cast<int>(sg % 5) - 2, cy + static_cast<int>((sg / 5) % 5) - 2,
+                          cz + static_cast<int>(sg / 25) - 2);
+        st = L.off[hcell];
+        len = L.off[hcell + 1] - st;
+      }
+    }
+    uint32_t x = len;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= static_cast<uint32_t>(o)) x += y;
+    }
+    if (sg < nseg) {
+      wb.seg_start[sg] = static_cast<uint16_t>(st);
+      wb.seg_pre[sg] = static_cast<uint16_t>(carry + x - len);
+      wb.seg_len[sg] = static_cast<uint16_t>(len);
+    }
+    carry += __shfl(x, 63, 64);
+  }
+  const uint32_t total = __builtin_amdgcn_readfirstlane(carry);
+  wave_lds_sync();
+  for (uint32_t pg = 0; pg < m; pg += 64) {
+    const uint32_t mg = (m - pg) < 64u ? (m - pg) : 64u;
+    uint32_t cnt_v = 0;  // lane l: count of particle pg + l
+    for (uint32_t w0 = 0; w0 < total; w0 += kCandWindow) {
+      const uint32_t wn = (total - w0) < kCandWindow ? (total - w0) : kCandWindow;
+      wave_lds_sync();  // the previous window has been read
+      for (uint32_t sg = lane; sg < nseg; sg += 64) {
+        const uint32_t pre = wb.seg_pre[sg], len = wb.seg_len[sg], st = wb.seg_start[sg];
+        const uint32_t lo = pre > w0 ? pre : w0;
+        const uint32_t hi = (pre + len) < (w0 + wn) ? (pre + len) : (w0 + wn);
+        for (uint32_t q = lo; q < hi; ++q) wb.cand[q - w0] = static_cast<uint16_t>(st + (q - pre));
+      }
+      wave_lds_sync();
+      for (uint32_t p = 0; p < mg; ++p) {
+        const uint32_t po = coff + pg + p;
+        const float4 pi = s_pos[po];
+        const float* crow = s_cut + __float_as_uint(pi.w) * nt;
+        const uint32_t i = gstart + pg + p;
+        uint32_t cnt = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt_v), static_cast<int>(p)));
+        // four 64-candidate chunks per pass: their LDS reads are issued
+        // together, so one round trip serves 256 candidates
+        for (uint32_t q0 = 0; q0 < wn; q0 += 4 * 64) {
+          uint32_t oo[4];
+          float4 rj[4];
+          float cut[4];
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t q = q0 + 64 * u + lane;
+            oo[u] = q < wn ? static_cast<uint32_t>(wb.cand[q]) : po;
+          }
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u) rj[u] = s_pos[oo[u]];
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u) cut[u] = crow[__float_as_uint(rj[u].w)];
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u) {
+            const float d2 = WRAPS ? min_image_d2_f32(pi, rj[u], bx) : plain_d2_f32(pi, rj[u]);
+            const bool acc = oo[u] != po && d2 < cut[u];
+            cnt = append_chunk(acc, encode_entry(oo[u], bytes), cnt, nbl16, i, nvp, max_nbrs);
+          }
+        }
+        cnt_v = lane == p ? cnt : cnt_v;
+      }
+    }
+    if (lane < mg) s_cnt[gstart + pg + lane - h.slot0] = cnt_v;
+  }
+}
+
+// Lists of one particle read from global memory (tiles whose halo does not
+// fit in LDS): the same stencil order, one lane per particle.
+__device__ __forceinline__ uint32_t build_particle_lane(uint32_t i, float4 ri, int cx, int cy, int cz, const Halo& h,
+                                                        const HaloLds& L, const TileGeom& tg, const float4* t_r,
+                                                        const float* crow, const Box& bx, u32x4* grp, uint32_t nvp,
+                                                        uint32_t max_nbrs, bool bytes) {
+  const uint32_t hc = halo_cell(h, tg, cx, cy, cz);
+  const uint32_t my_off = L.off[hc] + (i - L.start[hc]);
+  u32x4 w = {0u, 0u, 0u, 0u};
+  uint32_t cnt = 0;
+  for (int dz = -2; dz <= 2; ++dz)
+    for (int dy = -2; dy <= 2; ++dy)
+      for (int dx = -2; dx <= 2; ++dx) {
+        const uint32_t hcell = halo_cell(h, tg, cx + dx, cy + dy, cz + dz);
+        const uint32_t o0 = L.off[hcell], o1 = L.off[hcell + 1], g0 = L.start[hcell];
+        for (uint32_t oo = o0; oo < o1; ++oo) {
+          if (oo == my_off) continue;
+          const float4 rj = t_r[g0 + (oo - o0)];
+          const float d2 = h.wraps ? min_image_d2_f32(ri, rj, bx) : plain_d2_f32(ri, rj);
+          if (d2 < crow[__float_as_uint(rj.w)]) {
+            if (cnt < max_nbrs) group_append(w, cnt, encode_entry(oo, bytes), grp, nvp, i);
+            ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
+          }
+        }
+      }
+  const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
+  if (stored & 7u) grp[static_cast<size_t>(stored >> 3) * nvp + i] = w;  // partial last group
+  return cnt;
+}
+
 // calculateNBL_CellsSorted (src/NeighborList.cu:1204-1319), one workgroup per
 // cell tile with the tile's halo staged in LDS: for each particle, scan the
 // 5x5x5 cell stencil in RUMD's (dZ, dY, dX) order, ascending slot inside each
 // cell, and accept j != i with d^2 < (rc_ij + skin)^2 (FP32, the reference's
 // minimum-image arithmetic; a tile whose halo does not wrap needs no image
 // correction, and the plain difference is then the same number).  Entries are
-// halo offsets.  Each particle's sorted data is copied back to its home slot.
+// halo positions.  Each particle's sorted data is copied back to its home slot.
 __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ flags, SortScratch s, TileGeom tg,
                                                         const float* __restrict__ cut_skin2, uint32_t nt, Box bx,
                                                         HomeArrays o, uint16_t* __restrict__ nbl16,
@@ -360,7 +512,8 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   if (!rebuild_pending(flags)) return;
   __shared__ HaloLds L;
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
-  __shared__ uint32_t s_key[kOrderMax];
+  __shared__ uint32_t s_key[kOrderMax];  // list lengths, then sort keys (same index, same thread)
+  __shared__ WaveBuild s_wave[kBlock / 64];
   extern __shared__ float4 s_pos[];
   for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_skin2[k];
   const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
@@ -369,51 +522,42 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   const bool staged = h.npart <= tg.halo_cap;
   const bool bytes = h.npart <= kByteEntryHalo;
   const uint32_t np = h.slot1 - h.slot0;
+  // wave-cooperative lists for staged tiles; one lane per particle otherwise
+  const bool coop = staged && np <= kOrderMax;
+  if (coop) {
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int c0x = h.base[0] + 2, c0y = h.base[1] + 2, c0z = h.base[2] + 2;
+    const uint32_t nown = static_cast<uint32_t>(h.own[0] * h.own[1] * h.own[2]);
+    for (uint32_t cc = wave; cc < nown; cc += kBlock / 64) {
+      const int a = static_cast<int>(cc % h.own[0]);
+      const int b = static_cast<int>((cc / h.own[0]) % h.own[1]);
+      const int c = static_cast<int>(cc / (h.own[0] * h.own[1]));
+      int cx = c0x + a, cy = c0y + b, cz = c0z + c;
+      cx -= cx >= tg.nc0 ? tg.nc0 : 0;
+      cy -= cy >= tg.nc1 ? tg.nc1 : 0;
+      cz -= cz >= tg.nc2 ? tg.nc2 : 0;
+      if (h.wraps)
+        build_cell_wave<true>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp, max_nbrs,
+                              bytes);
+      else
+        build_cell_wave<false>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp, max_nbrs,
+                               bytes);
+    }
+  }
+  __syncthreads();
   for (uint32_t i = h.slot0 + threadIdx.x; i < h.slot1; i += kBlock) {
     const float4 ri = s.t_r[i];
-    const float* crow = s_cut + __float_as_uint(ri.w) * nt;
     const int cx = cell_coord(ri.x, bx.iL[0], tg.nc0);
     const int cy = cell_coord(ri.y, bx.iL[1], tg.nc1);
     const int cz = cell_coord(ri.z, bx.iL[2], tg.nc2);
-    const uint32_t hc = halo_cell(h, tg, cx, cy, cz);
-    const uint32_t my_off = L.off[hc] + (i - L.start[hc]);
-    u32x4* grp = reinterpret_cast<u32x4*>(nbl16);
-    u32x4 w = {0u, 0u, 0u, 0u};
-    uint32_t cnt = 0;
-    auto consider = [&](uint32_t oo, float4 rj) {
-      const float d2 = h.wraps ? min_image_d2_f32(ri, rj, bx) : plain_d2_f32(ri, rj);
-      if (d2 < crow[__float_as_uint(rj.w)]) {
-        if (cnt < max_nbrs) group_append(w, cnt, encode_entry(oo, bytes), grp, nvp, i);
-        ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
-      }
-    };
-    // The five cells (cx-2..cx+2, y, z) are consecutive halo cells, hence one
-    // contiguous run of LDS offsets in ascending dX order, unless the halo
-    // covers the whole x axis and the stencil wraps inside it.
-    const int hx = halo_axis(cx, h.base[0], tg.nc0);
-    const bool rows = staged && hx >= 2 && hx + 2 < h.ext[0];
-    for (int dz = -2; dz <= 2; ++dz) {
-      const int hz = halo_axis(cz + dz, h.base[2], tg.nc2);
-      for (int dy = -2; dy <= 2; ++dy) {
-        const int hy = halo_axis(cy + dy, h.base[1], tg.nc1);
-        if (rows) {
-          const uint32_t hc0 = static_cast<uint32_t>(hx - 2 + h.ext[0] * (hy + h.ext[1] * hz));
-          const uint32_t o1 = L.off[hc0 + 5];
-          for (uint32_t oo = L.off[hc0]; oo < o1; ++oo)
-            if (oo != my_off) consider(oo, s_pos[oo]);
-        } else {
-          for (int dx = -2; dx <= 2; ++dx) {
-            const uint32_t hcell = static_cast<uint32_t>(halo_axis(cx + dx, h.base[0], tg.nc0) +
-                                                         h.ext[0] * (hy + h.ext[1] * hz));
-            const uint32_t o0 = L.off[hcell], o1 = L.off[hcell + 1], g0 = L.start[hcell];
-            for (uint32_t oo = o0; oo < o1; ++oo)
-              if (oo != my_off) consider(oo, staged ? s_pos[oo] : s.t_r[g0 + (oo - o0)]);
-          }
-        }
-      }
+    uint32_t cnt;
+    if (coop) {
+      cnt = s_key[i - h.slot0];
+    } else {
+      cnt = build_particle_lane(i, ri, cx, cy, cz, h, L, tg, s.t_r, s_cut + __float_as_uint(ri.w) * nt, bx,
+                                reinterpret_cast<u32x4*>(nbl16), nvp, max_nbrs, bytes);
     }
     const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
-    if (stored & 7u) grp[static_cast<size_t>(stored >> 3) * nvp + i] = w;  // partial last group
     num_nbrs[i] = stored;  // == max_nbrs marks an overflowed list
     if (np <= kOrderMax) s_key[i - h.slot0] = ((0xffffu - min(stored, 0xffffu)) << 16) | (i - h.slot0);
     if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
@@ -594,21 +738,24 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
   const uint32_t nblk = blocks_for(p->n);
   const float4* r = reinterpret_cast<const float4*>(p->r_dev);
 
-  hipLaunchKernelGGL(k_cell_hist, dim3(nblk), dim3(kBlock), 0, st, r, nb->flags_dev, nb->cell_rank_dev, s.key_lin,
+  // The chain runs every step and exits at once unless a rebuild is pending,
+  // so the particle-parallel kernels loop over a capped grid: an idle launch
+  // then costs little more than its dispatch.
+  const uint32_t ngrid = std::min(nblk, kChainGridCap);
+  hipLaunchKernelGGL(k_cell_hist, dim3(ngrid), dim3(kBlock), 0, st, r, nb->flags_dev, nb->cell_rank_dev, s.key_lin,
                      nb->cell_count_dev, bx, g, p->n);
   const uint32_t nchunk = scan_chunks(g.ncells);
   hipLaunchKernelGGL(k_scan_reduce, dim3(nchunk), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.chunk_sum,
                      g.ncells);
   hipLaunchKernelGGL(k_scan_apply, dim3(nchunk), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.chunk_sum,
                      nb->cell_start_dev, g.ncells);
-  hipLaunchKernelGGL(k_scatter, dim3(nblk), dim3(kBlock), 0, st, s.key_lin, nb->flags_dev, nb->cell_rank_dev,
+  hipLaunchKernelGGL(k_scatter, dim3(ngrid), dim3(kBlock), 0, st, s.key_lin, nb->flags_dev, nb->cell_rank_dev,
                      nb->cell_start_dev, nb->cell_count_dev, s.perm, p->n);
-  hipLaunchKernelGGL(k_cell_order, dim3(blocks_for(g.ncells)), dim3(kBlock), 0, st, nb->flags_dev, nb->cell_start_dev,
-                     nb->cell_count_dev, s.perm, g.ncells);
-  hipLaunchKernelGGL(k_gather, dim3(nblk), dim3(kBlock), 0, st, nb->flags_dev, s.perm, r,
-                     reinterpret_cast<const float4*>(p->v_dev), reinterpret_cast<const float4*>(p->f_dev),
-                     reinterpret_cast<const float4*>(p->w_dev), reinterpret_cast<const float4*>(p->im_dev),
-                     reinterpret_cast<const float4*>(p->rlo_dev), p->unsorted_index_dev, s, p->n);
+  hipLaunchKernelGGL(k_gather, dim3(ngrid), dim3(kBlock), 0, st, nb->flags_dev, s.perm, s.key_lin, nb->cell_rank_dev,
+                     nb->cell_start_dev, r, reinterpret_cast<const float4*>(p->v_dev),
+                     reinterpret_cast<const float4*>(p->f_dev), reinterpret_cast<const float4*>(p->w_dev),
+                     reinterpret_cast<const float4*>(p->im_dev), reinterpret_cast<const float4*>(p->rlo_dev),
+                     p->unsorted_index_dev, s, p->n);
   const TileGeom tg{nb->nc[0], nb->nc[1], nb->nc[2], nb->cell_rank_dev, nb->cell_start_dev,
                     nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, nb->halo_cap};
   const HomeArrays home{reinterpret_cast<float4*>(p->r_dev), reinterpret_cast<float4*>(p->v_dev),

@@ -33,6 +33,7 @@ struct TileGeom {
 struct Halo {
   int base[3];   // first halo cell coordinate (may be negative: periodic)
   int ext[3];    // halo cells per axis (<= 8, <= nc)
+  int own[3];    // the tile's own cells per axis (<= 4); they sit at halo index 2..2+own-1
   bool wraps;    // some halo cell lies across the periodic boundary
   uint32_t ncell;
   uint32_t npart;  // particles in the halo
@@ -106,6 +107,7 @@ __device__ __forceinline__ void stage_halo(uint32_t t, const TileGeom& g, const 
       const int ext = (bw + 4) < nc[a] ? (bw + 4) : nc[a];
       L.h.base[a] = c0[a] - 2;
       L.h.ext[a] = ext;
+      L.h.own[a] = bw;
       wraps = wraps || (c0[a] - 2 < 0) || (c0[a] - 2 + ext > nc[a]);
       ncell *= static_cast<uint32_t>(ext);
     }
