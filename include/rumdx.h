@@ -86,7 +86,12 @@ typedef struct rumdx_particles {
  *                  rumdx_nblist_export() converts to slots.  lane_order_dev
  *                  holds, per tile, the tile's particles (local index) by
  *                  descending list length, so the force kernel gives each
- *                  wave lists of similar length.
+ *                  wave lists of similar length.  tile_halo_dev and
+ *                  tile_info_dev record each tile's halo (slots in halo order)
+ *                  as the build saw it, so the force kernel stages the halo
+ *                  without rebuilding its table; they are sized by halo_cap,
+ *                  and a change of halo_cap needs a rebuild before the next
+ *                  force call (rumdx_nblist_request_rebuild).
  * num_nbrs_dev[i] == max_nbrs marks a list that overflowed; the force kernel
  * then rescans the cell stencil for that particle. */
 typedef struct rumdx_nblist {
@@ -111,6 +116,8 @@ typedef struct rumdx_nblist {
   const uint32_t* tile_origin_dev;     /* ntiles: cx0 | cy0 << 10 | cz0 << 20 */
   uint32_t halo_cap;        /* halo particles a tile stages in LDS (<= RUMDX_HALO_CAP_MAX) */
   uint16_t* lane_order_dev; /* sorted method: nvp, written by the build */
+  uint32_t* tile_halo_dev;  /* sorted method: ntiles x halo_cap slots of each tile's halo, written by the build */
+  uint32_t* tile_info_dev;  /* sorted method: ntiles x 4, written by the build */
 } rumdx_nblist;
 
 /* Cells per tile edge and the largest LDS halo a launch may request. */
@@ -193,6 +200,16 @@ int rumdx_pair_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float*
                   uint32_t ntypes, const float box[8], int cutoff_method, int initialize,
                   int calc_stresses, void* stream);
 
+/* All-FP64 variant of rumdx_pair_lj, the comparison path of the precision
+ * study: positions r (+ rlo when set) widened to double, the minimum image,
+ * the LJ term, the cut-off shifts (recomputed in double from rc, A12, A6) and
+ * the per-particle sums all in double, rounded once into the float4 outputs.
+ * Same arguments, lists and error behaviour as rumdx_pair_lj.  No
+ * counterpart in the reference, whose precision is a build option. */
+int rumdx_pair_lj_fp64(const rumdx_particles* p, const rumdx_nblist* nb, const float* params_dev,
+                       uint32_t ntypes, const float box[8], int cutoff_method, int initialize,
+                       int calc_stresses, void* stream);
+
 /* ---------------------------------------------------------------- integrator
  * IntegratorNVT::Integrate (integrateNVTAlgorithm, src/IntegratorNVT.cu:64-79,
  * 364-418) followed, when thermostat_on_mask != 0, by UpdateThermostatState
@@ -225,6 +242,81 @@ int rumdx_kinetic_energy(const rumdx_particles* p, const rumdx_nh_state* nh_dev,
  * bitwise reproducible. */
 int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_mask, double* out_dev,
                     void* workspace_dev, void* stream);
+
+/* ---------------------------------------------------------------- replicas
+ * Batched independent samples ("replicas") in one set of device arrays,
+ * advanced by one launch per kernel for all of them.  No counterpart in the
+ * reference, which runs one Sample at a time; it is for workloads of many
+ * small systems (parameter sweeps, ensembles), where per-sample launches
+ * leave the GPU idle.  Each replica follows the single-sample algorithms
+ * above: a cubic box, RUMD's cell grid and 5^3 stencil with the FP32
+ * criterion, u32 lists (entry k of slot i at [k*nslots + i]), the LJ pair
+ * term in FP32 with FP64 displacements, and its own Nose-Hoover leapfrog.
+ * Replica r owns slots [slot0, slot0 + n); slot0 is a multiple of
+ * RUMDX_BATCH_ALIGN and the range is padded to ceil(n / RUMDX_BATCH_ALIGN)
+ * blocks, so no workgroup spans two replicas and each replica's sums have a
+ * fixed order: a replica's trajectory does not depend on the others. */
+#define RUMDX_BATCH_ALIGN 256
+#define RUMDX_BATCH_FLAGS 8  /* u32 per replica: rebuild, overflow, builds, longest list, internal */
+
+typedef struct rumdx_replica {
+  uint32_t slot0;  /* first slot, a multiple of RUMDX_BATCH_ALIGN */
+  uint32_t n;      /* particles */
+  uint32_t cell0;  /* first cell of its grid in the batch's cell arrays */
+  int32_t nc;      /* cells per axis, rumdx_cell_grid of its box */
+  float L;         /* cubic box length */
+  float inv_L;     /* 1/L rounded to float, as RUMD's box[4] */
+  float omega2;    /* 4 pi^2 / tau^2 of its thermostat; 0 = NVE */
+  uint32_t dof;    /* degrees of freedom, 3n - 3 */
+} rumdx_replica;
+
+typedef struct rumdx_batch {
+  float* r_dev;    /* nslots x 4, layout of rumdx_particles */
+  float* v_dev;
+  float* f_dev;
+  float* w_dev;
+  float* im_dev;
+  uint32_t nslots; /* sum of the padded replica ranges */
+  uint32_t nrep;
+  const rumdx_replica* rep_dev;  /* nrep */
+  const uint32_t* block_rep_dev; /* nslots / RUMDX_BATCH_ALIGN: replica of each slot block */
+  rumdx_nh_state* nh_dev;        /* nrep */
+  uint32_t* rep_flags_dev;       /* nrep x RUMDX_BATCH_FLAGS */
+  uint32_t* nbl_dev;             /* max_nbrs x nslots */
+  uint32_t* num_nbrs_dev;        /* nslots; == max_nbrs marks an overflowed list */
+  float* last_r_dev;             /* nslots x 4 */
+  const float* cut_skin2_dev;    /* ntypes^2: (rc_ij + skin)^2 in FP32 */
+  uint32_t max_nbrs;
+  uint32_t ntypes;
+  float skin;
+  uint32_t ncells;               /* sum over replicas of nc^3 */
+  uint32_t* cell_count_dev;      /* ncells, zero between builds */
+  uint32_t* cell_start_dev;      /* ncells */
+  uint32_t* cell_n_dev;          /* ncells */
+  uint32_t* cell_list_dev;       /* nslots: slots of each cell, ascending */
+  uint32_t* cell_tmp_dev;        /* nslots */
+  uint32_t* slot_cell_dev;       /* nslots: cell of each slot at its replica's last build */
+} rumdx_batch;
+
+/* Bytes of device workspace rumdx_batch_nvt_step / rumdx_batch_energies need. */
+size_t rumdx_batch_workspace_bytes(uint32_t nslots);
+/* Marks every replica's list stale. */
+int rumdx_batch_request_rebuild(rumdx_batch* b, void* stream);
+/* Rebuilds the lists of the replicas whose rebuild flag is set (the skin
+ * check of rumdx_batch_nvt_step sets it); the others are untouched. */
+int rumdx_batch_update_lists(rumdx_batch* b, void* stream);
+/* LJ 12-6 forces, energies (f.w) and virials (w.w) of every replica, one
+ * shared parameter table (rumdx_pair_lj's), stored (initialize semantics).
+ * A particle whose list overflowed rescans its build-time cell stencil, so
+ * forces stay exact; the host regrows max_nbrs at its next sync point. */
+int rumdx_batch_forces(rumdx_batch* b, const float* params_dev, int cutoff_method, void* stream);
+/* One leapfrog step of every replica (rumdx_nvt_step's update, all types
+ * thermostatted when omega2 > 0), its thermostat update, and the skin check
+ * that flags the replica's next rebuild. */
+int rumdx_batch_nvt_step(rumdx_batch* b, float dt, void* workspace_dev, void* stream);
+/* Per replica, out_dev[4r..4r+3] = {sum u_i, sum w_i, leapfrog kinetic
+ * energy (as rumdx_kinetic_energy), sum m v^2}, FP64 in a fixed order. */
+int rumdx_batch_energies(rumdx_batch* b, float dt, double* out_dev, void* workspace_dev, void* stream);
 
 #ifdef __cplusplus
 }

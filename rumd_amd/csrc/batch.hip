@@ -1,0 +1,541 @@
+// Batched replicas (gfx950): hundreds of independent samples of uneven size
+// and temperature advanced by one launch per kernel.
+//
+// Each replica follows the single-sample algorithms: RUMD's cell grid and 5^3
+// stencil for the list (calculateNBL_CellsSorted, src/NeighborList.cu:
+// 1204-1319, with the FP32 criterion of :1239-1240), the skin criterion
+// (displacement_since_nb_build, :970-997), the LJ pair term of fij
+// (include/rumd/PairPotential.h:1198-1244) and the Nose-Hoover leapfrog with
+// its thermostat update (src/IntegratorNVT.cu:64-92, 364-437).
+//
+// Layout: replica r owns slots [slot0, slot0 + n) of the shared float4
+// arrays, slot0 a multiple of RUMDX_BATCH_ALIGN (= the workgroup size), so a
+// workgroup never spans two replicas.  block_rep maps a workgroup to its
+// replica; per-replica sums are block partials added in a fixed order by one
+// workgroup per replica, so every replica's trajectory is independent of the
+// others and reproducible bit for bit.  Particles are not reordered: a
+// replica holds at most a few thousand particles, whose positions sit in L2,
+// so a rebuild only bins slots into cells (a stable counting sort of slot
+// indices) and rescans the stencil.  Lists are u32 slots, column-major over
+// the whole batch (entry k of slot i at k*nslots + i).  Only replicas whose
+// rebuild flag is set run the build kernels; the others exit at once.
+#include "leapfrog.h"
+#include "rumdx_common.h"
+
+namespace rumdx {
+namespace {
+
+static_assert(RUMDX_BATCH_ALIGN == kBlock, "one workgroup per aligned slot block");
+
+enum RepFlag : uint32_t { kRebuild = 0, kOverflow = 1, kBuilds = 2, kMaxLen = 3, kDone = 4 };
+constexpr uint32_t kRepFlags = RUMDX_BATCH_FLAGS;
+
+__device__ __forceinline__ Box rep_box(const rumdx_replica& rp) {
+  Box b;
+  for (int a = 0; a < 3; ++a) {
+    b.L[a] = rp.L;
+    b.iL[a] = rp.inv_L;
+    b.Ld[a] = static_cast<double>(rp.L);
+    b.iLd[a] = 1.0 / b.Ld[a];
+  }
+  return b;
+}
+
+__device__ __forceinline__ uint32_t rep_cell(float4 p, const Box& bx, int nc) {
+  const int cx = cell_coord(p.x, bx.iL[0], nc);
+  const int cy = cell_coord(p.y, bx.iL[1], nc);
+  const int cz = cell_coord(p.z, bx.iL[2], nc);
+  return static_cast<uint32_t>(cx + nc * (cy + nc * cz));
+}
+
+__device__ __forceinline__ bool rebuild_pending(const uint32_t* flags, uint32_t rep) {
+  return __hip_atomic_load(flags + kRepFlags * rep + kRebuild, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
+}
+
+struct BatchArgs {
+  float4* r;
+  float4* v;
+  float4* f;
+  float4* w;
+  float4* im;
+  const rumdx_replica* rep;
+  const uint32_t* block_rep;
+  uint32_t* flags;
+  uint32_t nslots;
+};
+
+BatchArgs args_of(const rumdx_batch* b) {
+  return BatchArgs{reinterpret_cast<float4*>(b->r_dev), reinterpret_cast<float4*>(b->v_dev),
+                   reinterpret_cast<float4*>(b->f_dev), reinterpret_cast<float4*>(b->w_dev),
+                   reinterpret_cast<float4*>(b->im_dev), b->rep_dev, b->block_rep_dev, b->rep_flags_dev,
+                   b->nslots};
+}
+
+// ------------------------------------------------------------------ lists
+// Cell of every slot of a rebuilding replica, and the per-cell counts.
+__global__ __launch_bounds__(kBlock) void k_b_hist(BatchArgs a, uint32_t* __restrict__ slot_cell,
+                                                   uint32_t* __restrict__ cell_count) {
+  const uint32_t rep = a.block_rep[blockIdx.x];
+  if (!rebuild_pending(a.flags, rep)) return;
+  const rumdx_replica rp = a.rep[rep];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i - rp.slot0 >= rp.n) return;
+  const uint32_t c = rp.cell0 + rep_cell(a.r[i], rep_box(rp), rp.nc);
+  slot_cell[i] = c;
+  atomicAdd(cell_count + c, 1u);
+}
+
+// One workgroup per replica: exclusive scan of its cell counts into slot
+// offsets (cell_start), the counts kept in cell_n and zeroed for the scatter.
+__global__ __launch_bounds__(kBlock) void k_b_scan(BatchArgs a, uint32_t* __restrict__ cell_count,
+                                                   uint32_t* __restrict__ cell_start, uint32_t* __restrict__ cell_n) {
+  const uint32_t rep = blockIdx.x;
+  if (!rebuild_pending(a.flags, rep)) return;
+  __shared__ uint32_t s_w[kBlock / 64];
+  const rumdx_replica rp = a.rep[rep];
+  const uint32_t ncell = static_cast<uint32_t>(rp.nc) * rp.nc * rp.nc;
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t carry = rp.slot0;
+  for (uint32_t base = 0; base < ncell; base += kBlock) {
+    const uint32_t c = base + threadIdx.x;
+    const uint32_t v = c < ncell ? cell_count[rp.cell0 + c] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= static_cast<uint32_t>(o)) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t off = carry, tot = 0;
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+      off += k < wave ? s_w[k] : 0u;
+      tot += s_w[k];
+    }
+    if (c < ncell) {
+      cell_start[rp.cell0 + c] = off + x - v;
+      cell_n[rp.cell0 + c] = v;
+      cell_count[rp.cell0 + c] = 0u;
+    }
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_scatter(BatchArgs a, const uint32_t* __restrict__ slot_cell,
+                                                      const uint32_t* __restrict__ cell_start,
+                                                      uint32_t* __restrict__ cell_count, uint32_t* __restrict__ cell_tmp) {
+  const uint32_t rep = a.block_rep[blockIdx.x];
+  if (!rebuild_pending(a.flags, rep)) return;
+  const rumdx_replica rp = a.rep[rep];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i - rp.slot0 >= rp.n) return;
+  const uint32_t c = slot_cell[i];
+  cell_tmp[cell_start[c] + atomicAdd(cell_count + c, 1u)] = i;
+}
+
+// Puts each cell's slots in ascending order (a stable counting sort) and
+// zeroes the counts for the next build.
+__global__ __launch_bounds__(kBlock) void k_b_order(BatchArgs a, const uint32_t* __restrict__ slot_cell,
+                                                    const uint32_t* __restrict__ cell_start,
+                                                    const uint32_t* __restrict__ cell_n,
+                                                    uint32_t* __restrict__ cell_count,
+                                                    const uint32_t* __restrict__ cell_tmp,
+                                                    uint32_t* __restrict__ cell_list) {
+  const uint32_t rep = a.block_rep[blockIdx.x];
+  if (!rebuild_pending(a.flags, rep)) return;
+  const rumdx_replica rp = a.rep[rep];
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k - rp.slot0 >= rp.n) return;
+  const uint32_t x = cell_tmp[k];
+  const uint32_t c = slot_cell[x];
+  const uint32_t b0 = cell_start[c], m = cell_n[c];
+  uint32_t d = b0;
+  for (uint32_t j = b0; j < b0 + m; ++j) d += cell_tmp[j] < x ? 1u : 0u;
+  cell_list[d] = x;
+  cell_count[c] = 0u;
+}
+
+// The lists of a rebuilding replica: RUMD's 5^3 stencil in (dZ, dY, dX)
+// order, ascending slot inside each cell, FP32 minimum-image criterion.  The
+// last workgroup of the replica clears its rebuild request.
+__global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t* __restrict__ slot_cell,
+                                                    const uint32_t* __restrict__ cell_start,
+                                                    const uint32_t* __restrict__ cell_n,
+                                                    const uint32_t* __restrict__ cell_list,
+                                                    const float* __restrict__ cut_skin2, uint32_t nt,
+                                                    uint32_t* __restrict__ nbl, uint32_t* __restrict__ num_nbrs,
+                                                    float4* __restrict__ last_r, uint32_t max_nbrs) {
+  const uint32_t rep = a.block_rep[blockIdx.x];
+  if (!rebuild_pending(a.flags, rep)) return;
+  __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
+  for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_skin2[k];
+  __syncthreads();
+  const rumdx_replica rp = a.rep[rep];
+  uint32_t* fl = a.flags + kRepFlags * rep;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i - rp.slot0 < rp.n) {
+    const Box bx = rep_box(rp);
+    const float4 ri = a.r[i];
+    const float* crow = s_cut + __float_as_uint(ri.w) * nt;
+    const int nc = rp.nc;
+    const uint32_t lc = slot_cell[i] - rp.cell0;
+    const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc), cz = static_cast<int>(lc / (nc * nc));
+    uint32_t cnt = 0;
+    for (int dz = -2; dz <= 2; ++dz) {
+      const int oz = (cz + dz + nc) % nc;
+      for (int dy = -2; dy <= 2; ++dy) {
+        const int oy = (cy + dy + nc) % nc;
+        for (int dx = -2; dx <= 2; ++dx) {
+          const int ox = (cx + dx + nc) % nc;
+          const uint32_t c = rp.cell0 + static_cast<uint32_t>(ox + nc * (oy + nc * oz));
+          const uint32_t b0 = cell_start[c], m = cell_n[c];
+          for (uint32_t k = b0; k < b0 + m; ++k) {
+            const uint32_t j = cell_list[k];
+            if (j == i) continue;
+            const float4 rj = a.r[j];
+            if (min_image_d2_f32(ri, rj, bx) < crow[__float_as_uint(rj.w)]) {
+              if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * a.nslots + i] = j;
+              ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
+            }
+          }
+        }
+      }
+    }
+    num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;
+    last_r[i] = ri;
+    if (cnt >= max_nbrs) fl[kOverflow] = 1u;
+    atomicMax(fl + kMaxLen, cnt);
+  }
+  const uint32_t nblk = (rp.n + kBlock - 1) / kBlock;
+  if (last_to_arrive(fl + kDone, nblk) && threadIdx.x == 0) {
+    fl[kRebuild] = 0u;
+    fl[kBuilds] += 1u;
+  }
+}
+
+// ------------------------------------------------------------------ forces
+// One lane per particle over its list; FP32 pair arithmetic with the
+// minimum-image displacement formed in FP64 (as the single-sample kernels).
+struct Acc {
+  float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f, vir = 0.f;
+};
+
+template <int CUT>
+__device__ __forceinline__ void pair_b(double xi, double yi, double zi, float4 rj, const float4* prow, const Box& bx,
+                                       Acc& a) {
+  double x = xi - static_cast<double>(rj.x);
+  double y = yi - static_cast<double>(rj.y);
+  double z = zi - static_cast<double>(rj.z);
+  x -= bx.Ld[0] * rint(x * bx.iLd[0]);
+  y -= bx.Ld[1] * rint(y * bx.iLd[1]);
+  z -= bx.Ld[2] * rint(z * bx.iLd[2]);
+  const float dx = static_cast<float>(x), dy = static_cast<float>(y), dz = static_cast<float>(z);
+  const float d2 = static_cast<float>(x * x + y * y + z * z);
+  const float4 pa = prow[2 * __float_as_uint(rj.w)];
+  const float4 pb = prow[2 * __float_as_uint(rj.w) + 1];
+  float rinv = 0.f, inv2;
+  if (CUT == RUMDX_CUTOFF_SF) {
+    rinv = __builtin_amdgcn_rsqf(d2);
+    inv2 = rinv * rinv;
+  } else {
+    inv2 = __builtin_amdgcn_rcpf(d2);
+  }
+  const float inv6 = inv2 * inv2 * inv2;
+  float s = (inv6 * inv2) * fmaf(pb.x, inv6, pb.y);
+  float u;
+  if (CUT == RUMDX_CUTOFF_SP) {
+    u = fmaf(inv6, fmaf(pa.y, inv6, pa.z), pb.z);
+  } else if (CUT == RUMDX_CUTOFF_SF) {
+    u = fmaf(inv6, fmaf(pa.y, inv6, pa.z), pb.w);
+    s = fmaf(-pa.w, rinv, s);
+    u = fmaf(pa.w, d2 * rinv, u);
+  } else {
+    u = inv6 * fmaf(pa.y, inv6, pa.z);
+  }
+  const bool in = d2 < pa.x;
+  s = in ? s : 0.f;
+  u = in ? u : 0.f;
+  a.fx += dx * s;
+  a.fy += dy * s;
+  a.fz += dz * s;
+  a.u += u;
+  a.vir += d2 * s;
+}
+
+struct CellArrays {
+  const uint32_t* slot_cell;
+  const uint32_t* cell_start;
+  const uint32_t* cell_n;
+  const uint32_t* cell_list;
+};
+
+template <int CUT>
+__global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t* __restrict__ nbl,
+                                                    const uint32_t* __restrict__ num_nbrs, uint32_t max_nbrs,
+                                                    CellArrays cells, const float* __restrict__ params,
+                                                    uint32_t nt) {
+  extern __shared__ float4 s_tab[];
+  for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) {
+    const float* q = params + static_cast<size_t>(k) * RUMDX_NUM_PARAM;
+    const float rc = q[0], a12 = q[1], a6 = q[2];
+    const float src = q[RUMDX_NUM_PARAM - 2], ucut = q[RUMDX_NUM_PARAM - 1];
+    s_tab[2 * k] = make_float4(rc * rc, a12, a6, src);
+    s_tab[2 * k + 1] = make_float4(12.f * a12, 6.f * a6, -ucut, -fmaf(src, rc, ucut));
+  }
+  __syncthreads();
+  const uint32_t rep = a.block_rep[blockIdx.x];
+  const rumdx_replica rp = a.rep[rep];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i - rp.slot0 >= rp.n) return;
+  const Box bx = rep_box(rp);
+  const float4 ri = a.r[i];
+  const float4* prow = s_tab + 2 * __float_as_uint(ri.w) * nt;
+  const double xi = ri.x, yi = ri.y, zi = ri.z;
+  const uint32_t nn = num_nbrs[i];
+  Acc acc;
+  if (nn >= max_nbrs) {
+    // overflowed list: rescan the build-time stencil (every pair now inside
+    // rc was inside rc + skin then)
+    const int nc = rp.nc;
+    const uint32_t lc = cells.slot_cell[i] - rp.cell0;
+    const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc), cz = static_cast<int>(lc / (nc * nc));
+    for (int dz = -2; dz <= 2; ++dz)
+      for (int dy = -2; dy <= 2; ++dy)
+        for (int dx = -2; dx <= 2; ++dx) {
+          const uint32_t c = rp.cell0 + static_cast<uint32_t>((cx + dx + nc) % nc +
+                                                              nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc)));
+          const uint32_t b0 = cells.cell_start[c], m = cells.cell_n[c];
+          for (uint32_t q = b0; q < b0 + m; ++q) {
+            const uint32_t j = cells.cell_list[q];
+            if (j != i) pair_b<CUT>(xi, yi, zi, a.r[j], prow, bx, acc);
+          }
+        }
+    a.f[i] = make_float4(acc.fx, acc.fy, acc.fz, 0.5f * acc.u);
+    a.w[i] = make_float4(0.f, 0.f, 0.f, acc.vir);
+    return;
+  }
+  const uint32_t* col = nbl + i;
+  uint32_t k = 0;
+  for (; k + 4 <= nn; k += 4) {
+    const uint32_t j0 = col[0], j1 = col[a.nslots], j2 = col[2 * static_cast<size_t>(a.nslots)],
+                   j3 = col[3 * static_cast<size_t>(a.nslots)];
+    col += 4 * static_cast<size_t>(a.nslots);
+    const float4 r0 = a.r[j0], r1 = a.r[j1], r2 = a.r[j2], r3 = a.r[j3];
+    pair_b<CUT>(xi, yi, zi, r0, prow, bx, acc);
+    pair_b<CUT>(xi, yi, zi, r1, prow, bx, acc);
+    pair_b<CUT>(xi, yi, zi, r2, prow, bx, acc);
+    pair_b<CUT>(xi, yi, zi, r3, prow, bx, acc);
+  }
+  for (; k < nn; ++k, col += a.nslots) pair_b<CUT>(xi, yi, zi, a.r[*col], prow, bx, acc);
+  a.f[i] = make_float4(acc.fx, acc.fy, acc.fz, 0.5f * acc.u);
+  a.w[i] = make_float4(0.f, 0.f, 0.f, acc.vir);
+}
+
+// ------------------------------------------------------------------ integrator
+// Leapfrog + Nose-Hoover per replica, the skin check fused in, and one FP64
+// partial of sum m v^2 per workgroup.
+__global__ __launch_bounds__(kBlock) void k_b_nvt(BatchArgs a, const rumdx_nh_state* __restrict__ nh,
+                                                  const float4* __restrict__ last_r, float dt, float skin2_over_4,
+                                                  double* __restrict__ partials) {
+  __shared__ double s_red[kBlock / 64];
+  const uint32_t rep = a.block_rep[blockIdx.x];
+  const rumdx_replica rp = a.rep[rep];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  float ke2 = 0.f;
+  if (i - rp.slot0 < rp.n) {
+    const Box bx = rep_box(rp);
+    const float ps = rp.omega2 > 0.f ? static_cast<float>(nh[rep].ps) : 0.f;
+    float4 r = a.r[i], v = a.v[i], im = a.im[i];
+    float4 lo = make_float4(0.f, 0.f, 0.f, 0.f);
+    ke2 = nvt_particle<false>(r, v, a.f[i], im, lo, ps, dt, bx);
+    a.r[i] = r;
+    a.v[i] = v;
+    a.im[i] = im;
+    if (min_image_d2_f32(r, last_r[i], bx) > skin2_over_4) a.flags[kRepFlags * rep + kRebuild] = 1u;
+  }
+  const double s = block_sum(static_cast<double>(ke2), s_red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// One workgroup per replica: its block partials in order, then
+// Ps += dt omega^2 min(KE2 / (dof T) - 1, 10) in FP64.
+__global__ __launch_bounds__(kBlock) void k_b_nvt_finish(const rumdx_replica* __restrict__ reps,
+                                                         const double* __restrict__ partials,
+                                                         rumdx_nh_state* __restrict__ nh, float dt) {
+  __shared__ double s_red[kBlock / 64];
+  const rumdx_replica rp = reps[blockIdx.x];
+  const uint32_t b0 = rp.slot0 / kBlock, nb = (rp.n + kBlock - 1) / kBlock;
+  const double tot = sum_partials(partials + b0, nb, 1, s_red);
+  if (threadIdx.x == 0) {
+    rumdx_nh_state& s = nh[blockIdx.x];
+    s.ke2_sum = tot;
+    if (rp.omega2 > 0.f) {
+      double dev = tot / (static_cast<double>(rp.dof) * s.target_T) - 1.0;
+      if (dev > 10.0) dev = 10.0;  // runaway clamp (src/IntegratorNVT.cu:432-434)
+      s.ps += static_cast<double>(dt) * static_cast<double>(rp.omega2) * dev;
+      s.target_T += s.dT_per_step;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ energies
+// Per-workgroup partials of {sum u_i, sum w_i, leapfrog kinetic energy
+// (IntegratorNVT::GetKineticEnergy, src/IntegratorNVT.cu:316-356), sum m v^2}.
+__global__ __launch_bounds__(kBlock) void k_b_energy(BatchArgs a, const rumdx_nh_state* __restrict__ nh, float dt,
+                                                     double* __restrict__ partials) {
+  __shared__ double s_red[kBlock / 64];
+  const uint32_t rep = a.block_rep[blockIdx.x];
+  const rumdx_replica rp = a.rep[rep];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double c[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i - rp.slot0 < rp.n) {
+    const float4 f = a.f[i], w = a.w[i], v = a.v[i];
+    const double m = 1.0 / static_cast<double>(v.w);
+    const double ps = rp.omega2 > 0.f ? nh[rep].ps : 0.0;
+    // v(t + h/2) predicted from the current force, as the reference does
+    const double factor = 0.5 * ps * dt, plus = 1.0 / (1.0 + factor), minus = 1.0 - factor;
+    const double vx = plus * (minus * v.x + v.w * f.x * dt), vy = plus * (minus * v.y + v.w * f.y * dt),
+                 vz = plus * (minus * v.z + v.w * f.z * dt);
+    const double vm2 = static_cast<double>(v.x) * v.x + static_cast<double>(v.y) * v.y + static_cast<double>(v.z) * v.z;
+    const double vp2 = vx * vx + vy * vy + vz * vz;
+    c[0] = f.w;
+    c[1] = w.w;
+    c[2] = 0.25 * m * (vm2 + vp2);
+    c[3] = m * vm2;
+  }
+  for (int k = 0; k < 4; ++k) {
+    const double s = block_sum(c[k], s_red);
+    if (threadIdx.x == 0) partials[4 * static_cast<size_t>(blockIdx.x) + k] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_b_energy_finish(const rumdx_replica* __restrict__ reps,
+                                                            const double* __restrict__ partials,
+                                                            double* __restrict__ out) {
+  __shared__ double s_red[kBlock / 64];
+  const rumdx_replica rp = reps[blockIdx.x];
+  const uint32_t b0 = rp.slot0 / kBlock, nb = (rp.n + kBlock - 1) / kBlock;
+  for (int k = 0; k < 4; ++k) {
+    const double s = sum_partials(partials + 4 * static_cast<size_t>(b0) + k, nb, 4, s_red);
+    if (threadIdx.x == 0) out[4 * static_cast<size_t>(blockIdx.x) + k] = s;
+  }
+}
+
+__global__ void k_b_request(uint32_t* flags, uint32_t nrep) {
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < nrep; r += gridDim.x * blockDim.x)
+    flags[kRepFlags * r + kRebuild] = 1u;
+}
+
+int check_batch(const rumdx_batch* b, const char* who) {
+  if (b == nullptr) return fail(RUMDX_E_INVALID, std::string(who) + ": null batch");
+  if (b->r_dev == nullptr || b->v_dev == nullptr || b->f_dev == nullptr || b->w_dev == nullptr ||
+      b->im_dev == nullptr || b->rep_dev == nullptr || b->block_rep_dev == nullptr || b->rep_flags_dev == nullptr)
+    return fail(RUMDX_E_INVALID, std::string(who) + ": batch arrays not allocated");
+  if (b->nslots == 0 || b->nslots % RUMDX_BATCH_ALIGN != 0 || b->nrep == 0)
+    return fail(RUMDX_E_INVALID, std::string(who) + ": nslots must be a positive multiple of RUMDX_BATCH_ALIGN");
+  return RUMDX_OK;
+}
+
+}  // namespace
+}  // namespace rumdx
+
+using namespace rumdx;
+
+extern "C" size_t rumdx_batch_workspace_bytes(uint32_t nslots) {
+  return sizeof(double) * 4 * (static_cast<size_t>(nslots) / RUMDX_BATCH_ALIGN + 1);
+}
+
+extern "C" int rumdx_batch_request_rebuild(rumdx_batch* b, void* stream) {
+  int rc = check_batch(b, "rumdx_batch_request_rebuild");
+  if (rc != RUMDX_OK) return rc;
+  hipLaunchKernelGGL(k_b_request, dim3(blocks_for(b->nrep)), dim3(kBlock), 0, as_stream(stream), b->rep_flags_dev,
+                     b->nrep);
+  return check_launch("rumdx_batch_request_rebuild");
+}
+
+extern "C" int rumdx_batch_update_lists(rumdx_batch* b, void* stream) {
+  int rc = check_batch(b, "rumdx_batch_update_lists");
+  if (rc != RUMDX_OK) return rc;
+  if (b->nbl_dev == nullptr || b->num_nbrs_dev == nullptr || b->last_r_dev == nullptr || b->cut_skin2_dev == nullptr ||
+      b->cell_count_dev == nullptr || b->cell_start_dev == nullptr || b->cell_n_dev == nullptr ||
+      b->cell_list_dev == nullptr || b->cell_tmp_dev == nullptr || b->slot_cell_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_update_lists: list arrays not allocated");
+  if (b->max_nbrs == 0 || b->ntypes == 0 || b->ntypes > RUMDX_MAX_TYPES)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_update_lists: max_nbrs or ntypes out of range");
+  if (static_cast<uint64_t>(b->max_nbrs) * b->nslots >= (1ull << 32))
+    return fail(RUMDX_E_INVALID, "rumdx_batch_update_lists: max_nbrs * nslots exceeds 2^32 entries");
+  const BatchArgs a = args_of(b);
+  const hipStream_t st = as_stream(stream);
+  const uint32_t nblk = b->nslots / kBlock;
+  hipLaunchKernelGGL(k_b_hist, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_count_dev);
+  hipLaunchKernelGGL(k_b_scan, dim3(b->nrep), dim3(kBlock), 0, st, a, b->cell_count_dev, b->cell_start_dev,
+                     b->cell_n_dev);
+  hipLaunchKernelGGL(k_b_scatter, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
+                     b->cell_count_dev, b->cell_tmp_dev);
+  hipLaunchKernelGGL(k_b_order, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
+                     b->cell_n_dev, b->cell_count_dev, b->cell_tmp_dev, b->cell_list_dev);
+  hipLaunchKernelGGL(k_b_build, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
+                     b->cell_n_dev, b->cell_list_dev, b->cut_skin2_dev, b->ntypes, b->nbl_dev, b->num_nbrs_dev,
+                     reinterpret_cast<float4*>(b->last_r_dev), b->max_nbrs);
+  return check_launch("rumdx_batch_update_lists");
+}
+
+extern "C" int rumdx_batch_forces(rumdx_batch* b, const float* params_dev, int cutoff_method, void* stream) {
+  int rc = check_batch(b, "rumdx_batch_forces");
+  if (rc != RUMDX_OK) return rc;
+  if (params_dev == nullptr || b->nbl_dev == nullptr || b->num_nbrs_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_forces: null params or list");
+  if (b->ntypes == 0 || b->ntypes > RUMDX_MAX_TYPES) return fail(RUMDX_E_INVALID, "rumdx_batch_forces: ntypes out of range");
+  if (b->slot_cell_dev == nullptr || b->cell_start_dev == nullptr || b->cell_n_dev == nullptr ||
+      b->cell_list_dev == nullptr || b->max_nbrs == 0)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_forces: cell arrays not allocated");
+  const BatchArgs a = args_of(b);
+  const hipStream_t st = as_stream(stream);
+  const uint32_t nblk = b->nslots / kBlock;
+  const size_t lds = sizeof(float4) * 2 * b->ntypes * b->ntypes;
+  const CellArrays cells{b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev, b->cell_list_dev};
+  switch (cutoff_method) {
+    case RUMDX_CUTOFF_NS:
+      hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_NS>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,
+                         b->max_nbrs, cells, params_dev, b->ntypes);
+      break;
+    case RUMDX_CUTOFF_SP:
+      hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_SP>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,
+                         b->max_nbrs, cells, params_dev, b->ntypes);
+      break;
+    case RUMDX_CUTOFF_SF:
+      hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_SF>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,
+                         b->max_nbrs, cells, params_dev, b->ntypes);
+      break;
+    default: return fail(RUMDX_E_INVALID, "rumdx_batch_forces: invalid cutoff method");
+  }
+  return check_launch("rumdx_batch_forces");
+}
+
+extern "C" int rumdx_batch_nvt_step(rumdx_batch* b, float dt, void* workspace_dev, void* stream) {
+  int rc = check_batch(b, "rumdx_batch_nvt_step");
+  if (rc != RUMDX_OK) return rc;
+  if (workspace_dev == nullptr || b->nh_dev == nullptr || b->last_r_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_nvt_step: null workspace, thermostat state or last_r");
+  const BatchArgs a = args_of(b);
+  const hipStream_t st = as_stream(stream);
+  double* partials = static_cast<double*>(workspace_dev);
+  hipLaunchKernelGGL(k_b_nvt, dim3(b->nslots / kBlock), dim3(kBlock), 0, st, a, b->nh_dev,
+                     reinterpret_cast<const float4*>(b->last_r_dev), dt, 0.25f * b->skin * b->skin, partials);
+  hipLaunchKernelGGL(k_b_nvt_finish, dim3(b->nrep), dim3(kBlock), 0, st, b->rep_dev, partials, b->nh_dev, dt);
+  return check_launch("rumdx_batch_nvt_step");
+}
+
+extern "C" int rumdx_batch_energies(rumdx_batch* b, float dt, double* out_dev, void* workspace_dev, void* stream) {
+  int rc = check_batch(b, "rumdx_batch_energies");
+  if (rc != RUMDX_OK) return rc;
+  if (workspace_dev == nullptr || out_dev == nullptr || b->nh_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_energies: null workspace, output or thermostat state");
+  const BatchArgs a = args_of(b);
+  const hipStream_t st = as_stream(stream);
+  double* partials = static_cast<double*>(workspace_dev);
+  hipLaunchKernelGGL(k_b_energy, dim3(b->nslots / kBlock), dim3(kBlock), 0, st, a, b->nh_dev, dt, partials);
+  hipLaunchKernelGGL(k_b_energy_finish, dim3(b->nrep), dim3(kBlock), 0, st, b->rep_dev, partials, out_dev);
+  return check_launch("rumdx_batch_energies");
+}

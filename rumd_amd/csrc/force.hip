@@ -279,35 +279,26 @@ __device__ __forceinline__ float4 halo_pos(const float4* s_pos, uint32_t entry) 
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pos) + entry);
 }
 
-// Tile force kernel for lists of the sorted method: one workgroup per cell
-// tile stages the tile's halo positions in LDS; list entries are LDS byte
-// offsets, so neighbour positions come from LDS.  Waves take the tile's
-// particles 64 at a time in the build's lane order (descending list length,
-// so a wave's lists have similar lengths); the last, partial chunk spreads
-// each list over 2^lg lanes (groups part, part + 2^lg, ...), combined by a
-// butterfly.  A tile whose halo does not wrap uses the direct FP32
-// difference; a wrapping tile forms displacements in FP64.  Overflowed lists,
-// and tiles whose halo exceeds the LDS capacity, rescan the particle's
+// Forces on the particles of one tile whose halo positions are in LDS.
+// Waves take the tile's particles 64 at a time in the build's lane order
+// (descending list length, so a wave's lists have similar lengths); the last,
+// partial chunk spreads each list over 2^lg lanes (groups part, part + 2^lg,
+// ...), combined by a butterfly.  FAST: the halo was staged in the shifted
+// frame and every list is complete, so every pair takes the direct FP32
+// difference.  Otherwise a wrapping halo forms displacements in FP64, and
+// overflowed lists, or a halo beyond the LDS capacity, rescan the particle's
 // build-time cell stencil (positions from LDS or global memory).
-template <int CUT, bool STR, bool INIT>
-__global__ __launch_bounds__(kBlock) void k_lj_tiles(const float4* __restrict__ r, float4* __restrict__ f,
-                                                     float4* __restrict__ w, float4* __restrict__ sts,
-                                                     const uint16_t* __restrict__ nbl16,
-                                                     const uint32_t* __restrict__ num_nbrs,
-                                                     const uint32_t* __restrict__ slot_cell,
-                                                     const uint16_t* __restrict__ lane_order,
-                                                     const float* __restrict__ params, uint32_t nt, Box bx, TileGeom tg,
-                                                     uint32_t nvp, uint32_t max_nbrs) {
-  __shared__ HaloLds L;
-  extern __shared__ float4 s_dyn[];
-  float4* s_pos = s_dyn;
-  float4* s_tab = s_dyn + tg.halo_cap;
-  load_table(s_tab, params, nt);
-  const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
-  stage_halo(t, tg, r, s_pos, L);
-  const Halo& h = L.h;
-  const bool staged = h.npart <= tg.halo_cap;
-  const uint32_t np = h.slot1 - h.slot0;
+template <int CUT, bool STR, bool INIT, bool FAST>
+__device__ __forceinline__ void tile_particles(uint32_t slot0, uint32_t slot1, bool staged, bool wraps,
+                                               const uint32_t shifts, const float4* __restrict__ r,
+                                               float4* __restrict__ f, float4* __restrict__ w,
+                                               float4* __restrict__ sts, const uint16_t* __restrict__ nbl16,
+                                               const uint32_t* __restrict__ num_nbrs,
+                                               const uint32_t* __restrict__ slot_cell,
+                                               const uint16_t* __restrict__ lane_order, const float4* s_pos,
+                                               const float4* s_tab, uint32_t nt, const Box& bx, const TileGeom& tg,
+                                               const HaloLds& L, uint32_t nvp, uint32_t max_nbrs) {
+  const uint32_t np = slot1 - slot0;
   const uint32_t lane = threadIdx.x & 63u;
   for (uint32_t c0 = (threadIdx.x >> 6) * 64u; c0 < np; c0 += kBlock) {
     const uint32_t cnt = (np - c0) < 64u ? (np - c0) : 64u;
@@ -318,46 +309,49 @@ __global__ __launch_bounds__(kBlock) void k_lj_tiles(const float4* __restrict__ 
     Acc a;
     uint32_t i = 0;
     if (live) {
-      i = h.slot0 + lane_order[h.slot0 + c0 + pl];
-      const float4 ri = r[i];
+      i = slot0 + lane_order[slot0 + c0 + pl];
+      float4 ri = r[i];
       const float4* prow = s_tab + 2 * __float_as_uint(ri.w) * nt;
       const uint32_t nn = num_nbrs[i];
-      if (staged && nn < max_nbrs) {
-        const uint32_t nfull = nn >> 3, rem = nn & 7u;
-        const uint32_t step = 1u << lg;
-        const size_t gstride = static_cast<size_t>(nvp) << lg;
-        const u32x4* grp = reinterpret_cast<const u32x4*>(nbl16) + static_cast<size_t>(part) * nvp + i;
-        const bool tail = rem != 0u && (nfull & (step - 1u)) == part;  // this lane owns group nfull
-        if (!h.wraps) {
-          for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
-            const u32x4 q = __builtin_nontemporal_load(grp);
-#pragma unroll
-            for (int e = 0; e < 8; ++e) pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
-          }
-          if (tail) {
-            const u32x4 q = __builtin_nontemporal_load(grp);
-#pragma unroll
-            for (int e = 0; e < 7; ++e)
-              if (static_cast<uint32_t>(e) < rem)
-                pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
-          }
-        } else {
-          const double xi = ri.x, yi = ri.y, zi = ri.z;
-          for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
-            const u32x4 q = __builtin_nontemporal_load(grp);
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-              pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
-          }
-          if (tail) {
-            const u32x4 q = __builtin_nontemporal_load(grp);
-#pragma unroll
-            for (int e = 0; e < 7; ++e)
-              if (static_cast<uint32_t>(e) < rem)
-                pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
-          }
+      const uint32_t nfull = nn >> 3, rem = nn & 7u;
+      const uint32_t step = 1u << lg;
+      const size_t gstride = static_cast<size_t>(nvp) << lg;
+      const u32x4* grp = reinterpret_cast<const u32x4*>(nbl16) + static_cast<size_t>(part) * nvp + i;
+      const bool tail = rem != 0u && (nfull & (step - 1u)) == part;  // this lane owns group nfull
+      if (FAST || (staged && nn < max_nbrs && !wraps)) {
+        if (FAST) {
+          ri.x = shift_coord(ri.x, shifts, 0, bx.L[0]);
+          ri.y = shift_coord(ri.y, shifts, 1, bx.L[1]);
+          ri.z = shift_coord(ri.z, shifts, 2, bx.L[2]);
         }
-      } else if (part == 0) {
+        for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
+          const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
+        }
+        if (tail) {
+          const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+          for (int e = 0; e < 7; ++e)
+            if (static_cast<uint32_t>(e) < rem) pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
+        }
+      } else if (!FAST && staged && nn < max_nbrs) {
+        const double xi = ri.x, yi = ri.y, zi = ri.z;
+        for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
+          const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+          for (int e = 0; e < 8; ++e)
+            pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
+        }
+        if (tail) {
+          const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+          for (int e = 0; e < 7; ++e)
+            if (static_cast<uint32_t>(e) < rem)
+              pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
+        }
+      } else if (!FAST && part == 0) {
+        const Halo& h = L.h;
         const double xi = ri.x, yi = ri.y, zi = ri.z;
         const uint32_t lin = slot_cell[i];
         const int cx = static_cast<int>(lin % tg.nc0);
@@ -382,6 +376,48 @@ __global__ __launch_bounds__(kBlock) void k_lj_tiles(const float4* __restrict__ 
   }
 }
 
+// Tile force kernel for lists of the sorted method: one workgroup per cell
+// tile stages the tile's halo positions in LDS; list entries are LDS byte
+// offsets, so neighbour positions come from LDS.  Tiles the build marked
+// fast stage their halo from the build's slot list in the shifted frame (one
+// coalesced pass, no table); the others rebuild the halo table from the cell
+// arrays and may need FP64 displacements or the stencil rescan.
+template <int CUT, bool STR, bool INIT>
+__global__ __launch_bounds__(kBlock) void k_lj_tiles(const float4* __restrict__ r, float4* __restrict__ f,
+                                                     float4* __restrict__ w, float4* __restrict__ sts,
+                                                     const uint16_t* __restrict__ nbl16,
+                                                     const uint32_t* __restrict__ num_nbrs,
+                                                     const uint32_t* __restrict__ slot_cell,
+                                                     const uint16_t* __restrict__ lane_order,
+                                                     const float* __restrict__ params, uint32_t nt, Box bx, TileGeom tg,
+                                                     uint32_t nvp, uint32_t max_nbrs) {
+  __shared__ HaloLds L;
+  extern __shared__ float4 s_dyn[];
+  float4* s_pos = s_dyn;
+  float4* s_tab = s_dyn + tg.halo_cap;
+  const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
+  const uint4 info = tg.tile_info[t];
+  load_table(s_tab, params, nt);
+  if ((info.y & kTileFast) != 0u && (info.y >> 16) == tg.halo_cap && info.x <= tg.halo_cap) {
+    const uint32_t* hs = tg.halo_slots + static_cast<size_t>(t) * tg.halo_cap;
+    for (uint32_t q = threadIdx.x; q < info.x; q += kBlock) {
+      float4 p = r[hs[q]];
+      p.x = shift_coord(p.x, info.y, 0, bx.L[0]);
+      p.y = shift_coord(p.y, info.y, 1, bx.L[1]);
+      p.z = shift_coord(p.z, info.y, 2, bx.L[2]);
+      s_pos[q] = p;
+    }
+    __syncthreads();
+    tile_particles<CUT, STR, INIT, true>(info.z, info.w, true, false, info.y, r, f, w, sts, nbl16, num_nbrs,
+                                         slot_cell, lane_order, s_pos, s_tab, nt, bx, tg, L, nvp, max_nbrs);
+    return;
+  }
+  stage_halo(t, tg, r, s_pos, L);
+  const Halo& h = L.h;
+  tile_particles<CUT, STR, INIT, false>(h.slot0, h.slot1, h.npart <= tg.halo_cap, h.wraps, 0u, r, f, w, sts, nbl16,
+                                        num_nbrs, slot_cell, lane_order, s_pos, s_tab, nt, bx, tg, L, nvp, max_nbrs);
+}
+
 template <int CUT, bool STR, bool INIT>
 int launch_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* params, uint32_t nt, const Box& bx,
               hipStream_t st) {
@@ -392,8 +428,12 @@ int launch_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* par
   float4* sts = reinterpret_cast<float4*>(p->sts_dev);
   const size_t tab_bytes = sizeof(float4) * 2 * nt * nt;
   if (nb != nullptr && nb->nbl16_dev != nullptr) {
-    const TileGeom tg{nb->nc[0], nb->nc[1], nb->nc[2], nb->cell_rank_dev, nb->cell_start_dev,
-                      nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, nb->halo_cap};
+    const TileGeom tg{nb->nc[0],           nb->nc[1],
+                      nb->nc[2],           nb->cell_rank_dev,
+                      nb->cell_start_dev,  nb->tile_first_rank_dev,
+                      nb->tile_origin_dev, nb->ntiles,
+                      nb->halo_cap,        nb->tile_halo_dev,
+                      reinterpret_cast<uint4*>(nb->tile_info_dev)};
     hipLaunchKernelGGL((k_lj_tiles<CUT, STR, INIT>), dim3(nb->ntiles), dim3(kBlock),
                        tab_bytes + sizeof(float4) * nb->halo_cap, st, r, f, w, sts, nb->nbl16_dev, nb->num_nbrs_dev,
                        nb->slot_cell_dev, nb->lane_order_dev, params, nt, bx, tg, p->nvp, nb->max_nbrs);
@@ -443,7 +483,8 @@ extern "C" int rumdx_pair_lj(const rumdx_particles* p, const rumdx_nblist* nb, c
     return fail(RUMDX_E_INVALID, "rumdx_pair_lj: neighbour list not allocated");
   if (nb != nullptr && nb->nbl16_dev != nullptr &&
       (nb->ntiles == 0 || nb->tile_first_rank_dev == nullptr || nb->tile_origin_dev == nullptr ||
-       nb->slot_cell_dev == nullptr || nb->lane_order_dev == nullptr || nb->halo_cap > RUMDX_HALO_CAP_MAX))
+       nb->slot_cell_dev == nullptr || nb->lane_order_dev == nullptr || nb->tile_halo_dev == nullptr ||
+       nb->tile_info_dev == nullptr || nb->halo_cap > RUMDX_HALO_CAP_MAX))
     return fail(RUMDX_E_INVALID, "rumdx_pair_lj: tile arrays not allocated");
   const Box bx = make_box(box);
   const hipStream_t st = as_stream(stream);

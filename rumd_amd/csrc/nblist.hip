@@ -522,6 +522,15 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   const bool staged = h.npart <= tg.halo_cap;
   const bool bytes = h.npart <= kByteEntryHalo;
   const uint32_t np = h.slot1 - h.slot0;
+  // the halo in halo order, for the force kernel's staging
+  if (staged) {
+    uint32_t* hs = tg.halo_slots + static_cast<size_t>(t) * tg.halo_cap;
+    for (uint32_t c = threadIdx.x; c < h.ncell; c += kBlock) {
+      const uint32_t o0 = L.off[c], cnt = L.off[c + 1] - o0, g0 = L.start[c];
+      for (uint32_t k = 0; k < cnt; ++k) hs[o0 + k] = g0 + k;
+    }
+  }
+  bool any_overflow = false;
   // wave-cooperative lists for staged tiles; one lane per particle otherwise
   const bool coop = staged && np <= kOrderMax;
   if (coop) {
@@ -560,7 +569,10 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
     const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
     num_nbrs[i] = stored;  // == max_nbrs marks an overflowed list
     if (np <= kOrderMax) s_key[i - h.slot0] = ((0xffffu - min(stored, 0xffffu)) << 16) | (i - h.slot0);
-    if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
+    if (cnt >= max_nbrs) {
+      flags[RUMDX_FLAG_OVERFLOW] = 1u;
+      any_overflow = true;
+    }
     atomicMax(flags + RUMDX_FLAG_MAXLEN, cnt);
     o.slot_cell[i] = static_cast<uint32_t>(cx + cy * tg.nc0 + cz * tg.nc0 * tg.nc1);
     o.r[i] = ri;
@@ -574,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   }
   // Lane order of the force kernel: the tile's particles by descending list
   // length, ties by slot (a rank count over unique keys; deterministic).
-  __syncthreads();
+  const bool overflowed = __syncthreads_or(any_overflow ? 1 : 0) != 0;
   for (uint32_t k = threadIdx.x; k < np; k += kBlock) {
     uint32_t rk = k;
     if (np <= kOrderMax) {
@@ -587,6 +599,19 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   if (threadIdx.x == 0) {
     atomicMax(flags + RUMDX_FLAG_MAXHALO, h.npart);
     if (h.npart > 65535u) flags[RUMDX_FLAG_ERROR] = 1u;
+    // wrapping axes, and whether the shifted frame serves them
+    const int nc[3] = {tg.nc0, tg.nc1, tg.nc2};
+    uint32_t fl = 0;
+    bool shiftable = true;
+    for (int a = 0; a < 3; ++a) {
+      const bool below = h.base[a] < 0, above = h.base[a] + h.ext[a] > nc[a];
+      if (below) fl |= kTileShiftDown << (2 * a);
+      if (above) fl |= kTileShiftUp << (2 * a);
+      shiftable = shiftable && ((!below && !above) || (nc[a] >= kShiftMinCells && h.ext[a] < nc[a] &&
+                                                       !(below && above) && shift_exact(bx.L[a], nc[a])));
+    }
+    if (staged && !overflowed && shiftable) fl |= kTileFast;
+    tg.tile_info[t] = make_uint4(h.npart, fl | (tg.halo_cap << 16), h.slot0, h.slot1);
   }
   finish_build(flags);
 }
@@ -720,7 +745,7 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
   if (box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: null box");
   if (nb->cell_rank_dev == nullptr || nb->cell_start_dev == nullptr || nb->cell_count_dev == nullptr ||
       nb->slot_cell_dev == nullptr || nb->sort_scratch_dev == nullptr || nb->nbl16_dev == nullptr ||
-      nb->lane_order_dev == nullptr ||
+      nb->lane_order_dev == nullptr || nb->tile_halo_dev == nullptr || nb->tile_info_dev == nullptr ||
       nb->tile_first_rank_dev == nullptr || nb->tile_origin_dev == nullptr || nb->ntiles == 0)
     return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: cell or tile arrays not allocated");
   if (nb->halo_cap > RUMDX_HALO_CAP_MAX) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: halo_cap too large");
@@ -756,8 +781,12 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
                      reinterpret_cast<const float4*>(p->f_dev), reinterpret_cast<const float4*>(p->w_dev),
                      reinterpret_cast<const float4*>(p->im_dev), reinterpret_cast<const float4*>(p->rlo_dev),
                      p->unsorted_index_dev, s, p->n);
-  const TileGeom tg{nb->nc[0], nb->nc[1], nb->nc[2], nb->cell_rank_dev, nb->cell_start_dev,
-                    nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, nb->halo_cap};
+  const TileGeom tg{nb->nc[0],           nb->nc[1],
+                    nb->nc[2],           nb->cell_rank_dev,
+                    nb->cell_start_dev,  nb->tile_first_rank_dev,
+                    nb->tile_origin_dev, nb->ntiles,
+                    nb->halo_cap,        nb->tile_halo_dev,
+                    reinterpret_cast<uint4*>(nb->tile_info_dev)};
   const HomeArrays home{reinterpret_cast<float4*>(p->r_dev), reinterpret_cast<float4*>(p->v_dev),
                         reinterpret_cast<float4*>(p->f_dev), reinterpret_cast<float4*>(p->w_dev),
                         reinterpret_cast<float4*>(p->im_dev), reinterpret_cast<float4*>(p->rlo_dev),
@@ -821,7 +850,7 @@ extern "C" int rumdx_nblist_export(const rumdx_particles* p, const rumdx_nblist*
         nb->cell_start_dev == nullptr)
       return fail(RUMDX_E_INVALID, "rumdx_nblist_export: tile arrays not allocated");
     const TileGeom tg{nb->nc[0], nb->nc[1], nb->nc[2], nb->cell_rank_dev, nb->cell_start_dev,
-                      nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, 0u};
+                      nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, 0u, nullptr, nullptr};
     hipLaunchKernelGGL(k_export_tiles, dim3(nb->ntiles), dim3(kBlock), 0, st, tg, nb->nbl16_dev, nb->num_nbrs_dev,
                        out_dev, p->nvp);
   } else {

@@ -27,7 +27,41 @@ struct TileGeom {
   const uint32_t* origin;      // tile -> cx0 | cy0 << 10 | cz0 << 20
   uint32_t ntiles;
   uint32_t halo_cap;           // particles the LDS position buffer holds
+  uint32_t* halo_slots;        // tile t's halo slots at [t * halo_cap, ...) (build output)
+  uint4* tile_info;            // per tile: {halo particles, kTileFast | shifts | cap << 16, slot0, slot1}
 };
+
+// tile_info flags.  kTileFast: the build staged the halo (npart <= halo_cap),
+// no list of the tile overflowed, and every axis on which the halo wraps
+// allows the shifted frame below.  Per axis a, (kTileShiftDown << 2a): the
+// halo wraps below the tile (its far cells sit at x >= 0, near +L/2) and
+// x >= 0 becomes x - L; (kTileShiftUp << 2a): it wraps above and x < 0
+// becomes x + L.  Positions are in [-L/2, L/2).
+constexpr uint32_t kTileFast = 1u;
+constexpr uint32_t kTileShiftDown = 2u;
+constexpr uint32_t kTileShiftUp = 4u;
+
+// The shifted frame puts a wrapping halo on one side of the periodic
+// boundary, so plain FP32 differences there are the correctly rounded
+// minimum-image displacements.  It needs at least kShiftMinCells cells on the
+// axis (the tile's own side of the halo then stays clear of x = 0) and an
+// exact shift: the moved coordinates lie within 2 cells of +-L/2 and move to
+// within 2 cells beyond it, and x -+ L is exact when all of
+// [L/2 - 2a, L/2 + 2a] lies in one binade (x is a multiple of that binade's
+// ulp and L of twice it).
+constexpr int kShiftMinCells = 16;
+
+__device__ __forceinline__ bool shift_exact(float len, int nc) {
+  const float half = 0.5f * len, reach = 2.05f * len / static_cast<float>(nc);
+  return ilogbf(half - reach) == ilogbf(half + reach);
+}
+
+__device__ __forceinline__ float shift_coord(float x, uint32_t fl, int axis, float len) {
+  const uint32_t f = fl >> (2 * axis);
+  if ((f & kTileShiftDown) && x >= 0.f) return x - len;
+  if ((f & kTileShiftUp) && x < 0.f) return x + len;
+  return x;
+}
 
 // Halo of one tile, in LDS.
 struct Halo {

@@ -110,6 +110,8 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetPotentialEnergy", &PairPotential::GetPotentialEnergy)
       .def("GetActualMaxNumNbrs", &PairPotential::GetActualMaxNumNbrs)
       .def("GetCutoffMethod", &PairPotential::GetCutoffMethod)
+      .def("SetFP64Forces", &PairPotential::SetFP64Forces, py::arg("on"))
+      .def("GetFP64Forces", &PairPotential::GetFP64Forces)
       .def("TimeForceKernel", &PairPotential::TimeForceKernel, py::arg("reps"),
            py::call_guard<py::gil_scoped_release>())
       .def("GetNeighborListStats", &PairPotential::GetNeighborListStats)

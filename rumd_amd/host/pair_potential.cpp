@@ -173,11 +173,13 @@ void NeighborList::Allocate() {
     tile_origin_.resize(origin.size());
     tile_origin_.upload(origin.data(), origin.size(), st);
     if (halo_cap_ == 0) halo_cap_ = rumdx_halo_cap_estimate(nc, n);
+    tile_info_.resize(4ull * ntiles);
     desc_.nbl16_dev = nbl16_.get();
     desc_.ntiles = static_cast<uint32_t>(ntiles);
     desc_.tile_first_rank_dev = tile_first_rank_.get();
     desc_.tile_origin_dev = tile_origin_.get();
-    desc_.halo_cap = halo_cap_;
+    desc_.tile_info_dev = tile_info_.get();
+    SizeTileHalo();
   } else {
     cell_rank_.release();
     cell_start_.release();
@@ -187,6 +189,8 @@ void NeighborList::Allocate() {
     sort_scratch_.release();
     tile_first_rank_.release();
     tile_origin_.release();
+    tile_halo_.release();
+    tile_info_.release();
   }
   allocated_nvp_ = nvp;
   allocated_ = true;
@@ -226,7 +230,7 @@ bool NeighborList::HandleOverflow() {
     // tiles beyond the LDS capacity computed exactly from global memory; give
     // the next launches room for them
     halo_cap_ = std::min<uint32_t>(RUMDX_HALO_CAP_MAX, ((f[RUMDX_FLAG_MAXHALO] + f[RUMDX_FLAG_MAXHALO] / 8) + 63u) & ~63u);
-    desc_.halo_cap = halo_cap_;
+    SizeTileHalo();
     sample_->InvalidateGraphs();
     if (!f[RUMDX_FLAG_OVERFLOW]) return true;
   }
@@ -248,8 +252,18 @@ bool NeighborList::HandleOverflow() {
 void NeighborList::SetHaloCapacity(unsigned cap) {
   if (cap > RUMDX_HALO_CAP_MAX) throw RUMD_Error("NeighborList", "SetHaloCapacity", "capacity above RUMDX_HALO_CAP_MAX");
   halo_cap_ = cap;
+  if (allocated_ && desc_.nbl16_dev != nullptr) SizeTileHalo();
   desc_.halo_cap = cap;
   if (sample_) sample_->InvalidateGraphs();
+}
+
+// The per-tile halo slot lists are laid out by the halo capacity: resize them
+// and rebuild before the next force call (the build writes them).
+void NeighborList::SizeTileHalo() {
+  tile_halo_.resize(static_cast<size_t>(desc_.ntiles) * std::max(halo_cap_, 1u));
+  desc_.tile_halo_dev = tile_halo_.get();
+  desc_.halo_cap = halo_cap_;
+  if (allocated_) rumdx_check(rumdx_nblist_request_rebuild(&desc_, sample_->GetStream()), "NeighborList", "SizeTileHalo");
 }
 
 std::vector<uint32_t> NeighborList::ExportList(std::vector<uint32_t>& counts) {
@@ -346,15 +360,29 @@ void PairPotential::Initialize() {
   neighborList.SetCutoffs(this);
 }
 
+// The force kernel of this potential: FP32 pair arithmetic, or the all-FP64
+// comparison path after SetFP64Forces(true).
+void PairPotential::LaunchForce(const rumdx_particles* p, bool initialize, bool calc_stresses, hipStream_t st,
+                                const char* method) {
+  rumdx_nblist* nb = neighborList.GetResolvedMethod() == NeighborList::NONE ? nullptr : neighborList.Descriptor();
+  const float* box = sample->GetSimulationBox()->GetBox8();
+  const int rc = fp64_forces_ ? rumdx_pair_lj_fp64(p, nb, d_params_.get(), table_types_, box, cutoff_method_,
+                                                   initialize ? 1 : 0, calc_stresses ? 1 : 0, st)
+                              : rumdx_pair_lj(p, nb, d_params_.get(), table_types_, box, cutoff_method_,
+                                              initialize ? 1 : 0, calc_stresses ? 1 : 0, st);
+  rumdx_check(rc, GetID_String().c_str(), method);
+}
+
+void PairPotential::SetFP64Forces(bool on) {
+  fp64_forces_ = on;
+  if (sample) sample->InvalidateGraphs();
+}
+
 void PairPotential::CalcF(bool initialize, bool calc_stresses) {
   if (!sample) throw RUMD_Error(GetID_String(), "CalcF", "no sample");
   if (table_types_ != sample->GetNumberOfTypes()) Initialize();
   neighborList.UpdateNBlist();
-  rumdx_nblist* nb = neighborList.GetResolvedMethod() == NeighborList::NONE ? nullptr : neighborList.Descriptor();
-  rumdx_check(rumdx_pair_lj(sample->GetParticleData().Descriptor(), nb, d_params_.get(), table_types_,
-                            sample->GetSimulationBox()->GetBox8(), cutoff_method_, initialize ? 1 : 0,
-                            calc_stresses ? 1 : 0, sample->GetStream()),
-              GetID_String().c_str(), "CalcF");
+  LaunchForce(sample->GetParticleData().Descriptor(), initialize, calc_stresses, sample->GetStream(), "CalcF");
 }
 
 // PairPotential::GetPotentialEnergy (src/PairPotential.cu:252-267): this
@@ -371,11 +399,8 @@ double PairPotential::GetPotentialEnergy() {
   rumdx_particles p = *sample->GetParticleData().Descriptor();
   p.f_dev = pe_f_.get();
   p.w_dev = pe_w_.get();
-  rumdx_nblist* nb = neighborList.GetResolvedMethod() == NeighborList::NONE ? nullptr : neighborList.Descriptor();
   const hipStream_t st = sample->GetStream();
-  rumdx_check(rumdx_pair_lj(&p, nb, d_params_.get(), table_types_, sample->GetSimulationBox()->GetBox8(),
-                            cutoff_method_, 1, 0, st),
-              GetID_String().c_str(), "GetPotentialEnergy");
+  LaunchForce(&p, true, false, st, "GetPotentialEnergy");
   rumdx_check(rumdx_reduce_f4(pe_f_.get(), p.n, 8u, sample->ScalarOut(), sample->Workspace(), st),
               GetID_String().c_str(), "GetPotentialEnergy");
   double out[4];
@@ -389,16 +414,13 @@ float PairPotential::TimeForceKernel(unsigned reps) {
   if (!sample) throw RUMD_Error(GetID_String(), "TimeForceKernel", "no sample");
   if (reps == 0) reps = 1;
   neighborList.UpdateNBlist();
-  rumdx_nblist* nb = neighborList.GetResolvedMethod() == NeighborList::NONE ? nullptr : neighborList.Descriptor();
   const hipStream_t st = sample->GetStream();
   hipEvent_t a, b;
   hip_check(hipEventCreate(&a), "PairPotential", "TimeForceKernel");
   hip_check(hipEventCreate(&b), "PairPotential", "TimeForceKernel");
   hip_check(hipEventRecord(a, st), "PairPotential", "TimeForceKernel");
   for (unsigned k = 0; k < reps; ++k)
-    rumdx_check(rumdx_pair_lj(sample->GetParticleData().Descriptor(), nb, d_params_.get(), table_types_,
-                              sample->GetSimulationBox()->GetBox8(), cutoff_method_, 1, 0, st),
-                GetID_String().c_str(), "TimeForceKernel");
+    LaunchForce(sample->GetParticleData().Descriptor(), true, false, st, "TimeForceKernel");
   hip_check(hipEventRecord(b, st), "PairPotential", "TimeForceKernel");
   hip_check(hipEventSynchronize(b), "PairPotential", "TimeForceKernel");
   float ms = 0.f;

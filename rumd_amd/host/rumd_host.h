@@ -179,6 +179,7 @@ class NeighborList {
   // LDS halo capacity of the tile kernels, in particles (0 = estimate from
   // the density); tiles with larger halos compute from global memory.
   void SetHaloCapacity(unsigned cap);
+  void SizeTileHalo();
   unsigned GetHaloCapacity() const { return halo_cap_; }
   // The list in RUMD's format (u32 slots, column-major), for checks and tools.
   std::vector<uint32_t> ExportList(std::vector<uint32_t>& counts);
@@ -205,6 +206,8 @@ class NeighborList {
   DeviceArray<float> last_r_, cut_skin2_;
   DeviceArray<char> sort_scratch_;
   DeviceArray<uint16_t> lane_order_;
+  DeviceArray<uint32_t> tile_halo_;  // ntiles x halo_cap
+  DeviceArray<uint32_t> tile_info_;  // ntiles x 4
   rumdx_nblist desc_{};
   uint32_t allocated_nvp_ = 0;
 };
@@ -239,6 +242,10 @@ class PairPotential : public Potential {
   // the sample's stream, bracketed by HIP events (the list is brought up to
   // date first and not rebuilt in between).
   float TimeForceKernel(unsigned reps);
+  // All-FP64 pair arithmetic (rumdx_pair_lj_fp64): the precision study's
+  // comparison path; FP32 pair arithmetic otherwise.
+  void SetFP64Forces(bool on);
+  bool GetFP64Forces() const { return fp64_forces_; }
   // Mean and maximum current list length (synchronises).
   std::vector<double> GetNeighborListStats();
   NeighborList& GetNeighborList() { return neighborList; }
@@ -251,6 +258,8 @@ class PairPotential : public Potential {
   NeighborList neighborList;
   CutoffMethod cutoff_method_;
   bool assume_newton3_ = true;
+  bool fp64_forces_ = false;
+  void LaunchForce(const rumdx_particles* p, bool initialize, bool calc_stresses, hipStream_t st, const char* method);
   std::map<std::pair<unsigned, unsigned>, std::vector<float>> params_map_;
   DeviceArray<float> d_params_;
   unsigned table_types_ = 0;

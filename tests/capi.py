@@ -56,6 +56,8 @@ class NbList(C.Structure):
         ("tile_origin_dev", C.c_void_p),
         ("halo_cap", C.c_uint32),
         ("lane_order_dev", C.c_void_p),
+        ("tile_halo_dev", C.c_void_p),
+        ("tile_info_dev", C.c_void_p),
     ]
 
 
@@ -97,6 +99,7 @@ def lib() -> C.CDLL:
     L.rumdx_nblist_build_n2.argtypes = [PP, NP, f32a, vp]
     L.rumdx_nblist_build_sorted.argtypes = [PP, NP, f32a, vp]
     L.rumdx_pair_lj.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
+    L.rumdx_pair_lj_fp64.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
     L.rumdx_nvt_step.argtypes = [PP, NHP, f32, u32, f32, u32, f32a, NP, vp, vp]
     L.rumdx_zero_momentum.argtypes = [PP, vp, vp]
     L.rumdx_kinetic_energy.argtypes = [PP, NHP, f32, u32, vp, vp, vp]
@@ -286,6 +289,9 @@ class GpuSystem:
             if cap is None:
                 cap = lib().rumdx_halo_cap_estimate(nc, self.n)
             nb.nbl16_dev = self.nbl16.value
+            self.tile_halo = DeviceBuffer(4 * ntiles * max(cap, 1))
+            self.tile_info = DeviceBuffer(16 * ntiles)
+            nb.tile_halo_dev, nb.tile_info_dev = self.tile_halo.value, self.tile_info.value
             nb.ntiles = ntiles
             nb.tile_first_rank_dev, nb.tile_origin_dev = self.tile_first.value, self.tile_origin.value
             nb.halo_cap = cap
@@ -301,12 +307,13 @@ class GpuSystem:
         else:
             check(L.rumdx_nblist_build_n2(C.byref(self.p), C.byref(self.nb), self.box8, None))
 
-    def calc_f(self, cutoff, initialize=True, stress=False):
+    def calc_f(self, cutoff, initialize=True, stress=False, fp64=False):
         if self.nb is not None:
             self.update_nblist()
         nbp = C.byref(self.nb) if self.nb is not None else None
-        check(lib().rumdx_pair_lj(C.byref(self.p), nbp, self.d_params.value, self.nt, self.box8, cutoff,
-                                  int(initialize), int(stress), None))
+        fn = lib().rumdx_pair_lj_fp64 if fp64 else lib().rumdx_pair_lj
+        check(fn(C.byref(self.p), nbp, self.d_params.value, self.nt, self.box8, cutoff, int(initialize), int(stress),
+                 None))
 
     def set_nh(self, ps=0.0, target_T=1.0, dT=0.0):
         self.nh.upload(np.frombuffer(bytes(NhState(ps, target_T, dT, 0.0)), dtype=np.uint8))

@@ -43,6 +43,7 @@ def test_host_side_helpers_without_gpu():
     first = np.nonzero(rank < 8)[0]
     assert sorted(first.tolist()) == [0, 1, 68, 69, 68 * 68, 68 * 68 + 1, 68 * 68 + 68, 68 * 68 + 69]
     assert lib.rumdx_pair_lj(None, None, None, 2, configs.box8(10.0), 1, 1, 0, None) == -1
+    assert lib.rumdx_pair_lj_fp64(None, None, None, 2, configs.box8(10.0), 1, 1, 0, None) == -1
     assert b"null" in lib.rumdx_last_error()
     bad = np.zeros(3, dtype=np.int32)
     assert lib.rumdx_cell_rank_table(bad, rank) == -1
