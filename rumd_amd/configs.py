@@ -110,6 +110,19 @@ def ka_random_lattice(n: int, density: float = 1.2, temperature: float = 0.45, s
     return System(_wrap(pos, box), vel, types, box, np.ones(2), f"KA_N{n}")
 
 
+def ka_replica_set(nrep: int = 512, n_min: int = 256, n_max: int = 8192, t_min: float = 0.4, t_max: float = 2.0,
+                   density: float = 1.2, seed_n: int = 4001, seed_t: int = 4002, seed_start: int = 4100):
+    """Batched-replica workload: nrep Kob-Andersen replicas with N drawn
+    seeded-uniform in [n_min, n_max] and T uniform in [t_min, t_max]; replica r
+    starts from ka_random_lattice(N_r, density, T_r, seed=seed_start + r).
+    Returns (systems, temperatures)."""
+    ns = np.random.default_rng(seed_n).integers(n_min, n_max + 1, size=nrep)
+    ts = np.random.default_rng(seed_t).uniform(t_min, t_max, size=nrep)
+    systems = [ka_random_lattice(int(n), density=density, temperature=float(t), seed=seed_start + r)
+               for r, (n, t) in enumerate(zip(ns, ts))]
+    return systems, [float(t) for t in ts]
+
+
 def ka_params(rcut_in_sigma: float = 2.5):
     """(i, j, sigma, epsilon, rcut-in-units-of-sigma) rows for Pot_LJ_12_6.SetParams."""
     return [(i, j, KA_SIGMA[(i, j)], KA_EPSILON[(i, j)], rcut_in_sigma) for i in (0, 1) for j in (0, 1)]

@@ -59,6 +59,7 @@ template class DeviceArray<uint32_t>;
 template class DeviceArray<uint16_t>;
 template class DeviceArray<char>;
 template class DeviceArray<rumdx_nh_state>;
+template class DeviceArray<rumdx_replica>;
 
 // ------------------------------------------------------------------ Device
 Device& Device::GetDevice() {

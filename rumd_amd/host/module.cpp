@@ -5,6 +5,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include "replica_batch.h"
 #include "rumd_host.h"
 
 namespace py = pybind11;
@@ -247,4 +248,70 @@ PYBIND11_MODULE(_rumd, m) {
         for (Potential* p : s.GetPotentials()) p->SetSample(&s);
         s.InvalidateGraphs();
       }, py::arg("types"));
+
+  // Batched replicas (no counterpart in RUMD's SWIG module).
+  py::class_<ReplicaBatch>(m, "ReplicaBatch")
+      .def(py::init<unsigned>(), py::arg("num_types"))
+      .def("AddReplica", [](ReplicaBatch& b, const f32arr& pos, const i32arr& types, const f32arr& vel, float L) {
+        if (pos.ndim() != 2 || pos.shape(1) != 3 || vel.ndim() != 2 || vel.shape(1) != 3 || types.ndim() != 1 ||
+            pos.shape(0) != types.shape(0) || vel.shape(0) != types.shape(0))
+          throw py::value_error("positions and velocities must be N x 3 and types N");
+        const size_t n = static_cast<size_t>(types.shape(0));
+        std::vector<float> p(pos.data(), pos.data() + 3 * n), v(vel.data(), vel.data() + 3 * n);
+        std::vector<unsigned> t(n);
+        for (size_t i = 0; i < n; ++i) {
+          if (types.data()[i] < 0) throw py::value_error("negative type");
+          t[i] = static_cast<unsigned>(types.data()[i]);
+        }
+        return b.AddReplica(p, t, v, L);
+      }, py::arg("positions"), py::arg("types"), py::arg("velocities"), py::arg("box_length"))
+      .def("SetMass", &ReplicaBatch::SetMass, py::arg("type"), py::arg("mass"))
+      .def("SetPotential", &ReplicaBatch::SetPotential, py::arg("potential"), py::arg("skin"))
+      .def("SetTimeStep", &ReplicaBatch::SetTimeStep, py::arg("dt"))
+      .def("SetThermostat", &ReplicaBatch::SetThermostat, py::arg("replica"), py::arg("targetTemperature"),
+           py::arg("thermostatRelaxationTime"))
+      .def("SetMaxNumNbrs", &ReplicaBatch::SetMaxNumNbrs, py::arg("max_nbrs"))
+      .def("GetMaxNumNbrs", &ReplicaBatch::GetMaxNumNbrs)
+      .def("CalcF", &ReplicaBatch::CalcF)
+      .def("RunSteps", &ReplicaBatch::RunSteps, py::arg("nsteps"), py::arg("block") = 10,
+           py::call_guard<py::gil_scoped_release>())
+      .def("Synchronize", &ReplicaBatch::Synchronize)
+      .def("GetNumberOfReplicas", &ReplicaBatch::GetNumberOfReplicas)
+      .def("GetNumberOfParticles", &ReplicaBatch::GetNumberOfParticles, py::arg("replica"))
+      .def("GetTotalParticles", &ReplicaBatch::GetTotalParticles)
+      .def("GetNumberOfSlots", &ReplicaBatch::GetNumberOfSlots)
+      .def("GetPositions", [](ReplicaBatch& b, unsigned r) {
+        auto v = b.GetPositions(r);
+        py::array_t<float> out({static_cast<py::ssize_t>(v.size() / 3), static_cast<py::ssize_t>(3)});
+        std::copy(v.begin(), v.end(), out.mutable_data());
+        return out;
+      }, py::arg("replica"))
+      .def("GetVelocities", [](ReplicaBatch& b, unsigned r) {
+        auto v = b.GetVelocities(r);
+        py::array_t<float> out({static_cast<py::ssize_t>(v.size() / 3), static_cast<py::ssize_t>(3)});
+        std::copy(v.begin(), v.end(), out.mutable_data());
+        return out;
+      }, py::arg("replica"))
+      .def("GetForcesAndEnergies", [](ReplicaBatch& b, unsigned r) {
+        auto v = b.GetForces(r);
+        py::array_t<float> out({static_cast<py::ssize_t>(v.size() / 4), static_cast<py::ssize_t>(4)});
+        std::copy(v.begin(), v.end(), out.mutable_data());
+        return out;
+      }, py::arg("replica"))
+      .def("GetEnergies", [](ReplicaBatch& b) {
+        auto v = b.GetEnergies();
+        py::array_t<double> out({static_cast<py::ssize_t>(v.size() / 4), static_cast<py::ssize_t>(4)});
+        std::copy(v.begin(), v.end(), out.mutable_data());
+        return out;
+      })
+      .def("GetThermostatStates", &ReplicaBatch::GetThermostatStates)
+      .def("GetNumberOfBuilds", &ReplicaBatch::GetNumberOfBuilds)
+      .def("GetNeighborListCSR", [](ReplicaBatch& b, unsigned r) {
+        std::vector<uint32_t> off, ids;
+        b.GetNeighborList(r, off, ids);
+        py::array_t<uint32_t> o(off.size()), d(ids.size());
+        std::copy(off.begin(), off.end(), o.mutable_data());
+        std::copy(ids.begin(), ids.end(), d.mutable_data());
+        return py::make_tuple(o, d);
+      }, py::arg("replica"));
 }

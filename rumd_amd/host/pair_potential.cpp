@@ -340,8 +340,7 @@ std::vector<float> PairPotential::GetPotentialParameters(unsigned num_types) con
   return v;
 }
 
-void PairPotential::UploadParams() {
-  const unsigned nt = sample->GetNumberOfTypes();
+std::vector<float> PairPotential::GetParamsTable(unsigned nt) const {
   std::vector<float> tab(static_cast<size_t>(nt) * nt * kNumParam, 0.f);
   for (unsigned i = 0; i < nt; ++i)
     for (unsigned j = 0; j < nt; ++j) {
@@ -349,6 +348,12 @@ void PairPotential::UploadParams() {
       if (it != params_map_.end())
         std::copy(it->second.begin(), it->second.end(), tab.begin() + (static_cast<size_t>(i) * nt + j) * kNumParam);
     }
+  return tab;
+}
+
+void PairPotential::UploadParams() {
+  const unsigned nt = sample->GetNumberOfTypes();
+  const std::vector<float> tab = GetParamsTable(nt);
   if (d_params_.size() != tab.size()) d_params_.resize(tab.size());
   d_params_.upload(tab.data(), tab.size(), sample->GetStream());
   table_types_ = nt;

@@ -1,0 +1,417 @@
+// ReplicaBatch: host side of the batched-replica kernels (rumdx_batch_*).
+#include "replica_batch.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace {
+
+uint32_t padded(uint32_t n) { return (n + RUMDX_BATCH_ALIGN - 1) / RUMDX_BATCH_ALIGN * RUMDX_BATCH_ALIGN; }
+
+}  // namespace
+
+ReplicaBatch::ReplicaBatch(unsigned num_types) : ntypes_(num_types), mass_(num_types, 1.f) {
+  if (num_types == 0 || num_types > RUMDX_MAX_TYPES)
+    throw RUMD_Error("ReplicaBatch", "ReplicaBatch", "number of types out of range");
+  Device::GetDevice().Init();
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "ReplicaBatch", "ReplicaBatch");
+}
+
+ReplicaBatch::~ReplicaBatch() {
+  InvalidateGraphs();
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+// Layout-changing calls keep the device state: positions and velocities are
+// read back into the replica specs before the arrays are rebuilt.
+void ReplicaBatch::Relayout() {
+  if (allocated_) {
+    for (unsigned r = 0; r < reps_.size() && r < slot0_.size(); ++r) {
+      reps_[r].pos = GetPositions(r);
+      reps_[r].vel = GetVelocities(r);
+    }
+  }
+  allocated_ = false;
+  InvalidateGraphs();
+}
+
+unsigned ReplicaBatch::AddReplica(const std::vector<float>& positions, const std::vector<unsigned>& types,
+                                  const std::vector<float>& velocities, float box_length) {
+  const size_t n = types.size();
+  if (n < 2 || positions.size() != 3 * n || velocities.size() != 3 * n)
+    throw RUMD_Error("ReplicaBatch", "AddReplica", "positions/velocities must be n x 3 with n = len(types) >= 2");
+  if (!(box_length > 0.f)) throw RUMD_Error("ReplicaBatch", "AddReplica", "box length must be positive");
+  for (unsigned t : types)
+    if (t >= ntypes_) throw RUMD_Error("ReplicaBatch", "AddReplica", "type index out of range");
+  Relayout();
+  Spec s;
+  s.pos = positions;
+  s.vel = velocities;
+  s.types = types;
+  s.L = box_length;
+  reps_.push_back(std::move(s));
+  return static_cast<unsigned>(reps_.size() - 1);
+}
+
+void ReplicaBatch::SetMass(unsigned type, float mass) {
+  if (type >= ntypes_ || !(mass > 0.f)) throw RUMD_Error("ReplicaBatch", "SetMass", "bad type or mass");
+  Relayout();
+  mass_[type] = mass;
+}
+
+void ReplicaBatch::SetPotential(const PairPotential& pot, float skin) {
+  if (!(skin > 0.f)) throw RUMD_Error("ReplicaBatch", "SetPotential", "skin must be positive");
+  Relayout();
+  table_ = pot.GetParamsTable(ntypes_);
+  cutoff_method_ = pot.GetCutoffMethod();
+  skin_ = skin;
+  max_cutoff_ = 0.f;
+  for (unsigned k = 0; k < ntypes_ * ntypes_; ++k) max_cutoff_ = std::max(max_cutoff_, table_[k * RUMDX_NUM_PARAM]);
+  if (!(max_cutoff_ > 0.f)) throw RUMD_Error("ReplicaBatch", "SetPotential", "potential has no parameters set");
+}
+
+void ReplicaBatch::SetTimeStep(float dt) {
+  dt_ = dt;
+  InvalidateGraphs();
+}
+
+void ReplicaBatch::SetThermostat(unsigned replica, float target_temperature, float tau) {
+  if (replica >= reps_.size()) throw RUMD_Error("ReplicaBatch", "SetThermostat", "replica index out of range");
+  reps_[replica].T = target_temperature;
+  reps_[replica].tau = tau;
+  if (allocated_) {
+    // in place: the replica's entry and its thermostat target
+    Synchronize();
+    const rumdx_replica rp = HostReplica(replica);
+    hip_check(hipMemcpyAsync(rep_.get() + replica, &rp, sizeof(rp), hipMemcpyHostToDevice, stream_), "ReplicaBatch",
+              "SetThermostat");
+    rumdx_nh_state st;
+    hip_check(hipMemcpyAsync(&st, nh_.get() + replica, sizeof(st), hipMemcpyDeviceToHost, stream_), "ReplicaBatch",
+              "SetThermostat");
+    hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "SetThermostat");
+    st.target_T = target_temperature;
+    hip_check(hipMemcpyAsync(nh_.get() + replica, &st, sizeof(st), hipMemcpyHostToDevice, stream_), "ReplicaBatch",
+              "SetThermostat");
+    hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "SetThermostat");
+  }
+}
+
+void ReplicaBatch::SetMaxNumNbrs(unsigned m) {
+  Relayout();
+  user_max_nbrs_ = m;
+  max_nbrs_ = 0;
+}
+
+unsigned ReplicaBatch::GetNumberOfParticles(unsigned replica) const {
+  if (replica >= reps_.size()) throw RUMD_Error("ReplicaBatch", "GetNumberOfParticles", "replica index out of range");
+  return static_cast<unsigned>(reps_[replica].types.size());
+}
+
+unsigned ReplicaBatch::GetTotalParticles() const {
+  unsigned n = 0;
+  for (const Spec& s : reps_) n += static_cast<unsigned>(s.types.size());
+  return n;
+}
+
+rumdx_replica ReplicaBatch::HostReplica(unsigned r) const {
+  const Spec& s = reps_[r];
+  rumdx_replica rp{};
+  rp.slot0 = slot0_[r];
+  rp.n = static_cast<uint32_t>(s.types.size());
+  rp.cell0 = cell0_[r];
+  rp.nc = nc_[r];
+  rp.L = s.L;
+  rp.inv_L = 1.f / s.L;
+  rp.omega2 = s.tau > 0.f ? static_cast<float>(4.0 * M_PI * M_PI / (static_cast<double>(s.tau) * s.tau)) : 0.f;
+  rp.dof = 3 * rp.n - 3;
+  return rp;
+}
+
+// Lays the replicas out (slot ranges padded to whole workgroups, cell grids
+// one after another), allocates every array and uploads the replica specs
+// (the state Relayout read back, or the configurations AddReplica was given).
+// Thermostat states restart at Ps = 0.
+void ReplicaBatch::Allocate() {
+  if (reps_.empty()) throw RUMD_Error("ReplicaBatch", "Allocate", "no replicas");
+  if (table_.empty()) throw RUMD_Error("ReplicaBatch", "Allocate", "no potential set");
+  const unsigned nrep = static_cast<unsigned>(reps_.size());
+  slot0_.assign(nrep, 0u);
+  cell0_.assign(nrep, 0u);
+  nc_.assign(nrep, 0);
+  uint64_t slots = 0, cells = 0;
+  double max_density = 0.0;
+  unsigned max_n = 0;
+  for (unsigned r = 0; r < nrep; ++r) {
+    const Spec& s = reps_[r];
+    slot0_[r] = static_cast<uint32_t>(slots);
+    slots += padded(static_cast<uint32_t>(s.types.size()));
+    const float box8[8] = {1.f, s.L, s.L, s.L, 1.f / s.L, 1.f / s.L, 1.f / s.L, 0.f};
+    int32_t nc[3];
+    rumdx_check(rumdx_cell_grid(box8, max_cutoff_, skin_, nc), "ReplicaBatch", "Allocate");
+    nc_[r] = nc[0];
+    cell0_[r] = static_cast<uint32_t>(cells);
+    cells += static_cast<uint64_t>(nc[0]) * nc[0] * nc[0];
+    max_density = std::max(max_density, s.types.size() / (static_cast<double>(s.L) * s.L * s.L));
+    max_n = std::max(max_n, static_cast<unsigned>(s.types.size()));
+  }
+  if (slots >= (1ull << 31) || cells >= (1ull << 31))
+    throw RUMD_cudaResourceError("ReplicaBatch", "Allocate", "batch too large");
+  nslots_ = static_cast<uint32_t>(slots);
+  ncells_ = static_cast<uint32_t>(cells);
+  if (user_max_nbrs_) {
+    max_nbrs_ = user_max_nbrs_;
+  } else if (max_nbrs_ == 0) {
+    // NeighborList's estimate (src/NeighborList.cu:595-610) at the densest replica
+    const double rs = max_cutoff_ + skin_;
+    max_nbrs_ = static_cast<unsigned>(std::max(50.0, 1.5 * 4.0 / 3.0 * M_PI * rs * rs * rs * max_density));
+  }
+  max_nbrs_ = std::max(1u, std::min(max_nbrs_, max_n));
+  if (static_cast<uint64_t>(max_nbrs_) * nslots_ >= (1ull << 32))
+    throw RUMD_cudaResourceError("ReplicaBatch", "Allocate", "neighbour lists exceed 2^32 entries");
+
+  const size_t ns = nslots_;
+  std::vector<float> hr(4 * ns, 0.f), hv(4 * ns, 0.f);
+  std::vector<uint32_t> hb(ns / RUMDX_BATCH_ALIGN);
+  std::vector<rumdx_replica> hrep(nrep);
+  std::vector<rumdx_nh_state> hnh(nrep);
+  std::vector<uint32_t> hflags(static_cast<size_t>(RUMDX_BATCH_FLAGS) * nrep, 0u);
+  for (unsigned r = 0; r < nrep; ++r) {
+    const Spec& s = reps_[r];
+    hrep[r] = HostReplica(r);
+    hnh[r] = rumdx_nh_state{0.0, static_cast<double>(s.T), 0.0, 0.0};
+    hflags[RUMDX_BATCH_FLAGS * r] = 1u;  // first build
+    const uint32_t n = static_cast<uint32_t>(s.types.size());
+    for (uint32_t b = slot0_[r] / RUMDX_BATCH_ALIGN; b < (slot0_[r] + padded(n)) / RUMDX_BATCH_ALIGN; ++b) hb[b] = r;
+    for (uint32_t i = 0; i < n; ++i) {
+      const size_t k = 4ull * (slot0_[r] + i);
+      hr[k] = s.pos[3ull * i];
+      hr[k + 1] = s.pos[3ull * i + 1];
+      hr[k + 2] = s.pos[3ull * i + 2];
+      uint32_t t = s.types[i];
+      std::memcpy(&hr[k + 3], &t, sizeof(t));
+      hv[k] = s.vel[3ull * i];
+      hv[k + 1] = s.vel[3ull * i + 1];
+      hv[k + 2] = s.vel[3ull * i + 2];
+      hv[k + 3] = 1.f / mass_[t];
+    }
+  }
+  InvalidateGraphs();
+  r_.resize(4 * ns);
+  v_.resize(4 * ns);
+  f_.resize(4 * ns);
+  w_.resize(4 * ns);
+  im_.resize(4 * ns);
+  last_r_.resize(4 * ns);
+  r_.upload(hr.data(), hr.size(), stream_);
+  v_.upload(hv.data(), hv.size(), stream_);
+  block_rep_.resize(hb.size());
+  block_rep_.upload(hb.data(), hb.size(), stream_);
+  rep_.resize(nrep);
+  rep_.upload(hrep.data(), nrep, stream_);
+  nh_.resize(nrep);
+  nh_.upload(hnh.data(), nrep, stream_);
+  flags_.resize(hflags.size());
+  flags_.upload(hflags.data(), hflags.size(), stream_);
+  nbl_.resize(static_cast<size_t>(max_nbrs_) * ns);
+  num_nbrs_.resize(ns);
+  cell_count_.resize(ncells_);
+  cell_start_.resize(ncells_);
+  cell_n_.resize(ncells_);
+  cell_list_.resize(ns);
+  cell_tmp_.resize(ns);
+  slot_cell_.resize(ns);
+  params_.resize(table_.size());
+  params_.upload(table_.data(), table_.size(), stream_);
+  std::vector<float> c2(static_cast<size_t>(ntypes_) * ntypes_);
+  for (size_t k = 0; k < c2.size(); ++k) {
+    const float c = table_[k * RUMDX_NUM_PARAM] + skin_;  // squared in float (src/NeighborList.cu:1239-1240)
+    c2[k] = c * c;
+  }
+  cut2_.resize(c2.size());
+  cut2_.upload(c2.data(), c2.size(), stream_);
+  workspace_.resize((rumdx_batch_workspace_bytes(nslots_) + sizeof(double) - 1) / sizeof(double));
+  energies_.resize(4ull * nrep);
+
+  std::memset(&desc_, 0, sizeof(desc_));
+  desc_.r_dev = r_.get();
+  desc_.v_dev = v_.get();
+  desc_.f_dev = f_.get();
+  desc_.w_dev = w_.get();
+  desc_.im_dev = im_.get();
+  desc_.nslots = nslots_;
+  desc_.nrep = nrep;
+  desc_.rep_dev = rep_.get();
+  desc_.block_rep_dev = block_rep_.get();
+  desc_.nh_dev = nh_.get();
+  desc_.rep_flags_dev = flags_.get();
+  desc_.nbl_dev = nbl_.get();
+  desc_.num_nbrs_dev = num_nbrs_.get();
+  desc_.last_r_dev = last_r_.get();
+  desc_.cut_skin2_dev = cut2_.get();
+  desc_.max_nbrs = max_nbrs_;
+  desc_.ntypes = ntypes_;
+  desc_.skin = skin_;
+  desc_.ncells = ncells_;
+  desc_.cell_count_dev = cell_count_.get();
+  desc_.cell_start_dev = cell_start_.get();
+  desc_.cell_n_dev = cell_n_.get();
+  desc_.cell_list_dev = cell_list_.get();
+  desc_.cell_tmp_dev = cell_tmp_.get();
+  desc_.slot_cell_dev = slot_cell_.get();
+  allocated_ = true;
+}
+
+void ReplicaBatch::CalcF() {
+  if (!allocated_) Allocate();
+  rumdx_check(rumdx_batch_update_lists(&desc_, stream_), "ReplicaBatch", "CalcF");
+  rumdx_check(rumdx_batch_forces(&desc_, params_.get(), cutoff_method_, stream_), "ReplicaBatch", "CalcF");
+}
+
+void ReplicaBatch::Step() {
+  CalcF();
+  rumdx_check(rumdx_batch_nvt_step(&desc_, dt_, workspace_.get(), stream_), "ReplicaBatch", "RunSteps");
+}
+
+void ReplicaBatch::InvalidateGraphs() {
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  graphs_.clear();
+}
+
+void ReplicaBatch::RunSteps(unsigned nsteps, unsigned block) {
+  if (!allocated_) Allocate();
+  if (block == 0) block = 1;
+  unsigned done = 0;
+  while (done < nsteps) {
+    if (block == 1 || done + block > nsteps) {
+      Step();
+      ++done;
+      continue;
+    }
+    auto it = graphs_.find(block);
+    if (it == graphs_.end()) {
+      hipGraph_t g = nullptr;
+      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "ReplicaBatch", "RunSteps");
+      try {
+        for (unsigned k = 0; k < block; ++k) Step();
+      } catch (...) {
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(stream_, &g), "ReplicaBatch", "RunSteps");
+      hipGraphExec_t exec = nullptr;
+      hip_check(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "ReplicaBatch", "RunSteps");
+      (void)hipGraphDestroy(g);
+      it = graphs_.emplace(block, exec).first;
+    }
+    hip_check(hipGraphLaunch(it->second, stream_), "ReplicaBatch", "RunSteps");
+    done += block;
+  }
+  Synchronize();
+}
+
+// Grows the lists when some replica's overflowed (the forces were exact: the
+// force kernel rescans the stencil for an overflowed list) and rebuilds all.
+void ReplicaBatch::Synchronize() {
+  hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "Synchronize");
+  if (!allocated_) return;
+  std::vector<uint32_t> fl(static_cast<size_t>(RUMDX_BATCH_FLAGS) * reps_.size());
+  flags_.download(fl.data(), fl.size(), stream_);
+  uint32_t longest = 0;
+  bool overflow = false;
+  for (size_t r = 0; r < reps_.size(); ++r) {
+    overflow = overflow || fl[RUMDX_BATCH_FLAGS * r + 1] != 0u;
+    longest = std::max(longest, fl[RUMDX_BATCH_FLAGS * r + 3]);
+  }
+  if (!overflow) return;
+  unsigned max_n = 0;
+  for (const Spec& s : reps_) max_n = std::max(max_n, static_cast<unsigned>(s.types.size()));
+  max_nbrs_ = std::min(std::max(max_nbrs_ + max_nbrs_ / 2, longest + longest / 4 + 8), max_n);
+  if (user_max_nbrs_) user_max_nbrs_ = max_nbrs_;
+  InvalidateGraphs();
+  nbl_.resize(static_cast<size_t>(max_nbrs_) * nslots_);
+  desc_.nbl_dev = nbl_.get();
+  desc_.max_nbrs = max_nbrs_;
+  for (size_t r = 0; r < reps_.size(); ++r) {
+    fl[RUMDX_BATCH_FLAGS * r] = 1u;  // rebuild
+    fl[RUMDX_BATCH_FLAGS * r + 1] = 0u;
+  }
+  flags_.upload(fl.data(), fl.size(), stream_);
+}
+
+std::vector<float> ReplicaBatch::Slots(const DeviceArray<float>& a, unsigned replica, unsigned comps) {
+  if (!allocated_) Allocate();
+  if (replica >= reps_.size()) throw RUMD_Error("ReplicaBatch", "Get", "replica index out of range");
+  Synchronize();
+  const unsigned n = GetNumberOfParticles(replica);
+  std::vector<float> all(4ull * n);
+  hip_check(hipMemcpyAsync(all.data(), a.get() + 4ull * slot0_[replica], all.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, stream_),
+            "ReplicaBatch", "Get");
+  hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "Get");
+  std::vector<float> out(static_cast<size_t>(comps) * n);
+  for (unsigned i = 0; i < n; ++i)
+    for (unsigned c = 0; c < comps; ++c) out[static_cast<size_t>(comps) * i + c] = all[4ull * i + c];
+  return out;
+}
+
+std::vector<float> ReplicaBatch::GetPositions(unsigned replica) { return Slots(r_, replica, 3); }
+std::vector<float> ReplicaBatch::GetVelocities(unsigned replica) { return Slots(v_, replica, 3); }
+std::vector<float> ReplicaBatch::GetForces(unsigned replica) { return Slots(f_, replica, 4); }
+
+std::vector<double> ReplicaBatch::GetEnergies() {
+  if (!allocated_) Allocate();
+  Synchronize();
+  rumdx_check(rumdx_batch_energies(&desc_, dt_, energies_.get(), workspace_.get(), stream_), "ReplicaBatch",
+              "GetEnergies");
+  std::vector<double> e(4 * reps_.size());
+  energies_.download(e.data(), e.size(), stream_);
+  for (size_t r = 0; r < reps_.size(); ++r) e[4 * r + 1] /= 6.0;  // virial = sum w_i / 6 (src/Sample.cu:370-391)
+  return e;
+}
+
+std::vector<double> ReplicaBatch::GetThermostatStates() {
+  if (!allocated_) Allocate();
+  Synchronize();
+  std::vector<rumdx_nh_state> s(reps_.size());
+  nh_.download(s.data(), s.size(), stream_);
+  std::vector<double> ps(s.size());
+  for (size_t r = 0; r < s.size(); ++r) ps[r] = s[r].ps;
+  return ps;
+}
+
+std::vector<unsigned> ReplicaBatch::GetNumberOfBuilds() {
+  if (!allocated_) Allocate();
+  Synchronize();
+  std::vector<uint32_t> fl(static_cast<size_t>(RUMDX_BATCH_FLAGS) * reps_.size());
+  flags_.download(fl.data(), fl.size(), stream_);
+  std::vector<unsigned> b(reps_.size());
+  for (size_t r = 0; r < reps_.size(); ++r) b[r] = fl[RUMDX_BATCH_FLAGS * r + 2];
+  return b;
+}
+
+void ReplicaBatch::GetNeighborList(unsigned replica, std::vector<uint32_t>& offsets, std::vector<uint32_t>& ids) {
+  if (!allocated_) Allocate();
+  if (replica >= reps_.size()) throw RUMD_Error("ReplicaBatch", "GetNeighborList", "replica index out of range");
+  Synchronize();
+  const unsigned n = GetNumberOfParticles(replica);
+  const uint32_t s0 = slot0_[replica];
+  std::vector<uint32_t> cnt(n);
+  hip_check(hipMemcpyAsync(cnt.data(), num_nbrs_.get() + s0, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
+            "ReplicaBatch", "GetNeighborList");
+  std::vector<uint32_t> col(n);
+  offsets.assign(n + 1, 0u);
+  for (unsigned i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + cnt[i];
+  ids.assign(offsets[n], 0u);
+  unsigned kmax = 0;
+  for (uint32_t c : cnt) kmax = std::max(kmax, c);
+  for (unsigned k = 0; k < kmax; ++k) {
+    hip_check(hipMemcpyAsync(col.data(), nbl_.get() + static_cast<size_t>(k) * nslots_ + s0, n * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, stream_),
+              "ReplicaBatch", "GetNeighborList");
+    hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "GetNeighborList");
+    for (unsigned i = 0; i < n; ++i)
+      if (k < cnt[i]) ids[offsets[i] + k] = col[i] - s0;
+  }
+}

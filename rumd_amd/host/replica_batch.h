@@ -1,0 +1,90 @@
+// ReplicaBatch: many independent samples advanced together on one GPU
+// (rumdx_batch_* in include/rumdx.h).  RUMD runs one Sample at a time; this
+// class is for ensembles of small systems, where per-sample launches leave
+// the GPU idle.  Each replica has its own cubic box, particles, Nose-Hoover
+// thermostat and neighbour-list rebuild schedule; all share one pair
+// potential (its parameter table and cut-off method) and one time step.
+#pragma once
+
+#include <map>
+#include <vector>
+
+#include "rumd_host.h"
+
+class ReplicaBatch {
+ public:
+  explicit ReplicaBatch(unsigned num_types);
+  ~ReplicaBatch();
+  ReplicaBatch(const ReplicaBatch&) = delete;
+  ReplicaBatch& operator=(const ReplicaBatch&) = delete;
+
+  // Adds a replica: positions and velocities n x 3 (in [-L/2, L/2)), types n.
+  // Returns its index.
+  unsigned AddReplica(const std::vector<float>& positions, const std::vector<unsigned>& types,
+                      const std::vector<float>& velocities, float box_length);
+  void SetMass(unsigned type, float mass);
+  // The pair potential's table and cut-off method, and the list skin.
+  void SetPotential(const PairPotential& pot, float skin);
+  void SetTimeStep(float dt);
+  // Nose-Hoover thermostat of one replica (relaxation time tau <= 0: NVE).
+  void SetThermostat(unsigned replica, float target_temperature, float tau);
+  void SetMaxNumNbrs(unsigned m);
+
+  // Forces and energies of the current positions (lists updated first).
+  void CalcF();
+  // nsteps steps of CalcF + leapfrog, as HIP-graph blocks of `block` steps.
+  void RunSteps(unsigned nsteps, unsigned block);
+  // Waits for the stream; grows the lists after an overflow.
+  void Synchronize();
+
+  unsigned GetNumberOfReplicas() const { return static_cast<unsigned>(reps_.size()); }
+  unsigned GetNumberOfParticles(unsigned replica) const;
+  unsigned GetTotalParticles() const;
+  unsigned GetNumberOfSlots() const { return nslots_; }
+  std::vector<float> GetPositions(unsigned replica);   // n x 3
+  std::vector<float> GetVelocities(unsigned replica);  // n x 3
+  std::vector<float> GetForces(unsigned replica);      // n x 4: F, u_i
+  // Per replica {potential energy, virial (sum w_i / 6), kinetic energy, sum m v^2}.
+  std::vector<double> GetEnergies();
+  std::vector<double> GetThermostatStates();  // Ps per replica
+  std::vector<unsigned> GetNumberOfBuilds();
+  unsigned GetMaxNumNbrs() const { return max_nbrs_; }
+  // Neighbour list of one replica in replica-local ids: CSR (offsets n + 1, ids).
+  void GetNeighborList(unsigned replica, std::vector<uint32_t>& offsets, std::vector<uint32_t>& ids);
+  hipStream_t GetStream() const { return stream_; }
+
+ private:
+  struct Spec {
+    std::vector<float> pos, vel;
+    std::vector<unsigned> types;
+    float L = 0.f, T = 1.f, tau = 0.f;
+  };
+  void Allocate();
+  void Relayout();
+  void Step();
+  void InvalidateGraphs();
+  rumdx_replica HostReplica(unsigned r) const;
+  std::vector<float> Slots(const DeviceArray<float>& a, unsigned replica, unsigned comps);
+
+  unsigned ntypes_;
+  std::vector<Spec> reps_;
+  std::vector<float> mass_;
+  std::vector<float> table_;
+  int cutoff_method_ = 2;
+  float skin_ = 0.3f, max_cutoff_ = 0.f, dt_ = 0.0025f;
+  unsigned user_max_nbrs_ = 0, max_nbrs_ = 0;
+  bool allocated_ = false;
+  hipStream_t stream_ = nullptr;
+
+  std::vector<uint32_t> slot0_, cell0_;
+  std::vector<int32_t> nc_;
+  uint32_t nslots_ = 0, ncells_ = 0;
+  DeviceArray<float> r_, v_, f_, w_, im_, last_r_, params_, cut2_;
+  DeviceArray<uint32_t> block_rep_, flags_, nbl_, num_nbrs_, cell_count_, cell_start_, cell_n_, cell_list_,
+      cell_tmp_, slot_cell_;
+  DeviceArray<rumdx_replica> rep_;
+  DeviceArray<rumdx_nh_state> nh_;
+  DeviceArray<double> workspace_, energies_;
+  rumdx_batch desc_{};
+  std::map<unsigned, hipGraphExec_t> graphs_;
+};

@@ -226,6 +226,8 @@ class PairPotential : public Potential {
   void SetParamsRow(unsigned i, unsigned j, const std::vector<float>& row);
   float GetPotentialParameter(unsigned i, unsigned j, unsigned idx) const;
   std::vector<float> GetPotentialParameters(unsigned num_types) const;
+  // The full device table (12 floats per type pair, shifts included).
+  std::vector<float> GetParamsTable(unsigned num_types) const;
   void SetNbListSkin(float skin) { neighborList.SetSkin(skin); }
   float GetNbListSkin() const { return neighborList.GetSkin(); }
   float GetMaxCutoff() const { return neighborList.GetMaxCutoff(); }

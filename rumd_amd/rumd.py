@@ -30,6 +30,7 @@ ShiftedPotential = _rumd.ShiftedPotential
 ShiftedForce = _rumd.ShiftedForce
 GetVersion = _rumd.GetVersion
 NumberOfPairPotentialParameters = _rumd.NumberOfPairPotentialParameters
+ReplicaBatch = _rumd.ReplicaBatch
 
 
 class Sample(_rumd.Sample):
@@ -91,5 +92,5 @@ class Sample(_rumd.Sample):
 __all__ = [
     "Device", "SimulationBox", "RectangularSimulationBox", "Potential", "PairPotential", "Pot_IPL_12_6",
     "Pot_LJ_12_6", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
-    "ShiftedForce", "GetVersion", "NumberOfPairPotentialParameters", "Sample",
+    "ShiftedForce", "GetVersion", "NumberOfPairPotentialParameters", "Sample", "ReplicaBatch",
 ]

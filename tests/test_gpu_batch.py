@@ -1,0 +1,145 @@
+"""Batched replicas (ReplicaBatch over rumdx_batch_*): per-replica forces
+against the FP64 oracle, neighbour sets bit-exact against the FP32
+restatement, each replica independent of the others bit for bit, the
+per-replica thermostat update, and exact forces through a list overflow."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from capi import gpu_available
+from helpers import FORCE_TOL_GUARDED, force_errors, ka_table
+from rumd_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+SF = 2
+
+
+@pytest.fixture(autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.fail("no GPU visible: the gpu tests must run on an MI355X")
+
+
+def _rumd():
+    import rumd_amd.rumd as rumd
+    return rumd
+
+
+def make_batch(systems, temps, tau=0.2, skin=0.3, dt=0.0025, max_nbrs=None):
+    rumd = _rumd()
+    b = rumd.ReplicaBatch(2)
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedForce)
+    pot.SetVerbose(False)
+    for (i, j, sg, ep, rc) in configs.ka_params(2.5):
+        pot.SetParams(i, j, Sigma=sg, Epsilon=ep, Rcut=rc)
+    for s in systems:
+        b.AddReplica(s.positions.astype(np.float32), s.types.astype(np.int32), s.velocities.astype(np.float32),
+                     float(s.box[0]))
+    b.SetPotential(pot, skin)
+    b.SetTimeStep(dt)
+    for r, t in enumerate(temps):
+        b.SetThermostat(r, t, tau)
+    if max_nbrs is not None:
+        b.SetMaxNumNbrs(max_nbrs)
+    return b
+
+
+def _systems(sizes, seed0=500):
+    return [configs.ka_random_lattice(n, density=1.2, temperature=0.45 + 0.3 * k, seed=seed0 + k)
+            for k, n in enumerate(sizes)]
+
+
+def test_batch_forces_and_lists_per_replica():
+    sizes = [256, 700, 1500, 4000, 8192]
+    systems = _systems(sizes)
+    b = make_batch(systems, [0.5] * len(sizes))
+    print(f"[case] batch forces sizes={sizes}", flush=True)
+    b.CalcF()
+    e = b.GetEnergies()
+    cut2 = configs.cut_skin2_table(ka_table(), 0.3)
+    for r, s in enumerate(systems):
+        pos4 = configs.pack_pos4(s.positions, s.types)
+        f = b.GetForcesAndEnergies(r)
+        f_ref, vir_ref = (oracle.forces_brute if s.n <= 2000 else oracle.forces_cells)(pos4, s.box, 2, ka_table(), SF)
+        err = force_errors(f[:, :3], f_ref[:, :3])
+        assert err["guarded_max"] <= FORCE_TOL_GUARDED, (r, err)
+        assert e[r, 0] == pytest.approx(float(f_ref[:, 3].sum()), rel=1e-5, abs=1e-3)
+        assert e[r, 1] == pytest.approx(float(vir_ref.sum()) / 6.0, rel=1e-5, abs=1e-3)
+        b8 = configs.box8(s.box)
+        off, ids = b.GetNeighborListCSR(r)
+        o2, i2 = oracle.nblist_cells_f32(pos4, b8, 2, cut2, oracle.cell_grid(b8, 2.5, 0.3))
+        assert np.array_equal(off, o2), r
+        for i in range(s.n):
+            assert set(ids[off[i]:off[i + 1]].tolist()) == set(i2[o2[i]:o2[i + 1]].tolist()), (r, i)
+
+
+def test_batch_replicas_are_independent():
+    """A replica run inside a batch equals the same replica run alone, bit for
+    bit (graph-captured blocks, rebuilds included)."""
+    sizes = [300, 1200, 5000]
+    temps = [0.45, 1.5, 0.8]
+    systems = _systems(sizes, seed0=900)
+    b = make_batch(systems, temps)
+    print("[case] batch of 3, 60 steps", flush=True)
+    b.RunSteps(60, 10)
+    builds = b.GetNumberOfBuilds()
+    assert all(k >= 2 for k in builds), builds
+    for r, s in enumerate(systems):
+        alone = make_batch([s], [temps[r]])
+        alone.RunSteps(60, 10)
+        print(f"[case] replica {r} alone", flush=True)
+        assert np.array_equal(alone.GetPositions(0), b.GetPositions(r)), r
+        assert np.array_equal(alone.GetVelocities(0), b.GetVelocities(r)), r
+        assert alone.GetThermostatStates()[0] == b.GetThermostatStates()[r]
+
+
+def test_batch_thermostat_update():
+    """One step: Ps = dt omega^2 (sum m v^2 / (dof T) - 1), each replica with
+    its own T and dof."""
+    sizes = [400, 2000]
+    temps = [0.6, 1.8]
+    systems = _systems(sizes, seed0=300)
+    dt, tau = 0.0025, 0.2
+    b = make_batch(systems, temps, tau=tau, dt=dt)
+    b.RunSteps(1, 1)
+    e = b.GetEnergies()
+    ps = b.GetThermostatStates()
+    omega2 = float(np.float32(4 * np.pi ** 2 / tau ** 2))
+    for r, s in enumerate(systems):
+        dof = 3 * s.n - 3
+        expect = dt * omega2 * (e[r, 3] / (dof * temps[r]) - 1.0)
+        assert ps[r] == pytest.approx(expect, rel=1e-5), r
+
+
+def test_batch_overflowed_lists_still_exact():
+    systems = _systems([1000, 3000], seed0=700)
+    b = make_batch(systems, [0.5, 0.5], max_nbrs=24)
+    print("[case] batch overflow max_nbrs=24", flush=True)
+    b.CalcF()
+    for r, s in enumerate(systems):
+        pos4 = configs.pack_pos4(s.positions, s.types)
+        f_ref, _ = oracle.forces_brute(pos4, s.box, 2, ka_table(), SF)
+        err = force_errors(b.GetForcesAndEnergies(r)[:, :3], f_ref[:, :3])
+        assert err["guarded_max"] <= FORCE_TOL_GUARDED, (r, err)
+    b.Synchronize()
+    assert b.GetMaxNumNbrs() > 24
+    b.CalcF()
+    f_ref, _ = oracle.forces_brute(configs.pack_pos4(systems[1].positions, systems[1].types), systems[1].box, 2,
+                                   ka_table(), SF)
+    err = force_errors(b.GetForcesAndEnergies(1)[:, :3], f_ref[:, :3])
+    assert err["guarded_max"] <= FORCE_TOL_GUARDED, err
+
+
+def test_batch_hot_replicas_rebuild_more_often():
+    systems = _systems([2000, 2000], seed0=100)
+    b = make_batch(systems, [0.4, 2.0])
+    print("[case] batch hot/cold rebuild counts", flush=True)
+    b.RunSteps(200, 10)
+    builds = b.GetNumberOfBuilds()
+    e = b.GetEnergies()
+    t_kin = [2.0 * e[r, 2] / (3 * s.n - 3) for r, s in enumerate(systems)]
+    assert t_kin[1] > t_kin[0]
+    assert builds[1] > builds[0], builds
