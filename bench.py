@@ -146,6 +146,127 @@ def counter_calibration(n: int, reps: int) -> dict:
             "k_momentum_apply_write_bytes": nbytes}
 
 
+def force_accuracy(sample, pot, system):
+    """Per-particle force error of the FP32 kernel and of the all-FP64 kernel
+    against the FP64 oracle (linked cells) on the current state."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    from rumd_amd import configs
+    sample.Synchronize()
+    pos = sample.GetPositions()
+    pos4 = configs.pack_pos4(pos, system.types)
+    table = configs.lj_param_table(configs.ka_params(2.5), 2)
+    f_ref, _ = oracle.forces_cells(pos4, system.box.astype(np.float32).astype(np.float64), 2, table, 2)
+    out = {}
+    for label, fp64 in (("fp32_kernel", False), ("fp64_kernel", True)):
+        pot.SetFP64Forces(fp64)
+        sample.CalcF()
+        f = sample.GetForces().astype(np.float64)
+        d = np.linalg.norm(f - f_ref[:, :3], axis=1)
+        mag = np.linalg.norm(f_ref[:, :3], axis=1)
+        rms = float(np.sqrt(np.mean(mag ** 2)))
+        guarded = d / np.maximum(mag, rms)
+        unguarded = d / np.maximum(mag, 1e-30)
+        out[label] = {"guarded_max": float(guarded.max()), "guarded_p99": float(np.quantile(guarded, 0.99)),
+                      "unguarded_median": float(np.median(unguarded))}
+    pot.SetFP64Forces(False)
+    sample.CalcF()
+    return out
+
+
+def nve_drift(sample, nsteps: int, dt: float):
+    """Total energy per particle over nsteps NVE steps from the current state
+    (momentum reset off): (E_end - E_start) / N per 1000 steps."""
+    import rumd_amd.rumd as rumd
+    nve = rumd.IntegratorNVE(timeStep=dt)
+    sample.SetIntegrator(nve)
+    n = sample.GetNumberOfParticles()
+    sample.CalcF()
+    e0 = (sample.GetPotentialEnergy() + nve.GetKineticEnergy()) / n
+    sample.RunSteps(0, nsteps, 10, 0)
+    sample.CalcF()
+    e1 = (sample.GetPotentialEnergy() + nve.GetKineticEnergy()) / n
+    return {"e_start": e0, "e_end": e1, "drift_per_1000_steps": (e1 - e0) * 1000.0 / nsteps, "steps": nsteps}
+
+
+def make_batch(systems, temps, dt: float, tau: float, skin: float):
+    import rumd_amd.rumd as rumd
+    from rumd_amd import configs
+    b = rumd.ReplicaBatch(2)
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedForce)
+    pot.SetVerbose(False)
+    for (i, j, sg, ep, rc) in configs.ka_params(2.5):
+        pot.SetParams(i, j, Sigma=sg, Epsilon=ep, Rcut=rc)
+    for s in systems:
+        b.AddReplica(s.positions.astype(np.float32), s.types.astype(np.int32), s.velocities.astype(np.float32),
+                     float(s.box[0]))
+    b.SetPotential(pot, skin)
+    b.SetTimeStep(dt)
+    for r, t in enumerate(temps):
+        b.SetThermostat(r, t, tau)
+    return b
+
+
+def batched_leg(nrep: int, warmup: int, steps: int, sample_reps: int, single_steps: int):
+    """KA replicas (N ~ U[256, 8192], T ~ U[0.4, 2.0], rho 1.2, SF, NH tau 0.2,
+    dt 0.0025, skin 0.3): all of them in one ReplicaBatch, against the same
+    replicas run one at a time through the single-sample engine (a Sample
+    each, RUMD's default list method), timed on an evenly spaced subset."""
+    import rumd_amd.rumd as rumd
+    from rumd_amd import configs, conf_io
+    systems, temps = configs.ka_replica_set(nrep)
+    b = make_batch(systems, temps, 0.0025, 0.2, 0.3)
+    ntot = b.GetTotalParticles()
+    b.RunSteps(warmup, 10)
+    t0 = time.perf_counter()
+    b.RunSteps(steps, 10)
+    el = time.perf_counter() - t0
+    batched = ntot * steps / el
+    print(f"[bench] batched {nrep} replicas, N_total={ntot}: {steps} steps in {el:.3f} s -> "
+          f"{batched / 1e9:.3f} G atom-steps/s", file=sys.stderr, flush=True)
+    picks = list(range(0, nrep, max(1, nrep // sample_reps)))[:sample_reps]
+    atom_steps, seconds = 0.0, 0.0
+    for r in picks:
+        s = systems[r]
+        sample = rumd.Sample(0, 0)
+        sample.SetVerbose(False)
+        sample.SetMixedPrecisionPositions(False)
+        sample.SetConfiguration(conf_io.Configuration(n=s.n, box=s.box, types=s.types, positions=s.positions,
+                                                      masses=s.masses, velocities=s.velocities,
+                                                      images=np.zeros_like(s.positions)))
+        pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedForce)
+        pot.SetVerbose(False)
+        for (i, j, sg, ep, rc) in configs.ka_params(2.5):
+            pot.SetParams(i, j, Sigma=sg, Epsilon=ep, Rcut=rc)
+        pot.SetNbListSkin(0.3)
+        sample.SetPotential(pot)
+        sample.SetIntegrator(rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=temps[r],
+                                                thermostatRelaxationTime=0.2))
+        sample.RunSteps(0, 20, 10, 0)
+        t0 = time.perf_counter()
+        sample.RunSteps(20, single_steps, 10, 0)
+        seconds += time.perf_counter() - t0
+        atom_steps += s.n * single_steps
+    single = atom_steps / seconds
+    print(f"[bench] one at a time ({len(picks)} replicas sampled): {single / 1e9:.4f} G atom-steps/s",
+          file=sys.stderr, flush=True)
+    return {
+        "value": round(batched, 1),
+        "unit": "atom-timesteps/s",
+        "replicas": nrep,
+        "particles": int(ntot),
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(el / steps * 1e3, 4),
+        "one_at_a_time": {"value": round(single, 1), "replicas_timed": len(picks), "steps_each": single_steps,
+                          "how": "a Sample per replica (default list method), graph blocks of 10, timed per replica "
+                                 "and aggregated over an evenly spaced subset"},
+        "speedup": round(batched / single, 2),
+        "workload": "KA replicas N~U[256,8192] (seed 4001), T~U[0.4,2.0] (seed 4002), rho=1.2, SF rc=2.5, "
+                    "NH tau=0.2, dt=0.0025, skin=0.3, FP32 positions",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-counters", type=int, default=0, metavar="N",
@@ -160,6 +281,10 @@ def main() -> int:
     ap.add_argument("--force-reps", type=int, default=50)
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="only the headline run (no accuracy, NVE, FP32, batch)")
+    ap.add_argument("--replicas", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--single-replicas", type=int, default=32, help="replicas timed one at a time")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -200,7 +325,7 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": read_traffic(ROOT / "profiles"),
-        "kernel": "k_lj_nbl (LJ SF force over the neighbour list)",
+        "kernel": "k_lj_tiles<SF> (LJ force over the cell-tile lists)",
         "kernel_ms": round(force_ms, 5),
         "bytes_per_atom": round(bytes_per_atom, 2),
     }
@@ -208,6 +333,29 @@ def main() -> int:
     step_bytes = bytes_per_atom + 64 + 48 + 16 + (4 * nbar + 260) / (kbar if kbar else float("inf"))
     pe = sample.GetPotentialEnergy() / n
     ke = itg.GetKineticEnergy() / n
+
+    extras = {}
+    if not args.no_extras:
+        print("[bench] force accuracy vs the FP64 oracle ...", file=sys.stderr, flush=True)
+        extras["force_error_vs_fp64_oracle"] = force_accuracy(sample, pot, system)
+        print("[bench] NVE drift ...", file=sys.stderr, flush=True)
+        drift = {args.precision: nve_drift(sample, 1000, 0.0025)}
+        if args.precision == "mixed":
+            s32, p32, _ = make_sample(conf, False)
+            s32.RunSteps(0, args.warmup, args.block, 100)
+            s32.Synchronize()
+            t0 = time.perf_counter()
+            s32.RunSteps(args.warmup, args.steps, args.block, 100)
+            el32 = time.perf_counter() - t0
+            extras["fp32_positions"] = {"value": round(n * args.steps / el32, 1), "unit": "atom-timesteps/s",
+                                        "ms_per_step": round(el32 / args.steps * 1e3, 5)}
+            print(f"[bench] FP32 positions: {n * args.steps / el32 / 1e9:.3f} G atom-steps/s", file=sys.stderr,
+                  flush=True)
+            drift["fp32"] = nve_drift(s32, 1000, 0.0025)
+            del s32, p32
+        extras["nve_drift"] = drift
+        print("[bench] batched replicas ...", file=sys.stderr, flush=True)
+        extras["batched"] = batched_leg(args.replicas, 100, args.batch_steps, args.single_replicas, 100)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -245,6 +393,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    line.update(extras)
     print(json.dumps(line))
     return 0
 

@@ -118,6 +118,8 @@ typedef struct rumdx_nblist {
   uint16_t* lane_order_dev; /* sorted method: nvp, written by the build */
   uint32_t* tile_halo_dev;  /* sorted method: ntiles x halo_cap slots of each tile's halo, written by the build */
   uint32_t* tile_info_dev;  /* sorted method: ntiles x 4, written by the build */
+  uint32_t lanes_per_particle; /* 1, 2, 4 or 8 lanes share a particle's list in the force
+                                  kernels (RUMD's threads per particle, tp); 0 = 1 */
 } rumdx_nblist;
 
 /* Cells per tile edge and the largest LDS halo a launch may request. */

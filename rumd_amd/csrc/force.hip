@@ -149,74 +149,104 @@ __device__ __forceinline__ void store_result(uint32_t i, Acc& a, float4* __restr
   }
 }
 
-// Neighbour-list force kernel.  fb.cell_rank == nullptr means the list was
-// built by the N2 method, so an overflowed particle falls back to all pairs.
+// Sums a particle's partial accumulators over the 2^lg lanes that share it
+// (a fixed butterfly, so the result does not depend on timing).  Called by
+// every lane of the wave.
+template <bool STR>
+__device__ __forceinline__ void combine_lanes(Acc& a, uint32_t lg) {
+  for (uint32_t o = 1; o < (1u << lg); o <<= 1) {
+    a.fx += __shfl_xor(a.fx, o, 64);
+    a.fy += __shfl_xor(a.fy, o, 64);
+    a.fz += __shfl_xor(a.fz, o, 64);
+    a.u += __shfl_xor(a.u, o, 64);
+    a.vir += __shfl_xor(a.vir, o, 64);
+    if (STR) {
+      a.sxx += __shfl_xor(a.sxx, o, 64);
+      a.syy += __shfl_xor(a.syy, o, 64);
+      a.szz += __shfl_xor(a.szz, o, 64);
+      a.syz += __shfl_xor(a.syz, o, 64);
+      a.sxz += __shfl_xor(a.sxz, o, 64);
+      a.sxy += __shfl_xor(a.sxy, o, 64);
+    }
+  }
+}
+
+// Neighbour-list force kernel for u32 slot lists (N2 method).  2^lgp lanes
+// share a particle (RUMD's threads per particle): lane part takes entries
+// part, part + 2^lgp, ..., and a fixed butterfly combines them.  An
+// overflowed list falls back to the cell stencil (fb.cell_rank != nullptr)
+// or to all pairs.
 template <int CUT, bool STR, bool INIT>
 __global__ __launch_bounds__(kBlock) void k_lj_nbl(const float4* __restrict__ r, float4* __restrict__ f,
                                                    float4* __restrict__ w, float4* __restrict__ sts,
                                                    const uint32_t* __restrict__ nbl,
                                                    const uint32_t* __restrict__ num_nbrs,
                                                    const float* __restrict__ params, uint32_t nt, Box bx,
-                                                   uint32_t n, uint32_t nvp, uint32_t max_nbrs, CellFallback fb) {
+                                                   uint32_t n, uint32_t nvp, uint32_t max_nbrs, CellFallback fb,
+                                                   uint32_t lgp) {
   extern __shared__ float4 s_tab[];
   load_table(s_tab, params, nt);
   __syncthreads();
 
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t i = b * kBlock + threadIdx.x;
-  if (i >= n) return;
-
-  const float4 ri = r[i];
-  const float4* prow = s_tab + 2 * __float_as_uint(ri.w) * nt;
-  const double xi = ri.x, yi = ri.y, zi = ri.z;
+  const uint32_t gid = b * kBlock + threadIdx.x;
+  const uint32_t i = gid >> lgp, part = gid & ((1u << lgp) - 1u), step = 1u << lgp;
+  const bool live = i < n;
   Acc a;
-  const uint32_t nn = num_nbrs[i];
-  if (nn < max_nbrs) {
-    const uint32_t* col = nbl + i;
-    uint32_t k = 0;
-    for (; k + 4 <= nn; k += 4) {
-      const uint32_t j0 = __builtin_nontemporal_load(col);
-      const uint32_t j1 = __builtin_nontemporal_load(col + nvp);
-      const uint32_t j2 = __builtin_nontemporal_load(col + 2 * static_cast<size_t>(nvp));
-      const uint32_t j3 = __builtin_nontemporal_load(col + 3 * static_cast<size_t>(nvp));
-      col += 4 * static_cast<size_t>(nvp);
-      const float4 r0 = r[j0], r1 = r[j1], r2 = r[j2], r3 = r[j3];
-      pair_term<CUT, STR>(xi, yi, zi, r0, prow, bx, a);
-      pair_term<CUT, STR>(xi, yi, zi, r1, prow, bx, a);
-      pair_term<CUT, STR>(xi, yi, zi, r2, prow, bx, a);
-      pair_term<CUT, STR>(xi, yi, zi, r3, prow, bx, a);
-    }
-    for (; k < nn; ++k) {
-      const uint32_t j = __builtin_nontemporal_load(col);
-      col += nvp;
-      pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
-    }
-  } else if (fb.cell_rank != nullptr) {
-    // Overflowed list: rescan the 5x5x5 stencil around the cell the particle
-    // was binned in at the last build.  Every pair now inside rc was inside
-    // rc + skin then, so it is in that stencil.
-    const uint32_t lin = fb.slot_cell[i];
-    const int cx = static_cast<int>(lin % fb.nc0);
-    const int cy = static_cast<int>((lin / fb.nc0) % fb.nc1);
-    const int cz = static_cast<int>(lin / (static_cast<uint32_t>(fb.nc0) * fb.nc1));
-    for (int dz = -2; dz <= 2; ++dz) {
-      const int oz = (cz + dz + fb.nc2) % fb.nc2;
-      for (int dy = -2; dy <= 2; ++dy) {
-        const int oy = (cy + dy + fb.nc1) % fb.nc1;
-        for (int dx = -2; dx <= 2; ++dx) {
-          const int ox = (cx + dx + fb.nc0) % fb.nc0;
-          const uint32_t rank = fb.cell_rank[ox + oy * fb.nc0 + oz * fb.nc0 * fb.nc1];
-          const uint32_t s = fb.cell_start[rank], e = fb.cell_start[rank + 1];
-          for (uint32_t j = s; j < e; ++j)
-            if (j != i) pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
+  if (live) {
+    const float4 ri = r[i];
+    const float4* prow = s_tab + 2 * __float_as_uint(ri.w) * nt;
+    const double xi = ri.x, yi = ri.y, zi = ri.z;
+    const uint32_t nn = num_nbrs[i];
+    if (nn < max_nbrs) {
+      const size_t stride = static_cast<size_t>(nvp) << lgp;
+      const uint32_t* col = nbl + static_cast<size_t>(part) * nvp + i;
+      uint32_t k = part;
+      for (; k + 3 * step < nn; k += 4 * step) {
+        const uint32_t j0 = __builtin_nontemporal_load(col);
+        const uint32_t j1 = __builtin_nontemporal_load(col + stride);
+        const uint32_t j2 = __builtin_nontemporal_load(col + 2 * stride);
+        const uint32_t j3 = __builtin_nontemporal_load(col + 3 * stride);
+        col += 4 * stride;
+        const float4 r0 = r[j0], r1 = r[j1], r2 = r[j2], r3 = r[j3];
+        pair_term<CUT, STR>(xi, yi, zi, r0, prow, bx, a);
+        pair_term<CUT, STR>(xi, yi, zi, r1, prow, bx, a);
+        pair_term<CUT, STR>(xi, yi, zi, r2, prow, bx, a);
+        pair_term<CUT, STR>(xi, yi, zi, r3, prow, bx, a);
+      }
+      for (; k < nn; k += step) {
+        const uint32_t j = __builtin_nontemporal_load(col);
+        col += stride;
+        pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
+      }
+    } else if (part == 0 && fb.cell_rank != nullptr) {
+      // Overflowed list: rescan the 5x5x5 stencil around the cell the particle
+      // was binned in at the last build.  Every pair now inside rc was inside
+      // rc + skin then, so it is in that stencil.
+      const uint32_t lin = fb.slot_cell[i];
+      const int cx = static_cast<int>(lin % fb.nc0);
+      const int cy = static_cast<int>((lin / fb.nc0) % fb.nc1);
+      const int cz = static_cast<int>(lin / (static_cast<uint32_t>(fb.nc0) * fb.nc1));
+      for (int dz = -2; dz <= 2; ++dz) {
+        const int oz = (cz + dz + fb.nc2) % fb.nc2;
+        for (int dy = -2; dy <= 2; ++dy) {
+          const int oy = (cy + dy + fb.nc1) % fb.nc1;
+          for (int dx = -2; dx <= 2; ++dx) {
+            const int ox = (cx + dx + fb.nc0) % fb.nc0;
+            const uint32_t rank = fb.cell_rank[ox + oy * fb.nc0 + oz * fb.nc0 * fb.nc1];
+            const uint32_t s = fb.cell_start[rank], e = fb.cell_start[rank + 1];
+            for (uint32_t j = s; j < e; ++j)
+              if (j != i) pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
+          }
         }
       }
+    } else if (part == 0) {
+      for (uint32_t j = 0; j < n; ++j)
+        if (j != i) pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
     }
-  } else {
-    for (uint32_t j = 0; j < n; ++j)
-      if (j != i) pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
   }
-  store_result<STR, INIT>(i, a, f, w, sts);
+  combine_lanes<STR>(a, lgp);
+  if (live && part == 0) store_result<STR, INIT>(i, a, f, w, sts);
 }
 
 // All-pairs force kernel (neighbour-list method "none", the nbl == 0 branch of
@@ -252,38 +282,16 @@ __global__ __launch_bounds__(kBlock) void k_lj_all(const float4* __restrict__ r,
   if (live) store_result<STR, INIT>(i, a, f, w, sts);
 }
 
-// Sums a particle's partial accumulators over the 2^lg lanes that share it
-// (a fixed butterfly, so the result does not depend on timing).  Called by
-// every lane of the wave.
-template <bool STR>
-__device__ __forceinline__ void combine_lanes(Acc& a, uint32_t lg) {
-  for (uint32_t o = 1; o < (1u << lg); o <<= 1) {
-    a.fx += __shfl_xor(a.fx, o, 64);
-    a.fy += __shfl_xor(a.fy, o, 64);
-    a.fz += __shfl_xor(a.fz, o, 64);
-    a.u += __shfl_xor(a.u, o, 64);
-    a.vir += __shfl_xor(a.vir, o, 64);
-    if (STR) {
-      a.sxx += __shfl_xor(a.sxx, o, 64);
-      a.syy += __shfl_xor(a.syy, o, 64);
-      a.szz += __shfl_xor(a.szz, o, 64);
-      a.syz += __shfl_xor(a.syz, o, 64);
-      a.sxz += __shfl_xor(a.sxz, o, 64);
-      a.sxy += __shfl_xor(a.sxy, o, 64);
-    }
-  }
-}
-
 // Neighbour position of a byte entry: the halo positions start the dynamic LDS.
 __device__ __forceinline__ float4 halo_pos(const float4* s_pos, uint32_t entry) {
   return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pos) + entry);
 }
 
 // Forces on the particles of one tile whose halo positions are in LDS.
-// Waves take the tile's particles 64 at a time in the build's lane order
-// (descending list length, so a wave's lists have similar lengths); the last,
-// partial chunk spreads each list over 2^lg lanes (groups part, part + 2^lg,
-// ...), combined by a butterfly.  FAST: the halo was staged in the shifted
+// Waves take the tile's particles 64 >> lgp at a time in the build's lane
+// order (descending list length, so a wave's lists have similar lengths);
+// each list is spread over 2^lg lanes (groups part, part + 2^lg, ...; lg =
+// lgp, more in a tile's last, partial chunk), combined by a butterfly.  FAST: the halo was staged in the shifted
 // frame and every list is complete, so every pair takes the direct FP32
 // difference.  Otherwise a wrapping halo forms displacements in FP64, and
 // overflowed lists, or a halo beyond the LDS capacity, rescan the particle's
@@ -297,12 +305,15 @@ __device__ __forceinline__ void tile_particles(uint32_t slot0, uint32_t slot1, b
                                                const uint32_t* __restrict__ slot_cell,
                                                const uint16_t* __restrict__ lane_order, const float4* s_pos,
                                                const float4* s_tab, uint32_t nt, const Box& bx, const TileGeom& tg,
-                                               const HaloLds& L, uint32_t nvp, uint32_t max_nbrs) {
+                                               const HaloLds& L, uint32_t nvp, uint32_t max_nbrs,
+                                               uint32_t lgp) {
   const uint32_t np = slot1 - slot0;
   const uint32_t lane = threadIdx.x & 63u;
-  for (uint32_t c0 = (threadIdx.x >> 6) * 64u; c0 < np; c0 += kBlock) {
-    const uint32_t cnt = (np - c0) < 64u ? (np - c0) : 64u;
-    const uint32_t lg = cnt > 32u ? 0u : cnt > 16u ? 1u : cnt > 8u ? 2u : 3u;
+  const uint32_t per_wave = 64u >> lgp;  // particles per wave and chunk: 2^lgp lanes each
+  for (uint32_t c0 = (threadIdx.x >> 6) * per_wave; c0 < np; c0 += (kBlock / 64) * per_wave) {
+    const uint32_t cnt = (np - c0) < per_wave ? (np - c0) : per_wave;
+    uint32_t lg = lgp;  // a partial chunk spreads its lists further, up to 8 lanes
+    while (lg < 3u && (cnt << (lg + 1u)) <= 64u) ++lg;
     const uint32_t part = lane & ((1u << lg) - 1u);
     const uint32_t pl = lane >> lg;
     const bool live = pl < cnt;
@@ -390,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_tiles(const float4* __restrict__ 
                                                      const uint32_t* __restrict__ slot_cell,
                                                      const uint16_t* __restrict__ lane_order,
                                                      const float* __restrict__ params, uint32_t nt, Box bx, TileGeom tg,
-                                                     uint32_t nvp, uint32_t max_nbrs) {
+                                                     uint32_t nvp, uint32_t max_nbrs, uint32_t lgp) {
   __shared__ HaloLds L;
   extern __shared__ float4 s_dyn[];
   float4* s_pos = s_dyn;
@@ -409,13 +420,14 @@ __global__ __launch_bounds__(kBlock) void k_lj_tiles(const float4* __restrict__ 
     }
     __syncthreads();
     tile_particles<CUT, STR, INIT, true>(info.z, info.w, true, false, info.y, r, f, w, sts, nbl16, num_nbrs,
-                                         slot_cell, lane_order, s_pos, s_tab, nt, bx, tg, L, nvp, max_nbrs);
+                                         slot_cell, lane_order, s_pos, s_tab, nt, bx, tg, L, nvp, max_nbrs, lgp);
     return;
   }
   stage_halo(t, tg, r, s_pos, L);
   const Halo& h = L.h;
   tile_particles<CUT, STR, INIT, false>(h.slot0, h.slot1, h.npart <= tg.halo_cap, h.wraps, 0u, r, f, w, sts, nbl16,
-                                        num_nbrs, slot_cell, lane_order, s_pos, s_tab, nt, bx, tg, L, nvp, max_nbrs);
+                                        num_nbrs, slot_cell, lane_order, s_pos, s_tab, nt, bx, tg, L, nvp, max_nbrs,
+                                        lgp);
 }
 
 template <int CUT, bool STR, bool INIT>
@@ -427,6 +439,8 @@ int launch_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* par
   float4* w = reinterpret_cast<float4*>(p->w_dev);
   float4* sts = reinterpret_cast<float4*>(p->sts_dev);
   const size_t tab_bytes = sizeof(float4) * 2 * nt * nt;
+  const uint32_t lpp = nb != nullptr ? nb->lanes_per_particle : 1u;
+  const uint32_t lgp = lpp >= 8u ? 3u : lpp >= 4u ? 2u : lpp >= 2u ? 1u : 0u;
   if (nb != nullptr && nb->nbl16_dev != nullptr) {
     const TileGeom tg{nb->nc[0],           nb->nc[1],
                       nb->nc[2],           nb->cell_rank_dev,
@@ -436,7 +450,7 @@ int launch_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* par
                       reinterpret_cast<uint4*>(nb->tile_info_dev)};
     hipLaunchKernelGGL((k_lj_tiles<CUT, STR, INIT>), dim3(nb->ntiles), dim3(kBlock),
                        tab_bytes + sizeof(float4) * nb->halo_cap, st, r, f, w, sts, nb->nbl16_dev, nb->num_nbrs_dev,
-                       nb->slot_cell_dev, nb->lane_order_dev, params, nt, bx, tg, p->nvp, nb->max_nbrs);
+                       nb->slot_cell_dev, nb->lane_order_dev, params, nt, bx, tg, p->nvp, nb->max_nbrs, lgp);
   } else if (nb != nullptr) {
     CellFallback fb{};
     if (nb->cell_rank_dev != nullptr) {
@@ -447,8 +461,8 @@ int launch_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* par
       fb.nc1 = nb->nc[1];
       fb.nc2 = nb->nc[2];
     }
-    hipLaunchKernelGGL((k_lj_nbl<CUT, STR, INIT>), dim3(nblk), dim3(kBlock), tab_bytes, st, r, f, w, sts,
-                       nb->nbl_dev, nb->num_nbrs_dev, params, nt, bx, p->n, p->nvp, nb->max_nbrs, fb);
+    hipLaunchKernelGGL((k_lj_nbl<CUT, STR, INIT>), dim3(blocks_for(p->n << lgp)), dim3(kBlock), tab_bytes, st, r, f,
+                       w, sts, nb->nbl_dev, nb->num_nbrs_dev, params, nt, bx, p->n, p->nvp, nb->max_nbrs, fb, lgp);
   } else {
     hipLaunchKernelGGL((k_lj_all<CUT, STR, INIT>), dim3(nblk), dim3(kBlock), tab_bytes + sizeof(float4) * kBlock,
                        st, r, f, w, sts, params, nt, bx, p->n);
@@ -486,6 +500,11 @@ extern "C" int rumdx_pair_lj(const rumdx_particles* p, const rumdx_nblist* nb, c
        nb->slot_cell_dev == nullptr || nb->lane_order_dev == nullptr || nb->tile_halo_dev == nullptr ||
        nb->tile_info_dev == nullptr || nb->halo_cap > RUMDX_HALO_CAP_MAX))
     return fail(RUMDX_E_INVALID, "rumdx_pair_lj: tile arrays not allocated");
+  if (nb != nullptr) {
+    const uint32_t lpp = nb->lanes_per_particle;
+    if (lpp > 8u || (lpp & (lpp - 1u)) != 0u || static_cast<uint64_t>(p->n) * (lpp ? lpp : 1u) >= (1ull << 32))
+      return fail(RUMDX_E_INVALID, "rumdx_pair_lj: lanes_per_particle must be 0, 1, 2, 4 or 8");
+  }
   const Box bx = make_box(box);
   const hipStream_t st = as_stream(stream);
   const bool init = initialize != 0, str = calc_stresses != 0;

@@ -134,6 +134,9 @@ void NeighborList::Allocate() {
   desc_.max_nbrs = max_nbrs_;
   desc_.ntypes = ntypes_;
   desc_.skin = skin_;
+  // the sample's threads per particle (Sample::SetPB_TP), as a power of two <= 8
+  const unsigned tp = sample_->GetThreadsPerParticle();
+  desc_.lanes_per_particle = tp >= 8 ? 8u : tp >= 4 ? 4u : tp >= 2 ? 2u : 1u;
   if (method == SORT) {
     int pair_pots = 0;
     for (Potential* p : sample_->GetPotentials())

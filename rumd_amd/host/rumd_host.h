@@ -180,6 +180,8 @@ class NeighborList {
   // the density); tiles with larger halos compute from global memory.
   void SetHaloCapacity(unsigned cap);
   void SizeTileHalo();
+  // Lanes sharing one particle's list in the force kernels: the sample's tp.
+  unsigned GetLanesPerParticle() const { return desc_.lanes_per_particle; }
   unsigned GetHaloCapacity() const { return halo_cap_; }
   // The list in RUMD's format (u32 slots, column-major), for checks and tools.
   std::vector<uint32_t> ExportList(std::vector<uint32_t>& counts);

@@ -58,6 +58,7 @@ class NbList(C.Structure):
         ("lane_order_dev", C.c_void_p),
         ("tile_halo_dev", C.c_void_p),
         ("tile_info_dev", C.c_void_p),
+        ("lanes_per_particle", C.c_uint32),
     ]
 
 

@@ -45,6 +45,15 @@ def test_host_side_helpers_without_gpu():
     assert lib.rumdx_pair_lj(None, None, None, 2, configs.box8(10.0), 1, 1, 0, None) == -1
     assert lib.rumdx_pair_lj_fp64(None, None, None, 2, configs.box8(10.0), 1, 1, 0, None) == -1
     assert b"null" in lib.rumdx_last_error()
+    # lanes per particle must be a power of two <= 8: rejected before any launch
+    p = capi.Particles()
+    p.r_dev = p.f_dev = p.w_dev = 16
+    p.n, p.nvp = 10, 32
+    nb = capi.NbList()
+    nb.nbl_dev = nb.num_nbrs_dev = 16
+    nb.max_nbrs, nb.lanes_per_particle = 8, 3
+    assert lib.rumdx_pair_lj(C.byref(p), C.byref(nb), 16, 2, configs.box8(10.0), 1, 1, 0, None) == -1
+    assert b"lanes_per_particle" in lib.rumdx_last_error()
     bad = np.zeros(3, dtype=np.int32)
     assert lib.rumdx_cell_rank_table(bad, rank) == -1
 

@@ -205,3 +205,42 @@ def test_error_paths():
         s.RunSteps(0, 10, 10, 0)  # no integrator / potential
     with pytest.raises(ValueError):
         s.SetPositions(np.zeros((10, 3), np.float32))
+
+
+def test_autotune_tunes_reuses_and_keeps_the_trajectory(tmp_path):
+    """Autotune times short runs over tp, list method and skin, saves the
+    choice, reuses it for a matching simulation, and leaves the state as it
+    was: the run after tuning equals a fresh run with those parameters."""
+    from rumd_amd.Autotune import Autotune
+
+    sysm = configs.ka_random_lattice(2000, density=1.2, temperature=0.45, seed=77)
+
+    def make_sim():
+        sim = Simulation(conf_from_system(sysm), verbose=False)
+        pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedForce)
+        pot.SetVerbose(False)
+        for (i, j, sg, ep, rc) in configs.ka_params(2.5):
+            pot.SetParams(i, j, Sigma=sg, Epsilon=ep, Rcut=rc)
+        pot.SetNbListSkin(0.3)
+        sim.SetPotential(pot)
+        sim.SetIntegrator(rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.45,
+                                             thermostatRelaxationTime=0.2))
+        return sim, pot
+
+    path = tmp_path / "autotune.json"
+    sim, pot = make_sim()
+    print("[case] autotune KA N=2000", flush=True)
+    tech = Autotune(auto_filename=path, trial_seconds=0.02).Tune(sim)
+    assert tech["tp"] in (1, 2, 4, 8) and tech["NB_method"] in ("none", "n2", "sort") and tech["skin"] > 0
+    assert path.exists()
+    assert sim.sample.GetThreadsPerParticle() == tech["tp"] and pot.GetNbListSkin() == pytest.approx(tech["skin"])
+    at2 = Autotune(auto_filename=path)
+    sim2, _ = make_sim()
+    assert at2.Tune(sim2) == tech and at2.trials == []          # reused, nothing timed
+    sim.Run(50)
+    fresh, fpot = make_sim()
+    fresh.SetPB_TP(tech["pb"], tech["tp"])
+    fpot.SetNB_Method(tech["NB_method"])
+    fpot.SetNbListSkin(tech["skin"])
+    fresh.Run(50)
+    np.testing.assert_array_equal(sim.sample.GetPositions(), fresh.sample.GetPositions())

@@ -100,6 +100,23 @@ def test_forces_ka_sorted_65536(cutoff):
     assert abs(w_gpu - w_ref) <= 1e-5 * abs(w_ref) + 1e-3
 
 
+@pytest.mark.parametrize("method", ["n2", "sorted"])
+@pytest.mark.parametrize("tp", [2, 4, 8])
+def test_forces_lanes_per_particle(golden_dir, method, tp):
+    """Several lanes per particle (RUMD's tp): each lane takes every tp-th
+    entry and a butterfly combines them; forces stay within tolerance."""
+    table = ka_table()
+    c, pos4, _, g = _golden_system(golden_dir, "KA_N1000_T0.466.xyz.gz", table, method)
+    g.nb.lanes_per_particle = tp
+    print(f"[case] KA_N1000 SP method={method} tp={tp}", flush=True)
+    g.calc_f(SP)
+    f = unsort(g.get("f"), g.unsorted_index())
+    f_ref, _ = oracle.forces_brute(pos4, c.box, 2, table, SP)
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+    np.testing.assert_allclose(f[:, 3], f_ref[:, 3], rtol=0, atol=2e-5)
+
+
 def test_nblist_sets_bit_exact_n2(golden_dir):
     table = ka_table()
     c, pos4, _, g = _golden_system(golden_dir, "KA_N1000_T0.466.xyz.gz", table, "n2")
