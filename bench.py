@@ -147,28 +147,40 @@ def counter_calibration(n: int, reps: int) -> dict:
 
 
 def force_accuracy(sample, pot, system):
-    """Per-particle force error of the FP32 kernel and of the all-FP64 kernel
-    against the FP64 oracle (linked cells) on the current state."""
+    """Per-particle force error against the FP64 oracle (linked cells) on the
+    current state.  The FP32 kernel computes from the float positions r, the
+    all-FP64 kernel from r + rlo (the FP64 positions of mixed mode): each is
+    compared with the oracle at the positions it was given, and the FP32
+    kernel also against the oracle at the FP64 positions (its total error)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
     from rumd_amd import configs
     sample.Synchronize()
-    pos = sample.GetPositions()
-    pos4 = configs.pack_pos4(pos, system.types)
     table = configs.lj_param_table(configs.ka_params(2.5), 2)
-    f_ref, _ = oracle.forces_cells(pos4, system.box.astype(np.float32).astype(np.float64), 2, table, 2)
-    out = {}
-    for label, fp64 in (("fp32_kernel", False), ("fp64_kernel", True)):
-        pot.SetFP64Forces(fp64)
-        sample.CalcF()
-        f = sample.GetForces().astype(np.float64)
-        d = np.linalg.norm(f - f_ref[:, :3], axis=1)
-        mag = np.linalg.norm(f_ref[:, :3], axis=1)
+    box = system.box.astype(np.float32).astype(np.float64)
+
+    def ref(pos):
+        return oracle.forces_cells(configs.pack_pos4(pos, system.types), box, 2, table, 2)[0][:, :3]
+
+    def errors(f, fr):
+        d = np.linalg.norm(f - fr, axis=1)
+        mag = np.linalg.norm(fr, axis=1)
         rms = float(np.sqrt(np.mean(mag ** 2)))
         guarded = d / np.maximum(mag, rms)
-        unguarded = d / np.maximum(mag, 1e-30)
-        out[label] = {"guarded_max": float(guarded.max()), "guarded_p99": float(np.quantile(guarded, 0.99)),
-                      "unguarded_median": float(np.median(unguarded))}
+        return {"guarded_max": float(guarded.max()), "guarded_p99": float(np.quantile(guarded, 0.99)),
+                "unguarded_median": float(np.median(d / np.maximum(mag, 1e-30)))}
+
+    f_ref32 = ref(sample.GetPositions().astype(np.float64))
+    f_ref64 = ref(sample.GetPositionsFP64())
+    out = {}
+    pot.SetFP64Forces(False)
+    sample.CalcF()
+    f32 = sample.GetForces().astype(np.float64)
+    out["fp32_kernel"] = errors(f32, f_ref32)
+    out["fp32_kernel_vs_fp64_positions"] = errors(f32, f_ref64)
+    pot.SetFP64Forces(True)
+    sample.CalcF()
+    out["fp64_kernel"] = errors(sample.GetForces().astype(np.float64), f_ref64)
     pot.SetFP64Forces(False)
     sample.CalcF()
     return out

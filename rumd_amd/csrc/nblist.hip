@@ -337,6 +337,8 @@ struct HomeArrays {
 // cell being built and a window of its candidates (LDS offsets, in stencil
 // order).
 constexpr uint32_t kCandWindow = 512;
+// Type counts up to this keep a candidate's cut-offs in registers.
+constexpr uint32_t kRegTypes = 4;
 struct WaveBuild {
   uint16_t cand[kCandWindow];
   uint16_t seg_start[128];  // LDS offset of the segment's first particle
@@ -354,10 +356,11 @@ __device__ __forceinline__ void wave_lds_sync() {
 // which holds cnt entries so far; returns the new count (uniform).
 __device__ __forceinline__ uint32_t append_chunk(bool acc, uint32_t entry, uint32_t cnt, uint16_t* __restrict__ nbl16,
                                                  uint32_t i, uint32_t nvp, uint32_t max_nbrs) {
-  const uint64_t mask = __ballot(acc);
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
   const uint32_t k = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-  if (acc && k < max_nbrs) nbl16[(static_cast<size_t>(k >> 3) * nvp + i) * 8 + (k & 7u)] = static_cast<uint16_t>(entry);
+  if (acc && k < max_nbrs)  // group (k/8)*nvp + i, element k%8; max_nbrs * nvp < 2^32
+    nbl16[static_cast<size_t>((k >> 3) * nvp + i) * 8 + (k & 7u)] = static_cast<uint16_t>(entry);
   return cnt + static_cast<uint32_t>(__popcll(mask));
 }
 
@@ -367,7 +370,7 @@ __device__ __forceinline__ uint32_t append_chunk(bool acc, uint32_t entry, uint3
 // test them against one particle at a time; a ballot places the accepted
 // entries in candidate order.  Writes the lists and leaves each particle's
 // full count (which may exceed max_nbrs) in s_cnt[slot - slot0].
-template <bool WRAPS>
+template <bool WRAPS, bool REGCUT>
 __device__ __forceinline__ void build_cell_wave(const Halo& h, const HaloLds& L, const TileGeom& tg,
                                                 const float4* s_pos, const float* s_cut, uint32_t nt, const Box& bx,
                                                 int cx, int cy, int cz, WaveBuild& wb, uint32_t* s_cnt,
@@ -421,7 +424,9 @@ __device__ __forceinline__ void build_cell_wave(const Halo& h, const HaloLds& L,
   wave_lds_sync();
   for (uint32_t pg = 0; pg < m; pg += 64) {
     const uint32_t mg = (m - pg) < 64u ? (m - pg) : 64u;
-    uint32_t cnt_v = 0;  // lane l: count of particle pg + l
+    // lane l holds particle pg + l: its position and its list length so far
+    const float4 pv = lane < mg ? s_pos[coff + pg + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t cnt_v = 0;
     for (uint32_t w0 = 0; w0 < total; w0 += kCandWindow) {
       const uint32_t wn = (total - w0) < kCandWindow ? (total - w0) : kCandWindow;
       wave_lds_sync();  // the previous window has been read
@@ -432,35 +437,40 @@ __device__ __forceinline__ void build_cell_wave(const Halo& h, const HaloLds& L,
         for (uint32_t q = lo; q < hi; ++q) wb.cand[q - w0] = static_cast<uint16_t>(st + (q - pre));
       }
       wave_lds_sync();
-      for (uint32_t p = 0; p < mg; ++p) {
-        const uint32_t po = coff + pg + p;
-        const float4 pi = s_pos[po];
-        const float* crow = s_cut + __float_as_uint(pi.w) * nt;
-        const uint32_t i = gstart + pg + p;
-        uint32_t cnt = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt_v), static_cast<int>(p)));
-        // four 64-candidate chunks per pass: their LDS reads are issued
-        // together, so one round trip serves 256 candidates
-        for (uint32_t q0 = 0; q0 < wn; q0 += 4 * 64) {
-          uint32_t oo[4];
-          float4 rj[4];
-          float cut[4];
+      // Each 64-candidate chunk is loaded once and tested against every
+      // particle of the cell: the particle's data comes from its lane
+      // (readlane), the cut-offs of the candidate's type from registers.
+      for (uint32_t q0 = 0; q0 < wn; q0 += 64) {
+        const uint32_t q = q0 + lane;
+        const bool valid = q < wn;
+        const uint32_t oo = valid ? static_cast<uint32_t>(wb.cand[q]) : 0u;
+        const float4 rj = s_pos[oo];
+        const uint32_t tj = __float_as_uint(rj.w);
+        const uint32_t entry = encode_entry(oo, bytes);
+        float cuts[kRegTypes];
 #pragma unroll
-          for (uint32_t u = 0; u < 4; ++u) {
-            const uint32_t q = q0 + 64 * u + lane;
-            oo[u] = q < wn ? static_cast<uint32_t>(wb.cand[q]) : po;
+        for (uint32_t t = 0; t < kRegTypes; ++t) cuts[t] = (REGCUT && t < nt) ? s_cut[t * nt + tj] : 0.f;
+        for (uint32_t p = 0; p < mg; ++p) {
+          float4 pi;
+          pi.x = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pv.x), static_cast<int>(p)));
+          pi.y = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pv.y), static_cast<int>(p)));
+          pi.z = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pv.z), static_cast<int>(p)));
+          const uint32_t ti =
+              static_cast<uint32_t>(__builtin_amdgcn_readlane(__builtin_bit_cast(int, pv.w), static_cast<int>(p)));
+          const float d2 = WRAPS ? min_image_d2_f32(pi, rj, bx) : plain_d2_f32(pi, rj);
+          float cut;
+          if (REGCUT) {
+            cut = cuts[0];
+#pragma unroll
+            for (uint32_t t = 1; t < kRegTypes; ++t) cut = ti == t ? cuts[t] : cut;
+          } else {
+            cut = s_cut[ti * nt + tj];
           }
-#pragma unroll
-          for (uint32_t u = 0; u < 4; ++u) rj[u] = s_pos[oo[u]];
-#pragma unroll
-          for (uint32_t u = 0; u < 4; ++u) cut[u] = crow[__float_as_uint(rj[u].w)];
-#pragma unroll
-          for (uint32_t u = 0; u < 4; ++u) {
-            const float d2 = WRAPS ? min_image_d2_f32(pi, rj[u], bx) : plain_d2_f32(pi, rj[u]);
-            const bool acc = oo[u] != po && d2 < cut[u];
-            cnt = append_chunk(acc, encode_entry(oo[u], bytes), cnt, nbl16, i, nvp, max_nbrs);
-          }
+          const bool acc = valid && oo != coff + pg + p && d2 < cut;
+          uint32_t cnt = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt_v), static_cast<int>(p)));
+          cnt = append_chunk(acc, entry, cnt, nbl16, gstart + pg + p, nvp, max_nbrs);
+          cnt_v = lane == p ? cnt : cnt_v;
         }
-        cnt_v = lane == p ? cnt : cnt_v;
       }
     }
     if (lane < mg) s_cnt[gstart + pg + lane - h.slot0] = cnt_v;
@@ -545,12 +555,19 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
       cx -= cx >= tg.nc0 ? tg.nc0 : 0;
       cy -= cy >= tg.nc1 ? tg.nc1 : 0;
       cz -= cz >= tg.nc2 ? tg.nc2 : 0;
-      if (h.wraps)
-        build_cell_wave<true>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp, max_nbrs,
-                              bytes);
+      // REGCUT: a candidate's cut-offs against every type fit in registers
+      if (h.wraps && nt <= kRegTypes)
+        build_cell_wave<true, true>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp,
+                                    max_nbrs, bytes);
+      else if (h.wraps)
+        build_cell_wave<true, false>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp,
+                                     max_nbrs, bytes);
+      else if (nt <= kRegTypes)
+        build_cell_wave<false, true>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp,
+                                     max_nbrs, bytes);
       else
-        build_cell_wave<false>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp, max_nbrs,
-                               bytes);
+        build_cell_wave<false, false>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp,
+                                      max_nbrs, bytes);
     }
   }
   __syncthreads();

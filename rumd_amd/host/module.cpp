@@ -216,6 +216,24 @@ PYBIND11_MODULE(_rumd, m) {
       .def("RunSteps", &Sample::RunSteps, py::arg("first_step"), py::arg("nsteps"), py::arg("block") = 10,
            py::arg("momentum_interval") = 100, py::call_guard<py::gil_scoped_release>())
       .def("GetPositions", [](Sample& s) { return get_xyz(s, "r"); })
+      // Nx3 float64: r + rlo in mixed-precision mode, r widened otherwise
+      .def("GetPositionsFP64", [](Sample& s) {
+        const unsigned n = s.GetNumberOfParticles();
+        if (!n) throw RUMD_Error("Sample", "GetPositionsFP64", "Sample has no particles");
+        s.Synchronize();
+        std::vector<float> hi(4ull * n), lo;
+        s.GetParticleData().Get("r", hi.data(), s.GetStream());
+        if (s.GetMixedPrecisionPositions()) {
+          lo.resize(4ull * n);
+          s.GetParticleData().Get("rlo", lo.data(), s.GetStream());
+        }
+        py::array_t<double> out({static_cast<py::ssize_t>(n), static_cast<py::ssize_t>(3)});
+        auto o = out.mutable_unchecked<2>();
+        for (unsigned i = 0; i < n; ++i)
+          for (int k = 0; k < 3; ++k)
+            o(i, k) = static_cast<double>(hi[4ull * i + k]) + (lo.empty() ? 0.0 : static_cast<double>(lo[4ull * i + k]));
+        return out;
+      })
       .def("GetVelocities", [](Sample& s) { return get_xyz(s, "v"); })
       .def("GetForces", [](Sample& s) { return get_xyz(s, "f"); })
       .def("GetImages", [](Sample& s) { return get_xyz(s, "im"); })
