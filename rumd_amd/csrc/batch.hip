@@ -171,9 +171,11 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
   for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_skin2[k];
   __syncthreads();
+  __shared__ uint32_t s_max[kBlock / 64];
   const rumdx_replica rp = a.rep[rep];
   uint32_t* fl = a.flags + kRepFlags * rep;
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t longest = 0;
   if (i - rp.slot0 < rp.n) {
     const Box bx = rep_box(rp);
     const float4 ri = a.r[i];
@@ -182,22 +184,44 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
     const uint32_t lc = slot_cell[i] - rp.cell0;
     const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc), cz = static_cast<int>(lc / (nc * nc));
     uint32_t cnt = 0;
+    auto consider = [&](uint32_t j, float4 rj) {
+      if (j != i && min_image_d2_f32(ri, rj, bx) < crow[__float_as_uint(rj.w)]) {
+        if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * a.nslots + i] = j;
+        ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
+      }
+    };
+    // a contiguous range of cell_list, four candidates' loads in flight at once
+    auto scan = [&](uint32_t k0, uint32_t k1) {
+      uint32_t k = k0;
+      for (; k + 4 <= k1; k += 4) {
+        const uint32_t j0 = cell_list[k], j1 = cell_list[k + 1], j2 = cell_list[k + 2], j3 = cell_list[k + 3];
+        const float4 r0 = a.r[j0], r1 = a.r[j1], r2 = a.r[j2], r3 = a.r[j3];
+        consider(j0, r0);
+        consider(j1, r1);
+        consider(j2, r2);
+        consider(j3, r3);
+      }
+      for (; k < k1; ++k) {
+        const uint32_t j = cell_list[k];
+        consider(j, a.r[j]);
+      }
+    };
+    // Cells are numbered x fastest and cell_list follows that order, so the
+    // five cells (cx-2..cx+2, y, z) are one contiguous range unless the row
+    // wraps around the box; then the cells go one by one in dX order.
+    const bool row = cx >= 2 && cx + 2 < nc;
     for (int dz = -2; dz <= 2; ++dz) {
       const int oz = (cz + dz + nc) % nc;
       for (int dy = -2; dy <= 2; ++dy) {
         const int oy = (cy + dy + nc) % nc;
-        for (int dx = -2; dx <= 2; ++dx) {
-          const int ox = (cx + dx + nc) % nc;
-          const uint32_t c = rp.cell0 + static_cast<uint32_t>(ox + nc * (oy + nc * oz));
-          const uint32_t b0 = cell_start[c], m = cell_n[c];
-          for (uint32_t k = b0; k < b0 + m; ++k) {
-            const uint32_t j = cell_list[k];
-            if (j == i) continue;
-            const float4 rj = a.r[j];
-            if (min_image_d2_f32(ri, rj, bx) < crow[__float_as_uint(rj.w)]) {
-              if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * a.nslots + i] = j;
-              ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
-            }
+        const uint32_t c_row = rp.cell0 + static_cast<uint32_t>(nc * (oy + nc * oz));
+        if (row) {
+          const uint32_t lo = c_row + static_cast<uint32_t>(cx - 2), hi = c_row + static_cast<uint32_t>(cx + 2);
+          scan(cell_start[lo], cell_start[hi] + cell_n[hi]);
+        } else {
+          for (int dx = -2; dx <= 2; ++dx) {
+            const uint32_t c = c_row + static_cast<uint32_t>((cx + dx + nc) % nc);
+            scan(cell_start[c], cell_start[c] + cell_n[c]);
           }
         }
       }
@@ -205,8 +229,10 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
     num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;
     last_r[i] = ri;
     if (cnt >= max_nbrs) fl[kOverflow] = 1u;
-    atomicMax(fl + kMaxLen, cnt);
+    longest = cnt;
   }
+  longest = block_max_u32(longest, s_max);
+  if (threadIdx.x == 0) atomicMax(fl + kMaxLen, longest);
   const uint32_t nblk = (rp.n + kBlock - 1) / kBlock;
   if (last_to_arrive(fl + kDone, nblk) && threadIdx.x == 0) {
     fl[kRebuild] = 0u;

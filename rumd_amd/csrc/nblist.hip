@@ -144,8 +144,10 @@ __global__ __launch_bounds__(kBlock) void k_build_n2(const float4* __restrict__ 
     last_r[i] = ri;
     num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;  // == max_nbrs marks an overflowed list
     if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
-    atomicMax(flags + RUMDX_FLAG_MAXLEN, cnt);
   }
+  __shared__ uint32_t s_max[kBlock / 64];
+  const uint32_t longest = block_max_u32(live ? cnt : 0u, s_max);
+  if (threadIdx.x == 0) atomicMax(flags + RUMDX_FLAG_MAXLEN, longest);
   finish_build(flags);
 }
 
@@ -337,8 +339,6 @@ struct HomeArrays {
 // cell being built and a window of its candidates (LDS offsets, in stencil
 // order).
 constexpr uint32_t kCandWindow = 512;
-// Type counts up to this keep a candidate's cut-offs in registers.
-constexpr uint32_t kRegTypes = 4;
 struct WaveBuild {
   uint16_t cand[kCandWindow];
   uint16_t seg_start[128];  // LDS offset of the segment's first particle
@@ -356,11 +356,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 // which holds cnt entries so far; returns the new count (uniform).
 __device__ __forceinline__ uint32_t append_chunk(bool acc, uint32_t entry, uint32_t cnt, uint16_t* __restrict__ nbl16,
                                                  uint32_t i, uint32_t nvp, uint32_t max_nbrs) {
-  const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
+  const uint64_t mask = __ballot(acc);
   const uint32_t k = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-  if (acc && k < max_nbrs)  // group (k/8)*nvp + i, element k%8; max_nbrs * nvp < 2^32
-    nbl16[static_cast<size_t>((k >> 3) * nvp + i) * 8 + (k & 7u)] = static_cast<uint16_t>(entry);
+  if (acc && k < max_nbrs) nbl16[(static_cast<size_t>(k >> 3) * nvp + i) * 8 + (k & 7u)] = static_cast<uint16_t>(entry);
   return cnt + static_cast<uint32_t>(__popcll(mask));
 }
 
@@ -370,7 +369,7 @@ __device__ __forceinline__ uint32_t append_chunk(bool acc, uint32_t entry, uint3
 // test them against one particle at a time; a ballot places the accepted
 // entries in candidate order.  Writes the lists and leaves each particle's
 // full count (which may exceed max_nbrs) in s_cnt[slot - slot0].
-template <bool WRAPS, bool REGCUT>
+template <bool WRAPS>
 __device__ __forceinline__ void build_cell_wave(const Halo& h, const HaloLds& L, const TileGeom& tg,
                                                 const float4* s_pos, const float* s_cut, uint32_t nt, const Box& bx,
                                                 int cx, int cy, int cz, WaveBuild& wb, uint32_t* s_cnt,
@@ -424,9 +423,7 @@ __device__ __forceinline__ void build_cell_wave(const Halo& h, const HaloLds& L,
   wave_lds_sync();
   for (uint32_t pg = 0; pg < m; pg += 64) {
     const uint32_t mg = (m - pg) < 64u ? (m - pg) : 64u;
-    // lane l holds particle pg + l: its position and its list length so far
-    const float4 pv = lane < mg ? s_pos[coff + pg + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t cnt_v = 0;
+    uint32_t cnt_v = 0;  // lane l: count of particle pg + l
     for (uint32_t w0 = 0; w0 < total; w0 += kCandWindow) {
       const uint32_t wn = (total - w0) < kCandWindow ? (total - w0) : kCandWindow;
       wave_lds_sync();  // the previous window has been read
@@ -437,40 +434,35 @@ __device__ __forceinline__ void build_cell_wave(const Halo& h, const HaloLds& L,
         for (uint32_t q = lo; q < hi; ++q) wb.cand[q - w0] = static_cast<uint16_t>(st + (q - pre));
       }
       wave_lds_sync();
-      // Each 64-candidate chunk is loaded once and tested against every
-      // particle of the cell: the particle's data comes from its lane
-      // (readlane), the cut-offs of the candidate's type from registers.
-      for (uint32_t q0 = 0; q0 < wn; q0 += 64) {
-        const uint32_t q = q0 + lane;
-        const bool valid = q < wn;
-        const uint32_t oo = valid ? static_cast<uint32_t>(wb.cand[q]) : 0u;
-        const float4 rj = s_pos[oo];
-        const uint32_t tj = __float_as_uint(rj.w);
-        const uint32_t entry = encode_entry(oo, bytes);
-        float cuts[kRegTypes];
+      for (uint32_t p = 0; p < mg; ++p) {
+        const uint32_t po = coff + pg + p;
+        const float4 pi = s_pos[po];
+        const float* crow = s_cut + __float_as_uint(pi.w) * nt;
+        const uint32_t i = gstart + pg + p;
+        uint32_t cnt = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt_v), static_cast<int>(p)));
+        // four 64-candidate chunks per pass: their LDS reads are issued
+        // together, so one round trip serves 256 candidates
+        for (uint32_t q0 = 0; q0 < wn; q0 += 4 * 64) {
+          uint32_t oo[4];
+          float4 rj[4];
+          float cut[4];
 #pragma unroll
-        for (uint32_t t = 0; t < kRegTypes; ++t) cuts[t] = (REGCUT && t < nt) ? s_cut[t * nt + tj] : 0.f;
-        for (uint32_t p = 0; p < mg; ++p) {
-          float4 pi;
-          pi.x = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pv.x), static_cast<int>(p)));
-          pi.y = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pv.y), static_cast<int>(p)));
-          pi.z = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, pv.z), static_cast<int>(p)));
-          const uint32_t ti =
-              static_cast<uint32_t>(__builtin_amdgcn_readlane(__builtin_bit_cast(int, pv.w), static_cast<int>(p)));
-          const float d2 = WRAPS ? min_image_d2_f32(pi, rj, bx) : plain_d2_f32(pi, rj);
-          float cut;
-          if (REGCUT) {
-            cut = cuts[0];
-#pragma unroll
-            for (uint32_t t = 1; t < kRegTypes; ++t) cut = ti == t ? cuts[t] : cut;
-          } else {
-            cut = s_cut[ti * nt + tj];
+          for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t q = q0 + 64 * u + lane;
+            oo[u] = q < wn ? static_cast<uint32_t>(wb.cand[q]) : po;
           }
-          const bool acc = valid && oo != coff + pg + p && d2 < cut;
-          uint32_t cnt = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cnt_v), static_cast<int>(p)));
-          cnt = append_chunk(acc, entry, cnt, nbl16, gstart + pg + p, nvp, max_nbrs);
-          cnt_v = lane == p ? cnt : cnt_v;
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u) rj[u] = s_pos[oo[u]];
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u) cut[u] = crow[__float_as_uint(rj[u].w)];
+#pragma unroll
+          for (uint32_t u = 0; u < 4; ++u) {
+            const float d2 = WRAPS ? min_image_d2_f32(pi, rj[u], bx) : plain_d2_f32(pi, rj[u]);
+            const bool acc = oo[u] != po && d2 < cut[u];
+            cnt = append_chunk(acc, encode_entry(oo[u], bytes), cnt, nbl16, i, nvp, max_nbrs);
+          }
         }
+        cnt_v = lane == p ? cnt : cnt_v;
       }
     }
     if (lane < mg) s_cnt[gstart + pg + lane - h.slot0] = cnt_v;
@@ -541,6 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
     }
   }
   bool any_overflow = false;
+  uint32_t longest = 0;  // this thread's longest list
   // wave-cooperative lists for staged tiles; one lane per particle otherwise
   const bool coop = staged && np <= kOrderMax;
   if (coop) {
@@ -555,19 +548,12 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
       cx -= cx >= tg.nc0 ? tg.nc0 : 0;
       cy -= cy >= tg.nc1 ? tg.nc1 : 0;
       cz -= cz >= tg.nc2 ? tg.nc2 : 0;
-      // REGCUT: a candidate's cut-offs against every type fit in registers
-      if (h.wraps && nt <= kRegTypes)
-        build_cell_wave<true, true>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp,
-                                    max_nbrs, bytes);
-      else if (h.wraps)
-        build_cell_wave<true, false>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp,
-                                     max_nbrs, bytes);
-      else if (nt <= kRegTypes)
-        build_cell_wave<false, true>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp,
-                                     max_nbrs, bytes);
+      if (h.wraps)
+        build_cell_wave<true>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp, max_nbrs,
+                              bytes);
       else
-        build_cell_wave<false, false>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp,
-                                      max_nbrs, bytes);
+        build_cell_wave<false>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp, max_nbrs,
+                               bytes);
     }
   }
   __syncthreads();
@@ -590,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
       flags[RUMDX_FLAG_OVERFLOW] = 1u;
       any_overflow = true;
     }
-    atomicMax(flags + RUMDX_FLAG_MAXLEN, cnt);
+    longest = cnt > longest ? cnt : longest;
     o.slot_cell[i] = static_cast<uint32_t>(cx + cy * tg.nc0 + cz * tg.nc0 * tg.nc1);
     o.r[i] = ri;
     o.last_r[i] = ri;
@@ -604,6 +590,8 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   // Lane order of the force kernel: the tile's particles by descending list
   // length, ties by slot (a rank count over unique keys; deterministic).
   const bool overflowed = __syncthreads_or(any_overflow ? 1 : 0) != 0;
+  __shared__ uint32_t s_max[kBlock / 64];
+  longest = block_max_u32(longest, s_max);
   for (uint32_t k = threadIdx.x; k < np; k += kBlock) {
     uint32_t rk = k;
     if (np <= kOrderMax) {
@@ -614,6 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
     o.lane_order[h.slot0 + rk] = static_cast<uint16_t>(k);
   }
   if (threadIdx.x == 0) {
+    atomicMax(flags + RUMDX_FLAG_MAXLEN, longest);
     atomicMax(flags + RUMDX_FLAG_MAXHALO, h.npart);
     if (h.npart > 65535u) flags[RUMDX_FLAG_ERROR] = 1u;
     // wrapping axes, and whether the shifted frame serves them

@@ -105,6 +105,24 @@ __device__ __forceinline__ double block_sum(double v, double* lds4) {
   return s;
 }
 
+// Maximum of v over a kBlock-thread workgroup, valid in thread 0.  Flags
+// that collect a maximum (the longest list) take one atomic per workgroup:
+// one per particle on a single address serialises at the memory side.
+__device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* lds4) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t y = __shfl_xor(v, o, 64);
+    v = y > v ? y : v;
+  }
+  if ((threadIdx.x & 63u) == 0) lds4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t m = 0;
+  if (threadIdx.x == 0)
+    for (uint32_t k = 0; k < kBlock / 64; ++k) m = lds4[k] > m ? lds4[k] : m;
+  __syncthreads();
+  return m;
+}
+
 // Returns true (in every thread) for the block that arrives last at `counter`,
 // which is left at zero for the next launch.  No memory fence is issued: this
 // orders nothing but the arrivals themselves, which is enough for a block to
