@@ -388,6 +388,7 @@ def main() -> int:
             "nmax": int(nmax),
             "list_builds": builds,
             "kbar": round(kbar, 2) if kbar else None,
+            "halo_capacity": pot.GetNbHaloCapacity(),
             "model_bytes_per_atom_step": round(step_bytes, 1),
             "model_hbm_fraction_step": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
             "pe_per_particle": round(pe, 6),

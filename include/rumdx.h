@@ -156,9 +156,11 @@ int rumdx_cell_rank_table(const int32_t nc[3], uint32_t* rank_out);
  * rumdx_cell_rank_table.  Host only; returns a negative code on error. */
 int rumdx_tile_table(const int32_t nc[3], const uint32_t* rank, uint32_t* first_rank_out, uint32_t* origin_out);
 /* Halo capacity (particles) to request for a grid of nc cells holding n
- * particles: the mean 8^3-cell halo with margin, rounded up to 64, at most
- * RUMDX_HALO_CAP_MAX.  Tiles whose halo is larger still compute exactly,
- * reading positions from global memory. */
+ * particles: the mean 8^3-cell halo plus 8 % and 64, rounded up to 64, at
+ * most RUMDX_HALO_CAP_MAX.  The force kernel's LDS is 16 bytes per halo
+ * particle, so a smaller capacity lets more tiles share a CU.  Tiles whose
+ * halo is larger still compute exactly, reading positions from global
+ * memory. */
 uint32_t rumdx_halo_cap_estimate(const int32_t nc[3], uint32_t n);
 
 /* ---------------------------------------------------------------- neighbour list

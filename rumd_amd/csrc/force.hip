@@ -118,6 +118,32 @@ __device__ __forceinline__ void pair_term_direct(float4 ri, float4 rj, const flo
   pair_core<CUT, STR>(dx, dy, dz, dx * dx + dy * dy + dz * dz, __float_as_uint(rj.w), prow, a);
 }
 
+// Forces on particle i from the 5x5x5 cell stencil around the cell it was
+// binned in at the last build, positions from global memory, displacements in
+// FP64 (an overflowed list, or a tile whose halo was not staged).  Every pair
+// now inside rc was inside rc + skin at the build, so it is in that stencil.
+template <int CUT, bool STR>
+__device__ __forceinline__ void rescan_cells(uint32_t i, double xi, double yi, double zi, const float4* __restrict__ r,
+                                          const float4* prow, const Box& bx, const CellFallback& fb, Acc& a) {
+  const uint32_t lin = fb.slot_cell[i];
+  const int cx = static_cast<int>(lin % fb.nc0);
+  const int cy = static_cast<int>((lin / fb.nc0) % fb.nc1);
+  const int cz = static_cast<int>(lin / (static_cast<uint32_t>(fb.nc0) * fb.nc1));
+  for (int dz = -2; dz <= 2; ++dz) {
+    const int oz = (cz + dz + fb.nc2) % fb.nc2;
+    for (int dy = -2; dy <= 2; ++dy) {
+      const int oy = (cy + dy + fb.nc1) % fb.nc1;
+      for (int dx = -2; dx <= 2; ++dx) {
+        const int ox = (cx + dx + fb.nc0) % fb.nc0;
+        const uint32_t rank = fb.cell_rank[ox + oy * fb.nc0 + oz * fb.nc0 * fb.nc1];
+        const uint32_t s = fb.cell_start[rank], e = fb.cell_start[rank + 1];
+        for (uint32_t j = s; j < e; ++j)
+          if (j != i) pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
+      }
+    }
+  }
+}
+
 template <bool STR, bool INIT>
 __device__ __forceinline__ void store_result(uint32_t i, Acc& a, float4* __restrict__ f, float4* __restrict__ w,
                                              float4* __restrict__ sts) {
@@ -220,26 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_lj_nbl(const float4* __restrict__ r,
         pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
       }
     } else if (part == 0 && fb.cell_rank != nullptr) {
-      // Overflowed list: rescan the 5x5x5 stencil around the cell the particle
-      // was binned in at the last build.  Every pair now inside rc was inside
-      // rc + skin then, so it is in that stencil.
-      const uint32_t lin = fb.slot_cell[i];
-      const int cx = static_cast<int>(lin % fb.nc0);
-      const int cy = static_cast<int>((lin / fb.nc0) % fb.nc1);
-      const int cz = static_cast<int>(lin / (static_cast<uint32_t>(fb.nc0) * fb.nc1));
-      for (int dz = -2; dz <= 2; ++dz) {
-        const int oz = (cz + dz + fb.nc2) % fb.nc2;
-        for (int dy = -2; dy <= 2; ++dy) {
-          const int oy = (cy + dy + fb.nc1) % fb.nc1;
-          for (int dx = -2; dx <= 2; ++dx) {
-            const int ox = (cx + dx + fb.nc0) % fb.nc0;
-            const uint32_t rank = fb.cell_rank[ox + oy * fb.nc0 + oz * fb.nc0 * fb.nc1];
-            const uint32_t s = fb.cell_start[rank], e = fb.cell_start[rank + 1];
-            for (uint32_t j = s; j < e; ++j)
-              if (j != i) pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
-          }
-        }
-      }
+      rescan_cells<CUT, STR>(i, xi, yi, zi, r, prow, bx, fb, a);
     } else if (part == 0) {
       for (uint32_t j = 0; j < n; ++j)
         if (j != i) pair_term<CUT, STR>(xi, yi, zi, r[j], prow, bx, a);
@@ -291,21 +298,20 @@ __device__ __forceinline__ float4 halo_pos(const float4* s_pos, uint32_t entry) 
 // Waves take the tile's particles 64 >> lgp at a time in the build's lane
 // order (descending list length, so a wave's lists have similar lengths);
 // each list is spread over 2^lg lanes (groups part, part + 2^lg, ...; lg =
-// lgp, more in a tile's last, partial chunk), combined by a butterfly.  FAST: the halo was staged in the shifted
-// frame and every list is complete, so every pair takes the direct FP32
-// difference.  Otherwise a wrapping halo forms displacements in FP64, and
-// overflowed lists, or a halo beyond the LDS capacity, rescan the particle's
-// build-time cell stencil (positions from LDS or global memory).
+// lgp, more in a tile's last, partial chunk), combined by a butterfly.  FAST:
+// the halo was staged in the shifted frame, so every pair takes the direct
+// FP32 difference; otherwise the halo wraps and displacements are formed in
+// FP64.  An overflowed list, or every list of a tile whose halo was not
+// staged, rescans the particle's build-time cell stencil from global memory.
 template <int CUT, bool STR, bool INIT, bool FAST>
-__device__ __forceinline__ void tile_particles(uint32_t slot0, uint32_t slot1, bool staged, bool wraps,
-                                               const uint32_t shifts, const float4* __restrict__ r,
-                                               float4* __restrict__ f, float4* __restrict__ w,
-                                               float4* __restrict__ sts, const uint16_t* __restrict__ nbl16,
+__device__ __forceinline__ void tile_particles(uint32_t slot0, uint32_t slot1, bool staged, const uint32_t shifts,
+                                               const float4* __restrict__ r, float4* __restrict__ f,
+                                               float4* __restrict__ w, float4* __restrict__ sts,
+                                               const uint16_t* __restrict__ nbl16,
                                                const uint32_t* __restrict__ num_nbrs,
-                                               const uint32_t* __restrict__ slot_cell,
                                                const uint16_t* __restrict__ lane_order, const float4* s_pos,
-                                               const float4* s_tab, uint32_t nt, const Box& bx, const TileGeom& tg,
-                                               const HaloLds& L, uint32_t nvp, uint32_t max_nbrs,
+                                               const float4* s_tab, uint32_t nt, const Box& bx,
+                                               const CellFallback& fb, uint32_t nvp, uint32_t max_nbrs,
                                                uint32_t lgp) {
   const uint32_t np = slot1 - slot0;
   const uint32_t lane = threadIdx.x & 63u;
@@ -329,57 +335,42 @@ __device__ __forceinline__ void tile_particles(uint32_t slot0, uint32_t slot1, b
       const size_t gstride = static_cast<size_t>(nvp) << lg;
       const u32x4* grp = reinterpret_cast<const u32x4*>(nbl16) + static_cast<size_t>(part) * nvp + i;
       const bool tail = rem != 0u && (nfull & (step - 1u)) == part;  // this lane owns group nfull
-      if (FAST || (staged && nn < max_nbrs && !wraps)) {
+      if (staged && nn < max_nbrs) {
         if (FAST) {
           ri.x = shift_coord(ri.x, shifts, 0, bx.L[0]);
           ri.y = shift_coord(ri.y, shifts, 1, bx.L[1]);
           ri.z = shift_coord(ri.z, shifts, 2, bx.L[2]);
-        }
-        for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
-          const u32x4 q = __builtin_nontemporal_load(grp);
+          for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
+            const u32x4 q = __builtin_nontemporal_load(grp);
 #pragma unroll
-          for (int e = 0; e < 8; ++e) pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
-        }
-        if (tail) {
-          const u32x4 q = __builtin_nontemporal_load(grp);
+            for (int e = 0; e < 8; ++e)
+              pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
+          }
+          if (tail) {
+            const u32x4 q = __builtin_nontemporal_load(grp);
 #pragma unroll
-          for (int e = 0; e < 7; ++e)
-            if (static_cast<uint32_t>(e) < rem) pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
-        }
-      } else if (!FAST && staged && nn < max_nbrs) {
-        const double xi = ri.x, yi = ri.y, zi = ri.z;
-        for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
-          const u32x4 q = __builtin_nontemporal_load(grp);
+            for (int e = 0; e < 7; ++e)
+              if (static_cast<uint32_t>(e) < rem)
+                pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
+          }
+        } else {
+          const double xi = ri.x, yi = ri.y, zi = ri.z;
+          for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
+            const u32x4 q = __builtin_nontemporal_load(grp);
 #pragma unroll
-          for (int e = 0; e < 8; ++e)
-            pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
-        }
-        if (tail) {
-          const u32x4 q = __builtin_nontemporal_load(grp);
-#pragma unroll
-          for (int e = 0; e < 7; ++e)
-            if (static_cast<uint32_t>(e) < rem)
+            for (int e = 0; e < 8; ++e)
               pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
+          }
+          if (tail) {
+            const u32x4 q = __builtin_nontemporal_load(grp);
+#pragma unroll
+            for (int e = 0; e < 7; ++e)
+              if (static_cast<uint32_t>(e) < rem)
+                pair_term<CUT, STR>(xi, yi, zi, halo_pos(s_pos, group_entry(q, e)), prow, bx, a);
+          }
         }
-      } else if (!FAST && part == 0) {
-        const Halo& h = L.h;
-        const double xi = ri.x, yi = ri.y, zi = ri.z;
-        const uint32_t lin = slot_cell[i];
-        const int cx = static_cast<int>(lin % tg.nc0);
-        const int cy = static_cast<int>((lin / tg.nc0) % tg.nc1);
-        const int cz = static_cast<int>(lin / (static_cast<uint32_t>(tg.nc0) * tg.nc1));
-        const uint32_t hc = halo_cell(h, tg, cx, cy, cz);
-        const uint32_t my_off = L.off[hc] + (i - L.start[hc]);
-        for (int dz = -2; dz <= 2; ++dz)
-          for (int dy = -2; dy <= 2; ++dy)
-            for (int dx = -2; dx <= 2; ++dx) {
-              const uint32_t hcell = halo_cell(h, tg, cx + dx, cy + dy, cz + dz);
-              const uint32_t o0 = L.off[hcell], o1 = L.off[hcell + 1], g0 = L.start[hcell];
-              for (uint32_t oo = o0; oo < o1; ++oo) {
-                if (oo == my_off) continue;
-                pair_term<CUT, STR>(xi, yi, zi, staged ? s_pos[oo] : r[g0 + (oo - o0)], prow, bx, a);
-              }
-            }
+      } else if (part == 0) {
+        rescan_cells<CUT, STR>(i, ri.x, ri.y, ri.z, r, prow, bx, fb, a);
       }
     }
     combine_lanes<STR>(a, lg);
@@ -387,47 +378,53 @@ __device__ __forceinline__ void tile_particles(uint32_t slot0, uint32_t slot1, b
   }
 }
 
+// Occupancy target of the tile force kernel: waves per SIMD.  Five fit the
+// LDS of a tile whose halo capacity is at most 2040 particles, and the
+// compiler then keeps the kernel within 102 VGPRs.
+#ifndef RUMDX_TILE_WAVES_PER_EU
+#define RUMDX_TILE_WAVES_PER_EU 5
+#endif
+
 // Tile force kernel for lists of the sorted method: one workgroup per cell
-// tile stages the tile's halo positions in LDS; list entries are LDS byte
-// offsets, so neighbour positions come from LDS.  Tiles the build marked
-// fast stage their halo from the build's slot list in the shifted frame (one
-// coalesced pass, no table); the others rebuild the halo table from the cell
-// arrays and may need FP64 displacements or the stencil rescan.
+// tile.  A tile whose halo fits the capacity stages it in LDS by one
+// coalesced gather over the slot list the build recorded (in the shifted
+// frame when the build marked the tile fast); list entries are LDS byte
+// offsets, so neighbour positions come from LDS.  The kernel uses no static
+// LDS: its dynamic LDS is the halo capacity and the parameter table, which
+// sets how many workgroups share a CU.
 template <int CUT, bool STR, bool INIT>
-__global__ __launch_bounds__(kBlock) void k_lj_tiles(const float4* __restrict__ r, float4* __restrict__ f,
-                                                     float4* __restrict__ w, float4* __restrict__ sts,
-                                                     const uint16_t* __restrict__ nbl16,
-                                                     const uint32_t* __restrict__ num_nbrs,
-                                                     const uint32_t* __restrict__ slot_cell,
-                                                     const uint16_t* __restrict__ lane_order,
-                                                     const float* __restrict__ params, uint32_t nt, Box bx, TileGeom tg,
-                                                     uint32_t nvp, uint32_t max_nbrs, uint32_t lgp) {
-  __shared__ HaloLds L;
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(RUMDX_TILE_WAVES_PER_EU))) void k_lj_tiles(
+    const float4* __restrict__ r, float4* __restrict__ f, float4* __restrict__ w, float4* __restrict__ sts,
+    const uint16_t* __restrict__ nbl16, const uint32_t* __restrict__ num_nbrs,
+    const uint16_t* __restrict__ lane_order, const float* __restrict__ params, uint32_t nt, Box bx,
+    const uint4* __restrict__ tile_info, const uint32_t* __restrict__ halo_slots, uint32_t halo_cap,
+    CellFallback fb, uint32_t nvp, uint32_t max_nbrs, uint32_t lgp) {
   extern __shared__ float4 s_dyn[];
   float4* s_pos = s_dyn;
-  float4* s_tab = s_dyn + tg.halo_cap;
+  float4* s_tab = s_dyn + halo_cap;
   const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
-  const uint4 info = tg.tile_info[t];
+  const uint4 info = tile_info[t];
   load_table(s_tab, params, nt);
-  if ((info.y & kTileFast) != 0u && (info.y >> 16) == tg.halo_cap && info.x <= tg.halo_cap) {
-    const uint32_t* hs = tg.halo_slots + static_cast<size_t>(t) * tg.halo_cap;
+  const bool staged = (info.y >> 16) == halo_cap && info.x <= halo_cap;
+  const bool fast = (info.y & kTileFast) != 0u;
+  if (staged) {
+    const uint32_t shifts = fast ? info.y : 0u;
+    const uint32_t* hs = halo_slots + static_cast<size_t>(t) * halo_cap;
     for (uint32_t q = threadIdx.x; q < info.x; q += kBlock) {
       float4 p = r[hs[q]];
-      p.x = shift_coord(p.x, info.y, 0, bx.L[0]);
-      p.y = shift_coord(p.y, info.y, 1, bx.L[1]);
-      p.z = shift_coord(p.z, info.y, 2, bx.L[2]);
+      p.x = shift_coord(p.x, shifts, 0, bx.L[0]);
+      p.y = shift_coord(p.y, shifts, 1, bx.L[1]);
+      p.z = shift_coord(p.z, shifts, 2, bx.L[2]);
       s_pos[q] = p;
     }
-    __syncthreads();
-    tile_particles<CUT, STR, INIT, true>(info.z, info.w, true, false, info.y, r, f, w, sts, nbl16, num_nbrs,
-                                         slot_cell, lane_order, s_pos, s_tab, nt, bx, tg, L, nvp, max_nbrs, lgp);
-    return;
   }
-  stage_halo(t, tg, r, s_pos, L);
-  const Halo& h = L.h;
-  tile_particles<CUT, STR, INIT, false>(h.slot0, h.slot1, h.npart <= tg.halo_cap, h.wraps, 0u, r, f, w, sts, nbl16,
-                                        num_nbrs, slot_cell, lane_order, s_pos, s_tab, nt, bx, tg, L, nvp, max_nbrs,
-                                        lgp);
+  __syncthreads();
+  if (staged && fast)
+    tile_particles<CUT, STR, INIT, true>(info.z, info.w, true, info.y, r, f, w, sts, nbl16, num_nbrs, lane_order,
+                                         s_pos, s_tab, nt, bx, fb, nvp, max_nbrs, lgp);
+  else
+    tile_particles<CUT, STR, INIT, false>(info.z, info.w, staged, 0u, r, f, w, sts, nbl16, num_nbrs, lane_order,
+                                          s_pos, s_tab, nt, bx, fb, nvp, max_nbrs, lgp);
 }
 
 template <int CUT, bool STR, bool INIT>
@@ -442,15 +439,11 @@ int launch_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float* par
   const uint32_t lpp = nb != nullptr ? nb->lanes_per_particle : 1u;
   const uint32_t lgp = lpp >= 8u ? 3u : lpp >= 4u ? 2u : lpp >= 2u ? 1u : 0u;
   if (nb != nullptr && nb->nbl16_dev != nullptr) {
-    const TileGeom tg{nb->nc[0],           nb->nc[1],
-                      nb->nc[2],           nb->cell_rank_dev,
-                      nb->cell_start_dev,  nb->tile_first_rank_dev,
-                      nb->tile_origin_dev, nb->ntiles,
-                      nb->halo_cap,        nb->tile_halo_dev,
-                      reinterpret_cast<uint4*>(nb->tile_info_dev)};
+    const CellFallback fb{nb->cell_rank_dev, nb->cell_start_dev, nb->slot_cell_dev, nb->nc[0], nb->nc[1], nb->nc[2]};
     hipLaunchKernelGGL((k_lj_tiles<CUT, STR, INIT>), dim3(nb->ntiles), dim3(kBlock),
                        tab_bytes + sizeof(float4) * nb->halo_cap, st, r, f, w, sts, nb->nbl16_dev, nb->num_nbrs_dev,
-                       nb->slot_cell_dev, nb->lane_order_dev, params, nt, bx, tg, p->nvp, nb->max_nbrs, lgp);
+                       nb->lane_order_dev, params, nt, bx, reinterpret_cast<const uint4*>(nb->tile_info_dev),
+                       nb->tile_halo_dev, nb->halo_cap, fb, p->nvp, nb->max_nbrs, lgp);
   } else if (nb != nullptr) {
     CellFallback fb{};
     if (nb->cell_rank_dev != nullptr) {
@@ -498,7 +491,8 @@ extern "C" int rumdx_pair_lj(const rumdx_particles* p, const rumdx_nblist* nb, c
   if (nb != nullptr && nb->nbl16_dev != nullptr &&
       (nb->ntiles == 0 || nb->tile_first_rank_dev == nullptr || nb->tile_origin_dev == nullptr ||
        nb->slot_cell_dev == nullptr || nb->lane_order_dev == nullptr || nb->tile_halo_dev == nullptr ||
-       nb->tile_info_dev == nullptr || nb->halo_cap > RUMDX_HALO_CAP_MAX))
+       nb->tile_info_dev == nullptr || nb->cell_rank_dev == nullptr || nb->cell_start_dev == nullptr ||
+       nb->halo_cap > RUMDX_HALO_CAP_MAX))
     return fail(RUMDX_E_INVALID, "rumdx_pair_lj: tile arrays not allocated");
   if (nb != nullptr) {
     const uint32_t lpp = nb->lanes_per_particle;

@@ -165,12 +165,14 @@ __device__ __forceinline__ uint32_t cell_of(float4 rr, const Box& bx, const Cell
 }
 
 // calculateCellIndices (src/NeighborList.cu:1142-1172) plus the histogram.
-__global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__ r, const uint32_t* __restrict__ flags,
+__global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__ r, uint32_t* __restrict__ flags,
                                                       const uint32_t* __restrict__ cell_rank,
                                                       uint32_t* __restrict__ key_lin,
                                                       uint32_t* __restrict__ cell_count, Box bx, CellGeom g,
                                                       uint32_t n) {
   if (!rebuild_pending(flags)) return;
+  // the tile build then records the largest halo of this build
+  if (blockIdx.x == 0 && threadIdx.x == 0) flags[RUMDX_FLAG_MAXHALO] = 0u;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const uint32_t lin = cell_of(r[i], bx, g);
     key_lin[i] = lin;
@@ -532,7 +534,6 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
       for (uint32_t k = 0; k < cnt; ++k) hs[o0 + k] = g0 + k;
     }
   }
-  bool any_overflow = false;
   uint32_t longest = 0;  // this thread's longest list
   // wave-cooperative lists for staged tiles; one lane per particle otherwise
   const bool coop = staged && np <= kOrderMax;
@@ -572,10 +573,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
     const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
     num_nbrs[i] = stored;  // == max_nbrs marks an overflowed list
     if (np <= kOrderMax) s_key[i - h.slot0] = ((0xffffu - min(stored, 0xffffu)) << 16) | (i - h.slot0);
-    if (cnt >= max_nbrs) {
-      flags[RUMDX_FLAG_OVERFLOW] = 1u;
-      any_overflow = true;
-    }
+    if (cnt >= max_nbrs) flags[RUMDX_FLAG_OVERFLOW] = 1u;
     longest = cnt > longest ? cnt : longest;
     o.slot_cell[i] = static_cast<uint32_t>(cx + cy * tg.nc0 + cz * tg.nc0 * tg.nc1);
     o.r[i] = ri;
@@ -589,7 +587,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   }
   // Lane order of the force kernel: the tile's particles by descending list
   // length, ties by slot (a rank count over unique keys; deterministic).
-  const bool overflowed = __syncthreads_or(any_overflow ? 1 : 0) != 0;
+  // block_max_u32 synchronises the workgroup, so every key is written.
   __shared__ uint32_t s_max[kBlock / 64];
   longest = block_max_u32(longest, s_max);
   for (uint32_t k = threadIdx.x; k < np; k += kBlock) {
@@ -616,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
       shiftable = shiftable && ((!below && !above) || (nc[a] >= kShiftMinCells && h.ext[a] < nc[a] &&
                                                        !(below && above) && shift_exact(bx.L[a], nc[a])));
     }
-    if (staged && !overflowed && shiftable) fl |= kTileFast;
+    if (staged && shiftable) fl |= kTileFast;
     tg.tile_info[t] = make_uint4(h.npart, fl | (tg.halo_cap << 16), h.slot0, h.slot1);
   }
   finish_build(flags);
@@ -840,7 +838,7 @@ extern "C" uint32_t rumdx_halo_cap_estimate(const int32_t nc[3], uint32_t n) {
   if (nc == nullptr) return 0;
   const double ncells = static_cast<double>(nc[0]) * nc[1] * nc[2];
   const double halo = static_cast<double>(kHaloCellsMax) * n / ncells;
-  uint32_t cap = static_cast<uint32_t>(1.25 * halo + 64.0);
+  uint32_t cap = static_cast<uint32_t>(1.08 * halo + 64.0);
   cap = (cap + 63u) & ~63u;
   return cap > RUMDX_HALO_CAP_MAX ? RUMDX_HALO_CAP_MAX : cap;
 }

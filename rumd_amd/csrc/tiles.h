@@ -31,9 +31,8 @@ struct TileGeom {
   uint4* tile_info;            // per tile: {halo particles, kTileFast | shifts | cap << 16, slot0, slot1}
 };
 
-// tile_info flags.  kTileFast: the build staged the halo (npart <= halo_cap),
-// no list of the tile overflowed, and every axis on which the halo wraps
-// allows the shifted frame below.  Per axis a, (kTileShiftDown << 2a): the
+// tile_info flags.  kTileFast: the build staged the halo (npart <= halo_cap)
+// and every axis on which the halo wraps allows the shifted frame below.  Per axis a, (kTileShiftDown << 2a): the
 // halo wraps below the tile (its far cells sit at x >= 0, near +L/2) and
 // x >= 0 becomes x - L; (kTileShiftUp << 2a): it wraps above and x < 0
 // becomes x + L.  Positions are in [-L/2, L/2).

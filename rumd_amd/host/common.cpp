@@ -26,6 +26,10 @@ void DeviceArray<T>::resize(size_t n) {
   void* p = nullptr;
   hip_check(hipMalloc(&p, n * sizeof(T)), "DeviceArray", "resize");
   hip_check(hipMemset(p, 0, n * sizeof(T)), "DeviceArray", "resize");
+  // hipMemset runs on the null stream, which does not order work on
+  // non-blocking streams (ReplicaBatch's): finish the fill before any kernel
+  // can read the array.
+  hip_check(hipStreamSynchronize(nullptr), "DeviceArray", "resize");
   ptr_ = static_cast<T*>(p);
   n_ = n;
 }

@@ -11,6 +11,7 @@ constexpr unsigned kNumParam = RUMDX_NUM_PARAM;
 constexpr unsigned kMinSizeMemoryOpt = 10000;   // src/NeighborList.cu:27
 constexpr unsigned kDefaultMaxNumNbrs = 50;     // src/NeighborList.cu:28
 constexpr unsigned kSortThreshold = 8000;       // src/NeighborList.cu:31
+constexpr uint32_t kHaloShrinkSlack = 128;     // halo capacity kept above the need before shrinking
 }  // namespace
 
 // ================================================================== NeighborList
@@ -229,13 +230,22 @@ bool NeighborList::HandleOverflow() {
   flags_.download(f, 8, sample_->GetStream());
   if (f[RUMDX_FLAG_ERROR])
     throw RUMD_Error("NeighborList", "UpdateNBlist", "a cell tile's halo holds more than 65535 particles");
-  if (desc_.nbl16_dev != nullptr && f[RUMDX_FLAG_MAXHALO] > halo_cap_ && halo_cap_ < RUMDX_HALO_CAP_MAX) {
-    // tiles beyond the LDS capacity computed exactly from global memory; give
-    // the next launches room for them
-    halo_cap_ = std::min<uint32_t>(RUMDX_HALO_CAP_MAX, ((f[RUMDX_FLAG_MAXHALO] + f[RUMDX_FLAG_MAXHALO] / 8) + 63u) & ~63u);
-    SizeTileHalo();
-    sample_->InvalidateGraphs();
-    if (!f[RUMDX_FLAG_OVERFLOW]) return true;
+  if (desc_.nbl16_dev != nullptr && f[RUMDX_FLAG_MAXHALO] > 0u) {
+    // Capacity for the largest halo of the last build plus 1/16.  Tiles beyond
+    // the capacity computed exactly from global memory: grow for them.  The
+    // force kernel's LDS is 16 bytes per unit of capacity, so a capacity well
+    // above the need (a lattice start aliases with the cell grid, and its
+    // halos are larger than the liquid's) costs occupancy: shrink then.
+    const uint32_t need =
+        std::min<uint32_t>(RUMDX_HALO_CAP_MAX, ((f[RUMDX_FLAG_MAXHALO] + f[RUMDX_FLAG_MAXHALO] / 16) + 63u) & ~63u);
+    const bool grow = f[RUMDX_FLAG_MAXHALO] > halo_cap_ && halo_cap_ < RUMDX_HALO_CAP_MAX;
+    const bool shrink = need + kHaloShrinkSlack <= halo_cap_;
+    if (grow || shrink) {
+      halo_cap_ = need;
+      SizeTileHalo();
+      sample_->InvalidateGraphs();
+      if (!f[RUMDX_FLAG_OVERFLOW]) return true;
+    }
   }
   if (!f[RUMDX_FLAG_OVERFLOW]) return false;
   const unsigned n = sample_->GetNumberOfParticles();

@@ -76,6 +76,26 @@ def test_batch_forces_and_lists_per_replica():
             assert set(ids[off[i]:off[i + 1]].tolist()) == set(i2[o2[i]:o2[i + 1]].tolist()), (r, i)
 
 
+def test_batch_first_build_on_reused_memory():
+    """The first build's counters start at zero even when the allocations
+    reuse freed device memory that held other data (the batch runs on a
+    non-blocking stream, so the zero fill must complete before it)."""
+    import torch
+    junk = torch.full((1 << 26,), -1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    del junk
+    torch.cuda.empty_cache()
+    systems = _systems([700, 3000], seed0=650)
+    b = make_batch(systems, [0.5, 0.5])
+    print("[case] batch first build on reused memory", flush=True)
+    b.CalcF()
+    for r, s in enumerate(systems):
+        pos4 = configs.pack_pos4(s.positions, s.types)
+        f_ref, _ = oracle.forces_brute(pos4, s.box, 2, ka_table(), SF)
+        err = force_errors(b.GetForcesAndEnergies(r)[:, :3], f_ref[:, :3])
+        assert err["guarded_max"] <= FORCE_TOL_GUARDED, (r, err)
+
+
 def test_batch_replicas_are_independent():
     """A replica run inside a batch equals the same replica run alone, bit for
     bit (graph-captured blocks, rebuilds included)."""

@@ -194,6 +194,30 @@ def test_overflow_grows_list_and_stays_exact():
     assert pot.GetActualMaxNumNbrs() > 30
 
 
+def test_halo_capacity_follows_the_largest_halo():
+    """The tile halo capacity grows when a halo exceeds it (those tiles are
+    computed from global memory meanwhile) and shrinks when it is well above
+    the largest halo; forces stay exact throughout."""
+    sysm = configs.ka_random_lattice(40000, density=1.2, seed=21)
+    conf = conf_from_system(sysm)
+    s, pot = ka_sample(conf, cutoff=rumd.ShiftedForce, method="sort")
+    f_ref, _ = oracle.forces_cells(configs.pack_pos4(sysm.positions, sysm.types), sysm.box, 2, ka_table(), SF)
+    for cap in (64, 3072):
+        print(f"[case] halo capacity {cap}", flush=True)
+        pot.SetNbHaloCapacity(cap)
+        s.CalcF()
+        assert force_errors(s.GetForces(), f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
+        adapted = pot.GetNbHaloCapacity()
+        s.CalcF()
+        assert force_errors(s.GetForces(), f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
+        assert adapted == pot.GetNbHaloCapacity()  # settled after one adaptation
+        if cap == 64:
+            assert adapted > 64
+            grown = adapted
+        else:
+            assert adapted == grown < cap
+
+
 def test_error_paths():
     s = rumd.Sample(0, 0)
     s.SetVerbose(False)
