@@ -316,6 +316,7 @@ def main() -> int:
 
     nbar, nmax = pot.GetNeighborListStats()
     force_ms = pot.TimeForceKernel(args.force_reps)
+    build_ms = pot.TimeNbListBuild(10)
     bytes_per_atom = 16 + 4 + 4 * nbar + 16 + 16
     achieved = bytes_per_atom * n / (force_ms * 1e-3) / 1e9
     roofline = {
@@ -389,6 +390,7 @@ def main() -> int:
             "list_builds": builds,
             "kbar": round(kbar, 2) if kbar else None,
             "halo_capacity": pot.GetNbHaloCapacity(),
+            "list_rebuild_ms": round(build_ms, 4),
             "model_bytes_per_atom_step": round(step_bytes, 1),
             "model_hbm_fraction_step": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
             "pe_per_particle": round(pe, 6),

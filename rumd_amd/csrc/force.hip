@@ -289,11 +289,6 @@ __global__ __launch_bounds__(kBlock) void k_lj_all(const float4* __restrict__ r,
   if (live) store_result<STR, INIT>(i, a, f, w, sts);
 }
 
-// Neighbour position of a byte entry: the halo positions start the dynamic LDS.
-__device__ __forceinline__ float4 halo_pos(const float4* s_pos, uint32_t entry) {
-  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pos) + entry);
-}
-
 // Forces on the particles of one tile whose halo positions are in LDS.
 // Waves take the tile's particles 64 >> lgp at a time in the build's lane
 // order (descending list length, so a wave's lists have similar lengths);

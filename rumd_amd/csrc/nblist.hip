@@ -341,11 +341,13 @@ struct HomeArrays {
 // cell being built and a window of its candidates (LDS offsets, in stencil
 // order).
 constexpr uint32_t kCandWindow = 512;
+constexpr uint32_t kStageMax = 256;  // list entries of one particle staged in LDS before the group stores
 struct WaveBuild {
   uint16_t cand[kCandWindow];
   uint16_t seg_start[128];  // LDS offset of the segment's first particle
   uint16_t seg_pre[128];    // candidates before the segment
   uint16_t seg_len[128];
+  __attribute__((aligned(16))) uint16_t list[kStageMax];
 };
 
 // Orders the LDS accesses of one wave (lanes read what other lanes wrote).
@@ -363,6 +365,84 @@ __device__ __forceinline__ uint32_t append_chunk(bool acc, uint32_t entry, uint3
                                                      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
   if (acc && k < max_nbrs) nbl16[(static_cast<size_t>(k >> 3) * nvp + i) * 8 + (k & 7u)] = static_cast<uint16_t>(entry);
   return cnt + static_cast<uint32_t>(__popcll(mask));
+}
+
+// Tests n chunks of 64 window candidates (from q0) against one particle and
+// appends the accepted ones to the wave's staged list; returns the new count.
+// The n chunks' LDS reads are issued together.
+template <bool WRAPS, int N>
+__device__ __forceinline__ uint32_t scan_chunks(const WaveBuild& wb, uint16_t* list, const float4* s_pos, float4 pi,
+                                                const float* crow, uint32_t self, const Box& bx, uint32_t q0,
+                                                uint32_t cnt, uint32_t max_nbrs) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t e[N];
+  float4 rj[N];
+  float cut[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) e[u] = wb.cand[q0 + 64u * u + lane];
+#pragma unroll
+  for (int u = 0; u < N; ++u) rj[u] = halo_pos(s_pos, e[u]);
+#pragma unroll
+  for (int u = 0; u < N; ++u) cut[u] = crow[__float_as_uint(rj[u].w)];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const float d2 = WRAPS ? min_image_d2_f32(pi, rj[u], bx) : plain_d2_f32(pi, rj[u]);
+    const bool acc = e[u] != self && d2 < cut[u];
+    const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
+    const uint32_t k = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+    if (acc && k < max_nbrs) list[k] = static_cast<uint16_t>(e[u]);
+    cnt += static_cast<uint32_t>(__builtin_popcountll(mask));
+  }
+  return cnt;
+}
+
+// The cooperative build of one cell when its whole stencil fits one window
+// and a list fits the wave's staging buffer: the window holds the
+// candidates' LDS byte offsets (the list entries themselves), padded to a
+// multiple of 64 with the halo's sentinel (a position at +inf, never
+// accepted), so chunks read without bounds checks; each particle's list is
+// staged in LDS and written as whole 16-byte groups.
+template <bool WRAPS>
+__device__ __forceinline__ void build_cell_staged(const float4* s_pos, const float* s_cut,
+                                                  uint32_t nt, const Box& bx, uint32_t coff, uint32_t m,
+                                                  uint32_t gstart, uint32_t nseg, uint32_t total, uint32_t sentinel,
+                                                  WaveBuild& wb, uint32_t* s_cnt, uint16_t* __restrict__ nbl16,
+                                                  uint32_t nvp, uint32_t max_nbrs) {
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t sg = lane; sg < nseg; sg += 64) {
+    const uint32_t pre = wb.seg_pre[sg], len = wb.seg_len[sg], st = wb.seg_start[sg];
+    for (uint32_t q = 0; q < len; ++q) wb.cand[pre + q] = static_cast<uint16_t>((st + q) << 4);
+  }
+  const uint32_t padded = (total + 63u) & ~63u;
+  if (total + lane < padded) wb.cand[total + lane] = static_cast<uint16_t>(sentinel);
+  wave_lds_sync();
+  const uint32_t full = padded & ~255u;  // passes of four chunks, then the rest
+  for (uint32_t p = 0; p < m; ++p) {
+    const uint32_t po = coff + p;
+    const float4 pi = s_pos[po];
+    const float* crow = s_cut + __float_as_uint(pi.w) * nt;
+    const uint32_t self = po << 4;
+    uint32_t cnt = 0;
+    for (uint32_t q0 = 0; q0 < full; q0 += 256)
+      cnt = scan_chunks<WRAPS, 4>(wb, wb.list, s_pos, pi, crow, self, bx, q0, cnt, max_nbrs);
+    uint32_t q0 = full;
+    if (padded - q0 >= 128) {
+      cnt = scan_chunks<WRAPS, 2>(wb, wb.list, s_pos, pi, crow, self, bx, q0, cnt, max_nbrs);
+      q0 += 128;
+    }
+    if (padded - q0 >= 64) cnt = scan_chunks<WRAPS, 1>(wb, wb.list, s_pos, pi, crow, self, bx, q0, cnt, max_nbrs);
+    wave_lds_sync();
+    // group g of the list to its 16 bytes at nbl16[(g nvp + i) 8]; entries of
+    // the last group past the count are never read
+    const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
+    if (lane * 8u < stored) {
+      const u32x4 g = *reinterpret_cast<const u32x4*>(wb.list + lane * 8u);
+      reinterpret_cast<u32x4*>(nbl16)[static_cast<size_t>(lane) * nvp + gstart + p] = g;
+    }
+    if (lane == 0) s_cnt[p] = cnt;
+    wave_lds_sync();  // the group stores have read the list
+  }
 }
 
 // Lists of one cell of a staged tile, built by one wave: lanes enumerate the
@@ -423,6 +503,11 @@ __device__ __forceinline__ void build_cell_wave(const Halo& h, const HaloLds& L,
   }
   const uint32_t total = __builtin_amdgcn_readfirstlane(carry);
   wave_lds_sync();
+  if (bytes && total <= kCandWindow && max_nbrs <= kStageMax) {
+    build_cell_staged<WRAPS>(s_pos, s_cut, nt, bx, coff, m, gstart, nseg, total, tg.halo_cap << 4, wb,
+                             s_cnt + (gstart - h.slot0), nbl16, nvp, max_nbrs);
+    return;
+  }
   for (uint32_t pg = 0; pg < m; pg += 64) {
     const uint32_t mg = (m - pg) < 64u ? (m - pg) : 64u;
     uint32_t cnt_v = 0;  // lane l: count of particle pg + l
@@ -501,6 +586,15 @@ __device__ __forceinline__ uint32_t build_particle_lane(uint32_t i, float4 ri, i
   return cnt;
 }
 
+// Workgroup of the tile build: one tile's halo positions are staged in LDS
+// once and shared by all its waves, so a larger workgroup has more waves in
+// flight per byte of LDS (the candidate loop is bound by LDS latency).
+#ifndef RUMDX_BUILD_BLOCK
+#define RUMDX_BUILD_BLOCK 512
+#endif
+constexpr uint32_t kBuildBlock = RUMDX_BUILD_BLOCK;
+static_assert(kBuildBlock % 64 == 0 && kBuildBlock >= kBlock && kBuildBlock <= kMaxWorkgroup, "build workgroup size");
+
 // calculateNBL_CellsSorted (src/NeighborList.cu:1204-1319), one workgroup per
 // cell tile with the tile's halo staged in LDS: for each particle, scan the
 // 5x5x5 cell stencil in RUMD's (dZ, dY, dX) order, ascending slot inside each
@@ -508,18 +602,21 @@ __device__ __forceinline__ uint32_t build_particle_lane(uint32_t i, float4 ri, i
 // minimum-image arithmetic; a tile whose halo does not wrap needs no image
 // correction, and the plain difference is then the same number).  Entries are
 // halo positions.  Each particle's sorted data is copied back to its home slot.
-__global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ flags, SortScratch s, TileGeom tg,
-                                                        const float* __restrict__ cut_skin2, uint32_t nt, Box bx,
-                                                        HomeArrays o, uint16_t* __restrict__ nbl16,
-                                                        uint32_t* __restrict__ num_nbrs, uint32_t nvp,
-                                                        uint32_t max_nbrs) {
+__global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restrict__ flags, SortScratch s,
+                                                             TileGeom tg, const float* __restrict__ cut_skin2,
+                                                             uint32_t nt, Box bx, HomeArrays o,
+                                                             uint16_t* __restrict__ nbl16,
+                                                             uint32_t* __restrict__ num_nbrs, uint32_t nvp,
+                                                             uint32_t max_nbrs) {
   if (!rebuild_pending(flags)) return;
   __shared__ HaloLds L;
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
   __shared__ uint32_t s_key[kOrderMax];  // list lengths, then sort keys (same index, same thread)
-  __shared__ WaveBuild s_wave[kBlock / 64];
+  __shared__ WaveBuild s_wave[kBuildBlock / 64];
   extern __shared__ float4 s_pos[];
-  for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_skin2[k];
+  for (uint32_t k = threadIdx.x; k < nt * nt; k += kBuildBlock) s_cut[k] = cut_skin2[k];
+  // the sentinel after the halo capacity: a position no distance test accepts
+  if (threadIdx.x == 0) s_pos[tg.halo_cap] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
   const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
   stage_halo(t, tg, s.t_r, s_pos, L);
   const Halo& h = L.h;
@@ -529,7 +626,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   // the halo in halo order, for the force kernel's staging
   if (staged) {
     uint32_t* hs = tg.halo_slots + static_cast<size_t>(t) * tg.halo_cap;
-    for (uint32_t c = threadIdx.x; c < h.ncell; c += kBlock) {
+    for (uint32_t c = threadIdx.x; c < h.ncell; c += kBuildBlock) {
       const uint32_t o0 = L.off[c], cnt = L.off[c + 1] - o0, g0 = L.start[c];
       for (uint32_t k = 0; k < cnt; ++k) hs[o0 + k] = g0 + k;
     }
@@ -541,7 +638,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int c0x = h.base[0] + 2, c0y = h.base[1] + 2, c0z = h.base[2] + 2;
     const uint32_t nown = static_cast<uint32_t>(h.own[0] * h.own[1] * h.own[2]);
-    for (uint32_t cc = wave; cc < nown; cc += kBlock / 64) {
+    for (uint32_t cc = wave; cc < nown; cc += kBuildBlock / 64) {
       const int a = static_cast<int>(cc % h.own[0]);
       const int b = static_cast<int>((cc / h.own[0]) % h.own[1]);
       const int c = static_cast<int>(cc / (h.own[0] * h.own[1]));
@@ -558,7 +655,7 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
     }
   }
   __syncthreads();
-  for (uint32_t i = h.slot0 + threadIdx.x; i < h.slot1; i += kBlock) {
+  for (uint32_t i = h.slot0 + threadIdx.x; i < h.slot1; i += kBuildBlock) {
     const float4 ri = s.t_r[i];
     const int cx = cell_coord(ri.x, bx.iL[0], tg.nc0);
     const int cy = cell_coord(ri.y, bx.iL[1], tg.nc1);
@@ -588,9 +685,9 @@ __global__ __launch_bounds__(kBlock) void k_build_tiles(uint32_t* __restrict__ f
   // Lane order of the force kernel: the tile's particles by descending list
   // length, ties by slot (a rank count over unique keys; deterministic).
   // block_max_u32 synchronises the workgroup, so every key is written.
-  __shared__ uint32_t s_max[kBlock / 64];
+  __shared__ uint32_t s_max[kBuildBlock / 64];
   longest = block_max_u32(longest, s_max);
-  for (uint32_t k = threadIdx.x; k < np; k += kBlock) {
+  for (uint32_t k = threadIdx.x; k < np; k += kBuildBlock) {
     uint32_t rk = k;
     if (np <= kOrderMax) {
       const uint32_t key = s_key[k];
@@ -796,7 +893,7 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
                         reinterpret_cast<float4*>(p->im_dev), reinterpret_cast<float4*>(p->rlo_dev),
                         p->unsorted_index_dev, reinterpret_cast<float4*>(nb->last_r_dev), nb->slot_cell_dev,
                         nb->lane_order_dev};
-  hipLaunchKernelGGL(k_build_tiles, dim3(nb->ntiles), dim3(kBlock), sizeof(float4) * nb->halo_cap, st, nb->flags_dev,
+  hipLaunchKernelGGL(k_build_tiles, dim3(nb->ntiles), dim3(kBuildBlock), sizeof(float4) * (nb->halo_cap + 1u), st, nb->flags_dev,
                      s, tg, nb->cut_skin2_dev, nb->ntypes, bx, home, nb->nbl16_dev, nb->num_nbrs_dev, p->nvp,
                      nb->max_nbrs);
   return check_launch("rumdx_nblist_build_sorted");

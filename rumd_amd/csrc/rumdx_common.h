@@ -12,6 +12,7 @@ namespace rumdx {
 
 // Threads per workgroup for the per-particle kernels: four wave64s.
 constexpr uint32_t kBlock = 256;
+constexpr uint32_t kMaxWorkgroup = 1024;  // largest workgroup of any kernel here
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs.
 constexpr uint32_t kNumXcd = 8;
 
@@ -105,7 +106,7 @@ __device__ __forceinline__ double block_sum(double v, double* lds4) {
   return s;
 }
 
-// Maximum of v over a kBlock-thread workgroup, valid in thread 0.  Flags
+// Maximum of v over the workgroup (lds4: blockDim.x / 64 entries), valid in thread 0.  Flags
 // that collect a maximum (the longest list) take one atomic per workgroup:
 // one per particle on a single address serialises at the memory side.
 __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* lds4) {
@@ -118,7 +119,7 @@ __device__ __forceinline__ uint32_t block_max_u32(uint32_t v, uint32_t* lds4) {
   __syncthreads();
   uint32_t m = 0;
   if (threadIdx.x == 0)
-    for (uint32_t k = 0; k < kBlock / 64; ++k) m = lds4[k] > m ? lds4[k] : m;
+    for (uint32_t k = 0; k < blockDim.x / 64; ++k) m = lds4[k] > m ? lds4[k] : m;
   __syncthreads();
   return m;
 }

@@ -76,7 +76,7 @@ struct Halo {
 struct HaloLds {
   uint32_t start[kHaloCellsMax];    // global first slot of each halo cell
   uint32_t off[kHaloCellsMax + 1];  // LDS offset of each halo cell's first particle
-  uint32_t wave_sum[kBlock / 64];
+  uint32_t wave_sum[kMaxWorkgroup / 64];
   Halo h;
 };
 
@@ -92,6 +92,10 @@ constexpr uint32_t kOrderMax = 1024;
 
 __device__ __forceinline__ uint32_t encode_entry(uint32_t h, bool bytes) { return bytes ? (h << 4) : h; }
 __device__ __forceinline__ uint32_t decode_entry(uint32_t e, bool bytes) { return bytes ? (e >> 4) : e; }
+// Halo position of a byte entry (s_pos: the staged halo positions).
+__device__ __forceinline__ float4 halo_pos(const float4* s_pos, uint32_t entry) {
+  return *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(s_pos) + entry);
+}
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint32_t group_entry(const u32x4& q, int e) {
@@ -125,7 +129,8 @@ __device__ __forceinline__ int halo_axis(int c, int base, int nc) {
 }
 
 // Fills the halo table of tile t and, if it fits, copies the halo particles'
-// positions from pos into s_pos.  Must be called by all threads of the block.
+// positions from pos into s_pos.  Must be called by all threads of the block
+// (at least 256, a multiple of 64, at most kMaxWorkgroup).
 __device__ __forceinline__ void stage_halo(uint32_t t, const TileGeom& g, const float4* __restrict__ pos,
                                            float4* s_pos, HaloLds& L) {
   if (threadIdx.x == 0) {
@@ -184,7 +189,7 @@ __device__ __forceinline__ void stage_halo(uint32_t t, const TileGeom& g, const 
   if (lane == 63) L.wave_sum[wave] = x;
   __syncthreads();
   uint32_t woff = 0, tot = 0;
-  for (uint32_t k = 0; k < kBlock / 64; ++k) {
+  for (uint32_t k = 0; k < blockDim.x / 64; ++k) {
     woff += (k < wave) ? L.wave_sum[k] : 0u;
     tot += L.wave_sum[k];
   }

@@ -115,6 +115,8 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetFP64Forces", &PairPotential::GetFP64Forces)
       .def("TimeForceKernel", &PairPotential::TimeForceKernel, py::arg("reps"),
            py::call_guard<py::gil_scoped_release>())
+      .def("TimeNbListBuild", &PairPotential::TimeNbListBuild, py::arg("reps"),
+           py::call_guard<py::gil_scoped_release>())
       .def("GetNeighborListStats", &PairPotential::GetNeighborListStats)
       .def("SetNbHaloCapacity", [](PairPotential& p, unsigned cap) { p.GetNeighborList().SetHaloCapacity(cap); },
            py::arg("cap"))

@@ -262,6 +262,12 @@ bool NeighborList::HandleOverflow() {
   return true;
 }
 
+void NeighborList::RequestRebuild() {
+  if (!sample_) throw RUMD_Error("NeighborList", "RequestRebuild", "no sample");
+  EnsureAllocated();
+  if (allocated_) rumdx_check(rumdx_nblist_request_rebuild(&desc_, sample_->GetStream()), "NeighborList", "RequestRebuild");
+}
+
 void NeighborList::SetHaloCapacity(unsigned cap) {
   if (cap > RUMDX_HALO_CAP_MAX) throw RUMD_Error("NeighborList", "SetHaloCapacity", "capacity above RUMDX_HALO_CAP_MAX");
   halo_cap_ = cap;
@@ -443,6 +449,30 @@ float PairPotential::TimeForceKernel(unsigned reps) {
   hip_check(hipEventSynchronize(b), "PairPotential", "TimeForceKernel");
   float ms = 0.f;
   hip_check(hipEventElapsedTime(&ms, a, b), "PairPotential", "TimeForceKernel");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+float PairPotential::TimeNbListBuild(unsigned reps) {
+  if (!sample) throw RUMD_Error(GetID_String(), "TimeNbListBuild", "no sample");
+  if (reps == 0) reps = 1;
+  neighborList.UpdateNBlist();
+  sample->Synchronize();  // settles list and halo capacities first
+  neighborList.UpdateNBlist();
+  const hipStream_t st = sample->GetStream();
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "PairPotential", "TimeNbListBuild");
+  hip_check(hipEventCreate(&b), "PairPotential", "TimeNbListBuild");
+  hip_check(hipEventRecord(a, st), "PairPotential", "TimeNbListBuild");
+  for (unsigned k = 0; k < reps; ++k) {
+    neighborList.RequestRebuild();
+    neighborList.UpdateNBlist();
+  }
+  hip_check(hipEventRecord(b, st), "PairPotential", "TimeNbListBuild");
+  hip_check(hipEventSynchronize(b), "PairPotential", "TimeNbListBuild");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, a, b), "PairPotential", "TimeNbListBuild");
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   return ms / reps;

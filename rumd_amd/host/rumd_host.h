@@ -174,6 +174,8 @@ class NeighborList {
   void ResetFusedCheck() { fused_check_pending_ = false; }
   // Reads the flags; grows the list after an overflow.  Synchronises.
   bool HandleOverflow();
+  // Asks the next UpdateNBlist to rebuild (device flag, on the sample's stream).
+  void RequestRebuild();
   bool IsSorted() const { return GetResolvedMethod() == SORT; }
   void DownloadCounts(std::vector<uint32_t>& out);
   // LDS halo capacity of the tile kernels, in particles (0 = estimate from
@@ -246,6 +248,10 @@ class PairPotential : public Potential {
   // the sample's stream, bracketed by HIP events (the list is brought up to
   // date first and not rebuilt in between).
   float TimeForceKernel(unsigned reps);
+  // Average duration (ms) of one full list rebuild (skin check, sort chain,
+  // build) over `reps` forced rebuilds of the current positions, bracketed
+  // by HIP events on the sample's stream.
+  float TimeNbListBuild(unsigned reps);
   // All-FP64 pair arithmetic (rumdx_pair_lj_fp64): the precision study's
   // comparison path; FP32 pair arithmetic otherwise.
   void SetFP64Forces(bool on);
