@@ -335,18 +335,29 @@ __device__ __forceinline__ void tile_particles(uint32_t slot0, uint32_t slot1, b
           ri.x = shift_coord(ri.x, shifts, 0, bx.L[0]);
           ri.y = shift_coord(ri.y, shifts, 1, bx.L[1]);
           ri.z = shift_coord(ri.z, shifts, 2, bx.L[2]);
-          for (uint32_t g = part; g < nfull; g += step, grp += gstride) {
-            const u32x4 q = __builtin_nontemporal_load(grp);
+          // this lane's groups part, part + step, ... below ng, the last one
+          // partial when it is the tail; each group's load is issued one
+          // group ahead, so its latency overlaps the previous 8 pair terms
+          const uint32_t ng = nfull + (tail ? 1u : 0u);
+          u32x4 q = {0u, 0u, 0u, 0u};
+          if (part < ng) q = __builtin_nontemporal_load(grp);
+          for (uint32_t g = part; g < ng; g += step) {
+            u32x4 qn = q;
+            if (g + step < ng) qn = __builtin_nontemporal_load(grp + gstride);
+            grp += gstride;
+            if (g < nfull) {
+              float4 rj[8];  // the group's positions first: eight LDS reads in flight
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-              pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
-          }
-          if (tail) {
-            const u32x4 q = __builtin_nontemporal_load(grp);
+              for (int e = 0; e < 8; ++e) rj[e] = halo_pos(s_pos, group_entry(q, e));
 #pragma unroll
-            for (int e = 0; e < 7; ++e)
-              if (static_cast<uint32_t>(e) < rem)
-                pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
+              for (int e = 0; e < 8; ++e) pair_term_direct<CUT, STR>(ri, rj[e], prow, a);
+            } else {
+#pragma unroll
+              for (int e = 0; e < 7; ++e)
+                if (static_cast<uint32_t>(e) < rem)
+                  pair_term_direct<CUT, STR>(ri, halo_pos(s_pos, group_entry(q, e)), prow, a);
+            }
+            q = qn;
           }
         } else {
           const double xi = ri.x, yi = ri.y, zi = ri.z;
