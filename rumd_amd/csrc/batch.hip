@@ -372,12 +372,13 @@ __global__ __launch_bounds__(kBlock) void k_b_nvt(BatchArgs a, const rumdx_nh_st
   if (i - rp.slot0 < rp.n) {
     const Box bx = rep_box(rp);
     const float ps = rp.omega2 > 0.f ? static_cast<float>(nh[rep].ps) : 0.f;
-    float4 r = a.r[i], v = a.v[i], im = a.im[i];
+    float4 r = a.r[i], v = a.v[i];
     float4 lo = make_float4(0.f, 0.f, 0.f, 0.f);
-    ke2 = nvt_particle<false>(r, v, a.f[i], im, lo, ps, dt, bx);
+    float3 shift;
+    ke2 = nvt_particle<false>(r, v, a.f[i], shift, lo, ps, dt, bx);
     a.r[i] = r;
     a.v[i] = v;
-    a.im[i] = im;
+    add_image_shift(a.im, i, shift);
     if (min_image_d2_f32(r, last_r[i], bx) > skin2_over_4) a.flags[kRepFlags * rep + kRebuild] = 1u;
   }
   const double s = block_sum(static_cast<double>(ke2), s_red);

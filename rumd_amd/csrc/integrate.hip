@@ -60,15 +60,16 @@ __global__ __launch_bounds__(kBlock) void k_nvt(NvtArgs a, Box bx) {
   float ke2 = 0.f;
   const float ps_all = THERMO ? static_cast<float>(a.nh->ps) : 0.f;
   if (i < a.n) {
-    float4 r = a.r[i], v = a.v[i], im = a.im[i];
+    float4 r = a.r[i], v = a.v[i];
     const float4 f = a.f[i];
     float4 lo = HILO ? a.rlo[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const bool apply = THERMO && ((a.on_mask >> (__float_as_uint(r.w) & 31u)) & 1u);
-    const float ke = nvt_particle<HILO>(r, v, f, im, lo, apply ? ps_all : 0.f, a.dt, bx);
+    float3 shift;
+    const float ke = nvt_particle<HILO>(r, v, f, shift, lo, apply ? ps_all : 0.f, a.dt, bx);
     if (apply) ke2 = ke;
     a.r[i] = r;
     a.v[i] = v;
-    a.im[i] = im;
+    add_image_shift(a.im, i, shift);
     if (HILO) a.rlo[i] = lo;
     if (SKIN) {
       if (min_image_d2_f32(r, a.last_r[i], bx) > a.skin2_over_4) a.nb_flags[RUMDX_FLAG_REBUILD] = 1u;

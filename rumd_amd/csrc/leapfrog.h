@@ -9,9 +9,11 @@ namespace rumdx {
 // One particle of integrateNVTAlgorithm, in source order without FMA
 // contraction (reproducible bit for bit on the host).  With HILO the position
 // is advanced and wrapped in FP64 from the hi/lo pair and stored back as a pair.
+// The image shift of this step goes to `shift` (all zero unless the particle
+// crossed the box), so the caller touches the image counts only then.
 template <bool HILO>
-__device__ __forceinline__ float nvt_particle(float4& r, float4& v, const float4 f, float4& im, float4& lo, float ps,
-                                              float dt, const Box& bx) {
+__device__ __forceinline__ float nvt_particle(float4& r, float4& v, const float4 f, float3& shift, float4& lo,
+                                              float ps, float dt, const Box& bx) {
 #pragma clang fp contract(off)
   const float factor = 0.5f * ps * dt;
   const float plus = 1.f / (1.f + factor);
@@ -49,10 +51,20 @@ __device__ __forceinline__ float nvt_particle(float4& r, float4& v, const float4
     iy = static_cast<float>(ky);
     iz = static_cast<float>(kz);
   }
-  im.x += ix;
-  im.y += iy;
-  im.z += iz;
+  shift = make_float3(ix, iy, iz);
   return (v.x * v.x + v.y * v.y + v.z * v.z) / v.w;
+}
+
+// Adds a wrap's image shift to the particle's image counts (RUMD's image
+// array); most steps move no particle across the box, and then nothing is read.
+__device__ __forceinline__ void add_image_shift(float4* __restrict__ im, uint32_t i, float3 shift) {
+  if (shift.x != 0.f || shift.y != 0.f || shift.z != 0.f) {
+    float4 m = im[i];
+    m.x += shift.x;
+    m.y += shift.y;
+    m.z += shift.z;
+    im[i] = m;
+  }
 }
 
 }  // namespace rumdx
