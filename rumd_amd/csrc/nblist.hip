@@ -164,6 +164,27 @@ __device__ __forceinline__ uint32_t cell_of(float4 rr, const Box& bx, const Cell
   return static_cast<uint32_t>(cx + cy * g.nc0 + cz * g.nc0 * g.nc1);
 }
 
+// Runs of equal keys among the live lanes of a wave (consecutive lanes with
+// the same key).  Particles stay in cell order between builds, so a wave's
+// 64 particles fall into a few cells: one atomic per run instead of one per
+// particle, and no lanes of one instruction on the same address.  Returns the
+// lane's index inside its run; `len` is the run's length and `head` its
+// first lane.  Called by all lanes of the wave.
+__device__ __forceinline__ uint32_t key_run(uint32_t key, bool live, uint32_t& len, uint32_t& head) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  const bool prev_live = __shfl_up(live ? 1 : 0, 1, 64) != 0;
+  const bool start = live && (lane == 0 || !prev_live || prev != key);
+  const uint64_t starts = __builtin_amdgcn_ballot_w64(start);
+  const uint64_t bounds = starts | __builtin_amdgcn_ballot_w64(!live);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);  // lanes 0..lane
+  head = 63u - static_cast<uint32_t>(__builtin_clzll((starts & upto) | 1ull));
+  const uint64_t above = bounds & ~upto;
+  const uint32_t end = above != 0ull ? static_cast<uint32_t>(__builtin_ctzll(above)) : 64u;
+  len = end - head;
+  return lane - head;
+}
+
 // calculateCellIndices (src/NeighborList.cu:1142-1172) plus the histogram.
 __global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__ r, uint32_t* __restrict__ flags,
                                                       const uint32_t* __restrict__ cell_rank,
@@ -173,10 +194,18 @@ __global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__
   if (!rebuild_pending(flags)) return;
   // the tile build then records the largest halo of this build
   if (blockIdx.x == 0 && threadIdx.x == 0) flags[RUMDX_FLAG_MAXHALO] = 0u;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const uint32_t lin = cell_of(r[i], bx, g);
-    key_lin[i] = lin;
-    atomicAdd(cell_count + cell_rank[lin], 1u);
+  for (uint32_t b0 = blockIdx.x * kBlock; b0 < n; b0 += gridDim.x * kBlock) {
+    const uint32_t i = b0 + threadIdx.x;
+    const bool live = i < n;
+    uint32_t rank = 0;
+    if (live) {
+      const uint32_t lin = cell_of(r[i], bx, g);
+      key_lin[i] = lin;
+      rank = cell_rank[lin];
+    }
+    uint32_t len, head;
+    const uint32_t pos = key_run(rank, live, len, head);
+    if (live && pos == 0) atomicAdd(cell_count + rank, len);
   }
 }
 
@@ -286,10 +315,18 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const uint32_t* __restrict__
                                                     uint32_t* __restrict__ cell_count, uint32_t* __restrict__ perm,
                                                     uint32_t n) {
   if (!rebuild_pending(flags)) return;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const uint32_t rank = cell_rank[key_lin[i]];
-    const uint32_t slot = cell_start[rank] + atomicSub(cell_count + rank, 1u) - 1u;
-    if (slot < n) perm[slot] = i;  // always true when hist and scatter saw the same keys
+  for (uint32_t b0 = blockIdx.x * kBlock; b0 < n; b0 += gridDim.x * kBlock) {
+    const uint32_t i = b0 + threadIdx.x;
+    const bool live = i < n;
+    const uint32_t rank = live ? cell_rank[key_lin[i]] : 0u;
+    uint32_t len, head;
+    const uint32_t pos = key_run(rank, live, len, head);
+    // the run's first lane takes len cursor values at once
+    uint32_t top = 0;
+    if (live && pos == 0) top = atomicSub(cell_count + rank, len);
+    top = __shfl(top, static_cast<int>(head), 64);
+    const uint32_t slot = cell_start[rank] + top - len + pos;
+    if (live && slot < n) perm[slot] = i;  // always in range when hist and scatter saw the same keys
   }
 }
 
