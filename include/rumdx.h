@@ -259,7 +259,10 @@ int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_mask, doubl
  * Replica r owns slots [slot0, slot0 + n); slot0 is a multiple of
  * RUMDX_BATCH_ALIGN and the range is padded to ceil(n / RUMDX_BATCH_ALIGN)
  * blocks, so no workgroup spans two replicas and each replica's sums have a
- * fixed order: a replica's trajectory does not depend on the others. */
+ * fixed order: a replica's trajectory does not depend on the others.  Each
+ * rebuild sorts the replica's particles by cell inside its slot range (a
+ * stable counting sort); idx_dev follows the particles, so idx_dev[slot]
+ * names the replica-local particle a slot holds. */
 #define RUMDX_BATCH_ALIGN 256
 #define RUMDX_BATCH_FLAGS 8  /* u32 per replica: rebuild, overflow, builds, longest list, internal */
 
@@ -295,11 +298,15 @@ typedef struct rumdx_batch {
   float skin;
   uint32_t ncells;               /* sum over replicas of nc^3 */
   uint32_t* cell_count_dev;      /* ncells, zero between builds */
-  uint32_t* cell_start_dev;      /* ncells */
+  uint32_t* cell_start_dev;      /* ncells: first slot of each cell (cells are contiguous slot ranges) */
   uint32_t* cell_n_dev;          /* ncells */
-  uint32_t* cell_list_dev;       /* nslots: slots of each cell, ascending */
   uint32_t* cell_tmp_dev;        /* nslots */
   uint32_t* slot_cell_dev;       /* nslots: cell of each slot at its replica's last build */
+  uint32_t* idx_dev;             /* nslots: replica-local id of the particle in each slot */
+  float* t_r_dev;                /* nslots x 4: sort staging of r, v, im and idx */
+  float* t_v_dev;
+  float* t_im_dev;
+  uint32_t* t_idx_dev;
 } rumdx_batch;
 
 /* Bytes of device workspace rumdx_batch_nvt_step / rumdx_batch_energies need. */

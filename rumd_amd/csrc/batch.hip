@@ -13,12 +13,13 @@
 // workgroup never spans two replicas.  block_rep maps a workgroup to its
 // replica; per-replica sums are block partials added in a fixed order by one
 // workgroup per replica, so every replica's trajectory is independent of the
-// others and reproducible bit for bit.  Particles are not reordered: a
-// replica holds at most a few thousand particles, whose positions sit in L2,
-// so a rebuild only bins slots into cells (a stable counting sort of slot
-// indices) and rescans the stencil.  Lists are u32 slots, column-major over
-// the whole batch (entry k of slot i at k*nslots + i).  Only replicas whose
-// rebuild flag is set run the build kernels; the others exit at once.
+// others and reproducible bit for bit.  A rebuild sorts the replica's
+// particles by cell inside its slot range (a stable counting sort staged
+// through t_*, with idx following the particles), so a cell is a contiguous
+// slot range and a particle's neighbours sit in a few contiguous runs of
+// memory.  Lists are u32 slots, column-major over the whole batch (entry k
+// of slot i at k*nslots + i).  Only replicas whose rebuild flag is set run
+// the build kernels; the others exit at once.
 #include "leapfrog.h"
 #include "rumdx_common.h"
 
@@ -134,14 +135,24 @@ __global__ __launch_bounds__(kBlock) void k_b_scatter(BatchArgs a, const uint32_
   cell_tmp[cell_start[c] + atomicAdd(cell_count + c, 1u)] = i;
 }
 
-// Puts each cell's slots in ascending order (a stable counting sort) and
-// zeroes the counts for the next build.
-__global__ __launch_bounds__(kBlock) void k_b_order(BatchArgs a, const uint32_t* __restrict__ slot_cell,
-                                                    const uint32_t* __restrict__ cell_start,
-                                                    const uint32_t* __restrict__ cell_n,
-                                                    uint32_t* __restrict__ cell_count,
-                                                    const uint32_t* __restrict__ cell_tmp,
-                                                    uint32_t* __restrict__ cell_list) {
+// Sort staging: the particle arrays of rebuilding replicas in cell order.
+struct SortStage {
+  float4* r;
+  float4* v;
+  float4* im;
+  uint32_t* idx;
+};
+
+// Gathers every particle of a rebuilding replica into its sorted slot: the
+// cell's first slot plus the number of the cell's particles with a smaller
+// old slot (a stable counting sort), and zeroes the counts for the next
+// build.
+__global__ __launch_bounds__(kBlock) void k_b_gather(BatchArgs a, const uint32_t* __restrict__ slot_cell,
+                                                     const uint32_t* __restrict__ cell_start,
+                                                     const uint32_t* __restrict__ cell_n,
+                                                     uint32_t* __restrict__ cell_count,
+                                                     const uint32_t* __restrict__ cell_tmp,
+                                                     const uint32_t* __restrict__ idx, SortStage st) {
   const uint32_t rep = a.block_rep[blockIdx.x];
   if (!rebuild_pending(a.flags, rep)) return;
   const rumdx_replica rp = a.rep[rep];
@@ -152,20 +163,24 @@ __global__ __launch_bounds__(kBlock) void k_b_order(BatchArgs a, const uint32_t*
   const uint32_t b0 = cell_start[c], m = cell_n[c];
   uint32_t d = b0;
   for (uint32_t j = b0; j < b0 + m; ++j) d += cell_tmp[j] < x ? 1u : 0u;
-  cell_list[d] = x;
+  st.r[d] = a.r[x];
+  st.v[d] = a.v[x];
+  st.im[d] = a.im[x];
+  st.idx[d] = idx[x];
   cell_count[c] = 0u;
 }
 
-// The lists of a rebuilding replica: RUMD's 5^3 stencil in (dZ, dY, dX)
-// order, ascending slot inside each cell, FP32 minimum-image criterion.  The
+// The lists of a rebuilding replica from its sorted staging: RUMD's 5^3
+// stencil in (dZ, dY, dX) order, ascending slot inside each cell, FP32
+// minimum-image criterion; then each particle moves to its sorted slot.  The
 // last workgroup of the replica clears its rebuild request.
-__global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t* __restrict__ slot_cell,
+__global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, uint32_t* __restrict__ slot_cell,
                                                     const uint32_t* __restrict__ cell_start,
-                                                    const uint32_t* __restrict__ cell_n,
-                                                    const uint32_t* __restrict__ cell_list,
-                                                    const float* __restrict__ cut_skin2, uint32_t nt,
-                                                    uint32_t* __restrict__ nbl, uint32_t* __restrict__ num_nbrs,
-                                                    float4* __restrict__ last_r, uint32_t max_nbrs) {
+                                                    const uint32_t* __restrict__ cell_n, SortStage st,
+                                                    uint32_t* __restrict__ idx, const float* __restrict__ cut_skin2,
+                                                    uint32_t nt, uint32_t* __restrict__ nbl,
+                                                    uint32_t* __restrict__ num_nbrs, float4* __restrict__ last_r,
+                                                    uint32_t max_nbrs) {
   const uint32_t rep = a.block_rep[blockIdx.x];
   if (!rebuild_pending(a.flags, rep)) return;
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
@@ -178,10 +193,10 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
   uint32_t longest = 0;
   if (i - rp.slot0 < rp.n) {
     const Box bx = rep_box(rp);
-    const float4 ri = a.r[i];
+    const float4 ri = st.r[i];
     const float* crow = s_cut + __float_as_uint(ri.w) * nt;
     const int nc = rp.nc;
-    const uint32_t lc = slot_cell[i] - rp.cell0;
+    const uint32_t lc = rep_cell(ri, bx, nc);  // the cell the histogram gave this particle
     const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc), cz = static_cast<int>(lc / (nc * nc));
     uint32_t cnt = 0;
     auto consider = [&](uint32_t j, float4 rj) {
@@ -190,23 +205,19 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
         ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
       }
     };
-    // a contiguous range of cell_list, four candidates' loads in flight at once
+    // a contiguous slot range, four candidates' loads in flight at once
     auto scan = [&](uint32_t k0, uint32_t k1) {
       uint32_t k = k0;
       for (; k + 4 <= k1; k += 4) {
-        const uint32_t j0 = cell_list[k], j1 = cell_list[k + 1], j2 = cell_list[k + 2], j3 = cell_list[k + 3];
-        const float4 r0 = a.r[j0], r1 = a.r[j1], r2 = a.r[j2], r3 = a.r[j3];
-        consider(j0, r0);
-        consider(j1, r1);
-        consider(j2, r2);
-        consider(j3, r3);
+        const float4 r0 = st.r[k], r1 = st.r[k + 1], r2 = st.r[k + 2], r3 = st.r[k + 3];
+        consider(k, r0);
+        consider(k + 1, r1);
+        consider(k + 2, r2);
+        consider(k + 3, r3);
       }
-      for (; k < k1; ++k) {
-        const uint32_t j = cell_list[k];
-        consider(j, a.r[j]);
-      }
+      for (; k < k1; ++k) consider(k, st.r[k]);
     };
-    // Cells are numbered x fastest and cell_list follows that order, so the
+    // Cells are numbered x fastest and sorted slots follow that order, so the
     // five cells (cx-2..cx+2, y, z) are one contiguous range unless the row
     // wraps around the box; then the cells go one by one in dX order.
     const bool row = cx >= 2 && cx + 2 < nc;
@@ -227,9 +238,15 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
       }
     }
     num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;
-    last_r[i] = ri;
     if (cnt >= max_nbrs) fl[kOverflow] = 1u;
     longest = cnt;
+    // the particle moves to its sorted slot
+    a.r[i] = ri;
+    a.v[i] = st.v[i];
+    a.im[i] = st.im[i];
+    idx[i] = st.idx[i];
+    slot_cell[i] = rp.cell0 + lc;
+    last_r[i] = ri;
   }
   longest = block_max_u32(longest, s_max);
   if (threadIdx.x == 0) atomicMax(fl + kMaxLen, longest);
@@ -293,7 +310,6 @@ struct CellArrays {
   const uint32_t* slot_cell;
   const uint32_t* cell_start;
   const uint32_t* cell_n;
-  const uint32_t* cell_list;
 };
 
 template <int CUT>
@@ -332,10 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t*
           const uint32_t c = rp.cell0 + static_cast<uint32_t>((cx + dx + nc) % nc +
                                                               nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc)));
           const uint32_t b0 = cells.cell_start[c], m = cells.cell_n[c];
-          for (uint32_t q = b0; q < b0 + m; ++q) {
-            const uint32_t j = cells.cell_list[q];
+          for (uint32_t j = b0; j < b0 + m; ++j)
             if (j != i) pair_b<CUT>(xi, yi, zi, a.r[j], prow, bx, acc);
-          }
         }
     a.f[i] = make_float4(acc.fx, acc.fy, acc.fz, 0.5f * acc.u);
     a.w[i] = make_float4(0.f, 0.f, 0.f, acc.vir);
@@ -486,7 +500,8 @@ extern "C" int rumdx_batch_update_lists(rumdx_batch* b, void* stream) {
   if (rc != RUMDX_OK) return rc;
   if (b->nbl_dev == nullptr || b->num_nbrs_dev == nullptr || b->last_r_dev == nullptr || b->cut_skin2_dev == nullptr ||
       b->cell_count_dev == nullptr || b->cell_start_dev == nullptr || b->cell_n_dev == nullptr ||
-      b->cell_list_dev == nullptr || b->cell_tmp_dev == nullptr || b->slot_cell_dev == nullptr)
+      b->cell_tmp_dev == nullptr || b->slot_cell_dev == nullptr || b->idx_dev == nullptr || b->t_r_dev == nullptr ||
+      b->t_v_dev == nullptr || b->t_im_dev == nullptr || b->t_idx_dev == nullptr)
     return fail(RUMDX_E_INVALID, "rumdx_batch_update_lists: list arrays not allocated");
   if (b->max_nbrs == 0 || b->ntypes == 0 || b->ntypes > RUMDX_MAX_TYPES)
     return fail(RUMDX_E_INVALID, "rumdx_batch_update_lists: max_nbrs or ntypes out of range");
@@ -500,10 +515,12 @@ extern "C" int rumdx_batch_update_lists(rumdx_batch* b, void* stream) {
                      b->cell_n_dev);
   hipLaunchKernelGGL(k_b_scatter, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
                      b->cell_count_dev, b->cell_tmp_dev);
-  hipLaunchKernelGGL(k_b_order, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
-                     b->cell_n_dev, b->cell_count_dev, b->cell_tmp_dev, b->cell_list_dev);
+  const SortStage stage{reinterpret_cast<float4*>(b->t_r_dev), reinterpret_cast<float4*>(b->t_v_dev),
+                        reinterpret_cast<float4*>(b->t_im_dev), b->t_idx_dev};
+  hipLaunchKernelGGL(k_b_gather, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
+                     b->cell_n_dev, b->cell_count_dev, b->cell_tmp_dev, b->idx_dev, stage);
   hipLaunchKernelGGL(k_b_build, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
-                     b->cell_n_dev, b->cell_list_dev, b->cut_skin2_dev, b->ntypes, b->nbl_dev, b->num_nbrs_dev,
+                     b->cell_n_dev, stage, b->idx_dev, b->cut_skin2_dev, b->ntypes, b->nbl_dev, b->num_nbrs_dev,
                      reinterpret_cast<float4*>(b->last_r_dev), b->max_nbrs);
   return check_launch("rumdx_batch_update_lists");
 }
@@ -514,14 +531,13 @@ extern "C" int rumdx_batch_forces(rumdx_batch* b, const float* params_dev, int c
   if (params_dev == nullptr || b->nbl_dev == nullptr || b->num_nbrs_dev == nullptr)
     return fail(RUMDX_E_INVALID, "rumdx_batch_forces: null params or list");
   if (b->ntypes == 0 || b->ntypes > RUMDX_MAX_TYPES) return fail(RUMDX_E_INVALID, "rumdx_batch_forces: ntypes out of range");
-  if (b->slot_cell_dev == nullptr || b->cell_start_dev == nullptr || b->cell_n_dev == nullptr ||
-      b->cell_list_dev == nullptr || b->max_nbrs == 0)
+  if (b->slot_cell_dev == nullptr || b->cell_start_dev == nullptr || b->cell_n_dev == nullptr || b->max_nbrs == 0)
     return fail(RUMDX_E_INVALID, "rumdx_batch_forces: cell arrays not allocated");
   const BatchArgs a = args_of(b);
   const hipStream_t st = as_stream(stream);
   const uint32_t nblk = b->nslots / kBlock;
   const size_t lds = sizeof(float4) * 2 * b->ntypes * b->ntypes;
-  const CellArrays cells{b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev, b->cell_list_dev};
+  const CellArrays cells{b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev};
   switch (cutoff_method) {
     case RUMDX_CUTOFF_NS:
       hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_NS>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,

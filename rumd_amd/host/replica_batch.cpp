@@ -176,6 +176,7 @@ void ReplicaBatch::Allocate() {
   std::vector<rumdx_replica> hrep(nrep);
   std::vector<rumdx_nh_state> hnh(nrep);
   std::vector<uint32_t> hflags(static_cast<size_t>(RUMDX_BATCH_FLAGS) * nrep, 0u);
+  std::vector<uint32_t> hidx(ns, 0u);
   for (unsigned r = 0; r < nrep; ++r) {
     const Spec& s = reps_[r];
     hrep[r] = HostReplica(r);
@@ -184,6 +185,7 @@ void ReplicaBatch::Allocate() {
     const uint32_t n = static_cast<uint32_t>(s.types.size());
     for (uint32_t b = slot0_[r] / RUMDX_BATCH_ALIGN; b < (slot0_[r] + padded(n)) / RUMDX_BATCH_ALIGN; ++b) hb[b] = r;
     for (uint32_t i = 0; i < n; ++i) {
+      hidx[slot0_[r] + i] = i;
       const size_t k = 4ull * (slot0_[r] + i);
       hr[k] = s.pos[3ull * i];
       hr[k + 1] = s.pos[3ull * i + 1];
@@ -218,9 +220,14 @@ void ReplicaBatch::Allocate() {
   cell_count_.resize(ncells_);
   cell_start_.resize(ncells_);
   cell_n_.resize(ncells_);
-  cell_list_.resize(ns);
   cell_tmp_.resize(ns);
   slot_cell_.resize(ns);
+  idx_.resize(ns);
+  idx_.upload(hidx.data(), hidx.size(), stream_);
+  t_r_.resize(4 * ns);
+  t_v_.resize(4 * ns);
+  t_im_.resize(4 * ns);
+  t_idx_.resize(ns);
   params_.resize(table_.size());
   params_.upload(table_.data(), table_.size(), stream_);
   std::vector<float> c2(static_cast<size_t>(ntypes_) * ntypes_);
@@ -256,7 +263,11 @@ void ReplicaBatch::Allocate() {
   desc_.cell_count_dev = cell_count_.get();
   desc_.cell_start_dev = cell_start_.get();
   desc_.cell_n_dev = cell_n_.get();
-  desc_.cell_list_dev = cell_list_.get();
+  desc_.idx_dev = idx_.get();
+  desc_.t_r_dev = t_r_.get();
+  desc_.t_v_dev = t_v_.get();
+  desc_.t_im_dev = t_im_.get();
+  desc_.t_idx_dev = t_idx_.get();
   desc_.cell_tmp_dev = cell_tmp_.get();
   desc_.slot_cell_dev = slot_cell_.get();
   allocated_ = true;
@@ -340,6 +351,18 @@ void ReplicaBatch::Synchronize() {
   flags_.upload(fl.data(), fl.size(), stream_);
 }
 
+std::vector<uint32_t> ReplicaBatch::SlotIds(unsigned replica) {
+  const unsigned n = GetNumberOfParticles(replica);
+  std::vector<uint32_t> id(n);
+  hip_check(hipMemcpyAsync(id.data(), idx_.get() + slot0_[replica], n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           stream_),
+            "ReplicaBatch", "Get");
+  hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "Get");
+  for (uint32_t k : id)
+    if (k >= n) throw RUMD_Error("ReplicaBatch", "Get", "slot ids are not a permutation");
+  return id;
+}
+
 std::vector<float> ReplicaBatch::Slots(const DeviceArray<float>& a, unsigned replica, unsigned comps) {
   if (!allocated_) Allocate();
   if (replica >= reps_.size()) throw RUMD_Error("ReplicaBatch", "Get", "replica index out of range");
@@ -350,9 +373,10 @@ std::vector<float> ReplicaBatch::Slots(const DeviceArray<float>& a, unsigned rep
                            hipMemcpyDeviceToHost, stream_),
             "ReplicaBatch", "Get");
   hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "Get");
+  const std::vector<uint32_t> id = SlotIds(replica);
   std::vector<float> out(static_cast<size_t>(comps) * n);
-  for (unsigned i = 0; i < n; ++i)
-    for (unsigned c = 0; c < comps; ++c) out[static_cast<size_t>(comps) * i + c] = all[4ull * i + c];
+  for (unsigned s = 0; s < n; ++s)
+    for (unsigned c = 0; c < comps; ++c) out[static_cast<size_t>(comps) * id[s] + c] = all[4ull * s + c];
   return out;
 }
 
@@ -400,9 +424,12 @@ void ReplicaBatch::GetNeighborList(unsigned replica, std::vector<uint32_t>& offs
   std::vector<uint32_t> cnt(n);
   hip_check(hipMemcpyAsync(cnt.data(), num_nbrs_.get() + s0, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_),
             "ReplicaBatch", "GetNeighborList");
+  const std::vector<uint32_t> id = SlotIds(replica);  // synchronises
   std::vector<uint32_t> col(n);
+  // rows in particle order: row id[s] holds slot s's list, entries as particle ids
   offsets.assign(n + 1, 0u);
-  for (unsigned i = 0; i < n; ++i) offsets[i + 1] = offsets[i] + cnt[i];
+  for (unsigned s = 0; s < n; ++s) offsets[id[s] + 1] = cnt[s];
+  for (unsigned i = 0; i < n; ++i) offsets[i + 1] += offsets[i];
   ids.assign(offsets[n], 0u);
   unsigned kmax = 0;
   for (uint32_t c : cnt) kmax = std::max(kmax, c);
@@ -411,7 +438,7 @@ void ReplicaBatch::GetNeighborList(unsigned replica, std::vector<uint32_t>& offs
                              hipMemcpyDeviceToHost, stream_),
               "ReplicaBatch", "GetNeighborList");
     hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "GetNeighborList");
-    for (unsigned i = 0; i < n; ++i)
-      if (k < cnt[i]) ids[offsets[i] + k] = col[i] - s0;
+    for (unsigned s = 0; s < n; ++s)
+      if (k < cnt[s]) ids[offsets[id[s]] + k] = id[col[s] - s0];
   }
 }

@@ -65,6 +65,9 @@ class ReplicaBatch {
   void InvalidateGraphs();
   rumdx_replica HostReplica(unsigned r) const;
   std::vector<float> Slots(const DeviceArray<float>& a, unsigned replica, unsigned comps);
+  // Replica-local particle id held in each of the replica's slots (the spatial
+  // sort of every rebuild moves particles between slots).
+  std::vector<uint32_t> SlotIds(unsigned replica);
 
   unsigned ntypes_;
   std::vector<Spec> reps_;
@@ -79,9 +82,9 @@ class ReplicaBatch {
   std::vector<uint32_t> slot0_, cell0_;
   std::vector<int32_t> nc_;
   uint32_t nslots_ = 0, ncells_ = 0;
-  DeviceArray<float> r_, v_, f_, w_, im_, last_r_, params_, cut2_;
-  DeviceArray<uint32_t> block_rep_, flags_, nbl_, num_nbrs_, cell_count_, cell_start_, cell_n_, cell_list_,
-      cell_tmp_, slot_cell_;
+  DeviceArray<float> r_, v_, f_, w_, im_, last_r_, params_, cut2_, t_r_, t_v_, t_im_;
+  DeviceArray<uint32_t> block_rep_, flags_, nbl_, num_nbrs_, cell_count_, cell_start_, cell_n_, cell_tmp_,
+      slot_cell_, idx_, t_idx_;
   DeviceArray<rumdx_replica> rep_;
   DeviceArray<rumdx_nh_state> nh_;
   DeviceArray<double> workspace_, energies_;
