@@ -218,6 +218,15 @@ def batched_leg(nrep: int, warmup: int, steps: int, sample_reps: int, single_ste
     b = make_batch(systems, temps, 0.0025, 0.2, 0.3)
     ntot = b.GetTotalParticles()
     b.RunSteps(warmup, 10)
+    # lanes per particle of the batch force kernel, tuned on this state (the
+    # trials run between a device-side save and restore of the batch state)
+    import tempfile
+    from rumd_amd.Autotune import Autotune
+    with tempfile.TemporaryDirectory() as td:
+        at = Autotune(auto_filename=Path(td) / "autotune.json", ignore_file=True, trial_seconds=0.15)
+        tuned = at.TuneBatch(b, block=10)
+    trials = ", ".join("%d: %.3f" % (t["params"]["tp"], t["ms_per_step"]) for t in at.trials)
+    print(f"[bench] batch autotune: lanes per particle {tuned['tp']} ({trials} ms/step)", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     b.RunSteps(steps, 10)
     el = time.perf_counter() - t0
@@ -262,6 +271,8 @@ def batched_leg(nrep: int, warmup: int, steps: int, sample_reps: int, single_ste
                           "how": "a Sample per replica (default list method), graph blocks of 10, timed per replica "
                                  "and aggregated over an evenly spaced subset"},
         "speedup": round(batched / single, 2),
+        "lanes_per_particle": tuned["tp"],
+        "autotune_ms_per_step": {str(t["params"]["tp"]): round(t["ms_per_step"], 4) for t in at.trials},
         "workload": "KA replicas N~U[256,8192] (seed 4001), T~U[0.4,2.0] (seed 4002), rho=1.2, SF rc=2.5, "
                     "NH tau=0.2, dt=0.0025, skin=0.3, FP32 positions",
     }

@@ -21,6 +21,12 @@ Typical use, as with RUMD:
 Tuning runs advance a copy of the state: the configuration and the
 thermostat state are restored afterwards, so the user's trajectory is the
 one it would have been with the tuned parameters set from the start.
+
+TuneBatch(batch) does the same for a ReplicaBatch (batched replicas, which
+RUMD does not have): it times lanes per particle of the batch force kernel
+over graph blocks, keyed by the replicas' sizes, boxes, thermostats, time
+step, skin and potential, with the batch state saved and restored on the
+device around the trials.
 """
 from __future__ import annotations
 
@@ -117,6 +123,59 @@ class Autotune:
         entries = [e for e in self._read_file() if not _same_user(e.get("user", {}), user)]
         entries.append({"user": user, "tech": tech, "trials": self.trials})
         self.auto_filename.write_text(json.dumps(entries, indent=1))
+
+    # ------------------------------------------------------ batched replicas
+    def TuneBatch(self, batch, block: int = 10) -> dict:
+        """Lanes per particle of a ReplicaBatch's force kernel: the saved
+        choice when the user parameters match, else the fastest of
+        {1, 2, 4, 8} (or the fixed tp) over timed graph blocks."""
+        user = self.GetBatchUserParameters(batch)
+        tech = None if self.ignore_file else self.CompareSavedParameters(user)
+        if tech is None:
+            tps = [self.fixed["tp"]] if "tp" in self.fixed else list(_TP)
+            self.trials = []
+            batch.SaveState()
+            try:
+                nsteps = None
+                best = None
+                for tp in tps:
+                    batch.SetLanesPerParticle(int(tp))
+                    batch.RunSteps(2 * block, block)  # warm-up: list builds, graph capture
+                    if nsteps is None:
+                        t0 = time.perf_counter()
+                        batch.RunSteps(2 * block, block)
+                        per = (time.perf_counter() - t0) / (2 * block)
+                        nsteps = max(2 * block, min(2000, block * math.ceil(self.trial_seconds / max(per, 1e-6) / block)))
+                    t0 = time.perf_counter()
+                    batch.RunSteps(nsteps, block)
+                    ms = (time.perf_counter() - t0) * 1e3 / nsteps
+                    batch.RestoreState()
+                    self.trials.append({"params": {"tp": int(tp)}, "ms_per_step": ms})
+                    if self.verbose:
+                        print(f"[Autotune] batch tp={tp} -> {ms:.4f} ms/step")
+                    if best is None or ms < best[0]:
+                        best = (ms, int(tp))
+            except BaseException:
+                batch.RestoreState()
+                raise
+            tech = {"tp": best[1]}
+            self.WriteParametersToFile(user, tech)
+        batch.SetLanesPerParticle(int(tech["tp"]))
+        return tech
+
+    def GetBatchUserParameters(self, batch) -> dict:
+        nrep = batch.GetNumberOfReplicas()
+        return {
+            "deviceName": rumd.Device.GetDevice().GetDeviceName(),
+            "kind": "ReplicaBatch",
+            "N": [batch.GetNumberOfParticles(r) for r in range(nrep)],
+            "L": [float(batch.GetBoxLength(r)) for r in range(nrep)],
+            "thermostats": [[float(x) for x in batch.GetThermostat(r)] for r in range(nrep)],
+            "timeStep": float(batch.GetTimeStep()),
+            "skin": float(batch.GetSkin()),
+            "cutoffMethod": int(batch.GetCutoffMethod()),
+            "params": [float(x) for x in batch.GetParamsTable()],
+        }
 
     # --------------------------------------------------------- optimisation
     def OptimizeTechnicalParameters(self, sim) -> dict:

@@ -312,11 +312,15 @@ struct CellArrays {
   const uint32_t* cell_n;
 };
 
+// One particle's list over 2^lgp lanes (lane part takes entries part,
+// part + 2^lgp, ...), combined by a fixed butterfly: the workgroup covers
+// kBlock >> lgp slots, all of one replica (its 256-slot block).  An
+// overflowed list rescans the build-time stencil in its first lane.
 template <int CUT>
 __global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t* __restrict__ nbl,
                                                     const uint32_t* __restrict__ num_nbrs, uint32_t max_nbrs,
                                                     CellArrays cells, const float* __restrict__ params,
-                                                    uint32_t nt) {
+                                                    uint32_t nt, uint32_t lgp) {
   extern __shared__ float4 s_tab[];
   for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) {
     const float* q = params + static_cast<size_t>(k) * RUMDX_NUM_PARAM;
@@ -326,50 +330,63 @@ __global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t*
     s_tab[2 * k + 1] = make_float4(12.f * a12, 6.f * a6, -ucut, -fmaf(src, rc, ucut));
   }
   __syncthreads();
-  const uint32_t rep = a.block_rep[blockIdx.x];
+  const uint32_t step = 1u << lgp;
+  const uint32_t i = (blockIdx.x * kBlock + threadIdx.x) >> lgp, part = threadIdx.x & (step - 1u);
+  const uint32_t rep = a.block_rep[i / kBlock];
   const rumdx_replica rp = a.rep[rep];
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i - rp.slot0 >= rp.n) return;
-  const Box bx = rep_box(rp);
-  const float4 ri = a.r[i];
-  const float4* prow = s_tab + 2 * __float_as_uint(ri.w) * nt;
-  const double xi = ri.x, yi = ri.y, zi = ri.z;
-  const uint32_t nn = num_nbrs[i];
+  const bool live = i - rp.slot0 < rp.n;
   Acc acc;
-  if (nn >= max_nbrs) {
-    // overflowed list: rescan the build-time stencil (every pair now inside
-    // rc was inside rc + skin then)
-    const int nc = rp.nc;
-    const uint32_t lc = cells.slot_cell[i] - rp.cell0;
-    const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc), cz = static_cast<int>(lc / (nc * nc));
-    for (int dz = -2; dz <= 2; ++dz)
-      for (int dy = -2; dy <= 2; ++dy)
-        for (int dx = -2; dx <= 2; ++dx) {
-          const uint32_t c = rp.cell0 + static_cast<uint32_t>((cx + dx + nc) % nc +
-                                                              nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc)));
-          const uint32_t b0 = cells.cell_start[c], m = cells.cell_n[c];
-          for (uint32_t j = b0; j < b0 + m; ++j)
-            if (j != i) pair_b<CUT>(xi, yi, zi, a.r[j], prow, bx, acc);
-        }
+  if (live) {
+    const Box bx = rep_box(rp);
+    const float4 ri = a.r[i];
+    const float4* prow = s_tab + 2 * __float_as_uint(ri.w) * nt;
+    const double xi = ri.x, yi = ri.y, zi = ri.z;
+    const uint32_t nn = num_nbrs[i];
+    if (nn >= max_nbrs) {
+      if (part == 0) {
+        // overflowed list: rescan the build-time stencil (every pair now
+        // inside rc was inside rc + skin then)
+        const int nc = rp.nc;
+        const uint32_t lc = cells.slot_cell[i] - rp.cell0;
+        const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc),
+                  cz = static_cast<int>(lc / (nc * nc));
+        for (int dz = -2; dz <= 2; ++dz)
+          for (int dy = -2; dy <= 2; ++dy)
+            for (int dx = -2; dx <= 2; ++dx) {
+              const uint32_t c = rp.cell0 + static_cast<uint32_t>((cx + dx + nc) % nc +
+                                                                  nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc)));
+              const uint32_t b0 = cells.cell_start[c], m = cells.cell_n[c];
+              for (uint32_t j = b0; j < b0 + m; ++j)
+                if (j != i) pair_b<CUT>(xi, yi, zi, a.r[j], prow, bx, acc);
+            }
+      }
+    } else {
+      const size_t stride = static_cast<size_t>(a.nslots) << lgp;
+      const uint32_t* col = nbl + static_cast<size_t>(part) * a.nslots + i;
+      uint32_t k = part;
+      for (; k + 3 * step < nn; k += 4 * step) {
+        const uint32_t j0 = col[0], j1 = col[stride], j2 = col[2 * stride], j3 = col[3 * stride];
+        col += 4 * stride;
+        const float4 r0 = a.r[j0], r1 = a.r[j1], r2 = a.r[j2], r3 = a.r[j3];
+        pair_b<CUT>(xi, yi, zi, r0, prow, bx, acc);
+        pair_b<CUT>(xi, yi, zi, r1, prow, bx, acc);
+        pair_b<CUT>(xi, yi, zi, r2, prow, bx, acc);
+        pair_b<CUT>(xi, yi, zi, r3, prow, bx, acc);
+      }
+      for (; k < nn; k += step, col += stride) pair_b<CUT>(xi, yi, zi, a.r[*col], prow, bx, acc);
+    }
+  }
+  for (uint32_t o = 1; o < step; o <<= 1) {
+    acc.fx += __shfl_xor(acc.fx, o, 64);
+    acc.fy += __shfl_xor(acc.fy, o, 64);
+    acc.fz += __shfl_xor(acc.fz, o, 64);
+    acc.u += __shfl_xor(acc.u, o, 64);
+    acc.vir += __shfl_xor(acc.vir, o, 64);
+  }
+  if (live && part == 0) {
     a.f[i] = make_float4(acc.fx, acc.fy, acc.fz, 0.5f * acc.u);
     a.w[i] = make_float4(0.f, 0.f, 0.f, acc.vir);
-    return;
   }
-  const uint32_t* col = nbl + i;
-  uint32_t k = 0;
-  for (; k + 4 <= nn; k += 4) {
-    const uint32_t j0 = col[0], j1 = col[a.nslots], j2 = col[2 * static_cast<size_t>(a.nslots)],
-                   j3 = col[3 * static_cast<size_t>(a.nslots)];
-    col += 4 * static_cast<size_t>(a.nslots);
-    const float4 r0 = a.r[j0], r1 = a.r[j1], r2 = a.r[j2], r3 = a.r[j3];
-    pair_b<CUT>(xi, yi, zi, r0, prow, bx, acc);
-    pair_b<CUT>(xi, yi, zi, r1, prow, bx, acc);
-    pair_b<CUT>(xi, yi, zi, r2, prow, bx, acc);
-    pair_b<CUT>(xi, yi, zi, r3, prow, bx, acc);
-  }
-  for (; k < nn; ++k, col += a.nslots) pair_b<CUT>(xi, yi, zi, a.r[*col], prow, bx, acc);
-  a.f[i] = make_float4(acc.fx, acc.fy, acc.fz, 0.5f * acc.u);
-  a.w[i] = make_float4(0.f, 0.f, 0.f, acc.vir);
 }
 
 // ------------------------------------------------------------------ integrator
@@ -533,23 +550,27 @@ extern "C" int rumdx_batch_forces(rumdx_batch* b, const float* params_dev, int c
   if (b->ntypes == 0 || b->ntypes > RUMDX_MAX_TYPES) return fail(RUMDX_E_INVALID, "rumdx_batch_forces: ntypes out of range");
   if (b->slot_cell_dev == nullptr || b->cell_start_dev == nullptr || b->cell_n_dev == nullptr || b->max_nbrs == 0)
     return fail(RUMDX_E_INVALID, "rumdx_batch_forces: cell arrays not allocated");
+  const uint32_t lpp = b->lanes_per_particle;
+  if (lpp > 8u || (lpp & (lpp - 1u)) != 0u)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_forces: lanes_per_particle must be 0, 1, 2, 4 or 8");
+  const uint32_t lgp = lpp >= 8u ? 3u : lpp >= 4u ? 2u : lpp >= 2u ? 1u : 0u;
   const BatchArgs a = args_of(b);
   const hipStream_t st = as_stream(stream);
-  const uint32_t nblk = b->nslots / kBlock;
+  const uint32_t nblk = (b->nslots / kBlock) << lgp;
   const size_t lds = sizeof(float4) * 2 * b->ntypes * b->ntypes;
   const CellArrays cells{b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev};
   switch (cutoff_method) {
     case RUMDX_CUTOFF_NS:
       hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_NS>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,
-                         b->max_nbrs, cells, params_dev, b->ntypes);
+                         b->max_nbrs, cells, params_dev, b->ntypes, lgp);
       break;
     case RUMDX_CUTOFF_SP:
       hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_SP>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,
-                         b->max_nbrs, cells, params_dev, b->ntypes);
+                         b->max_nbrs, cells, params_dev, b->ntypes, lgp);
       break;
     case RUMDX_CUTOFF_SF:
       hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_SF>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,
-                         b->max_nbrs, cells, params_dev, b->ntypes);
+                         b->max_nbrs, cells, params_dev, b->ntypes, lgp);
       break;
     default: return fail(RUMDX_E_INVALID, "rumdx_batch_forces: invalid cutoff method");
   }

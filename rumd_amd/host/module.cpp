@@ -292,6 +292,16 @@ PYBIND11_MODULE(_rumd, m) {
            py::arg("thermostatRelaxationTime"))
       .def("SetMaxNumNbrs", &ReplicaBatch::SetMaxNumNbrs, py::arg("max_nbrs"))
       .def("GetMaxNumNbrs", &ReplicaBatch::GetMaxNumNbrs)
+      .def("SetLanesPerParticle", &ReplicaBatch::SetLanesPerParticle, py::arg("tp"))
+      .def("GetLanesPerParticle", &ReplicaBatch::GetLanesPerParticle)
+      .def("GetBoxLength", &ReplicaBatch::GetBoxLength, py::arg("replica"))
+      .def("GetThermostat", &ReplicaBatch::GetThermostat, py::arg("replica"))
+      .def("GetTimeStep", &ReplicaBatch::GetTimeStep)
+      .def("GetSkin", &ReplicaBatch::GetSkin)
+      .def("GetCutoffMethod", &ReplicaBatch::GetCutoffMethod)
+      .def("GetParamsTable", &ReplicaBatch::GetParamsTable)
+      .def("SaveState", &ReplicaBatch::SaveState)
+      .def("RestoreState", &ReplicaBatch::RestoreState)
       .def("CalcF", &ReplicaBatch::CalcF)
       .def("RunSteps", &ReplicaBatch::RunSteps, py::arg("nsteps"), py::arg("block") = 10,
            py::call_guard<py::gil_scoped_release>())

@@ -33,6 +33,7 @@ void ReplicaBatch::Relayout() {
     }
   }
   allocated_ = false;
+  saved_ = false;
   InvalidateGraphs();
 }
 
@@ -103,9 +104,68 @@ void ReplicaBatch::SetMaxNumNbrs(unsigned m) {
   max_nbrs_ = 0;
 }
 
+void ReplicaBatch::SetLanesPerParticle(unsigned tp) {
+  if (tp != 1 && tp != 2 && tp != 4 && tp != 8)
+    throw RUMD_Error("ReplicaBatch", "SetLanesPerParticle", "lanes per particle must be 1, 2, 4 or 8");
+  lanes_per_particle_ = tp;
+  desc_.lanes_per_particle = tp;
+  InvalidateGraphs();
+}
+
+namespace {
+template <class T>
+void copy_device(DeviceArray<T>& dst, const DeviceArray<T>& src, hipStream_t st) {
+  if (dst.size() != src.size()) dst.resize(src.size());
+  if (src.size())
+    hip_check(hipMemcpyAsync(dst.get(), src.get(), src.size() * sizeof(T), hipMemcpyDeviceToDevice, st),
+              "ReplicaBatch", "SaveState");
+}
+}  // namespace
+
+void ReplicaBatch::SaveState() {
+  if (!allocated_) Allocate();
+  Synchronize();
+  copy_device(save_r_, r_, stream_);
+  copy_device(save_v_, v_, stream_);
+  copy_device(save_f_, f_, stream_);
+  copy_device(save_w_, w_, stream_);
+  copy_device(save_im_, im_, stream_);
+  copy_device(save_idx_, idx_, stream_);
+  copy_device(save_nh_, nh_, stream_);
+  copy_device(save_flags_, flags_, stream_);
+  hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "SaveState");
+  saved_ = true;
+}
+
+void ReplicaBatch::RestoreState() {
+  if (!saved_ || !allocated_ || save_r_.size() != r_.size())
+    throw RUMD_Error("ReplicaBatch", "RestoreState", "no saved state for the current layout");
+  Synchronize();
+  copy_device(r_, save_r_, stream_);
+  copy_device(v_, save_v_, stream_);
+  copy_device(f_, save_f_, stream_);
+  copy_device(w_, save_w_, stream_);
+  copy_device(im_, save_im_, stream_);
+  copy_device(idx_, save_idx_, stream_);
+  copy_device(nh_, save_nh_, stream_);
+  copy_device(flags_, save_flags_, stream_);
+  rumdx_check(rumdx_batch_request_rebuild(&desc_, stream_), "ReplicaBatch", "RestoreState");
+  hip_check(hipStreamSynchronize(stream_), "ReplicaBatch", "RestoreState");
+}
+
 unsigned ReplicaBatch::GetNumberOfParticles(unsigned replica) const {
   if (replica >= reps_.size()) throw RUMD_Error("ReplicaBatch", "GetNumberOfParticles", "replica index out of range");
   return static_cast<unsigned>(reps_[replica].types.size());
+}
+
+float ReplicaBatch::GetBoxLength(unsigned replica) const {
+  if (replica >= reps_.size()) throw RUMD_Error("ReplicaBatch", "GetBoxLength", "replica index out of range");
+  return reps_[replica].L;
+}
+
+std::vector<float> ReplicaBatch::GetThermostat(unsigned replica) const {
+  if (replica >= reps_.size()) throw RUMD_Error("ReplicaBatch", "GetThermostat", "replica index out of range");
+  return {reps_[replica].T, reps_[replica].tau};
 }
 
 unsigned ReplicaBatch::GetTotalParticles() const {
@@ -264,6 +324,7 @@ void ReplicaBatch::Allocate() {
   desc_.cell_start_dev = cell_start_.get();
   desc_.cell_n_dev = cell_n_.get();
   desc_.idx_dev = idx_.get();
+  desc_.lanes_per_particle = lanes_per_particle_;
   desc_.t_r_dev = t_r_.get();
   desc_.t_v_dev = t_v_.get();
   desc_.t_im_dev = t_im_.get();

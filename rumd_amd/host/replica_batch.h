@@ -29,6 +29,14 @@ class ReplicaBatch {
   // Nose-Hoover thermostat of one replica (relaxation time tau <= 0: NVE).
   void SetThermostat(unsigned replica, float target_temperature, float tau);
   void SetMaxNumNbrs(unsigned m);
+  // Lanes that share one particle's list in the force kernel (1, 2, 4, 8).
+  void SetLanesPerParticle(unsigned tp);
+  unsigned GetLanesPerParticle() const { return lanes_per_particle_; }
+  // Device copies of the dynamic state (positions, velocities, images, slot
+  // ids, forces, thermostats, flags); RestoreState puts them back and marks
+  // every list stale, so trial runs leave no trace in the trajectory.
+  void SaveState();
+  void RestoreState();
 
   // Forces and energies of the current positions (lists updated first).
   void CalcF();
@@ -49,6 +57,13 @@ class ReplicaBatch {
   std::vector<double> GetThermostatStates();  // Ps per replica
   std::vector<unsigned> GetNumberOfBuilds();
   unsigned GetMaxNumNbrs() const { return max_nbrs_; }
+  float GetBoxLength(unsigned replica) const;
+  // {target temperature, relaxation time} of a replica's thermostat
+  std::vector<float> GetThermostat(unsigned replica) const;
+  float GetTimeStep() const { return dt_; }
+  float GetSkin() const { return skin_; }
+  int GetCutoffMethod() const { return cutoff_method_; }
+  const std::vector<float>& GetParamsTable() const { return table_; }
   // Neighbour list of one replica in replica-local ids: CSR (offsets n + 1, ids).
   void GetNeighborList(unsigned replica, std::vector<uint32_t>& offsets, std::vector<uint32_t>& ids);
   hipStream_t GetStream() const { return stream_; }
@@ -75,7 +90,8 @@ class ReplicaBatch {
   std::vector<float> table_;
   int cutoff_method_ = 2;
   float skin_ = 0.3f, max_cutoff_ = 0.f, dt_ = 0.0025f;
-  unsigned user_max_nbrs_ = 0, max_nbrs_ = 0;
+  unsigned user_max_nbrs_ = 0, max_nbrs_ = 0, lanes_per_particle_ = 1;
+  bool saved_ = false;
   bool allocated_ = false;
   hipStream_t stream_ = nullptr;
 
@@ -86,7 +102,9 @@ class ReplicaBatch {
   DeviceArray<uint32_t> block_rep_, flags_, nbl_, num_nbrs_, cell_count_, cell_start_, cell_n_, cell_tmp_,
       slot_cell_, idx_, t_idx_;
   DeviceArray<rumdx_replica> rep_;
-  DeviceArray<rumdx_nh_state> nh_;
+  DeviceArray<rumdx_nh_state> nh_, save_nh_;
+  DeviceArray<float> save_r_, save_v_, save_f_, save_w_, save_im_;
+  DeviceArray<uint32_t> save_idx_, save_flags_;
   DeviceArray<double> workspace_, energies_;
   rumdx_batch desc_{};
   std::map<unsigned, hipGraphExec_t> graphs_;

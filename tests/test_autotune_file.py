@@ -44,3 +44,18 @@ def test_fixed_parameters_validated():
         Autotune(NB_method="cells")
     with pytest.raises(TypeError):
         Autotune(bogus=1)
+
+
+def test_batch_entries_match_only_batches(tmp_path):
+    """Batch user parameters (lists per replica) compare element by element
+    and never match a single-sample entry."""
+    from rumd_amd.Autotune import _same_user
+    batch = {"deviceName": "d", "kind": "ReplicaBatch", "N": [300, 1200], "L": [6.3, 10.0],
+             "thermostats": [[0.5, 0.2], [1.5, 0.2]], "timeStep": 0.0025, "skin": 0.3, "cutoffMethod": 2,
+             "params": [2.5, 4.0, -4.0]}
+    assert _same_user(batch, dict(batch))
+    other = dict(batch, N=[300, 1201])
+    assert not _same_user(batch, other)
+    assert not _same_user(batch, dict(batch, thermostats=[[0.5, 0.2], [1.6, 0.2]]))
+    single = {"deviceName": "d", "N": 1500}
+    assert not _same_user(batch, single)

@@ -163,3 +163,65 @@ def test_batch_hot_replicas_rebuild_more_often():
     t_kin = [2.0 * e[r, 2] / (3 * s.n - 3) for r, s in enumerate(systems)]
     assert t_kin[1] > t_kin[0]
     assert builds[1] > builds[0], builds
+
+
+@pytest.mark.parametrize("tp", [2, 4, 8])
+def test_batch_lanes_per_particle(tp):
+    """Lists spread over 2, 4 or 8 lanes (a fixed butterfly combines them)
+    give the oracle's forces; an overflowed list is rescanned by one lane."""
+    systems = _systems([600, 3000], seed0=830)
+    b = make_batch(systems, [0.5, 0.5])
+    b.SetLanesPerParticle(tp)
+    print(f"[case] batch lanes per particle {tp}", flush=True)
+    b.CalcF()
+    for r, s in enumerate(systems):
+        pos4 = configs.pack_pos4(s.positions, s.types)
+        f_ref, _ = oracle.forces_brute(pos4, s.box, 2, ka_table(), SF)
+        err = force_errors(b.GetForcesAndEnergies(r)[:, :3], f_ref[:, :3])
+        assert err["guarded_max"] <= FORCE_TOL_GUARDED, (r, err)
+    o = make_batch(systems[:1], [0.5], max_nbrs=24)
+    o.SetLanesPerParticle(tp)
+    o.CalcF()
+    f_ref, _ = oracle.forces_brute(configs.pack_pos4(systems[0].positions, systems[0].types), systems[0].box, 2,
+                                   ka_table(), SF)
+    assert force_errors(o.GetForcesAndEnergies(0)[:, :3], f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
+
+
+def test_batch_save_restore_leaves_no_trace():
+    """Trial steps between SaveState and RestoreState do not change the
+    trajectory: the run continues bit for bit as if they never happened."""
+    systems = _systems([500, 2500], seed0=870)
+    a = make_batch(systems, [0.6, 1.2])
+    a.RunSteps(40, 10)
+    b = make_batch(systems, [0.6, 1.2])
+    b.SaveState()
+    b.SetLanesPerParticle(4)
+    b.RunSteps(30, 10)
+    b.RestoreState()
+    b.SetLanesPerParticle(1)
+    print("[case] batch save/restore", flush=True)
+    b.RunSteps(40, 10)
+    for r in range(2):
+        assert np.array_equal(a.GetPositions(r), b.GetPositions(r)), r
+        assert np.array_equal(a.GetVelocities(r), b.GetVelocities(r)), r
+    assert a.GetThermostatStates() == b.GetThermostatStates()
+
+
+def test_batch_autotune(tmp_path):
+    from rumd_amd.Autotune import Autotune
+    systems = _systems([400, 1600, 3000], seed0=910)
+    b = make_batch(systems, [0.5, 0.9, 1.4])
+    at = Autotune(auto_filename=tmp_path / "autotune.json", trial_seconds=0.02)
+    print("[case] batch autotune", flush=True)
+    tech = at.TuneBatch(b)
+    assert tech["tp"] in (1, 2, 4, 8) and len(at.trials) == 4
+    assert b.GetLanesPerParticle() == tech["tp"]
+    # state untouched: the same run with the chosen lanes set directly
+    ref = make_batch(systems, [0.5, 0.9, 1.4])
+    ref.SetLanesPerParticle(tech["tp"])
+    ref.RunSteps(20, 10)
+    b.RunSteps(20, 10)
+    for r in range(3):
+        assert np.array_equal(ref.GetPositions(r), b.GetPositions(r)), r
+    again = Autotune(auto_filename=tmp_path / "autotune.json")
+    assert again.TuneBatch(b) == tech and again.trials == []
