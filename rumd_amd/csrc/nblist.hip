@@ -648,26 +648,22 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
   if (!rebuild_pending(flags)) return;
   __shared__ HaloLds L;
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
-  __shared__ uint32_t s_key[kOrderMax];  // list lengths, then sort keys (same index, same thread)
+  __shared__ __align__(16) uint32_t s_key[kOrderMax];  // list lengths, then sort keys (same index, same thread)
   __shared__ WaveBuild s_wave[kBuildBlock / 64];
   extern __shared__ float4 s_pos[];
   for (uint32_t k = threadIdx.x; k < nt * nt; k += kBuildBlock) s_cut[k] = cut_skin2[k];
   // the sentinel after the halo capacity: a position no distance test accepts
   if (threadIdx.x == 0) s_pos[tg.halo_cap] = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
   const uint32_t t = xcd_block(blockIdx.x, gridDim.x);
-  stage_halo(t, tg, s.t_r, s_pos, L);
+  // the halo in halo order also goes to the force kernel's slot list; the
+  // waves' build buffers are free during staging and hold its cell table
+  static_assert(sizeof(s_wave) >= sizeof(uint16_t) * RUMDX_HALO_CAP_MAX, "cell table scratch");
+  stage_halo(t, tg, s.t_r, s_pos, L, reinterpret_cast<uint16_t*>(s_wave),
+             tg.halo_slots + static_cast<size_t>(t) * tg.halo_cap);
   const Halo& h = L.h;
   const bool staged = h.npart <= tg.halo_cap;
   const bool bytes = h.npart <= kByteEntryHalo;
   const uint32_t np = h.slot1 - h.slot0;
-  // the halo in halo order, for the force kernel's staging
-  if (staged) {
-    uint32_t* hs = tg.halo_slots + static_cast<size_t>(t) * tg.halo_cap;
-    for (uint32_t c = threadIdx.x; c < h.ncell; c += kBuildBlock) {
-      const uint32_t o0 = L.off[c], cnt = L.off[c + 1] - o0, g0 = L.start[c];
-      for (uint32_t k = 0; k < cnt; ++k) hs[o0 + k] = g0 + k;
-    }
-  }
   uint32_t longest = 0;  // this thread's longest list
   // wave-cooperative lists for staged tiles; one lane per particle otherwise
   const bool coop = staged && np <= kOrderMax;
@@ -721,15 +717,30 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
   }
   // Lane order of the force kernel: the tile's particles by descending list
   // length, ties by slot (a rank count over unique keys; deterministic).
-  // block_max_u32 synchronises the workgroup, so every key is written.
+  // Keys past np are padded with a value above every key, so the count reads
+  // four keys per LDS load.  block_max_u32 synchronises the workgroup, so
+  // every key is written.
+  const uint32_t np4 = (np + 3u) & ~3u;
+  if (np <= kOrderMax && threadIdx.x < np4 - np) s_key[np + threadIdx.x] = 0xffffffffu;
   __shared__ uint32_t s_max[kBuildBlock / 64];
   longest = block_max_u32(longest, s_max);
   for (uint32_t k = threadIdx.x; k < np; k += kBuildBlock) {
     uint32_t rk = k;
     if (np <= kOrderMax) {
       const uint32_t key = s_key[k];
-      rk = 0;
-      for (uint32_t j = 0; j < np; ++j) rk += s_key[j] < key ? 1u : 0u;
+      const uint4* k4 = reinterpret_cast<const uint4*>(s_key);
+      uint32_t r0 = 0, r1 = 0;
+      uint32_t j = 0;
+      for (; j + 8 <= np4; j += 8) {
+        const uint4 a = k4[j >> 2], b = k4[(j >> 2) + 1];
+        r0 += (a.x < key ? 1u : 0u) + (a.y < key ? 1u : 0u) + (a.z < key ? 1u : 0u) + (a.w < key ? 1u : 0u);
+        r1 += (b.x < key ? 1u : 0u) + (b.y < key ? 1u : 0u) + (b.z < key ? 1u : 0u) + (b.w < key ? 1u : 0u);
+      }
+      if (j < np4) {
+        const uint4 a = k4[j >> 2];
+        r0 += (a.x < key ? 1u : 0u) + (a.y < key ? 1u : 0u) + (a.z < key ? 1u : 0u) + (a.w < key ? 1u : 0u);
+      }
+      rk = r0 + r1;
     }
     o.lane_order[h.slot0 + rk] = static_cast<uint16_t>(k);
   }

@@ -130,9 +130,13 @@ __device__ __forceinline__ int halo_axis(int c, int base, int nc) {
 
 // Fills the halo table of tile t and, if it fits, copies the halo particles'
 // positions from pos into s_pos.  Must be called by all threads of the block
-// (at least 256, a multiple of 64, at most kMaxWorkgroup).
+// (at least 256, a multiple of 64, at most kMaxWorkgroup).  With cellof (LDS
+// scratch of halo_cap u16) the copy is flat: each halo particle learns its
+// cell from cellof and every thread issues up to four independent loads, and
+// the halo's slots also go to slots_out (halo order) when that is given.
 __device__ __forceinline__ void stage_halo(uint32_t t, const TileGeom& g, const float4* __restrict__ pos,
-                                           float4* s_pos, HaloLds& L) {
+                                           float4* s_pos, HaloLds& L, uint16_t* cellof = nullptr,
+                                           uint32_t* __restrict__ slots_out = nullptr) {
   if (threadIdx.x == 0) {
     const uint32_t o = g.origin[t];
     const int c0[3] = {static_cast<int>(o & 1023u), static_cast<int>((o >> 10) & 1023u),
@@ -201,7 +205,34 @@ __device__ __forceinline__ void stage_halo(uint32_t t, const TileGeom& g, const 
     L.h.npart = tot;
   }
   __syncthreads();
-  if (tot <= g.halo_cap) {
+  if (tot <= g.halo_cap && cellof != nullptr) {
+    for (uint32_t c = threadIdx.x; c < h.ncell; c += blockDim.x) {
+      const uint32_t o1 = L.off[c + 1];
+      for (uint32_t q = L.off[c]; q < o1; ++q) cellof[q] = static_cast<uint16_t>(c);
+    }
+    __syncthreads();
+    for (uint32_t q0 = 0; q0 < tot; q0 += 4 * blockDim.x) {
+      uint32_t src[4];
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t q = q0 + u * blockDim.x + threadIdx.x;
+        const uint32_t c = q < tot ? cellof[q] : 0u;
+        src[u] = L.start[c] + (q - L.off[c]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (q0 + u * blockDim.x + threadIdx.x < tot) v[u] = pos[src[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t q = q0 + u * blockDim.x + threadIdx.x;
+        if (q < tot) {
+          s_pos[q] = v[u];
+          if (slots_out != nullptr) slots_out[q] = src[u];
+        }
+      }
+    }
+  } else if (tot <= g.halo_cap) {
     // one thread per halo cell copies that cell's (contiguous) particles
     for (uint32_t c = threadIdx.x; c < h.ncell; c += blockDim.x) {
       const uint32_t s = L.start[c], o0 = L.off[c], n = L.off[c + 1] - o0;
