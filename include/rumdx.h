@@ -304,6 +304,8 @@ typedef struct rumdx_batch {
   uint32_t* slot_cell_dev;       /* nslots: cell of each slot at its replica's last build */
   uint32_t* idx_dev;             /* nslots: replica-local id of the particle in each slot */
   uint32_t lanes_per_particle;   /* 0/1, 2, 4 or 8 lanes share a list in rumdx_batch_forces */
+  int32_t* fixed_dev;            /* nslots x 4: (x/L) 2^32 per axis as int32 and the type bits; written by
+                                    rumdx_batch_update_lists and rumdx_batch_nvt_step, read by the forces */
   float* t_r_dev;                /* nslots x 4: sort staging of r, v, im and idx */
   float* t_v_dev;
   float* t_im_dev;

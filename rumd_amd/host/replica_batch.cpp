@@ -288,6 +288,7 @@ void ReplicaBatch::Allocate() {
   t_v_.resize(4 * ns);
   t_im_.resize(4 * ns);
   t_idx_.resize(ns);
+  fixed_.resize(4 * ns);  // written by every build (all replicas rebuild first)
   params_.resize(table_.size());
   params_.upload(table_.data(), table_.size(), stream_);
   std::vector<float> c2(static_cast<size_t>(ntypes_) * ntypes_);
@@ -325,6 +326,7 @@ void ReplicaBatch::Allocate() {
   desc_.cell_n_dev = cell_n_.get();
   desc_.idx_dev = idx_.get();
   desc_.lanes_per_particle = lanes_per_particle_;
+  desc_.fixed_dev = reinterpret_cast<int32_t*>(fixed_.get());
   desc_.t_r_dev = t_r_.get();
   desc_.t_v_dev = t_v_.get();
   desc_.t_im_dev = t_im_.get();
