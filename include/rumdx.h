@@ -255,7 +255,8 @@ int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_mask, doubl
  * leave the GPU idle.  Each replica follows the single-sample algorithms
  * above: a cubic box, RUMD's cell grid and 5^3 stencil with the FP32
  * criterion, u32 lists (entry k of slot i at [k*nslots + i]), the LJ pair
- * term in FP32 with FP64 displacements, and its own Nose-Hoover leapfrog.
+ * term in FP32 with exact minimum-image displacements (taken on fixed-point
+ * copies of the positions, fixed_dev), and its own Nose-Hoover leapfrog.
  * Replica r owns slots [slot0, slot0 + n); slot0 is a multiple of
  * RUMDX_BATCH_ALIGN and the range is padded to ceil(n / RUMDX_BATCH_ALIGN)
  * blocks, so no workgroup spans two replicas and each replica's sums have a
