@@ -294,3 +294,44 @@ def test_momentum_and_kinetic_energy(golden_dir):
     m = 1.0 / v[:, 3].astype(np.float64)
     p = (v[:, :3].astype(np.float64) * m[:, None]).sum(axis=0)
     assert np.all(np.abs(p) < 1e-4)
+
+
+def _five_type_system(n=12000, seed=515):
+    """A 5-type LJ mixture with a distinct sigma, epsilon and cut-off per type
+    pair (parameter rows and cut-off tables beyond KA's 2 x 2)."""
+    sysm = configs.ka_random_lattice(n, density=1.0, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    types = rng.integers(0, 5, size=n).astype(np.int32)
+    rows = []
+    for i in range(5):
+        for j in range(i, 5):
+            rows.append((i, j, float(0.85 + 0.05 * ((i + 2 * j) % 5)), float(0.6 + 0.2 * ((2 * i + j) % 5)),
+                         float(2.2 + 0.1 * ((i * j) % 4))))
+    table = configs.lj_param_table(rows, 5)
+    pos4 = configs.pack_pos4(sysm.positions, types)
+    vel4 = configs.pack_vel4(sysm.velocities, types, np.ones(5))
+    return sysm, table, pos4, vel4
+
+
+@pytest.mark.parametrize("cutoff", [SP, SF])
+def test_five_types_sorted_forces_and_lists(cutoff):
+    """More types than KA: per-pair parameter rows and (rc_ij + skin)^2 rows
+    in the tile build and the tile force kernel, against the oracle."""
+    sysm, table, pos4, vel4 = _five_type_system()
+    b8 = configs.box8(sysm.box)
+    g = GpuSystem(pos4, vel4, b8, 5, table, method="sorted", max_nbrs=160)
+    print(f"[case] five types sorted cutoff={cutoff}", flush=True)
+    g.calc_f(cutoff)
+    uidx = g.unsorted_index()
+    f = unsort(g.get("f"), uidx)
+    f_ref, vir_ref = oracle.forces_cells(pos4, sysm.box, 5, table, cutoff)
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+    pe_gpu, pe_ref = float(np.sum(f[:, 3], dtype=np.float64)), float(f_ref[:, 3].sum())
+    assert abs(pe_gpu - pe_ref) <= 1e-5 * abs(pe_ref) + 1e-3
+    cnt, nbl = g.neighbor_lists()
+    sets_gpu = gpu_sets_original_ids(cnt, nbl, uidx)
+    rc_max = float(table[:, 0].max())
+    nc = oracle.cell_grid(b8, rc_max, 0.3)
+    off, idx = oracle.nblist_cells_f32(pos4, b8, 5, configs.cut_skin2_table(table, 0.3), nc)
+    assert sets_gpu == csr_to_sets(off, idx)
