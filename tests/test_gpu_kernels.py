@@ -335,3 +335,33 @@ def test_five_types_sorted_forces_and_lists(cutoff):
     nc = oracle.cell_grid(b8, rc_max, 0.3)
     off, idx = oracle.nblist_cells_f32(pos4, b8, 5, configs.cut_skin2_table(table, 0.3), nc)
     assert sets_gpu == csr_to_sets(off, idx)
+
+
+def test_rectangular_box_sorted_forces_and_lists():
+    """A rectangular box (cell grids of 29 x 20 x 16): per-axis cell counts,
+    per-axis wrap flags and shifted frames, against the oracle."""
+    rng = np.random.default_rng(4242)
+    a = (1.0 / 1.2) ** (1.0 / 3.0)
+    nx, ny, nz = 44, 30, 24
+    g3 = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    L = np.array([nx * a, ny * a, nz * a])
+    pos = (g3 + 0.5 + rng.uniform(-0.1, 0.1, g3.shape)) * a - L / 2
+    n = pos.shape[0]
+    types = np.zeros(n, dtype=np.int32)
+    types[rng.permutation(n)[: round(0.2 * n)]] = 1
+    pos4 = configs.pack_pos4(pos, types)
+    vel4 = configs.pack_vel4(rng.normal(0, 0.5, (n, 3)), types, np.ones(2))
+    b8 = configs.box8(L)
+    table = ka_table()
+    g = GpuSystem(pos4, vel4, b8, 2, table, method="sorted", max_nbrs=160)
+    print(f"[case] rectangular box N={n} nc={list(g.nc)}", flush=True)
+    assert len(set(int(c) for c in g.nc)) == 3
+    g.calc_f(SF)
+    uidx = g.unsorted_index()
+    f = unsort(g.get("f"), uidx)
+    f_ref, _ = oracle.forces_cells(pos4, L.astype(np.float32).astype(np.float64), 2, table, SF)
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+    cnt, nbl = g.neighbor_lists()
+    off, idx = oracle.nblist_cells_f32(pos4, b8, 2, configs.cut_skin2_table(table, 0.3), oracle.cell_grid(b8, 2.5, 0.3))
+    assert gpu_sets_original_ids(cnt, nbl, uidx) == csr_to_sets(off, idx)
