@@ -266,11 +266,12 @@ int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_mask, doubl
  * names the replica-local particle a slot holds. */
 #define RUMDX_BATCH_ALIGN 256
 #define RUMDX_BATCH_FLAGS 8  /* u32 per replica: rebuild, overflow, builds, longest list, internal */
+#define RUMDX_BATCH_CELL_BLOCK 16  /* a replica's cell range starts at a multiple of this and is padded to it */
 
 typedef struct rumdx_replica {
   uint32_t slot0;  /* first slot, a multiple of RUMDX_BATCH_ALIGN */
   uint32_t n;      /* particles */
-  uint32_t cell0;  /* first cell of its grid in the batch's cell arrays */
+  uint32_t cell0;  /* first cell of its grid in the batch's cell arrays, a multiple of RUMDX_BATCH_CELL_BLOCK */
   int32_t nc;      /* cells per axis, rumdx_cell_grid of its box */
   float L;         /* cubic box length */
   float inv_L;     /* 1/L rounded to float, as RUMD's box[4] */
@@ -297,7 +298,8 @@ typedef struct rumdx_batch {
   uint32_t max_nbrs;
   uint32_t ntypes;
   float skin;
-  uint32_t ncells;               /* sum over replicas of nc^3 */
+  uint32_t ncells;               /* sum over replicas of nc^3, each padded to RUMDX_BATCH_CELL_BLOCK */
+  const uint32_t* cellblock_rep_dev; /* ncells / RUMDX_BATCH_CELL_BLOCK: replica of each cell block */
   uint32_t* cell_count_dev;      /* ncells, zero between builds */
   uint32_t* cell_start_dev;      /* ncells: first slot of each cell (cells are contiguous slot ranges) */
   uint32_t* cell_n_dev;          /* ncells */

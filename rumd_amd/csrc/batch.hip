@@ -66,6 +66,12 @@ __device__ __forceinline__ double fixed_scale(const rumdx_replica& rp) {
   return 4294967296.0 / static_cast<double>(rp.L);
 }
 
+// Orders the LDS accesses of one wave (lanes read what other lanes wrote).
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ bool rebuild_pending(const uint32_t* flags, uint32_t rep) {
   return __hip_atomic_load(flags + kRepFlags * rep + kRebuild, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u;
 }
@@ -188,89 +194,187 @@ __global__ __launch_bounds__(kBlock) void k_b_gather(BatchArgs a, const uint32_t
   cell_count[c] = 0u;
 }
 
-// The lists of a rebuilding replica from its sorted staging: RUMD's 5^3
-// stencil in (dZ, dY, dX) order, ascending slot inside each cell, FP32
-// minimum-image criterion; then each particle moves to its sorted slot.  The
+// Stencil candidates of one cell in RUMD's (dZ, dY, dX) order, ascending slot
+// inside each cell: 25 rows of five cells (each a contiguous range of sorted
+// slots) when the cell's row does not wrap around the box, else 125 cells.
+// seg gives segment sg's first slot and length; returns the count.
+__device__ __forceinline__ uint32_t stencil_segment(const rumdx_replica& rp, const uint32_t* __restrict__ cell_start,
+                                                    const uint32_t* __restrict__ cell_n, int cx, int cy, int cz,
+                                                    bool rows, uint32_t sg, uint32_t& first) {
+  const int nc = rp.nc;
+  if (rows) {
+    const int oz = (cz + static_cast<int>(sg / 5) - 2 + nc) % nc, oy = (cy + static_cast<int>(sg % 5) - 2 + nc) % nc;
+    const uint32_t c_row = rp.cell0 + static_cast<uint32_t>(nc * (oy + nc * oz));
+    const uint32_t lo = c_row + static_cast<uint32_t>(cx - 2), hi = c_row + static_cast<uint32_t>(cx + 2);
+    first = cell_start[lo];
+    return cell_start[hi] + cell_n[hi] - first;
+  }
+  const int oz = (cz + static_cast<int>(sg / 25) - 2 + nc) % nc;
+  const int oy = (cy + static_cast<int>((sg / 5) % 5) - 2 + nc) % nc;
+  const int ox = (cx + static_cast<int>(sg % 5) - 2 + nc) % nc;
+  const uint32_t c = rp.cell0 + static_cast<uint32_t>(ox + nc * (oy + nc * oz));
+  first = cell_start[c];
+  return cell_n[c];
+}
+
+// Per-wave LDS of the batch build: the cell's stencil as a window of
+// replica-local slots, the segment table, and one particle's list.
+constexpr uint32_t kBatchWindow = 1024;
+constexpr uint32_t kBatchStage = 256;
+constexpr uint32_t kBatchCellsPerWave = RUMDX_BATCH_CELL_BLOCK / (kBlock / 64);
+struct BatchWave {
+  uint16_t cand[kBatchWindow];  // slot - slot0
+  uint32_t seg_first[128];
+  uint16_t seg_pre[128];
+  uint16_t seg_len[128];
+  uint32_t list[kBatchStage];
+};
+static_assert(RUMDX_BATCH_CELL_BLOCK % (kBlock / 64) == 0, "whole cells per wave");
+
+// The lists of a rebuilding replica from its sorted staging, one wave per
+// cell (RUMDX_BATCH_CELL_BLOCK cells per workgroup, all of one replica): the
+// cell's stencil goes into an LDS window once, and each of its particles
+// tests it 64 candidates at a time (FP32 minimum-image criterion), a ballot
+// placing the accepted ones in candidate order.  A cell whose stencil does
+// not fit the window, or lists longer than the staging, is built one lane
+// per particle.  Then the cell's particles move to their sorted slots.  The
 // last workgroup of the replica clears its rebuild request.
-__global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, uint32_t* __restrict__ slot_cell,
+__global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t* __restrict__ cellblock_rep,
+                                                    uint32_t* __restrict__ slot_cell,
                                                     const uint32_t* __restrict__ cell_start,
                                                     const uint32_t* __restrict__ cell_n, SortStage st,
                                                     uint32_t* __restrict__ idx, const float* __restrict__ cut_skin2,
                                                     uint32_t nt, uint32_t* __restrict__ nbl,
                                                     uint32_t* __restrict__ num_nbrs, float4* __restrict__ last_r,
                                                     uint32_t max_nbrs) {
-  const uint32_t rep = a.block_rep[blockIdx.x];
+  const uint32_t rep = cellblock_rep[blockIdx.x];
   if (!rebuild_pending(a.flags, rep)) return;
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
+  __shared__ BatchWave s_wave[kBlock / 64];
+  __shared__ uint32_t s_max[kBlock / 64];
   for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_skin2[k];
   __syncthreads();
-  __shared__ uint32_t s_max[kBlock / 64];
   const rumdx_replica rp = a.rep[rep];
   uint32_t* fl = a.flags + kRepFlags * rep;
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const Box bx = rep_box(rp);
+  const int nc = rp.nc;
+  const uint32_t ncell = static_cast<uint32_t>(nc) * nc * nc;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  BatchWave& wb = s_wave[wave];
   uint32_t longest = 0;
-  if (i - rp.slot0 < rp.n) {
-    const Box bx = rep_box(rp);
-    const float4 ri = st.r[i];
-    const float* crow = s_cut + __float_as_uint(ri.w) * nt;
-    const int nc = rp.nc;
-    const uint32_t lc = rep_cell(ri, bx, nc);  // the cell the histogram gave this particle
-    const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc), cz = static_cast<int>(lc / (nc * nc));
-    uint32_t cnt = 0;
-    auto consider = [&](uint32_t j, float4 rj) {
-      if (j != i && min_image_d2_f32(ri, rj, bx) < crow[__float_as_uint(rj.w)]) {
-        if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * a.nslots + i] = j;
-        ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
-      }
-    };
-    // a contiguous slot range, eight candidates' loads in flight at once
-    // (measured: 4 -> 8 gave the batch 4.5 %; 16 was slower)
-    auto scan = [&](uint32_t k0, uint32_t k1) {
-      uint32_t k = k0;
-      for (; k + 8 <= k1; k += 8) {
-        float4 rr[8];
+  for (uint32_t cc = 0; cc < kBatchCellsPerWave; ++cc) {
+    const uint32_t local = blockIdx.x * RUMDX_BATCH_CELL_BLOCK - rp.cell0 + wave * kBatchCellsPerWave + cc;
+    if (local >= ncell) break;
+    const uint32_t c = rp.cell0 + local;
+    const uint32_t m = __builtin_amdgcn_readfirstlane(cell_n[c]);
+    const uint32_t cs = __builtin_amdgcn_readfirstlane(cell_start[c]);
+    if (m == 0) continue;
+    const int cx = static_cast<int>(local % nc), cy = static_cast<int>((local / nc) % nc),
+              cz = static_cast<int>(local / (nc * nc));
+    const bool rows = cx >= 2 && cx + 2 < nc;
+    const uint32_t nseg = rows ? 25u : 125u;
+    // segment table and its exclusive prefix
+    uint32_t carry = 0;
+    for (uint32_t s0 = 0; s0 < nseg; s0 += 64) {
+      const uint32_t sg = s0 + lane;
+      uint32_t first = 0, len = 0;
+      if (sg < nseg) len = stencil_segment(rp, cell_start, cell_n, cx, cy, cz, rows, sg, first);
+      uint32_t x = len;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) rr[u] = st.r[k + u];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) consider(k + u, rr[u]);
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= static_cast<uint32_t>(o)) x += y;
       }
-      for (; k < k1; ++k) consider(k, st.r[k]);
-    };
-    // Cells are numbered x fastest and sorted slots follow that order, so the
-    // five cells (cx-2..cx+2, y, z) are one contiguous range unless the row
-    // wraps around the box; then the cells go one by one in dX order.
-    const bool row = cx >= 2 && cx + 2 < nc;
-    for (int dz = -2; dz <= 2; ++dz) {
-      const int oz = (cz + dz + nc) % nc;
-      for (int dy = -2; dy <= 2; ++dy) {
-        const int oy = (cy + dy + nc) % nc;
-        const uint32_t c_row = rp.cell0 + static_cast<uint32_t>(nc * (oy + nc * oz));
-        if (row) {
-          const uint32_t lo = c_row + static_cast<uint32_t>(cx - 2), hi = c_row + static_cast<uint32_t>(cx + 2);
-          scan(cell_start[lo], cell_start[hi] + cell_n[hi]);
-        } else {
-          for (int dx = -2; dx <= 2; ++dx) {
-            const uint32_t c = c_row + static_cast<uint32_t>((cx + dx + nc) % nc);
-            scan(cell_start[c], cell_start[c] + cell_n[c]);
+      if (sg < nseg) {
+        wb.seg_first[sg] = first;
+        wb.seg_pre[sg] = static_cast<uint16_t>(carry + x - len < 0xffffu ? carry + x - len : 0xffffu);
+        wb.seg_len[sg] = static_cast<uint16_t>(len < 0xffffu ? len : 0xffffu);
+      }
+      carry += __shfl(x, 63, 64);
+    }
+    const uint32_t total = __builtin_amdgcn_readfirstlane(carry);
+    if (total <= kBatchWindow && max_nbrs <= kBatchStage && rp.n <= 65536u) {  // u16 window entries
+      wave_sync_lds();
+      for (uint32_t sg = lane; sg < nseg; sg += 64) {
+        const uint32_t pre = wb.seg_pre[sg], len = wb.seg_len[sg], first = wb.seg_first[sg];
+        for (uint32_t q = 0; q < len; ++q) wb.cand[pre + q] = static_cast<uint16_t>(first + q - rp.slot0);
+      }
+      wave_sync_lds();
+      for (uint32_t p = 0; p < m; ++p) {
+        const uint32_t i = cs + p;
+        const float4 ri = st.r[i];
+        const float* crow = s_cut + __float_as_uint(ri.w) * nt;
+        uint32_t cnt = 0;
+        for (uint32_t q0 = 0; q0 < total; q0 += 4 * 64) {
+          uint32_t j[4];
+          float4 rj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t q = q0 + 64u * u + lane;
+            j[u] = q < total ? rp.slot0 + wb.cand[q] : i;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) rj[u] = st.r[j[u]];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const bool acc = j[u] != i && min_image_d2_f32(ri, rj[u], bx) < crow[__float_as_uint(rj[u].w)];
+            const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
+            const uint32_t k = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+            if (acc && k < max_nbrs) wb.list[k] = j[u];
+            cnt += static_cast<uint32_t>(__builtin_popcountll(mask));
           }
         }
+        wave_sync_lds();
+        const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
+        for (uint32_t k = lane; k < stored; k += 64) nbl[static_cast<size_t>(k) * a.nslots + i] = wb.list[k];
+        if (lane == 0) {
+          num_nbrs[i] = stored;
+          if (cnt >= max_nbrs) fl[kOverflow] = 1u;
+        }
+        longest = cnt > longest ? cnt : longest;
+        wave_sync_lds();  // the flush has read the list
+      }
+    } else {
+      // one lane per particle over the segments (any stencil size)
+      for (uint32_t p = lane; p < m; p += 64) {
+        const uint32_t i = cs + p;
+        const float4 ri = st.r[i];
+        const float* crow = s_cut + __float_as_uint(ri.w) * nt;
+        uint32_t cnt = 0;
+        for (uint32_t sg = 0; sg < nseg; ++sg) {
+          uint32_t first;
+          const uint32_t len = stencil_segment(rp, cell_start, cell_n, cx, cy, cz, rows, sg, first);
+          for (uint32_t j = first; j < first + len; ++j) {
+            const float4 rj = st.r[j];
+            if (j != i && min_image_d2_f32(ri, rj, bx) < crow[__float_as_uint(rj.w)]) {
+              if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * a.nslots + i] = j;
+              ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
+            }
+          }
+        }
+        num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;
+        if (cnt >= max_nbrs) fl[kOverflow] = 1u;
+        longest = cnt > longest ? cnt : longest;
       }
     }
-    num_nbrs[i] = cnt < max_nbrs ? cnt : max_nbrs;
-    if (cnt >= max_nbrs) fl[kOverflow] = 1u;
-    longest = cnt;
-    // the particle moves to its sorted slot
-    a.r[i] = ri;
-    a.fixed[i] = to_fixed(ri, fixed_scale(rp));
-    a.v[i] = st.v[i];
-    a.im[i] = st.im[i];
-    idx[i] = st.idx[i];
-    slot_cell[i] = rp.cell0 + lc;
-    last_r[i] = ri;
+    // the cell's particles move to their sorted slots
+    for (uint32_t p = lane; p < m; p += 64) {
+      const uint32_t i = cs + p;
+      const float4 ri = st.r[i];
+      a.r[i] = ri;
+      a.fixed[i] = to_fixed(ri, fixed_scale(rp));
+      a.v[i] = st.v[i];
+      a.im[i] = st.im[i];
+      idx[i] = st.idx[i];
+      slot_cell[i] = c;
+      last_r[i] = ri;
+    }
   }
   longest = block_max_u32(longest, s_max);
   if (threadIdx.x == 0) atomicMax(fl + kMaxLen, longest);
-  const uint32_t nblk = (rp.n + kBlock - 1) / kBlock;
+  const uint32_t nblk = (ncell + RUMDX_BATCH_CELL_BLOCK - 1) / RUMDX_BATCH_CELL_BLOCK;
   if (last_to_arrive(fl + kDone, nblk) && threadIdx.x == 0) {
     fl[kRebuild] = 0u;
     fl[kBuilds] += 1u;
@@ -534,7 +638,8 @@ extern "C" int rumdx_batch_update_lists(rumdx_batch* b, void* stream) {
   if (b->nbl_dev == nullptr || b->num_nbrs_dev == nullptr || b->last_r_dev == nullptr || b->cut_skin2_dev == nullptr ||
       b->cell_count_dev == nullptr || b->cell_start_dev == nullptr || b->cell_n_dev == nullptr ||
       b->cell_tmp_dev == nullptr || b->slot_cell_dev == nullptr || b->idx_dev == nullptr || b->t_r_dev == nullptr ||
-      b->t_v_dev == nullptr || b->t_im_dev == nullptr || b->t_idx_dev == nullptr || b->fixed_dev == nullptr)
+      b->t_v_dev == nullptr || b->t_im_dev == nullptr || b->t_idx_dev == nullptr || b->fixed_dev == nullptr ||
+      b->cellblock_rep_dev == nullptr || b->ncells == 0 || b->ncells % RUMDX_BATCH_CELL_BLOCK != 0)
     return fail(RUMDX_E_INVALID, "rumdx_batch_update_lists: list arrays not allocated");
   if (b->max_nbrs == 0 || b->ntypes == 0 || b->ntypes > RUMDX_MAX_TYPES)
     return fail(RUMDX_E_INVALID, "rumdx_batch_update_lists: max_nbrs or ntypes out of range");
@@ -552,8 +657,9 @@ extern "C" int rumdx_batch_update_lists(rumdx_batch* b, void* stream) {
                         reinterpret_cast<float4*>(b->t_im_dev), b->t_idx_dev};
   hipLaunchKernelGGL(k_b_gather, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
                      b->cell_n_dev, b->cell_count_dev, b->cell_tmp_dev, b->idx_dev, stage);
-  hipLaunchKernelGGL(k_b_build, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
-                     b->cell_n_dev, stage, b->idx_dev, b->cut_skin2_dev, b->ntypes, b->nbl_dev, b->num_nbrs_dev,
+  hipLaunchKernelGGL(k_b_build, dim3(b->ncells / RUMDX_BATCH_CELL_BLOCK), dim3(kBlock), 0, st, a,
+                     b->cellblock_rep_dev, b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev, stage, b->idx_dev,
+                     b->cut_skin2_dev, b->ntypes, b->nbl_dev, b->num_nbrs_dev,
                      reinterpret_cast<float4*>(b->last_r_dev), b->max_nbrs);
   return check_launch("rumdx_batch_update_lists");
 }

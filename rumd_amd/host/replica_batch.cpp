@@ -211,7 +211,8 @@ void ReplicaBatch::Allocate() {
     rumdx_check(rumdx_cell_grid(box8, max_cutoff_, skin_, nc), "ReplicaBatch", "Allocate");
     nc_[r] = nc[0];
     cell0_[r] = static_cast<uint32_t>(cells);
-    cells += static_cast<uint64_t>(nc[0]) * nc[0] * nc[0];
+    const uint64_t ncell = static_cast<uint64_t>(nc[0]) * nc[0] * nc[0];
+    cells += (ncell + RUMDX_BATCH_CELL_BLOCK - 1) / RUMDX_BATCH_CELL_BLOCK * RUMDX_BATCH_CELL_BLOCK;
     max_density = std::max(max_density, s.types.size() / (static_cast<double>(s.L) * s.L * s.L));
     max_n = std::max(max_n, static_cast<unsigned>(s.types.size()));
   }
@@ -277,6 +278,15 @@ void ReplicaBatch::Allocate() {
   flags_.upload(hflags.data(), hflags.size(), stream_);
   nbl_.resize(static_cast<size_t>(max_nbrs_) * ns);
   num_nbrs_.resize(ns);
+  std::vector<uint32_t> hcb(ncells_ / RUMDX_BATCH_CELL_BLOCK);
+  for (unsigned r = 0; r < nrep; ++r) {
+    const uint64_t ncell = static_cast<uint64_t>(nc_[r]) * nc_[r] * nc_[r];
+    const uint32_t b0 = cell0_[r] / RUMDX_BATCH_CELL_BLOCK;
+    const uint32_t nb = static_cast<uint32_t>((ncell + RUMDX_BATCH_CELL_BLOCK - 1) / RUMDX_BATCH_CELL_BLOCK);
+    for (uint32_t k = 0; k < nb; ++k) hcb[b0 + k] = r;
+  }
+  cellblock_rep_.resize(hcb.size());
+  cellblock_rep_.upload(hcb.data(), hcb.size(), stream_);
   cell_count_.resize(ncells_);
   cell_start_.resize(ncells_);
   cell_n_.resize(ncells_);
@@ -321,6 +331,7 @@ void ReplicaBatch::Allocate() {
   desc_.ntypes = ntypes_;
   desc_.skin = skin_;
   desc_.ncells = ncells_;
+  desc_.cellblock_rep_dev = cellblock_rep_.get();
   desc_.cell_count_dev = cell_count_.get();
   desc_.cell_start_dev = cell_start_.get();
   desc_.cell_n_dev = cell_n_.get();

@@ -100,7 +100,7 @@ class ReplicaBatch {
   uint32_t nslots_ = 0, ncells_ = 0;
   DeviceArray<float> r_, v_, f_, w_, im_, last_r_, params_, cut2_, t_r_, t_v_, t_im_;
   DeviceArray<uint32_t> block_rep_, flags_, nbl_, num_nbrs_, cell_count_, cell_start_, cell_n_, cell_tmp_,
-      slot_cell_, idx_, t_idx_, fixed_;
+      slot_cell_, idx_, t_idx_, fixed_, cellblock_rep_;
   DeviceArray<rumdx_replica> rep_;
   DeviceArray<rumdx_nh_state> nh_, save_nh_;
   DeviceArray<float> save_r_, save_v_, save_f_, save_w_, save_im_;

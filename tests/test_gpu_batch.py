@@ -225,3 +225,25 @@ def test_batch_autotune(tmp_path):
         assert np.array_equal(ref.GetPositions(r), b.GetPositions(r)), r
     again = Autotune(auto_filename=tmp_path / "autotune.json")
     assert again.TuneBatch(b) == tech and again.trials == []
+
+
+def test_batch_build_per_lane_path():
+    """Lists longer than the build's LDS staging (capacity 300 here) take the
+    one-lane-per-particle build; sets and forces still match the oracle."""
+    sizes = [900, 2600]
+    systems = _systems(sizes, seed0=960)
+    b = make_batch(systems, [0.5, 0.5], max_nbrs=300)
+    print("[case] batch build per-lane path", flush=True)
+    b.CalcF()
+    cut2 = configs.cut_skin2_table(ka_table(), 0.3)
+    for r, s in enumerate(systems):
+        pos4 = configs.pack_pos4(s.positions, s.types)
+        f_ref, _ = oracle.forces_brute(pos4, s.box, 2, ka_table(), SF)
+        err = force_errors(b.GetForcesAndEnergies(r)[:, :3], f_ref[:, :3])
+        assert err["guarded_max"] <= FORCE_TOL_GUARDED, (r, err)
+        b8 = configs.box8(s.box)
+        off, ids = b.GetNeighborListCSR(r)
+        o2, i2 = oracle.nblist_cells_f32(pos4, b8, 2, cut2, oracle.cell_grid(b8, 2.5, 0.3))
+        assert np.array_equal(off, o2), r
+        for i in range(s.n):
+            assert set(ids[off[i]:off[i + 1]].tolist()) == set(i2[o2[i]:o2[i + 1]].tolist()), (r, i)
