@@ -296,12 +296,19 @@ def main() -> int:
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--single-replicas", type=int, default=32, help="replicas timed one at a time")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched-replica workload and print its figures (for profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(the single large system would shard by spatial domains, see the design notes).", file=sys.stderr)
         return 2
+
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(args.replicas, 100, args.batch_steps, args.single_replicas, 100)}),
+              flush=True)
+        return 0
 
     if args.calibrate_counters > 0:
         print(json.dumps({"calibration": counter_calibration(args.calibrate_counters, 3)}), file=sys.stderr, flush=True)
