@@ -34,6 +34,7 @@ def lib() -> C.CDLL:
             build()
         L = C.CDLL(str(LIB_PATH))
         L.orc_forces_brute.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p, f64p]
+        L.orc_stress_brute.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p]
         L.orc_forces_nl.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, i64p, i32p, f64p, f64p]
         L.orc_forces_cells.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p, f64p]
         L.orc_forces_cells.restype = C.c_int
@@ -82,6 +83,16 @@ def forces_brute(pos4, L, nt, params, cutoff):
     vir = np.zeros(n)
     lib().orc_forces_brute(n, pos4, _box3(L), nt, _f32(params), cutoff, f4, vir)
     return f4, vir
+
+
+def stress_brute(pos4, L, nt, params, cutoff):
+    """FP64 per-particle configurational stress over all pairs: [n, 6] as
+    (sxx, syy, szz, syz, sxz, sxy), each -sum_j d_a d_b s_ij."""
+    pos4 = _f32(pos4)
+    n = pos4.shape[0]
+    sts = np.zeros((n, 6))
+    lib().orc_stress_brute(n, pos4, _box3(L), nt, _f32(params), cutoff, sts)
+    return sts
 
 
 def forces_nl(pos4, L, nt, params, cutoff, offsets, idx):

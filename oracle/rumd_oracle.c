@@ -20,6 +20,10 @@
  *     Conf/KobAndersenAlt/KA_N864_L9.0_T0.50AltMass; forces, pe, vir columns),
  *     LJ_N1000_rho0.80_T0.80 at r - v dt, closed forms for SF (F(rc)=0,
  *     u(rc)=0, -du/dr = F by finite differences), Newton's third law.
+ *   orc_stress_brute: per particle, sxx + syy + szz = -vir (the pinned
+ *     virial); summed, the strain derivatives of the pinned total energy
+ *     (dU/de for x -> x (1 + e) and for the shears x -> x + e y etc., by
+ *     Richardson-extrapolated central differences on an isolated cluster).
  *   orc_nblist_*: brute force vs cell scan, symmetry, superset of the rc sphere.
  *   orc_nvt_step: closed form for one particle, Ps = 0 => NVE, time
  *     reversibility, the r(t+dt) = r + v(t+dt/2) dt relation held by the
@@ -96,9 +100,9 @@ static pair_par make_par(const float* param) {
   return p;
 }
 
-/* Adds the (i,j) term to fi[0..2], u, vir.  Returns 1 if inside the cut-off. */
-static inline int pair_add(const pair_par* p, int cutoff, const double d[3], double d2, double* fi, double* u,
-                           double* vir) {
+/* s = -u'(r)/r and u of one pair (fij, include/rumd/PairPotential.h:1198-1220).
+ * Returns 0 at or beyond the cut-off. */
+static inline int pair_su(const pair_par* p, int cutoff, double d2, double* s_out, double* e_out) {
   if (!(d2 < p->rc2)) return 0;
   const double i2 = 1.0 / d2, i6 = i2 * i2 * i2, t = p->A12 * i6;
   double s = i6 * i2 * (12.0 * t + 6.0 * p->A6);
@@ -110,6 +114,16 @@ static inline int pair_add(const pair_par* p, int cutoff, const double d[3], dou
     s -= p->src / r;
     e += p->src * (r - p->rc) - p->ucut;
   }
+  *s_out = s;
+  *e_out = e;
+  return 1;
+}
+
+/* Adds the (i,j) term to fi[0..2], u, vir.  Returns 1 if inside the cut-off. */
+static inline int pair_add(const pair_par* p, int cutoff, const double d[3], double d2, double* fi, double* u,
+                           double* vir) {
+  double s, e;
+  if (!pair_su(p, cutoff, d2, &s, &e)) return 0;
   fi[0] += d[0] * s;
   fi[1] += d[1] * s;
   fi[2] += d[2] * s;
@@ -146,6 +160,35 @@ void orc_forces_brute(int n, const float* pos4, const double L[3], int nt, const
     f4[4 * (size_t)i + 2] = fi[2];
     f4[4 * (size_t)i + 3] = 0.5 * u;
     vir[i] = w;
+  }
+  free(tab);
+}
+
+/* Per-particle configurational stress over all pairs, the STR branch of fij
+ * (include/rumd/PairPotential.h:1232-1240): sts6[6i..6i+5] =
+ * (sxx, syy, szz, syz, sxz, sxy), each -sum_j d_a d_b s_ij.  RUMD keeps the
+ * first four in sts and sxz, sxy in w.y, w.z; Sample::GetStress
+ * (src/Sample.cu:393-444) sums them, subtracts 2 m v_a v_b per particle and
+ * scales by 1/(2V). */
+void orc_stress_brute(int n, const float* pos4, const double L[3], int nt, const float* params, int cutoff,
+                      double* sts6) {
+  pair_par* tab = make_table(nt, params);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    double a[6] = {0, 0, 0, 0, 0, 0}, d[3], s, e;
+    const uint32_t ti = type_of(pos4 + 4 * (size_t)i);
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double d2 = min_image_f64(pos4 + 4 * (size_t)i, pos4 + 4 * (size_t)j, L, d);
+      if (!pair_su(&tab[ti * nt + type_of(pos4 + 4 * (size_t)j)], cutoff, d2, &s, &e)) continue;
+      a[0] -= d[0] * d[0] * s;
+      a[1] -= d[1] * d[1] * s;
+      a[2] -= d[2] * d[2] * s;
+      a[3] -= d[1] * d[2] * s;
+      a[4] -= d[0] * d[2] * s;
+      a[5] -= d[0] * d[1] * s;
+    }
+    memcpy(sts6 + 6 * (size_t)i, a, sizeof(a));
   }
   free(tab);
 }

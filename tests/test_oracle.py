@@ -121,6 +121,73 @@ def test_sf_continuity_and_finite_difference():
         assert f4[1, 0] == pytest.approx(-dudr, rel=2e-4)
 
 
+def _cluster(n=40, seed=77, margin=0.02):
+    """An isolated two-type KA cluster in a large box: positions on a 2^-8 grid
+    inside [-3, 3]^3, so x + e y with e = +-2^-12, +-2^-11 stays exact in FP32;
+    no pair within margin of its cut-off (the strain then moves no pair across
+    it) or closer than 0.85."""
+    rng = np.random.default_rng(seed)
+    tab = ka_table()
+    rc = {(0, 0): 2.5, (0, 1): 2.0, (1, 0): 2.0, (1, 1): 2.2}
+    pts, types = [], []
+    while len(pts) < n:
+        x = np.round(rng.uniform(-3.0, 3.0, 3) * 256.0) / 256.0
+        t = int(rng.random() < 0.2)
+        ok = True
+        for y, ty in zip(pts, types):
+            r = float(np.linalg.norm(x - y))
+            if r < 0.85 or abs(r - rc[(t, ty)]) < margin:
+                ok = False
+                break
+        if ok:
+            pts.append(x)
+            types.append(t)
+    pos = np.array(pts, np.float32)
+    return pos, np.array(types, np.int32), tab, np.array([40.0, 40.0, 40.0])
+
+
+@pytest.mark.parametrize("cutoff", [SP, SF])
+def test_stress_is_the_strain_derivative_of_the_energy(cutoff):
+    """Summed over particles, component ab of the stress is twice dU/de for the
+    affine map x_a -> x_a + e x_b (d_a -> d_a + e d_b, so dU/de = sum over
+    pairs u'(r) d_a d_b / r = -sum s d_a d_b): Richardson-extrapolated central
+    differences of the oracle's own (fixture-pinned) energy.  A swapped
+    off-diagonal slot, a dropped sign or a lost factor fails it."""
+    pos, types, tab, L = _cluster()
+    pos4 = configs.pack_pos4(pos, types)
+    sts = oracle.stress_brute(pos4, L, 2, tab, cutoff)
+    total = sts.sum(axis=0)
+    # component order (sxx, syy, szz, syz, sxz, sxy): (a, b) of x_a += e x_b
+    comps = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+
+    def energy(a, b, e):
+        q = pos.astype(np.float64).copy()
+        q[:, a] = q[:, a] + e * q[:, b]
+        q32 = q.astype(np.float32)
+        assert np.array_equal(q32.astype(np.float64), q)  # the strain is exact in FP32
+        f4, _ = oracle.forces_brute(configs.pack_pos4(q32, types), L, 2, tab, cutoff)
+        return float(f4[:, 3].sum())
+
+    h = 2.0 ** -12
+    scale = np.abs(total).max()
+    for k, (a, b) in enumerate(comps):
+        d1 = (energy(a, b, h) - energy(a, b, -h)) / (2 * h)
+        d2 = (energy(a, b, 2 * h) - energy(a, b, -2 * h)) / (4 * h)
+        dude = (4.0 * d1 - d2) / 3.0
+        assert total[k] == pytest.approx(2.0 * dude, rel=1e-7, abs=1e-7 * scale), (k, total[k], 2 * dude)
+
+
+def test_stress_trace_is_minus_the_virial(golden_dir):
+    """Per particle, sxx + syy + szz = -sum_j r^2 s = -vir (the virial the
+    fixtures pin), across the periodic boundary too."""
+    c = load_golden(golden_dir / "KA_N1000_T0.466.xyz.gz")
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    for cutoff in (NS, SP, SF):
+        sts = oracle.stress_brute(pos4, c.box, 2, ka_table(), cutoff)
+        _, vir = oracle.forces_brute(pos4, c.box, 2, ka_table(), cutoff)
+        np.testing.assert_allclose(sts[:, 0] + sts[:, 1] + sts[:, 2], -vir, rtol=1e-12, atol=1e-10)
+
+
 def test_newton3_and_cells_vs_brute():
     s = configs.ka_random_lattice(4000, density=1.2, seed=11)
     pos4 = configs.pack_pos4(s.positions, s.types)
