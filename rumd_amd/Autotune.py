@@ -249,10 +249,10 @@ class Autotune:
         self.SetTechnicalParameters(sim, params)
         self._restore(sim, conf, ps)
         s = sim.sample
-        s.RunSteps(0, 20, sim.block_size, 0)  # warm-up: list build, graph capture
+        s.RunSteps(0, 20, sim.graph_block, 0)  # warm-up: list build, graph capture
         s.Synchronize()
         t0 = time.perf_counter()
-        s.RunSteps(20, nsteps, sim.block_size, 0)
+        s.RunSteps(20, nsteps, sim.graph_block, 0)
         return (time.perf_counter() - t0) * 1e3 / nsteps
 
 

@@ -1,12 +1,15 @@
 """Simulation driver with the run-loop semantics of RUMD's
 Python/rumd/Simulation.py: per step, runtime actions (momentum reset every
-100 steps by default), CalcF, Integrate; one more CalcF after the loop so the
-forces match the final positions; timing reported as MATS (millions of
-atom-timesteps per second).
+100 steps by default), CalcF, the output managers, Integrate; one more CalcF
+after the loop so the forces match the final positions; output scheduling
+(energies linear every 256 steps, trajectory logarithmic, restart at every
+block start) and the output block size chosen as the reference chooses it;
+timing reported as MATS (millions of atom-timesteps per second).
 
-When no Python runtime action other than the momentum reset is registered, the
-loop runs as HIP-graph-captured blocks of steps (Sample.RunSteps); otherwise it
-steps from Python exactly like the reference.
+Steps at which nothing happens on the host run as HIP-graph-captured blocks
+(Sample.RunSteps, which applies the momentum reset itself).  A step at which an
+output manager writes or a Python runtime action fires runs eagerly, in the
+reference's order.
 """
 from __future__ import annotations
 
@@ -20,7 +23,7 @@ from . import rumd
 
 class Simulation:
     def __init__(self, filename_or_conf, pb: int = 0, tp: int = 0, verbose: bool = True,
-                 mixed_precision_positions: bool = False):
+                 mixed_precision_positions: bool = False, timing_filename: str | None = "rumd_timing_info.dat"):
         self.sample = rumd.Sample(pb, tp)
         self.sample.SetVerbose(verbose)
         if mixed_precision_positions:
@@ -32,12 +35,29 @@ class Simulation:
         self.itg = None
         self.potentialList = []
         self.verbose = verbose
-        self.block_size = 10
+        self.graph_block = 10
+        self.blockSize = None
+        self.output_is_initialized = False
+        # (Simulation.py:43-46)
+        self.outputParameters = OrderedDict()
+        self.outputParameters["energies"] = ("linear", 256)
+        self.outputParameters["trajectory"] = ("logarithmic", 1)
+        self.allowedManagers = ["energies", "trajectory"]
+        self.outputDirectory = self.sample.GetOutputDirectory()
+        self.nominalNumBlocks = 100
         self.runtime_actions = {}
         self.runtime_action_intervals = OrderedDict()
         self.runtime_action_intervals["momentum_reset"] = 100
+        self.runtime_action_intervals["sort"] = 0  # Sample.SortParticles
         self.elapsed = 0.0
         self.last_mats = None
+        # timing file written at construction and after each run (Simulation.py:73-79)
+        self.timing_filename = timing_filename
+        self.write_timing_info = timing_filename is not None
+        if self.write_timing_info:
+            with open(self.timing_filename, "w") as fh:
+                fh.write("# %s \n" % rumd.Device.GetDevice().GetDeviceReport())
+                fh.write("# nsteps, time(s), MATS\n")
 
     # ------------------------------------------------------------ set-up
     def SetVerbose(self, vb: bool) -> None:
@@ -59,12 +79,50 @@ class Simulation:
     def SetPB_TP(self, pb: int, tp: int) -> None:
         self.sample.SetPB_TP(pb, tp)
 
-    def SetBlockSize(self, block: int) -> None:
-        """Steps per captured HIP graph in the fast path."""
-        self.block_size = max(1, int(block))
+    def SetBlockSize(self, blockSize) -> None:
+        """Output block size (None = chosen at Run); a power of 2 for
+        logarithmic scheduling (Simulation.py:222-227)."""
+        self.blockSize = blockSize
+
+    def SetGraphBlockSize(self, block: int) -> None:
+        """Steps per captured HIP graph between host events."""
+        self.graph_block = max(1, int(block))
+
+    def SetOutputDirectory(self, outputDirectory: str) -> None:
+        self.outputDirectory = outputDirectory
+        self.sample.SetOutputDirectory(outputDirectory)
+
+    def GetNumberOfParticles(self) -> int:
+        return self.sample.GetNumberOfParticles()
+
+    def GetVolume(self) -> float:
+        return self.sample.GetSimulationBox().GetVolume()
+
+    def SetOutputMetaData(self, manager_name: str, **kwargs) -> None:
+        """e.g. SetOutputMetaData("energies", stress_xy=True, precision=9)
+        (Simulation.py:252-271)."""
+        for key, value in kwargs.items():
+            self.sample.SetOutputManagerMetaData(manager_name, key, value)
+
+    def SetOutputScheduling(self, manager_name: str, schedule: str, **kwargs) -> None:
+        """schedule "none", "linear" (interval=), "logarithmic" (base=, default
+        1), "loglin" (base=, maxInterval=) or "limlin" (interval=, numitems=)
+        (Simulation.py:327-366)."""
+        if manager_name not in self.allowedManagers:
+            raise ValueError("SetOutputScheduling -- first argument must be one of " + str(self.allowedManagers))
+        param_names = {"none": [], "linear": ["interval"], "logarithmic": ["base"],
+                       "loglin": ["base", "maxInterval"], "limlin": ["interval", "numitems"]}
+        if schedule not in param_names:
+            raise ValueError("SetOutputScheduling -- second argument must be " + str(list(param_names)))
+        if schedule == "logarithmic" and "base" not in kwargs:
+            kwargs["base"] = 1
+        self.outputParameters[manager_name] = tuple([schedule] + [kwargs[name] for name in param_names[schedule]])
 
     def SetMomentumResetInterval(self, interval: int) -> None:
         self.runtime_action_intervals["momentum_reset"] = int(interval)
+
+    def SetSortInterval(self, sort_interval: int) -> None:
+        self.runtime_action_intervals["sort"] = int(sort_interval)
 
     def SetRuntimeAction(self, name: str, method, interval: int) -> None:
         self.runtime_actions[name] = (weakref.ref(method.__self__), weakref.ref(method.__func__))
@@ -80,6 +138,55 @@ class Simulation:
     def WriteConf(self, filename, mode: str = "w") -> None:
         self.sample.WriteConf(filename, mode)
 
+    # ------------------------------------------------------------ output
+    def InitializeOutput(self, num_iter: int) -> None:
+        """Manager schedules and the output block size (Simulation.py:444-510):
+        unless set, the smallest power of two >= 1024 that covers every linear
+        interval and num_iter / 200."""
+        num_active = 0
+        min_block = 1
+        for name, par in self.outputParameters.items():
+            if par[0] == "none":
+                self.sample.SetOutputManagerActive(name, False)
+                continue
+            num_active += 1
+            self.sample.SetOutputManagerActive(name, True)
+            schedule = par[0]
+            if schedule in ("linear", "limlin"):
+                interval = par[1]
+                user_max = par[2] - 1 if schedule == "limlin" else -1
+                self.sample.SetLogLinParameters(name, interval, interval, user_max)
+                min_block = max(min_block, interval)
+            elif schedule == "logarithmic":
+                self.sample.SetLogLinParameters(name, par[1], 0)
+            elif schedule == "loglin":
+                self.sample.SetLogLinParameters(name, par[1], par[2])
+            else:
+                raise ValueError("Simulation.InitializeOutput: Unknown schedule %s" % schedule)
+        if self.blockSize is None:
+            if num_active > 0:
+                min_block = max(min_block, num_iter // self.nominalNumBlocks // 2)
+                block = 1024
+                while block < min_block:
+                    block *= 2
+            else:
+                block = num_iter // self.nominalNumBlocks
+            self.blockSize = block
+        if self.verbose:
+            print("Output blockSize=%d, numBlocks=%d" % (self.blockSize, num_iter // self.blockSize + 1))
+            print("Size of the last incomplete block", num_iter % self.blockSize)
+        self.sample.SetOutputBlockSize(self.blockSize)
+        self.output_is_initialized = True
+
+    def CheckSortingClash(self) -> None:
+        """A sort action is redundant with the sorted neighbour-list method,
+        which sorts at every rebuild (Simulation.py:512-527)."""
+        uses_sort = any(getattr(p, "GetNB_Method", lambda: "")() == "sort" for p in self.potentialList)
+        if uses_sort and self.runtime_action_intervals.get("sort", 0) > 0:
+            print("[Warning] sort_interval for ordinary sorting should be zero when using the sort-based NB method. "
+                  "Setting to zero now.")
+            self.SetSortInterval(0)
+
     # ------------------------------------------------------------ run loop
     def ApplyRuntimeActions(self, i: int) -> None:
         for key, interval in self.runtime_action_intervals.items():
@@ -87,9 +194,24 @@ class Simulation:
                 continue
             if key == "momentum_reset":
                 self.ResetMomentum()
+            elif key == "sort":
+                self.sample.SortParticles()
             else:
                 obj, func = (r() for r in self.runtime_actions[key])
                 func(obj)
+
+    def _next_host_event(self, i: int, output: bool) -> int | None:
+        """First step >= i that must run on the host: an output write or a
+        Python runtime action (the momentum reset runs inside RunSteps)."""
+        cand = []
+        if output:
+            t = self.sample.NextOutputStep(i)
+            if t is not None:
+                cand.append(t)
+        for key, interval in self.runtime_action_intervals.items():
+            if key != "momentum_reset" and interval > 0:
+                cand.append(i + (-i) % interval)
+        return min(cand) if cand else None
 
     def Run(self, num_iter: int, initializeOutput: bool = True, restartBlock=None, suppressAllOutput: bool = False,
             force_timing: bool = False) -> float:
@@ -97,24 +219,53 @@ class Simulation:
             raise ValueError("Integrator has not been set")
         if not self.potentialList:
             raise ValueError("Potential has not been set")
-        if restartBlock is not None:
-            raise ValueError("restart blocks need the output managers, which this build does not provide")
+        self.CheckSortingClash()
+        first = 0
+        output = not suppressAllOutput
+        if output:
+            if initializeOutput:
+                self.InitializeOutput(num_iter)
+            elif not self.output_is_initialized:
+                raise ValueError("Cannot have initializeOutput=False if output has not previously been initialized")
+            if restartBlock is not None:
+                if not initializeOutput:
+                    raise ValueError("When restarting, initializeOutput must be True")
+                self.sample.ReadRestartConf(restartBlock)
+                first = int(restartBlock) * self.blockSize
+            self.sample.InitializeOutputManagers(first)
+        elif restartBlock is not None:
+            raise ValueError("restartBlock needs output (suppressAllOutput=False)")
+        timing = output or force_timing
         t0 = time.perf_counter()
-        python_actions = [k for k in self.runtime_action_intervals
-                          if k != "momentum_reset" and self.runtime_action_intervals[k] > 0]
-        if python_actions:
-            for i in range(num_iter):
-                self.ApplyRuntimeActions(i)
-                self.sample.CalcF()
-                self.itg.Integrate()
-            self.sample.Synchronize()
-        else:
-            self.sample.RunSteps(0, int(num_iter), self.block_size,
-                                 max(0, self.runtime_action_intervals.get("momentum_reset", 0)))
+        mom = max(0, self.runtime_action_intervals.get("momentum_reset", 0))
+        i = first
+        while i < num_iter:
+            nxt = self._next_host_event(i, output)
+            stop = num_iter if nxt is None else min(nxt, num_iter)
+            if stop > i:
+                self.sample.RunSteps(i, stop - i, self.graph_block, mom)
+                i = stop
+                continue
+            # a host step, in the reference's order (Simulation.py:581-597)
+            self.ApplyRuntimeActions(i)
+            self.sample.CalcF()
+            if output:
+                self.sample.NotifyOutputManagers(i)
+            self.itg.Integrate()
+            i += 1
+        self.sample.Synchronize()
         self.elapsed = time.perf_counter() - t0
-        self.last_mats = self.sample.GetNumberOfParticles() * num_iter / 1e6 / max(self.elapsed, 1e-12)
-        if (not suppressAllOutput or force_timing) and self.verbose:
-            print(f"{num_iter} steps, {self.elapsed:.3f} s, {self.last_mats:.3f} MATS")
+        nsteps = num_iter - first
+        self.last_mats = self.sample.GetNumberOfParticles() * nsteps / 1e6 / max(self.elapsed, 1e-12)
+        if timing:
+            if self.verbose:
+                print(f"{nsteps} steps, {self.elapsed:.3f} s, {self.last_mats:.3f} MATS")
+            if self.write_timing_info:
+                with open(self.timing_filename, "a") as fh:
+                    fh.write("%d %.3f %.3f\n" % (nsteps, self.elapsed, self.last_mats))
+        rumd.Device.GetDevice().CheckErrors()
         # forces consistent with the final positions (Simulation.py:610-614)
         self.sample.CalcF()
+        if output:
+            self.sample.TerminateOutputManagers()
         return self.elapsed

@@ -43,17 +43,22 @@ def _open(path: Path, mode: str):
     return open(path, mode)
 
 
-def read_conf(path) -> Configuration:
-    with _open(path, "r") as fh:
-        n = int(fh.readline().split()[0])
-        meta = {}
-        for tok in fh.readline().split():
-            if "=" in tok:
-                k, v = tok.split("=", 1)
-                meta[k] = v
-        rows = np.loadtxt(fh, dtype=np.float64, max_rows=n, ndmin=2)
-    if rows.shape[0] != n:
-        raise ValueError(f"{path}: expected {n} particle lines, found {rows.shape[0]}")
+def _read_frame(fh, path) -> Configuration | None:
+    """One frame (count line, comment line, N particle lines); None at the
+    end of the file."""
+    first = fh.readline()
+    if not first.strip():
+        return None
+    n = int(first.split()[0])
+    meta = {}
+    for tok in fh.readline().split():
+        if "=" in tok:
+            k, v = tok.split("=", 1)
+            meta[k] = v
+    lines = [fh.readline() for _ in range(n)]
+    if n and not lines[-1].strip():
+        raise ValueError(f"{path}: expected {n} particle lines")
+    rows = np.array([ln.split() for ln in lines], dtype=np.float64).reshape(n, -1)
     cols = meta.get("columns", "type,x,y,z").split(",")
     col = {c: k for k, c in enumerate(cols)}
     if "sim_box" in meta:
@@ -73,7 +78,7 @@ def read_conf(path) -> Configuration:
             return np.ascontiguousarray(rows[:, [col[c] for c in names]], dtype=dtype)
         return None
 
-    conf = Configuration(
+    return Configuration(
         n=n,
         box=box,
         types=rows[:, col["type"]].astype(np.int32),
@@ -86,11 +91,37 @@ def read_conf(path) -> Configuration:
         vir=rows[:, col["vir"]].copy() if "vir" in col else None,
         meta=meta,
     )
+
+
+def read_conf(path) -> Configuration:
+    """The first frame of a configuration file."""
+    with _open(path, "r") as fh:
+        conf = _read_frame(fh, path)
+    if conf is None:
+        raise ValueError(f"{path}: empty configuration file")
     return conf
 
 
-def write_conf(path, conf: Configuration, precision: int = 9, integrator_info: str | None = None) -> None:
-    """Writes ioformat 2 (the reference's current format)."""
+def read_trajectory(path) -> list[Configuration]:
+    """Every frame of a trajectory file (frames appended one after another)."""
+    frames = []
+    with _open(path, "r") as fh:
+        while True:
+            c = _read_frame(fh, path)
+            if c is None:
+                return frames
+            frames.append(c)
+
+
+def write_conf(path, conf: Configuration, precision: int = 9, integrator_info: str | None = None,
+               box_info: str | None = None, loglin: tuple | None = None, mode: str = "w") -> None:
+    """Writes ioformat 2 (the reference's current format,
+    src/ConfigurationWriterReader.cu:90-180).  ``loglin`` = (timeStepIndex,
+    block, base, index, maxIndex, maxInterval) adds the trajectory's schedule
+    tokens; ``box_info`` replaces the sim_box token (the box's GetInfoString);
+    mode "a" appends a frame (a new gzip member for .gz files)."""
+    if mode not in ("w", "a"):
+        raise ValueError("mode must be 'w' or 'a'")
     cols = ["type", "x", "y", "z"]
     data = [conf.types.astype(np.float64)[:, None], conf.positions.astype(np.float64)]
     if conf.images is not None:
@@ -109,17 +140,23 @@ def write_conf(path, conf: Configuration, precision: int = 9, integrator_info: s
         cols += ["vir"]
         data.append(np.asarray(conf.vir, dtype=np.float64)[:, None])
     arr = np.hstack(data)
-    header = [f"ioformat=2", f"numTypes={conf.num_types}"]
+    header = ["ioformat=2"]
+    if loglin is not None:
+        header.append("timeStepIndex=%d logLin=%d,%d,%d,%d,%d" % tuple(int(x) for x in loglin))
+    header.append(f"numTypes={conf.num_types}")
     if integrator_info:
         header.append(f"integrator={integrator_info}")
-    header.append("sim_box=RectangularSimulationBox," + ",".join(f"{x:.{precision}g}" for x in conf.box))
+    if box_info:
+        header.append(f"sim_box={box_info}")
+    else:
+        header.append("sim_box=RectangularSimulationBox," + ",".join(f"{x:.{precision}g}" for x in conf.box))
     header.append("mass=" + ",".join(f"{m:.{precision}f}" for m in conf.masses))
     header.append("columns=" + ",".join(cols))
     fmt = ["%d"] + [f"%.{precision}f"] * 3
     if conf.images is not None:
         fmt += ["%.0f"] * 3
     fmt += [f"%.{precision}f"] * (arr.shape[1] - len(fmt))
-    with _open(path, "w") as fh:
+    with _open(path, mode) as fh:
         fh.write(f"{conf.n}\n")
         fh.write(" ".join(header) + "\n")
         np.savetxt(fh, arr, fmt=fmt)

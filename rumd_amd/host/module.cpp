@@ -199,6 +199,7 @@ PYBIND11_MODULE(_rumd, m) {
       .def("AddPotential", &Sample::AddPotential, py::arg("potential"), py::keep_alive<1, 2>())
       .def("SetIntegrator", &Sample::SetIntegrator, py::arg("newItg"), py::keep_alive<1, 2>())
       .def("GetIntegrator", &Sample::GetIntegrator, py::return_value_policy::reference_internal)
+      .def("GetPotentials", [](Sample& s) { return s.GetPotentials(); }, py::return_value_policy::reference_internal)
       .def("SetPB_TP", &Sample::SetPB_TP, py::arg("pb"), py::arg("tp"))
       .def("GetParticlesPerBlock", &Sample::GetParticlesPerBlock)
       .def("GetThreadsPerParticle", &Sample::GetThreadsPerParticle)

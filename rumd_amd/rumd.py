@@ -13,6 +13,7 @@ from __future__ import annotations
 import numpy as np
 
 from . import conf_io
+from . import output as _output
 from ._native import _rumd
 
 Device = _rumd.Device
@@ -39,6 +40,116 @@ class Sample(_rumd.Sample):
     def __init__(self, pb: int = 0, tp: int = 0):
         super().__init__(pb, tp)
         self._box = None
+        self._output_dir = "TrajectoryFiles"
+        self._managers = _output.default_managers(self, self._output_dir)
+        self._managers_initialized = False
+        self._output_block_size = 0
+
+    # ------------------------------------------------------------ potentials
+    def SetPotential(self, potential) -> None:
+        super().SetPotential(potential)
+        self._managers["energies"].RegisterPotential(potential)
+        self._managers_initialized = False
+
+    def AddPotential(self, potential) -> None:
+        super().AddPotential(potential)
+        self._managers["energies"].RegisterPotential(potential)
+        self._managers_initialized = False
+
+    def SetIntegrator(self, newItg) -> None:
+        super().SetIntegrator(newItg)
+        self._managers["energies"].RegisterIntegrator(newItg)
+        self._managers_initialized = False
+
+    def SetVerbose(self, vb: bool) -> None:
+        super().SetVerbose(vb)
+        for m in self._managers.values():
+            m.SetVerbose(vb)
+
+    # ------------------------------------------------------------ output managers
+    # Sample's output-manager interface (src/Sample.cu:482-575).
+    def GetOutputDirectory(self) -> str:
+        return self._output_dir
+
+    def SetOutputDirectory(self, outputDirectory: str) -> None:
+        self._output_dir = outputDirectory
+        for m in self._managers.values():
+            m.SetOutputDirectory(outputDirectory)
+        self._managers_initialized = False
+
+    def GetOutputManager(self, manager_name: str):
+        return self._managers[manager_name]
+
+    def AddOutputManager(self, manager_name: str, manager) -> None:
+        self._managers[manager_name] = manager
+        self._managers_initialized = False
+
+    def SetOutputBlockSize(self, blockSize: int) -> None:
+        if not blockSize:
+            raise ValueError("Sample::SetOutputBlockSize: blockSize must be greater than zero")
+        self._output_block_size = int(blockSize)
+        for m in self._managers.values():
+            m.SetBlockSize(blockSize)
+        self._managers["restart"].SetLogLinParams(blockSize, 0)
+        self._managers_initialized = False
+
+    def GetOutputBlockSize(self) -> int:
+        return self._output_block_size
+
+    def SetOutputManagerActive(self, manager_name: str, active: bool) -> None:
+        if manager_name not in self._managers:
+            print(f"Warning [Sample::SetOutputManagerActive]: Manager {manager_name} not present")
+            return
+        self._managers[manager_name].SetActive(active)
+
+    def SetOutputManagerMetaData(self, manager_name: str, key: str, value) -> None:
+        if manager_name not in self._managers:
+            print(f"Warning [Sample::SetOutputManagerMetaData]: Manager {manager_name} not present")
+            return
+        self._managers[manager_name].SetMetaData(key, value)
+
+    def SetLogLinParameters(self, manager_name: str, base: int, maxInterval: int, user_maxIndex: int = -1) -> None:
+        m = self._managers[manager_name]
+        m.SetLogLinParams(base, maxInterval)
+        m.SetUserMaxIndex(user_maxIndex)
+        # linear, or logarithmic limited to the first items: no duplicate block ends
+        m.SetDuplicateBlockEnds(not (base == maxInterval or user_maxIndex >= 0))
+        self._managers_initialized = False
+
+    def InitializeOutputManagers(self, timeStepIndex: int) -> None:
+        energies = self._managers["energies"]
+        for pot in self.GetPotentials():
+            energies.RegisterPotential(pot)
+        energies.RegisterIntegrator(self.GetIntegrator())
+        create_dir = True
+        for name in sorted(self._managers):
+            self._managers[name].Initialize(int(timeStepIndex), create_dir)
+            create_dir = False  # only the first one creates the directory
+        self._managers_initialized = True
+
+    def NotifyOutputManagers(self, timeStepIndex: int) -> None:
+        if not self._managers_initialized:
+            self.InitializeOutputManagers(timeStepIndex)
+        for name in sorted(self._managers):
+            self._managers[name].Update(int(timeStepIndex))
+
+    def NextOutputStep(self, timeStepIndex: int):
+        """The first step >= timeStepIndex at which a manager writes (None if
+        no manager is active)."""
+        steps = [m.next_write_step(int(timeStepIndex)) for m in self._managers.values()]
+        steps = [t for t in steps if t is not None]
+        return min(steps) if steps else None
+
+    def TerminateOutputManagers(self) -> None:
+        for m in self._managers.values():
+            m.Terminate()
+
+    def ReadRestartConf(self, restartBlock: int) -> None:
+        """Reads restartNNNN.xyz.gz from the output directory, integrator
+        state included (Sample::ReadRestartConf)."""
+        m = self._managers["restart"]
+        fn = f"{self._output_dir}/restart{int(restartBlock):0{m.numFilenameDigits}d}.xyz.gz"
+        self.ReadConf(fn, init_itg=True)
 
     def SetConfiguration(self, conf: conf_io.Configuration) -> None:
         """Loads a configuration: box, particle count, types, masses,

@@ -156,7 +156,8 @@ def test_nve_energy_drift_small(golden_dir):
     assert abs(e1 - e0) / c.n < 2e-3
 
 
-def test_simulation_driver_and_conf_round_trip(tmp_path, golden_dir):
+def test_simulation_driver_and_conf_round_trip(tmp_path, golden_dir, monkeypatch):
+    monkeypatch.chdir(tmp_path)  # the timing file goes to the working directory
     sim = Simulation(golden_dir / "KA_N256_L6.0_T0.50.xyz.gz", verbose=False)
     pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedPotential)
     pot.SetVerbose(False)
@@ -231,12 +232,13 @@ def test_error_paths():
         s.SetPositions(np.zeros((10, 3), np.float32))
 
 
-def test_autotune_tunes_reuses_and_keeps_the_trajectory(tmp_path):
+def test_autotune_tunes_reuses_and_keeps_the_trajectory(tmp_path, monkeypatch):
     """Autotune times short runs over tp, list method and skin, saves the
     choice, reuses it for a matching simulation, and leaves the state as it
     was: the run after tuning equals a fresh run with those parameters."""
     from rumd_amd.Autotune import Autotune
 
+    monkeypatch.chdir(tmp_path)  # the timing file goes to the working directory
     sysm = configs.ka_random_lattice(2000, density=1.2, temperature=0.45, seed=77)
 
     def make_sim():
