@@ -344,7 +344,7 @@ def main() -> int:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": read_traffic(ROOT / "profiles"),
-        "kernel": "k_lj_tiles<SF> (LJ force over the cell-tile lists)",
+        "kernel": "k_pair_tiles<LjPair<SF>> (LJ force over the cell-tile lists)",
         "kernel_ms": round(force_ms, 5),
         "bytes_per_atom": round(bytes_per_atom, 2),
     }

@@ -43,6 +43,17 @@ extern "C" {
 
 /* Number of floats per type pair in the parameter table (include/rumd/rumd_base.h:28). */
 #define RUMDX_NUM_PARAM 12
+
+/* Pair potential forms served by rumdx_pair_form (RUMD's Pot_* classes
+ * outside the 12-6 family; parameter rows as each class's SetParams stores
+ * them, include/rumd/PairPotential.h). */
+#define RUMDX_POT_LJ_SMOOTH_2_4 1 /* Pot_LJ_12_6_smooth_2_4 (:218-256) */
+#define RUMDX_POT_GLJ_M_N 2       /* Pot_gLJ_m_n (:264-304) */
+#define RUMDX_POT_GAUSS 3         /* Pot_Gauss (:383-414) */
+#define RUMDX_POT_BUCKINGHAM 4    /* Pot_Buckingham (:422-461) */
+#define RUMDX_POT_IPL_N 5         /* Pot_IPL_n (:600-636) */
+#define RUMDX_POT_YUKAWA 6        /* Pot_Yukawa (:795-840) */
+#define RUMDX_POT_EXP 7           /* Pot_Exp (:1019-1052) */
 /* Largest number of particle types a launch accepts (thermostat mask is 32 bits,
  * src/IntegratorNVT.cu:157-174). */
 #define RUMDX_MAX_TYPES 32
@@ -213,6 +224,18 @@ int rumdx_pair_lj(const rumdx_particles* p, const rumdx_nblist* nb, const float*
 int rumdx_pair_lj_fp64(const rumdx_particles* p, const rumdx_nblist* nb, const float* params_dev,
                        uint32_t ntypes, const float box[8], int cutoff_method, int initialize,
                        int calc_stresses, void* stream);
+
+/* Force, energy and virial of another pair potential form (RUMDX_POT_*):
+ * PairPotential::CalcF of that Pot_* class, whose kernels the reference
+ * generates from one skeleton per potential (src/Generate_PP_FunctionBodies.py:3-127,
+ * ComputeInteraction of each class in include/rumd/PairPotential.h).
+ * params_dev is RUMD's table as for rumdx_pair_lj (param 0 = Rcut, 1..7 the
+ * form's parameters, 10 = s(rc)*rc, 11 = u(rc)).  Same lists, cut-off
+ * methods, stress option, initialize semantics and errors as rumdx_pair_lj;
+ * pair arithmetic in FP32 with FP32 exp/log/sqrt. */
+int rumdx_pair_form(const rumdx_particles* p, const rumdx_nblist* nb, const float* params_dev,
+                    uint32_t ntypes, const float box[8], int form, int cutoff_method, int initialize,
+                    int calc_stresses, void* stream);
 
 /* ---------------------------------------------------------------- integrator
  * IntegratorNVT::Integrate (integrateNVTAlgorithm, src/IntegratorNVT.cu:64-79,

@@ -35,6 +35,7 @@ def lib() -> C.CDLL:
         L = C.CDLL(str(LIB_PATH))
         L.orc_forces_brute.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p, f64p]
         L.orc_stress_brute.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p]
+        L.orc_forces_form_brute.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, C.c_int, f64p, f64p]
         L.orc_forces_nl.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, i64p, i32p, f64p, f64p]
         L.orc_forces_cells.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p, f64p]
         L.orc_forces_cells.restype = C.c_int
@@ -82,6 +83,17 @@ def forces_brute(pos4, L, nt, params, cutoff):
     f4 = np.zeros((n, 4))
     vir = np.zeros(n)
     lib().orc_forces_brute(n, pos4, _box3(L), nt, _f32(params), cutoff, f4, vir)
+    return f4, vir
+
+
+def forces_form_brute(pos4, L, nt, params, form, cutoff):
+    """FP64 forces of a pair form (RUMDX_POT_* number) over all pairs:
+    (f4 [n,4], vir [n]) as forces_brute."""
+    pos4 = _f32(pos4)
+    n = pos4.shape[0]
+    f4 = np.zeros((n, 4))
+    vir = np.zeros(n)
+    lib().orc_forces_form_brute(n, pos4, _box3(L), nt, _f32(params), form, cutoff, f4, vir)
     return f4, vir
 
 

@@ -24,6 +24,12 @@
  *     virial); summed, the strain derivatives of the pinned total energy
  *     (dU/de for x -> x (1 + e) and for the shears x -> x + e y etc., by
  *     Richardson-extrapolated central differences on an isolated cluster).
+ *   orc_forces_form_brute: each form's energy against its defining formula
+ *     in the physical parameters (sigma, epsilon, ...), -du/dr = F by finite
+ *     differences, special cases that reduce to the pinned 12-6 path
+ *     (gLJ 12-6 = LJA, IPL_n with n = 12 = IPL 12-6 with A6 = 0), the
+ *     Buckingham minimum -epsilon at rm and the smooth LJ's vanishing force
+ *     at Rcut.
  *   orc_nblist_*: brute force vs cell scan, symmetry, superset of the rc sphere.
  *   orc_nvt_step: closed form for one particle, Ps = 0 => NVE, time
  *     reversibility, the r(t+dt) = r + v(t+dt/2) dt relation held by the
@@ -191,6 +197,102 @@ void orc_stress_brute(int n, const float* pos4, const double L[3], int nt, const
     memcpy(sts6 + 6 * (size_t)i, a, sizeof(a));
   }
   free(tab);
+}
+
+/* s = -u'(r)/r and u of the pair forms outside the 12-6 family, in FP64 from
+ * the float parameter row q (ComputeInteraction of each class; the form
+ * numbers are those of the C ABI):
+ *   1 Pot_LJ_12_6_smooth_2_4 include/rumd/PairPotential.h:249-255
+ *   2 Pot_gLJ_m_n            :296-303
+ *   3 Pot_Gauss              :409-414
+ *   4 Pot_Buckingham         :451-460
+ *   5 Pot_IPL_n              :627-634
+ *   6 Pot_Yukawa             :829-838
+ *   7 Pot_Exp                :1046-1052 */
+static double form_su(int form, double d2, const float* q, double* u) {
+  const double p1 = q[1], p2 = q[2], p3 = q[3], p4 = q[4], p5 = q[5];
+  switch (form) {
+    case 1: {
+      const double x2 = d2 * p1, i2 = 1.0 / x2, i6 = i2 * i2 * i2;
+      *u = p2 * (i6 * (i6 - 1.0) + p3 * x2 + p4 * x2 * x2);
+      return p2 * p1 * (i2 * i6 * (12.0 * i6 - 6.0) - 2.0 * p3 - 4.0 * p4 * x2);
+    }
+    case 2: {
+      const double i2 = 1.0 / (d2 * p1);
+      const double im = pow(i2, p3), in = pow(i2, p4);
+      *u = p5 * (2.0 * p4 * im - 2.0 * p3 * in);
+      return p1 * p2 * (im - in) * i2;
+    }
+    case 3: {
+      const double e = exp(-d2 * p1);
+      *u = p2 * e;
+      return 2.0 * p1 * p2 * e;
+    }
+    case 4: {
+      const double i2 = 1.0 / (d2 * p1), i6 = i2 * i2 * i2, i1 = sqrt(i2);
+      const double e = exp(p2 * (1.0 - 1.0 / i1));
+      *u = p3 * e - p4 * i6;
+      return p1 * p5 * (i1 * e - i6 * i2);
+    }
+    case 5: {
+      const double i2 = 1.0 / (d2 * p1), in = pow(i2, 0.5 * p3);
+      *u = p2 * in;
+      return p1 * p2 * p3 * in * i2;
+    }
+    case 6: {
+      const double r = sqrt(d2), v = p3 * p1 / r * exp(-p2 * r);
+      *u = v;
+      return (1.0 / r) * (1.0 / r + p2) * v;
+    }
+    case 7: {
+      const double r = sqrt(d2), e = exp(-r * p1);
+      *u = p2 * e;
+      return p1 * p2 / r * e;
+    }
+    default:
+      *u = 0.0;
+      return 0.0;
+  }
+}
+
+/* Forces of a pair form over all pairs, with fij's cut-off handling
+ * (include/rumd/PairPotential.h:1198-1220) and the shifts u(rc), s(rc) rc of
+ * CopyParamsToGPU (src/PairPotential.cu:82-125) formed in FP64.  Output as
+ * orc_forces_brute. */
+void orc_forces_form_brute(int n, const float* pos4, const double L[3], int nt, const float* params, int form,
+                           int cutoff, double* f4, double* vir) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int i = 0; i < n; ++i) {
+    double fi[3] = {0, 0, 0}, ui = 0, wi = 0, d[3];
+    const uint32_t ti = type_of(pos4 + 4 * (size_t)i);
+    for (int j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const float* q = params + (size_t)(ti * nt + type_of(pos4 + 4 * (size_t)j)) * NUM_PARAM;
+      const double rc = q[0];
+      const double d2 = min_image_f64(pos4 + 4 * (size_t)i, pos4 + 4 * (size_t)j, L, d);
+      if (!(d2 < rc * rc)) continue;
+      double e, ec;
+      double s = form_su(form, d2, q, &e);
+      const double src = form_su(form, rc * rc, q, &ec) * rc;
+      if (cutoff == 1) {
+        e -= ec;
+      } else if (cutoff == 2) {
+        const double r = sqrt(d2);
+        s -= src / r;
+        e += src * (r - rc) - ec;
+      }
+      fi[0] += d[0] * s;
+      fi[1] += d[1] * s;
+      fi[2] += d[2] * s;
+      ui += e;
+      wi += d2 * s;
+    }
+    f4[4 * (size_t)i] = fi[0];
+    f4[4 * (size_t)i + 1] = fi[1];
+    f4[4 * (size_t)i + 2] = fi[2];
+    f4[4 * (size_t)i + 3] = 0.5 * ui;
+    vir[i] = wi;
+  }
 }
 
 /* Over a given neighbour list in CSR form (offsets[n+1], idx). */

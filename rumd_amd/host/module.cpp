@@ -107,6 +107,15 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetNbMaxNumNbrs", &PairPotential::SetNbMaxNumNbrs, py::arg("maxNumNbrs"))
       .def("SetAssumeNewton3", &PairPotential::SetAssumeNewton3, py::arg("assume_N3"))
       .def("GetPotentialParameter", &PairPotential::GetPotentialParameter, py::arg("i"), py::arg("j"), py::arg("index"))
+      .def("GetParamsTable",
+           [](const PairPotential& p, unsigned nt) {
+             const std::vector<float> t = p.GetParamsTable(nt);
+             py::array_t<float> out({static_cast<py::ssize_t>(nt * nt), static_cast<py::ssize_t>(RUMDX_NUM_PARAM)});
+             std::copy(t.begin(), t.end(), out.mutable_data());
+             return out;
+           },
+           py::arg("num_types"))
+      .def("GetPotentialForm", &PairPotential::GetPotentialForm)
       .def("GetParameterList", &PairPotential::GetPotentialParameters, py::arg("num_types"))
       .def("GetPotentialEnergy", &PairPotential::GetPotentialEnergy)
       .def("GetActualMaxNumNbrs", &PairPotential::GetActualMaxNumNbrs)
@@ -150,6 +159,40 @@ PYBIND11_MODULE(_rumd, m) {
   py::class_<Pot_LJ_12_6, Pot_IPL_12_6>(m, "Pot_LJ_12_6")
       .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
       .def("SetParams", &Pot_LJ_12_6::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  py::class_<Pot_gLJ_m12_n6, Pot_IPL_12_6>(m, "Pot_gLJ_m12_n6")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_gLJ_m12_n6::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  py::class_<FormPairPotential, PairPotential>(m, "FormPairPotential");
+  py::class_<Pot_LJ_12_6_smooth_2_4, FormPairPotential>(m, "Pot_LJ_12_6_smooth_2_4")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_LJ_12_6_smooth_2_4::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"),
+           py::arg("Epsilon"), py::arg("Rcut"));
+  py::class_<Pot_gLJ_m_n, FormPairPotential>(m, "Pot_gLJ_m_n")
+      .def(py::init<float, float, PairPotential::CutoffMethod>(), py::arg("set_m"), py::arg("set_n"),
+           py::arg("cutoff_method"))
+      .def("SetParams", &Pot_gLJ_m_n::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  py::class_<Pot_Gauss, FormPairPotential>(m, "Pot_Gauss")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_Gauss::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  py::class_<Pot_Buckingham, FormPairPotential>(m, "Pot_Buckingham")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_Buckingham::SetParams, py::arg("i"), py::arg("j"), py::arg("rm"), py::arg("Alpha"),
+           py::arg("Epsilon"), py::arg("Rcut"));
+  py::class_<Pot_IPL_n, FormPairPotential>(m, "Pot_IPL_n")
+      .def(py::init<float, PairPotential::CutoffMethod>(), py::arg("n"), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_IPL_n::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  py::class_<Pot_Yukawa, FormPairPotential>(m, "Pot_Yukawa")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_Yukawa::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  py::class_<Pot_Exp, FormPairPotential>(m, "Pot_Exp")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_Exp::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
            py::arg("Rcut"));
   m.attr("NoShift") = PairPotential::NS;
   m.attr("ShiftedPotential") = PairPotential::SP;

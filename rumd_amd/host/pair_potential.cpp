@@ -3,8 +3,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <sstream>
 
 #include "rumd_host.h"
+#include "../csrc/pot_forms.h"
 
 namespace {
 constexpr unsigned kNumParam = RUMDX_NUM_PARAM;
@@ -390,14 +392,23 @@ void PairPotential::LaunchForce(const rumdx_particles* p, bool initialize, bool 
                                 const char* method) {
   rumdx_nblist* nb = neighborList.GetResolvedMethod() == NeighborList::NONE ? nullptr : neighborList.Descriptor();
   const float* box = sample->GetSimulationBox()->GetBox8();
-  const int rc = fp64_forces_ ? rumdx_pair_lj_fp64(p, nb, d_params_.get(), table_types_, box, cutoff_method_,
-                                                   initialize ? 1 : 0, calc_stresses ? 1 : 0, st)
-                              : rumdx_pair_lj(p, nb, d_params_.get(), table_types_, box, cutoff_method_,
-                                              initialize ? 1 : 0, calc_stresses ? 1 : 0, st);
+  const int form = GetPotentialForm();
+  int rc;
+  if (form != 0)
+    rc = rumdx_pair_form(p, nb, d_params_.get(), table_types_, box, form, cutoff_method_, initialize ? 1 : 0,
+                         calc_stresses ? 1 : 0, st);
+  else if (fp64_forces_)
+    rc = rumdx_pair_lj_fp64(p, nb, d_params_.get(), table_types_, box, cutoff_method_, initialize ? 1 : 0,
+                            calc_stresses ? 1 : 0, st);
+  else
+    rc = rumdx_pair_lj(p, nb, d_params_.get(), table_types_, box, cutoff_method_, initialize ? 1 : 0,
+                       calc_stresses ? 1 : 0, st);
   rumdx_check(rc, GetID_String().c_str(), method);
 }
 
 void PairPotential::SetFP64Forces(bool on) {
+  if (on && GetPotentialForm() != 0)
+    throw RUMD_Error(GetID_String(), "SetFP64Forces", "the all-FP64 comparison path serves the 12-6 potentials only");
   fp64_forces_ = on;
   if (sample) sample->InvalidateGraphs();
 }
@@ -519,4 +530,110 @@ void Pot_LJ_12_6::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, 
   // include/rumd/PairPotential.h:202-207
   Pot_IPL_12_6::SetParams(i, j, static_cast<float>(4. * Epsilon * std::pow(static_cast<double>(Sigma), 12.)),
                           static_cast<float>(-4. * Epsilon * std::pow(static_cast<double>(Sigma), 6.)), Rcut * Sigma);
+}
+
+void Pot_gLJ_m12_n6::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  if (verbose && sample && sample->GetVerbose())
+    printf("%s: Sigma = %g, Epsilon = %g, Rcut = %g.\n", GetID_String().c_str(), Sigma, Epsilon, Rcut);
+  // include/rumd/PairPotential.h:322-327
+  Pot_IPL_12_6::SetParams(i, j, static_cast<float>(Epsilon * std::pow(static_cast<double>(Sigma), 12.)),
+                          static_cast<float>(-2. * Epsilon * std::pow(static_cast<double>(Sigma), 6.)), Rcut * Sigma);
+}
+
+// ================================================================== other potential forms
+// u(rc) and s(rc) rc of a form (CopyParamsToGPU, src/PairPotential.cu:82-125),
+// formed in FP64 from the stored float parameters.
+void FormPairPotential::ComputeShifts(std::vector<float>& row) const {
+  double p[8];
+  for (int k = 0; k < 8; ++k) p[k] = row[k];
+  const double rc = p[0];
+  double u = 0.0, s = 0.0;
+  if (rc > 0.0) s = rumdx::pot_form<double>(form_, rc * rc, p, u);
+  row[kNumParam - 1] = static_cast<float>(u);
+  row[kNumParam - 2] = static_cast<float>(s * rc);
+}
+
+void FormPairPotential::Report(const char* what) const {
+  if (verbose && sample && sample->GetVerbose()) printf("%s: %s.\n", GetID_String().c_str(), what);
+}
+
+Pot_LJ_12_6_smooth_2_4::Pot_LJ_12_6_smooth_2_4(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_LJ_SMOOTH_2_4) {
+  SetID_String("potLJ_12_6_smooth_2_4");
+}
+
+// Rcut in units of Sigma; c2, c4 make the force and its derivative vanish
+// at Rcut (include/rumd/PairPotential.h:234-247).
+void Pot_LJ_12_6_smooth_2_4::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  const float c2 = static_cast<float>(48. * std::pow(static_cast<double>(Rcut), -14.) -
+                                      15. * std::pow(static_cast<double>(Rcut), -8.));
+  const float c4 = static_cast<float>(6. * std::pow(static_cast<double>(Rcut), -10.) -
+                                      21. * std::pow(static_cast<double>(Rcut), -16.));
+  SetParamsRow(i, j, {Rcut * Sigma, 1.0f / (Sigma * Sigma), 4.0f * Epsilon, c2, c4});
+  Report("smooth LJ parameters set");
+}
+
+static std::string glj_id(float m, float n) {
+  std::ostringstream s;
+  s << "potgLJ_" << m << "_" << n;
+  return s.str();
+}
+
+Pot_gLJ_m_n::Pot_gLJ_m_n(float m_, float n_, CutoffMethod cm) : FormPairPotential(cm, RUMDX_POT_GLJ_M_N), m(m_), n(n_) {
+  SetID_String(glj_id(m, n));
+}
+
+// include/rumd/PairPotential.h:280-292
+void Pot_gLJ_m_n::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, 1.f / (Sigma * Sigma), m * n / (m - n) * Epsilon, m * 0.5f, n * 0.5f,
+                      Epsilon / (m - n)});
+  Report("generalised LJ parameters set");
+}
+
+Pot_Gauss::Pot_Gauss(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_GAUSS) { SetID_String("potGau"); }
+
+// include/rumd/PairPotential.h:397-407
+void Pot_Gauss::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, 1.f / (Sigma * Sigma), Epsilon});
+  Report("Gaussian parameters set");
+}
+
+Pot_Buckingham::Pot_Buckingham(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_BUCKINGHAM) { SetID_String("potBuc"); }
+
+// Rcut is absolute here (include/rumd/PairPotential.h:436-449)
+void Pot_Buckingham::SetParams(unsigned i, unsigned j, float rm, float Alpha, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut, 1.f / (rm * rm), Alpha, 6.f * Epsilon / (Alpha - 6.f), Alpha * Epsilon / (Alpha - 6.f),
+                      6.f * Epsilon * Alpha / (Alpha - 6)});
+  Report("Buckingham parameters set");
+}
+
+static std::string ipl_id(float n) {
+  std::ostringstream s;
+  s << "potIPL_" << n;
+  return s.str();
+}
+
+Pot_IPL_n::Pot_IPL_n(float n_, CutoffMethod cm) : FormPairPotential(cm, RUMDX_POT_IPL_N), n(n_) {
+  SetID_String(ipl_id(n));
+}
+
+// include/rumd/PairPotential.h:614-625
+void Pot_IPL_n::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, 1.f / (Sigma * Sigma), Epsilon, n});
+  Report("IPL parameters set");
+}
+
+Pot_Yukawa::Pot_Yukawa(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_YUKAWA) { SetID_String("potYukawa"); }
+
+// include/rumd/PairPotential.h:814-826
+void Pot_Yukawa::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, Sigma, 1.f / Sigma, Epsilon});
+  Report("Yukawa parameters set");
+}
+
+Pot_Exp::Pot_Exp(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_EXP) { SetID_String("potExp"); }
+
+// include/rumd/PairPotential.h:1035-1044
+void Pot_Exp::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, 1.f / Sigma, Epsilon});
+  Report("exponential parameters set");
 }

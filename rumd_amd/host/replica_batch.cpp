@@ -63,6 +63,8 @@ void ReplicaBatch::SetMass(unsigned type, float mass) {
 
 void ReplicaBatch::SetPotential(const PairPotential& pot, float skin) {
   if (!(skin > 0.f)) throw RUMD_Error("ReplicaBatch", "SetPotential", "skin must be positive");
+  if (pot.GetPotentialForm() != 0)
+    throw RUMD_Error("ReplicaBatch", "SetPotential", "batched replicas run the 12-6 potentials only");
   Relayout();
   table_ = pot.GetParamsTable(ntypes_);
   cutoff_method_ = pot.GetCutoffMethod();

@@ -261,6 +261,9 @@ class PairPotential : public Potential {
   NeighborList& GetNeighborList() { return neighborList; }
   // Shift constants u(rc), s(rc) rc of one parameter row.
   virtual void ComputeShifts(std::vector<float>& row) const = 0;
+  // 0 for the 12-6 family (rumdx_pair_lj), else the RUMDX_POT_* form served
+  // by rumdx_pair_form.
+  virtual int GetPotentialForm() const { return 0; }
   unsigned NumTypesInTable() const { return table_types_; }
 
  protected:
@@ -288,6 +291,79 @@ class Pot_IPL_12_6 : public PairPotential {
 class Pot_LJ_12_6 : public Pot_IPL_12_6 {
  public:
   explicit Pot_LJ_12_6(CutoffMethod m) : Pot_IPL_12_6(m) { SetID_String("potLJ_12_6"); }
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+};
+
+// Pot_gLJ_m12_n6 (include/rumd/PairPotential.h:312-330): 12-6 LJ with its
+// minimum, -Epsilon, at Sigma.
+class Pot_gLJ_m12_n6 : public Pot_IPL_12_6 {
+ public:
+  explicit Pot_gLJ_m12_n6(CutoffMethod m) : Pot_IPL_12_6(m) { SetID_String("potLJA"); }
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+};
+
+// Pair potentials of the other forms (rumdx_pair_form, one form per class):
+// parameter rows as the reference's SetParams stores them, cut-off shifts from
+// the form at rc.
+class FormPairPotential : public PairPotential {
+ public:
+  FormPairPotential(CutoffMethod m, int form) : PairPotential(m), form_(form) {}
+  int GetPotentialForm() const override { return form_; }
+  void ComputeShifts(std::vector<float>& row) const override;
+
+ protected:
+  void Report(const char* what) const;
+  int form_;
+};
+
+// Pot_LJ_12_6_smooth_2_4 (include/rumd/PairPotential.h:218-256).
+class Pot_LJ_12_6_smooth_2_4 : public FormPairPotential {
+ public:
+  explicit Pot_LJ_12_6_smooth_2_4(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+};
+
+// Pot_gLJ_m_n (include/rumd/PairPotential.h:264-304).
+class Pot_gLJ_m_n : public FormPairPotential {
+ public:
+  Pot_gLJ_m_n(float m, float n, CutoffMethod cm);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+  float m, n;
+};
+
+// Pot_Gauss (include/rumd/PairPotential.h:383-414).
+class Pot_Gauss : public FormPairPotential {
+ public:
+  explicit Pot_Gauss(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+};
+
+// Pot_Buckingham (include/rumd/PairPotential.h:422-461).
+class Pot_Buckingham : public FormPairPotential {
+ public:
+  explicit Pot_Buckingham(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float rm, float Alpha, float Epsilon, float Rcut);
+};
+
+// Pot_IPL_n (include/rumd/PairPotential.h:600-636).
+class Pot_IPL_n : public FormPairPotential {
+ public:
+  Pot_IPL_n(float n, CutoffMethod cm);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+  float n;
+};
+
+// Pot_Yukawa (include/rumd/PairPotential.h:795-840).
+class Pot_Yukawa : public FormPairPotential {
+ public:
+  explicit Pot_Yukawa(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+};
+
+// Pot_Exp (include/rumd/PairPotential.h:1019-1052).
+class Pot_Exp : public FormPairPotential {
+ public:
+  explicit Pot_Exp(CutoffMethod m);
   void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
 };
 

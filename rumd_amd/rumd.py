@@ -23,6 +23,14 @@ Potential = _rumd.Potential
 PairPotential = _rumd.PairPotential
 Pot_IPL_12_6 = _rumd.Pot_IPL_12_6
 Pot_LJ_12_6 = _rumd.Pot_LJ_12_6
+Pot_gLJ_m12_n6 = _rumd.Pot_gLJ_m12_n6
+Pot_LJ_12_6_smooth_2_4 = _rumd.Pot_LJ_12_6_smooth_2_4
+Pot_gLJ_m_n = _rumd.Pot_gLJ_m_n
+Pot_Gauss = _rumd.Pot_Gauss
+Pot_Buckingham = _rumd.Pot_Buckingham
+Pot_IPL_n = _rumd.Pot_IPL_n
+Pot_Yukawa = _rumd.Pot_Yukawa
+Pot_Exp = _rumd.Pot_Exp
 Integrator = _rumd.Integrator
 IntegratorNVT = _rumd.IntegratorNVT
 IntegratorNVE = _rumd.IntegratorNVE
@@ -202,6 +210,7 @@ class Sample(_rumd.Sample):
 
 __all__ = [
     "Device", "SimulationBox", "RectangularSimulationBox", "Potential", "PairPotential", "Pot_IPL_12_6",
-    "Pot_LJ_12_6", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
+    "Pot_LJ_12_6", "Pot_gLJ_m12_n6", "Pot_LJ_12_6_smooth_2_4", "Pot_gLJ_m_n", "Pot_Gauss", "Pot_Buckingham",
+    "Pot_IPL_n", "Pot_Yukawa", "Pot_Exp", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
     "ShiftedForce", "GetVersion", "NumberOfPairPotentialParameters", "Sample", "ReplicaBatch",
 ]

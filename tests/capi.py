@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
     L.rumdx_nblist_build_sorted.argtypes = [PP, NP, f32a, vp]
     L.rumdx_pair_lj.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
     L.rumdx_pair_lj_fp64.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
+    L.rumdx_pair_form.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, i32, vp]
     L.rumdx_nvt_step.argtypes = [PP, NHP, f32, u32, f32, u32, f32a, NP, vp, vp]
     L.rumdx_zero_momentum.argtypes = [PP, vp, vp]
     L.rumdx_kinetic_energy.argtypes = [PP, NHP, f32, u32, vp, vp, vp]
@@ -308,10 +309,14 @@ class GpuSystem:
         else:
             check(L.rumdx_nblist_build_n2(C.byref(self.p), C.byref(self.nb), self.box8, None))
 
-    def calc_f(self, cutoff, initialize=True, stress=False, fp64=False):
+    def calc_f(self, cutoff, initialize=True, stress=False, fp64=False, form=0):
         if self.nb is not None:
             self.update_nblist()
         nbp = C.byref(self.nb) if self.nb is not None else None
+        if form:
+            check(lib().rumdx_pair_form(C.byref(self.p), nbp, self.d_params.value, self.nt, self.box8, form, cutoff,
+                                        int(initialize), int(stress), None))
+            return
         fn = lib().rumdx_pair_lj_fp64 if fp64 else lib().rumdx_pair_lj
         check(fn(C.byref(self.p), nbp, self.d_params.value, self.nt, self.box8, cutoff, int(initialize), int(stress),
                  None))
