@@ -237,6 +237,41 @@ int rumdx_pair_form(const rumdx_particles* p, const rumdx_nblist* nb, const floa
                     uint32_t ntypes, const float box[8], int form, int cutoff_method, int initialize,
                     int calc_stresses, void* stream);
 
+/* ---------------------------------------------------------------- intramolecular forces
+ * Topology lists are CSR over original particle ids (what MoleculeData holds,
+ * src/MoleculeData.cu:479-700): particle o's entries are [start[o], start[o+1]).
+ * Kernels reach particles through slot_of (original id -> current slot), which
+ * rumdx_slot_index derives from p->unsorted_index_dev; refresh it after any
+ * list build that sorts.  A thread owns one particle and gathers its share of
+ * every term it belongs to (no atomics, reproducible sums); initialize != 0
+ * stores the particle's bonded sums (zero for a particle with none),
+ * otherwise adds them. */
+
+/* slot_of[unsorted_index[s]] = s for s < n. */
+int rumdx_slot_index(const rumdx_particles* p, uint32_t* slot_of_dev, void* stream);
+
+/* BondHarmonic::CalcF (src/BondPotential.cu:70-135, kernel Harmonic :281-357):
+ * u = k/2 (r - l0)^2 per bond; each end gets its force, half the energy in
+ * f.w and the virial ft r^2 in w.w (and the stress in sts, w.y, w.z when
+ * calc_stresses).  bond_other/bond_type hold, for each entry, the other end's
+ * original id and the bond type; params_dev = (l0, k) per type. */
+int rumdx_bond_harmonic(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                        const uint32_t* bond_start_dev, const uint32_t* bond_other_dev,
+                        const uint32_t* bond_type_dev, const float* params_dev, uint32_t ntypes,
+                        const float box[8], int initialize, int calc_stresses, void* stream);
+
+/* AngleCosSq::CalcF (src/AnglePotential.cu:69-98, kernel CosSq :137-217):
+ * u = k/2 (cos theta - cos theta0)^2 per angle (a, b, c), the angle at b.
+ * angle_atoms_dev holds 4 u32 per entry: a, b, c (original ids) and the angle
+ * type, listed under each of its three atoms; params_dev = (theta0, k) per
+ * type.  The angle's energy goes to atom a's f.w and no virial or stress is
+ * added, as in the reference; with initialize, w (and sts when calc_stresses)
+ * are cleared for the potentials that add after it. */
+int rumdx_angle_cossq(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                      const uint32_t* angle_start_dev, const uint32_t* angle_atoms_dev,
+                      const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
+                      int calc_stresses, void* stream);
+
 /* ---------------------------------------------------------------- integrator
  * IntegratorNVT::Integrate (integrateNVTAlgorithm, src/IntegratorNVT.cu:64-79,
  * 364-418) followed, when thermostat_on_mask != 0, by UpdateThermostatState

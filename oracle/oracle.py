@@ -36,6 +36,8 @@ def lib() -> C.CDLL:
         L.orc_forces_brute.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p, f64p]
         L.orc_stress_brute.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p]
         L.orc_forces_form_brute.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, C.c_int, f64p, f64p]
+        L.orc_bonds_harmonic.argtypes = [C.c_int, f32p, f64p, C.c_int, i32p, f32p, f64p, f64p]
+        L.orc_angles_cossq.argtypes = [C.c_int, f32p, f64p, C.c_int, i32p, f32p, f64p]
         L.orc_forces_nl.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, i64p, i32p, f64p, f64p]
         L.orc_forces_cells.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p, f64p]
         L.orc_forces_cells.restype = C.c_int
@@ -94,6 +96,23 @@ def forces_form_brute(pos4, L, nt, params, form, cutoff):
     f4 = np.zeros((n, 4))
     vir = np.zeros(n)
     lib().orc_forces_form_brute(n, pos4, _box3(L), nt, _f32(params), form, cutoff, f4, vir)
+    return f4, vir
+
+
+def bonded(pos4, L, bonds=None, bond_params=None, angles=None, angle_params=None):
+    """FP64 intramolecular forces: bonds (m, 3) int (a, b, type) with
+    bond_params (l0, k) per type; angles (m, 4) int (a, b, c, type) with
+    angle_params (theta0, k) per type.  Returns (f4 [n,4], vir [n])."""
+    pos4 = _f32(pos4)
+    n = pos4.shape[0]
+    f4 = np.zeros((n, 4))
+    vir = np.zeros(n)
+    if bonds is not None and len(bonds):
+        b = np.ascontiguousarray(bonds, dtype=np.int32).reshape(-1, 3)
+        lib().orc_bonds_harmonic(n, pos4, _box3(L), b.shape[0], b, _f32(bond_params).reshape(-1), f4, vir)
+    if angles is not None and len(angles):
+        a = np.ascontiguousarray(angles, dtype=np.int32).reshape(-1, 4)
+        lib().orc_angles_cossq(n, pos4, _box3(L), a.shape[0], a, _f32(angle_params).reshape(-1), f4)
     return f4, vir
 
 

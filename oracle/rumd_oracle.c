@@ -30,6 +30,9 @@
  *     (gLJ 12-6 = LJA, IPL_n with n = 12 = IPL 12-6 with A6 = 0), the
  *     Buckingham minimum -epsilon at rm and the smooth LJ's vanishing force
  *     at Rcut.
+ *   orc_bonds_harmonic, orc_angles_cossq: closed forms for one bond and one
+ *     angle, -grad E = F by finite differences of every coordinate of a small
+ *     molecule, zero total force and torque.
  *   orc_nblist_*: brute force vs cell scan, symmetry, superset of the rc sphere.
  *   orc_nvt_step: closed form for one particle, Ps = 0 => NVE, time
  *     reversibility, the r(t+dt) = r + v(t+dt/2) dt relation held by the
@@ -292,6 +295,63 @@ void orc_forces_form_brute(int n, const float* pos4, const double L[3], int nt, 
     f4[4 * (size_t)i + 2] = fi[2];
     f4[4 * (size_t)i + 3] = 0.5 * ui;
     vir[i] = wi;
+  }
+}
+
+/* Harmonic bonds in FP64 (BondHarmonic, src/BondPotential.cu:281-357): for
+ * each bond (a, b, type) with params (l0, k), u = k/2 (r - l0)^2; adds the
+ * forces to f4, half of u to each end's f4[.,3] and F_a . (r_a - r_b) to each
+ * end's vir. */
+void orc_bonds_harmonic(int n, const float* pos4, const double L[3], int nbonds, const int32_t* bonds,
+                        const float* params, double* f4, double* vir) {
+  (void)n;
+  for (int q = 0; q < nbonds; ++q) {
+    const int a = bonds[3 * q], b = bonds[3 * q + 1], t = bonds[3 * q + 2];
+    const double l0 = params[2 * t], k = params[2 * t + 1];
+    double d[3];
+    const double r = sqrt(min_image_f64(pos4 + 4 * (size_t)a, pos4 + 4 * (size_t)b, L, d));
+    const double u = 0.5 * k * (r - l0) * (r - l0);
+    const double g = -k * (r - l0) / r; /* F_a = g d, d = r_a - r_b */
+    for (int c = 0; c < 3; ++c) {
+      f4[4 * (size_t)a + c] += g * d[c];
+      f4[4 * (size_t)b + c] -= g * d[c];
+    }
+    f4[4 * (size_t)a + 3] += 0.5 * u;
+    f4[4 * (size_t)b + 3] += 0.5 * u;
+    vir[a] += g * r * r;
+    vir[b] += g * r * r;
+  }
+}
+
+/* Bond angles cos^2 in FP64 (AngleCosSq, src/AnglePotential.cu:137-217), from
+ * the angle theta at b between e_a = (r_a - r_b)/|.| and e_c = (r_c - r_b)/|.|:
+ * u = k/2 (cos theta - cos theta0)^2, F_a = -k (cos theta - cos theta0)
+ * (e_c - cos theta e_a) / |r_a - r_b|, F_c likewise, F_b = -F_a - F_c.  The whole
+ * u goes to atom a's f4[.,3], as in the reference. */
+void orc_angles_cossq(int n, const float* pos4, const double L[3], int nangles, const int32_t* angles,
+                      const float* params, double* f4) {
+  (void)n;
+  for (int q = 0; q < nangles; ++q) {
+    const int a = angles[4 * q], b = angles[4 * q + 1], c = angles[4 * q + 2], t = angles[4 * q + 3];
+    const double th0 = params[2 * t], k = params[2 * t + 1];
+    double da[3], dc[3];
+    const double ra = sqrt(min_image_f64(pos4 + 4 * (size_t)a, pos4 + 4 * (size_t)b, L, da));
+    const double rc = sqrt(min_image_f64(pos4 + 4 * (size_t)c, pos4 + 4 * (size_t)b, L, dc));
+    double cs = 0.0;
+    for (int m = 0; m < 3; ++m) {
+      da[m] /= ra;
+      dc[m] /= rc;
+      cs += da[m] * dc[m];
+    }
+    const double dcos = cs - cos(th0);
+    for (int m = 0; m < 3; ++m) {
+      const double fa = -k * dcos * (dc[m] - cs * da[m]) / ra;
+      const double fc = -k * dcos * (da[m] - cs * dc[m]) / rc;
+      f4[4 * (size_t)a + m] += fa;
+      f4[4 * (size_t)c + m] += fc;
+      f4[4 * (size_t)b + m] -= fa + fc;
+    }
+    f4[4 * (size_t)a + 3] += 0.5 * k * dcos * dcos;
   }
 }
 

@@ -76,6 +76,11 @@ class Simulation:
         self.itg = itg
         self.sample.SetIntegrator(itg)
 
+    def ReadMoleculeData(self, topologyFile) -> None:
+        """Bonds and angles of the sample's molecules from a RUMD topology
+        file (Simulation.py:113-120)."""
+        self.sample.ReadMoleculeData(str(topologyFile))
+
     def SetPB_TP(self, pb: int, tp: int) -> None:
         self.sample.SetPB_TP(pb, tp)
 

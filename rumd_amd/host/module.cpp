@@ -88,7 +88,9 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetVerbose", &Potential::SetVerbose, py::arg("vb"))
       .def("SetID_String", &Potential::SetID_String, py::arg("set_ID_String"))
       .def("GetID_String", &Potential::GetID_String)
-      .def("ResetInternalData", &Potential::ResetInternalData);
+      .def("ResetInternalData", &Potential::ResetInternalData)
+      .def("GetPotentialEnergy", &Potential::GetPotentialEnergy)
+      .def("EnergyIncludedInParticleSum", &Potential::EnergyIncludedInParticleSum);
 
   py::class_<PairPotential, Potential> pp(m, "PairPotential");
   py::enum_<PairPotential::CutoffMethod>(pp, "CutoffMethod")
@@ -194,6 +196,25 @@ PYBIND11_MODULE(_rumd, m) {
       .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
       .def("SetParams", &Pot_Exp::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
            py::arg("Rcut"));
+  py::class_<BondHarmonic, Potential>(m, "BondHarmonic")
+      .def(py::init<>())
+      .def("SetParams", &BondHarmonic::SetParams, py::arg("bond_type"), py::arg("bond_length"), py::arg("stiffness"),
+           py::arg("exclude") = false)
+      .def("GetParams", &BondHarmonic::GetParams)
+      .def("GetExcludedTypes", &BondHarmonic::GetExcludedTypes);
+  py::class_<AngleCosSq, Potential>(m, "AngleCosSq")
+      .def(py::init<>())
+      .def("SetParams", &AngleCosSq::SetParams, py::arg("angle_type"), py::arg("theta0"), py::arg("ktheta"))
+      .def("GetParams", &AngleCosSq::GetParams);
+  py::class_<MoleculeData>(m, "MoleculeData")
+      .def("ReadTopology", &MoleculeData::ReadTopology, py::arg("filename"))
+      .def("SetBonds", &MoleculeData::SetBonds, py::arg("abt"), py::arg("molecule") = std::vector<uint32_t>{})
+      .def("SetAngles", &MoleculeData::SetAngles, py::arg("abct"))
+      .def("GetNumberOfBonds", &MoleculeData::GetNumberOfBonds)
+      .def("GetNumberOfAngles", &MoleculeData::GetNumberOfAngles)
+      .def("GetNumberOfMolecules", &MoleculeData::GetNumberOfMolecules)
+      .def("GetBonds", &MoleculeData::GetBonds)
+      .def("GetAngles", &MoleculeData::GetAngles);
   m.attr("NoShift") = PairPotential::NS;
   m.attr("ShiftedPotential") = PairPotential::SP;
   m.attr("ShiftedForce") = PairPotential::SF;
@@ -243,6 +264,9 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetIntegrator", &Sample::SetIntegrator, py::arg("newItg"), py::keep_alive<1, 2>())
       .def("GetIntegrator", &Sample::GetIntegrator, py::return_value_policy::reference_internal)
       .def("GetPotentials", [](Sample& s) { return s.GetPotentials(); }, py::return_value_policy::reference_internal)
+      .def("ReadMoleculeData", &Sample::ReadMoleculeData, py::arg("topology_file"))
+      .def("GetMoleculeData", &Sample::GetMoleculeData, py::return_value_policy::reference_internal)
+      .def("CreateMoleculeData", &Sample::CreateMoleculeData, py::return_value_policy::reference_internal)
       .def("SetPB_TP", &Sample::SetPB_TP, py::arg("pb"), py::arg("tp"))
       .def("GetParticlesPerBlock", &Sample::GetParticlesPerBlock)
       .def("GetThreadsPerParticle", &Sample::GetThreadsPerParticle)

@@ -367,6 +367,86 @@ class Pot_Exp : public FormPairPotential {
   void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
 };
 
+// ------------------------------------------------------------------ molecules
+// MoleculeData (include/rumd/MoleculeData.h, src/MoleculeData.cu:479-700): the
+// bond and angle topology, held on the device as per-particle lists (CSR over
+// original particle ids) for the gather kernels of rumdx_bond_harmonic and
+// rumdx_angle_cossq, and the slot index they read particles through.
+class MoleculeData {
+ public:
+  explicit MoleculeData(Sample* s) : sample_(s) {}
+  // Reads the [ bonds ] (mol a b type) and [ angles ] (mol a b c type)
+  // sections of a RUMD topology file; other sections are ignored.
+  void ReadTopology(const std::string& filename);
+  // Replaces the bonds: flat (a, b, type) triples of original particle ids.
+  void SetBonds(const std::vector<uint32_t>& abt, const std::vector<uint32_t>& molecule = {});
+  // Replaces the angles: flat (a, b, c, type) quadruples.
+  void SetAngles(const std::vector<uint32_t>& abct);
+  unsigned GetNumberOfBonds() const { return static_cast<unsigned>(bonds_.size() / 3); }
+  unsigned GetNumberOfAngles() const { return static_cast<unsigned>(angles_.size() / 4); }
+  unsigned GetNumberOfMolecules() const { return num_mol_; }
+  unsigned GetNumberOfBondTypes() const;
+  unsigned GetNumberOfAngleTypes() const;
+  const std::vector<uint32_t>& GetBonds() const { return bonds_; }
+  const std::vector<uint32_t>& GetAngles() const { return angles_; }
+  // Refreshes slot_of from the particles' unsorted index (on the sample's stream).
+  const uint32_t* SlotIndex();
+  const uint32_t* BondStart() const { return bond_start_.get(); }
+  const uint32_t* BondOther() const { return bond_other_.get(); }
+  const uint32_t* BondType() const { return bond_type_.get(); }
+  const uint32_t* AngleStart() const { return angle_start_.get(); }
+  const uint32_t* AngleAtoms() const { return angle_atoms_.get(); }
+
+ private:
+  void UploadBonds();
+  void UploadAngles();
+  Sample* sample_;
+  unsigned num_mol_ = 0;
+  std::vector<uint32_t> bonds_, angles_;
+  DeviceArray<uint32_t> bond_start_, bond_other_, bond_type_, angle_start_, angle_atoms_, slot_of_;
+};
+
+// BondHarmonic (include/rumd/BondPotential.h, src/BondPotential.cu:70-135):
+// u = k/2 (r - l0)^2 per bond of the sample's topology.
+class BondHarmonic : public Potential {
+ public:
+  BondHarmonic() { SetID_String("bondHarmonic"); }
+  void SetParams(unsigned bond_type, float bond_length, float stiffness, bool exclude);
+  void CalcF(bool initialize, bool calc_stresses) override;
+  bool EnergyIncludedInParticleSum() const override { return true; }
+  void Initialize() override {}
+  double GetPotentialEnergy() override;
+  std::vector<float> GetParams() const { return params_; }
+  // Bond types whose pairs the non-bonded potentials skip.
+  std::vector<unsigned> GetExcludedTypes() const;
+
+ private:
+  void Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method);
+  std::vector<float> params_;  // (l0, k) per bond type
+  std::vector<bool> exclude_;
+  DeviceArray<float> d_params_;
+  DeviceArray<float> pe_f_, pe_w_;
+};
+
+// AngleCosSq (include/rumd/AnglePotential.h, src/AnglePotential.cu:69-98):
+// u = k/2 (cos theta - cos theta0)^2 per angle.
+class AngleCosSq : public Potential {
+ public:
+  AngleCosSq() { SetID_String("angleCosSq"); }
+  void SetParams(unsigned angle_type, float theta0, float ktheta);
+  void CalcF(bool initialize, bool calc_stresses) override;
+  bool EnergyIncludedInParticleSum() const override { return true; }
+  void Initialize() override {}
+  double GetPotentialEnergy() override;
+  std::vector<float> GetParams() const { return params_; }
+
+ private:
+  void Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method);
+  std::vector<float> params_;  // (theta0, k) per angle type
+  DeviceArray<float> d_params_;
+  DeviceArray<float> pe_f_, pe_w_;
+};
+
 // ------------------------------------------------------------------ integrators
 // Integrator (include/rumd/Integrator.h:18-70).
 class Integrator {
@@ -512,6 +592,10 @@ class Sample {
   void* Workspace() { return workspace_.get(); }
   double* ScalarOut() { return scalars_.get(); }
   const std::vector<Potential*>& GetPotentials() const { return potentials_; }
+  // Topology of the sample's molecules (Sample::ReadMoleculeData); null until read.
+  void ReadMoleculeData(const std::string& topology_file);
+  MoleculeData* GetMoleculeData();
+  MoleculeData* CreateMoleculeData();
   NeighborList* FusedCheckList();
   void Synchronize();
 
@@ -531,4 +615,5 @@ class Sample {
   DeviceArray<double> scalars_;
   // captured step blocks: key = (block size, momentum reset at first step)
   std::map<std::pair<unsigned, bool>, hipGraphExec_t> graphs_;
+  std::unique_ptr<MoleculeData> molecules_;
 };

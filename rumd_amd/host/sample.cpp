@@ -88,8 +88,14 @@ void Sample::SetPotential(Potential* p) {
 
 void Sample::AddPotential(Potential* p) {
   p->SetVerbose(verbose_);
-  // non-constraint potentials go to the front (src/Sample.cu:268-274)
-  potentials_.insert(potentials_.begin(), p);
+  // non-constraint potentials go to the front (src/Sample.cu:268-274), pair
+  // potentials ahead of the others: the first force call of a step then is a
+  // pair potential's, whose list build may sort the particles, before any
+  // other term (bonds, angles) has been written in the old order
+  auto pos = potentials_.begin();
+  if (!dynamic_cast<PairPotential*>(p))
+    while (pos != potentials_.end() && dynamic_cast<PairPotential*>(*pos)) ++pos;
+  potentials_.insert(pos, p);
   if (GetNumberOfParticles()) p->SetSample(this);
   InvalidateGraphs();
 }
@@ -265,4 +271,18 @@ void Sample::RunSteps(unsigned long first_step, unsigned nsteps, unsigned block,
     i += block;
   }
   Synchronize();
+}
+
+// Sample::ReadMoleculeData (src/Sample.cu, MoleculeData::ReadTopology).
+void Sample::ReadMoleculeData(const std::string& topology_file) {
+  CreateMoleculeData()->ReadTopology(topology_file);
+  InvalidateGraphs();
+}
+
+MoleculeData* Sample::GetMoleculeData() { return molecules_.get(); }
+
+MoleculeData* Sample::CreateMoleculeData() {
+  if (!GetNumberOfParticles()) throw RUMD_Error("Sample", "ReadMoleculeData", "Sample has no particles");
+  molecules_ = std::make_unique<MoleculeData>(this);
+  return molecules_.get();
 }

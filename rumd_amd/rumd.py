@@ -31,6 +31,9 @@ Pot_Buckingham = _rumd.Pot_Buckingham
 Pot_IPL_n = _rumd.Pot_IPL_n
 Pot_Yukawa = _rumd.Pot_Yukawa
 Pot_Exp = _rumd.Pot_Exp
+BondHarmonic = _rumd.BondHarmonic
+AngleCosSq = _rumd.AngleCosSq
+MoleculeData = _rumd.MoleculeData
 Integrator = _rumd.Integrator
 IntegratorNVT = _rumd.IntegratorNVT
 IntegratorNVE = _rumd.IntegratorNVE
@@ -211,6 +214,6 @@ class Sample(_rumd.Sample):
 __all__ = [
     "Device", "SimulationBox", "RectangularSimulationBox", "Potential", "PairPotential", "Pot_IPL_12_6",
     "Pot_LJ_12_6", "Pot_gLJ_m12_n6", "Pot_LJ_12_6_smooth_2_4", "Pot_gLJ_m_n", "Pot_Gauss", "Pot_Buckingham",
-    "Pot_IPL_n", "Pot_Yukawa", "Pot_Exp", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
+    "Pot_IPL_n", "Pot_Yukawa", "Pot_Exp", "BondHarmonic", "AngleCosSq", "MoleculeData", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
     "ShiftedForce", "GetVersion", "NumberOfPairPotentialParameters", "Sample", "ReplicaBatch",
 ]

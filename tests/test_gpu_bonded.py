@@ -1,0 +1,137 @@
+"""Intramolecular forces on the GPU (rumdx_bond_harmonic, rumdx_angle_cossq
+through BondHarmonic / AngleCosSq and MoleculeData) together with a pair
+potential, against the FP64 oracle, on RUMD's pentamer melt fixture: every
+list method (the sorted one permutes particles, so the bonded kernels read
+them through the slot index), energies per potential, the virial, graph
+capture, and energy conservation."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from capi import gpu_available
+from helpers import FORCE_TOL_GUARDED, force_errors, load_golden
+from rumd_amd import configs, rumd
+
+pytestmark = pytest.mark.gpu
+
+SP = 1
+BOND_PARAMS = [(1.0, 100.0), (1.1, 150.0)]
+ANGLE_PARAMS = [(2.0, 20.0), (1.8, 30.0)]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not gpu_available():
+        pytest.fail("no GPU visible: the gpu tests must run on an MI355X")
+
+
+def _topology(golden_dir):
+    bonds = []
+    for ln in open(golden_dir / "pentamer200_rho0.85_T0.5.top"):
+        parts = ln.split()
+        if ln.startswith("["):
+            section = ln.strip()[1:-1].strip()
+            continue
+        if section == "bonds" and len(parts) == 4 and parts[0].isdigit():
+            bonds.append([int(parts[1]), int(parts[2]), int(parts[3])])
+    bonds = np.array(bonds, np.int32)
+    # angles (i, i+1, i+2) inside each five-bead chain, type by chain parity
+    angles = np.array([[5 * m + i, 5 * m + i + 1, 5 * m + i + 2, m % 2] for m in range(200) for i in range(3)], np.int32)
+    return bonds, angles
+
+
+def _sample(golden_dir, method, with_angles=True):
+    c = load_golden(golden_dir / "pentamer200_rho0.85_T0.5.xyz.gz")
+    s = rumd.Sample(0, 0)
+    s.SetVerbose(False)
+    s.SetConfiguration(c)
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedPotential)
+    pot.SetVerbose(False)
+    pot.SetParams(0, 0, Sigma=1.0, Epsilon=1.0, Rcut=2.5)
+    pot.SetNbListSkin(0.3)
+    pot.SetNB_Method(method)
+    s.SetPotential(pot)
+    bonds, angles = _topology(golden_dir)
+    s.ReadMoleculeData(str(golden_dir / "pentamer200_rho0.85_T0.5.top"))
+    md = s.GetMoleculeData()
+    assert md.GetNumberOfBonds() == len(bonds) and md.GetNumberOfMolecules() == 200
+    bond = rumd.BondHarmonic()
+    bond.SetVerbose(False)
+    for t, (l0, k) in enumerate(BOND_PARAMS):
+        bond.SetParams(bond_type=t, bond_length=l0, stiffness=k)
+    s.AddPotential(bond)
+    ang = None
+    if with_angles:
+        md.SetAngles(angles.reshape(-1).tolist())
+        ang = rumd.AngleCosSq()
+        ang.SetVerbose(False)
+        for t, (th0, k) in enumerate(ANGLE_PARAMS):
+            ang.SetParams(angle_type=t, theta0=th0, ktheta=k)
+        s.AddPotential(ang)
+    return c, s, pot, bond, ang, bonds, angles
+
+
+@pytest.mark.parametrize("method", ["n2", "sort", "none"])
+def test_bonded_with_pair_forces_match_oracle(golden_dir, method):
+    c, s, pot, bond, ang, bonds, angles = _sample(golden_dir, method)
+    print(f"[case] pentamer200 LJ + bonds + angles, list {method}", flush=True)
+    s.CalcF()
+    f = s.GetForcesAndEnergies().astype(np.float64)
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    table = configs.lj_param_table([(0, 0, 1.0, 1.0, 2.5)], 1)
+    f_lj, vir_lj = oracle.forces_brute(pos4, c.box, 1, table, SP)
+    f_b, vir_b = oracle.bonded(pos4, c.box, bonds, BOND_PARAMS, None, None)
+    f_a, _ = oracle.bonded(pos4, c.box, None, None, angles, ANGLE_PARAMS)
+    f_ref = f_lj + f_b + f_a
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+    np.testing.assert_allclose(f[:, 3], f_ref[:, 3], rtol=0, atol=5e-5)
+    # each potential's own energy, and the total virial (angles add none)
+    assert bond.GetPotentialEnergy() == pytest.approx(f_b[:, 3].sum(), rel=2e-6)
+    assert ang.GetPotentialEnergy() == pytest.approx(f_a[:, 3].sum(), rel=2e-6)
+    assert s.GetPotentialEnergy() == pytest.approx(f_ref[:, 3].sum(), rel=2e-6)
+    assert s.GetVirial() == pytest.approx((vir_lj.sum() + vir_b.sum()) / 6.0, rel=1e-5)
+
+
+def test_bonded_stress_and_initialize_order(golden_dir):
+    """The pair potential runs first (its first list build sorts the
+    particles), then angles and bonds add: the per-particle stress trace is
+    minus the virial with the bonds included."""
+    c, s, pot, bond, ang, bonds, angles = _sample(golden_dir, "sort")
+    print("[case] pentamer200 stresses", flush=True)
+    s.CalcF(True)
+    w = s.GetVirials().astype(np.float64)
+    sts = s.GetStresses().astype(np.float64)
+    np.testing.assert_allclose(sts[:, 0] + sts[:, 1] + sts[:, 2], -w[:, 3], rtol=0, atol=2e-3)
+
+
+def test_bonded_graph_blocks_equal_eager_and_conserve_energy(golden_dir):
+    states = []
+    for graph in (True, False):
+        c, s, pot, bond, ang, bonds, angles = _sample(golden_dir, "sort")
+        itg = rumd.IntegratorNVE(timeStep=0.002)
+        s.SetIntegrator(itg)
+        print(f"[case] pentamer200 NVE graph={graph}", flush=True)
+        if graph:
+            s.RunSteps(0, 200, 10, 0)
+        else:
+            for _ in range(200):
+                s.CalcF()
+                itg.Integrate()
+        s.Synchronize()
+        states.append((s.GetPositions(), s.GetVelocities()))
+    np.testing.assert_array_equal(states[0][0], states[1][0])
+    np.testing.assert_array_equal(states[0][1], states[1][1])
+    # NVE drift over 2000 further steps
+    c, s, pot, bond, ang, bonds, angles = _sample(golden_dir, "sort")
+    itg = rumd.IntegratorNVE(timeStep=0.002)
+    s.SetIntegrator(itg)
+    s.CalcF()
+    e0 = s.GetPotentialEnergy() + itg.GetKineticEnergy()
+    s.RunSteps(0, 2000, 10, 0)
+    s.CalcF()
+    e1 = s.GetPotentialEnergy() + itg.GetKineticEnergy()
+    print(f"  drift per particle {(e1 - e0) / c.n:.2e}", flush=True)
+    assert abs(e1 - e0) / c.n < 2e-3

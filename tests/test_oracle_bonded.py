@@ -1,0 +1,83 @@
+"""Pins the oracle's intramolecular terms (orc_bonds_harmonic,
+orc_angles_cossq): closed forms for one bond and one angle, -grad E = F by
+finite differences over every coordinate of a small molecule, and zero total
+force and torque.  CPU only."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from rumd_amd import configs
+
+L = np.array([50.0, 50.0, 50.0])
+
+
+def _pos4(x):
+    return configs.pack_pos4(np.asarray(x, np.float32), np.zeros(len(x), np.int32))
+
+
+def test_one_bond_closed_form():
+    k, l0 = 150.0, float(np.float32(1.1))  # parameters are stored as float
+    for r in (0.9, 1.1, 1.3):
+        p = _pos4([[0.0, 0.0, 0.0], [0.0, r, 0.0]])
+        r32 = float(np.float32(r))
+        f4, vir = oracle.bonded(p, L, bonds=[[0, 1, 0]], bond_params=[[l0, k]])
+        assert f4[:, 3].sum() == pytest.approx(0.5 * k * (r32 - l0) ** 2, rel=1e-12, abs=1e-15)
+        assert f4[0, 3] == pytest.approx(f4[1, 3])
+        # stretched bond pulls particle 1 back towards 0 (-y), compressed pushes it out
+        assert f4[1, 1] == pytest.approx(-k * (r32 - l0), rel=1e-12, abs=1e-12)
+        assert f4[0, 1] == pytest.approx(k * (r32 - l0), rel=1e-12, abs=1e-12)
+        assert vir[0] == pytest.approx(-k * (r32 - l0) * r32, rel=1e-12, abs=1e-12)
+
+
+def test_one_angle_closed_form():
+    k, th0 = 40.0, float(np.float32(1.9))
+    p = _pos4([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 2.0, 0.0]])  # a right angle at b
+    f4, vir = oracle.bonded(p, L, angles=[[0, 1, 2, 0]], angle_params=[[th0, k]])
+    assert f4[0, 3] == pytest.approx(0.5 * k * np.cos(th0) ** 2, rel=1e-12)
+    assert f4[1, 3] == 0.0 and f4[2, 3] == 0.0 and not vir.any()
+    # theta < theta0 (cos th0 < 0): the arms are pushed apart, perpendicular to each arm
+    fa = -k * (0.0 - np.cos(th0)) * np.array([0.0, 1.0, 0.0]) / 1.0
+    np.testing.assert_allclose(f4[0, :3], fa, rtol=1e-12, atol=1e-12)
+    assert f4[0, 1] < 0 and f4[2, 0] < 0
+    # at theta0 there is neither energy nor force
+    p0 = _pos4([[np.cos(0.0), 0.0, 0.0], [0.0, 0.0, 0.0], [np.cos(th0), np.sin(th0), 0.0]])
+    f4, _ = oracle.bonded(p0, L, angles=[[0, 1, 2, 0]], angle_params=[[th0, k]])
+    assert abs(f4[:, 3].sum()) < 1e-9 and np.abs(f4[:, :3]).max() < 1e-5
+
+
+def _molecule(seed=3):
+    rng = np.random.default_rng(seed)
+    x = np.cumsum(rng.normal(0.0, 0.6, size=(6, 3)) + np.array([0.9, 0.0, 0.0]), axis=0)
+    bonds = [[i, i + 1, i % 2] for i in range(5)]
+    angles = [[i, i + 1, i + 2, i % 2] for i in range(4)] + [[5, 2, 0, 1]]
+    return x.astype(np.float32), bonds, angles, [[1.0, 120.0], [1.2, 80.0]], [[1.9, 30.0], [2.3, 45.0]]
+
+
+def test_bonded_forces_are_minus_the_energy_gradient():
+    x, bonds, angles, bp, ap = _molecule()
+
+    def energy(xx):
+        f4, _ = oracle.bonded(_pos4(xx), L, bonds, bp, angles, ap)
+        return f4[:, 3].sum()
+
+    f4, _ = oracle.bonded(_pos4(x), L, bonds, bp, angles, ap)
+    h = 1e-3
+    for i in range(len(x)):
+        for c in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, c] += h
+            xm[i, c] -= h
+            dx = float(xp[i, c]) - float(xm[i, c])
+            g = (energy(xp) - energy(xm)) / dx
+            assert f4[i, c] == pytest.approx(-g, rel=2e-4, abs=2e-4), (i, c)
+
+
+def test_bonded_total_force_and_torque_vanish():
+    x, bonds, angles, bp, ap = _molecule(seed=11)
+    f4, _ = oracle.bonded(_pos4(x), L, bonds, bp, angles, ap)
+    scale = np.abs(f4[:, :3]).max()
+    np.testing.assert_allclose(f4[:, :3].sum(axis=0), 0.0, atol=1e-12 * scale)
+    torque = np.cross(x.astype(np.float64), f4[:, :3]).sum(axis=0)
+    np.testing.assert_allclose(torque, 0.0, atol=1e-10 * scale)
