@@ -66,6 +66,7 @@ extern "C" {
 #define RUMDX_FLAG_DONE 4      /* internal completion counter, zero between calls */
 #define RUMDX_FLAG_MAXHALO 5   /* largest tile halo (particles) of the last sorted build */
 #define RUMDX_FLAG_ERROR 6     /* nonzero: a tile halo exceeded 65535 particles (list unusable) */
+#define RUMDX_FLAG_EXCL_SEEN 7 /* internal: value of BUILDS when exclusions were last applied */
 
 /* Device-resident particle arrays (ParticleData, include/rumd/ParticleData.h:41-62). */
 typedef struct rumdx_particles {
@@ -131,6 +132,15 @@ typedef struct rumdx_nblist {
   uint32_t* tile_info_dev;  /* sorted method: ntiles x 4, written by the build */
   uint32_t lanes_per_particle; /* 1, 2, 4 or 8 lanes share a particle's list in the force
                                   kernels (RUMD's threads per particle, tp); 0 = 1 */
+  /* Exclusions (NeighborList::SetExclusion, src/NeighborList.cu:791-933): the
+   * particles each particle does not interact with through this list, CSR
+   * over original ids (p->unsorted_index_dev maps slots to them); NULL = none.
+   * The N2 build skips them, the sorted build removes them from its lists
+   * after a rebuild (apply_exclusion_list, src/NeighborList.cu:1323-1432), and
+   * the force kernels' cell rescans skip them.  NB method "none" is not
+   * available with exclusions (the reference switches to N2). */
+  const uint32_t* excl_start_dev; /* n + 1 */
+  const uint32_t* excl_ids_dev;
 } rumdx_nblist;
 
 /* Cells per tile edge and the largest LDS halo a launch may request. */

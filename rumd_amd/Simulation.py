@@ -33,7 +33,8 @@ class Simulation:
         else:
             self.sample.ReadConf(filename_or_conf)
         self.itg = None
-        self.potentialList = []
+        self.potentialList = []  # non-molecular potentials
+        self.moleculePotentials = {}
         self.verbose = verbose
         self.graph_block = 10
         self.blockSize = None
@@ -65,12 +66,36 @@ class Simulation:
         self.sample.SetVerbose(vb)
 
     def SetPotential(self, potential) -> None:
+        """Replaces every potential (Simulation.py:160-176); molecular
+        potentials must be added with AddPotential."""
+        if potential.IsMolecularPotential():
+            raise RuntimeError("Must use AddPotential for molecular potentials.")
         self.potentialList = [potential]
+        self.moleculePotentials = {}
         self.sample.SetPotential(potential)
 
-    def AddPotential(self, potential) -> None:
-        self.potentialList.append(potential)
-        self.sample.AddPotential(potential)
+    def AddPotential(self, potential, clear: bool = False) -> None:
+        """Adds a potential (Simulation.py:178-206): a molecular potential
+        sets its exclusions on every non-bonded potential, a non-bonded one
+        receives the exclusions of the molecular potentials present."""
+        if potential.GetID_String() in self.moleculePotentials:
+            raise RuntimeError("Simulation may only have one potential of type %s" % potential.GetID_String())
+        if clear:
+            self.sample.SetPotential(potential)
+            self.moleculePotentials = {}
+            self.potentialList = [potential]
+        else:
+            self.sample.AddPotential(potential)
+        if not potential.IsMolecularPotential():
+            if not clear:
+                self.potentialList.append(potential)
+            for pot_molecular in self.moleculePotentials.values():
+                pot_molecular.SetExclusions(potential)
+        else:
+            self.moleculePotentials[potential.GetID_String()] = potential
+            for pot_non_bond in self.potentialList:
+                if not pot_non_bond.IsMolecularPotential() and hasattr(pot_non_bond, "SetExclusion"):
+                    potential.SetExclusions(pot_non_bond)
 
     def SetIntegrator(self, itg) -> None:
         self.itg = itg

@@ -272,6 +272,8 @@ extern "C" int rumdx_pair_lj_fp64(const rumdx_particles* p, const rumdx_nblist* 
     return fail(RUMDX_E_INVALID, "rumdx_pair_lj_fp64: particle arrays not allocated");
   if (calc_stresses && p->sts_dev == nullptr)
     return fail(RUMDX_E_INVALID, "rumdx_pair_lj_fp64: stresses requested but sts is null");
+  if (nb != nullptr && nb->excl_start_dev != nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj_fp64: the FP64 comparison path takes lists without exclusions");
   if (ntypes == 0 || ntypes > RUMDX_MAX_TYPES) return fail(RUMDX_E_INVALID, "rumdx_pair_lj_fp64: ntypes out of range");
   if (p->n == 0) return RUMDX_OK;
   if (nb != nullptr && ((nb->nbl_dev == nullptr && nb->nbl16_dev == nullptr) || nb->num_nbrs_dev == nullptr ||

@@ -62,6 +62,7 @@ struct CellFallback {
   const uint32_t* cell_start;
   const uint32_t* slot_cell;
   int nc0, nc1, nc2;
+  Exclusions ex;  // the list's exclusions, which a rescan must skip
 };
 
 // Pair term with the minimum-image displacement formed in FP64 and rounded
@@ -95,6 +96,7 @@ template <bool STR, class PP>
 __device__ __forceinline__ void rescan_cells(const PP& pp, uint32_t i, double xi, double yi, double zi,
                                              const float4* __restrict__ r, const float4* prow, const Box& bx,
                                              const CellFallback& fb, Acc& a) {
+  const ExclusionRange er = fb.ex.of(i);
   const uint32_t lin = fb.slot_cell[i];
   const int cx = static_cast<int>(lin % fb.nc0);
   const int cy = static_cast<int>((lin / fb.nc0) % fb.nc1);
@@ -108,7 +110,7 @@ __device__ __forceinline__ void rescan_cells(const PP& pp, uint32_t i, double xi
         const uint32_t rank = fb.cell_rank[ox + oy * fb.nc0 + oz * fb.nc0 * fb.nc1];
         const uint32_t s = fb.cell_start[rank], e = fb.cell_start[rank + 1];
         for (uint32_t j = s; j < e; ++j)
-          if (j != i) pair_term<STR>(pp, xi, yi, zi, r[j], prow, bx, a);
+          if (j != i && !fb.ex.excludes(er, j)) pair_term<STR>(pp, xi, yi, zi, r[j], prow, bx, a);
       }
     }
   }
@@ -218,8 +220,9 @@ __global__ __launch_bounds__(kBlock) void k_pair_nbl(PP pp, const float4* __rest
     } else if (part == 0 && fb.cell_rank != nullptr) {
       rescan_cells<STR>(pp, i, xi, yi, zi, r, prow, bx, fb, a);
     } else if (part == 0) {
+      const ExclusionRange er = fb.ex.of(i);
       for (uint32_t j = 0; j < n; ++j)
-        if (j != i) pair_term<STR>(pp, xi, yi, zi, r[j], prow, bx, a);
+        if (j != i && !fb.ex.excludes(er, j)) pair_term<STR>(pp, xi, yi, zi, r[j], prow, bx, a);
     }
   }
   combine_lanes<STR>(a, lgp);
@@ -415,13 +418,15 @@ int launch_pair(const PP& pp, const char* entry, const rumdx_particles* p, const
   const uint32_t lpp = nb != nullptr ? nb->lanes_per_particle : 1u;
   const uint32_t lgp = lpp >= 8u ? 3u : lpp >= 4u ? 2u : lpp >= 2u ? 1u : 0u;
   if (nb != nullptr && nb->nbl16_dev != nullptr) {
-    const CellFallback fb{nb->cell_rank_dev, nb->cell_start_dev, nb->slot_cell_dev, nb->nc[0], nb->nc[1], nb->nc[2]};
+    const CellFallback fb{nb->cell_rank_dev, nb->cell_start_dev, nb->slot_cell_dev, nb->nc[0], nb->nc[1], nb->nc[2],
+                          Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev}};
     hipLaunchKernelGGL((k_pair_tiles<PP, STR, INIT>), dim3(nb->ntiles), dim3(kBlock),
                        tab_bytes + sizeof(float4) * nb->halo_cap, st, pp, r, f, w, sts, nb->nbl16_dev, nb->num_nbrs_dev,
                        nb->lane_order_dev, params, nt, bx, reinterpret_cast<const uint4*>(nb->tile_info_dev),
                        nb->tile_halo_dev, nb->halo_cap, fb, p->nvp, nb->max_nbrs, lgp);
   } else if (nb != nullptr) {
     CellFallback fb{};
+    fb.ex = Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev};
     if (nb->cell_rank_dev != nullptr) {
       fb.cell_rank = nb->cell_rank_dev;
       fb.cell_start = nb->cell_start_dev;
@@ -469,6 +474,8 @@ inline int check_pair_args(const char* entry, const rumdx_particles* p, const ru
        nb->tile_info_dev == nullptr || nb->cell_rank_dev == nullptr || nb->cell_start_dev == nullptr ||
        nb->halo_cap > RUMDX_HALO_CAP_MAX))
     return fail(RUMDX_E_INVALID, e + ": tile arrays not allocated");
+  if (nb != nullptr && nb->excl_start_dev != nullptr && (nb->excl_ids_dev == nullptr || p->unsorted_index_dev == nullptr))
+    return fail(RUMDX_E_INVALID, e + ": exclusions need the exclusion ids and the particles' unsorted index");
   if (nb != nullptr) {
     const uint32_t lpp = nb->lanes_per_particle;
     if (lpp > 8u || (lpp & (lpp - 1u)) != 0u || static_cast<uint64_t>(p->n) * (lpp ? lpp : 1u) >= (1ull << 32))

@@ -111,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_build_n2(const float4* __restrict__ 
                                                      uint32_t* __restrict__ nbl, uint32_t* __restrict__ num_nbrs,
                                                      uint32_t* __restrict__ flags, const float* __restrict__ cut_skin2,
                                                      uint32_t nt, Box bx, uint32_t n, uint32_t nvp,
-                                                     uint32_t max_nbrs) {
+                                                     uint32_t max_nbrs, Exclusions ex) {
   if (!rebuild_pending(flags)) return;
   extern __shared__ float4 s_dyn[];
   float4* s_r = s_dyn;
@@ -121,6 +121,7 @@ __global__ __launch_bounds__(kBlock) void k_build_n2(const float4* __restrict__ 
   const bool live = i < n;
   const float4 ri = live ? r[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   const uint32_t ti = __float_as_uint(ri.w);
+  const ExclusionRange er = live ? ex.of(i) : ExclusionRange{};
   uint32_t cnt = 0;
   for (uint32_t base = 0; base < n; base += kBlock) {
     __syncthreads();
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_build_n2(const float4* __restrict__ 
         if (j == i) continue;
         const float4 rj = s_r[k];
         const float d2 = min_image_d2_f32(ri, rj, bx);
-        if (d2 < s_cut[ti * nt + __float_as_uint(rj.w)]) {
+        if (d2 < s_cut[ti * nt + __float_as_uint(rj.w)] && !ex.excludes(er, j)) {
           if (cnt < max_nbrs) nbl[static_cast<size_t>(cnt) * nvp + i] = j;
           ++cnt;  // keeps counting past capacity: the true length sizes the regrowth
         }
@@ -765,6 +766,47 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
   finish_build(flags);
 }
 
+// Exclusions on the lists of the sorted method (apply_exclusion_list,
+// src/NeighborList.cu:1323-1432), after a rebuild only: each list loses the
+// entries of the particles its owner must not interact with, in place and in
+// order.  Entries are decoded through the tile's halo slot list, so tiles whose
+// halo was not staged (their force kernel rescans cells and skips exclusions
+// there) and overflowed lists are left alone.  Every block arrives at the
+// completion counter; the last one records the build as seen.
+__global__ __launch_bounds__(kBlock) void k_apply_exclusions(uint32_t* __restrict__ flags,
+                                                             const uint4* __restrict__ tile_info,
+                                                             const uint32_t* __restrict__ tile_halo, uint32_t halo_cap,
+                                                             Exclusions ex, uint16_t* __restrict__ nbl16,
+                                                             uint32_t* __restrict__ num_nbrs, uint32_t nvp,
+                                                             uint32_t max_nbrs) {
+  const uint32_t builds = __hip_atomic_load(flags + RUMDX_FLAG_BUILDS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t seen = __hip_atomic_load(flags + RUMDX_FLAG_EXCL_SEEN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (builds != seen) {
+    const uint32_t t = blockIdx.x;
+    const uint4 info = tile_info[t];
+    const bool staged = (info.y >> 16) == halo_cap && info.x <= halo_cap;
+    const bool bytes = info.x <= kByteEntryHalo;
+    const uint32_t* hs = tile_halo + static_cast<size_t>(t) * halo_cap;
+    for (uint32_t i = info.z + threadIdx.x; staged && i < info.w; i += kBlock) {
+      const ExclusionRange er = ex.of(i);
+      if (er.e0 == er.e1) continue;
+      const uint32_t nn = num_nbrs[i];
+      if (nn >= max_nbrs) continue;
+      uint32_t kept = 0;
+      for (uint32_t k = 0; k < nn; ++k) {
+        const size_t at = (static_cast<size_t>(k >> 3) * nvp + i) * 8u + (k & 7u);
+        const uint16_t e = nbl16[at];
+        const uint32_t h = decode_entry(e, bytes);
+        if (h < info.x && ex.excludes(er, hs[h])) continue;
+        nbl16[(static_cast<size_t>(kept >> 3) * nvp + i) * 8u + (kept & 7u)] = e;
+        ++kept;
+      }
+      num_nbrs[i] = kept;
+    }
+  }
+  if (last_to_arrive(flags + RUMDX_FLAG_DONE, gridDim.x) && threadIdx.x == 0) flags[RUMDX_FLAG_EXCL_SEEN] = builds;
+}
+
 // Converts tile-format entries (halo offsets) back to slots.
 __global__ __launch_bounds__(kBlock) void k_export_tiles(TileGeom tg, const uint16_t* __restrict__ nbl16,
                                                          const uint32_t* __restrict__ num_nbrs,
@@ -879,12 +921,14 @@ extern "C" int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb,
   if (rc != RUMDX_OK) return rc;
   if (box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_n2: null box");
   if (nb->nbl_dev == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_n2: slot list not allocated");
+  if (nb->excl_start_dev != nullptr && (nb->excl_ids_dev == nullptr || p->unsorted_index_dev == nullptr))
+    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_n2: exclusions need the exclusion ids and the unsorted index");
   if (p->n == 0) return RUMDX_OK;
   const size_t lds = sizeof(float4) * kBlock + sizeof(float) * nb->ntypes * nb->ntypes;
   hipLaunchKernelGGL(k_build_n2, dim3(blocks_for(p->n)), dim3(kBlock), lds, as_stream(stream),
                      reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<float4*>(nb->last_r_dev),
                      nb->nbl_dev, nb->num_nbrs_dev, nb->flags_dev, nb->cut_skin2_dev, nb->ntypes, make_box(box), p->n,
-                     p->nvp, nb->max_nbrs);
+                     p->nvp, nb->max_nbrs, Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev});
   return check_launch("rumdx_nblist_build_n2");
 }
 
@@ -901,6 +945,8 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
   if (p->v_dev == nullptr || p->im_dev == nullptr || p->unsorted_index_dev == nullptr || p->f_dev == nullptr ||
       p->w_dev == nullptr)
     return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: particle arrays not allocated");
+  if (nb->excl_start_dev != nullptr && nb->excl_ids_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: exclusion ids missing");
   for (int a = 0; a < 3; ++a)
     if (nb->nc[a] < 5) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: fewer than 5 cells per axis");
   if (p->n == 0) return RUMDX_OK;
@@ -944,6 +990,11 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
   hipLaunchKernelGGL(k_build_tiles, dim3(nb->ntiles), dim3(kBuildBlock), sizeof(float4) * (nb->halo_cap + 1u), st, nb->flags_dev,
                      s, tg, nb->cut_skin2_dev, nb->ntypes, bx, home, nb->nbl16_dev, nb->num_nbrs_dev, p->nvp,
                      nb->max_nbrs);
+  if (nb->excl_start_dev != nullptr)
+    hipLaunchKernelGGL(k_apply_exclusions, dim3(nb->ntiles), dim3(kBlock), 0, st, nb->flags_dev,
+                       reinterpret_cast<const uint4*>(nb->tile_info_dev), nb->tile_halo_dev, nb->halo_cap,
+                       Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev}, nb->nbl16_dev,
+                       nb->num_nbrs_dev, p->nvp, nb->max_nbrs);
   return check_launch("rumdx_nblist_build_sorted");
 }
 

@@ -58,6 +58,39 @@ inline Box make_box(const float box[8]) {
   return b;
 }
 
+// ------------------------------------------------------------------ exclusions
+// A slot's excluded partners: [e0, e1) of the exclusion list, owner's id.
+struct ExclusionRange {
+  uint32_t e0 = 0, e1 = 0;
+};
+
+// Exclusion lists of a neighbour list (CSR over original ids, see
+// rumdx_nblist.excl_*); start == nullptr means none.  Slots map to original
+// ids through idx.
+struct Exclusions {
+  const uint32_t* idx;
+  const uint32_t* start;
+  const uint32_t* ids;
+
+  __device__ __forceinline__ ExclusionRange of(uint32_t slot) const {
+    ExclusionRange r;
+    if (start != nullptr) {
+      const uint32_t o = idx[slot];
+      r.e0 = start[o];
+      r.e1 = start[o + 1];
+    }
+    return r;
+  }
+  // Whether the particle in slot j is excluded for the owner of er.
+  __device__ __forceinline__ bool excludes(const ExclusionRange& er, uint32_t j) const {
+    if (er.e0 == er.e1) return false;
+    const uint32_t oj = idx[j];
+    for (uint32_t q = er.e0; q < er.e1; ++q)
+      if (ids[q] == oj) return true;
+    return false;
+  }
+};
+
 // RUMD's FP32 minimum-image squared distance (RectangularSimulationBox::
 // calculateDistance, include/rumd/SimulationBox.h:111-125), evaluated in source
 // order without FMA contraction so that it is reproducible bit for bit on the

@@ -90,7 +90,8 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetID_String", &Potential::GetID_String)
       .def("ResetInternalData", &Potential::ResetInternalData)
       .def("GetPotentialEnergy", &Potential::GetPotentialEnergy)
-      .def("EnergyIncludedInParticleSum", &Potential::EnergyIncludedInParticleSum);
+      .def("EnergyIncludedInParticleSum", &Potential::EnergyIncludedInParticleSum)
+      .def("IsMolecularPotential", &Potential::IsMolecularPotential);
 
   py::class_<PairPotential, Potential> pp(m, "PairPotential");
   py::enum_<PairPotential::CutoffMethod>(pp, "CutoffMethod")
@@ -118,6 +119,9 @@ PYBIND11_MODULE(_rumd, m) {
            },
            py::arg("num_types"))
       .def("GetPotentialForm", &PairPotential::GetPotentialForm)
+      .def("SetExclusion", [](PairPotential& p, unsigned i, unsigned j) { p.GetNeighborList().SetExclusion(i, j); },
+           py::arg("particleI"), py::arg("particleJ"))
+      .def("GetNumberOfExclusions", [](PairPotential& p) { return p.GetNeighborList().GetNumberOfExclusions(); })
       .def("GetParameterList", &PairPotential::GetPotentialParameters, py::arg("num_types"))
       .def("GetPotentialEnergy", &PairPotential::GetPotentialEnergy)
       .def("GetActualMaxNumNbrs", &PairPotential::GetActualMaxNumNbrs)
@@ -201,11 +205,14 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetParams", &BondHarmonic::SetParams, py::arg("bond_type"), py::arg("bond_length"), py::arg("stiffness"),
            py::arg("exclude") = false)
       .def("GetParams", &BondHarmonic::GetParams)
-      .def("GetExcludedTypes", &BondHarmonic::GetExcludedTypes);
+      .def("GetExcludedTypes", &BondHarmonic::GetExcludedTypes)
+      .def("SetExclusions", &BondHarmonic::SetExclusions, py::arg("non_bond_pot"));
   py::class_<AngleCosSq, Potential>(m, "AngleCosSq")
       .def(py::init<>())
       .def("SetParams", &AngleCosSq::SetParams, py::arg("angle_type"), py::arg("theta0"), py::arg("ktheta"))
-      .def("GetParams", &AngleCosSq::GetParams);
+      .def("GetParams", &AngleCosSq::GetParams)
+      .def("SetExclude", &AngleCosSq::SetExclude, py::arg("exclude"))
+      .def("SetExclusions", &AngleCosSq::SetExclusions, py::arg("non_bond_pot"));
   py::class_<MoleculeData>(m, "MoleculeData")
       .def("ReadTopology", &MoleculeData::ReadTopology, py::arg("filename"))
       .def("SetBonds", &MoleculeData::SetBonds, py::arg("abt"), py::arg("molecule") = std::vector<uint32_t>{})

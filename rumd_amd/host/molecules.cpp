@@ -182,6 +182,16 @@ std::vector<unsigned> BondHarmonic::GetExcludedTypes() const {
   return t;
 }
 
+void BondHarmonic::SetExclusions(PairPotential* non_bond_pot) {
+  if (!sample || sample->GetMoleculeData() == nullptr)
+    throw RUMD_Error(GetID_String(), "SetExclusions", "no molecule data: read a topology first");
+  const std::vector<uint32_t>& b = sample->GetMoleculeData()->GetBonds();
+  std::vector<uint32_t> pairs;
+  for (size_t k = 0; k < b.size(); k += 3)
+    if (b[k + 2] < exclude_.size() && exclude_[b[k + 2]]) pairs.insert(pairs.end(), {b[k], b[k + 1]});
+  if (!pairs.empty()) non_bond_pot->GetNeighborList().AddExclusions(pairs);
+}
+
 void BondHarmonic::Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method) {
   if (!sample) throw RUMD_Error(GetID_String(), method, "no sample");
   MoleculeData* m = sample->GetMoleculeData();
@@ -223,6 +233,16 @@ void AngleCosSq::SetParams(unsigned angle_type, float theta0, float ktheta) {
   d_params_.resize(params_.size());
   d_params_.upload(params_.data(), params_.size(), sample ? sample->GetStream() : nullptr);
   if (sample) sample->InvalidateGraphs();
+}
+
+void AngleCosSq::SetExclusions(PairPotential* non_bond_pot) {
+  if (!exclude_angle_) return;
+  if (!sample || sample->GetMoleculeData() == nullptr)
+    throw RUMD_Error(GetID_String(), "SetExclusions", "no molecule data: read a topology first");
+  const std::vector<uint32_t>& a = sample->GetMoleculeData()->GetAngles();
+  std::vector<uint32_t> pairs;
+  for (size_t k = 0; k < a.size(); k += 4) pairs.insert(pairs.end(), {a[k], a[k + 2]});
+  if (!pairs.empty()) non_bond_pot->GetNeighborList().AddExclusions(pairs);
 }
 
 void AngleCosSq::Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method) {

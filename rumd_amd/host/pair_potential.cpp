@@ -68,6 +68,9 @@ std::string NeighborList::GetNB_Method() const {
 }
 
 NeighborList::Method NeighborList::GetResolvedMethod() const {
+  // with exclusions "none" runs as N2 (NeighborList::SetExclusionType and
+  // UpdateNBlist do the same switch in the reference)
+  if (method_ == NONE && HasExclusions()) return N2;
   if (method_ != DEFAULT) return method_;
   const unsigned n = sample_ ? sample_->GetNumberOfParticles() : 0;
   return n > kSortThreshold ? SORT : N2;  // src/NeighborList.cu:586-587
@@ -76,6 +79,61 @@ NeighborList::Method NeighborList::GetResolvedMethod() const {
 void NeighborList::SetMaxNumNbrs(unsigned m) {
   user_max_nbrs_ = m;
   allocated_ = false;
+  if (sample_) sample_->InvalidateGraphs();
+}
+
+void NeighborList::SetExclusion(unsigned i, unsigned j) { AddExclusions({i, j}); }
+
+void NeighborList::AddExclusions(const std::vector<uint32_t>& pairs) {
+  if (!sample_) throw RUMD_Error("NeighborList", "SetExclusion", "no sample");
+  const unsigned n = sample_->GetNumberOfParticles();
+  if (pairs.size() % 2) throw RUMD_Error("NeighborList", "SetExclusion", "exclusions come in pairs");
+  if (excl_.size() != n) excl_.assign(n, {});
+  for (size_t k = 0; k < pairs.size(); k += 2) {
+    const uint32_t a = pairs[k], b = pairs[k + 1];
+    if (a >= n || b >= n || a == b) throw RUMD_Error("NeighborList", "SetExclusion", "invalid particle pair");
+    auto add = [&](uint32_t x, uint32_t y) {
+      if (std::find(excl_[x].begin(), excl_[x].end(), y) == excl_[x].end()) {
+        excl_[x].push_back(y);
+        return true;
+      }
+      return false;
+    };
+    if (add(a, b)) ++num_exclusions_;
+    add(b, a);
+  }
+  UploadExclusions();
+}
+
+void NeighborList::ClearExclusions() {
+  excl_.clear();
+  num_exclusions_ = 0;
+  UploadExclusions();
+}
+
+// CSR over original ids, uploaded now (not under a graph capture); the next
+// update rebuilds the list with them.
+void NeighborList::UploadExclusions() {
+  if (num_exclusions_ == 0) {
+    desc_.excl_start_dev = nullptr;
+    desc_.excl_ids_dev = nullptr;
+  } else {
+    const unsigned n = sample_->GetNumberOfParticles();
+    std::vector<uint32_t> start(n + 1, 0u), ids;
+    for (unsigned o = 0; o < n; ++o) {
+      start[o] = static_cast<uint32_t>(ids.size());
+      ids.insert(ids.end(), excl_[o].begin(), excl_[o].end());
+    }
+    start[n] = static_cast<uint32_t>(ids.size());
+    const hipStream_t st = sample_->GetStream();
+    excl_start_.resize(start.size());
+    excl_start_.upload(start.data(), start.size(), st);
+    excl_ids_.resize(ids.size());
+    excl_ids_.upload(ids.data(), ids.size(), st);
+    desc_.excl_start_dev = excl_start_.get();
+    desc_.excl_ids_dev = excl_ids_.get();
+  }
+  allocated_ = false;  // the method may resolve differently; rebuild from scratch
   if (sample_) sample_->InvalidateGraphs();
 }
 
@@ -129,6 +187,10 @@ void NeighborList::Allocate() {
   cut_skin2_.resize(cs2.size());
   cut_skin2_.upload(cs2.data(), cs2.size(), st);
   std::memset(&desc_, 0, sizeof(desc_));
+  if (num_exclusions_ > 0) {
+    desc_.excl_start_dev = excl_start_.get();
+    desc_.excl_ids_dev = excl_ids_.get();
+  }
   desc_.nbl_dev = nbl_.get();
   desc_.num_nbrs_dev = num_nbrs_.get();
   desc_.last_r_dev = last_r_.get();

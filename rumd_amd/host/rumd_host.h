@@ -132,6 +132,8 @@ class Potential {
   virtual void ResetInternalData() {}
   virtual double GetPotentialEnergy() { return 0.0; }
   virtual double GetVirial() { return 0.0; }
+  // Bonds, angles: potentials whose terms come from the sample's topology.
+  virtual bool IsMolecularPotential() const { return false; }
   void SetVerbose(bool vb) { verbose = vb; }
   void SetID_String(const std::string& s) { ID_String = s; }
   std::string GetID_String() const { return ID_String; }
@@ -176,6 +178,14 @@ class NeighborList {
   bool HandleOverflow();
   // Asks the next UpdateNBlist to rebuild (device flag, on the sample's stream).
   void RequestRebuild();
+  // Exclusions (NeighborList::SetExclusion, src/NeighborList.cu:791-850):
+  // pairs of original particle ids that this list leaves out.  The lists are
+  // rebuilt with them at the next update; method "none" then runs as N2.
+  void SetExclusion(unsigned i, unsigned j);
+  void AddExclusions(const std::vector<uint32_t>& pairs);
+  void ClearExclusions();
+  bool HasExclusions() const { return num_exclusions_ > 0; }
+  unsigned GetNumberOfExclusions() const { return num_exclusions_; }
   bool IsSorted() const { return GetResolvedMethod() == SORT; }
   void DownloadCounts(std::vector<uint32_t>& out);
   // LDS halo capacity of the tile kernels, in particles (0 = estimate from
@@ -214,6 +224,10 @@ class NeighborList {
   DeviceArray<uint32_t> tile_info_;  // ntiles x 4
   rumdx_nblist desc_{};
   uint32_t allocated_nvp_ = 0;
+  void UploadExclusions();
+  std::vector<std::vector<uint32_t>> excl_;  // per original id
+  unsigned num_exclusions_ = 0;
+  DeviceArray<uint32_t> excl_start_, excl_ids_;
 };
 
 // PairPotential (include/rumd/PairPotential.h:33-128, src/PairPotential.cu).
@@ -416,9 +430,13 @@ class BondHarmonic : public Potential {
   bool EnergyIncludedInParticleSum() const override { return true; }
   void Initialize() override {}
   double GetPotentialEnergy() override;
+  bool IsMolecularPotential() const override { return true; }
   std::vector<float> GetParams() const { return params_; }
   // Bond types whose pairs the non-bonded potentials skip.
   std::vector<unsigned> GetExcludedTypes() const;
+  // BondPotential::SetExclusions (src/BondPotential.cu:54-66): the pairs of
+  // bonds of excluded types leave the pair potential's neighbour list.
+  void SetExclusions(PairPotential* non_bond_pot);
 
  private:
   void Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method);
@@ -438,11 +456,18 @@ class AngleCosSq : public Potential {
   bool EnergyIncludedInParticleSum() const override { return true; }
   void Initialize() override {}
   double GetPotentialEnergy() override;
+  bool IsMolecularPotential() const override { return true; }
   std::vector<float> GetParams() const { return params_; }
+  // AnglePotential::SetExclude / SetExclusions (src/AnglePotential.cu:63-66):
+  // the end atoms (a, c) of every angle leave the pair potential's list
+  // (on by default, as in the reference).
+  void SetExclude(bool exclude) { exclude_angle_ = exclude; }
+  void SetExclusions(PairPotential* non_bond_pot);
 
  private:
   void Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method);
   std::vector<float> params_;  // (theta0, k) per angle type
+  bool exclude_angle_ = true;
   DeviceArray<float> d_params_;
   DeviceArray<float> pe_f_, pe_w_;
 };

@@ -59,6 +59,8 @@ class NbList(C.Structure):
         ("tile_halo_dev", C.c_void_p),
         ("tile_info_dev", C.c_void_p),
         ("lanes_per_particle", C.c_uint32),
+        ("excl_start_dev", C.c_void_p),
+        ("excl_ids_dev", C.c_void_p),
     ]
 
 

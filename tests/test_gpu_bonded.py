@@ -42,7 +42,7 @@ def _topology(golden_dir):
     return bonds, angles
 
 
-def _sample(golden_dir, method, with_angles=True):
+def _sample(golden_dir, method, with_angles=True, exclude_bonds=False):
     c = load_golden(golden_dir / "pentamer200_rho0.85_T0.5.xyz.gz")
     s = rumd.Sample(0, 0)
     s.SetVerbose(False)
@@ -60,7 +60,7 @@ def _sample(golden_dir, method, with_angles=True):
     bond = rumd.BondHarmonic()
     bond.SetVerbose(False)
     for t, (l0, k) in enumerate(BOND_PARAMS):
-        bond.SetParams(bond_type=t, bond_length=l0, stiffness=k)
+        bond.SetParams(bond_type=t, bond_length=l0, stiffness=k, exclude=exclude_bonds)
     s.AddPotential(bond)
     ang = None
     if with_angles:
@@ -135,3 +135,91 @@ def test_bonded_graph_blocks_equal_eager_and_conserve_energy(golden_dir):
     e1 = s.GetPotentialEnergy() + itg.GetKineticEnergy()
     print(f"  drift per particle {(e1 - e0) / c.n:.2e}", flush=True)
     assert abs(e1 - e0) / c.n < 2e-3
+
+
+def _excluded(bonds, angles):
+    ex = {(min(a, b), max(a, b)) for a, b, _ in bonds}
+    ex |= {(min(a, c), max(a, c)) for a, _, c, _ in angles}
+    return ex
+
+
+def _oracle_lj_excluding(c, pos4, table, excl):
+    """FP64 LJ over the rc + skin pairs minus the excluded ones (the oracle's
+    list-driven sum), and those lists as sets."""
+    b8 = configs.box8(c.box)
+    off, idx = oracle.nblist_brute_f32(pos4, b8, 1, configs.cut_skin2_table(table, 0.3))
+    sets = []
+    off2, idx2 = [0], []
+    for i in range(c.n):
+        keep = [int(j) for j in idx[off[i]:off[i + 1]] if (min(i, j), max(i, j)) not in excl]
+        sets.append(set(keep))
+        idx2.extend(keep)
+        off2.append(len(idx2))
+    f, vir = oracle.forces_nl(pos4, c.box, 1, table, SP, np.array(off2, np.int64), np.array(idx2, np.int32))
+    return f, vir, sets
+
+
+@pytest.mark.parametrize("method,variant", [("n2", "plain"), ("sort", "plain"), ("none", "plain"),
+                                            ("sort", "overflow"), ("sort", "small_halo"), ("n2", "overflow")])
+def test_exclusions_leave_bonded_pairs_out(golden_dir, method, variant):
+    """Bond pairs (exclude=True) and angle end pairs (excluded by default)
+    leave the LJ lists: N2 build filter, sorted-list compaction, and the cell
+    rescans of overflowed lists and unstaged tiles."""
+    c, s, pot, bond, ang, bonds, angles = _sample(golden_dir, method, exclude_bonds=True)
+    bond.SetExclusions(pot)
+    ang.SetExclusions(pot)
+    excl = _excluded(bonds, angles)
+    assert pot.GetNumberOfExclusions() == len(excl) == 1400
+    if variant == "overflow":
+        pot.SetNbMaxNumNbrs(30)
+    elif variant == "small_halo":
+        pot.SetNbHaloCapacity(64)
+    print(f"[case] exclusions list={method} {variant}", flush=True)
+    s.CalcF()
+    f = s.GetForcesAndEnergies().astype(np.float64)
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    table = configs.lj_param_table([(0, 0, 1.0, 1.0, 2.5)], 1)
+    f_lj, vir_lj, sets = _oracle_lj_excluding(c, pos4, table, excl)
+    f_b, vir_b = oracle.bonded(pos4, c.box, bonds, BOND_PARAMS, None, None)
+    f_a, _ = oracle.bonded(pos4, c.box, None, None, angles, ANGLE_PARAMS)
+    f_ref = f_lj + f_b + f_a
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+    np.testing.assert_allclose(f[:, 3], f_ref[:, 3], rtol=0, atol=5e-5)
+    assert s.GetVirial() == pytest.approx((vir_lj.sum() + vir_b.sum()) / 6.0, rel=1e-5)
+    if variant == "plain":
+        off, ids = pot.GetNeighborListCSR(s)
+        got = [set(ids[off[i]:off[i + 1]].tolist()) for i in range(c.n)]
+        assert got == sets
+
+
+def test_simulation_sets_exclusions_and_runs(golden_dir, tmp_path, monkeypatch):
+    """Simulation.AddPotential wires the molecular potentials' exclusions into
+    the pair potential in either order; a short NVT run stays finite."""
+    from rumd_amd.Simulation import Simulation
+    monkeypatch.chdir(tmp_path)
+    sim = Simulation(str(golden_dir / "pentamer200_rho0.85_T0.5.xyz.gz"), verbose=False)
+    sim.ReadMoleculeData(golden_dir / "pentamer200_rho0.85_T0.5.top")
+    bond = rumd.BondHarmonic()
+    bond.SetVerbose(False)
+    bond.SetParams(bond_type=0, bond_length=1.0, stiffness=100.0, exclude=True)
+    bond.SetParams(bond_type=1, bond_length=1.1, stiffness=150.0, exclude=True)
+    sim.AddPotential(bond)  # before the pair potential
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedPotential)
+    pot.SetVerbose(False)
+    pot.SetParams(0, 0, Sigma=1.0, Epsilon=1.0, Rcut=2.5)
+    pot.SetNbListSkin(0.3)
+    sim.AddPotential(pot)
+    ang = rumd.AngleCosSq()
+    ang.SetVerbose(False)
+    ang.SetParams(angle_type=0, theta0=2.0, ktheta=20.0)
+    ang.SetParams(angle_type=1, theta0=1.8, ktheta=30.0)
+    sim.AddPotential(ang)  # after it; the topology file holds 9 angles
+    assert pot.GetNumberOfExclusions() == 800 + 9
+    with pytest.raises(RuntimeError):
+        sim.SetPotential(bond)
+    sim.SetIntegrator(rumd.IntegratorNVT(timeStep=0.002, targetTemperature=0.5, thermostatRelaxationTime=0.2))
+    print("[case] Simulation pentamer200 with exclusions", flush=True)
+    sim.Run(300, suppressAllOutput=True)
+    assert np.isfinite(sim.sample.GetPositions()).all()
+    assert np.isfinite(sim.sample.GetPotentialEnergy())
