@@ -270,6 +270,24 @@ int rumdx_bond_harmonic(const rumdx_particles* p, const uint32_t* slot_of_dev,
                         const uint32_t* bond_type_dev, const float* params_dev, uint32_t ntypes,
                         const float box[8], int initialize, int calc_stresses, void* stream);
 
+/* BondFENE::CalcF (kernel FENE, src/BondPotential.cu:358-430): u = -k R0^2/2
+ * ln(1 - r^2/R0^2) per bond, params_dev = (R0, k) per type; otherwise as
+ * rumdx_bond_harmonic. */
+int rumdx_bond_fene(const rumdx_particles* p, const uint32_t* slot_of_dev, const uint32_t* bond_start_dev,
+                    const uint32_t* bond_other_dev, const uint32_t* bond_type_dev, const float* params_dev,
+                    uint32_t ntypes, const float box[8], int initialize, int calc_stresses, void* stream);
+
+/* DihedralRyckaert::CalcF (kernel Ryckaert, src/DihedralPotential.cu:83-110,
+ * 150-260): u = sum_k c_k cos^k(psi), psi = 0 for trans, per dihedral
+ * (a, b, c, d).  dihedral_atoms_dev holds 5 u32 per entry (a, b, c, d,
+ * type), listed under each of its four atoms; params_dev = c_0..c_5 per type.
+ * The energy goes to atom a's f.w; no virial or stress is added (with
+ * initialize, w and sts are cleared as for angles). */
+int rumdx_dihedral_ryckaert(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                            const uint32_t* dihedral_start_dev, const uint32_t* dihedral_atoms_dev,
+                            const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
+                            int calc_stresses, void* stream);
+
 /* AngleCosSq::CalcF (src/AnglePotential.cu:69-98, kernel CosSq :137-217):
  * u = k/2 (cos theta - cos theta0)^2 per angle (a, b, c), the angle at b.
  * angle_atoms_dev holds 4 u32 per entry: a, b, c (original ids) and the angle

@@ -30,8 +30,9 @@
  *     (gLJ 12-6 = LJA, IPL_n with n = 12 = IPL 12-6 with A6 = 0), the
  *     Buckingham minimum -epsilon at rm and the smooth LJ's vanishing force
  *     at Rcut.
- *   orc_bonds_harmonic, orc_angles_cossq: closed forms for one bond and one
- *     angle, -grad E = F by finite differences of every coordinate of a small
+ *   orc_bonds, orc_angles_cossq, orc_dihedrals_ryckaert: closed forms for
+ *     one bond (harmonic, FENE), one angle and the trans / cis dihedral,
+ *     -grad E = F by finite differences of every coordinate of a small
  *     molecule, zero total force and torque.
  *   orc_nblist_*: brute force vs cell scan, symmetry, superset of the rc sphere.
  *   orc_nvt_step: closed form for one particle, Ps = 0 => NVE, time
@@ -298,20 +299,27 @@ void orc_forces_form_brute(int n, const float* pos4, const double L[3], int nt, 
   }
 }
 
-/* Harmonic bonds in FP64 (BondHarmonic, src/BondPotential.cu:281-357): for
- * each bond (a, b, type) with params (l0, k), u = k/2 (r - l0)^2; adds the
- * forces to f4, half of u to each end's f4[.,3] and F_a . (r_a - r_b) to each
- * end's vir. */
-void orc_bonds_harmonic(int n, const float* pos4, const double L[3], int nbonds, const int32_t* bonds,
-                        const float* params, double* f4, double* vir) {
+/* Bonds in FP64 (BondHarmonic and BondFENE, src/BondPotential.cu:281-430):
+ * for each bond (a, b, type) with params (p0, k), harmonic u = k/2 (r - p0)^2
+ * or (fene != 0) u = -k p0^2/2 ln(1 - (r/p0)^2); adds the forces to f4, half
+ * of u to each end's f4[.,3] and F_a . (r_a - r_b) to each end's vir. */
+void orc_bonds(int n, const float* pos4, const double L[3], int nbonds, const int32_t* bonds, const float* params,
+               int fene, double* f4, double* vir) {
   (void)n;
   for (int q = 0; q < nbonds; ++q) {
     const int a = bonds[3 * q], b = bonds[3 * q + 1], t = bonds[3 * q + 2];
     const double l0 = params[2 * t], k = params[2 * t + 1];
     double d[3];
     const double r = sqrt(min_image_f64(pos4 + 4 * (size_t)a, pos4 + 4 * (size_t)b, L, d));
-    const double u = 0.5 * k * (r - l0) * (r - l0);
-    const double g = -k * (r - l0) / r; /* F_a = g d, d = r_a - r_b */
+    double u, g; /* F_a = g d, d = r_a - r_b */
+    if (fene) {  /* u = -k R0^2/2 ln(1 - (r/R0)^2), R0 = l0 */
+      const double x = (r / l0) * (r / l0);
+      u = -0.5 * k * l0 * l0 * log(1.0 - x);
+      g = -k / (1.0 - x);
+    } else {
+      u = 0.5 * k * (r - l0) * (r - l0);
+      g = -k * (r - l0) / r;
+    }
     for (int c = 0; c < 3; ++c) {
       f4[4 * (size_t)a + c] += g * d[c];
       f4[4 * (size_t)b + c] -= g * d[c];
@@ -352,6 +360,71 @@ void orc_angles_cossq(int n, const float* pos4, const double L[3], int nangles, 
       f4[4 * (size_t)b + m] -= fa + fc;
     }
     f4[4 * (size_t)a + 3] += 0.5 * k * dcos * dcos;
+  }
+}
+
+static void cross3(const double* x, const double* y, double* z) {
+  z[0] = x[1] * y[2] - x[2] * y[1];
+  z[1] = x[2] * y[0] - x[0] * y[2];
+  z[2] = x[0] * y[1] - x[1] * y[0];
+}
+
+/* Ryckaert-Bellemans dihedrals in FP64 (DihedralRyckaert, src/DihedralPotential.cu:150-260),
+ * from the IUPAC torsion phi of (a, b, c, d): b1 = r_b - r_a, b2 = r_c - r_b,
+ * b3 = r_d - r_c, m = b1 x b2, n = b2 x b3, phi = atan2(|b2| b1.n, m.n)
+ * (0 = cis); psi = phi - pi, u = sum_k c_k cos^k(psi).  Forces from
+ * dphi/dr_a = -|b2| m / |m|^2, dphi/dr_d = |b2| n / |n|^2 and the two inner
+ * atoms' terms (Blondel & Karplus, J. Comput. Chem. 17, 1132 (1996)).  The
+ * energy goes to atom a; nearly collinear bonds (|cos| within 1e-3 of 1 on
+ * either side) contribute nothing, as in the reference. */
+void orc_dihedrals_ryckaert(int n, const float* pos4, const double L[3], int ndih, const int32_t* dih,
+                            const float* params, double* f4) {
+  (void)n;
+  for (int q = 0; q < ndih; ++q) {
+    const int a = dih[5 * q], b = dih[5 * q + 1], c = dih[5 * q + 2], d = dih[5 * q + 3], t = dih[5 * q + 4];
+    const float* cf = params + 6 * t;
+    double b1[3], b2[3], b3[3], m[3], nn[3];
+    min_image_f64(pos4 + 4 * (size_t)b, pos4 + 4 * (size_t)a, L, b1);
+    min_image_f64(pos4 + 4 * (size_t)c, pos4 + 4 * (size_t)b, L, b2);
+    min_image_f64(pos4 + 4 * (size_t)d, pos4 + 4 * (size_t)c, L, b3);
+    const double l1 = b1[0] * b1[0] + b1[1] * b1[1] + b1[2] * b1[2];
+    const double l2 = b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2];
+    const double l3 = b3[0] * b3[0] + b3[1] * b3[1] + b3[2] * b3[2];
+    const double d12 = b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2];
+    const double d23 = b2[0] * b3[0] + b2[1] * b3[1] + b2[2] * b3[2];
+    if (fabs(d23 * d23 / (l2 * l3) - 1.0) < 1e-3 || fabs(d12 * d12 / (l1 * l2) - 1.0) < 1e-3) continue;
+    cross3(b1, b2, m);
+    cross3(b2, b3, nn);
+    const double mm = m[0] * m[0] + m[1] * m[1] + m[2] * m[2];
+    const double nq = nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2];
+    const double lb2 = sqrt(l2);
+    const double y = lb2 * (b1[0] * nn[0] + b1[1] * nn[1] + b1[2] * nn[2]);
+    const double x = m[0] * nn[0] + m[1] * nn[1] + m[2] * nn[2];
+    const double phi = atan2(y, x);
+    const double cpsi = -cos(phi); /* cos(phi - pi) */
+    double u = 0.0, dudc = 0.0, pw = 1.0;
+    for (int k = 0; k < 6; ++k) {
+      u += cf[k] * pw;
+      if (k < 5) dudc += (k + 1) * cf[k + 1] * pw;
+      pw *= cpsi;
+    }
+    /* du/dphi = du/dcpsi * dcpsi/dphi = dudc * sin(phi) */
+    const double dudphi = dudc * sin(phi);
+    double ga[3], gd[3];
+    for (int k = 0; k < 3; ++k) {
+      ga[k] = -lb2 * m[k] / mm;
+      gd[k] = lb2 * nn[k] / nq;
+    }
+    const double s1 = d12 / l2, s3 = d23 / l2;
+    for (int k = 0; k < 3; ++k) {
+      const double gb = -(1.0 + s1) * ga[k] + s3 * gd[k];
+      const double gc = s1 * ga[k] - (1.0 + s3) * gd[k];
+      f4[4 * (size_t)a + k] -= dudphi * ga[k];
+      f4[4 * (size_t)b + k] -= dudphi * gb;
+      f4[4 * (size_t)c + k] -= dudphi * gc;
+      f4[4 * (size_t)d + k] -= dudphi * gd[k];
+    }
+    f4[4 * (size_t)a + 3] += u;
   }
 }
 

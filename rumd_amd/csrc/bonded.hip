@@ -32,6 +32,18 @@ __device__ __forceinline__ float3 displacement(float4 a, float4 b, const Box& bx
   return make_float3(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z));
 }
 
+// The same displacement kept in FP64 (for the dihedral algebra, whose
+// differences of products cancel).
+__device__ __forceinline__ double3 displacement_f64(float4 a, float4 b, const Box& bx) {
+  double x = static_cast<double>(a.x) - static_cast<double>(b.x);
+  double y = static_cast<double>(a.y) - static_cast<double>(b.y);
+  double z = static_cast<double>(a.z) - static_cast<double>(b.z);
+  x -= bx.Ld[0] * rint(x * bx.iLd[0]);
+  y -= bx.Ld[1] * rint(y * bx.iLd[1]);
+  z -= bx.Ld[2] * rint(z * bx.iLd[2]);
+  return make_double3(x, y, z);
+}
+
 __global__ __launch_bounds__(kBlock) void k_slot_index(const uint32_t* __restrict__ idx,
                                                        uint32_t* __restrict__ slot_of, uint32_t n) {
   const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
@@ -67,11 +79,16 @@ __device__ __forceinline__ void store_sums(uint32_t s, const BondSums& a, bool i
   w[s] = wo;
 }
 
-// Harmonic bonds, u = k/2 (r - l0)^2 per bond: each end takes F = ft d with
-// ft = -k (1 - l0/r), half the bond energy and the virial ft r^2 (the
-// reference's per-end bookkeeping).  params: (l0, k) per bond type.
-template <bool STR>
-__global__ __launch_bounds__(kBlock) void k_bond_harmonic(
+// Bonds: each end takes F = ft d, half the bond energy and the virial ft r^2
+// (the reference's per-end bookkeeping).  params: (p0, k) per bond type.
+//   harmonic (FORM 0): u = k/2 (r - l0)^2, ft = -k (1 - l0/r), p0 = l0
+//   FENE     (FORM 1): u = -k R0^2/2 ln(1 - r^2/R0^2), ft = -k / (1 - r^2/R0^2),
+//                      p0 = R0 (the FENE kernel, src/BondPotential.cu:358-430)
+constexpr int kBondHarmonic = 0;
+constexpr int kBondFene = 1;
+
+template <bool STR, int FORM>
+__global__ __launch_bounds__(kBlock) void k_bond(
     const float4* __restrict__ r, float4* __restrict__ f, float4* __restrict__ w, float4* __restrict__ sts,
     const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ start, const uint32_t* __restrict__ other,
     const uint32_t* __restrict__ btype, const float2* __restrict__ params, uint32_t ntypes, Box bx, uint32_t n,
@@ -85,16 +102,26 @@ __global__ __launch_bounds__(kBlock) void k_bond_harmonic(
   BondSums a;
   for (uint32_t b = b0; b < b1; ++b) {
     const uint32_t t = btype[b];
-    const float2 p = params[t < ntypes ? t : 0u];
+    if (t >= ntypes) continue;  // a bond type this potential has no parameters for
+    const float2 p = params[t];
+    if (p.y == 0.f) continue;
     const float3 d = displacement(ri, r[slot_of[other[b]]], bx);
     const float d2 = d.x * d.x + d.y * d.y + d.z * d.z;
-    const float rij = sqrtf(d2);
-    const float dl = rij - p.x;
-    const float ft = -p.y * (1.0f - p.x / rij);
+    float ft, half_u;
+    if (FORM == kBondFene) {
+      const float rr = d2 / (p.x * p.x);
+      ft = -p.y / (1.0f - rr);
+      half_u = -0.25f * p.x * p.x * p.y * logf(1.0f - rr);
+    } else {
+      const float rij = sqrtf(d2);
+      const float dl = rij - p.x;
+      ft = -p.y * (1.0f - p.x / rij);
+      half_u = 0.25f * p.y * dl * dl;  // half of k/2 (r - l0)^2
+    }
     a.fx += ft * d.x;
     a.fy += ft * d.y;
     a.fz += ft * d.z;
-    a.u += 0.25f * p.y * dl * dl;  // half of k/2 (r - l0)^2
+    a.u += half_u;
     a.vir += ft * d2;
     if (STR) {
       a.sxx -= ft * d.x * d.x;
@@ -173,6 +200,89 @@ __global__ __launch_bounds__(kBlock) void k_angle_cossq(const float4* __restrict
   }
 }
 
+// Ryckaert-Bellemans dihedrals, u = sum_k c_k cos^k(psi) for (a, b, c, d)
+// with psi the angle between the planes (a, b, c) and (b, c, d), zero for
+// trans (the Ryckaert kernel, src/DihedralPotential.cu:150-260): with
+// d1 = r_b - r_a, d2 = r_c - r_b, d3 = r_d - r_c, cos psi = cA / sqrt(cB1 cB2)
+// (cA = c13 c22 - c12 c23, cB1 = c11 c22 - c12^2, cB2 = c22 c33 - c23^2),
+// f_a = f1, f_b = -(1 + c12/c22) f1 + (c23/c22) f2, f_c = (c12/c22) f1 -
+// (1 + c23/c22) f2, f_d = f2.  Nearly collinear bonds contribute nothing.  The
+// energy goes to atom a.  The algebra runs in FP64: its differences of
+// products (cA, cB1, cB2) cancel for nearly planar or nearly collinear
+// geometries, which in FP32 costs several digits.  Each list entry is 5 u32:
+// a, b, c, d (original ids) and the type; params: c_0..c_5 per type.
+__global__ __launch_bounds__(kBlock) void k_dihedral_ryckaert(
+    const float4* __restrict__ r, float4* __restrict__ f, float4* __restrict__ w, float4* __restrict__ sts,
+    const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ start, const uint32_t* __restrict__ dih,
+    const float* __restrict__ params, uint32_t ntypes, Box bx, uint32_t n, int init, int str) {
+  const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+  if (o >= n) return;
+  const uint32_t q0 = start[o], q1 = start[o + 1];
+  if (q0 == q1 && !init) return;
+  double fx = 0.0, fy = 0.0, fz = 0.0, u = 0.0;
+  for (uint32_t q = q0; q < q1; ++q) {
+    const uint32_t* e = dih + 5ull * q;
+    const uint32_t ia = e[0], ib = e[1], ic = e[2], id = e[3];
+    const float* pf = params + 6u * (e[4] < ntypes ? e[4] : 0u);
+    double p[6];
+    for (int k = 0; k < 6; ++k) p[k] = pf[k];
+    const float4 ra = r[slot_of[ia]], rb = r[slot_of[ib]], rc = r[slot_of[ic]], rd = r[slot_of[id]];
+    const double3 d1 = displacement_f64(rb, ra, bx);
+    const double3 d2 = displacement_f64(rc, rb, bx);
+    const double3 d3 = displacement_f64(rd, rc, bx);
+    const double c11 = d1.x * d1.x + d1.y * d1.y + d1.z * d1.z;
+    const double c12 = d1.x * d2.x + d1.y * d2.y + d1.z * d2.z;
+    const double c13 = d1.x * d3.x + d1.y * d3.y + d1.z * d3.z;
+    const double c22 = d2.x * d2.x + d2.y * d2.y + d2.z * d2.z;
+    const double c23 = d2.x * d3.x + d2.y * d3.y + d2.z * d3.z;
+    const double c33 = d3.x * d3.x + d3.y * d3.y + d3.z * d3.z;
+    const double cA = c13 * c22 - c12 * c23;
+    const double cB1 = c11 * c22 - c12 * c12;
+    const double cB2 = c22 * c33 - c23 * c23;
+    constexpr double kCollinear = 1e-3;  // DIHEDRAL_EPS (include/rumd/DihedralPotential.h:9)
+    if (fabs(c23 * c23 / (c22 * c33) - 1.0) < kCollinear || fabs(c12 * c12 / (c11 * c22) - 1.0) < kCollinear)
+      continue;
+    const double cD = sqrt(cB1 * cB2);
+    const double cc = cA / cD;
+    const double ff = -(p[1] + (2.0 * p[2] + (3.0 * p[3] + (4.0 * p[4] + 5.0 * p[5] * cc) * cc) * cc) * cc) * c22;
+    const double t1 = cA, t2 = c11 * c23 - c12 * c13, t3 = -cB1;
+    const double t4 = cB2, t5 = c13 * c23 - c12 * c33, t6 = -cA;
+    const double cr1 = c12 / c22, cr2 = c23 / c22;
+    const double g1 = ff / (cD * cB1), g2 = ff / (cD * cB2);
+    const double3 f1 = make_double3(g1 * (t1 * d1.x + t2 * d2.x + t3 * d3.x), g1 * (t1 * d1.y + t2 * d2.y + t3 * d3.y),
+                                    g1 * (t1 * d1.z + t2 * d2.z + t3 * d3.z));
+    const double3 f2 = make_double3(g2 * (t4 * d1.x + t5 * d2.x + t6 * d3.x), g2 * (t4 * d1.y + t5 * d2.y + t6 * d3.y),
+                                    g2 * (t4 * d1.z + t5 * d2.z + t6 * d3.z));
+    double ka = 0.0, kb = 0.0;  // own force = ka f1 + kb f2
+    if (o == ia) {
+      ka = 1.0;
+      u += p[0] + (p[1] + (p[2] + (p[3] + (p[4] + p[5] * cc) * cc) * cc) * cc) * cc;
+    } else if (o == ib) {
+      ka = -(1.0 + cr1);
+      kb = cr2;
+    } else if (o == ic) {
+      ka = cr1;
+      kb = -(1.0 + cr2);
+    } else {
+      kb = 1.0;
+    }
+    fx += ka * f1.x + kb * f2.x;
+    fy += ka * f1.y + kb * f2.y;
+    fz += ka * f1.z + kb * f2.z;
+  }
+  const uint32_t s = slot_of[o];
+  float4 fo = init ? make_float4(0.f, 0.f, 0.f, 0.f) : f[s];
+  fo.x += static_cast<float>(fx);
+  fo.y += static_cast<float>(fy);
+  fo.z += static_cast<float>(fz);
+  fo.w += static_cast<float>(u);
+  f[s] = fo;
+  if (init) {
+    w[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (str) sts[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
 int check_common(const char* entry, const rumdx_particles* p, const void* slot_of, const void* start,
                  const void* list, const void* params, const float* box) {
   if (p == nullptr || slot_of == nullptr || start == nullptr || list == nullptr || params == nullptr ||
@@ -196,16 +306,20 @@ extern "C" int rumdx_slot_index(const rumdx_particles* p, uint32_t* slot_of_dev,
   return check_launch("rumdx_slot_index");
 }
 
-extern "C" int rumdx_bond_harmonic(const rumdx_particles* p, const uint32_t* slot_of_dev,
-                                   const uint32_t* bond_start_dev, const uint32_t* bond_other_dev,
-                                   const uint32_t* bond_type_dev, const float* params_dev, uint32_t ntypes,
-                                   const float box[8], int initialize, int calc_stresses, void* stream) {
-  using namespace rumdx;
-  const int rc = check_common("rumdx_bond_harmonic", p, slot_of_dev, bond_start_dev, bond_other_dev, params_dev, box);
+namespace rumdx {
+namespace {
+
+template <int FORM>
+int launch_bonds(const char* entry, const rumdx_particles* p, const uint32_t* slot_of_dev,
+                 const uint32_t* bond_start_dev, const uint32_t* bond_other_dev, const uint32_t* bond_type_dev,
+                 const float* params_dev, uint32_t ntypes, const float box[8], int initialize, int calc_stresses,
+                 void* stream) {
+  const std::string e(entry);
+  const int rc = check_common(entry, p, slot_of_dev, bond_start_dev, bond_other_dev, params_dev, box);
   if (rc != RUMDX_OK) return rc;
-  if (ntypes == 0) return fail(RUMDX_E_INVALID, "rumdx_bond_harmonic: no bond types");
-  if (calc_stresses && p->sts_dev == nullptr)
-    return fail(RUMDX_E_INVALID, "rumdx_bond_harmonic: stresses requested but sts is null");
+  if (bond_type_dev == nullptr) return fail(RUMDX_E_INVALID, e + ": null argument");
+  if (ntypes == 0) return fail(RUMDX_E_INVALID, e + ": no bond types");
+  if (calc_stresses && p->sts_dev == nullptr) return fail(RUMDX_E_INVALID, e + ": stresses requested but sts is null");
   if (p->n == 0) return RUMDX_OK;
   const Box bx = make_box(box);
   float4* r = reinterpret_cast<float4*>(p->r_dev);
@@ -214,14 +328,52 @@ extern "C" int rumdx_bond_harmonic(const rumdx_particles* p, const uint32_t* slo
   float4* sts = reinterpret_cast<float4*>(p->sts_dev);
   const float2* par = reinterpret_cast<const float2*>(params_dev);
   if (calc_stresses)
-    hipLaunchKernelGGL(k_bond_harmonic<true>, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream), r, f, w,
-                       sts, slot_of_dev, bond_start_dev, bond_other_dev, bond_type_dev, par, ntypes, bx, p->n,
-                       initialize);
+    hipLaunchKernelGGL((k_bond<true, FORM>), dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream), r, f, w, sts,
+                       slot_of_dev, bond_start_dev, bond_other_dev, bond_type_dev, par, ntypes, bx, p->n, initialize);
   else
-    hipLaunchKernelGGL(k_bond_harmonic<false>, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream), r, f, w,
-                       sts, slot_of_dev, bond_start_dev, bond_other_dev, bond_type_dev, par, ntypes, bx, p->n,
-                       initialize);
-  return check_launch("rumdx_bond_harmonic");
+    hipLaunchKernelGGL((k_bond<false, FORM>), dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream), r, f, w, sts,
+                       slot_of_dev, bond_start_dev, bond_other_dev, bond_type_dev, par, ntypes, bx, p->n, initialize);
+  return check_launch(entry);
+}
+
+}  // namespace
+}  // namespace rumdx
+
+extern "C" int rumdx_bond_harmonic(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                                   const uint32_t* bond_start_dev, const uint32_t* bond_other_dev,
+                                   const uint32_t* bond_type_dev, const float* params_dev, uint32_t ntypes,
+                                   const float box[8], int initialize, int calc_stresses, void* stream) {
+  return rumdx::launch_bonds<rumdx::kBondHarmonic>("rumdx_bond_harmonic", p, slot_of_dev, bond_start_dev,
+                                                   bond_other_dev, bond_type_dev, params_dev, ntypes, box, initialize,
+                                                   calc_stresses, stream);
+}
+
+extern "C" int rumdx_bond_fene(const rumdx_particles* p, const uint32_t* slot_of_dev, const uint32_t* bond_start_dev,
+                               const uint32_t* bond_other_dev, const uint32_t* bond_type_dev, const float* params_dev,
+                               uint32_t ntypes, const float box[8], int initialize, int calc_stresses, void* stream) {
+  return rumdx::launch_bonds<rumdx::kBondFene>("rumdx_bond_fene", p, slot_of_dev, bond_start_dev, bond_other_dev,
+                                               bond_type_dev, params_dev, ntypes, box, initialize, calc_stresses,
+                                               stream);
+}
+
+extern "C" int rumdx_dihedral_ryckaert(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                                       const uint32_t* dihedral_start_dev, const uint32_t* dihedral_atoms_dev,
+                                       const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
+                                       int calc_stresses, void* stream) {
+  using namespace rumdx;
+  const int rc = check_common("rumdx_dihedral_ryckaert", p, slot_of_dev, dihedral_start_dev, dihedral_atoms_dev,
+                              params_dev, box);
+  if (rc != RUMDX_OK) return rc;
+  if (ntypes == 0) return fail(RUMDX_E_INVALID, "rumdx_dihedral_ryckaert: no dihedral types");
+  if (calc_stresses && p->sts_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_dihedral_ryckaert: stresses requested but sts is null");
+  if (p->n == 0) return RUMDX_OK;
+  hipLaunchKernelGGL(k_dihedral_ryckaert, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
+                     reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<float4*>(p->f_dev),
+                     reinterpret_cast<float4*>(p->w_dev), reinterpret_cast<float4*>(p->sts_dev), slot_of_dev,
+                     dihedral_start_dev, dihedral_atoms_dev, params_dev, ntypes, make_box(box), p->n, initialize,
+                     calc_stresses);
+  return check_launch("rumdx_dihedral_ryckaert");
 }
 
 extern "C" int rumdx_angle_cossq(const rumdx_particles* p, const uint32_t* slot_of_dev,

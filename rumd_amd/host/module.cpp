@@ -207,6 +207,12 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetParams", &BondHarmonic::GetParams)
       .def("GetExcludedTypes", &BondHarmonic::GetExcludedTypes)
       .def("SetExclusions", &BondHarmonic::SetExclusions, py::arg("non_bond_pot"));
+  py::class_<BondFENE, BondHarmonic>(m, "BondFENE").def(py::init<>());
+  py::class_<DihedralRyckaert, Potential>(m, "DihedralRyckaert")
+      .def(py::init<>())
+      .def("SetParams", &DihedralRyckaert::SetParams, py::arg("dihedral_type"), py::arg("coeffs"))
+      .def("SetExclude", &DihedralRyckaert::SetExclude, py::arg("exclude"))
+      .def("SetExclusions", &DihedralRyckaert::SetExclusions, py::arg("non_bond_pot"));
   py::class_<AngleCosSq, Potential>(m, "AngleCosSq")
       .def(py::init<>())
       .def("SetParams", &AngleCosSq::SetParams, py::arg("angle_type"), py::arg("theta0"), py::arg("ktheta"))
@@ -217,6 +223,9 @@ PYBIND11_MODULE(_rumd, m) {
       .def("ReadTopology", &MoleculeData::ReadTopology, py::arg("filename"))
       .def("SetBonds", &MoleculeData::SetBonds, py::arg("abt"), py::arg("molecule") = std::vector<uint32_t>{})
       .def("SetAngles", &MoleculeData::SetAngles, py::arg("abct"))
+      .def("SetDihedrals", &MoleculeData::SetDihedrals, py::arg("abcdt"))
+      .def("GetNumberOfDihedrals", &MoleculeData::GetNumberOfDihedrals)
+      .def("GetDihedrals", &MoleculeData::GetDihedrals)
       .def("GetNumberOfBonds", &MoleculeData::GetNumberOfBonds)
       .def("GetNumberOfAngles", &MoleculeData::GetNumberOfAngles)
       .def("GetNumberOfMolecules", &MoleculeData::GetNumberOfMolecules)

@@ -46,7 +46,7 @@ double sum_energy(Sample* s, const float* f, const char* cls) {
 void MoleculeData::ReadTopology(const std::string& filename) {
   std::ifstream in(filename);
   if (!in) throw RUMD_Error("MoleculeData", "ReadTopology", "Couldn't open topology file " + filename);
-  std::vector<uint32_t> bonds, angles, mols;
+  std::vector<uint32_t> bonds, angles, dihedrals, mols;
   std::string line, section;
   while (std::getline(in, line)) {
     const size_t b = line.find_first_not_of(" \t\r");
@@ -69,13 +69,19 @@ void MoleculeData::ReadTopology(const std::string& filename) {
       if (!(row >> m >> a >> c >> d >> t))
         throw RUMD_Error("MoleculeData", "ReadTopology", "Format not correct: " + line);
       angles.insert(angles.end(), {a, c, d, t});
+    } else if (section == "dihedrals") {
+      unsigned m, a, c, d, e, t;
+      if (!(row >> m >> a >> c >> d >> e >> t))
+        throw RUMD_Error("MoleculeData", "ReadTopology", "Format not correct: " + line);
+      dihedrals.insert(dihedrals.end(), {a, c, d, e, t});
     }
   }
   SetBonds(bonds, mols);
   SetAngles(angles);
+  SetDihedrals(dihedrals);
   if (sample_->GetVerbose())
-    printf("Topology file %s: %u molecules, %u bonds, %u angles.\n", filename.c_str(), num_mol_, GetNumberOfBonds(),
-           GetNumberOfAngles());
+    printf("Topology file %s: %u molecules, %u bonds, %u angles, %u dihedrals.\n", filename.c_str(), num_mol_,
+           GetNumberOfBonds(), GetNumberOfAngles(), GetNumberOfDihedrals());
 }
 
 void MoleculeData::SetBonds(const std::vector<uint32_t>& abt, const std::vector<uint32_t>& molecule) {
@@ -99,6 +105,40 @@ void MoleculeData::SetAngles(const std::vector<uint32_t>& abct) {
       throw RUMD_Error("MoleculeData", "SetAngles", "angle must name three distinct particles of the sample");
   angles_ = abct;
   UploadAngles();
+}
+
+void MoleculeData::SetDihedrals(const std::vector<uint32_t>& abcdt) {
+  if (abcdt.size() % 5) throw RUMD_Error("MoleculeData", "SetDihedrals", "dihedrals must be (a, b, c, d, type)");
+  const unsigned n = sample_->GetNumberOfParticles();
+  for (size_t k = 0; k < abcdt.size(); k += 5) {
+    for (int u = 0; u < 4; ++u)
+      for (int v = 0; v < 4; ++v)
+        if (abcdt[k + u] >= n || (u < v && abcdt[k + u] == abcdt[k + v]))
+          throw RUMD_Error("MoleculeData", "SetDihedrals", "dihedral must name four distinct particles of the sample");
+  }
+  dihedrals_ = abcdt;
+  UploadDihedrals();
+}
+
+unsigned MoleculeData::GetNumberOfDihedralTypes() const {
+  unsigned t = 0;
+  for (size_t k = 4; k < dihedrals_.size(); k += 5) t = std::max(t, dihedrals_[k] + 1);
+  return t;
+}
+
+void MoleculeData::UploadDihedrals() {
+  const unsigned n = sample_->GetNumberOfParticles();
+  std::vector<std::vector<uint32_t>> per(n);
+  for (size_t k = 0; k < dihedrals_.size(); k += 5)
+    for (int u = 0; u < 4; ++u)
+      per[dihedrals_[k + u]].insert(per[dihedrals_[k + u]].end(), dihedrals_.begin() + k, dihedrals_.begin() + k + 5);
+  std::vector<uint32_t> start, flat;
+  build_csr(n, per, 5, start, flat);
+  const hipStream_t st = sample_->GetStream();
+  upload_vec(dihedral_start_, start, st);
+  upload_vec(dihedral_atoms_, flat, st);
+  if (slot_of_.size() != n) slot_of_.resize(n);
+  sample_->InvalidateGraphs();
 }
 
 unsigned MoleculeData::GetNumberOfBondTypes() const {
@@ -197,13 +237,11 @@ void BondHarmonic::Launch(const rumdx_particles* p, bool initialize, bool calc_s
   MoleculeData* m = sample->GetMoleculeData();
   if (m == nullptr) throw RUMD_Error(GetID_String(), method, "no molecule data: read a topology first");
   if (params_.empty()) throw RUMD_Error(GetID_String(), method, "no bond parameters set");
-  if (m->GetNumberOfBondTypes() > params_.size() / 2)
-    throw RUMD_Error(GetID_String(), method, "a bond type of the topology has no parameters");
   const uint32_t* slot_of = m->SlotIndex();
-  rumdx_check(rumdx_bond_harmonic(p, slot_of, m->BondStart(), m->BondOther(), m->BondType(), d_params_.get(),
-                                  static_cast<uint32_t>(params_.size() / 2),
-                                  sample->GetSimulationBox()->GetBox8(), initialize ? 1 : 0, calc_stresses ? 1 : 0,
-                                  sample->GetStream()),
+  const auto entry = fene_ ? rumdx_bond_fene : rumdx_bond_harmonic;
+  rumdx_check(entry(p, slot_of, m->BondStart(), m->BondOther(), m->BondType(), d_params_.get(),
+                    static_cast<uint32_t>(params_.size() / 2), sample->GetSimulationBox()->GetBox8(),
+                    initialize ? 1 : 0, calc_stresses ? 1 : 0, sample->GetStream()),
               GetID_String().c_str(), method);
 }
 
@@ -274,4 +312,57 @@ double AngleCosSq::GetPotentialEnergy() {
   p.w_dev = pe_w_.get();
   Launch(&p, true, false, "GetPotentialEnergy");
   return sum_energy(sample, pe_f_.get(), "AngleCosSq");
+}
+
+// ------------------------------------------------------------------ DihedralRyckaert
+// DihedralPotential::SetParams (src/DihedralPotential.cu:31-38): six
+// coefficients per dihedral type.
+void DihedralRyckaert::SetParams(unsigned dihedral_type, const std::vector<float>& coeffs) {
+  if (coeffs.size() != 6) throw RUMD_Error(GetID_String(), "SetParams", "Wrong number of parameters in SetParams");
+  if (params_.size() < 6ull * (dihedral_type + 1)) params_.resize(6ull * (dihedral_type + 1), 0.f);
+  std::copy(coeffs.begin(), coeffs.end(), params_.begin() + 6ull * dihedral_type);
+  d_params_.resize(params_.size());
+  d_params_.upload(params_.data(), params_.size(), sample ? sample->GetStream() : nullptr);
+  if (sample) sample->InvalidateGraphs();
+}
+
+void DihedralRyckaert::SetExclusions(PairPotential* non_bond_pot) {
+  if (!exclude_dihedral_) return;
+  if (!sample || sample->GetMoleculeData() == nullptr)
+    throw RUMD_Error(GetID_String(), "SetExclusions", "no molecule data: read a topology first");
+  const std::vector<uint32_t>& d = sample->GetMoleculeData()->GetDihedrals();
+  std::vector<uint32_t> pairs;
+  for (size_t k = 0; k < d.size(); k += 5) pairs.insert(pairs.end(), {d[k], d[k + 3]});
+  if (!pairs.empty()) non_bond_pot->GetNeighborList().AddExclusions(pairs);
+}
+
+void DihedralRyckaert::Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method) {
+  if (!sample) throw RUMD_Error(GetID_String(), method, "no sample");
+  MoleculeData* m = sample->GetMoleculeData();
+  if (m == nullptr) throw RUMD_Error(GetID_String(), method, "no molecule data: read a topology first");
+  if (params_.empty()) throw RUMD_Error(GetID_String(), method, "no dihedral parameters set");
+  if (m->GetNumberOfDihedralTypes() > params_.size() / 6)
+    throw RUMD_Error(GetID_String(), method, "a dihedral type of the topology has no parameters");
+  const uint32_t* slot_of = m->SlotIndex();
+  rumdx_check(rumdx_dihedral_ryckaert(p, slot_of, m->DihedralStart(), m->DihedralAtoms(), d_params_.get(),
+                                      static_cast<uint32_t>(params_.size() / 6), sample->GetSimulationBox()->GetBox8(),
+                                      initialize ? 1 : 0, calc_stresses ? 1 : 0, sample->GetStream()),
+              GetID_String().c_str(), method);
+}
+
+void DihedralRyckaert::CalcF(bool initialize, bool calc_stresses) {
+  Launch(sample->GetParticleData().Descriptor(), initialize, calc_stresses, "CalcF");
+}
+
+double DihedralRyckaert::GetPotentialEnergy() {
+  const unsigned nvp = sample->GetNumberOfVirtualParticles();
+  if (pe_f_.size() != 4ull * nvp) {
+    pe_f_.resize(4ull * nvp);
+    pe_w_.resize(4ull * nvp);
+  }
+  rumdx_particles p = *sample->GetParticleData().Descriptor();
+  p.f_dev = pe_f_.get();
+  p.w_dev = pe_w_.get();
+  Launch(&p, true, false, "GetPotentialEnergy");
+  return sum_energy(sample, pe_f_.get(), "DihedralRyckaert");
 }

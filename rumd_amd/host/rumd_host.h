@@ -396,6 +396,13 @@ class MoleculeData {
   void SetBonds(const std::vector<uint32_t>& abt, const std::vector<uint32_t>& molecule = {});
   // Replaces the angles: flat (a, b, c, type) quadruples.
   void SetAngles(const std::vector<uint32_t>& abct);
+  // Replaces the dihedrals: flat (a, b, c, d, type) quintuples.
+  void SetDihedrals(const std::vector<uint32_t>& abcdt);
+  unsigned GetNumberOfDihedrals() const { return static_cast<unsigned>(dihedrals_.size() / 5); }
+  unsigned GetNumberOfDihedralTypes() const;
+  const std::vector<uint32_t>& GetDihedrals() const { return dihedrals_; }
+  const uint32_t* DihedralStart() const { return dihedral_start_.get(); }
+  const uint32_t* DihedralAtoms() const { return dihedral_atoms_.get(); }
   unsigned GetNumberOfBonds() const { return static_cast<unsigned>(bonds_.size() / 3); }
   unsigned GetNumberOfAngles() const { return static_cast<unsigned>(angles_.size() / 4); }
   unsigned GetNumberOfMolecules() const { return num_mol_; }
@@ -414,14 +421,18 @@ class MoleculeData {
  private:
   void UploadBonds();
   void UploadAngles();
+  void UploadDihedrals();
   Sample* sample_;
   unsigned num_mol_ = 0;
-  std::vector<uint32_t> bonds_, angles_;
+  std::vector<uint32_t> bonds_, angles_, dihedrals_;
   DeviceArray<uint32_t> bond_start_, bond_other_, bond_type_, angle_start_, angle_atoms_, slot_of_;
+  DeviceArray<uint32_t> dihedral_start_, dihedral_atoms_;
 };
 
 // BondHarmonic (include/rumd/BondPotential.h, src/BondPotential.cu:70-135):
-// u = k/2 (r - l0)^2 per bond of the sample's topology.
+// u = k/2 (r - l0)^2 per bond of the sample's topology whose type has
+// parameters here (a bond type belongs to the bond potential whose SetParams
+// named it, as in the reference).
 class BondHarmonic : public Potential {
  public:
   BondHarmonic() { SetID_String("bondHarmonic"); }
@@ -440,8 +451,45 @@ class BondHarmonic : public Potential {
 
  private:
   void Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method);
-  std::vector<float> params_;  // (l0, k) per bond type
+
+ protected:
+  bool fene_ = false;
+  std::vector<float> params_;  // (l0 or R0, k) per bond type; k = 0: not this potential's type
   std::vector<bool> exclude_;
+  DeviceArray<float> d_params_;
+  DeviceArray<float> pe_f_, pe_w_;
+};
+
+// BondFENE (include/rumd/BondPotential.h:47-56, src/BondPotential.cu):
+// u = -k R0^2/2 ln(1 - r^2/R0^2); SetParams(bond_type, R0, k, exclude).
+class BondFENE : public BondHarmonic {
+ public:
+  BondFENE() {
+    SetID_String("bondFENE");
+    fene_ = true;
+  }
+};
+
+// DihedralRyckaert (include/rumd/DihedralPotential.h, src/DihedralPotential.cu:78-110):
+// u = sum_k c_k cos^k(psi) per dihedral of the topology.
+class DihedralRyckaert : public Potential {
+ public:
+  DihedralRyckaert() { SetID_String("DihedralRyckaert"); }
+  void SetParams(unsigned dihedral_type, const std::vector<float>& coeffs);
+  void CalcF(bool initialize, bool calc_stresses) override;
+  bool EnergyIncludedInParticleSum() const override { return true; }
+  void Initialize() override {}
+  double GetPotentialEnergy() override;
+  bool IsMolecularPotential() const override { return true; }
+  // DihedralPotential::SetExclude / SetExclusions: the end atoms (a, d) leave
+  // the pair potential's list (on by default).
+  void SetExclude(bool exclude) { exclude_dihedral_ = exclude; }
+  void SetExclusions(PairPotential* non_bond_pot);
+
+ private:
+  void Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method);
+  std::vector<float> params_;  // c_0..c_5 per type
+  bool exclude_dihedral_ = true;
   DeviceArray<float> d_params_;
   DeviceArray<float> pe_f_, pe_w_;
 };

@@ -33,6 +33,8 @@ Pot_Yukawa = _rumd.Pot_Yukawa
 Pot_Exp = _rumd.Pot_Exp
 BondHarmonic = _rumd.BondHarmonic
 AngleCosSq = _rumd.AngleCosSq
+BondFENE = _rumd.BondFENE
+DihedralRyckaert = _rumd.DihedralRyckaert
 MoleculeData = _rumd.MoleculeData
 Integrator = _rumd.Integrator
 IntegratorNVT = _rumd.IntegratorNVT
@@ -214,6 +216,6 @@ class Sample(_rumd.Sample):
 __all__ = [
     "Device", "SimulationBox", "RectangularSimulationBox", "Potential", "PairPotential", "Pot_IPL_12_6",
     "Pot_LJ_12_6", "Pot_gLJ_m12_n6", "Pot_LJ_12_6_smooth_2_4", "Pot_gLJ_m_n", "Pot_Gauss", "Pot_Buckingham",
-    "Pot_IPL_n", "Pot_Yukawa", "Pot_Exp", "BondHarmonic", "AngleCosSq", "MoleculeData", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
+    "Pot_IPL_n", "Pot_Yukawa", "Pot_Exp", "BondHarmonic", "BondFENE", "AngleCosSq", "DihedralRyckaert", "MoleculeData", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
     "ShiftedForce", "GetVersion", "NumberOfPairPotentialParameters", "Sample", "ReplicaBatch",
 ]

@@ -223,3 +223,71 @@ def test_simulation_sets_exclusions_and_runs(golden_dir, tmp_path, monkeypatch):
     sim.Run(300, suppressAllOutput=True)
     assert np.isfinite(sim.sample.GetPositions()).all()
     assert np.isfinite(sim.sample.GetPotentialEnergy())
+
+
+DIH_PARAMS = [(0.9, 2.1, -0.4, -3.0, 0.2, 0.1), (1.2, -1.0, 0.6, 0.9, -0.5, 0.3)]
+
+
+def _dihedrals():
+    return np.array([[5 * m + i, 5 * m + i + 1, 5 * m + i + 2, 5 * m + i + 3, (m + i) % 2]
+                     for m in range(200) for i in range(2)], np.int32)
+
+
+@pytest.mark.parametrize("method", ["n2", "sort"])
+def test_fene_and_harmonic_types_and_dihedrals(golden_dir, method):
+    """Bond type 0 through BondFENE, type 1 through BondHarmonic (a type belongs
+    to the potential that set it), Ryckaert dihedrals with their end pairs
+    excluded from the LJ list, all against the oracle."""
+    c = load_golden(golden_dir / "pentamer200_rho0.85_T0.5.xyz.gz")
+    s = rumd.Sample(0, 0)
+    s.SetVerbose(False)
+    s.SetConfiguration(c)
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedPotential)
+    pot.SetVerbose(False)
+    pot.SetParams(0, 0, Sigma=1.0, Epsilon=1.0, Rcut=2.5)
+    pot.SetNbListSkin(0.3)
+    pot.SetNB_Method(method)
+    s.SetPotential(pot)
+    s.ReadMoleculeData(str(golden_dir / "pentamer200_rho0.85_T0.5.top"))
+    md = s.GetMoleculeData()
+    dih = _dihedrals()
+    md.SetDihedrals(dih.reshape(-1).tolist())
+    fene = rumd.BondFENE()
+    fene.SetVerbose(False)
+    fene.SetParams(bond_type=0, bond_length=1.5, stiffness=30.0)
+    harm = rumd.BondHarmonic()
+    harm.SetVerbose(False)
+    harm.SetParams(bond_type=1, bond_length=1.1, stiffness=150.0)
+    rb = rumd.DihedralRyckaert()
+    rb.SetVerbose(False)
+    for t, cf in enumerate(DIH_PARAMS):
+        rb.SetParams(dihedral_type=t, coeffs=list(cf))
+    for q in (fene, harm, rb):
+        s.AddPotential(q)
+    rb.SetExclusions(pot)
+    assert pot.GetNumberOfExclusions() == len(dih)
+    print(f"[case] FENE + harmonic + dihedrals, list {method}", flush=True)
+    s.CalcF()
+    f = s.GetForcesAndEnergies().astype(np.float64)
+    bonds, _ = _topology(golden_dir)
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    table = configs.lj_param_table([(0, 0, 1.0, 1.0, 2.5)], 1)
+    excl = {(min(a, d), max(a, d)) for a, _, _, d, _ in dih}
+    f_lj, vir_lj, _ = _oracle_lj_excluding(c, pos4, table, excl)
+    f_f, vir_f = oracle.bonded(pos4, c.box, bonds[bonds[:, 2] == 0], [(1.5, 30.0), (0.0, 0.0)], fene=True)
+    f_h, vir_h = oracle.bonded(pos4, c.box, bonds[bonds[:, 2] == 1], [(0.0, 0.0), (1.1, 150.0)])
+    f_d, _ = oracle.bonded(pos4, c.box, dihedrals=dih, dihedral_params=DIH_PARAMS)
+    f_ref = f_lj + f_f + f_h + f_d
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
+    np.testing.assert_allclose(f[:, 3], f_ref[:, 3], rtol=0, atol=5e-5)
+    assert fene.GetPotentialEnergy() == pytest.approx(f_f[:, 3].sum(), rel=2e-6)
+    assert harm.GetPotentialEnergy() == pytest.approx(f_h[:, 3].sum(), rel=2e-6)
+    assert rb.GetPotentialEnergy() == pytest.approx(f_d[:, 3].sum(), rel=2e-5, abs=1e-4)
+    assert s.GetVirial() == pytest.approx((vir_lj.sum() + vir_f.sum() + vir_h.sum()) / 6.0, rel=1e-5)
+    # graph-captured steps with every term equal the eager loop
+    itg = rumd.IntegratorNVE(timeStep=0.002)
+    s.SetIntegrator(itg)
+    s.RunSteps(0, 100, 10, 0)
+    s.Synchronize()
+    assert np.isfinite(s.GetPositions()).all()

@@ -81,3 +81,74 @@ def test_bonded_total_force_and_torque_vanish():
     np.testing.assert_allclose(f4[:, :3].sum(axis=0), 0.0, atol=1e-12 * scale)
     torque = np.cross(x.astype(np.float64), f4[:, :3]).sum(axis=0)
     np.testing.assert_allclose(torque, 0.0, atol=1e-10 * scale)
+
+
+def test_fene_closed_form():
+    k, r0 = 30.0, float(np.float32(1.5))
+    for r in (0.7, 0.97, 1.3):
+        p = _pos4([[0.0, 0.0, 0.0], [r, 0.0, 0.0]])
+        r32 = float(np.float32(r))
+        f4, vir = oracle.bonded(p, L, bonds=[[0, 1, 0]], bond_params=[[r0, k]], fene=True)
+        x = (r32 / r0) ** 2
+        assert f4[:, 3].sum() == pytest.approx(-0.5 * k * r0 ** 2 * np.log(1 - x), rel=1e-12)
+        # always attractive: -du/dr = -k r / (1 - x) on particle 1 along +x
+        assert f4[1, 0] == pytest.approx(-k * r32 / (1 - x), rel=1e-12)
+        assert vir[1] == pytest.approx(-k * r32 ** 2 / (1 - x), rel=1e-12)
+
+
+def _chain4(phi):
+    """Four atoms with unit bonds, 109.5 degree bond angles and IUPAC torsion phi."""
+    th = np.deg2rad(109.5)
+    b = np.array([0.0, 0.0, 0.0])
+    c = np.array([1.0, 0.0, 0.0])
+    a = b + np.array([-np.cos(np.pi - th), np.sin(np.pi - th), 0.0])
+    d0 = np.array([np.cos(np.pi - th), np.sin(np.pi - th), 0.0])  # cis to a
+    d = c + np.array([d0[0], d0[1] * np.cos(phi), d0[1] * np.sin(phi)])
+    return np.array([a, b, c, d], np.float32)
+
+
+COEFFS = [[1.1, -2.0, 0.7, 1.5, -0.3, 0.2]]
+
+
+def test_dihedral_trans_and_cis_closed_form():
+    c = np.array(COEFFS[0])
+    for phi, cos_psi in ((np.pi, 1.0), (0.0, -1.0), (np.pi / 2, 0.0)):
+        f4, vir = oracle.bonded(_pos4(_chain4(phi)), L, dihedrals=[[0, 1, 2, 3, 0]], dihedral_params=COEFFS)
+        want = sum(c[k] * cos_psi ** k for k in range(6))
+        assert f4[0, 3] == pytest.approx(want, rel=1e-6, abs=1e-6), phi
+        assert f4[1:, 3].sum() == 0.0 and not vir.any()
+
+
+def test_dihedral_forces_are_minus_the_energy_gradient():
+    rng = np.random.default_rng(5)
+    for phi in (0.4, 1.9, -2.6):
+        x = _chain4(phi) + rng.normal(0.0, 0.05, size=(4, 3)).astype(np.float32)
+
+        def energy(xx):
+            f4, _ = oracle.bonded(_pos4(xx), L, dihedrals=[[0, 1, 2, 3, 0]], dihedral_params=COEFFS)
+            return f4[:, 3].sum()
+
+        f4, _ = oracle.bonded(_pos4(x), L, dihedrals=[[0, 1, 2, 3, 0]], dihedral_params=COEFFS)
+        assert np.abs(f4[:, :3].sum(axis=0)).max() < 1e-10 * max(1.0, np.abs(f4).max())
+        torque = np.cross(x.astype(np.float64), f4[:, :3]).sum(axis=0)
+        assert np.abs(torque).max() < 1e-9 * max(1.0, np.abs(f4).max())
+        h = 1e-3
+        for i in range(4):
+            for c in range(3):
+                xp, xm = x.copy(), x.copy()
+                xp[i, c] += h
+                xm[i, c] -= h
+                g = (energy(xp) - energy(xm)) / (float(xp[i, c]) - float(xm[i, c]))
+                assert f4[i, c] == pytest.approx(-g, rel=2e-4, abs=2e-4), (phi, i, c)
+
+
+def test_fene_force_is_minus_the_energy_slope():
+    k, r0 = 30.0, 1.5
+    for r in (0.8, 1.2):
+        h = 1e-4
+        es = []
+        for rr in (r - h, r + h):
+            f4, _ = oracle.bonded(_pos4([[0, 0, 0], [rr, 0, 0]]), L, [[0, 1, 0]], [[r0, k]], fene=True)
+            es.append((f4[:, 3].sum(), float(np.float32(rr))))
+        f4, _ = oracle.bonded(_pos4([[0, 0, 0], [r, 0, 0]]), L, [[0, 1, 0]], [[r0, k]], fene=True)
+        assert f4[1, 0] == pytest.approx(-(es[1][0] - es[0][0]) / (es[1][1] - es[0][1]), rel=1e-4)
