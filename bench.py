@@ -278,6 +278,62 @@ def batched_leg(nrep: int, warmup: int, steps: int, sample_reps: int, single_ste
     }
 
 
+def polymer_leg(warmup: int, steps: int, block: int) -> dict:
+    """The polymer workload: 10^4 chains of 10 beads (N = 100,000, rho = 1.0),
+    WCA pairs (LJ cut and shifted at 2^(1/6)) without the 1-2 pairs, harmonic
+    bonds k = 3000, l0 = 1, cos^2 angles k = 10 with theta0 = pi (the workload
+    does not fix theta0), NVT T = 1.0, tau = 0.2, dt = 0.002; chains start
+    straight on a lattice."""
+    import rumd_amd.rumd as rumd
+    from rumd_amd import configs, conf_io
+    sysm, bonds, angles = configs.polymer_melt()
+    conf = conf_io.Configuration(n=sysm.n, box=sysm.box, types=sysm.types, positions=sysm.positions,
+                                 masses=sysm.masses, velocities=sysm.velocities, images=np.zeros_like(sysm.positions))
+    s = rumd.Sample(0, 0)
+    s.SetVerbose(False)
+    s.SetConfiguration(conf)
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedPotential)
+    pot.SetVerbose(False)
+    pot.SetParams(0, 0, Sigma=1.0, Epsilon=1.0, Rcut=2.0 ** (1.0 / 6.0))
+    pot.SetNbListSkin(0.3)
+    s.SetPotential(pot)
+    md = s.CreateMoleculeData()
+    md.SetBonds(bonds.reshape(-1).tolist())
+    md.SetAngles(angles.reshape(-1).tolist())
+    bond = rumd.BondHarmonic()
+    bond.SetVerbose(False)
+    bond.SetParams(bond_type=0, bond_length=1.0, stiffness=3000.0, exclude=True)
+    ang = rumd.AngleCosSq()
+    ang.SetVerbose(False)
+    ang.SetParams(angle_type=0, theta0=float(np.pi), ktheta=10.0)
+    ang.SetExclude(False)
+    s.AddPotential(bond)
+    s.AddPotential(ang)
+    bond.SetExclusions(pot)
+    itg = rumd.IntegratorNVT(timeStep=0.002, targetTemperature=1.0, thermostatRelaxationTime=0.2)
+    s.SetIntegrator(itg)
+    s.RunSteps(0, warmup, block, 100)
+    s.Synchronize()
+    b0 = pot.GetNumberOfNbListBuilds()
+    t0 = time.perf_counter()
+    s.RunSteps(warmup, steps, block, 100)
+    s.Synchronize()
+    el = time.perf_counter() - t0
+    n = sysm.n
+    out = {
+        "value": round(n * steps / el, 1), "unit": "atom-timesteps/s", "ms_per_step": round(el / steps * 1e3, 5),
+        "workload": "10^4 chains x 10 beads N=100000 rho=1.0: WCA without 1-2 pairs, harmonic bonds k=3000 l0=1, "
+                    "cos^2 angles k=10 theta0=pi, NVT T=1.0 tau=0.2 dt=0.002, sorted lists skin 0.3",
+        "steps": steps, "warmup": warmup, "list_builds": pot.GetNumberOfNbListBuilds() - b0,
+        "pe_per_particle": round(s.GetPotentialEnergy() / n, 6),
+        "bond_energy_per_particle": round(bond.GetPotentialEnergy() / n, 6),
+        "angle_energy_per_particle": round(ang.GetPotentialEnergy() / n, 6),
+        "temperature": round(2.0 * itg.GetKineticEnergy() / (3 * n - 3), 4),
+    }
+    print(f"[bench] polymer: {out['value'] / 1e9:.3f} G atom-steps/s", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-counters", type=int, default=0, metavar="N",
@@ -296,6 +352,8 @@ def main() -> int:
     ap.add_argument("--replicas", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--single-replicas", type=int, default=32, help="replicas timed one at a time")
+    ap.add_argument("--polymer-only", action="store_true",
+                    help="run only the polymer workload and print its figures (for profiling)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched-replica workload and print its figures (for profiling)")
     args = ap.parse_args()
@@ -304,6 +362,10 @@ def main() -> int:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               "(the single large system would shard by spatial domains, see the design notes).", file=sys.stderr)
         return 2
+
+    if args.polymer_only:
+        print(json.dumps({"polymer": polymer_leg(500, args.steps, args.block)}), flush=True)
+        return 0
 
     if args.batched_only:
         print(json.dumps({"batched": batched_leg(args.replicas, 100, args.batch_steps, args.single_replicas, 100)}),
@@ -378,6 +440,8 @@ def main() -> int:
         extras["nve_drift"] = drift
         print("[bench] batched replicas ...", file=sys.stderr, flush=True)
         extras["batched"] = batched_leg(args.replicas, 100, args.batch_steps, args.single_replicas, 100)
+        print("[bench] polymer melt ...", file=sys.stderr, flush=True)
+        extras["polymer"] = polymer_leg(500, 2000, args.block)
 
     cpu = None
     if not args.no_cpu_baseline:

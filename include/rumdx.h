@@ -66,7 +66,7 @@ extern "C" {
 #define RUMDX_FLAG_DONE 4      /* internal completion counter, zero between calls */
 #define RUMDX_FLAG_MAXHALO 5   /* largest tile halo (particles) of the last sorted build */
 #define RUMDX_FLAG_ERROR 6     /* nonzero: a tile halo exceeded 65535 particles (list unusable) */
-#define RUMDX_FLAG_EXCL_SEEN 7 /* internal: value of BUILDS when exclusions were last applied */
+#define RUMDX_FLAG_EXCL_PENDING 7 /* internal: set by a completed build, cleared once its lists' exclusions are applied */
 
 /* Device-resident particle arrays (ParticleData, include/rumd/ParticleData.h:41-62). */
 typedef struct rumdx_particles {

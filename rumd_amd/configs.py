@@ -123,6 +123,39 @@ def ka_replica_set(nrep: int = 512, n_min: int = 256, n_max: int = 8192, t_min: 
     return systems, [float(t) for t in ts]
 
 
+def polymer_melt(n_chains: int = 10000, beads: int = 10, density: float = 1.0, temperature: float = 1.0,
+                 seed: int = 5001, grid=(50, 50, 40)):
+    """Linear bead-spring chains on a lattice (the polymer workload: 10^4
+    chains of 10 beads, rho = 1.0, N = 100,000).  Sites of a grid[0] x grid[1]
+    x grid[2] lattice filling the cubic box; each chain takes `beads`
+    consecutive sites along x, so bonds start at the x spacing (L / grid[0]).
+    Returns (System, bonds (m, 3) as (a, b, 0), angles (m, 4) as (a, b, c, 0)).
+    Velocities are Maxwellian at `temperature` (seed) with zero total momentum."""
+    gx, gy, gz = grid
+    n = n_chains * beads
+    if gx * gy * gz != n or gx % beads:
+        raise ValueError("the lattice must hold the chains exactly, with whole chains along x")
+    L = (n / density) ** (1.0 / 3.0)
+    ix, iy, iz = np.meshgrid(np.arange(gx), np.arange(gy), np.arange(gz), indexing="ij")
+    # x fastest within a chain: order sites by (z, y, x)
+    order = np.lexsort((ix.ravel(), iy.ravel(), iz.ravel()))
+    pos = np.stack([(ix.ravel()[order] + 0.5) * L / gx, (iy.ravel()[order] + 0.5) * L / gy,
+                    (iz.ravel()[order] + 0.5) * L / gz], axis=1) - L / 2
+    rng = np.random.default_rng(seed)
+    types = np.zeros(n, np.int32)
+    masses = np.ones(1)
+    vel = _maxwell(rng, n, temperature, masses[types])
+    first = np.arange(n_chains) * beads
+    bonds = np.stack([(first[:, None] + np.arange(beads - 1)).ravel(),
+                      (first[:, None] + np.arange(1, beads)).ravel(), np.zeros(n_chains * (beads - 1), int)], axis=1)
+    angles = np.stack([(first[:, None] + np.arange(beads - 2)).ravel(),
+                       (first[:, None] + np.arange(1, beads - 1)).ravel(),
+                       (first[:, None] + np.arange(2, beads)).ravel(), np.zeros(n_chains * (beads - 2), int)], axis=1)
+    sysm = System(_wrap(pos, np.array([L, L, L])).astype(np.float32), vel.astype(np.float32), types,
+                  np.array([L, L, L]), masses, name=f"polymer_{n_chains}x{beads}")
+    return sysm, bonds.astype(np.int64), angles.astype(np.int64)
+
+
 def ka_params(rcut_in_sigma: float = 2.5):
     """(i, j, sigma, epsilon, rcut-in-units-of-sigma) rows for Pot_LJ_12_6.SetParams."""
     return [(i, j, KA_SIGMA[(i, j)], KA_EPSILON[(i, j)], rcut_in_sigma) for i in (0, 1) for j in (0, 1)]

@@ -87,6 +87,7 @@ __device__ __forceinline__ void finish_build(uint32_t* flags) {
   if (last_to_arrive(flags + RUMDX_FLAG_DONE, gridDim.x) && threadIdx.x == 0) {
     flags[RUMDX_FLAG_REBUILD] = 0u;
     flags[RUMDX_FLAG_BUILDS] += 1u;
+    flags[RUMDX_FLAG_EXCL_PENDING] = 1u;
   }
 }
 
@@ -771,17 +772,18 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
 // entries of the particles its owner must not interact with, in place and in
 // order.  Entries are decoded through the tile's halo slot list, so tiles whose
 // halo was not staged (their force kernel rescans cells and skips exclusions
-// there) and overflowed lists are left alone.  Every block arrives at the
-// completion counter; the last one records the build as seen.
+// there) and overflowed lists are left alone.  A completed build raises
+// flags[EXCL_PENDING]; without it every block returns at once (no atomics on
+// the many idle steps).  With it, every block arrives at the completion
+// counter and the last one clears the flag, so no block can miss it.
 __global__ __launch_bounds__(kBlock) void k_apply_exclusions(uint32_t* __restrict__ flags,
                                                              const uint4* __restrict__ tile_info,
                                                              const uint32_t* __restrict__ tile_halo, uint32_t halo_cap,
                                                              Exclusions ex, uint16_t* __restrict__ nbl16,
                                                              uint32_t* __restrict__ num_nbrs, uint32_t nvp,
                                                              uint32_t max_nbrs) {
-  const uint32_t builds = __hip_atomic_load(flags + RUMDX_FLAG_BUILDS, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t seen = __hip_atomic_load(flags + RUMDX_FLAG_EXCL_SEEN, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (builds != seen) {
+  if (__hip_atomic_load(flags + RUMDX_FLAG_EXCL_PENDING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  {
     const uint32_t t = blockIdx.x;
     const uint4 info = tile_info[t];
     const bool staged = (info.y >> 16) == halo_cap && info.x <= halo_cap;
@@ -804,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_exclusions(uint32_t* __restric
       num_nbrs[i] = kept;
     }
   }
-  if (last_to_arrive(flags + RUMDX_FLAG_DONE, gridDim.x) && threadIdx.x == 0) flags[RUMDX_FLAG_EXCL_SEEN] = builds;
+  if (last_to_arrive(flags + RUMDX_FLAG_DONE, gridDim.x) && threadIdx.x == 0) flags[RUMDX_FLAG_EXCL_PENDING] = 0u;
 }
 
 // Converts tile-format entries (halo offsets) back to slots.

@@ -138,6 +138,7 @@ void MoleculeData::UploadDihedrals() {
   upload_vec(dihedral_start_, start, st);
   upload_vec(dihedral_atoms_, flat, st);
   if (slot_of_.size() != n) slot_of_.resize(n);
+  slot_epoch_ = 0;
   sample_->InvalidateGraphs();
 }
 
@@ -171,6 +172,7 @@ void MoleculeData::UploadBonds() {
   upload_vec(bond_other_, flat_o, st);
   upload_vec(bond_type_, flat_t, st);
   if (slot_of_.size() != n) slot_of_.resize(n);
+  slot_epoch_ = 0;
   sample_->InvalidateGraphs();
 }
 
@@ -187,6 +189,7 @@ void MoleculeData::UploadAngles() {
   upload_vec(angle_start_, start, st);
   upload_vec(angle_atoms_, flat, st);
   if (slot_of_.size() != n) slot_of_.resize(n);
+  slot_epoch_ = 0;
   sample_->InvalidateGraphs();
 }
 
@@ -195,8 +198,13 @@ const uint32_t* MoleculeData::SlotIndex() {
   // allocated with the lists, never here: this runs inside captured graphs
   if (slot_of_.size() != n)
     throw RUMD_Error("MoleculeData", "SlotIndex", "the number of particles changed since the topology was set");
-  rumdx_check(rumdx_slot_index(sample_->GetParticleData().Descriptor(), slot_of_.get(), sample_->GetStream()),
-              "MoleculeData", "SlotIndex");
+  // once per force call (the bonded potentials of a step share it; under a
+  // graph capture this decides what is recorded)
+  if (slot_epoch_ != sample_->GetSlotEpoch()) {
+    rumdx_check(rumdx_slot_index(sample_->GetParticleData().Descriptor(), slot_of_.get(), sample_->GetStream()),
+                "MoleculeData", "SlotIndex");
+    slot_epoch_ = sample_->GetSlotEpoch();
+  }
   return slot_of_.get();
 }
 

@@ -282,8 +282,10 @@ void NeighborList::UpdateNBlist() {
   const hipStream_t st = sample_->GetStream();
   if (!fused_check_pending_) rumdx_check(rumdx_nblist_check(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
   fused_check_pending_ = false;
-  if (method == SORT)
+  if (method == SORT) {
     rumdx_check(rumdx_nblist_build_sorted(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
+    sample_->BumpSlotEpoch();  // the build may have moved particles between slots
+  }
   else
     rumdx_check(rumdx_nblist_build_n2(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
 }

@@ -425,6 +425,7 @@ class MoleculeData {
   Sample* sample_;
   unsigned num_mol_ = 0;
   std::vector<uint32_t> bonds_, angles_, dihedrals_;
+  uint64_t slot_epoch_ = 0;  // sample epoch of the last slot index
   DeviceArray<uint32_t> bond_start_, bond_other_, bond_type_, angle_start_, angle_atoms_, slot_of_;
   DeviceArray<uint32_t> dihedral_start_, dihedral_atoms_;
 };
@@ -665,6 +666,11 @@ class Sample {
   void* Workspace() { return workspace_.get(); }
   double* ScalarOut() { return scalars_.get(); }
   const std::vector<Potential*>& GetPotentials() const { return potentials_; }
+  // Counts events that may change which slot holds which particle (force
+  // calls, sorted list updates); a slot index computed at the same count is
+  // still valid.
+  uint64_t GetSlotEpoch() const { return slot_epoch_; }
+  void BumpSlotEpoch() { ++slot_epoch_; }
   // Topology of the sample's molecules (Sample::ReadMoleculeData); null until read.
   void ReadMoleculeData(const std::string& topology_file);
   MoleculeData* GetMoleculeData();
@@ -689,4 +695,5 @@ class Sample {
   // captured step blocks: key = (block size, momentum reset at first step)
   std::map<std::pair<unsigned, bool>, hipGraphExec_t> graphs_;
   std::unique_ptr<MoleculeData> molecules_;
+  uint64_t slot_epoch_ = 1;
 };

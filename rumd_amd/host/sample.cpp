@@ -131,6 +131,7 @@ void Sample::CalcF(bool calc_stresses) {
   if (!box_) throw RUMD_Error("Sample", "CalcF", "No simulation box has been set");
   if (!GetNumberOfParticles()) throw RUMD_Error("Sample", "CalcF", "Sample has no particles");
   if (potentials_.empty()) std::cerr << "Warning [Sample::CalcF]: No potential has been set" << std::endl;
+  BumpSlotEpoch();
   for (size_t k = 0; k < potentials_.size(); ++k) potentials_[k]->CalcF(k == 0, calc_stresses);
 }
 

@@ -291,3 +291,52 @@ def test_fene_and_harmonic_types_and_dihedrals(golden_dir, method):
     s.RunSteps(0, 100, 10, 0)
     s.Synchronize()
     assert np.isfinite(s.GetPositions()).all()
+
+
+def test_polymer_workload_small_against_oracle():
+    """The polymer workload's force field (WCA without 1-2 pairs, stiff
+    harmonic bonds, cos^2 angles) on a small version, after some NVT steps
+    have bent the chains, against the oracle (1000 beads on a cubic lattice
+    of spacing 1)."""
+    from rumd_amd import conf_io
+    sysm, bonds, angles = configs.polymer_melt(n_chains=100, beads=10, grid=(10, 10, 10))
+    conf = conf_io.Configuration(n=sysm.n, box=sysm.box, types=sysm.types, positions=sysm.positions,
+                                 masses=sysm.masses, velocities=sysm.velocities, images=np.zeros_like(sysm.positions))
+    s = rumd.Sample(0, 0)
+    s.SetVerbose(False)
+    s.SetConfiguration(conf)
+    rc = float(np.float32(2.0 ** (1.0 / 6.0)))
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedPotential)
+    pot.SetVerbose(False)
+    pot.SetParams(0, 0, Sigma=1.0, Epsilon=1.0, Rcut=2.0 ** (1.0 / 6.0))
+    pot.SetNbListSkin(0.3)
+    pot.SetNB_Method("sort")
+    s.SetPotential(pot)
+    md = s.CreateMoleculeData()
+    md.SetBonds(bonds.reshape(-1).tolist())
+    md.SetAngles(angles.reshape(-1).tolist())
+    bond = rumd.BondHarmonic()
+    bond.SetVerbose(False)
+    bond.SetParams(bond_type=0, bond_length=1.0, stiffness=3000.0, exclude=True)
+    ang = rumd.AngleCosSq()
+    ang.SetVerbose(False)
+    ang.SetParams(angle_type=0, theta0=float(np.pi), ktheta=10.0)
+    s.AddPotential(bond)
+    s.AddPotential(ang)
+    bond.SetExclusions(pot)
+    s.SetIntegrator(rumd.IntegratorNVT(timeStep=0.002, targetTemperature=1.0, thermostatRelaxationTime=0.2))
+    print("[case] polymer 100x10", flush=True)
+    s.RunSteps(0, 300, 10, 100)
+    s.CalcF()
+    f = s.GetForcesAndEnergies().astype(np.float64)
+    pos = s.GetPositions()
+    pos4 = configs.pack_pos4(pos, sysm.types)
+    table = configs.lj_param_table([(0, 0, 1.0, 1.0, 2.0 ** (1.0 / 6.0))], 1)
+    assert abs(table[0, 0] - rc) < 1e-6
+    excl = {(int(a), int(b)) for a, b, _ in bonds}
+    conf2 = conf_io.Configuration(n=sysm.n, box=sysm.box, types=sysm.types, positions=pos, masses=sysm.masses)
+    f_lj, _, _ = _oracle_lj_excluding(conf2, pos4, table, excl)
+    f_b, _ = oracle.bonded(pos4, sysm.box, bonds, [(1.0, 3000.0)], angles, [(float(np.pi), 10.0)])
+    f_ref = f_lj + f_b
+    e = force_errors(f[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e

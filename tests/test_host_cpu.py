@@ -103,3 +103,17 @@ def test_workload_generators():
     # identical seeds give identical configurations
     assert np.array_equal(configs.ka_random_lattice(1000, seed=5).positions,
                           configs.ka_random_lattice(1000, seed=5).positions)
+
+
+def test_polymer_melt_generator():
+    s, bonds, angles = configs.polymer_melt(n_chains=200, beads=10, grid=(10, 10, 20))
+    assert s.n == 2000 and bonds.shape == (1800, 3) and angles.shape == (1600, 4)
+    assert s.box[0] == pytest.approx(2000 ** (1 / 3))
+    d = s.positions[bonds[:, 0]] - s.positions[bonds[:, 1]]
+    d -= s.box * np.rint(d / s.box)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), s.box[0] / 10, rtol=1e-5)
+    # chains never cross: bonded beads belong to the same chain of 10
+    assert (bonds[:, 0] // 10 == bonds[:, 1] // 10).all()
+    assert np.abs(s.velocities.mean(axis=0)).max() < 1e-6
+    with pytest.raises(ValueError):
+        configs.polymer_melt(n_chains=10, beads=10, grid=(7, 7, 2))
