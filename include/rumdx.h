@@ -54,6 +54,18 @@ extern "C" {
 #define RUMDX_POT_IPL_N 5         /* Pot_IPL_n (:600-636) */
 #define RUMDX_POT_YUKAWA 6        /* Pot_Yukawa (:795-840) */
 #define RUMDX_POT_EXP 7           /* Pot_Exp (:1019-1052) */
+#define RUMDX_POT_LJ_GAUSS 8      /* Pot_LJ_12_6_Gauss (:336-378) */
+#define RUMDX_POT_IPL_12 9        /* Pot_IPL_12 (:469-505) */
+#define RUMDX_POT_IPL_9 10        /* Pot_IPL_9 (:512-549) */
+#define RUMDX_POT_IPL_18 11       /* Pot_IPL_18 (:558-593) */
+#define RUMDX_POT_IPL6_LN 12      /* Pot_IPL6_LN (:644-680) */
+#define RUMDX_POT_LJ_LN 13        /* Pot_LJ_LN (:689-733) */
+#define RUMDX_POT_GIRIFALCO 14    /* Pot_Girifalco (:743-786) */
+#define RUMDX_POT_DZUGUTOV 15     /* Pot_Dzugutov (:846-900) */
+#define RUMDX_POT_FERMI_JAGLA 16  /* Pot_Fermi_Jagla (:910-961) */
+#define RUMDX_POT_REP_SHOULDER 17 /* Pot_Repulsive_Shoulder (:969-1012) */
+#define RUMDX_POT_SAAP 18         /* Pot_SAAP (:1060-1135) */
+#define RUMDX_POT_SF_COULOMB 19   /* Pot_ShiftedForceCoulomb (:1143-1186) */
 /* Largest number of particle types a launch accepts (thermostat mask is 32 bits,
  * src/IntegratorNVT.cu:157-174). */
 #define RUMDX_MAX_TYPES 32
@@ -239,7 +251,7 @@ int rumdx_pair_lj_fp64(const rumdx_particles* p, const rumdx_nblist* nb, const f
  * PairPotential::CalcF of that Pot_* class, whose kernels the reference
  * generates from one skeleton per potential (src/Generate_PP_FunctionBodies.py:3-127,
  * ComputeInteraction of each class in include/rumd/PairPotential.h).
- * params_dev is RUMD's table as for rumdx_pair_lj (param 0 = Rcut, 1..7 the
+ * params_dev is RUMD's table as for rumdx_pair_lj (param 0 = Rcut, 1..9 the
  * form's parameters, 10 = s(rc)*rc, 11 = u(rc)).  Same lists, cut-off
  * methods, stress option, initialize semantics and errors as rumdx_pair_lj;
  * pair arithmetic in FP32 with FP32 exp/log/sqrt. */

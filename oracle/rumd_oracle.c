@@ -212,9 +212,23 @@ void orc_stress_brute(int n, const float* pos4, const double L[3], int nt, const
  *   4 Pot_Buckingham         :451-460
  *   5 Pot_IPL_n              :627-634
  *   6 Pot_Yukawa             :829-838
- *   7 Pot_Exp                :1046-1052 */
+ *   7 Pot_Exp                :1046-1052
+ *   8 Pot_LJ_12_6_Gauss      :367-377
+ *   9, 10, 11 Pot_IPL_12, Pot_IPL_9, Pot_IPL_18  :497-504, :540-548, :585-592
+ *  12 Pot_IPL6_LN            :671-679
+ *  13 Pot_LJ_LN              :722-731
+ *  14 Pot_Girifalco          :771-785
+ *  15 Pot_Dzugutov           :878-899
+ *  16 Pot_Fermi_Jagla        :944-960
+ *  17 Pot_Repulsive_Shoulder :997-1011
+ *  18 Pot_SAAP               :1093-1116
+ *  19 Pot_ShiftedForceCoulomb :1178-1185
+ * Pot_Fermi_Jagla and Pot_Repulsive_Shoulder assign the pair energy to the
+ * particle's sum instead of adding it (:958, :1010); here it is added, as for
+ * every other form. */
 static double form_su(int form, double d2, const float* q, double* u) {
   const double p1 = q[1], p2 = q[2], p3 = q[3], p4 = q[4], p5 = q[5];
+  const double p6 = q[6], p7 = q[7], p8 = q[8], p9 = q[9];
   switch (form) {
     case 1: {
       const double x2 = d2 * p1, i2 = 1.0 / x2, i6 = i2 * i2 * i2;
@@ -252,6 +266,79 @@ static double form_su(int form, double d2, const float* q, double* u) {
       const double r = sqrt(d2), e = exp(-r * p1);
       *u = p2 * e;
       return p1 * p2 / r * e;
+    }
+    case 8: { /* eps ((s/r)^12 - 2 (s/r)^6) - eps eps0 exp(-(r - r0)^2 / (2 s0^2)) */
+      const double r = sqrt(d2), sr2 = 1.0 / (d2 * p1), sr6 = pow(sr2, 3.0);
+      const double g = exp(-(r - p4) * (r - p4) / p3);
+      *u = p6 * (sr6 * sr6 - 2.0 * sr6) - p6 * p5 * g;
+      return p1 * p2 * sr2 * (sr6 * sr6 - sr6) - p2 * p5 * (r - p4) * g / (6.0 * p3 * r);
+    }
+    case 9:
+    case 10:
+    case 11: { /* eps (s/r)^n with p2 = n eps */
+      const double n = form == 9 ? 12.0 : form == 10 ? 9.0 : 18.0;
+      const double srn = pow(d2 * p1, -0.5 * n);
+      *u = p3 * srn;
+      return p2 * srn / d2;
+    }
+    case 12: { /* eps (s/r)^6 / ln(r/s): p1 = eps, p2 = 1/s^2, p3 = eps/s^2 */
+      const double x2 = d2 * p2, lx = 0.5 * log(x2), sr6 = pow(x2, -3.0);
+      *u = p1 * sr6 / lx;
+      return p3 * sr6 / x2 * (6.0 * lx + 1.0) / (lx * lx);
+    }
+    case 13: { /* eps a ((Rm/r)^12 - (b/a) (Rm/r)^6) / ln(r/R0) */
+      const double lx = 0.5 * log(d2 * p2), x2 = d2 * p1, sr6 = pow(x2, -3.0);
+      *u = p3 * (sr6 * sr6 - p5 * sr6) / lx;
+      return p4 * sr6 / x2 / (lx * lx) * ((12.0 * lx + 1.0) * sr6 - p5 * (6.0 * lx + 1.0));
+    }
+    case 14: { /* s = r/R0; -alpha(...)^-3 terms + beta(...)^-9 terms */
+      const double x = sqrt(d2 * p1);
+      const double am = pow(x - 1.0, -3.0), ap = pow(x + 1.0, -3.0), a0 = pow(x, -3.0);
+      *u = (-p2 * (am + ap - 2.0 * a0) + p3 * (am * am * am + ap * ap * ap - 2.0 * a0 * a0 * a0)) / x;
+      const double bm = pow(x - 1.0, -4.0), bp = pow(x + 1.0, -4.0);
+      return p1 * a0 *
+             (-p2 * ((4.0 * x - 1.0) * bm + (4.0 * x + 1.0) * bp - 8.0 * a0) +
+              p3 * ((10.0 * x - 1.0) * pow(x - 1.0, -10.0) + (10.0 * x + 1.0) * pow(x + 1.0, -10.0) -
+                    20.0 * pow(x, -9.0)));
+    }
+    case 15: { /* A (r^-16 - B) e^{c/(r-a)} [r < a] + B e^{d/(r-b)}; p1..p6 = a, b, c, d, A, B */
+      const double r = sqrt(d2);
+      const double eb = exp(p4 / (r - p2));
+      double s = p4 * p6 * eb / ((r - p2) * (r - p2) * r);
+      *u = p6 * eb;
+      if (r < p1) {
+        const double ea = exp(p3 / (r - p1)), v1 = p5 * ea * (pow(r, -16.0) - p6);
+        s += 16.0 * p5 * ea * pow(r, -18.0) + p3 * v1 / ((r - p1) * (r - p1) * r);
+        *u += v1;
+      }
+      return s;
+    }
+    case 16: { /* eps0 ((a/r)^n + A0/(1 + e^{kA (r/a - A2)}) - B0/(1 + e^{kB (r/a - B2)})) */
+      const double x = sqrt(d2 * p1), xn = pow(x, -p9);
+      const double ea = exp(p3 * (x - p4)), eb = exp(p6 * (x - p7));
+      *u = p8 * (xn + p2 / (1.0 + ea) - p5 / (1.0 + eb));
+      return p8 * p1 * (p9 * xn / (x * x) + (p2 * p3 * ea / ((1.0 + ea) * (1.0 + ea)) -
+                                              p5 * p6 * eb / ((1.0 + eb) * (1.0 + eb))) / x);
+    }
+    case 17: { /* (s/r)^14 + eps/2 (1 - tanh(k0 (r - s1))); p2 = eps/2 */
+      const double r = sqrt(d2), x2 = d2 * p1, t = tanh(p3 * (r - p4));
+      *u = pow(x2, -7.0) + p2 * (1.0 - t);
+      return 14.0 * p1 * pow(x2, -8.0) + p2 * p3 * (1.0 - t * t) / r;
+    }
+    case 18: { /* eps (a0/x e^{a1 x + a6 x^2} + a2 e^{a3 x} + a4) / (1 + a5 x^6), x = r/s;
+                * the force keeps the reference's eps/(s^2 r) prefactor (:1115) */
+      const double r = sqrt(d2), x = sqrt(d2 * p8);
+      const double g = p1 * exp(p2 * x + p7 * x * x), h = p3 * exp(p4 * x);
+      const double num = g / x + h + p5, den = 1.0 + p6 * pow(x, 6.0);
+      *u = p9 * num / den;
+      const double dnum = -g / (x * x) + g * (p2 + 2.0 * p7 * x) / x + p4 * h;
+      const double dden = 6.0 * p6 * pow(x, 5.0);
+      return -p9 * p8 / r * (dnum * den - num * dden) / (den * den);
+    }
+    case 19: { /* eps/r + eps/Rc^2 (r - Rc) - eps/Rc; q[0] = Rc */
+      const double r = sqrt(d2);
+      *u = p1 / r + p3 * (r - (double)q[0]) - p2;
+      return (p1 / d2 - p3) / r;
     }
     default:
       *u = 0.0;

@@ -7,7 +7,8 @@
 // Here the skeleton is the shared kernels of force_kernels.h and the form is a
 // kernel argument: a wave-uniform switch, so one set of kernels serves every
 // form.  Per type pair the LDS table holds three float4:
-// {rc^2, p1, p2, p3}, {p4, p5, p6, p7}, {rc, s(rc) rc, u(rc), 0}.
+// {rc^2, p1, p2, p3}, {p4, p5, p6, p7}, {s(rc) rc, shift, p8, p9}, with the
+// energy shift u(rc) for SP and s(rc) rc^2 + u(rc) for SF.
 #include "force_kernels.h"
 #include "pot_forms.h"
 
@@ -23,15 +24,16 @@ struct FormPair {
   __device__ __forceinline__ void load_table(float4* s_tab, const float* __restrict__ params, uint32_t nt) const {
     for (uint32_t k = threadIdx.x; k < nt * nt; k += blockDim.x) {
       const float* q = params + static_cast<size_t>(k) * RUMDX_NUM_PARAM;
-      const float rc = q[0];
+      const float rc = q[0], src = q[RUMDX_NUM_PARAM - 2], ucut = q[RUMDX_NUM_PARAM - 1];
       s_tab[3 * k] = make_float4(rc * rc, q[1], q[2], q[3]);
       s_tab[3 * k + 1] = make_float4(q[4], q[5], q[6], q[7]);
-      s_tab[3 * k + 2] = make_float4(rc, q[RUMDX_NUM_PARAM - 2], q[RUMDX_NUM_PARAM - 1], 0.f);
+      s_tab[3 * k + 2] = make_float4(src, CUT == RUMDX_CUTOFF_SF ? fmaf(src, rc, ucut) : ucut, q[8], q[9]);
     }
   }
 
   // fij: the form's s and u inside rc, shifted as the cut-off method asks
-  // (SP: u - u(rc); SF: s - s(rc) rc / r, u + s(rc) rc (r - rc) - u(rc)).
+  // (SP: u - u(rc); SF: s - s(rc) rc / r, u + s(rc) rc (r - rc) - u(rc),
+  // the last two terms as s(rc) rc r - (s(rc) rc^2 + u(rc))).
   template <bool STR>
   __device__ __forceinline__ void core(float dx, float dy, float dz, float d2, uint32_t tj, const float4* prow,
                                        Acc& a) const {
@@ -40,14 +42,14 @@ struct FormPair {
     if (d2 < c0.x) {
       const float4 c1 = prow[3 * tj + 1];
       const float4 c2 = prow[3 * tj + 2];
-      const float p[8] = {0.f, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+      const float p[10] = {0.f, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w, c2.z, c2.w};
       s = pot_form<float>(form, d2, p, u);
       if (CUT == RUMDX_CUTOFF_SP) {
-        u -= c2.z;
+        u -= c2.y;
       } else if (CUT == RUMDX_CUTOFF_SF) {
         const float r = sqrtf(d2);
-        s -= c2.y / r;
-        u += c2.y * (r - c2.x) - c2.z;
+        s -= c2.x / r;
+        u += fmaf(c2.x, r, -c2.y);
       }
     }
     accumulate<STR>(dx, dy, dz, d2, s, u, a);

@@ -200,6 +200,53 @@ PYBIND11_MODULE(_rumd, m) {
       .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
       .def("SetParams", &Pot_Exp::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
            py::arg("Rcut"));
+  py::class_<Pot_LJ_12_6_Gauss, FormPairPotential>(m, "Pot_LJ_12_6_Gauss")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_LJ_12_6_Gauss::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"),
+           py::arg("Epsilon"), py::arg("Rcut"), py::arg("Sigma0"), py::arg("Epsilon0"), py::arg("r0"));
+  py::class_<Pot_IPL_fixed, FormPairPotential>(m, "Pot_IPL_fixed")
+      .def("SetParams", &Pot_IPL_fixed::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  py::class_<Pot_IPL_12, Pot_IPL_fixed>(m, "Pot_IPL_12")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"));
+  py::class_<Pot_IPL_9, Pot_IPL_fixed>(m, "Pot_IPL_9")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"));
+  py::class_<Pot_IPL_18, Pot_IPL_fixed>(m, "Pot_IPL_18")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"));
+  py::class_<Pot_IPL6_LN, FormPairPotential>(m, "Pot_IPL6_LN")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_IPL6_LN::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("Rcut"));
+  py::class_<Pot_LJ_LN, FormPairPotential>(m, "Pot_LJ_LN")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_LJ_LN::SetParams, py::arg("i"), py::arg("j"), py::arg("Rm"), py::arg("R0"),
+           py::arg("Epsilon"), py::arg("Rcut"));
+  py::class_<Pot_Girifalco, FormPairPotential>(m, "Pot_Girifalco")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_Girifalco::SetParams, py::arg("i"), py::arg("j"), py::arg("alpha"), py::arg("beta"),
+           py::arg("R0"), py::arg("Rcut"));
+  py::class_<Pot_Dzugutov, FormPairPotential>(m, "Pot_Dzugutov")
+      .def(py::init<>())
+      .def("SetParams", &Pot_Dzugutov::SetParams, py::arg("i"), py::arg("j"), py::arg("a"), py::arg("b"),
+           py::arg("c"), py::arg("d"), py::arg("A"), py::arg("B"));
+  py::class_<Pot_Fermi_Jagla, FormPairPotential>(m, "Pot_Fermi_Jagla")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_Fermi_Jagla::SetParams, py::arg("i"), py::arg("j"), py::arg("a"), py::arg("epsilon0"),
+           py::arg("A0"), py::arg("A1"), py::arg("A2"), py::arg("B0"), py::arg("B1"), py::arg("B2"), py::arg("n"),
+           py::arg("Rcut"));
+  py::class_<Pot_Repulsive_Shoulder, FormPairPotential>(m, "Pot_Repulsive_Shoulder")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_Repulsive_Shoulder::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"),
+           py::arg("Epsilon"), py::arg("k0"), py::arg("Sigma1"), py::arg("Rcut"));
+  py::class_<Pot_SAAP, FormPairPotential>(m, "Pot_SAAP")
+      .def(py::init<PairPotential::CutoffMethod>(), py::arg("cutoff_method"))
+      .def("SetParams", &Pot_SAAP::SetParams, py::arg("i"), py::arg("j"), py::arg("Sigma"), py::arg("Epsilon"),
+           py::arg("a0"), py::arg("a1"), py::arg("a2"), py::arg("a3"), py::arg("a4"), py::arg("a5"), py::arg("a6"),
+           py::arg("Rcut"));
+  py::class_<Pot_ShiftedForceCoulomb, FormPairPotential>(m, "Pot_ShiftedForceCoulomb")
+      .def(py::init<>())
+      .def("SetParams", &Pot_ShiftedForceCoulomb::SetParams, py::arg("i"), py::arg("j"), py::arg("Epsilon"),
+           py::arg("Rcut"));
   py::class_<BondHarmonic, Potential>(m, "BondHarmonic")
       .def(py::init<>())
       .def("SetParams", &BondHarmonic::SetParams, py::arg("bond_type"), py::arg("bond_length"), py::arg("stiffness"),

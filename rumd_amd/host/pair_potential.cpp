@@ -608,8 +608,8 @@ void Pot_gLJ_m12_n6::SetParams(unsigned i, unsigned j, float Sigma, float Epsilo
 // u(rc) and s(rc) rc of a form (CopyParamsToGPU, src/PairPotential.cu:82-125),
 // formed in FP64 from the stored float parameters.
 void FormPairPotential::ComputeShifts(std::vector<float>& row) const {
-  double p[8];
-  for (int k = 0; k < 8; ++k) p[k] = row[k];
+  double p[10];
+  for (int k = 0; k < 10; ++k) p[k] = row[k];
   const double rc = p[0];
   double u = 0.0, s = 0.0;
   if (rc > 0.0) s = rumdx::pot_form<double>(form_, rc * rc, p, u);
@@ -700,4 +700,104 @@ Pot_Exp::Pot_Exp(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_EXP) { SetID_S
 void Pot_Exp::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
   SetParamsRow(i, j, {Rcut * Sigma, 1.f / Sigma, Epsilon});
   Report("exponential parameters set");
+}
+
+Pot_LJ_12_6_Gauss::Pot_LJ_12_6_Gauss(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_LJ_GAUSS) {
+  SetID_String("potLJG");
+}
+
+// include/rumd/PairPotential.h:350-365
+void Pot_LJ_12_6_Gauss::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut, float Sigma0,
+                                  float Epsilon0, float r0) {
+  SetParamsRow(i, j, {Rcut * Sigma, 1.f / (Sigma * Sigma), 12.f * Epsilon, 2.0f * Sigma0 * Sigma0, r0, Epsilon0,
+                      Epsilon});
+  Report("LJ-Gauss parameters set");
+}
+
+Pot_IPL_fixed::Pot_IPL_fixed(CutoffMethod m, int form, float n, const char* id) : FormPairPotential(m, form), n_(n) {
+  SetID_String(id);
+}
+
+// include/rumd/PairPotential.h:483-495 (and :526-538, :572-584)
+void Pot_IPL_fixed::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, 1.f / (Sigma * Sigma), n_ * Epsilon, Epsilon});
+  Report("IPL parameters set");
+}
+
+Pot_IPL6_LN::Pot_IPL6_LN(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_IPL6_LN) { SetID_String("potIPL6LN"); }
+
+// include/rumd/PairPotential.h:658-669
+void Pot_IPL6_LN::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, Epsilon, 1.f / (Sigma * Sigma), Epsilon / (Sigma * Sigma)});
+  Report("IPL6/ln parameters set");
+}
+
+Pot_LJ_LN::Pot_LJ_LN(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_LJ_LN) { SetID_String("pot_LJ_LN"); }
+
+// include/rumd/PairPotential.h:703-719: a = ln(Rm/R0) + 1/6, b = 2 ln(Rm/R0) + 1/6
+void Pot_LJ_LN::SetParams(unsigned i, unsigned j, float Rm, float R0, float Epsilon, float Rcut) {
+  const float l = std::log(Rm / R0);
+  const float p1 = 1.f / (Rm * Rm);
+  const float p3 = Epsilon * (l + 1.f / 6.f);
+  SetParamsRow(i, j, {Rcut * Rm, p1, 1.f / (R0 * R0), p3, p1 * p3, (12.f * l + 1) / (6.f * l + 1)});
+  Report("LJ/ln parameters set");
+}
+
+Pot_Girifalco::Pot_Girifalco(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_GIRIFALCO) {
+  SetID_String("pot_Girifalco");
+}
+
+// include/rumd/PairPotential.h:757-768: Rcut in units of R0
+void Pot_Girifalco::SetParams(unsigned i, unsigned j, float alpha, float beta, float R0, float Rcut) {
+  SetParamsRow(i, j, {Rcut * R0, 1.f / (R0 * R0), alpha, beta});
+  Report("Girifalco parameters set");
+}
+
+Pot_Dzugutov::Pot_Dzugutov() : FormPairPotential(NS, RUMDX_POT_DZUGUTOV) { SetID_String("potDzu"); }
+
+// include/rumd/PairPotential.h:860-871: the cut-off sits just inside b
+void Pot_Dzugutov::SetParams(unsigned i, unsigned j, float a, float b, float c, float d, float A, float B) {
+  SetParamsRow(i, j, {b - 0.000001f, a, b, c, d, A, B});
+  Report("Dzugutov parameters set");
+}
+
+Pot_Fermi_Jagla::Pot_Fermi_Jagla(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_FERMI_JAGLA) {
+  SetID_String("potFermiJagla");
+}
+
+// include/rumd/PairPotential.h:924-941
+void Pot_Fermi_Jagla::SetParams(unsigned i, unsigned j, float a, float epsilon0, float A0, float A1, float A2,
+                                float B0, float B1, float B2, float n, float Rcut) {
+  SetParamsRow(i, j, {Rcut * a, 1.f / (a * a), A0, A1 / A0, A2, B0, B1 / B0, B2, epsilon0, n});
+  Report("Fermi-Jagla parameters set");
+}
+
+Pot_Repulsive_Shoulder::Pot_Repulsive_Shoulder(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_REP_SHOULDER) {
+  SetID_String("potRepulsiveShoulder");
+}
+
+// include/rumd/PairPotential.h:983-994
+void Pot_Repulsive_Shoulder::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float k0, float Sigma1,
+                                       float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, 1.f / (Sigma * Sigma), 0.5f * Epsilon, k0, Sigma1});
+  Report("repulsive shoulder parameters set");
+}
+
+Pot_SAAP::Pot_SAAP(CutoffMethod m) : FormPairPotential(m, RUMDX_POT_SAAP) { SetID_String("SAAP"); }
+
+// include/rumd/PairPotential.h:1074-1090
+void Pot_SAAP::SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float a0, float a1, float a2, float a3,
+                         float a4, float a5, float a6, float Rcut) {
+  SetParamsRow(i, j, {Rcut * Sigma, a0, a1, a2, a3, a4, a5, a6, 1.f / (Sigma * Sigma), Epsilon});
+  Report("SAAP parameters set");
+}
+
+Pot_ShiftedForceCoulomb::Pot_ShiftedForceCoulomb() : FormPairPotential(NS, RUMDX_POT_SF_COULOMB) {
+  SetID_String("potCoul");
+}
+
+// include/rumd/PairPotential.h:1165-1172: Rcut is absolute
+void Pot_ShiftedForceCoulomb::SetParams(unsigned i, unsigned j, float Epsilon, float Rcut) {
+  SetParamsRow(i, j, {Rcut, Epsilon, Epsilon / Rcut, Epsilon / (Rcut * Rcut)});
+  Report("shifted-force Coulomb parameters set");
 }

@@ -381,6 +381,96 @@ class Pot_Exp : public FormPairPotential {
   void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
 };
 
+// Pot_LJ_12_6_Gauss (include/rumd/PairPotential.h:336-378).
+class Pot_LJ_12_6_Gauss : public FormPairPotential {
+ public:
+  explicit Pot_LJ_12_6_Gauss(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut, float Sigma0, float Epsilon0,
+                 float r0);
+};
+
+// Pot_IPL_12, Pot_IPL_9, Pot_IPL_18 (include/rumd/PairPotential.h:469-593):
+// one class per fixed exponent.
+class Pot_IPL_fixed : public FormPairPotential {
+ public:
+  Pot_IPL_fixed(CutoffMethod m, int form, float n, const char* id);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+
+ private:
+  float n_;
+};
+class Pot_IPL_12 : public Pot_IPL_fixed {
+ public:
+  explicit Pot_IPL_12(CutoffMethod m) : Pot_IPL_fixed(m, RUMDX_POT_IPL_12, 12.f, "potIPL12") {}
+};
+class Pot_IPL_9 : public Pot_IPL_fixed {
+ public:
+  explicit Pot_IPL_9(CutoffMethod m) : Pot_IPL_fixed(m, RUMDX_POT_IPL_9, 9.f, "potIPL9") {}
+};
+class Pot_IPL_18 : public Pot_IPL_fixed {
+ public:
+  explicit Pot_IPL_18(CutoffMethod m) : Pot_IPL_fixed(m, RUMDX_POT_IPL_18, 18.f, "potIPL18") {}
+};
+
+// Pot_IPL6_LN (include/rumd/PairPotential.h:644-680).
+class Pot_IPL6_LN : public FormPairPotential {
+ public:
+  explicit Pot_IPL6_LN(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float Rcut);
+};
+
+// Pot_LJ_LN (include/rumd/PairPotential.h:689-733).
+class Pot_LJ_LN : public FormPairPotential {
+ public:
+  explicit Pot_LJ_LN(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float Rm, float R0, float Epsilon, float Rcut);
+};
+
+// Pot_Girifalco (include/rumd/PairPotential.h:743-786).
+class Pot_Girifalco : public FormPairPotential {
+ public:
+  explicit Pot_Girifalco(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float alpha, float beta, float R0, float Rcut);
+};
+
+// Pot_Dzugutov (include/rumd/PairPotential.h:846-900): no cut-off shift.
+class Pot_Dzugutov : public FormPairPotential {
+ public:
+  Pot_Dzugutov();
+  void SetParams(unsigned i, unsigned j, float a, float b, float c, float d, float A, float B);
+};
+
+// Pot_Fermi_Jagla (include/rumd/PairPotential.h:910-961).
+class Pot_Fermi_Jagla : public FormPairPotential {
+ public:
+  explicit Pot_Fermi_Jagla(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float a, float epsilon0, float A0, float A1, float A2, float B0, float B1,
+                 float B2, float n, float Rcut);
+};
+
+// Pot_Repulsive_Shoulder (include/rumd/PairPotential.h:969-1012).
+class Pot_Repulsive_Shoulder : public FormPairPotential {
+ public:
+  explicit Pot_Repulsive_Shoulder(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float k0, float Sigma1, float Rcut);
+};
+
+// Pot_SAAP (include/rumd/PairPotential.h:1060-1135).
+class Pot_SAAP : public FormPairPotential {
+ public:
+  explicit Pot_SAAP(CutoffMethod m);
+  void SetParams(unsigned i, unsigned j, float Sigma, float Epsilon, float a0, float a1, float a2, float a3, float a4,
+                 float a5, float a6, float Rcut);
+};
+
+// Pot_ShiftedForceCoulomb (include/rumd/PairPotential.h:1143-1186): the
+// shifted force is part of the form, so the class uses no cut-off shift.
+class Pot_ShiftedForceCoulomb : public FormPairPotential {
+ public:
+  Pot_ShiftedForceCoulomb();
+  void SetParams(unsigned i, unsigned j, float Epsilon, float Rcut);
+};
+
 // ------------------------------------------------------------------ molecules
 // MoleculeData (include/rumd/MoleculeData.h, src/MoleculeData.cu:479-700): the
 // bond and angle topology, held on the device as per-particle lists (CSR over

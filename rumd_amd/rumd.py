@@ -31,6 +31,18 @@ Pot_Buckingham = _rumd.Pot_Buckingham
 Pot_IPL_n = _rumd.Pot_IPL_n
 Pot_Yukawa = _rumd.Pot_Yukawa
 Pot_Exp = _rumd.Pot_Exp
+Pot_LJ_12_6_Gauss = _rumd.Pot_LJ_12_6_Gauss
+Pot_IPL_12 = _rumd.Pot_IPL_12
+Pot_IPL_9 = _rumd.Pot_IPL_9
+Pot_IPL_18 = _rumd.Pot_IPL_18
+Pot_IPL6_LN = _rumd.Pot_IPL6_LN
+Pot_LJ_LN = _rumd.Pot_LJ_LN
+Pot_Girifalco = _rumd.Pot_Girifalco
+Pot_Dzugutov = _rumd.Pot_Dzugutov
+Pot_Fermi_Jagla = _rumd.Pot_Fermi_Jagla
+Pot_Repulsive_Shoulder = _rumd.Pot_Repulsive_Shoulder
+Pot_SAAP = _rumd.Pot_SAAP
+Pot_ShiftedForceCoulomb = _rumd.Pot_ShiftedForceCoulomb
 BondHarmonic = _rumd.BondHarmonic
 AngleCosSq = _rumd.AngleCosSq
 BondFENE = _rumd.BondFENE
@@ -216,6 +228,7 @@ class Sample(_rumd.Sample):
 __all__ = [
     "Device", "SimulationBox", "RectangularSimulationBox", "Potential", "PairPotential", "Pot_IPL_12_6",
     "Pot_LJ_12_6", "Pot_gLJ_m12_n6", "Pot_LJ_12_6_smooth_2_4", "Pot_gLJ_m_n", "Pot_Gauss", "Pot_Buckingham",
-    "Pot_IPL_n", "Pot_Yukawa", "Pot_Exp", "BondHarmonic", "BondFENE", "AngleCosSq", "DihedralRyckaert", "MoleculeData", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
+    "Pot_IPL_n", "Pot_Yukawa", "Pot_Exp", "Pot_LJ_12_6_Gauss", "Pot_IPL_12", "Pot_IPL_9", "Pot_IPL_18", "Pot_IPL6_LN", "Pot_LJ_LN", "Pot_Girifalco", "Pot_Dzugutov", "Pot_Fermi_Jagla", "Pot_Repulsive_Shoulder", "Pot_SAAP", "Pot_ShiftedForceCoulomb",
+    "BondHarmonic", "BondFENE", "AngleCosSq", "DihedralRyckaert", "MoleculeData", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
     "ShiftedForce", "GetVersion", "NumberOfPairPotentialParameters", "Sample", "ReplicaBatch",
 ]
