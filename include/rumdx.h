@@ -312,6 +312,14 @@ int rumdx_angle_cossq(const rumdx_particles* p, const uint32_t* slot_of_dev,
                       const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
                       int calc_stresses, void* stream);
 
+/* AngleSq::CalcF (src/AnglePotential.cu:220-250, kernel SquaredAngle :289-373):
+ * u = k/2 (theta - theta0)^2 per angle (a, b, c), theta the angle at b, with
+ * the reference's force factor -k (theta - theta0) / (1e-6 + sin theta).
+ * Lists, parameters, energy placement and errors as rumdx_angle_cossq. */
+int rumdx_angle_sq(const rumdx_particles* p, const uint32_t* slot_of_dev, const uint32_t* angle_start_dev,
+                   const uint32_t* angle_atoms_dev, const float* params_dev, uint32_t ntypes, const float box[8],
+                   int initialize, int calc_stresses, void* stream);
+
 /* ---------------------------------------------------------------- integrator
  * IntegratorNVT::Integrate (integrateNVTAlgorithm, src/IntegratorNVT.cu:64-79,
  * 364-418) followed, when thermostat_on_mask != 0, by UpdateThermostatState

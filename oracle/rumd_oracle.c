@@ -30,8 +30,9 @@
  *     (gLJ 12-6 = LJA, IPL_n with n = 12 = IPL 12-6 with A6 = 0), the
  *     Buckingham minimum -epsilon at rm and the smooth LJ's vanishing force
  *     at Rcut.
- *   orc_bonds, orc_angles_cossq, orc_dihedrals_ryckaert: closed forms for
- *     one bond (harmonic, FENE), one angle and the trans / cis dihedral,
+ *   orc_bonds, orc_angles_cossq, orc_angles_sq, orc_dihedrals_ryckaert:
+ *     closed forms for one bond (harmonic, FENE), one angle (both forms) and
+ *     the trans / cis dihedral,
  *     -grad E = F by finite differences of every coordinate of a small
  *     molecule, zero total force and torque.
  *   orc_nblist_*: brute force vs cell scan, symmetry, superset of the rc sphere.
@@ -447,6 +448,40 @@ void orc_angles_cossq(int n, const float* pos4, const double L[3], int nangles, 
       f4[4 * (size_t)b + m] -= fa + fc;
     }
     f4[4 * (size_t)a + 3] += 0.5 * k * dcos * dcos;
+  }
+}
+
+/* Squared bond angles in FP64 (AngleSq, SquaredAngle, src/AnglePotential.cu:289-373):
+ * theta = acos(e_a . e_c) at b, u = k/2 (theta - theta0)^2 to atom a's
+ * f4[.,3]; F_a = k (theta - theta0) / (1e-6 + sin theta) (e_c - cos theta e_a)
+ * / |r_a - r_b|, F_c likewise, F_b = -F_a - F_c.  The 1e-6 is the reference's
+ * regularisation of the straight angle. */
+void orc_angles_sq(int n, const float* pos4, const double L[3], int nangles, const int32_t* angles,
+                   const float* params, double* f4) {
+  (void)n;
+  for (int q = 0; q < nangles; ++q) {
+    const int a = angles[4 * q], b = angles[4 * q + 1], c = angles[4 * q + 2], t = angles[4 * q + 3];
+    const double th0 = params[2 * t], k = params[2 * t + 1];
+    double da[3], dc[3];
+    const double ra = sqrt(min_image_f64(pos4 + 4 * (size_t)a, pos4 + 4 * (size_t)b, L, da));
+    const double rc = sqrt(min_image_f64(pos4 + 4 * (size_t)c, pos4 + 4 * (size_t)b, L, dc));
+    double cs = 0.0;
+    for (int m = 0; m < 3; ++m) {
+      da[m] /= ra;
+      dc[m] /= rc;
+      cs += da[m] * dc[m];
+    }
+    if (cs > 1.0) cs = 1.0;
+    if (cs < -1.0) cs = -1.0;
+    const double th = acos(cs), g = k * (th - th0) / (1e-6 + sin(th));
+    for (int m = 0; m < 3; ++m) {
+      const double fa = g * (dc[m] - cs * da[m]) / ra;
+      const double fc = g * (da[m] - cs * dc[m]) / rc;
+      f4[4 * (size_t)a + m] += fa;
+      f4[4 * (size_t)c + m] += fc;
+      f4[4 * (size_t)b + m] -= fa + fc;
+    }
+    f4[4 * (size_t)a + 3] += 0.5 * k * (th - th0) * (th - th0);
   }
 }
 

@@ -135,14 +135,24 @@ __global__ __launch_bounds__(kBlock) void k_bond(
   store_sums<STR>(s, a, init != 0, f, w, sts);
 }
 
-// Bond angles, u = k/2 (cos theta - cos theta0)^2 for the angle at b of
-// (a, b, c), written with the bond vectors d1 = r_b - r_a, d2 = r_c - r_b
-// (so cos(pi - theta) appears): F_a = ff (c1 d1 - d2) / D, F_c = ff (d1 -
-// c2 d2) / D, F_b = -F_a - F_c, ff = -k (cos(pi - theta) - cos(pi - theta0)),
-// c1 = d1.d2/|d1|^2, c2 = d1.d2/|d2|^2, D = |d1||d2|.  The angle's energy goes
-// to atom a, as in the reference; angles add no virial there.  Each list entry
-// is (a, b, c, type) in original ids; params: (theta0, k) per angle type.
-__global__ __launch_bounds__(kBlock) void k_angle_cossq(const float4* __restrict__ r, float4* __restrict__ f,
+// Bond angles for the angle theta at b of (a, b, c), written with the bond
+// vectors d1 = r_b - r_a, d2 = r_c - r_b, whose cosine cc is cos(pi - theta):
+// F_a = ff (c1 d1 - d2) / D, F_c = ff (d1 - c2 d2) / D, F_b = -F_a - F_c with
+// c1 = d1.d2/|d1|^2, c2 = d1.d2/|d2|^2, D = |d1||d2| and ff = -du/dcc.
+//   ANGLE_COSSQ (CosSq, src/AnglePotential.cu:137-217):
+//     u = k/2 (cc - cos(pi - theta0))^2, ff = -k (cc - cos(pi - theta0));
+//   ANGLE_SQ (SquaredAngle, src/AnglePotential.cu:289-373): theta =
+//     pi - acos(cc), u = k/2 (theta - theta0)^2, ff = -k (theta - theta0) /
+//     (1e-6 + sin theta), the reference's regularisation of the straight
+//     angle.
+// The angle's energy goes to atom a, as in the reference; angles add no
+// virial there.  Each list entry is (a, b, c, type) in original ids; params:
+// (theta0, k) per angle type.
+constexpr int ANGLE_COSSQ = 0;
+constexpr int ANGLE_SQ = 1;
+
+template <int FORM>
+__global__ __launch_bounds__(kBlock) void k_angle(const float4* __restrict__ r, float4* __restrict__ f,
                                                         float4* __restrict__ w, float4* __restrict__ sts,
                                                         const uint32_t* __restrict__ slot_of,
                                                         const uint32_t* __restrict__ start,
@@ -165,15 +175,28 @@ __global__ __launch_bounds__(kBlock) void k_angle_cossq(const float4* __restrict
     const float c22 = d2.x * d2.x + d2.y * d2.y + d2.z * d2.z;
     const float icd = 1.f / sqrtf(c11 * c22);
     const float cc = c12 * icd;
-    const float ccon = cosf(static_cast<float>(M_PI) - p.x);
-    const float ff = -p.y * (cc - ccon);
+    float ff, ua;
+    if (FORM == ANGLE_SQ) {
+      // theta = pi - acos(cc) taken as atan2(|d1 x d2|, -d1.d2) and sin theta
+      // as |d1 x d2| / D: the same numbers, without acos's loss of digits
+      // near the straight angle, where 1/sin theta amplifies them
+      const float cx = d1.y * d2.z - d1.z * d2.y, cy = d1.z * d2.x - d1.x * d2.z, cz = d1.x * d2.y - d1.y * d2.x;
+      const float cr = sqrtf(cx * cx + cy * cy + cz * cz);
+      const float theta = atan2f(cr, -c12);
+      ff = -p.y * (theta - p.x) / (1.0e-6f + cr * icd);
+      ua = 0.5f * p.y * (theta - p.x) * (theta - p.x);
+    } else {
+      const float ccon = cosf(static_cast<float>(M_PI) - p.x);
+      ff = -p.y * (cc - ccon);
+      ua = 0.5f * p.y * (cc - ccon) * (cc - ccon);
+    }
     const float c1 = c12 / c11, c2 = c12 / c22;
     float gx, gy, gz;  // this atom's share
     if (o == an.x) {
       gx = ff * (c1 * d1.x - d2.x) * icd;
       gy = ff * (c1 * d1.y - d2.y) * icd;
       gz = ff * (c1 * d1.z - d2.z) * icd;
-      u += 0.5f * p.y * (cc - ccon) * (cc - ccon);
+      u += ua;
     } else if (o == an.z) {
       gx = ff * (d1.x - c2 * d2.x) * icd;
       gy = ff * (d1.y - c2 * d2.y) * icd;
@@ -376,22 +399,39 @@ extern "C" int rumdx_dihedral_ryckaert(const rumdx_particles* p, const uint32_t*
   return check_launch("rumdx_dihedral_ryckaert");
 }
 
-extern "C" int rumdx_angle_cossq(const rumdx_particles* p, const uint32_t* slot_of_dev,
-                                 const uint32_t* angle_start_dev, const uint32_t* angle_atoms_dev,
-                                 const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
-                                 int calc_stresses, void* stream) {
+namespace {
+template <int FORM>
+int launch_angles(const char* entry, const rumdx_particles* p, const uint32_t* slot_of_dev,
+                  const uint32_t* angle_start_dev, const uint32_t* angle_atoms_dev, const float* params_dev,
+                  uint32_t ntypes, const float box[8], int initialize, int calc_stresses, void* stream) {
   using namespace rumdx;
-  const int rc = check_common("rumdx_angle_cossq", p, slot_of_dev, angle_start_dev, angle_atoms_dev, params_dev, box);
+  const std::string e(entry);
+  const int rc = check_common(entry, p, slot_of_dev, angle_start_dev, angle_atoms_dev, params_dev, box);
   if (rc != RUMDX_OK) return rc;
-  if (ntypes == 0) return fail(RUMDX_E_INVALID, "rumdx_angle_cossq: no angle types");
-  if (calc_stresses && p->sts_dev == nullptr)
-    return fail(RUMDX_E_INVALID, "rumdx_angle_cossq: stresses requested but sts is null");
+  if (ntypes == 0) return fail(RUMDX_E_INVALID, e + ": no angle types");
+  if (calc_stresses && p->sts_dev == nullptr) return fail(RUMDX_E_INVALID, e + ": stresses requested but sts is null");
   if (p->n == 0) return RUMDX_OK;
-  hipLaunchKernelGGL(k_angle_cossq, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
+  hipLaunchKernelGGL((k_angle<FORM>), dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
                      reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<float4*>(p->f_dev),
                      reinterpret_cast<float4*>(p->w_dev), reinterpret_cast<float4*>(p->sts_dev), slot_of_dev,
                      angle_start_dev,
                      reinterpret_cast<const uint4*>(angle_atoms_dev), reinterpret_cast<const float2*>(params_dev),
                      ntypes, make_box(box), p->n, initialize, calc_stresses);
-  return check_launch("rumdx_angle_cossq");
+  return check_launch(entry);
+}
+}  // namespace
+
+extern "C" int rumdx_angle_cossq(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                                 const uint32_t* angle_start_dev, const uint32_t* angle_atoms_dev,
+                                 const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
+                                 int calc_stresses, void* stream) {
+  return launch_angles<rumdx::ANGLE_COSSQ>("rumdx_angle_cossq", p, slot_of_dev, angle_start_dev, angle_atoms_dev,
+                                           params_dev, ntypes, box, initialize, calc_stresses, stream);
+}
+
+extern "C" int rumdx_angle_sq(const rumdx_particles* p, const uint32_t* slot_of_dev, const uint32_t* angle_start_dev,
+                              const uint32_t* angle_atoms_dev, const float* params_dev, uint32_t ntypes,
+                              const float box[8], int initialize, int calc_stresses, void* stream) {
+  return launch_angles<rumdx::ANGLE_SQ>("rumdx_angle_sq", p, slot_of_dev, angle_start_dev, angle_atoms_dev,
+                                        params_dev, ntypes, box, initialize, calc_stresses, stream);
 }

@@ -266,6 +266,7 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetParams", &AngleCosSq::GetParams)
       .def("SetExclude", &AngleCosSq::SetExclude, py::arg("exclude"))
       .def("SetExclusions", &AngleCosSq::SetExclusions, py::arg("non_bond_pot"));
+  py::class_<AngleSq, AngleCosSq>(m, "AngleSq").def(py::init<>());
   py::class_<MoleculeData>(m, "MoleculeData")
       .def("ReadTopology", &MoleculeData::ReadTopology, py::arg("filename"))
       .def("SetBonds", &MoleculeData::SetBonds, py::arg("abt"), py::arg("molecule") = std::vector<uint32_t>{})

@@ -299,9 +299,10 @@ void AngleCosSq::Launch(const rumdx_particles* p, bool initialize, bool calc_str
   if (m->GetNumberOfAngleTypes() > params_.size() / 2)
     throw RUMD_Error(GetID_String(), method, "an angle type of the topology has no parameters");
   const uint32_t* slot_of = m->SlotIndex();
-  rumdx_check(rumdx_angle_cossq(p, slot_of, m->AngleStart(), m->AngleAtoms(), d_params_.get(),
-                                static_cast<uint32_t>(params_.size() / 2), sample->GetSimulationBox()->GetBox8(),
-                                initialize ? 1 : 0, calc_stresses ? 1 : 0, sample->GetStream()),
+  auto* entry = squared_ ? rumdx_angle_sq : rumdx_angle_cossq;
+  rumdx_check(entry(p, slot_of, m->AngleStart(), m->AngleAtoms(), d_params_.get(),
+                    static_cast<uint32_t>(params_.size() / 2), sample->GetSimulationBox()->GetBox8(),
+                    initialize ? 1 : 0, calc_stresses ? 1 : 0, sample->GetStream()),
               GetID_String().c_str(), method);
 }
 
@@ -319,7 +320,7 @@ double AngleCosSq::GetPotentialEnergy() {
   p.f_dev = pe_f_.get();
   p.w_dev = pe_w_.get();
   Launch(&p, true, false, "GetPotentialEnergy");
-  return sum_energy(sample, pe_f_.get(), "AngleCosSq");
+  return sum_energy(sample, pe_f_.get(), GetID_String().c_str());
 }
 
 // ------------------------------------------------------------------ DihedralRyckaert

@@ -556,7 +556,7 @@ class BondHarmonic : public Potential {
 class BondFENE : public BondHarmonic {
  public:
   BondFENE() {
-    SetID_String("bondFENE");
+    SetID_String("bondFene");
     fene_ = true;
   }
 };
@@ -589,7 +589,7 @@ class DihedralRyckaert : public Potential {
 // u = k/2 (cos theta - cos theta0)^2 per angle.
 class AngleCosSq : public Potential {
  public:
-  AngleCosSq() { SetID_String("angleCosSq"); }
+  AngleCosSq() { SetID_String("AngleCosSq"); }
   void SetParams(unsigned angle_type, float theta0, float ktheta);
   void CalcF(bool initialize, bool calc_stresses) override;
   bool EnergyIncludedInParticleSum() const override { return true; }
@@ -603,12 +603,25 @@ class AngleCosSq : public Potential {
   void SetExclude(bool exclude) { exclude_angle_ = exclude; }
   void SetExclusions(PairPotential* non_bond_pot);
 
+ protected:
+  bool squared_ = false;  // AngleSq: u = k/2 (theta - theta0)^2
+
  private:
   void Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method);
   std::vector<float> params_;  // (theta0, k) per angle type
   bool exclude_angle_ = true;
   DeviceArray<float> d_params_;
   DeviceArray<float> pe_f_, pe_w_;
+};
+
+// AngleSq (src/AnglePotential.cu:216-373): u = k/2 (theta - theta0)^2 per
+// angle, with AngleCosSq's parameters, exclusions and energy placement.
+class AngleSq : public AngleCosSq {
+ public:
+  AngleSq() {
+    SetID_String("AngleSq");
+    squared_ = true;
+  }
 };
 
 // ------------------------------------------------------------------ integrators

@@ -1,5 +1,5 @@
-"""Intramolecular forces on the GPU (rumdx_bond_harmonic, rumdx_angle_cossq
-through BondHarmonic / AngleCosSq and MoleculeData) together with a pair
+"""Intramolecular forces on the GPU (rumdx_bond_harmonic, rumdx_angle_cossq,
+rumdx_angle_sq through BondHarmonic / AngleCosSq / AngleSq and MoleculeData) together with a pair
 potential, against the FP64 oracle, on RUMD's pentamer melt fixture: every
 list method (the sorted one permutes particles, so the bonded kernels read
 them through the slot index), energies per potential, the virial, graph
@@ -42,7 +42,7 @@ def _topology(golden_dir):
     return bonds, angles
 
 
-def _sample(golden_dir, method, with_angles=True, exclude_bonds=False):
+def _sample(golden_dir, method, with_angles=True, exclude_bonds=False, angle_cls=None):
     c = load_golden(golden_dir / "pentamer200_rho0.85_T0.5.xyz.gz")
     s = rumd.Sample(0, 0)
     s.SetVerbose(False)
@@ -65,7 +65,7 @@ def _sample(golden_dir, method, with_angles=True, exclude_bonds=False):
     ang = None
     if with_angles:
         md.SetAngles(angles.reshape(-1).tolist())
-        ang = rumd.AngleCosSq()
+        ang = (angle_cls or rumd.AngleCosSq)()
         ang.SetVerbose(False)
         for t, (th0, k) in enumerate(ANGLE_PARAMS):
             ang.SetParams(angle_type=t, theta0=th0, ktheta=k)
@@ -73,17 +73,19 @@ def _sample(golden_dir, method, with_angles=True, exclude_bonds=False):
     return c, s, pot, bond, ang, bonds, angles
 
 
+@pytest.mark.parametrize("angle_form", ["cossq", "sq"])
 @pytest.mark.parametrize("method", ["n2", "sort", "none"])
-def test_bonded_with_pair_forces_match_oracle(golden_dir, method):
-    c, s, pot, bond, ang, bonds, angles = _sample(golden_dir, method)
-    print(f"[case] pentamer200 LJ + bonds + angles, list {method}", flush=True)
+def test_bonded_with_pair_forces_match_oracle(golden_dir, method, angle_form):
+    sq = angle_form == "sq"
+    c, s, pot, bond, ang, bonds, angles = _sample(golden_dir, method, angle_cls=rumd.AngleSq if sq else None)
+    print(f"[case] pentamer200 LJ + bonds + angles ({angle_form}), list {method}", flush=True)
     s.CalcF()
     f = s.GetForcesAndEnergies().astype(np.float64)
     pos4 = configs.pack_pos4(c.positions, c.types)
     table = configs.lj_param_table([(0, 0, 1.0, 1.0, 2.5)], 1)
     f_lj, vir_lj = oracle.forces_brute(pos4, c.box, 1, table, SP)
     f_b, vir_b = oracle.bonded(pos4, c.box, bonds, BOND_PARAMS, None, None)
-    f_a, _ = oracle.bonded(pos4, c.box, None, None, angles, ANGLE_PARAMS)
+    f_a, _ = oracle.bonded(pos4, c.box, None, None, angles, ANGLE_PARAMS, angle_sq=sq)
     f_ref = f_lj + f_b + f_a
     e = force_errors(f[:, :3], f_ref[:, :3])
     assert e["guarded_max"] <= FORCE_TOL_GUARDED, e

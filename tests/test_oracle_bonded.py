@@ -152,3 +152,32 @@ def test_fene_force_is_minus_the_energy_slope():
             es.append((f4[:, 3].sum(), float(np.float32(rr))))
         f4, _ = oracle.bonded(_pos4([[0, 0, 0], [r, 0, 0]]), L, [[0, 1, 0]], [[r0, k]], fene=True)
         assert f4[1, 0] == pytest.approx(-(es[1][0] - es[0][0]) / (es[1][1] - es[0][1]), rel=1e-4)
+
+
+def test_squared_angle_closed_form_and_gradient():
+    k, th0 = 40.0, float(np.float32(1.9))
+    p = _pos4([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 2.0, 0.0]])  # a right angle at b
+    f4, vir = oracle.bonded(p, L, angles=[[0, 1, 2, 0]], angle_params=[[th0, k]], angle_sq=True)
+    assert f4[0, 3] == pytest.approx(0.5 * k * (np.pi / 2 - th0) ** 2, rel=1e-12)
+    assert f4[1, 3] == 0.0 and f4[2, 3] == 0.0 and not vir.any()
+    # theta < theta0: the arms open; |F_a| = k (theta0 - theta) / |r_a - r_b| (sin theta = 1)
+    np.testing.assert_allclose(f4[0, :3], [0.0, -k * (th0 - np.pi / 2) / (1 + 1e-6), 0.0], rtol=1e-12, atol=1e-12)
+    x, bonds, angles, bp, ap = _molecule(seed=7)
+
+    def energy(xx):
+        f, _ = oracle.bonded(_pos4(xx), L, angles=angles, angle_params=ap, angle_sq=True)
+        return f[:, 3].sum()
+
+    f4, _ = oracle.bonded(_pos4(x), L, angles=angles, angle_params=ap, angle_sq=True)
+    scale = np.abs(f4[:, :3]).max()
+    np.testing.assert_allclose(f4[:, :3].sum(axis=0), 0.0, atol=1e-12 * scale)
+    torque = np.cross(x.astype(np.float64), f4[:, :3]).sum(axis=0)
+    np.testing.assert_allclose(torque, 0.0, atol=1e-10 * scale)
+    h = 1e-3
+    for i in range(len(x)):
+        for c in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, c] += h
+            xm[i, c] -= h
+            g = (energy(xp) - energy(xm)) / (float(xp[i, c]) - float(xm[i, c]))
+            assert f4[i, c] == pytest.approx(-g, rel=2e-4, abs=2e-4), (i, c)
