@@ -300,6 +300,17 @@ int rumdx_dihedral_ryckaert(const rumdx_particles* p, const uint32_t* slot_of_de
                             const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
                             int calc_stresses, void* stream);
 
+/* PeriodicDihedral::CalcF (src/DihedralPotential.cu:270-300, kernel PeriodicDih
+ * :337-458): params (psi0, k1, k2, k3, 0, 0) per type, psi = acos(cos psi) in
+ * [0, pi], u = 1/2 (k1 (1 - cos x) + k2 (1 - cos 2x) + k3 (1 - cos 3x)) with
+ * x = psi - psi0, and the reference's forces, -du/dpsi times the gradient of
+ * cos psi (-sin psi times the gradient force).  Lists, energy placement and
+ * errors as rumdx_dihedral_ryckaert. */
+int rumdx_dihedral_periodic(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                            const uint32_t* dihedral_start_dev, const uint32_t* dihedral_atoms_dev,
+                            const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
+                            int calc_stresses, void* stream);
+
 /* AngleCosSq::CalcF (src/AnglePotential.cu:69-98, kernel CosSq :137-217):
  * u = k/2 (cos theta - cos theta0)^2 per angle (a, b, c), the angle at b.
  * angle_atoms_dev holds 4 u32 per entry: a, b, c (original ids) and the angle

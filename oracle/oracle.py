@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
         L.orc_dihedrals_ryckaert.argtypes = [C.c_int, f32p, f64p, C.c_int, i32p, f32p, f64p]
         L.orc_angles_cossq.argtypes = [C.c_int, f32p, f64p, C.c_int, i32p, f32p, f64p]
         L.orc_angles_sq.argtypes = [C.c_int, f32p, f64p, C.c_int, i32p, f32p, f64p]
+        L.orc_dihedrals_periodic.argtypes = [C.c_int, f32p, f64p, C.c_int, i32p, f32p, f64p]
         L.orc_forces_nl.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, i64p, i32p, f64p, f64p]
         L.orc_forces_cells.argtypes = [C.c_int, f32p, f64p, C.c_int, f32p, C.c_int, f64p, f64p]
         L.orc_forces_cells.restype = C.c_int
@@ -102,12 +103,13 @@ def forces_form_brute(pos4, L, nt, params, form, cutoff):
 
 
 def bonded(pos4, L, bonds=None, bond_params=None, angles=None, angle_params=None, fene=False, dihedrals=None,
-           dihedral_params=None, angle_sq=False):
+           dihedral_params=None, angle_sq=False, periodic_dihedrals=False):
     """FP64 intramolecular forces: bonds (m, 3) int (a, b, type) with
     bond_params (l0 or R0, k) per type (harmonic, or FENE when fene); angles
     (m, 4) int (a, b, c, type) with angle_params (theta0, k) per type;
     dihedrals (m, 5) int (a, b, c, d, type) with dihedral_params c_0..c_5 per
-    type; angle_sq selects the squared-angle form for the angles.  Returns
+    type (psi0, k1, k2, k3 with periodic_dihedrals); angle_sq selects the
+    squared-angle form for the angles.  Returns
     (f4 [n,4], vir [n])."""
     pos4 = _f32(pos4)
     n = pos4.shape[0]
@@ -118,7 +120,11 @@ def bonded(pos4, L, bonds=None, bond_params=None, angles=None, angle_params=None
         lib().orc_bonds(n, pos4, _box3(L), b.shape[0], b, _f32(bond_params).reshape(-1), int(fene), f4, vir)
     if dihedrals is not None and len(dihedrals):
         d = np.ascontiguousarray(dihedrals, dtype=np.int32).reshape(-1, 5)
-        lib().orc_dihedrals_ryckaert(n, pos4, _box3(L), d.shape[0], d, _f32(dihedral_params).reshape(-1), f4)
+        dp = np.zeros((len(dihedral_params), 6), np.float32)  # six per type, zero-padded
+        for t, row in enumerate(dihedral_params):
+            dp[t, :len(row)] = row
+        fn = lib().orc_dihedrals_periodic if periodic_dihedrals else lib().orc_dihedrals_ryckaert
+        fn(n, pos4, _box3(L), d.shape[0], d, dp.reshape(-1), f4)
     if angles is not None and len(angles):
         a = np.ascontiguousarray(angles, dtype=np.int32).reshape(-1, 4)
         fn = lib().orc_angles_sq if angle_sq else lib().orc_angles_cossq

@@ -30,9 +30,10 @@
  *     (gLJ 12-6 = LJA, IPL_n with n = 12 = IPL 12-6 with A6 = 0), the
  *     Buckingham minimum -epsilon at rm and the smooth LJ's vanishing force
  *     at Rcut.
- *   orc_bonds, orc_angles_cossq, orc_angles_sq, orc_dihedrals_ryckaert:
- *     closed forms for one bond (harmonic, FENE), one angle (both forms) and
- *     the trans / cis dihedral,
+ *   orc_bonds, orc_angles_cossq, orc_angles_sq, orc_dihedrals_ryckaert,
+ *     orc_dihedrals_periodic: closed forms for one bond (harmonic, FENE), one
+ *     angle (both forms) and the trans / cis dihedral (both forms; the
+ *     periodic form's force is -sin psi times the gradient force),
  *     -grad E = F by finite differences of every coordinate of a small
  *     molecule, zero total force and torque.
  *   orc_nblist_*: brute force vs cell scan, symmetry, superset of the rc sphere.
@@ -498,10 +499,13 @@ static void cross3(const double* x, const double* y, double* z) {
  * dphi/dr_a = -|b2| m / |m|^2, dphi/dr_d = |b2| n / |n|^2 and the two inner
  * atoms' terms (Blondel & Karplus, J. Comput. Chem. 17, 1132 (1996)).  The
  * energy goes to atom a; nearly collinear bonds (|cos| within 1e-3 of 1 on
- * either side) contribute nothing, as in the reference. */
-void orc_dihedrals_ryckaert(int n, const float* pos4, const double L[3], int ndih, const int32_t* dih,
-                            const float* params, double* f4) {
-  (void)n;
+ * either side) contribute nothing, as in the reference.
+ * periodic != 0: PeriodicDihedral (PeriodicDih, src/DihedralPotential.cu:337-458),
+ * params (psi0, k1, k2, k3) of six per type, psi = acos(cos psi) in [0, pi],
+ * u = 1/2 sum_m k_m (1 - cos m (psi - psi0)); the reference puts du/dpsi where
+ * the Ryckaert form has du/dcos psi, so the force is du/dpsi sin(phi) dphi/dr. */
+static void dihedrals(int ndih, const int32_t* dih, const float* pos4, const double L[3], const float* params,
+                      int periodic, double* f4) {
   for (int q = 0; q < ndih; ++q) {
     const int a = dih[5 * q], b = dih[5 * q + 1], c = dih[5 * q + 2], d = dih[5 * q + 3], t = dih[5 * q + 4];
     const float* cf = params + 6 * t;
@@ -525,10 +529,16 @@ void orc_dihedrals_ryckaert(int n, const float* pos4, const double L[3], int ndi
     const double phi = atan2(y, x);
     const double cpsi = -cos(phi); /* cos(phi - pi) */
     double u = 0.0, dudc = 0.0, pw = 1.0;
-    for (int k = 0; k < 6; ++k) {
-      u += cf[k] * pw;
-      if (k < 5) dudc += (k + 1) * cf[k + 1] * pw;
-      pw *= cpsi;
+    if (periodic) {
+      const double x = acos(cpsi) - cf[0];
+      u = 0.5 * (cf[1] * (1.0 - cos(x)) + cf[2] * (1.0 - cos(2.0 * x)) + cf[3] * (1.0 - cos(3.0 * x)));
+      dudc = 0.5 * (cf[1] * sin(x) + 2.0 * cf[2] * sin(2.0 * x) + 3.0 * cf[3] * sin(3.0 * x));
+    } else {
+      for (int k = 0; k < 6; ++k) {
+        u += cf[k] * pw;
+        if (k < 5) dudc += (k + 1) * cf[k + 1] * pw;
+        pw *= cpsi;
+      }
     }
     /* du/dphi = du/dcpsi * dcpsi/dphi = dudc * sin(phi) */
     const double dudphi = dudc * sin(phi);
@@ -548,6 +558,18 @@ void orc_dihedrals_ryckaert(int n, const float* pos4, const double L[3], int ndi
     }
     f4[4 * (size_t)a + 3] += u;
   }
+}
+
+void orc_dihedrals_ryckaert(int n, const float* pos4, const double L[3], int ndih, const int32_t* dih,
+                            const float* params, double* f4) {
+  (void)n;
+  dihedrals(ndih, dih, pos4, L, params, 0, f4);
+}
+
+void orc_dihedrals_periodic(int n, const float* pos4, const double L[3], int ndih, const int32_t* dih,
+                            const float* params, double* f4) {
+  (void)n;
+  dihedrals(ndih, dih, pos4, L, params, 1, f4);
 }
 
 /* Over a given neighbour list in CSR form (offsets[n+1], idx). */

@@ -234,7 +234,16 @@ __global__ __launch_bounds__(kBlock) void k_angle(const float4* __restrict__ r, 
 // products (cA, cB1, cB2) cancel for nearly planar or nearly collinear
 // geometries, which in FP32 costs several digits.  Each list entry is 5 u32:
 // a, b, c, d (original ids) and the type; params: c_0..c_5 per type.
-__global__ __launch_bounds__(kBlock) void k_dihedral_ryckaert(
+// DIH_PERIODIC (PeriodicDih, src/DihedralPotential.cu:337-458): psi =
+// acos(cos psi) in [0, pi], params (psi0, k1, k2, k3, 0, 0), u = 1/2 (k1 (1 -
+// cos x) + k2 (1 - cos 2x) + k3 (1 - cos 3x)) with x = psi - psi0, and the
+// reference's force: -du/dpsi in the place where the Ryckaert form has
+// -du/dcos psi, which is -sin psi times the gradient force.
+constexpr int DIH_RYCKAERT = 0;
+constexpr int DIH_PERIODIC = 1;
+
+template <int FORM>
+__global__ __launch_bounds__(kBlock) void k_dihedral(
     const float4* __restrict__ r, float4* __restrict__ f, float4* __restrict__ w, float4* __restrict__ sts,
     const uint32_t* __restrict__ slot_of, const uint32_t* __restrict__ start, const uint32_t* __restrict__ dih,
     const float* __restrict__ params, uint32_t ntypes, Box bx, uint32_t n, int init, int str) {
@@ -267,7 +276,16 @@ __global__ __launch_bounds__(kBlock) void k_dihedral_ryckaert(
       continue;
     const double cD = sqrt(cB1 * cB2);
     const double cc = cA / cD;
-    const double ff = -(p[1] + (2.0 * p[2] + (3.0 * p[3] + (4.0 * p[4] + 5.0 * p[5] * cc) * cc) * cc) * cc) * c22;
+    double ff, ud;
+    if (FORM == DIH_PERIODIC) {
+      const double ceq = cos(p[0]), seq = sin(p[0]), ss = sqrt(fabs(1.0 - cc * cc));
+      const double cx = cc * ceq + ss * seq, sx = ss * ceq - cc * seq;  // cos and sin of psi - psi0
+      ud = 0.5 * (p[1] * (1.0 - cx) + p[2] * (1.0 - cx * cx + sx * sx) + p[3] * (1.0 + 3.0 * cx - 4.0 * cx * cx * cx));
+      ff = (-0.5 * p[1] * sx - 2.0 * p[2] * sx * cx - 1.5 * p[3] * (3.0 * sx - 4.0 * sx * sx * sx)) * c22;
+    } else {
+      ud = p[0] + (p[1] + (p[2] + (p[3] + (p[4] + p[5] * cc) * cc) * cc) * cc) * cc;
+      ff = -(p[1] + (2.0 * p[2] + (3.0 * p[3] + (4.0 * p[4] + 5.0 * p[5] * cc) * cc) * cc) * cc) * c22;
+    }
     const double t1 = cA, t2 = c11 * c23 - c12 * c13, t3 = -cB1;
     const double t4 = cB2, t5 = c13 * c23 - c12 * c33, t6 = -cA;
     const double cr1 = c12 / c22, cr2 = c23 / c22;
@@ -279,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_dihedral_ryckaert(
     double ka = 0.0, kb = 0.0;  // own force = ka f1 + kb f2
     if (o == ia) {
       ka = 1.0;
-      u += p[0] + (p[1] + (p[2] + (p[3] + (p[4] + p[5] * cc) * cc) * cc) * cc) * cc;
+      u += ud;
     } else if (o == ib) {
       ka = -(1.0 + cr1);
       kb = cr2;
@@ -379,24 +397,43 @@ extern "C" int rumdx_bond_fene(const rumdx_particles* p, const uint32_t* slot_of
                                                stream);
 }
 
-extern "C" int rumdx_dihedral_ryckaert(const rumdx_particles* p, const uint32_t* slot_of_dev,
-                                       const uint32_t* dihedral_start_dev, const uint32_t* dihedral_atoms_dev,
-                                       const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
-                                       int calc_stresses, void* stream) {
+namespace {
+template <int FORM>
+int launch_dihedrals(const char* entry, const rumdx_particles* p, const uint32_t* slot_of_dev,
+                     const uint32_t* dihedral_start_dev, const uint32_t* dihedral_atoms_dev, const float* params_dev,
+                     uint32_t ntypes, const float box[8], int initialize, int calc_stresses, void* stream) {
   using namespace rumdx;
-  const int rc = check_common("rumdx_dihedral_ryckaert", p, slot_of_dev, dihedral_start_dev, dihedral_atoms_dev,
-                              params_dev, box);
+  const std::string e(entry);
+  const int rc = check_common(entry, p, slot_of_dev, dihedral_start_dev, dihedral_atoms_dev, params_dev, box);
   if (rc != RUMDX_OK) return rc;
-  if (ntypes == 0) return fail(RUMDX_E_INVALID, "rumdx_dihedral_ryckaert: no dihedral types");
-  if (calc_stresses && p->sts_dev == nullptr)
-    return fail(RUMDX_E_INVALID, "rumdx_dihedral_ryckaert: stresses requested but sts is null");
+  if (ntypes == 0) return fail(RUMDX_E_INVALID, e + ": no dihedral types");
+  if (calc_stresses && p->sts_dev == nullptr) return fail(RUMDX_E_INVALID, e + ": stresses requested but sts is null");
   if (p->n == 0) return RUMDX_OK;
-  hipLaunchKernelGGL(k_dihedral_ryckaert, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
+  hipLaunchKernelGGL((k_dihedral<FORM>), dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
                      reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<float4*>(p->f_dev),
                      reinterpret_cast<float4*>(p->w_dev), reinterpret_cast<float4*>(p->sts_dev), slot_of_dev,
                      dihedral_start_dev, dihedral_atoms_dev, params_dev, ntypes, make_box(box), p->n, initialize,
                      calc_stresses);
-  return check_launch("rumdx_dihedral_ryckaert");
+  return check_launch(entry);
+}
+}  // namespace
+
+extern "C" int rumdx_dihedral_ryckaert(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                                       const uint32_t* dihedral_start_dev, const uint32_t* dihedral_atoms_dev,
+                                       const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
+                                       int calc_stresses, void* stream) {
+  return launch_dihedrals<rumdx::DIH_RYCKAERT>("rumdx_dihedral_ryckaert", p, slot_of_dev, dihedral_start_dev,
+                                              dihedral_atoms_dev, params_dev, ntypes, box, initialize, calc_stresses,
+                                              stream);
+}
+
+extern "C" int rumdx_dihedral_periodic(const rumdx_particles* p, const uint32_t* slot_of_dev,
+                                       const uint32_t* dihedral_start_dev, const uint32_t* dihedral_atoms_dev,
+                                       const float* params_dev, uint32_t ntypes, const float box[8], int initialize,
+                                       int calc_stresses, void* stream) {
+  return launch_dihedrals<rumdx::DIH_PERIODIC>("rumdx_dihedral_periodic", p, slot_of_dev, dihedral_start_dev,
+                                              dihedral_atoms_dev, params_dev, ntypes, box, initialize, calc_stresses,
+                                              stream);
 }
 
 namespace {

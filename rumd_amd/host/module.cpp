@@ -260,6 +260,7 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetParams", &DihedralRyckaert::SetParams, py::arg("dihedral_type"), py::arg("coeffs"))
       .def("SetExclude", &DihedralRyckaert::SetExclude, py::arg("exclude"))
       .def("SetExclusions", &DihedralRyckaert::SetExclusions, py::arg("non_bond_pot"));
+  py::class_<PeriodicDihedral, DihedralRyckaert>(m, "PeriodicDihedral").def(py::init<>());
   py::class_<AngleCosSq, Potential>(m, "AngleCosSq")
       .def(py::init<>())
       .def("SetParams", &AngleCosSq::SetParams, py::arg("angle_type"), py::arg("theta0"), py::arg("ktheta"))

@@ -327,8 +327,11 @@ double AngleCosSq::GetPotentialEnergy() {
 // DihedralPotential::SetParams (src/DihedralPotential.cu:31-38): six
 // coefficients per dihedral type.
 void DihedralRyckaert::SetParams(unsigned dihedral_type, const std::vector<float>& coeffs) {
-  if (coeffs.size() != 6) throw RUMD_Error(GetID_String(), "SetParams", "Wrong number of parameters in SetParams");
+  if (coeffs.size() != num_params_)
+    throw RUMD_Error(GetID_String(), "SetParams", "Wrong number of parameters in SetParams");
   if (params_.size() < 6ull * (dihedral_type + 1)) params_.resize(6ull * (dihedral_type + 1), 0.f);
+  // six floats per type on the device, zero-padded (CopyParamsToGPU, src/DihedralPotential.cu:51-66)
+  std::fill(params_.begin() + 6ull * dihedral_type, params_.begin() + 6ull * (dihedral_type + 1), 0.f);
   std::copy(coeffs.begin(), coeffs.end(), params_.begin() + 6ull * dihedral_type);
   d_params_.resize(params_.size());
   d_params_.upload(params_.data(), params_.size(), sample ? sample->GetStream() : nullptr);
@@ -353,9 +356,10 @@ void DihedralRyckaert::Launch(const rumdx_particles* p, bool initialize, bool ca
   if (m->GetNumberOfDihedralTypes() > params_.size() / 6)
     throw RUMD_Error(GetID_String(), method, "a dihedral type of the topology has no parameters");
   const uint32_t* slot_of = m->SlotIndex();
-  rumdx_check(rumdx_dihedral_ryckaert(p, slot_of, m->DihedralStart(), m->DihedralAtoms(), d_params_.get(),
-                                      static_cast<uint32_t>(params_.size() / 6), sample->GetSimulationBox()->GetBox8(),
-                                      initialize ? 1 : 0, calc_stresses ? 1 : 0, sample->GetStream()),
+  auto* entry = periodic_ ? rumdx_dihedral_periodic : rumdx_dihedral_ryckaert;
+  rumdx_check(entry(p, slot_of, m->DihedralStart(), m->DihedralAtoms(), d_params_.get(),
+                    static_cast<uint32_t>(params_.size() / 6), sample->GetSimulationBox()->GetBox8(),
+                    initialize ? 1 : 0, calc_stresses ? 1 : 0, sample->GetStream()),
               GetID_String().c_str(), method);
 }
 
@@ -373,5 +377,5 @@ double DihedralRyckaert::GetPotentialEnergy() {
   p.f_dev = pe_f_.get();
   p.w_dev = pe_w_.get();
   Launch(&p, true, false, "GetPotentialEnergy");
-  return sum_energy(sample, pe_f_.get(), "DihedralRyckaert");
+  return sum_energy(sample, pe_f_.get(), GetID_String().c_str());
 }

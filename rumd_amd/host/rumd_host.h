@@ -577,12 +577,29 @@ class DihedralRyckaert : public Potential {
   void SetExclude(bool exclude) { exclude_dihedral_ = exclude; }
   void SetExclusions(PairPotential* non_bond_pot);
 
+ protected:
+  size_t num_params_ = 6;  // coefficients SetParams takes (DihedralPotential's num_dihedral_params)
+  bool periodic_ = false;  // PeriodicDihedral
+
  private:
   void Launch(const rumdx_particles* p, bool initialize, bool calc_stresses, const char* method);
   std::vector<float> params_;  // c_0..c_5 per type
   bool exclude_dihedral_ = true;
   DeviceArray<float> d_params_;
   DeviceArray<float> pe_f_, pe_w_;
+};
+
+// PeriodicDihedral (src/DihedralPotential.cu:265-458): params (psi0, k1, k2,
+// k3), u = 1/2 sum_m k_m (1 - cos m (psi - psi0)) with the reference's force,
+// which is -sin psi times the gradient force (a property of that kernel,
+// reproduced here).
+class PeriodicDihedral : public DihedralRyckaert {
+ public:
+  PeriodicDihedral() {
+    SetID_String("PeriodicDihedral");
+    num_params_ = 4;
+    periodic_ = true;
+  }
 };
 
 // AngleCosSq (include/rumd/AnglePotential.h, src/AnglePotential.cu:69-98):

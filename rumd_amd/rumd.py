@@ -48,6 +48,7 @@ AngleCosSq = _rumd.AngleCosSq
 AngleSq = _rumd.AngleSq
 BondFENE = _rumd.BondFENE
 DihedralRyckaert = _rumd.DihedralRyckaert
+PeriodicDihedral = _rumd.PeriodicDihedral
 MoleculeData = _rumd.MoleculeData
 Integrator = _rumd.Integrator
 IntegratorNVT = _rumd.IntegratorNVT
@@ -230,6 +231,6 @@ __all__ = [
     "Device", "SimulationBox", "RectangularSimulationBox", "Potential", "PairPotential", "Pot_IPL_12_6",
     "Pot_LJ_12_6", "Pot_gLJ_m12_n6", "Pot_LJ_12_6_smooth_2_4", "Pot_gLJ_m_n", "Pot_Gauss", "Pot_Buckingham",
     "Pot_IPL_n", "Pot_Yukawa", "Pot_Exp", "Pot_LJ_12_6_Gauss", "Pot_IPL_12", "Pot_IPL_9", "Pot_IPL_18", "Pot_IPL6_LN", "Pot_LJ_LN", "Pot_Girifalco", "Pot_Dzugutov", "Pot_Fermi_Jagla", "Pot_Repulsive_Shoulder", "Pot_SAAP", "Pot_ShiftedForceCoulomb",
-    "BondHarmonic", "BondFENE", "AngleCosSq", "AngleSq", "DihedralRyckaert", "MoleculeData", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
+    "BondHarmonic", "BondFENE", "AngleCosSq", "AngleSq", "DihedralRyckaert", "PeriodicDihedral", "MoleculeData", "Integrator", "IntegratorNVT", "IntegratorNVE", "NoShift", "ShiftedPotential",
     "ShiftedForce", "GetVersion", "NumberOfPairPotentialParameters", "Sample", "ReplicaBatch",
 ]

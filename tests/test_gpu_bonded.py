@@ -228,6 +228,7 @@ def test_simulation_sets_exclusions_and_runs(golden_dir, tmp_path, monkeypatch):
 
 
 DIH_PARAMS = [(0.9, 2.1, -0.4, -3.0, 0.2, 0.1), (1.2, -1.0, 0.6, 0.9, -0.5, 0.3)]
+PERIODIC_PARAMS = [(0.4, 2.0, -0.7, 0.9), (1.0, 1.5, 0.5, -0.4)]  # psi0, k1, k2, k3
 
 
 def _dihedrals():
@@ -235,8 +236,9 @@ def _dihedrals():
                      for m in range(200) for i in range(2)], np.int32)
 
 
+@pytest.mark.parametrize("dihedral_form", ["ryckaert", "periodic"])
 @pytest.mark.parametrize("method", ["n2", "sort"])
-def test_fene_and_harmonic_types_and_dihedrals(golden_dir, method):
+def test_fene_and_harmonic_types_and_dihedrals(golden_dir, method, dihedral_form):
     """Bond type 0 through BondFENE, type 1 through BondHarmonic (a type belongs
     to the potential that set it), Ryckaert dihedrals with their end pairs
     excluded from the LJ list, all against the oracle."""
@@ -260,15 +262,17 @@ def test_fene_and_harmonic_types_and_dihedrals(golden_dir, method):
     harm = rumd.BondHarmonic()
     harm.SetVerbose(False)
     harm.SetParams(bond_type=1, bond_length=1.1, stiffness=150.0)
-    rb = rumd.DihedralRyckaert()
+    periodic = dihedral_form == "periodic"
+    dparams = PERIODIC_PARAMS if periodic else DIH_PARAMS
+    rb = rumd.PeriodicDihedral() if periodic else rumd.DihedralRyckaert()
     rb.SetVerbose(False)
-    for t, cf in enumerate(DIH_PARAMS):
+    for t, cf in enumerate(dparams):
         rb.SetParams(dihedral_type=t, coeffs=list(cf))
     for q in (fene, harm, rb):
         s.AddPotential(q)
     rb.SetExclusions(pot)
     assert pot.GetNumberOfExclusions() == len(dih)
-    print(f"[case] FENE + harmonic + dihedrals, list {method}", flush=True)
+    print(f"[case] FENE + harmonic + dihedrals ({dihedral_form}), list {method}", flush=True)
     s.CalcF()
     f = s.GetForcesAndEnergies().astype(np.float64)
     bonds, _ = _topology(golden_dir)
@@ -278,7 +282,7 @@ def test_fene_and_harmonic_types_and_dihedrals(golden_dir, method):
     f_lj, vir_lj, _ = _oracle_lj_excluding(c, pos4, table, excl)
     f_f, vir_f = oracle.bonded(pos4, c.box, bonds[bonds[:, 2] == 0], [(1.5, 30.0), (0.0, 0.0)], fene=True)
     f_h, vir_h = oracle.bonded(pos4, c.box, bonds[bonds[:, 2] == 1], [(0.0, 0.0), (1.1, 150.0)])
-    f_d, _ = oracle.bonded(pos4, c.box, dihedrals=dih, dihedral_params=DIH_PARAMS)
+    f_d, _ = oracle.bonded(pos4, c.box, dihedrals=dih, dihedral_params=dparams, periodic_dihedrals=periodic)
     f_ref = f_lj + f_f + f_h + f_d
     e = force_errors(f[:, :3], f_ref[:, :3])
     assert e["guarded_max"] <= FORCE_TOL_GUARDED, e

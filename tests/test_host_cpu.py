@@ -117,3 +117,16 @@ def test_polymer_melt_generator():
     assert np.abs(s.velocities.mean(axis=0)).max() < 1e-6
     with pytest.raises(ValueError):
         configs.polymer_melt(n_chains=10, beads=10, grid=(7, 7, 2))
+
+
+def test_bonded_class_ids_and_parameter_counts():
+    """ID strings as the reference sets them, and DihedralPotential's parameter
+    count check (six for Ryckaert, four for the periodic form)."""
+    assert rumd.AngleCosSq().GetID_String() == "AngleCosSq"
+    assert rumd.AngleSq().GetID_String() == "AngleSq"
+    assert rumd.BondFENE().GetID_String() == "bondFene"
+    assert rumd.PeriodicDihedral().GetID_String() == "PeriodicDihedral"
+    with pytest.raises(RuntimeError, match="Wrong number of parameters"):
+        rumd.PeriodicDihedral().SetParams(dihedral_type=0, coeffs=[0.1] * 6)
+    with pytest.raises(RuntimeError, match="Wrong number of parameters"):
+        rumd.DihedralRyckaert().SetParams(dihedral_type=0, coeffs=[0.1] * 4)

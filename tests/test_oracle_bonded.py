@@ -181,3 +181,54 @@ def test_squared_angle_closed_form_and_gradient():
             xm[i, c] -= h
             g = (energy(xp) - energy(xm)) / (float(xp[i, c]) - float(xm[i, c]))
             assert f4[i, c] == pytest.approx(-g, rel=2e-4, abs=2e-4), (i, c)
+
+
+PERIODIC = [[0.4, 2.0, -0.7, 0.9]]  # psi0, k1, k2, k3
+
+
+def _periodic_u(psi, c=PERIODIC[0]):
+    x = psi - np.float32(c[0])
+    return 0.5 * sum(np.float32(c[m]) * (1 - np.cos(m * x)) for m in (1, 2, 3))
+
+
+def _psi(x):
+    x = x.astype(np.float64)
+    b1, b2, b3 = x[1] - x[0], x[2] - x[1], x[3] - x[2]
+    m, n = np.cross(b1, b2), np.cross(b2, b3)
+    return np.arccos(-(m @ n) / np.linalg.norm(m) / np.linalg.norm(n))
+
+
+def test_periodic_dihedral_closed_form():
+    for phi, psi in ((np.pi, 0.0), (0.0, np.pi), (np.pi / 2, np.pi / 2), (-2.0, np.pi - 2.0)):
+        f4, vir = oracle.bonded(_pos4(_chain4(phi)), L, dihedrals=[[0, 1, 2, 3, 0]], dihedral_params=PERIODIC,
+                                periodic_dihedrals=True)
+        assert f4[0, 3] == pytest.approx(_periodic_u(psi), rel=1e-6, abs=1e-6), phi
+        assert f4[1:, 3].sum() == 0.0 and not vir.any()
+
+
+def test_periodic_dihedral_force_is_minus_sin_psi_times_the_gradient_force():
+    """The reference's PeriodicDih puts du/dpsi where the Ryckaert kernel has
+    du/dcos psi (src/DihedralPotential.cu:422 against :221), so its force
+    is -sin psi times -grad u; the oracle restates that force, and this pins it
+    to the energy by finite differences."""
+    rng = np.random.default_rng(9)
+    for phi in (0.5, 1.7, -2.4):
+        x = _chain4(phi) + rng.normal(0.0, 0.05, size=(4, 3)).astype(np.float32)
+
+        def energy(xx):
+            f, _ = oracle.bonded(_pos4(xx), L, dihedrals=[[0, 1, 2, 3, 0]], dihedral_params=PERIODIC,
+                                 periodic_dihedrals=True)
+            return f[:, 3].sum()
+
+        f4, _ = oracle.bonded(_pos4(x), L, dihedrals=[[0, 1, 2, 3, 0]], dihedral_params=PERIODIC,
+                              periodic_dihedrals=True)
+        assert np.abs(f4[:, :3].sum(axis=0)).max() < 1e-10 * max(1.0, np.abs(f4).max())
+        s = np.sin(_psi(x))
+        h = 1e-3
+        for i in range(4):
+            for c in range(3):
+                xp, xm = x.copy(), x.copy()
+                xp[i, c] += h
+                xm[i, c] -= h
+                g = (energy(xp) - energy(xm)) / (float(xp[i, c]) - float(xm[i, c]))
+                assert f4[i, c] == pytest.approx(-s * -g, rel=2e-4, abs=2e-4), (phi, i, c)
