@@ -9,9 +9,13 @@ forces; FP64 energy/virial/kinetic sums, thermostat state and position
 bookkeeping ("mixed").
 
 A step is CalcF + Integrate (with the device-side skin check and, when it
-fires, the sort + list rebuild), run as HIP-graph-captured blocks of 10 steps;
-momentum is reset every 100 steps as RUMD's Simulation.Run does.  W warm-up
-steps are untimed; the K timed steps are bracketed by stream synchronisation.
+fires, the sort + list rebuild), run as HIP-graph-captured blocks of 10 steps
+that start at the first timed step; momentum is reset every 100 steps (at
+absolute step multiples of 100) as RUMD's Simulation.Run does.  Set-up runs
+E equilibration steps from the lattice (the liquid the timed steps measure),
+then W warm-up steps; the graphs of the timed steps are captured and
+instantiated after the warm-up, before the timer starts; the K timed steps
+are bracketed by stream synchronisation.
 
 Also reported:
   roofline     the force kernel's algorithmic HBM bytes per launch
@@ -19,7 +23,7 @@ Also reported:
                duration, timed with HIP events on the sample's stream;
   cpu_baseline the FP64 CPU MD of the oracle (same algorithm: Verlet list with
                skin, cells, Nose-Hoover leapfrog) on the same start, on all host
-               cores, over a bounded number of steps.
+               cores and on one core, over bounded samples.
 """
 from __future__ import annotations
 
@@ -64,47 +68,74 @@ def make_sample(conf, mixed: bool):
     return sample, pot, itg
 
 
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(system, budget_s: float, max_steps: int):
-    """FP64 CPU MD (oracle) on the same start; steps bounded by a time budget."""
+    """FP64 CPU MD (oracle) on the same start; steps bounded by a time budget,
+    on all host threads and on one thread."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
     from rumd_amd import configs
     table = configs.lj_param_table(configs.ka_params(2.5), 2)
     threads = oracle.num_threads()
-    pos = system.positions.astype(np.float64).copy()
-    vel = system.velocities.astype(np.float64).copy()
     inv_m = np.ones(system.n)
     omega2 = float(np.float32(4 * np.pi ** 2 / 0.2 ** 2))
-    # one short run to size the sample (includes a list build)
-    t0 = time.perf_counter()
-    _, _, ps = oracle.md_run(pos, vel, system.types, inv_m, system.box, 2, table, 2, 0.3, 0.0025, 2, True, 0.45,
-                             omega2, 0.0)
-    t_probe = time.perf_counter() - t0
-    steps = int(max(3, min(max_steps, budget_s / max(t_probe / 2.0, 1e-6))))
-    t0 = time.perf_counter()
-    builds, _, _ = oracle.md_run(pos, vel, system.types, inv_m, system.box, 2, table, 2, 0.3, 0.0025, steps, True,
-                                 0.45, omega2, ps)
-    dt = time.perf_counter() - t0
+
+    def timed(budget, cap):
+        pos = system.positions.astype(np.float64).copy()
+        vel = system.velocities.astype(np.float64).copy()
+        # one short run to size the sample (includes a list build)
+        t0 = time.perf_counter()
+        _, _, ps = oracle.md_run(pos, vel, system.types, inv_m, system.box, 2, table, 2, 0.3, 0.0025, 2, True,
+                                 0.45, omega2, 0.0)
+        t_probe = time.perf_counter() - t0
+        steps = int(max(3, min(cap, budget / max(t_probe / 2.0, 1e-6))))
+        t0 = time.perf_counter()
+        builds, _, _ = oracle.md_run(pos, vel, system.types, inv_m, system.box, 2, table, 2, 0.3, 0.0025, steps,
+                                     True, 0.45, omega2, ps)
+        dt = time.perf_counter() - t0
+        return system.n * steps / dt, steps, builds
+
+    value, steps, builds = timed(budget_s, max_steps)
+    oracle.set_num_threads(1)
+    try:
+        value1, steps1, builds1 = timed(budget_s / 2.0, max_steps)
+    finally:
+        oracle.set_num_threads(threads)
     return {
-        "value": system.n * steps / dt,
+        "value": value,
         "unit": "atom-timesteps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"KA N={system.n}, {steps} NVT steps (FP64, Verlet list skin 0.3, {builds} list builds incl. "
                   f"the first), same start as the GPU run, {threads} OpenMP threads",
+        "value_1core": value1,
+        "sample_1core": f"{steps1} NVT steps ({builds1} list builds) on 1 thread",
+        "cpu_model": _cpu_model(),
+        "host_cpus": os.cpu_count(),
     }
 
 
 def read_traffic(profile_dir: Path):
     """HBM bytes per force-kernel launch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)."""
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, each calibrated
+    against kernels of known traffic); None when absent."""
     f = profile_dir / "force_kernel_ka1m_pmc.json"
     if not f.exists():
-        return None
+        return None, None
     try:
-        return float(json.loads(f.read_text())["hbm_bytes_per_launch"])
+        d = json.loads(f.read_text())
+        return float(d["hbm_bytes_per_launch"]), str(d.get("kernel", ""))
     except (KeyError, ValueError):
-        return None
+        return None, None
 
 
 def counter_calibration(n: int, reps: int) -> dict:
@@ -174,8 +205,9 @@ def force_accuracy(sample, pot, system):
     return out
 
 
-def nve_drift(sample, nsteps: int, dt: float):
-    """Total energy per particle over nsteps NVE steps from the current state
+def nve_drift(sample, nsteps: int, dt: float, nh_steps: int):
+    """Total energy per particle over nsteps NVE steps from the current state,
+    which the caller has equilibrated with nh_steps Nose-Hoover steps
     (momentum reset off): (E_end - E_start) / N per 1000 steps."""
     import rumd_amd.rumd as rumd
     nve = rumd.IntegratorNVE(timeStep=dt)
@@ -186,7 +218,8 @@ def nve_drift(sample, nsteps: int, dt: float):
     sample.RunSteps(0, nsteps, 10, 0)
     sample.CalcF()
     e1 = (sample.GetPotentialEnergy() + nve.GetKineticEnergy()) / n
-    return {"e_start": e0, "e_end": e1, "drift_per_1000_steps": (e1 - e0) * 1000.0 / nsteps, "steps": nsteps}
+    return {"e_start": e0, "e_end": e1, "drift_per_1000_steps": (e1 - e0) * 1000.0 / nsteps, "steps": nsteps,
+            "after_nh_steps": nh_steps}
 
 
 def make_batch(systems, temps, dt: float, tau: float, skin: float):
@@ -233,6 +266,16 @@ def batched_leg(nrep: int, warmup: int, steps: int, sample_reps: int, single_ste
     batched = ntot * steps / el
     print(f"[bench] batched {nrep} replicas, N_total={ntot}: {steps} steps in {el:.3f} s -> "
           f"{batched / 1e9:.3f} G atom-steps/s", file=sys.stderr, flush=True)
+    # the batch force kernel against the HBM roofline: RUMD-layout bytes per
+    # particle (16 + 4 + 4 nbar + 16 + 16) over its event-timed duration
+    nbar_b, _ = b.GetNeighborListStats()
+    fms = b.TimeForceKernel(20)
+    bpp = 16 + 4 + 4 * nbar_b + 16 + 16
+    ach = bpp * ntot / (fms * 1e-3) / 1e9
+    batch_roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "k_b_force (batched LJ force)",
+                      "kernel_ms": round(fms, 5), "bytes_per_atom": round(bpp, 2), "nbar": round(nbar_b, 2)}
+    print(f"[bench] batch force kernel {fms * 1e3:.1f} us, {ach:.0f} GB/s", file=sys.stderr, flush=True)
     picks = list(range(0, nrep, max(1, nrep // sample_reps)))[:sample_reps]
     atom_steps, seconds = 0.0, 0.0
     for r in picks:
@@ -271,6 +314,7 @@ def batched_leg(nrep: int, warmup: int, steps: int, sample_reps: int, single_ste
                           "how": "a Sample per replica (default list method), graph blocks of 10, timed per replica "
                                  "and aggregated over an evenly spaced subset"},
         "speedup": round(batched / single, 2),
+        "roofline": batch_roofline,
         "lanes_per_particle": tuned["tp"],
         "autotune_ms_per_step": {str(t["params"]["tp"]): round(t["ms_per_step"], 4) for t in at.trials},
         "workload": "KA replicas N~U[256,8192] (seed 4001), T~U[0.4,2.0] (seed 4002), rho=1.2, SF rc=2.5, "
@@ -334,6 +378,48 @@ def polymer_leg(warmup: int, steps: int, block: int) -> dict:
     return out
 
 
+def timed_run(sample, pot, args, label: str) -> dict:
+    """Equilibration and warm-up (untimed), the timed steps' graphs captured
+    and instantiated, then K timed steps bracketed by stream synchronisation."""
+    n = sample.GetNumberOfParticles()
+    t0 = time.perf_counter()
+    half = args.equilibrate // 2
+    sample.RunSteps(0, half, args.block, 100)
+    b_half = pot.GetNumberOfNbListBuilds()
+    sample.RunSteps(half, args.equilibrate - half, args.block, 100)
+    # rebuild interval of the liquid, over the second half of the equilibration
+    eq_builds = pot.GetNumberOfNbListBuilds() - b_half
+    kbar_eq = (args.equilibrate - half) / eq_builds if eq_builds else None
+    print(f"[bench] {label}: equilibration {args.equilibrate} steps in {time.perf_counter() - t0:.2f} s "
+          f"(second half: {eq_builds} list builds)", file=sys.stderr, flush=True)
+    first = args.equilibrate
+    t0 = time.perf_counter()
+    sample.RunSteps(first, args.warmup, args.block, 100)
+    print(f"[bench] {label}: warm-up {args.warmup} steps in {time.perf_counter() - t0:.2f} s", file=sys.stderr,
+          flush=True)
+    first += args.warmup
+    # every graph the timed steps launch is captured and instantiated now
+    t0 = time.perf_counter()
+    made = sample.PrepareRunSteps(first, args.steps, args.block, 100)
+    segments = sample.RunSegments(first, args.steps, args.block, 100)
+    eager = sum(k for k, _ in segments if k < 2)
+    captured = args.steps - eager
+    print(f"[bench] {label}: {made} graphs captured before timing in {time.perf_counter() - t0:.3f} s; timed "
+          f"steps: {captured} in {sum(1 for k, _ in segments if k >= 2)} graph launches, {eager} eager",
+          file=sys.stderr, flush=True)
+    builds0 = pot.GetNumberOfNbListBuilds()
+    sample.Synchronize()
+    t0 = time.perf_counter()
+    sample.RunSteps(first, args.steps, args.block, 100)  # synchronises at the end
+    elapsed = time.perf_counter() - t0
+    builds = pot.GetNumberOfNbListBuilds() - builds0
+    value = n * args.steps / elapsed
+    print(f"[bench] {label}: timed {args.steps} steps in {elapsed:.4f} s -> {value / 1e9:.3f} G atom-steps/s "
+          f"({builds} list builds)", file=sys.stderr, flush=True)
+    return {"value": value, "elapsed": elapsed, "builds": builds, "captured": captured, "eager": eager,
+            "kbar_equilibration": kbar_eq, "nh_steps": first + args.steps}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--calibrate-counters", type=int, default=0, metavar="N",
@@ -341,6 +427,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--equilibrate", type=int, default=1000,
+                    help="NVT steps from the lattice before the warm-up (set-up: the timed steps run on the liquid)")
     ap.add_argument("--cells", type=int, default=64, help="FCC cells per axis (N = 4 cells^3)")
     ap.add_argument("--precision", choices=["mixed", "fp32"], default="mixed")
     ap.add_argument("--block", type=int, default=10)
@@ -380,62 +468,54 @@ def main() -> int:
     n = sample.GetNumberOfParticles()
     print(f"[bench] KA N={n} L={system.box[0]:.5f} precision={args.precision}", file=sys.stderr, flush=True)
 
-    t0 = time.perf_counter()
-    sample.RunSteps(0, args.warmup, args.block, 100)
-    print(f"[bench] warm-up {args.warmup} steps in {time.perf_counter() - t0:.2f} s", file=sys.stderr, flush=True)
-
-    builds0 = pot.GetNumberOfNbListBuilds()
-    sample.Synchronize()
-    t0 = time.perf_counter()
-    sample.RunSteps(args.warmup, args.steps, args.block, 100)  # synchronises at the end
-    elapsed = time.perf_counter() - t0
-    builds = pot.GetNumberOfNbListBuilds() - builds0
-    value = n * args.steps / elapsed
-    print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s -> {value / 1e9:.3f} G atom-steps/s "
-          f"({builds} list builds)", file=sys.stderr, flush=True)
+    run = timed_run(sample, pot, args, "headline")
+    value, elapsed, builds = run["value"], run["elapsed"], run["builds"]
+    captured, eager = run["captured"], run["eager"]
 
     nbar, nmax = pot.GetNeighborListStats()
     force_ms = pot.TimeForceKernel(args.force_reps)
     build_ms = pot.TimeNbListBuild(10)
     bytes_per_atom = 16 + 4 + 4 * nbar + 16 + 16
     achieved = bytes_per_atom * n / (force_ms * 1e-3) / 1e9
+    traffic, traffic_kernel = read_traffic(ROOT / "profiles")
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": read_traffic(ROOT / "profiles"),
+        "traffic": traffic,
+        "traffic_source": "committed rocprofv3 PMC summary of this workload (profiles/force_kernel_ka1m_pmc.md: "
+                          "FETCH_SIZE + WRITE_SIZE per launch, calibrated); not measured in this run",
         "kernel": "k_pair_tiles<LjPair<SF>> (LJ force over the cell-tile lists)",
         "kernel_ms": round(force_ms, 5),
         "bytes_per_atom": round(bytes_per_atom, 2),
     }
+    # steps per rebuild: over the timed steps, and over the second half of the
+    # equilibration (a longer window; the byte model uses it)
     kbar = args.steps / builds if builds else None
-    step_bytes = bytes_per_atom + 64 + 48 + 16 + (4 * nbar + 260) / (kbar if kbar else float("inf"))
+    kbar_model = run["kbar_equilibration"] or kbar
+    rebuild_bytes = 4 * nbar + 260
+    step_bytes = bytes_per_atom + (64 + 48 + 4) + 32 + (rebuild_bytes / kbar_model if kbar_model else 0.0)
     pe = sample.GetPotentialEnergy() / n
     ke = itg.GetKineticEnergy() / n
 
     extras = {}
     if not args.no_extras:
         print("[bench] NVE drift ...", file=sys.stderr, flush=True)
-        drift = {args.precision: nve_drift(sample, 1000, 0.0025)}
+        drift = {args.precision: nve_drift(sample, 1000, 0.0025, run["nh_steps"])}
         if args.precision == "fp32":
             print("[bench] force accuracy vs the FP64 oracle ...", file=sys.stderr, flush=True)
             extras["force_error_vs_fp64_oracle"] = force_accuracy(sample, pot, system)
         if args.precision == "mixed":
             s32, p32, _ = make_sample(conf, False)
-            s32.RunSteps(0, args.warmup, args.block, 100)
-            s32.Synchronize()
-            t0 = time.perf_counter()
-            s32.RunSteps(args.warmup, args.steps, args.block, 100)
-            el32 = time.perf_counter() - t0
-            extras["fp32_positions"] = {"value": round(n * args.steps / el32, 1), "unit": "atom-timesteps/s",
-                                        "ms_per_step": round(el32 / args.steps * 1e3, 5)}
-            print(f"[bench] FP32 positions: {n * args.steps / el32 / 1e9:.3f} G atom-steps/s", file=sys.stderr,
-                  flush=True)
+            r32 = timed_run(s32, p32, args, "FP32 positions")
+            extras["fp32_positions"] = {"value": round(r32["value"], 1), "unit": "atom-timesteps/s",
+                                        "ms_per_step": round(r32["elapsed"] / args.steps * 1e3, 5),
+                                        "list_builds": r32["builds"]}
             print("[bench] force accuracy vs the FP64 oracle ...", file=sys.stderr, flush=True)
             extras["force_error_vs_fp64_oracle"] = force_accuracy(s32, p32, system)
-            drift["fp32"] = nve_drift(s32, 1000, 0.0025)
+            drift["fp32"] = nve_drift(s32, 1000, 0.0025, r32["nh_steps"])
             del s32, p32
         extras["nve_drift"] = drift
         print("[bench] batched replicas ...", file=sys.stderr, flush=True)
@@ -471,9 +551,14 @@ def main() -> int:
             "nmax": int(nmax),
             "list_builds": builds,
             "kbar": round(kbar, 2) if kbar else None,
+            "kbar_equilibration": round(run["kbar_equilibration"], 2) if run["kbar_equilibration"] else None,
             "halo_capacity": pot.GetNbHaloCapacity(),
             "list_rebuild_ms": round(build_ms, 4),
+            "equilibration_steps": args.equilibrate,
+            "timed_steps_captured": captured,
+            "timed_steps_eager": eager,
             "model_bytes_per_atom_step": round(step_bytes, 1),
+            "model_terms": "force 16+4+4nbar+16+16, NVT 64+48+4, skin check 32, rebuild (4nbar+260)/kbar",
             "model_hbm_fraction_step": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
             "pe_per_particle": round(pe, 6),
             "ke_per_particle": round(ke, 6),

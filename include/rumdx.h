@@ -158,6 +158,14 @@ typedef struct rumdx_nblist {
 /* Cells per tile edge and the largest LDS halo a launch may request. */
 #define RUMDX_TILE_EDGE 4
 #define RUMDX_HALO_CAP_MAX 3072
+/* tile_info_dev[4 t + 1] of a sorted build: bit 0 set when the tile's halo
+ * was staged and every pair takes the plain FP32 difference (in the shifted
+ * frame where the halo wraps); bits 1-6 mark, per axis, a halo wrapping below
+ * (bit 2a+1) or above (bit 2a+2) the tile; bits 16-31 hold the halo capacity
+ * the build used.  tile_info_dev[4 t] is the tile's halo size, [4 t + 2] and
+ * [4 t + 3] its slot range. */
+#define RUMDX_TILE_FAST 1u
+#define RUMDX_TILE_SHIFT_MASK 0x7Eu
 
 /* Device-resident Nose-Hoover state (IntegratorNVT, src/IntegratorNVT.cu:82-92, 426-437). */
 typedef struct rumdx_nh_state {

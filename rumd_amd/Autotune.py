@@ -18,9 +18,11 @@ Typical use, as with RUMD:
     at = Autotune()            # or Autotune(skin=0.3) to fix a parameter
     at.Tune(sim)
 
-Tuning runs advance a copy of the state: the configuration and the
-thermostat state are restored afterwards, so the user's trajectory is the
-one it would have been with the tuned parameters set from the start.
+Tuning runs advance a copy of the state: the configuration (positions in
+full, with their FP64 low-order parts in mixed precision), the thermostat
+state and the target temperature (which a temperature ramp advances every
+step) are restored afterwards, so the user's trajectory is the one it would
+have been with the tuned parameters set from the start.
 
 TuneBatch(batch) does the same for a ReplicaBatch (batched replicas, which
 RUMD does not have): it times lanes per particle of the batch force kernel
@@ -185,8 +187,7 @@ class Autotune:
         build; the state is restored after every trial."""
         s = sim.sample
         n = s.GetNumberOfParticles()
-        conf = s.GetConfiguration()
-        ps = sim.itg.GetThermostatState() if hasattr(sim.itg, "GetThermostatState") else None
+        saved = self._save(sim)
         current = self.GetCurrentTechnicalParameters(sim)
         best = dict(current)
         for key, val in self.fixed.items():
@@ -197,10 +198,10 @@ class Autotune:
             [m for m in _METHODS if (m != "none" or n <= self.size_stop_no_nb) and (m != "n2" or n <= self.size_stop_n2)]
         tps = [self.fixed["tp"]] if "tp" in self.fixed else list(_TP)
         self.trials = []
-        nsteps = self._trial_steps(sim, best, conf, ps)
+        nsteps = self._trial_steps(sim, best, saved)
 
         def trial(cand):
-            t = self._time(sim, cand, conf, ps, nsteps)
+            t = self._time(sim, cand, saved, nsteps)
             self.trials.append({"params": dict(cand), "ms_per_step": t})
             if self.verbose:
                 print(f"[Autotune] {cand} -> {t:.4f} ms/step")
@@ -220,7 +221,7 @@ class Autotune:
             if "tp" not in self.fixed:
                 scored = [(trial(dict(best, tp=tp)), dict(best, tp=tp)) for tp in tps]
                 best = min(scored, key=lambda x: x[0])[1]
-        self._restore(sim, conf, ps)
+        self._restore(sim, saved)
         return best
 
     # --------------------------------------------------------------- helpers
@@ -233,21 +234,38 @@ class Autotune:
             return []
         return data if isinstance(data, list) else []
 
-    def _restore(self, sim, conf, ps):
-        sim.sample.SetConfiguration(conf)
-        if ps is not None:
-            sim.itg.SetThermostatState(ps)
+    @staticmethod
+    def _save(sim):
+        """What tuning trials change besides the technical parameters: the
+        configuration, the FP64 positions, Ps and the target temperature."""
+        s = sim.sample
+        itg = sim.itg
+        return {
+            "conf": s.GetConfiguration(),
+            "pos64": s.GetPositionsFP64(),
+            "ps": itg.GetThermostatState() if hasattr(itg, "GetThermostatState") else None,
+            "T": itg.GetTargetTemperature() if hasattr(itg, "GetTargetTemperature") else None,
+        }
 
-    def _trial_steps(self, sim, params, conf, ps) -> int:
+    @staticmethod
+    def _restore(sim, saved):
+        sim.sample.SetConfiguration(saved["conf"])
+        sim.sample.SetPositionsFP64(saved["pos64"])
+        if saved["T"] is not None:
+            sim.itg.SetTargetTemperature(saved["T"], False)
+        if saved["ps"] is not None:
+            sim.itg.SetThermostatState(saved["ps"])
+
+    def _trial_steps(self, sim, params, saved) -> int:
         """Steps per trial for about trial_seconds, from one short probe."""
         probe = 20
-        t = self._time(sim, params, conf, ps, probe)  # ms per step
+        t = self._time(sim, params, saved, probe)  # ms per step
         steps = int(self.trial_seconds * 1e3 / max(t, 1e-3))
         return max(40, min(4000, 10 * math.ceil(steps / 10)))
 
-    def _time(self, sim, params, conf, ps, nsteps) -> float:
+    def _time(self, sim, params, saved, nsteps) -> float:
         self.SetTechnicalParameters(sim, params)
-        self._restore(sim, conf, ps)
+        self._restore(sim, saved)
         s = sim.sample
         s.RunSteps(0, 20, sim.graph_block, 0)  # warm-up: list build, graph capture
         s.Synchronize()

@@ -39,6 +39,10 @@ struct TileGeom {
 constexpr uint32_t kTileFast = 1u;
 constexpr uint32_t kTileShiftDown = 2u;
 constexpr uint32_t kTileShiftUp = 4u;
+static_assert(kTileFast == RUMDX_TILE_FAST, "tile flag bits of the header");
+static_assert((((kTileShiftDown | kTileShiftUp) << 0) | ((kTileShiftDown | kTileShiftUp) << 2) |
+               ((kTileShiftDown | kTileShiftUp) << 4)) == RUMDX_TILE_SHIFT_MASK,
+              "tile shift bits of the header");
 
 // The shifted frame puts a wrapping halo on one side of the periodic
 // boundary, so plain FP32 differences there are the correctly rounded

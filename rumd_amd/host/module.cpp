@@ -136,6 +136,19 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetNbHaloCapacity", [](PairPotential& p, unsigned cap) { p.GetNeighborList().SetHaloCapacity(cap); },
            py::arg("cap"))
       .def("GetNbHaloCapacity", [](PairPotential& p) { return p.GetNeighborList().GetHaloCapacity(); })
+      .def("GetNbTileStats", [](PairPotential& p) {
+        const std::vector<uint32_t> t = p.GetNeighborList().TileStats();
+        py::dict d;
+        if (t.size() == 6) {
+          d["tiles"] = t[0];
+          d["staged"] = t[1];
+          d["fast"] = t[2];
+          d["wrapping"] = t[3];
+          d["shifted"] = t[4];
+          d["max_halo"] = t[5];
+        }
+        return d;
+      })
       .def("GetNeighborListCSR", [](PairPotential& p, Sample& s) {
         // neighbour lists in original particle ids, CSR (offsets[n+1], ids)
         s.Synchronize();
@@ -350,6 +363,10 @@ PYBIND11_MODULE(_rumd, m) {
       .def("Synchronize", &Sample::Synchronize)
       .def("RunSteps", &Sample::RunSteps, py::arg("first_step"), py::arg("nsteps"), py::arg("block") = 10,
            py::arg("momentum_interval") = 100, py::call_guard<py::gil_scoped_release>())
+      .def("RunSegments", &Sample::RunSegments, py::arg("first_step"), py::arg("nsteps"), py::arg("block") = 10,
+           py::arg("momentum_interval") = 100)
+      .def("PrepareRunSteps", &Sample::PrepareRunSteps, py::arg("first_step"), py::arg("nsteps"),
+           py::arg("block") = 10, py::arg("momentum_interval") = 100, py::call_guard<py::gil_scoped_release>())
       .def("GetPositions", [](Sample& s) { return get_xyz(s, "r"); })
       // Nx3 float64: r + rlo in mixed-precision mode, r widened otherwise
       .def("GetPositionsFP64", [](Sample& s) {
@@ -383,6 +400,13 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetPositions", [](Sample& s, const f32arr& a) {
         s.Synchronize();
         s.GetParticleData().SetPositions(check_nx3(s, a), s.GetStream());
+        reset_lists(s);
+      }, py::arg("positions"))
+      .def("SetPositionsFP64", [](Sample& s, const py::array_t<double, py::array::c_style | py::array::forcecast>& a) {
+        if (a.ndim() != 2 || a.shape(0) != s.GetNumberOfParticles() || a.shape(1) != 3)
+          throw py::value_error("Wrong array shape, must be Nx3");
+        s.Synchronize();
+        s.GetParticleData().SetPositionsFP64(a.data(), s.GetStream());
         reset_lists(s);
       }, py::arg("positions"))
       .def("SetVelocities", [](Sample& s, const f32arr& a) {
@@ -436,6 +460,9 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SaveState", &ReplicaBatch::SaveState)
       .def("RestoreState", &ReplicaBatch::RestoreState)
       .def("CalcF", &ReplicaBatch::CalcF)
+      .def("TimeForceKernel", &ReplicaBatch::TimeForceKernel, py::arg("reps"),
+           py::call_guard<py::gil_scoped_release>())
+      .def("GetNeighborListStats", &ReplicaBatch::GetNeighborListStats)
       .def("RunSteps", &ReplicaBatch::RunSteps, py::arg("nsteps"), py::arg("block") = 10,
            py::call_guard<py::gil_scoped_release>())
       .def("Synchronize", &ReplicaBatch::Synchronize)

@@ -389,6 +389,27 @@ unsigned NeighborList::GetNumberOfBuilds() {
   return f[RUMDX_FLAG_BUILDS];
 }
 
+std::vector<uint32_t> NeighborList::TileStats() {
+  if (!allocated_ || desc_.nbl16_dev == nullptr) return {};
+  const hipStream_t st = sample_->GetStream();
+  std::vector<uint32_t> info(4ull * desc_.ntiles), f(8);
+  tile_info_.download(info.data(), info.size(), st);
+  flags_.download(f.data(), 8, st);
+  // tile_info = {halo particles, flags | halo capacity << 16, slot0, slot1};
+  // flag bit 0: fast, bits 1-6: per-axis shift below / above (RUMDX_TILE_*)
+  uint32_t staged = 0, fast = 0, wrapping = 0, shifted = 0;
+  for (uint32_t t = 0; t < desc_.ntiles; ++t) {
+    const uint32_t npart = info[4ull * t], fl = info[4ull * t + 1];
+    const bool s = (fl >> 16) == desc_.halo_cap && npart <= desc_.halo_cap;
+    const bool fa = (fl & RUMDX_TILE_FAST) != 0u, wr = (fl & RUMDX_TILE_SHIFT_MASK) != 0u;
+    staged += s ? 1u : 0u;
+    fast += fa ? 1u : 0u;
+    wrapping += wr ? 1u : 0u;
+    shifted += (fa && wr) ? 1u : 0u;
+  }
+  return {desc_.ntiles, staged, fast, wrapping, shifted, f[RUMDX_FLAG_MAXHALO]};
+}
+
 // ================================================================== PairPotential
 PairPotential::PairPotential(CutoffMethod m) : cutoff_method_(m) {
   if (m < NS || m > SF) throw RUMD_Error("PairPotential", "PairPotential", "Invalid cutoffMethod");

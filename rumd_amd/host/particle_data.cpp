@@ -97,6 +97,19 @@ void ParticleData::SetPositions(const float* xyz, hipStream_t st) {
   }
 }
 
+void ParticleData::SetPositionsFP64(const double* xyz, hipStream_t st) {
+  std::vector<float> hi(3ull * n_);
+  for (size_t k = 0; k < hi.size(); ++k) hi[k] = static_cast<float>(xyz[k]);
+  SetPositions(hi.data(), st);
+  if (!mixed_) return;
+  std::vector<float> lo(4ull * n_, 0.f);
+  for (unsigned i = 0; i < n_; ++i)
+    for (int k = 0; k < 3; ++k)
+      lo[4ull * i + k] = static_cast<float>(xyz[3ull * i + k] - static_cast<double>(hi[3ull * i + k]));
+  const std::vector<float> slot = GatherToSlots(lo.data(), st);
+  d_rlo.upload(slot.data(), slot.size(), st);
+}
+
 void ParticleData::SetTypes(const int32_t* types, hipStream_t st) {
   unsigned nt = static_cast<unsigned>(masses_.size());
   for (unsigned i = 0; i < n_; ++i) {

@@ -360,6 +360,41 @@ void ReplicaBatch::Step() {
   rumdx_check(rumdx_batch_nvt_step(&desc_, dt_, workspace_.get(), stream_), "ReplicaBatch", "RunSteps");
 }
 
+float ReplicaBatch::TimeForceKernel(unsigned reps) {
+  if (!allocated_) Allocate();
+  if (reps == 0) reps = 1;
+  rumdx_check(rumdx_batch_update_lists(&desc_, stream_), "ReplicaBatch", "TimeForceKernel");
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "ReplicaBatch", "TimeForceKernel");
+  hip_check(hipEventCreate(&b), "ReplicaBatch", "TimeForceKernel");
+  hip_check(hipEventRecord(a, stream_), "ReplicaBatch", "TimeForceKernel");
+  for (unsigned k = 0; k < reps; ++k)
+    rumdx_check(rumdx_batch_forces(&desc_, params_.get(), cutoff_method_, stream_), "ReplicaBatch", "TimeForceKernel");
+  hip_check(hipEventRecord(b, stream_), "ReplicaBatch", "TimeForceKernel");
+  hip_check(hipEventSynchronize(b), "ReplicaBatch", "TimeForceKernel");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, a, b), "ReplicaBatch", "TimeForceKernel");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+std::vector<double> ReplicaBatch::GetNeighborListStats() {
+  if (!allocated_) return {0.0, 0.0};
+  std::vector<uint32_t> cnt(nslots_);
+  num_nbrs_.download(cnt.data(), nslots_, stream_);
+  double sum = 0.0, mx = 0.0, n = 0.0;
+  for (size_t r = 0; r < reps_.size(); ++r) {
+    const size_t nr = reps_[r].types.size();
+    for (size_t k = 0; k < nr; ++k) {
+      sum += cnt[slot0_[r] + k];
+      mx = std::max(mx, static_cast<double>(cnt[slot0_[r] + k]));
+    }
+    n += static_cast<double>(nr);
+  }
+  return {n > 0.0 ? sum / n : 0.0, mx};
+}
+
 void ReplicaBatch::InvalidateGraphs() {
   for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
   graphs_.clear();

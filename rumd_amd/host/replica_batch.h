@@ -64,6 +64,11 @@ class ReplicaBatch {
   float GetSkin() const { return skin_; }
   int GetCutoffMethod() const { return cutoff_method_; }
   const std::vector<float>& GetParamsTable() const { return table_; }
+  // Average duration (ms) of one rumdx_batch_forces launch over `reps`
+  // launches on the current lists, timed with HIP events on the batch's stream.
+  float TimeForceKernel(unsigned reps);
+  // {mean, longest} list length over the real particles of every replica.
+  std::vector<double> GetNeighborListStats();
   // Neighbour list of one replica in replica-local ids: CSR (offsets n + 1, ids).
   void GetNeighborList(unsigned replica, std::vector<uint32_t>& offsets, std::vector<uint32_t>& ids);
   hipStream_t GetStream() const { return stream_; }

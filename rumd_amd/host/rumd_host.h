@@ -174,6 +174,7 @@ class NeighborList {
   // Called after an integrator step that ran this list's skin check fused.
   void NoteFusedCheck() { fused_check_pending_ = true; }
   void ResetFusedCheck() { fused_check_pending_ = false; }
+  bool FusedCheckPending() const { return fused_check_pending_; }
   // Reads the flags; grows the list after an overflow.  Synchronises.
   bool HandleOverflow();
   // Asks the next UpdateNBlist to rebuild (device flag, on the sample's stream).
@@ -195,6 +196,11 @@ class NeighborList {
   // Lanes sharing one particle's list in the force kernels: the sample's tp.
   unsigned GetLanesPerParticle() const { return desc_.lanes_per_particle; }
   unsigned GetHaloCapacity() const { return halo_cap_; }
+  // Tiles of the last sorted build: {tiles, staged (halo in LDS), fast
+  // (staged, pairs take the FP32 difference), wrapping (halo crosses the
+  // box), shifted (wrapping and fast: staged in the shifted frame), largest
+  // halo}.  Empty for the other methods.
+  std::vector<uint32_t> TileStats();
   // The list in RUMD's format (u32 slots, column-major), for checks and tools.
   std::vector<uint32_t> ExportList(std::vector<uint32_t>& counts);
   rumdx_nblist* Descriptor() { return &desc_; }
@@ -711,6 +717,9 @@ class ParticleData {
   bool MixedPositions() const { return mixed_; }
   // host <-> device in original order (the unsorted index is applied here)
   void SetPositions(const float* xyz, hipStream_t st);
+  // FP64 positions: the float nearest each coordinate, and in mixed mode the
+  // remainder as the low-order part (so GetPositionsFP64 returns them)
+  void SetPositionsFP64(const double* xyz, hipStream_t st);
   void SetTypes(const int32_t* types, hipStream_t st);
   void SetVelocities(const float* xyz, hipStream_t st);
   void SetImages(const float* xyz, hipStream_t st);
@@ -778,7 +787,17 @@ class Sample {
   // Runs nsteps of [CalcF, Integrate] with blocks of `block` steps captured as
   // HIP graphs; momentum is reset before every step whose index is a
   // multiple of momentum_interval (0 = never), as Simulation.Run does.
+  // Blocks start at first_step and end early at a reset step, so a reset is
+  // always the first step of a block (see RunSegments).
   void RunSteps(unsigned long first_step, unsigned nsteps, unsigned block, unsigned momentum_interval);
+  // The blocks RunSteps would run for these arguments: (steps, reset at the
+  // first step) each; blocks of one step run without a graph.
+  std::vector<std::pair<unsigned, bool>> RunSegments(unsigned long first_step, unsigned nsteps, unsigned block,
+                                                     unsigned momentum_interval) const;
+  // Captures and instantiates every graph RunSteps needs for these arguments
+  // without running any step, so that a later RunSteps launches graphs only
+  // (graphs stay valid until InvalidateGraphs).  Returns the number captured.
+  unsigned PrepareRunSteps(unsigned long first_step, unsigned nsteps, unsigned block, unsigned momentum_interval);
   void InvalidateGraphs();
 
   ParticleData& GetParticleData() { return pd_; }
@@ -801,6 +820,7 @@ class Sample {
  private:
   void PrepareRun();
   void StepOnce(bool zero_momentum);
+  hipGraphExec_t CaptureBlock(unsigned nsteps, bool reset_first);
   unsigned pb_ = 0, tp_ = 0;
   unsigned user_dofs_ = 0;
   bool verbose_ = true;

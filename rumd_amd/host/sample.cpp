@@ -229,47 +229,79 @@ void Sample::StepOnce(bool zero_momentum) {
   itg_->Integrate();
 }
 
+std::vector<std::pair<unsigned, bool>> Sample::RunSegments(unsigned long first_step, unsigned nsteps,
+                                                           unsigned block, unsigned momentum_interval) const {
+  if (block == 0) block = 1;
+  std::vector<std::pair<unsigned, bool>> seg;
+  const unsigned long end = first_step + nsteps;
+  for (unsigned long i = first_step; i < end;) {
+    unsigned long stop = std::min<unsigned long>(i + block, end);
+    if (momentum_interval > 0) {
+      // the next reset step after i ends the block before it
+      const unsigned long next_reset = (i / momentum_interval + 1) * momentum_interval;
+      stop = std::min(stop, next_reset);
+    }
+    seg.emplace_back(static_cast<unsigned>(stop - i), momentum_interval > 0 && i % momentum_interval == 0);
+    i = stop;
+  }
+  return seg;
+}
+
+// Records nsteps steps into a graph (nothing runs); the host-side state that
+// recording changes (the fused skin check's pending mark) is restored, so the
+// graph can be launched later as if recorded then.
+hipGraphExec_t Sample::CaptureBlock(unsigned nsteps, bool reset_first) {
+  NeighborList* nl = FusedCheckList();
+  hipGraph_t g = nullptr;
+  hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "Sample", "RunSteps");
+  try {
+    // the first step re-checks the skin explicitly; later steps use the
+    // check fused into the previous integrator step
+    if (nl) nl->ResetFusedCheck();
+    for (unsigned k = 0; k < nsteps; ++k) StepOnce(k == 0 && reset_first);
+  } catch (...) {
+    (void)hipStreamEndCapture(stream_, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hip_check(hipStreamEndCapture(stream_, &g), "Sample", "RunSteps");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  hip_check(e, "Sample", "RunSteps");
+  return exec;
+}
+
+unsigned Sample::PrepareRunSteps(unsigned long first_step, unsigned nsteps, unsigned block,
+                                 unsigned momentum_interval) {
+  PrepareRun();
+  NeighborList* nl = FusedCheckList();
+  const bool pending = nl != nullptr && nl->FusedCheckPending();
+  unsigned made = 0;
+  for (const auto& key : RunSegments(first_step, nsteps, block, momentum_interval)) {
+    if (key.first < 2 || graphs_.count(key)) continue;
+    graphs_.emplace(key, CaptureBlock(key.first, key.second));
+    ++made;
+  }
+  if (nl) {
+    if (pending) nl->NoteFusedCheck(); else nl->ResetFusedCheck();
+  }
+  return made;
+}
+
 void Sample::RunSteps(unsigned long first_step, unsigned nsteps, unsigned block, unsigned momentum_interval) {
   PrepareRun();
-  if (block == 0) block = 1;
-  unsigned long i = first_step;
-  const unsigned long end = first_step + nsteps;
-  auto reset_at = [&](unsigned long step) { return momentum_interval > 0 && step % momentum_interval == 0; };
-  while (i < end) {
-    const bool aligned = (i % block) == 0;
-    bool inner_reset = false;
-    for (unsigned long k = i + 1; k < i + block; ++k) inner_reset = inner_reset || reset_at(k);
-    if (!aligned || i + block > end || inner_reset || block == 1) {
-      StepOnce(reset_at(i));
-      ++i;
+  for (const auto& key : RunSegments(first_step, nsteps, block, momentum_interval)) {
+    if (key.first < 2) {
+      StepOnce(key.second);
       continue;
     }
-    const std::pair<unsigned, bool> key{block, reset_at(i)};
     auto it = graphs_.find(key);
-    if (it == graphs_.end()) {
-      hipGraph_t g = nullptr;
-      hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "Sample", "RunSteps");
-      try {
-        // the first step re-checks the skin explicitly; later steps use the
-        // check fused into the previous integrator step
-        if (NeighborList* nl = FusedCheckList()) nl->ResetFusedCheck();
-        for (unsigned k = 0; k < block; ++k) StepOnce(k == 0 && key.second);
-      } catch (...) {
-        (void)hipStreamEndCapture(stream_, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      hip_check(hipStreamEndCapture(stream_, &g), "Sample", "RunSteps");
-      hipGraphExec_t exec = nullptr;
-      hip_check(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "Sample", "RunSteps");
-      (void)hipGraphDestroy(g);
-      it = graphs_.emplace(key, exec).first;
-    } else if (NeighborList* nl = FusedCheckList()) {
-      nl->ResetFusedCheck();
-    }
+    if (it == graphs_.end()) it = graphs_.emplace(key, CaptureBlock(key.first, key.second)).first;
+    // the graph's first step checks the skin itself (see CaptureBlock)
+    if (NeighborList* nl = FusedCheckList()) nl->ResetFusedCheck();
     hip_check(hipGraphLaunch(it->second, stream_), "Sample", "RunSteps");
     if (NeighborList* nl = FusedCheckList()) nl->NoteFusedCheck();
-    i += block;
   }
   Synchronize();
 }

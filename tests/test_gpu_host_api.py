@@ -270,3 +270,67 @@ def test_autotune_tunes_reuses_and_keeps_the_trajectory(tmp_path, monkeypatch):
     fpot.SetNbListSkin(tech["skin"])
     fresh.Run(50)
     np.testing.assert_array_equal(sim.sample.GetPositions(), fresh.sample.GetPositions())
+
+
+def test_autotune_restores_fp64_positions_and_temperature_ramp(tmp_path, monkeypatch):
+    """Tuning trials advance the state and may reallocate the particle arrays
+    (SetPB_TP); afterwards the FP64 positions (hi + lo in mixed precision),
+    velocities, Ps and a ramped target temperature are bit for bit what they
+    were before Tune."""
+    from rumd_amd.Autotune import Autotune
+
+    monkeypatch.chdir(tmp_path)  # the run's output files go to the working directory
+    sysm = configs.ka_random_lattice(3000, density=1.2, temperature=0.45, seed=78)
+    sim = Simulation(conf_from_system(sysm), verbose=False, mixed_precision_positions=True, timing_filename=None)
+    pot = rumd.Pot_LJ_12_6(cutoff_method=rumd.ShiftedForce)
+    pot.SetVerbose(False)
+    for (i, j, sg, ep, rc) in configs.ka_params(2.5):
+        pot.SetParams(i, j, Sigma=sg, Epsilon=ep, Rcut=rc)
+    pot.SetNbListSkin(0.3)
+    sim.SetPotential(pot)
+    itg = rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.45, thermostatRelaxationTime=0.2)
+    sim.SetIntegrator(itg)
+    itg.SetTemperatureChangePerTimeStep(1e-4)
+    sim.Run(40)  # the low-order position parts and Ps are non-trivial now
+    s = sim.sample
+    pos64, vel = s.GetPositionsFP64(), s.GetVelocities()
+    T0, ps0 = itg.GetTargetTemperature(), itg.GetThermostatState()
+    assert np.abs(pos64 - s.GetPositions().astype(np.float64)).max() > 0.0 and ps0 != 0.0
+    print("[case] autotune restore, mixed KA N=3000 with a T ramp", flush=True)
+    Autotune(auto_filename=tmp_path / "autotune.json", trial_seconds=0.02).Tune(sim)
+    np.testing.assert_array_equal(s.GetPositionsFP64(), pos64)
+    np.testing.assert_array_equal(s.GetVelocities(), vel)
+    assert itg.GetTargetTemperature() == T0
+    assert itg.GetThermostatState() == ps0
+
+
+def test_run_segments_start_at_the_first_step_and_end_at_resets():
+    """RunSteps' blocks start at the run's first step and end early at each
+    momentum-reset step (absolute multiples of the interval); PrepareRunSteps
+    instantiates their graphs without running a step."""
+    sysm = configs.ka_random_lattice(2000, density=1.2, temperature=0.45, seed=79)
+    s, pot = ka_sample(conf_from_system(sysm), cutoff=rumd.ShiftedForce)
+    itg = rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.45, thermostatRelaxationTime=0.2)
+    s.SetIntegrator(itg)
+    assert s.RunSegments(5, 20, 10, 100) == [(10, False), (10, False)]
+    assert s.RunSegments(95, 20, 10, 100) == [(5, False), (10, True), (5, False)]
+    assert s.RunSegments(0, 5, 10, 100) == [(5, True)]
+    assert s.RunSegments(7, 3, 10, 0) == [(3, False)]
+    assert s.RunSegments(99, 2, 10, 100) == [(1, False), (1, True)]
+    s.CalcF()
+    r0 = s.GetPositions()
+    assert s.PrepareRunSteps(95, 20, 10, 100) == 3
+    assert s.PrepareRunSteps(95, 20, 10, 100) == 0
+    np.testing.assert_array_equal(s.GetPositions(), r0)  # nothing ran
+    s.RunSteps(95, 20, 10, 100)
+    ref, rpot = ka_sample(conf_from_system(sysm), cutoff=rumd.ShiftedForce)
+    ritg = rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.45, thermostatRelaxationTime=0.2)
+    ref.SetIntegrator(ritg)
+    ref.CalcF()
+    for i in range(95, 115):
+        if i % 100 == 0:
+            ritg.SetMomentumToZero()
+        ref.CalcF()
+        ritg.Integrate()
+    np.testing.assert_array_equal(s.GetPositions(), ref.GetPositions())
+    np.testing.assert_array_equal(s.GetVelocities(), ref.GetVelocities())

@@ -2,11 +2,13 @@
 (librumdx.so), against the CPU oracle and the RUMD-written golden fixtures."""
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
 import oracle
-from capi import GpuSystem, gpu_available
+from capi import GpuSystem, check, gpu_available, lib
 from helpers import (FORCE_TOL_GUARDED, force_errors, gpu_sets_original_ids, csr_to_sets, ka_table, lj_table,
                      load_golden, unsort)
 from rumd_amd import configs
@@ -228,6 +230,101 @@ def test_skin_trigger_and_no_rebuild():
     assert g.flags_host()[2] == 2
 
 
+def _skin_flip(base, direction, box8, skin, s_lo, s_hi):
+    """Bisects the displacement s of one particle from `base` along
+    `direction` (positions rounded to float32, then wrapped into the box in
+    FP32 as the integrator wraps them) for the oracle's skin criterion;
+    returns the two float32 positions on either side of the flip, one float
+    step apart in the coordinate that moves fastest."""
+    last = base.reshape(1, 4)
+    L = box8[1:4]
+    iL = box8[4:7]
+
+    def at(s):
+        p = base.copy()
+        x = (base[:3].astype(np.float64) + s * direction).astype(np.float32)
+        p[:3] = x - L * np.rint(x * iL).astype(np.float32)
+        return p
+
+    def exceeded(s):
+        return oracle.skin_exceeded(at(s).reshape(1, 4), last, box8, skin)
+
+    assert not exceeded(s_lo) and exceeded(s_hi)
+    for _ in range(200):
+        mid = 0.5 * (s_lo + s_hi)
+        if np.array_equal(at(mid), at(s_lo)) or np.array_equal(at(mid), at(s_hi)):
+            break
+        if exceeded(mid):
+            s_hi = mid
+        else:
+            s_lo = mid
+    return at(s_lo), at(s_hi)
+
+
+def test_skin_check_one_float_either_side_against_oracle():
+    """displacement_since_nb_build (src/NeighborList.cu:970-997): the GPU skin
+    check and the oracle's FP32 restatement agree on positions one float step
+    either side of the skin/2 threshold (strict >), for displacements inside
+    the box and across the periodic boundary on each axis."""
+    rng = np.random.default_rng(97)
+    n, L, skin = 500, 20.0, 0.3
+    b8 = configs.box8(L)
+    pos = rng.uniform(-L / 2, L / 2, size=(n, 3)).astype(np.float32)
+    base4 = configs.pack_pos4(pos, rng.integers(0, 2, n).astype(np.int32))
+    g = GpuSystem(base4, configs.pack_vel4(np.zeros((n, 3)), np.zeros(n, np.int32), np.ones(2)), b8, 2, ka_table(),
+                  method="n2", skin=skin, max_nbrs=64)
+    pad = np.zeros((g.nvp - n, 4), np.float32)
+    g.last_r.upload(np.vstack([base4, pad]))
+    cases = []
+    for k in range(6):  # inside the box, random directions
+        d = rng.normal(size=3)
+        cases.append((int(rng.integers(n)), None, d / np.linalg.norm(d)))
+    for axis in range(3):  # across the boundary: from near +L/2 through it
+        cases.append((int(rng.integers(n)), axis, np.eye(3)[axis]))
+    seen = set()
+    for (k, axis, d) in cases:
+        base = base4[k].copy()
+        if axis is not None:
+            base[axis] = np.float32(L / 2 - 0.07)  # the move crosses +L/2 and wraps to near -L/2
+        last_all = base4.copy()
+        last_all[k] = base
+        g.last_r.upload(np.vstack([last_all, pad]))
+        lo, hi = _skin_flip(base, d, b8, skin, 0.0, 0.3)
+        for p_k, expect in ((lo, False), (hi, True)):
+            r = last_all.copy()
+            r[k] = p_k
+            if axis is not None:
+                assert r[k, axis] < 0  # wrapped to the far side of the box
+            ref = oracle.skin_exceeded(r, last_all, b8, skin)
+            assert ref == expect
+            g.r.upload(np.vstack([r, pad]))
+            g.flags.upload(np.zeros(8, np.uint32))
+            print(f"[case] skin check particle {k} axis={axis} expect={expect}", flush=True)
+            check(lib().rumdx_nblist_check(C.byref(g.p), C.byref(g.nb), g.box8, None))
+            assert bool(g.flags_host()[0]) == ref, (k, axis, p_k, base)
+            seen.add((axis is None, ref))
+    assert seen == {(True, False), (True, True), (False, False), (False, True)}
+
+
+def test_skin_trigger_and_no_rebuild():
+    sysm = configs.ka_random_lattice(10000, density=1.2, seed=9)
+    table = ka_table()
+    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
+    g = GpuSystem(pos4, configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses), configs.box8(sysm.box), 2,
+                  table, method="sorted", max_nbrs=200)
+    g.update_nblist()
+    assert g.flags_host()[2] == 1
+    g.update_nblist()  # nothing moved: no rebuild
+    assert g.flags_host()[2] == 1
+    r = g.get("r").copy()
+    last = g.last_r.download(np.float32, (g.nvp, 4))[: g.n]
+    r[17, 0] += np.float32(0.151)  # beyond skin/2 = 0.15
+    assert oracle.skin_exceeded(r, last, g.box8, 0.3)
+    g.r.upload(np.vstack([r, np.zeros((g.nvp - g.n, 4), np.float32)]))
+    g.update_nblist()
+    assert g.flags_host()[2] == 2
+
+
 @pytest.mark.parametrize("hilo", [False, True])
 @pytest.mark.parametrize("thermo", [False, True])
 def test_integrator_bit_exact(golden_dir, hilo, thermo):
@@ -282,18 +379,39 @@ def test_momentum_and_kinetic_energy(golden_dir):
     g.calc_f(SP)
     ps = float(c.meta["Nose-Hoover-Ps"])
     g.set_nh(ps=ps, target_T=0.5)
-    import ctypes as C
-    from capi import lib, check, NhState
+    from capi import NhState
     check(lib().rumdx_kinetic_energy(C.byref(g.p), C.cast(g.nh.value, C.POINTER(NhState)), 0.005, 0xFFFFFFFF,
                                      g.out.value, g.ws.value, None))
     ke = g.out.download(np.float64, 1)[0]
     ke_ref = oracle.kinetic_energy(g.get("r"), g.get("v"), g.get("f"), 0.005, ps, 0xFFFFFFFF)
     assert ke == pytest.approx(ke_ref, rel=1e-12)
+
+
+def test_zero_momentum_subtracts_total_momentum_over_total_mass(golden_dir):
+    """SetMomentumToZero (calculateMomentum / zeroTotalMomentum,
+    src/IntegratorNVT.cu:444-472): v_i -= P / M with P = sum m_i v_i and
+    M = sum m_i.  On the unequal-mass fixture with a uniform boost added (so P
+    is far from zero), the GPU velocities equal v - P/M formed in FP64 to
+    float rounding; v = 0, or subtracting P / (N m_i), would not."""
+    c = load_golden(golden_dir / "KA_N864_L9.0_T0.50AltMass.xyz.gz")
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    vel4 = configs.pack_vel4(c.velocities, c.types, c.masses)
+    vel4[:, :3] += np.array([0.3, -0.2, 0.1], np.float32)
+    g = GpuSystem(pos4, vel4, configs.box8(c.box), 2, ka_table(), method="none")
+    v0 = g.get("v").astype(np.float64)
+    m = (1.0 / g.get("v")[:, 3]).astype(np.float32).astype(np.float64)  # 1/v.w in float, as calculateMomentum
+    p = (np.float32(1.0) * (g.get("v")[:, :3] * (1.0 / g.get("v")[:, 3:4]))).astype(np.float64).sum(axis=0)
+    v_ref = v0[:, :3] - p / m.sum()
+    assert np.ptp(m) > 0.5 and np.abs(p).min() > 10.0
     check(lib().rumdx_zero_momentum(C.byref(g.p), g.ws.value, None))
-    v = g.get("v")
-    m = 1.0 / v[:, 3].astype(np.float64)
-    p = (v[:, :3].astype(np.float64) * m[:, None]).sum(axis=0)
-    assert np.all(np.abs(p) < 1e-4)
+    v = g.get("v").astype(np.float64)
+    tol = 2.0 * np.spacing(np.maximum(np.abs(v0[:, :3]), np.abs(p / m.sum())).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(v[:, :3] - v_ref) <= tol)
+    assert np.array_equal(v[:, 3], v0[:, 3])
+    p_after = (v[:, :3] * m[:, None]).sum(axis=0)
+    assert np.all(np.abs(p_after) < 1e-3)
+    wrong = v0[:, :3] - p / (c.n * m[:, None])
+    assert np.abs(wrong - v_ref).max() > 100 * tol.max()
 
 
 def _five_type_system(n=12000, seed=515):

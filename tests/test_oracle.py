@@ -320,6 +320,97 @@ def test_kinetic_energy_closed_form():
     assert ke == pytest.approx(0.5 * (1.0 * 1.0 + 2.0 * 4.0))
 
 
+def test_kinetic_energy_leapfrog_average_one_particle():
+    """GetKineticEnergy's leapfrog average with a force and a thermostat
+    (src/IntegratorNVT.cu:316-356), on numbers exact in FP32: m = 2 (1/m =
+    0.5), dt = 0.5, Ps = 2 give factor = 0.5 dt Ps = 0.5, plus = 2/3, minus =
+    1/2.  With v(t - dt/2) = (1, 2, 0) and F = (10, 4, 6):
+    v(t + dt/2) = plus (minus v + (dt/m) F) = (2/3)(3, 2, 1.5) = (2, 4/3, 1),
+    so KE = 1/2 m (|v_b|^2 + |v_f|^2) / 2 = 0.5 (5 + 61/9) = 53/9."""
+    pos4 = configs.pack_pos4(np.zeros((1, 3), np.float32), np.zeros(1, np.int32))
+    vel4 = np.array([[1.0, 2.0, 0.0, 0.5]], np.float32)
+    f4 = np.array([[10.0, 4.0, 6.0, 0.0]], np.float32)
+    ke = oracle.kinetic_energy(pos4, vel4, f4, 0.5, 2.0, 0xFFFFFFFF)
+    assert ke == pytest.approx(53.0 / 9.0, rel=1e-6)
+    # a particle type outside the thermostat mask takes factor 0: v_f = v + (dt/m) F = (3.5, 3, 1.5)
+    ke_off = oracle.kinetic_energy(pos4, vel4, f4, 0.5, 2.0, 0b10)
+    assert ke_off == pytest.approx(0.5 * (5.0 + 3.5 ** 2 + 9.0 + 2.25), rel=1e-6)
+    # and neither half step alone gives it
+    assert abs(ke - 5.0) > 0.5 and abs(ke - 61.0 / 9.0) > 0.5
+
+
+def test_kinetic_energy_equals_average_over_the_integrator_half_steps(golden_dir):
+    """KE(t) is the mean of the kinetic energies at t - dt/2 and t + dt/2 that
+    the (separately pinned) leapfrog step produces from the same forces and
+    Ps: on the AltMass fixture (unequal masses) with a thermostat running."""
+    c = load_golden(golden_dir / "KA_N864_L9.0_T0.50AltMass.xyz.gz")
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    vel4 = configs.pack_vel4(c.velocities, c.types, c.masses)
+    f4, _ = oracle.forces_brute(pos4, c.box, 2, ka_table(), SP)
+    f4 = f4.astype(np.float32)
+    dt, ps = 0.004, 0.37
+    ke = oracle.kinetic_energy(pos4, vel4, f4, dt, ps, 0xFFFFFFFF)
+    m = 1.0 / vel4[:, 3].astype(np.float64)
+    ke_b = 0.5 * float((m * (vel4[:, :3].astype(np.float64) ** 2).sum(axis=1)).sum())
+    v_next = vel4.copy()
+    oracle.nvt_step(pos4.copy(), v_next, f4, np.zeros_like(pos4), configs.box8(c.box), dt, ps, 0xFFFFFFFF)
+    ke_f = 0.5 * float((m * (v_next[:, :3].astype(np.float64) ** 2).sum(axis=1)).sum())
+    assert ke == pytest.approx(0.5 * (ke_b + ke_f), rel=1e-6)
+    assert abs(ke - ke_b) > 1e-3 * ke and abs(ke - ke_f) > 1e-3 * ke  # the average is not either end
+
+
+def test_kinetic_energy_average_conserves_nve_energy_better(golden_dir):
+    """The leapfrog-averaged kinetic energy makes PE + KE conserved to O(dt^2)
+    along an NVE trajectory; the kinetic energy of either half step alone
+    fluctuates at O(dt).  A dropped half step in the average fails this."""
+    c = load_golden(golden_dir / "KA_N256_L6.0_T0.50.xyz.gz")
+    tab = ka_table()
+    b8 = configs.box8(c.box)
+    pos4 = configs.pack_pos4(c.positions, c.types)
+    vel4 = configs.pack_vel4(c.velocities, c.types, c.masses)
+    im, rlo = np.zeros_like(pos4), np.zeros_like(pos4)
+    dt = 0.005
+    e_avg, e_half = [], []
+    for _ in range(60):
+        f4, _ = oracle.forces_brute(pos4, c.box, 2, tab, SP)
+        f4 = f4.astype(np.float32)
+        pe = float(f4[:, 3].astype(np.float64).sum())
+        m = 1.0 / vel4[:, 3].astype(np.float64)
+        e_avg.append(pe + oracle.kinetic_energy(pos4, vel4, f4, dt, 0.0, 0))
+        e_half.append(pe + 0.5 * float((m * (vel4[:, :3].astype(np.float64) ** 2).sum(axis=1)).sum()))
+        oracle.nvt_step(pos4, vel4, f4, im, b8, dt, 0.0, 0, rlo4=rlo)
+    spread_avg = np.ptp(e_avg) / c.n
+    spread_half = np.ptp(e_half) / c.n
+    assert spread_avg < 2e-3
+    assert spread_avg < 0.2 * spread_half, (spread_avg, spread_half)
+
+
+def test_forces_over_the_full_list_equal_all_pairs():
+    """orc_forces_nl over the complete rc + skin list (FP32 criterion, brute
+    force) gives the all-pairs FP64 forces, per-particle energies and virials;
+    a list that misses in-range pairs does not."""
+    s = configs.ka_random_lattice(3000, density=1.2, seed=41)
+    pos4 = configs.pack_pos4(s.positions, s.types)
+    b8 = configs.box8(s.box)
+    tab = ka_table()
+    off, idx = oracle.nblist_brute_f32(pos4, b8, 2, configs.cut_skin2_table(tab, 0.3))
+    for cut in (NS, SP, SF):
+        fb, vb = oracle.forces_brute(pos4, s.box, 2, tab, cut)
+        fl, vl = oracle.forces_nl(pos4, s.box, 2, tab, cut, off, idx)
+        np.testing.assert_allclose(fl, fb, rtol=1e-12, atol=1e-11)
+        np.testing.assert_allclose(vl, vb, rtol=1e-12, atol=1e-11)
+    # entries in reverse order: the same forces up to the order of the sums
+    idx_rev = idx.copy()
+    for i in range(s.n):
+        idx_rev[off[i]:off[i + 1]] = idx[off[i]:off[i + 1]][::-1]
+    fr, _ = oracle.forces_nl(pos4, s.box, 2, tab, SF, off, idx_rev)
+    np.testing.assert_allclose(fr, fb, rtol=1e-10, atol=1e-10)
+    # a list cut at rc - 0.3 (pairs inside rc missing) gives different forces
+    off2, idx2 = oracle.nblist_brute_f32(pos4, b8, 2, configs.cut_skin2_table(tab, -0.3))
+    f2, _ = oracle.forces_nl(pos4, s.box, 2, tab, SF, off2, idx2)
+    assert np.abs(f2[:, :3] - fb[:, :3]).max() > 1e-3
+
+
 def test_sum_integer_exact():
     a = np.arange(40000, dtype=np.float32).reshape(10000, 4)
     assert oracle.sum_f4(a, 1) == float(np.arange(1, 40000, 4, dtype=np.int64).sum())
