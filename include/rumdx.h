@@ -70,15 +70,24 @@ extern "C" {
  * src/IntegratorNVT.cu:157-174). */
 #define RUMDX_MAX_TYPES 32
 
-/* Slots of rumdx_nblist.flags_dev (device uint32[8]). */
+/* Slots of rumdx_nblist.flags_dev (device uint32[RUMDX_NB_FLAGS]). */
+#define RUMDX_NB_FLAGS 16
 #define RUMDX_FLAG_REBUILD 0   /* != 0: the next build call rebuilds (updateRequired[0]) */
 #define RUMDX_FLAG_OVERFLOW 1  /* sticky: some list hit max_nbrs (updateRequired[1]) */
 #define RUMDX_FLAG_BUILDS 2    /* number of completed builds */
 #define RUMDX_FLAG_MAXLEN 3    /* longest list seen since the flag was cleared */
 #define RUMDX_FLAG_DONE 4      /* internal completion counter, zero between calls */
 #define RUMDX_FLAG_MAXHALO 5   /* largest tile halo (particles) of the last sorted build */
-#define RUMDX_FLAG_ERROR 6     /* nonzero: a tile halo exceeded 65535 particles (list unusable) */
+#define RUMDX_FLAG_ERROR 6     /* nonzero: the last sorted build is unusable, RUMDX_BUILD_ERR_* bits */
 #define RUMDX_FLAG_EXCL_PENDING 7 /* internal: set by a completed build, cleared once its lists' exclusions are applied */
+#define RUMDX_FLAG_MAXTILE 8   /* most particles in one cell tile, last sorted build */
+/* Bits of flags[RUMDX_FLAG_ERROR]. */
+#define RUMDX_BUILD_ERR_HALO 1u     /* a tile halo held more than 65535 particles */
+#define RUMDX_BUILD_ERR_MOVED 2u    /* rebuild_sorted: a particle moved more than one cell since the last build */
+#define RUMDX_BUILD_ERR_WAIT 4u     /* rebuild_sorted: a tile's wait for the tiles before it timed out */
+#define RUMDX_BUILD_ERR_TILE 8u     /* rebuild_sorted: a tile held more than RUMDX_RESORT_TILE_MAX particles */
+/* Particles one tile may hold for rumdx_nblist_rebuild_sorted. */
+#define RUMDX_RESORT_TILE_MAX 2048u
 
 /* Device-resident particle arrays (ParticleData, include/rumd/ParticleData.h:41-62). */
 typedef struct rumdx_particles {
@@ -122,7 +131,7 @@ typedef struct rumdx_nblist {
   uint32_t* nbl_dev;        /* N2 method: max_nbrs x nvp */
   uint32_t* num_nbrs_dev;   /* nvp */
   float* last_r_dev;        /* nvp x 4: positions at the last build */
-  uint32_t* flags_dev;      /* 8 x uint32, see RUMDX_FLAG_* */
+  uint32_t* flags_dev;      /* RUMDX_NB_FLAGS x uint32, see RUMDX_FLAG_* */
   const float* cut_skin2_dev; /* ntypes^2: (rc_ij + skin)^2 formed in FP32 (src/NeighborList.cu:1239-1240) */
   uint32_t max_nbrs;
   uint32_t ntypes;
@@ -227,6 +236,20 @@ int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb, const floa
  * the cells and by original slot inside a cell (a stable sort).  Does nothing
  * on the device unless flags[REBUILD] is set. */
 int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
+
+/* The same result as rumdx_nblist_build_sorted -- particle order, cells and
+ * lists bit for bit -- for particles that a previous sorted build of this list
+ * left in cell order and that have since moved less than one cell (the state
+ * between skin-triggered rebuilds; the skin check keeps displacements below
+ * skin/2 + one step, well under a cell of (rc + skin)/2).  Each cell tile
+ * re-sorts only the particles of its own cells and their neighbours (no global
+ * histogram or scatter), and tiles find their first slot from the counts of
+ * the tiles before them.  Three launches, each returning at once unless a
+ * rebuild is pending.  A broken precondition is reported in
+ * flags[RUMDX_FLAG_ERROR] (RUMDX_BUILD_ERR_MOVED, _TILE), never as an out-of-
+ * range access.  The build of NeighborList::UpdateNBlist for skin-triggered
+ * rebuilds (src/NeighborList.cu:583-690, 717-781); no separate counterpart. */
+int rumdx_nblist_rebuild_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
 
 /* Writes the current list in RUMD's format -- u32 neighbour slots, column
  * major, out_dev[k*nvp + i] for k < num_nbrs[i] -- whatever the internal

@@ -10,7 +10,8 @@
 //
 // Every build kernel first reads flags[REBUILD] and exits when it is clear, so a
 // step can launch the whole chain unconditionally (no host round trip, capturable
-// in a graph).  The last workgroup of the final kernel clears the flag.
+// in a graph).  A one-workgroup kernel after the tile build clears the flag and
+// records the build (the N2 build's last workgroup does it there).
 //
 // The sorted build is a deterministic counting sort: histogram of cell ranks,
 // two-pass exclusive scan, scatter, then a gather that puts each cell's
@@ -39,6 +40,9 @@ struct SortScratch {
   float4* t_rlo;
   uint32_t* t_idx;      // nvp
   uint32_t* chunk_sum;  // one per kScanChunk cells
+  uint32_t* start_next; // ncells + 1: cell starts of the build in progress (the build copies them home)
+  uint32_t* tile_max;   // per tile: longest list of the last build
+  unsigned long long* lookback;  // per tile: the re-sort's published counts
 };
 
 // Grid cap of the particle-parallel rebuild kernels (they loop over the rest).
@@ -66,15 +70,19 @@ SortScratch carve(void* base, uint32_t nvp, uint32_t ncells) {
   s.t_im = reinterpret_cast<float4*>(p); p += f4n;
   s.t_rlo = reinterpret_cast<float4*>(p); p += f4n;
   s.t_idx = reinterpret_cast<uint32_t*>(p); p += u32n;
-  s.chunk_sum = reinterpret_cast<uint32_t*>(p);
-  (void)ncells;
+  s.chunk_sum = reinterpret_cast<uint32_t*>(p); p += align256(sizeof(uint32_t) * scan_chunks(ncells));
+  s.start_next = reinterpret_cast<uint32_t*>(p); p += align256(sizeof(uint32_t) * (ncells + 1ull));
+  s.tile_max = reinterpret_cast<uint32_t*>(p); p += align256(sizeof(uint32_t) * ncells);  // ntiles <= ncells
+  s.lookback = reinterpret_cast<unsigned long long*>(p);
   return s;
 }
 
 size_t scratch_bytes(uint32_t nvp, uint32_t ncells) {
   const size_t u32n = align256(sizeof(uint32_t) * nvp);
   const size_t f4n = align256(sizeof(float4) * nvp);
-  return 3 * u32n + 6 * f4n + align256(sizeof(uint32_t) * scan_chunks(ncells));
+  return 3 * u32n + 6 * f4n + align256(sizeof(uint32_t) * scan_chunks(ncells)) +
+         align256(sizeof(uint32_t) * (ncells + 1ull)) + align256(sizeof(uint32_t) * ncells) +
+         align256(sizeof(unsigned long long) * ncells);
 }
 
 __device__ __forceinline__ bool rebuild_pending(const uint32_t* flags) {
@@ -194,8 +202,6 @@ __global__ __launch_bounds__(kBlock) void k_cell_hist(const float4* __restrict__
                                                       uint32_t* __restrict__ cell_count, Box bx, CellGeom g,
                                                       uint32_t n) {
   if (!rebuild_pending(flags)) return;
-  // the tile build then records the largest halo of this build
-  if (blockIdx.x == 0 && threadIdx.x == 0) flags[RUMDX_FLAG_MAXHALO] = 0u;
   for (uint32_t b0 = blockIdx.x * kBlock; b0 < n; b0 += gridDim.x * kBlock) {
     const uint32_t i = b0 + threadIdx.x;
     const bool live = i < n;
@@ -374,7 +380,255 @@ struct HomeArrays {
   float4* last_r;
   uint32_t* slot_cell;
   uint16_t* lane_order;
+  uint32_t* cell_start;  // the build copies its cells' starts here from start_next
 };
+
+// ------------------------------------------------------------------ incremental re-sort
+// Between skin-triggered rebuilds particles stay in the cell order of the last
+// build and move less than a cell, so the particles of a tile's cells after
+// the move all sat in those cells or their face/edge/corner neighbours before
+// it.  One workgroup per tile gathers those candidates, keeps the ones whose
+// new cell is its own, orders them by (cell, old slot) -- the order the
+// global counting sort gives -- and places them after the particles of all
+// tiles before it.  That offset comes from a decoupled look-back in tile
+// (= slot) order: each tile publishes its count as soon as it has it, then
+// adds its predecessors' counts until it meets one that already holds the
+// running total.  Published words are relaxed agent-scope atomics carrying
+// their own payload, so no fences are needed; the data the tiles write is
+// read by the next kernel.
+//
+// Look-back word of a tile: count (bits 0-31), build epoch (bits 32-61, so
+// words of earlier builds are stale and the array is never cleared), state
+// (bits 62-63: 1 = the tile's own count, 2 = the count of every tile up to
+// and including it).
+constexpr unsigned long long kLbOwn = 1ull, kLbInclusive = 2ull;
+constexpr uint32_t kLbEpochMask = 0x3fffffffu;
+constexpr uint32_t kLookbackSpin = 1u << 20;  // polls of one predecessor's word before giving up
+constexpr uint32_t kResortCap = RUMDX_RESORT_TILE_MAX;
+constexpr uint32_t kTileCells = static_cast<uint32_t>(kTileEdge * kTileEdge * kTileEdge);
+constexpr uint32_t kDilCellsMax = static_cast<uint32_t>((kTileEdge + 2) * (kTileEdge + 2) * (kTileEdge + 2));
+static_assert(kDilCellsMax <= kBlock && kTileCells <= 64, "one thread per dilated cell, one lane per tile cell");
+static_assert(kResortCap <= 65536, "member indices are 16 bit");
+
+__device__ __forceinline__ unsigned long long lb_pack(unsigned long long state, uint32_t epoch, uint32_t count) {
+  return (state << 62) | (static_cast<unsigned long long>(epoch & kLbEpochMask) << 32) | count;
+}
+
+struct ParticleArrays {
+  const float4* r;
+  const float4* v;
+  const float4* f;
+  const float4* w;
+  const float4* im;
+  const float4* rlo;
+  const uint32_t* idx;
+};
+
+struct ResortLds {
+  uint32_t start[kDilCellsMax];    // old first slot of each cell of the dilated tile
+  uint32_t off[kDilCellsMax + 1];  // first candidate of each of those cells
+  uint32_t wave_sum[kBlock / 64];
+  uint32_t cnt[kTileCells];        // members per cell of the tile (tile-local rank)
+  uint32_t coff[kTileCells + 1];   // their exclusive prefix
+  uint32_t cur[kTileCells];
+  uint32_t nmem, prefix, rank0, ncell_t;
+  int c0[3], own[3], ext[3];
+  uint32_t slot[kResortCap];       // old slot of each member
+  uint8_t lr[kResortCap];          // tile-local cell of each member
+  uint16_t order[kResortCap];      // members grouped by cell
+};
+
+__global__ __launch_bounds__(kBlock) void k_tile_resort(uint32_t* __restrict__ flags, SortScratch s, TileGeom tg,
+                                                        Box bx, ParticleArrays o, uint32_t n) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ ResortLds L;
+  // tiles in slot order, in dispatch order: a tile waits only on earlier ones
+  const uint32_t t = blockIdx.x;
+  const uint32_t epoch = flags[RUMDX_FLAG_BUILDS];  // unchanged during the launch
+  const uint32_t tid = threadIdx.x;
+  const int nc[3] = {tg.nc0, tg.nc1, tg.nc2};
+  if (tid == 0) {
+    const uint32_t org = tg.origin[t];
+    const int c0[3] = {static_cast<int>(org & 1023u), static_cast<int>((org >> 10) & 1023u),
+                       static_cast<int>((org >> 20) & 1023u)};
+    for (int a = 0; a < 3; ++a) {
+      const int own = (nc[a] - c0[a]) < kTileEdge ? (nc[a] - c0[a]) : kTileEdge;
+      L.c0[a] = c0[a];
+      L.own[a] = own;
+      L.ext[a] = (own + 2) < nc[a] ? (own + 2) : nc[a];
+    }
+    L.rank0 = tg.first_rank[t];
+    L.ncell_t = tg.first_rank[t + 1] - L.rank0;
+    L.nmem = 0u;
+  }
+  if (tid < kTileCells) {
+    L.cnt[tid] = 0u;
+    L.cur[tid] = 0u;
+  }
+  __syncthreads();
+  // old slot ranges of the tile's cells dilated by one cell (each cell once)
+  const uint32_t ndil = static_cast<uint32_t>(L.ext[0] * L.ext[1] * L.ext[2]);
+  uint32_t cnt = 0;
+  if (tid < ndil) {
+    const int h[3] = {static_cast<int>(tid % L.ext[0]), static_cast<int>((tid / L.ext[0]) % L.ext[1]),
+                      static_cast<int>(tid / (L.ext[0] * L.ext[1]))};
+    int c[3];
+    for (int a = 0; a < 3; ++a) {
+      c[a] = L.c0[a] - 1 + h[a];
+      c[a] += c[a] < 0 ? nc[a] : 0;
+      c[a] -= c[a] >= nc[a] ? nc[a] : 0;
+    }
+    const uint32_t rank = tg.cell_rank[c[0] + c[1] * nc[0] + c[2] * nc[0] * nc[1]];
+    const uint32_t st = tg.cell_start[rank];
+    L.start[tid] = st;
+    cnt = tg.cell_start[rank + 1] - st;
+  }
+  {
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    uint32_t x = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= static_cast<uint32_t>(d)) x += y;
+    }
+    if (lane == 63) L.wave_sum[wave] = x;
+    __syncthreads();
+    uint32_t woff = 0;
+    for (uint32_t k = 0; k < wave; ++k) woff += L.wave_sum[k];
+    if (tid < ndil) L.off[tid] = woff + x - cnt;
+    if (tid == ndil - 1) L.off[ndil] = woff + x;
+  }
+  __syncthreads();
+  // candidates: members are those whose new cell is one of the tile's
+  const uint32_t tot = L.off[ndil];
+  for (uint32_t q = tid; q < tot; q += kBlock) {
+    uint32_t lo = 0, hi = ndil;  // last dilated cell with off <= q
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (L.off[mid] <= q) lo = mid; else hi = mid;
+    }
+    const uint32_t slot = L.start[lo] + (q - L.off[lo]);
+    if (slot >= n) continue;  // never, for cell starts left by a sorted build of these particles
+    const float4 rr = o.r[slot];
+    const int cx = cell_coord(rr.x, bx.iL[0], nc[0]);
+    const int cy = cell_coord(rr.y, bx.iL[1], nc[1]);
+    const int cz = cell_coord(rr.z, bx.iL[2], nc[2]);
+    int dx = cx - L.c0[0], dy = cy - L.c0[1], dz = cz - L.c0[2];
+    dx += dx < 0 ? nc[0] : 0;
+    dy += dy < 0 ? nc[1] : 0;
+    dz += dz < 0 ? nc[2] : 0;
+    if (dx < L.own[0] && dy < L.own[1] && dz < L.own[2]) {
+      const uint32_t lc = tg.cell_rank[cx + cy * nc[0] + cz * nc[0] * nc[1]] - L.rank0;
+      if (lc >= L.ncell_t) continue;  // never: the tile's cells are its rank range
+      const uint32_t k = atomicAdd(&L.nmem, 1u);
+      if (k < kResortCap) {
+        L.slot[k] = slot;
+        L.lr[k] = static_cast<uint8_t>(lc);
+      }
+      atomicAdd(&L.cnt[lc], 1u);
+    }
+  }
+  __syncthreads();
+  const uint32_t nmem = L.nmem;
+  const bool fits = nmem <= kResortCap;
+  if (tid == 0) __hip_atomic_store(s.lookback + t, lb_pack(kLbOwn, epoch, nmem), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (tid < 64) {  // exclusive prefix of the per-cell counts (one wave)
+    const uint32_t v = tid < L.ncell_t ? L.cnt[tid] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (tid >= static_cast<uint32_t>(d)) x += y;
+    }
+    if (tid <= kTileCells) L.coff[tid] = x - v;
+    if (tid == 63) L.coff[kTileCells] = x;
+  }
+  __syncthreads();
+  if (fits)
+    for (uint32_t k = tid; k < nmem; k += kBlock) {
+      const uint32_t c = L.lr[k];
+      L.order[L.coff[c] + atomicAdd(&L.cur[c], 1u)] = static_cast<uint16_t>(k);
+    }
+  if (tid == 0) {
+    // this tile's first slot: the particles of every tile before it
+    uint32_t acc = 0;
+    bool ok = true;
+    for (uint32_t j = t; j-- > 0;) {
+      unsigned long long w = 0;
+      for (uint32_t spin = 0;; ++spin) {
+        w = __hip_atomic_load(s.lookback + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((w >> 62) != 0ull && (static_cast<uint32_t>(w >> 32) & kLbEpochMask) == (epoch & kLbEpochMask)) break;
+        if (spin == kLookbackSpin) {
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (!ok) break;
+      acc += static_cast<uint32_t>(w);
+      if ((w >> 62) == kLbInclusive) break;
+    }
+    if (!ok) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_WAIT);
+    __hip_atomic_store(s.lookback + t, lb_pack(kLbInclusive, epoch, acc + nmem), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    L.prefix = acc;
+    if (!fits) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_TILE);
+    // every particle is a member of exactly one tile unless one moved more than a cell
+    if (t == tg.ntiles - 1 && acc + nmem != n) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_MOVED);
+  }
+  __syncthreads();
+  const uint32_t S = L.prefix;
+  for (uint32_t c = tid; c < L.ncell_t; c += kBlock) s.start_next[L.rank0 + c] = S + L.coff[c];
+  if (t == tg.ntiles - 1 && tid == 0) s.start_next[L.rank0 + L.ncell_t] = S + nmem;
+  if (!fits) return;
+  // new slot: the cell's first slot plus the members of the cell with a smaller old slot
+  for (uint32_t j = tid; j < nmem; j += kBlock) {
+    const uint32_t k = L.order[j];
+    const uint32_t c = L.lr[k], my = L.slot[k];
+    const uint32_t b = L.coff[c], e = L.coff[c + 1];
+    uint32_t rk = 0;
+    for (uint32_t jj = b; jj < e; ++jj) rk += L.slot[L.order[jj]] < my ? 1u : 0u;
+    const uint32_t d = S + b + rk;
+    if (d < n) {  // in range unless particles were lost (then the build is flagged)
+      s.t_r[d] = o.r[my];
+      s.t_v[d] = o.v[my];
+      s.t_f[d] = o.f[my];
+      s.t_w[d] = o.w[my];
+      s.t_im[d] = o.im[my];
+      if (o.rlo != nullptr) s.t_rlo[d] = o.rlo[my];
+      s.t_idx[d] = o.idx[my];
+    }
+  }
+}
+
+// Completes a sorted build (after the tile build): longest list, largest halo
+// and largest tile from the per-tile records, the build count, and the
+// rebuild request cleared.  One workgroup; the kernel boundary orders it after
+// every tile.
+__global__ __launch_bounds__(kBlock) void k_build_finish(uint32_t* __restrict__ flags,
+                                                         const uint4* __restrict__ tile_info,
+                                                         const uint32_t* __restrict__ tile_max, uint32_t ntiles) {
+  if (!rebuild_pending(flags)) return;
+  __shared__ uint32_t s_max[kBlock / 64];
+  uint32_t ml = 0, mh = 0, mt = 0;
+  for (uint32_t t = threadIdx.x; t < ntiles; t += kBlock) {
+    const uint4 info = tile_info[t];
+    ml = tile_max[t] > ml ? tile_max[t] : ml;
+    mh = info.x > mh ? info.x : mh;
+    mt = (info.w - info.z) > mt ? (info.w - info.z) : mt;
+  }
+  ml = block_max_u32(ml, s_max);
+  mh = block_max_u32(mh, s_max);
+  mt = block_max_u32(mt, s_max);
+  if (threadIdx.x == 0) {
+    flags[RUMDX_FLAG_MAXLEN] = ml > flags[RUMDX_FLAG_MAXLEN] ? ml : flags[RUMDX_FLAG_MAXLEN];
+    flags[RUMDX_FLAG_MAXHALO] = mh;
+    flags[RUMDX_FLAG_MAXTILE] = mt;
+    flags[RUMDX_FLAG_REBUILD] = 0u;
+    flags[RUMDX_FLAG_BUILDS] += 1u;
+    flags[RUMDX_FLAG_EXCL_PENDING] = 1u;
+  }
+}
 
 // Per-wave LDS of the cooperative list build: the stencil segments of the
 // cell being built and a window of its candidates (LDS offsets, in stencil
@@ -747,9 +1001,8 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
     o.lane_order[h.slot0 + rk] = static_cast<uint16_t>(k);
   }
   if (threadIdx.x == 0) {
-    atomicMax(flags + RUMDX_FLAG_MAXLEN, longest);
-    atomicMax(flags + RUMDX_FLAG_MAXHALO, h.npart);
-    if (h.npart > 65535u) flags[RUMDX_FLAG_ERROR] = 1u;
+    s.tile_max[t] = longest;  // k_build_finish reduces these
+    if (h.npart > 65535u) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_HALO);
     // wrapping axes, and whether the shifted frame serves them
     const int nc[3] = {tg.nc0, tg.nc1, tg.nc2};
     uint32_t fl = 0;
@@ -764,7 +1017,11 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
     if (staged && shiftable) fl |= kTileFast;
     tg.tile_info[t] = make_uint4(h.npart, fl | (tg.halo_cap << 16), h.slot0, h.slot1);
   }
-  finish_build(flags);
+  // the cell starts of this build, home for the force kernels' rescans and
+  // the next build (the other tiles of this launch read start_next)
+  const uint32_t r0 = tg.first_rank[t], r1 = tg.first_rank[t + 1];
+  for (uint32_t c = r0 + threadIdx.x; c < r1; c += kBuildBlock) o.cell_start[c] = tg.cell_start[c];
+  if (t == tg.ntiles - 1 && threadIdx.x == 0) o.cell_start[r1] = tg.cell_start[r1];
 }
 
 // Exclusions on the lists of the sorted method (apply_exclusion_list,
@@ -934,23 +1191,63 @@ extern "C" int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb,
   return check_launch("rumdx_nblist_build_n2");
 }
 
-extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream) {
-  int rc = check_nb(p, nb, "rumdx_nblist_build_sorted");
+namespace rumdx {
+namespace {
+
+int check_sorted_args(const char* entry, const rumdx_particles* p, const rumdx_nblist* nb, const float* box) {
+  const std::string e(entry);
+  int rc = check_nb(p, nb, entry);
   if (rc != RUMDX_OK) return rc;
-  if (box == nullptr) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: null box");
+  if (box == nullptr) return fail(RUMDX_E_INVALID, e + ": null box");
   if (nb->cell_rank_dev == nullptr || nb->cell_start_dev == nullptr || nb->cell_count_dev == nullptr ||
       nb->slot_cell_dev == nullptr || nb->sort_scratch_dev == nullptr || nb->nbl16_dev == nullptr ||
       nb->lane_order_dev == nullptr || nb->tile_halo_dev == nullptr || nb->tile_info_dev == nullptr ||
       nb->tile_first_rank_dev == nullptr || nb->tile_origin_dev == nullptr || nb->ntiles == 0)
-    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: cell or tile arrays not allocated");
-  if (nb->halo_cap > RUMDX_HALO_CAP_MAX) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: halo_cap too large");
+    return fail(RUMDX_E_INVALID, e + ": cell or tile arrays not allocated");
+  if (nb->halo_cap > RUMDX_HALO_CAP_MAX) return fail(RUMDX_E_INVALID, e + ": halo_cap too large");
   if (p->v_dev == nullptr || p->im_dev == nullptr || p->unsorted_index_dev == nullptr || p->f_dev == nullptr ||
       p->w_dev == nullptr)
-    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: particle arrays not allocated");
+    return fail(RUMDX_E_INVALID, e + ": particle arrays not allocated");
   if (nb->excl_start_dev != nullptr && nb->excl_ids_dev == nullptr)
-    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: exclusion ids missing");
+    return fail(RUMDX_E_INVALID, e + ": exclusion ids missing");
   for (int a = 0; a < 3; ++a)
-    if (nb->nc[a] < 5) return fail(RUMDX_E_INVALID, "rumdx_nblist_build_sorted: fewer than 5 cells per axis");
+    if (nb->nc[a] < 5) return fail(RUMDX_E_INVALID, e + ": fewer than 5 cells per axis");
+  return RUMDX_OK;
+}
+
+// The tile build, its finishing kernel and the exclusion pass: the common
+// end of both sorted builds (the particles are staged in sorted order in s,
+// their cell starts in s.start_next).
+void launch_tile_build(rumdx_particles* p, rumdx_nblist* nb, const Box& bx, const SortScratch& s, hipStream_t st) {
+  const TileGeom tg{nb->nc[0],           nb->nc[1],
+                    nb->nc[2],           nb->cell_rank_dev,
+                    s.start_next,        nb->tile_first_rank_dev,
+                    nb->tile_origin_dev, nb->ntiles,
+                    nb->halo_cap,        nb->tile_halo_dev,
+                    reinterpret_cast<uint4*>(nb->tile_info_dev)};
+  const HomeArrays home{reinterpret_cast<float4*>(p->r_dev), reinterpret_cast<float4*>(p->v_dev),
+                        reinterpret_cast<float4*>(p->f_dev), reinterpret_cast<float4*>(p->w_dev),
+                        reinterpret_cast<float4*>(p->im_dev), reinterpret_cast<float4*>(p->rlo_dev),
+                        p->unsorted_index_dev, reinterpret_cast<float4*>(nb->last_r_dev), nb->slot_cell_dev,
+                        nb->lane_order_dev, nb->cell_start_dev};
+  hipLaunchKernelGGL(k_build_tiles, dim3(nb->ntiles), dim3(kBuildBlock), sizeof(float4) * (nb->halo_cap + 1u), st,
+                     nb->flags_dev, s, tg, nb->cut_skin2_dev, nb->ntypes, bx, home, nb->nbl16_dev, nb->num_nbrs_dev,
+                     p->nvp, nb->max_nbrs);
+  hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(kBlock), 0, st, nb->flags_dev,
+                     reinterpret_cast<const uint4*>(nb->tile_info_dev), s.tile_max, nb->ntiles);
+  if (nb->excl_start_dev != nullptr)
+    hipLaunchKernelGGL(k_apply_exclusions, dim3(nb->ntiles), dim3(kBlock), 0, st, nb->flags_dev,
+                       reinterpret_cast<const uint4*>(nb->tile_info_dev), nb->tile_halo_dev, nb->halo_cap,
+                       Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev}, nb->nbl16_dev,
+                       nb->num_nbrs_dev, p->nvp, nb->max_nbrs);
+}
+
+}  // namespace
+}  // namespace rumdx
+
+extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream) {
+  int rc = check_sorted_args("rumdx_nblist_build_sorted", p, nb, box);
+  if (rc != RUMDX_OK) return rc;
   if (p->n == 0) return RUMDX_OK;
   const CellGeom g{nb->nc[0], nb->nc[1], nb->nc[2],
                    static_cast<uint32_t>(nb->nc[0]) * static_cast<uint32_t>(nb->nc[1]) * static_cast<uint32_t>(nb->nc[2])};
@@ -970,34 +1267,37 @@ extern "C" int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, c
   hipLaunchKernelGGL(k_scan_reduce, dim3(nchunk), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.chunk_sum,
                      g.ncells);
   hipLaunchKernelGGL(k_scan_apply, dim3(nchunk), dim3(kBlock), 0, st, nb->cell_count_dev, nb->flags_dev, s.chunk_sum,
-                     nb->cell_start_dev, g.ncells);
+                     s.start_next, g.ncells);
   hipLaunchKernelGGL(k_scatter, dim3(ngrid), dim3(kBlock), 0, st, s.key_lin, nb->flags_dev, nb->cell_rank_dev,
-                     nb->cell_start_dev, nb->cell_count_dev, s.perm, p->n);
+                     s.start_next, nb->cell_count_dev, s.perm, p->n);
   hipLaunchKernelGGL(k_gather, dim3(ngrid), dim3(kBlock), 0, st, nb->flags_dev, s.perm, s.key_lin, nb->cell_rank_dev,
-                     nb->cell_start_dev, r, reinterpret_cast<const float4*>(p->v_dev),
+                     s.start_next, r, reinterpret_cast<const float4*>(p->v_dev),
                      reinterpret_cast<const float4*>(p->f_dev), reinterpret_cast<const float4*>(p->w_dev),
                      reinterpret_cast<const float4*>(p->im_dev), reinterpret_cast<const float4*>(p->rlo_dev),
                      p->unsorted_index_dev, s, p->n);
-  const TileGeom tg{nb->nc[0],           nb->nc[1],
-                    nb->nc[2],           nb->cell_rank_dev,
-                    nb->cell_start_dev,  nb->tile_first_rank_dev,
-                    nb->tile_origin_dev, nb->ntiles,
-                    nb->halo_cap,        nb->tile_halo_dev,
-                    reinterpret_cast<uint4*>(nb->tile_info_dev)};
-  const HomeArrays home{reinterpret_cast<float4*>(p->r_dev), reinterpret_cast<float4*>(p->v_dev),
-                        reinterpret_cast<float4*>(p->f_dev), reinterpret_cast<float4*>(p->w_dev),
-                        reinterpret_cast<float4*>(p->im_dev), reinterpret_cast<float4*>(p->rlo_dev),
-                        p->unsorted_index_dev, reinterpret_cast<float4*>(nb->last_r_dev), nb->slot_cell_dev,
-                        nb->lane_order_dev};
-  hipLaunchKernelGGL(k_build_tiles, dim3(nb->ntiles), dim3(kBuildBlock), sizeof(float4) * (nb->halo_cap + 1u), st, nb->flags_dev,
-                     s, tg, nb->cut_skin2_dev, nb->ntypes, bx, home, nb->nbl16_dev, nb->num_nbrs_dev, p->nvp,
-                     nb->max_nbrs);
-  if (nb->excl_start_dev != nullptr)
-    hipLaunchKernelGGL(k_apply_exclusions, dim3(nb->ntiles), dim3(kBlock), 0, st, nb->flags_dev,
-                       reinterpret_cast<const uint4*>(nb->tile_info_dev), nb->tile_halo_dev, nb->halo_cap,
-                       Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev}, nb->nbl16_dev,
-                       nb->num_nbrs_dev, p->nvp, nb->max_nbrs);
+  launch_tile_build(p, nb, bx, s, st);
   return check_launch("rumdx_nblist_build_sorted");
+}
+
+extern "C" int rumdx_nblist_rebuild_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream) {
+  int rc = check_sorted_args("rumdx_nblist_rebuild_sorted", p, nb, box);
+  if (rc != RUMDX_OK) return rc;
+  if (p->n == 0) return RUMDX_OK;
+  const uint32_t ncells =
+      static_cast<uint32_t>(nb->nc[0]) * static_cast<uint32_t>(nb->nc[1]) * static_cast<uint32_t>(nb->nc[2]);
+  const Box bx = make_box(box);
+  const hipStream_t st = as_stream(stream);
+  SortScratch s = carve(nb->sort_scratch_dev, p->nvp, ncells);
+  // the previous build's cell starts (home) are the old cells of the re-sort
+  const TileGeom tg{nb->nc[0], nb->nc[1], nb->nc[2], nb->cell_rank_dev, nb->cell_start_dev,
+                    nb->tile_first_rank_dev, nb->tile_origin_dev, nb->ntiles, nb->halo_cap, nullptr, nullptr};
+  const ParticleArrays old{reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<const float4*>(p->v_dev),
+                           reinterpret_cast<const float4*>(p->f_dev), reinterpret_cast<const float4*>(p->w_dev),
+                           reinterpret_cast<const float4*>(p->im_dev), reinterpret_cast<const float4*>(p->rlo_dev),
+                           p->unsorted_index_dev};
+  hipLaunchKernelGGL(k_tile_resort, dim3(nb->ntiles), dim3(kBlock), 0, st, nb->flags_dev, s, tg, bx, old, p->n);
+  launch_tile_build(p, nb, bx, s, st);
+  return check_launch("rumdx_nblist_rebuild_sorted");
 }
 
 extern "C" int rumdx_tile_table(const int32_t nc[3], const uint32_t* rank, uint32_t* first_rank_out,

@@ -130,12 +130,16 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetFP64Forces", &PairPotential::GetFP64Forces)
       .def("TimeForceKernel", &PairPotential::TimeForceKernel, py::arg("reps"),
            py::call_guard<py::gil_scoped_release>())
-      .def("TimeNbListBuild", &PairPotential::TimeNbListBuild, py::arg("reps"),
+      .def("TimeNbListBuild", &PairPotential::TimeNbListBuild, py::arg("reps"), py::arg("full_sort") = false,
            py::call_guard<py::gil_scoped_release>())
       .def("GetNeighborListStats", &PairPotential::GetNeighborListStats)
       .def("SetNbHaloCapacity", [](PairPotential& p, unsigned cap) { p.GetNeighborList().SetHaloCapacity(cap); },
            py::arg("cap"))
       .def("GetNbHaloCapacity", [](PairPotential& p) { return p.GetNeighborList().GetHaloCapacity(); })
+      .def("SetNbForceFullSort", [](PairPotential& p, bool on) {
+        p.GetNeighborList().SetForceFullSort(on);
+        if (p.GetSample()) p.GetSample()->InvalidateGraphs();
+      }, py::arg("on"))
       .def("GetNbTileStats", [](PairPotential& p) {
         const std::vector<uint32_t> t = p.GetNeighborList().TileStats();
         py::dict d;

@@ -23,6 +23,15 @@ void NeighborList::Initialize(Sample* s) {
   sample_ = s;
   allocated_ = false;
   fused_check_pending_ = false;
+  sort_valid_ = false;
+}
+
+// The next sorted build must sort from scratch (positions were set, the
+// arrays reallocated, or the build asked for by the host); graphs recorded
+// with the tile re-sort are stale then.
+void NeighborList::InvalidateSortOrder() {
+  if (sort_valid_ && sample_) sample_->InvalidateGraphs();
+  sort_valid_ = false;
 }
 
 void NeighborList::SetCutoffs(const PairPotential* pot) {
@@ -140,6 +149,7 @@ void NeighborList::UploadExclusions() {
 void NeighborList::ResetNeighborList() {
   if (allocated_) rumdx_check(rumdx_nblist_request_rebuild(&desc_, sample_->GetStream()), "NeighborList", "ResetNeighborList");
   fused_check_pending_ = false;
+  InvalidateSortOrder();
 }
 
 void NeighborList::Allocate() {
@@ -178,7 +188,7 @@ void NeighborList::Allocate() {
   }
   num_nbrs_.resize(nvp);
   last_r_.resize(4ull * nvp);
-  flags_.resize(8);
+  flags_.resize(RUMDX_NB_FLAGS);
   std::vector<float> cs2(ntypes_ * ntypes_);
   for (size_t k = 0; k < cs2.size(); ++k) {
     const float c = cutoffs_[k] + skin_;  // squared in float (src/NeighborList.cu:1239-1240)
@@ -263,6 +273,7 @@ void NeighborList::Allocate() {
   allocated_nvp_ = nvp;
   allocated_ = true;
   fused_check_pending_ = false;
+  InvalidateSortOrder();
   rumdx_check(rumdx_nblist_request_rebuild(&desc_, st), "NeighborList", "Allocate");
   sample_->InvalidateGraphs();
 }
@@ -283,7 +294,12 @@ void NeighborList::UpdateNBlist() {
   if (!fused_check_pending_) rumdx_check(rumdx_nblist_check(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
   fused_check_pending_ = false;
   if (method == SORT) {
-    rumdx_check(rumdx_nblist_build_sorted(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
+    // skin-triggered rebuilds re-sort tile by tile; the first build, and any
+    // after the host changed positions or arrays, sorts from scratch
+    const int rc = (UseTileResort() && !force_full_sort_) ? rumdx_nblist_rebuild_sorted(p, &desc_, box, st)
+                                                          : rumdx_nblist_build_sorted(p, &desc_, box, st);
+    rumdx_check(rc, "NeighborList", "UpdateNBlist");
+    sort_valid_ = true;
     sample_->BumpSlotEpoch();  // the build may have moved particles between slots
   }
   else
@@ -292,10 +308,24 @@ void NeighborList::UpdateNBlist() {
 
 bool NeighborList::HandleOverflow() {
   if (!allocated_) return false;
-  uint32_t f[8];
-  flags_.download(f, 8, sample_->GetStream());
-  if (f[RUMDX_FLAG_ERROR])
-    throw RUMD_Error("NeighborList", "UpdateNBlist", "a cell tile's halo holds more than 65535 particles");
+  uint32_t f[RUMDX_NB_FLAGS];
+  flags_.download(f, RUMDX_NB_FLAGS, sample_->GetStream());
+  if (f[RUMDX_FLAG_ERROR]) {
+    std::string why;
+    if (f[RUMDX_FLAG_ERROR] & RUMDX_BUILD_ERR_HALO) why += " a cell tile's halo holds more than 65535 particles;";
+    if (f[RUMDX_FLAG_ERROR] & RUMDX_BUILD_ERR_MOVED)
+      why += " a particle moved more than one cell between list builds (the simulation is unstable);";
+    if (f[RUMDX_FLAG_ERROR] & RUMDX_BUILD_ERR_WAIT) why += " a tile of the re-sort waited too long for the tiles before it;";
+    if (f[RUMDX_FLAG_ERROR] & RUMDX_BUILD_ERR_TILE)
+      why += " a cell tile holds more than " + std::to_string(RUMDX_RESORT_TILE_MAX) + " particles;";
+    throw RUMD_Error("NeighborList", "UpdateNBlist", "the sorted neighbour list is unusable:" + why);
+  }
+  if (desc_.nbl16_dev != nullptr) {
+    // a crowded tile (more than half the re-sort's capacity) sorts from scratch
+    const bool was = UseTileResort();
+    last_max_tile_ = f[RUMDX_FLAG_MAXTILE];
+    if (was != UseTileResort()) InvalidateSortOrder();
+  }
   if (desc_.nbl16_dev != nullptr && f[RUMDX_FLAG_MAXHALO] > 0u) {
     // Capacity for the largest halo of the last build plus 1/16.  Tiles beyond
     // the capacity computed exactly from global memory: grow for them.  The
@@ -370,31 +400,31 @@ void NeighborList::DownloadCounts(std::vector<uint32_t>& out) {
 
 unsigned NeighborList::GetActualMaxNumNbrs() {
   if (!allocated_) return 0;
-  uint32_t f[8];
-  flags_.download(f, 8, sample_->GetStream());
+  uint32_t f[RUMDX_NB_FLAGS];
+  flags_.download(f, RUMDX_NB_FLAGS, sample_->GetStream());
   return f[RUMDX_FLAG_MAXLEN];
 }
 
 int NeighborList::GetRebuildRequired() {
   if (!allocated_) return 1;
-  uint32_t f[8];
-  flags_.download(f, 8, sample_->GetStream());
+  uint32_t f[RUMDX_NB_FLAGS];
+  flags_.download(f, RUMDX_NB_FLAGS, sample_->GetStream());
   return f[RUMDX_FLAG_REBUILD] ? 1 : 0;
 }
 
 unsigned NeighborList::GetNumberOfBuilds() {
   if (!allocated_) return 0;
-  uint32_t f[8];
-  flags_.download(f, 8, sample_->GetStream());
+  uint32_t f[RUMDX_NB_FLAGS];
+  flags_.download(f, RUMDX_NB_FLAGS, sample_->GetStream());
   return f[RUMDX_FLAG_BUILDS];
 }
 
 std::vector<uint32_t> NeighborList::TileStats() {
   if (!allocated_ || desc_.nbl16_dev == nullptr) return {};
   const hipStream_t st = sample_->GetStream();
-  std::vector<uint32_t> info(4ull * desc_.ntiles), f(8);
+  std::vector<uint32_t> info(4ull * desc_.ntiles), f(RUMDX_NB_FLAGS);
   tile_info_.download(info.data(), info.size(), st);
-  flags_.download(f.data(), 8, st);
+  flags_.download(f.data(), RUMDX_NB_FLAGS, st);
   // tile_info = {halo particles, flags | halo capacity << 16, slot0, slot1};
   // flag bit 0: fast, bits 1-6: per-axis shift below / above (RUMDX_TILE_*)
   uint32_t staged = 0, fast = 0, wrapping = 0, shifted = 0;
@@ -550,12 +580,13 @@ float PairPotential::TimeForceKernel(unsigned reps) {
   return ms / reps;
 }
 
-float PairPotential::TimeNbListBuild(unsigned reps) {
+float PairPotential::TimeNbListBuild(unsigned reps, bool full_sort) {
   if (!sample) throw RUMD_Error(GetID_String(), "TimeNbListBuild", "no sample");
   if (reps == 0) reps = 1;
   neighborList.UpdateNBlist();
   sample->Synchronize();  // settles list and halo capacities first
   neighborList.UpdateNBlist();
+  neighborList.SetForceFullSort(full_sort);
   const hipStream_t st = sample->GetStream();
   hipEvent_t a, b;
   hip_check(hipEventCreate(&a), "PairPotential", "TimeNbListBuild");
@@ -567,6 +598,7 @@ float PairPotential::TimeNbListBuild(unsigned reps) {
   }
   hip_check(hipEventRecord(b, st), "PairPotential", "TimeNbListBuild");
   hip_check(hipEventSynchronize(b), "PairPotential", "TimeNbListBuild");
+  neighborList.SetForceFullSort(false);
   float ms = 0.f;
   hip_check(hipEventElapsedTime(&ms, a, b), "PairPotential", "TimeNbListBuild");
   (void)hipEventDestroy(a);

@@ -137,6 +137,7 @@ class Potential {
   void SetVerbose(bool vb) { verbose = vb; }
   void SetID_String(const std::string& s) { ID_String = s; }
   std::string GetID_String() const { return ID_String; }
+  Sample* GetSample() const { return sample; }
 
  protected:
   Sample* sample = nullptr;
@@ -175,6 +176,12 @@ class NeighborList {
   void NoteFusedCheck() { fused_check_pending_ = true; }
   void ResetFusedCheck() { fused_check_pending_ = false; }
   bool FusedCheckPending() const { return fused_check_pending_; }
+  // True when the next sorted rebuild must be a full one (rumdx_nblist_build_sorted).
+  bool NeedsFullSort() const { return GetResolvedMethod() == SORT && !UseTileResort(); }
+  bool UseTileResort() const { return sort_valid_ && last_max_tile_ <= RUMDX_RESORT_TILE_MAX / 2; }
+  void InvalidateSortOrder();
+  // Every sorted rebuild sorts from scratch while set (for timing the two paths).
+  void SetForceFullSort(bool on) { force_full_sort_ = on; }
   // Reads the flags; grows the list after an overflow.  Synchronises.
   bool HandleOverflow();
   // Asks the next UpdateNBlist to rebuild (device flag, on the sample's stream).
@@ -223,6 +230,12 @@ class NeighborList {
   DeviceArray<uint32_t> tile_first_rank_, tile_origin_;
   DeviceArray<uint16_t> nbl16_;
   uint32_t halo_cap_ = 0;  // LDS halo capacity of the tile kernels (particles); 0 = estimate
+  // Particles are in the cell order of this list's last sorted build (and the
+  // home cell starts describe it), so a rebuild may re-sort tile by tile
+  // (rumdx_nblist_rebuild_sorted); false until a full sorted build is queued.
+  bool sort_valid_ = false;
+  bool force_full_sort_ = false;
+  uint32_t last_max_tile_ = 0;  // most particles in one tile, last synchronised build
   DeviceArray<float> last_r_, cut_skin2_;
   DeviceArray<char> sort_scratch_;
   DeviceArray<uint16_t> lane_order_;
@@ -271,7 +284,9 @@ class PairPotential : public Potential {
   // Average duration (ms) of one full list rebuild (skin check, sort chain,
   // build) over `reps` forced rebuilds of the current positions, bracketed
   // by HIP events on the sample's stream.
-  float TimeNbListBuild(unsigned reps);
+  // Average duration (ms) of a forced rebuild: the tile re-sort path, or with
+  // full_sort the full counting sort (each rep includes the skin check launch).
+  float TimeNbListBuild(unsigned reps, bool full_sort = false);
   // All-FP64 pair arithmetic (rumdx_pair_lj_fp64): the precision study's
   // comparison path; FP32 pair arithmetic otherwise.
   void SetFP64Forces(bool on);

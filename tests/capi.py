@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
     L.rumdx_nblist_request_rebuild.argtypes = [NP, vp]
     L.rumdx_nblist_build_n2.argtypes = [PP, NP, f32a, vp]
     L.rumdx_nblist_build_sorted.argtypes = [PP, NP, f32a, vp]
+    L.rumdx_nblist_rebuild_sorted.argtypes = [PP, NP, f32a, vp]
     L.rumdx_pair_lj.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
     L.rumdx_pair_lj_fp64.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
     L.rumdx_pair_form.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, i32, vp]
@@ -256,7 +257,7 @@ class GpuSystem:
         self.export_buf = DeviceBuffer(4 * self.nvp * max_nbrs)
         self.num_nbrs = DeviceBuffer(4 * self.nvp)
         self.last_r = DeviceBuffer(16 * self.nvp)
-        self.flags = DeviceBuffer(32)
+        self.flags = DeviceBuffer(4 * 16)  # RUMDX_NB_FLAGS
         nb = NbList()
         nb.nbl_dev = self.nbl.value if self.nbl else None
         nb.num_nbrs_dev, nb.last_r_dev = self.num_nbrs.value, self.last_r.value
@@ -303,11 +304,15 @@ class GpuSystem:
         check(lib().rumdx_nblist_request_rebuild(C.byref(self.nb), None))
 
     # ------------------------------------------------------------ operations
-    def update_nblist(self):
+    def update_nblist(self, resort=False):
+        """The skin check and the list build; with resort the sorted build is
+        the tile re-sort (rumdx_nblist_rebuild_sorted), which needs a previous
+        sorted build of these particles."""
         L = lib()
         check(L.rumdx_nblist_check(C.byref(self.p), C.byref(self.nb), self.box8, None))
         if self.method == "sorted":
-            check(L.rumdx_nblist_build_sorted(C.byref(self.p), C.byref(self.nb), self.box8, None))
+            build = L.rumdx_nblist_rebuild_sorted if resort else L.rumdx_nblist_build_sorted
+            check(build(C.byref(self.p), C.byref(self.nb), self.box8, None))
         else:
             check(L.rumdx_nblist_build_n2(C.byref(self.p), C.byref(self.nb), self.box8, None))
 
@@ -349,7 +354,7 @@ class GpuSystem:
         return self.idx.download(np.uint32, self.nvp)[: self.n]
 
     def flags_host(self):
-        return self.flags.download(np.uint32, 8)
+        return self.flags.download(np.uint32, 16)
 
     def neighbor_lists(self):
         """(counts, slots) in RUMD's column-major u32 format, via rumdx_nblist_export."""

@@ -334,3 +334,27 @@ def test_run_segments_start_at_the_first_step_and_end_at_resets():
         ritg.Integrate()
     np.testing.assert_array_equal(s.GetPositions(), ref.GetPositions())
     np.testing.assert_array_equal(s.GetVelocities(), ref.GetVelocities())
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_tile_resort_trajectory_equals_full_sort_trajectory(mixed):
+    """Skin-triggered rebuilds take the tile re-sort inside captured blocks;
+    since it reproduces the full sort's particle order bit for bit, 300 NVT
+    steps give the same trajectory as with a full sort at every rebuild."""
+    sysm = configs.ka_random_lattice(40000, density=1.2, temperature=0.8, seed=33)
+    out = []
+    for full in (False, True):
+        s, pot = ka_sample(conf_from_system(sysm), cutoff=rumd.ShiftedForce, mixed=mixed)
+        itg = rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.8, thermostatRelaxationTime=0.2)
+        s.SetIntegrator(itg)
+        pot.SetNbForceFullSort(full)
+        print(f"[case] 300 steps, full sort at every rebuild: {full}, mixed={mixed}", flush=True)
+        s.RunSteps(0, 300, 10, 100)
+        out.append((s.GetPositions(), s.GetVelocities(), s.GetImages(), itg.GetThermostatState(),
+                    pot.GetNumberOfNbListBuilds()))
+    (r0, v0, i0, p0, b0), (r1, v1, i1, p1, b1) = out
+    assert b0 == b1 and b0 >= 4
+    np.testing.assert_array_equal(r0, r1)
+    np.testing.assert_array_equal(v0, v1)
+    np.testing.assert_array_equal(i0, i1)
+    assert p0 == p1

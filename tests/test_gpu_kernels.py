@@ -483,3 +483,82 @@ def test_rectangular_box_sorted_forces_and_lists():
     cnt, nbl = g.neighbor_lists()
     off, idx = oracle.nblist_cells_f32(pos4, b8, 2, configs.cut_skin2_table(table, 0.3), oracle.cell_grid(b8, 2.5, 0.3))
     assert gpu_sets_original_ids(cnt, nbl, uidx) == csr_to_sets(off, idx)
+
+
+def _pair_of_sorted_systems(n, seed, move, rlo=False):
+    """Two identical sorted systems (C ABI) after a full sorted build, then
+    the same displacement of every particle (move x U(-1, 1) per axis, in
+    slot order) applied to both and a rebuild requested on both."""
+    sysm = configs.ka_random_lattice(n, density=1.2, seed=seed)
+    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
+    vel4 = configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses)
+    b8 = configs.box8(sysm.box)
+    gs = [GpuSystem(pos4, vel4, b8, 2, ka_table(), method="sorted", max_nbrs=200, rlo=rlo) for _ in range(2)]
+    for g in gs:
+        g.update_nblist()
+    r = gs[0].get("r").astype(np.float64)
+    rng = np.random.default_rng(seed + 1)
+    r[:, :3] += rng.uniform(-move, move, size=(n, 3))
+    L = sysm.box
+    r[:, :3] -= L * np.rint(r[:, :3] / L)
+    r32 = r.astype(np.float32)
+    r32[:, 3] = gs[0].get("r")[:, 3]  # the type bits
+    for g in gs:
+        g.r.upload(np.vstack([r32, np.zeros((g.nvp - n, 4), np.float32)]))
+        check(lib().rumdx_nblist_request_rebuild(C.byref(g.nb), None))
+    return sysm, gs
+
+
+@pytest.mark.parametrize("n,rlo", [(30000, False), (65536, True)])
+def test_tile_resort_equals_full_sort(n, rlo):
+    """rumdx_nblist_rebuild_sorted (tiles re-sort their own neighbourhood,
+    offsets by look-back) gives the full counting sort's result bit for bit:
+    slot order, cell starts, every particle array, lists, lane order and tile
+    records; the build is recorded the same way."""
+    sysm, (a, b) = _pair_of_sorted_systems(n, 90 + n % 7, 0.4, rlo)
+    print(f"[case] tile re-sort vs full sort n={n} rlo={rlo}", flush=True)
+    a.update_nblist(resort=True)
+    b.update_nblist(resort=False)
+    fa, fb = a.flags_host(), b.flags_host()
+    assert fa[6] == 0 and fb[6] == 0, (fa, fb)
+    assert np.array_equal(fa, fb), (fa, fb)
+    assert fa[2] == 2 and fa[0] == 0 and fa[8] > 0
+    for name in ("r", "v", "f", "w", "im") + (("rlo",) if rlo else ()):
+        assert np.array_equal(a.get(name), b.get(name)), name
+    assert np.array_equal(a.unsorted_index(), b.unsorted_index())
+    ncells = int(a.nc.prod())
+    assert np.array_equal(a.cell_start.download(np.uint32, ncells + 1), b.cell_start.download(np.uint32, ncells + 1))
+    for name, dt, cnt in (("num_nbrs", np.uint32, a.nvp), ("lane_order", np.uint16, a.nvp),
+                          ("tile_info", np.uint32, 4 * a.ntiles), ("slot_cell", np.uint32, a.nvp),
+                          ("last_r", np.float32, 4 * a.nvp), ("tile_halo", np.uint32, a.ntiles * a.nb.halo_cap)):
+        assert np.array_equal(getattr(a, name).download(dt, cnt), getattr(b, name).download(dt, cnt)), name
+    ca, la = a.neighbor_lists()
+    cb, lb = b.neighbor_lists()
+    assert np.array_equal(ca, cb)
+    for i in range(0, n, 101):
+        assert np.array_equal(la[: ca[i], i], lb[: cb[i], i])
+    # and the sets are RUMD's at the moved positions
+    uidx = a.unsorted_index()
+    pos4 = unsort(a.get("r"), uidx)
+    b8 = configs.box8(sysm.box)
+    off, idx = oracle.nblist_cells_f32(pos4, b8, 2, configs.cut_skin2_table(ka_table(), 0.3),
+                                       oracle.cell_grid(b8, 2.5, 0.3))
+    assert gpu_sets_original_ids(ca, la, uidx) == csr_to_sets(off, idx)
+
+
+def test_tile_resort_reports_a_particle_moved_beyond_a_cell():
+    """The tile re-sort's precondition (every particle within a cell of its
+    last build's cell) is checked on the device: a particle moved too far
+    is reported in flags[ERROR] (RUMDX_BUILD_ERR_MOVED) with every access in
+    range, and the host would throw.  (A move inside one tile is still sorted
+    correctly; 6.5 cells takes the particle out of every tile that held its
+    old cell.)"""
+    sysm, (a, _) = _pair_of_sorted_systems(20000, 95, 0.05)
+    r = a.get("r").copy()
+    w = float(sysm.box[0]) / float(a.nc[0])
+    r[123, 0] = np.float32(r[123, 0] + 6.5 * w if r[123, 0] < 0 else r[123, 0] - 6.5 * w)  # beyond any tile's reach
+    a.r.upload(np.vstack([r, np.zeros((a.nvp - a.n, 4), np.float32)]))
+    print("[case] tile re-sort with one particle moved 6.5 cells", flush=True)
+    a.update_nblist(resort=True)
+    f = a.flags_host()
+    assert f[6] & 2, f
