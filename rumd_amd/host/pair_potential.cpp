@@ -296,14 +296,38 @@ void NeighborList::UpdateNBlist() {
   if (method == SORT) {
     // skin-triggered rebuilds re-sort tile by tile; the first build, and any
     // after the host changed positions or arrays, sorts from scratch
-    const int rc = (UseTileResort() && !force_full_sort_) ? rumdx_nblist_rebuild_sorted(p, &desc_, box, st)
-                                                          : rumdx_nblist_build_sorted(p, &desc_, box, st);
+    const bool resort = UseTileResort() && !force_full_sort_;
+    const int rc = resort ? rumdx_nblist_rebuild_sorted(p, &desc_, box, st) : rumdx_nblist_build_sorted(p, &desc_, box, st);
     rumdx_check(rc, "NeighborList", "UpdateNBlist");
     sort_valid_ = true;
     sample_->BumpSlotEpoch();  // the build may have moved particles between slots
+    if (!resort && !force_full_sort_) Settle();
   }
   else
     rumdx_check(rumdx_nblist_build_n2(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
+}
+
+// A full sort starts a new particle order.  Its halo and tile occupancy decide
+// the halo capacity and whether later rebuilds re-sort tile by tile, so they
+// are read back at once (outside a graph capture; a captured full sort is
+// settled at the next synchronisation) and any rebuild they ask for runs now.
+// Eager steps and graph blocks therefore see the same list history, whenever
+// the host next synchronises.
+void NeighborList::Settle() {
+  const hipStream_t st = sample_->GetStream();
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hip_check(hipStreamIsCapturing(st, &cs), "NeighborList", "UpdateNBlist");
+  if (cs != hipStreamCaptureStatusNone) return;
+  for (int round = 0; round < 4 && HandleOverflow(); ++round) {
+    // capacities changed and a rebuild is pending
+    rumdx_particles* p = sample_->GetParticleData().Descriptor();
+    const float* box = sample_->GetSimulationBox()->GetBox8();
+    const int rc = UseTileResort() ? rumdx_nblist_rebuild_sorted(p, &desc_, box, st)
+                                   : rumdx_nblist_build_sorted(p, &desc_, box, st);
+    rumdx_check(rc, "NeighborList", "UpdateNBlist");
+    sort_valid_ = true;
+    sample_->BumpSlotEpoch();
+  }
 }
 
 bool NeighborList::HandleOverflow() {

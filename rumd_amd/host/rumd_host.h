@@ -214,6 +214,7 @@ class NeighborList {
 
  private:
   void Allocate();
+  void Settle();
   Sample* sample_ = nullptr;
   Method method_ = DEFAULT;
   float skin_ = 0.5f;  // reference default (src/NeighborList.cu:50)

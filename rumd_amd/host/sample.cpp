@@ -216,7 +216,6 @@ void Sample::PrepareRun() {
   if (!box_) throw RUMD_Error("Sample", "Run", "No simulation box has been set");
   if (!itg_) throw RUMD_Error("Sample", "Run", "Integrator has not been set");
   if (potentials_.empty()) throw RUMD_Error("Sample", "Run", "Potential has not been set");
-  bool settle = false;
   for (Potential* p : potentials_)
     if (auto* pp = dynamic_cast<PairPotential*>(p)) {
       if (pp->NumTypesInTable() != GetNumberOfTypes()) pp->Initialize();
@@ -224,13 +223,9 @@ void Sample::PrepareRun() {
       nl.EnsureAllocated();
       // a full sort is due (first build, or positions set by the host): run
       // it now, outside any graph, so that recorded steps hold the tile
-      // re-sort of skin-triggered rebuilds
-      if (nl.NeedsFullSort()) {
-        nl.UpdateNBlist();
-        settle = true;
-      }
+      // re-sort of skin-triggered rebuilds (the build settles its capacities)
+      if (nl.NeedsFullSort()) nl.UpdateNBlist();
     }
-  if (settle) Synchronize();  // learns the tile occupancy that selects the re-sort
 }
 
 void Sample::StepOnce(bool zero_momentum) {

@@ -38,10 +38,11 @@ def _conf(sysm):
                                  masses=sysm.masses, velocities=sysm.velocities, images=np.zeros_like(sysm.positions))
 
 
-def _ka_1m_state_checks(s, pot, sysm, what, lists):
+def _ka_1m_state_checks(s, pot, sysm, what, lists, nbar_range=(88.0, 97.0)):
     """Forces, PE and W against the FP64 oracle at the sample's current float
     positions; with `lists`, the sets of the current list (built at these
-    positions) against the FP32 restatement of RUMD's cell build."""
+    positions) against the FP32 restatement of RUMD's cell build, and the mean
+    list length within nbar_range."""
     pos4 = configs.pack_pos4(s.GetPositions(), sysm.types)
     table = ka_table()
     print(f"[case] KA N=1048576 {what}: oracle forces", flush=True)
@@ -62,7 +63,7 @@ def _ka_1m_state_checks(s, pot, sysm, what, lists):
     assert np.array_equal(off, off_ref), what
     assert np.array_equal(sorted_rows(off, ids), sorted_rows(off_ref, idx_ref)), what
     nbar = off[-1] / sysm.n
-    assert 88.0 < nbar < 97.0, nbar
+    assert nbar_range[0] < nbar < nbar_range[1], nbar
 
 
 @pytest.mark.parametrize("mixed", [False, True])
@@ -83,7 +84,8 @@ def test_ka_1m_forces_sums_and_sets(mixed):
     s.SetIntegrator(itg)
     print(f"[case] KA N=1048576 mixed={mixed}: FCC start", flush=True)
     s.CalcF()
-    _ka_1m_state_checks(s, pot, sysm, "FCC start", lists=True)
+    # the lattice's neighbour shells give longer lists than the liquid's ~92
+    _ka_1m_state_checks(s, pot, sysm, "FCC start", lists=True, nbar_range=(100.0, 112.0))
     print(f"[case] KA N=1048576 mixed={mixed}: 200 NVT steps", flush=True)
     s.RunSteps(0, 200, 10, 100)
     builds = pot.GetNumberOfNbListBuilds()
@@ -92,7 +94,8 @@ def test_ka_1m_forces_sums_and_sets(mixed):
     _ka_1m_state_checks(s, pot, sysm, "after 200 steps (list of the last build)", lists=False)
     s.SortParticles()  # rebuild at the current positions, then compare the sets
     s.CalcF()
-    assert pot.GetNumberOfNbListBuilds() == builds + 1
+    # a full sort; one more build when it changes the halo capacity
+    assert pot.GetNumberOfNbListBuilds() in (builds + 1, builds + 2)
     _ka_1m_state_checks(s, pot, sysm, "after 200 steps (rebuilt)", lists=True)
     st = pot.GetNbTileStats()
     cap = pot.GetNbHaloCapacity()
