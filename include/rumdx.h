@@ -395,6 +395,18 @@ int rumdx_kinetic_energy(const rumdx_particles* p, const rumdx_nh_state* nh_dev,
 int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_mask, double* out_dev,
                     void* workspace_dev, void* stream);
 
+/* rumdx_reduce_f4 without a caller workspace, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * block partials live in a file-scope device array of the library (1 MiB,
+ * one copy per GPU, held until the process ends), so n is limited to
+ * RUMDX_STATIC_REDUCE_MAX_N (larger n: RUMDX_E_INVALID).  The same kernels
+ * and summation order as rumdx_reduce_f4, so the results are bit-identical.
+ * Calls that overlap on one GPU (different streams or threads) share the
+ * array and race; serialise them.  Allocates nothing. */
+#define RUMDX_STATIC_REDUCE_MAX_N 8388608u
+int rumdx_reduce_f4_static(const float* arr4_dev, uint32_t n, uint32_t comp_mask, double* out_dev, void* stream);
+
 /* ---------------------------------------------------------------- replicas
  * Batched independent samples ("replicas") in one set of device arrays,
  * advanced by one launch per kernel for all of them.  No counterpart in the

@@ -11,6 +11,9 @@
 // kernel then adds the block partials in a fixed order and advances Ps in FP64
 // (bitwise reproducible).  The reference instead stores m v^2 per particle and
 // reduces it in FP32 over ~4 launches.
+#include <atomic>
+#include <mutex>
+
 #include "leapfrog.h"
 #include "rumdx_common.h"
 
@@ -285,6 +288,67 @@ extern "C" int rumdx_kinetic_energy(const rumdx_particles* p, const rumdx_nh_sta
   hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, as_stream(stream), Workspace::partials(workspace_dev), nb,
                      1u, 1u, 0.5, out_dev);
   return check_launch("rumdx_kinetic_energy");
+}
+
+namespace rumdx {
+namespace {
+
+// Block partials of rumdx_reduce_f4_static: four doubles per kBlock elements
+// for up to RUMDX_STATIC_REDUCE_MAX_N elements.  Each GPU holds its own copy
+// at its own address; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (scratch_on_current_device).
+constexpr size_t kStaticPartials = 4ull * (RUMDX_STATIC_REDUCE_MAX_N / kBlock);
+static_assert(RUMDX_STATIC_REDUCE_MAX_N % kBlock == 0, "whole blocks");
+__device__ __align__(256) double scratch[kStaticPartials];
+
+constexpr int kMaxDevices = 64;
+std::atomic<double*> g_scratch_addr[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+int scratch_on_current_device(const char* entry, double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(RUMDX_E_HIP, std::string(entry) + ": hipGetDevice: " + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(RUMDX_E_INVALID, std::string(entry) + ": device index out of range");
+  double* ptr = g_scratch_addr[dev].load(std::memory_order_acquire);
+  if (ptr == nullptr) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    ptr = g_scratch_addr[dev].load(std::memory_order_relaxed);
+    if (ptr == nullptr) {
+      void* addr = nullptr;
+      e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || addr == nullptr)
+        return fail(RUMDX_E_HIP, std::string(entry) + ": hipGetSymbolAddress(scratch): " + hipGetErrorString(e));
+      ptr = static_cast<double*>(addr);
+      g_scratch_addr[dev].store(ptr, std::memory_order_release);
+    }
+  }
+  *out = ptr;
+  return RUMDX_OK;
+}
+
+}  // namespace
+}  // namespace rumdx
+
+extern "C" int rumdx_reduce_f4_static(const float* arr4_dev, uint32_t n, uint32_t comp_mask, double* out_dev,
+                                      void* stream) {
+  const char* entry = "rumdx_reduce_f4_static";
+  if (arr4_dev == nullptr || out_dev == nullptr) return fail(RUMDX_E_INVALID, std::string(entry) + ": null argument");
+  if (n == 0) return fail(RUMDX_E_INVALID, std::string(entry) + ": empty array");
+  if (n > RUMDX_STATIC_REDUCE_MAX_N)
+    return fail(RUMDX_E_INVALID, std::string(entry) + ": n = " + std::to_string(n) + " exceeds the static scratch (" +
+                                     std::to_string(RUMDX_STATIC_REDUCE_MAX_N) + " elements); use rumdx_reduce_f4");
+  double* partials = nullptr;
+  const int rc = scratch_on_current_device(entry, &partials);
+  if (rc != RUMDX_OK) return rc;
+  const uint32_t nb = blocks_for(n);
+  hipLaunchKernelGGL(k_reduce_f4, dim3(nb), dim3(kBlock), 0, as_stream(stream), reinterpret_cast<const float4*>(arr4_dev),
+                     comp_mask & 15u, partials, n);
+  hipLaunchKernelGGL(k_finish_sums, dim3(1), dim3(kBlock), 0, as_stream(stream), partials, nb, 4u, comp_mask & 15u,
+                     1.0, out_dev);
+  return check_launch(entry);
 }
 
 extern "C" int rumdx_reduce_f4(const float* arr4_dev, uint32_t n, uint32_t comp_mask, double* out_dev,
