@@ -631,16 +631,17 @@ __global__ __launch_bounds__(kBlock) void k_build_finish(uint32_t* __restrict__ 
 }
 
 // Per-wave LDS of the cooperative list build: the stencil segments of the
-// cell being built and a window of its candidates (LDS offsets, in stencil
-// order).
+// cell being built, a window of its candidates (LDS offsets, in stencil
+// order) and the staged lists of the two particles tested together.  A staged
+// list holds as many entries as the window, so appends need no bounds check.
 constexpr uint32_t kCandWindow = 512;
-constexpr uint32_t kStageMax = 256;  // list entries of one particle staged in LDS before the group stores
+constexpr uint32_t kStageMax = 256;  // longest list the staged path writes (max_nbrs bound)
 struct WaveBuild {
   uint16_t cand[kCandWindow];
   uint16_t seg_start[128];  // LDS offset of the segment's first particle
   uint16_t seg_pre[128];    // candidates before the segment
   uint16_t seg_len[128];
-  __attribute__((aligned(16))) uint16_t list[kStageMax];
+  __attribute__((aligned(16))) uint16_t list[2][kCandWindow];
 };
 
 // Orders the LDS accesses of one wave (lanes read what other lanes wrote).
@@ -660,42 +661,90 @@ __device__ __forceinline__ uint32_t append_chunk(bool acc, uint32_t entry, uint3
   return cnt + static_cast<uint32_t>(__popcll(mask));
 }
 
-// Tests n chunks of 64 window candidates (from q0) against one particle and
-// appends the accepted ones to the wave's staged list; returns the new count.
-// The n chunks' LDS reads are issued together.
-template <bool WRAPS, int N>
-__device__ __forceinline__ uint32_t scan_chunks(const WaveBuild& wb, uint16_t* list, const float4* s_pos, float4 pi,
-                                                const float* crow, uint32_t self, const Box& bx, uint32_t q0,
-                                                uint32_t cnt, uint32_t max_nbrs) {
+// One particle's test of one chunk of candidates: the accepted ones go to its
+// staged list in candidate order (ballot + lane count); returns the new count.
+// The particle's own entry is rejected through an infinite distance, so the
+// acceptance is one compare whose lane mask is the ballot.
+template <bool WRAPS>
+__device__ __forceinline__ uint32_t test_chunk(float4 pi, float4 rj, uint32_t e, float cut, uint32_t self,
+                                               const Box& bx, uint16_t* list, uint32_t cnt) {
+  const float d2 = WRAPS ? min_image_d2_f32(pi, rj, bx) : plain_d2_f32(pi, rj);
+  const bool acc = (e != self ? d2 : __builtin_inff()) < cut;
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
+  const uint32_t k = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+  uint16_t* at = list + cnt;  // cnt is uniform: the base stays scalar
+  if (acc) at[k] = static_cast<uint16_t>(e);
+  return cnt + static_cast<uint32_t>(__builtin_popcountll(mask));
+}
+
+// Tests N chunks of 64 window candidates (from q0) against particle a and,
+// when TWO, particle b, appending the accepted ones to their staged lists.
+// The chunks' window and position reads are issued together and shared by
+// both particles.
+template <bool WRAPS, bool TWO, int N>
+__device__ __forceinline__ void scan_chunks(const WaveBuild& wb, uint16_t* list_a, uint16_t* list_b,
+                                            const float4* s_pos, float4 pa, float4 pb, const float* crow_a,
+                                            const float* crow_b, uint32_t self_a, uint32_t self_b, const Box& bx,
+                                            uint32_t q0, uint32_t& cnt_a, uint32_t& cnt_b) {
   const uint32_t lane = threadIdx.x & 63u;
   uint32_t e[N];
   float4 rj[N];
-  float cut[N];
+  float cut_a[N], cut_b[N];
 #pragma unroll
   for (int u = 0; u < N; ++u) e[u] = wb.cand[q0 + 64u * u + lane];
 #pragma unroll
   for (int u = 0; u < N; ++u) rj[u] = halo_pos(s_pos, e[u]);
 #pragma unroll
-  for (int u = 0; u < N; ++u) cut[u] = crow[__float_as_uint(rj[u].w)];
+  for (int u = 0; u < N; ++u) {
+    const uint32_t tj = __float_as_uint(rj[u].w);
+    cut_a[u] = crow_a[tj];
+    if (TWO) cut_b[u] = crow_b[tj];
+  }
 #pragma unroll
   for (int u = 0; u < N; ++u) {
-    const float d2 = WRAPS ? min_image_d2_f32(pi, rj[u], bx) : plain_d2_f32(pi, rj[u]);
-    const bool acc = e[u] != self && d2 < cut[u];
-    const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
-    const uint32_t k = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                                       __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-    if (acc && k < max_nbrs) list[k] = static_cast<uint16_t>(e[u]);
-    cnt += static_cast<uint32_t>(__builtin_popcountll(mask));
+    cnt_a = test_chunk<WRAPS>(pa, rj[u], e[u], cut_a[u], self_a, bx, list_a, cnt_a);
+    if (TWO) cnt_b = test_chunk<WRAPS>(pb, rj[u], e[u], cut_b[u], self_b, bx, list_b, cnt_b);
   }
-  return cnt;
+}
+
+// Scans the whole padded window for particle a (and b when TWO).
+template <bool WRAPS, bool TWO>
+__device__ __forceinline__ void scan_window(const WaveBuild& wb, uint16_t* list_a, uint16_t* list_b,
+                                            const float4* s_pos, float4 pa, float4 pb, const float* crow_a,
+                                            const float* crow_b, uint32_t self_a, uint32_t self_b, const Box& bx,
+                                            uint32_t padded, uint32_t& cnt_a, uint32_t& cnt_b) {
+  const uint32_t full = padded & ~255u;  // passes of four chunks, then the rest
+  uint32_t q0 = 0;
+  for (; q0 < full; q0 += 256)
+    scan_chunks<WRAPS, TWO, 4>(wb, list_a, list_b, s_pos, pa, pb, crow_a, crow_b, self_a, self_b, bx, q0, cnt_a, cnt_b);
+  if (padded - q0 >= 128) {
+    scan_chunks<WRAPS, TWO, 2>(wb, list_a, list_b, s_pos, pa, pb, crow_a, crow_b, self_a, self_b, bx, q0, cnt_a, cnt_b);
+    q0 += 128;
+  }
+  if (padded - q0 >= 64)
+    scan_chunks<WRAPS, TWO, 1>(wb, list_a, list_b, s_pos, pa, pb, crow_a, crow_b, self_a, self_b, bx, q0, cnt_a, cnt_b);
+}
+
+// Writes one staged list to slot i as whole 16-byte groups (group g at
+// nbl16[(g nvp + i) 8]); entries of the last group past the count are never
+// read.
+__device__ __forceinline__ void store_staged(const uint16_t* list, uint32_t cnt, uint32_t max_nbrs,
+                                             uint16_t* __restrict__ nbl16, uint32_t nvp, uint32_t i, uint32_t lane) {
+  const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
+  if (lane * 8u < stored) {
+    const u32x4 g = *reinterpret_cast<const u32x4*>(list + lane * 8u);
+    reinterpret_cast<u32x4*>(nbl16)[static_cast<size_t>(lane) * nvp + i] = g;
+  }
 }
 
 // The cooperative build of one cell when its whole stencil fits one window
 // and a list fits the wave's staging buffer: the window holds the
 // candidates' LDS byte offsets (the list entries themselves), padded to a
 // multiple of 64 with the halo's sentinel (a position at +inf, never
-// accepted), so chunks read without bounds checks; each particle's list is
-// staged in LDS and written as whole 16-byte groups.
+// accepted), so chunks read without bounds checks.  The cell's particles are
+// tested two at a time against each chunk; each list is staged in LDS and
+// written as whole 16-byte groups.
 template <bool WRAPS>
 __device__ __forceinline__ void build_cell_staged(const float4* s_pos, const float* s_cut,
                                                   uint32_t nt, const Box& bx, uint32_t coff, uint32_t m,
@@ -710,33 +759,35 @@ __device__ __forceinline__ void build_cell_staged(const float4* s_pos, const flo
   const uint32_t padded = (total + 63u) & ~63u;
   if (total + lane < padded) wb.cand[total + lane] = static_cast<uint16_t>(sentinel);
   wave_lds_sync();
-  const uint32_t full = padded & ~255u;  // passes of four chunks, then the rest
-  for (uint32_t p = 0; p < m; ++p) {
-    const uint32_t po = coff + p;
-    const float4 pi = s_pos[po];
-    const float* crow = s_cut + __float_as_uint(pi.w) * nt;
-    const uint32_t self = po << 4;
-    uint32_t cnt = 0;
-    for (uint32_t q0 = 0; q0 < full; q0 += 256)
-      cnt = scan_chunks<WRAPS, 4>(wb, wb.list, s_pos, pi, crow, self, bx, q0, cnt, max_nbrs);
-    uint32_t q0 = full;
-    if (padded - q0 >= 128) {
-      cnt = scan_chunks<WRAPS, 2>(wb, wb.list, s_pos, pi, crow, self, bx, q0, cnt, max_nbrs);
-      q0 += 128;
-    }
-    if (padded - q0 >= 64) cnt = scan_chunks<WRAPS, 1>(wb, wb.list, s_pos, pi, crow, self, bx, q0, cnt, max_nbrs);
+  for (uint32_t p = 0; p < m; p += 2) {
+    const bool two = p + 1 < m;
+    const uint32_t pa_off = coff + p, pb_off = two ? pa_off + 1 : pa_off;
+    const float4 pa = s_pos[pa_off], pb = s_pos[pb_off];
+    const float* crow_a = s_cut + __float_as_uint(pa.w) * nt;
+    const float* crow_b = s_cut + __float_as_uint(pb.w) * nt;
+    uint32_t cnt_a = 0, cnt_b = 0;
+    if (two)
+      scan_window<WRAPS, true>(wb, wb.list[0], wb.list[1], s_pos, pa, pb, crow_a, crow_b, pa_off << 4, pb_off << 4, bx,
+                               padded, cnt_a, cnt_b);
+    else
+      scan_window<WRAPS, false>(wb, wb.list[0], wb.list[1], s_pos, pa, pb, crow_a, crow_b, pa_off << 4, pb_off << 4,
+                                bx, padded, cnt_a, cnt_b);
     wave_lds_sync();
-    // group g of the list to its 16 bytes at nbl16[(g nvp + i) 8]; entries of
-    // the last group past the count are never read
-    const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
-    if (lane * 8u < stored) {
-      const u32x4 g = *reinterpret_cast<const u32x4*>(wb.list + lane * 8u);
-      reinterpret_cast<u32x4*>(nbl16)[static_cast<size_t>(lane) * nvp + gstart + p] = g;
+    store_staged(wb.list[0], cnt_a, max_nbrs, nbl16, nvp, gstart + p, lane);
+    if (two) store_staged(wb.list[1], cnt_b, max_nbrs, nbl16, nvp, gstart + p + 1, lane);
+    if (lane == 0) {
+      s_cnt[p] = cnt_a;
+      if (two) s_cnt[p + 1] = cnt_b;
     }
-    if (lane == 0) s_cnt[p] = cnt;
-    wave_lds_sync();  // the group stores have read the list
+    wave_lds_sync();  // the group stores have read the lists
   }
 }
+
+// Whether the 5-cell stencil of cell c wraps around the box on an axis of nc
+// cells.  When it does not and nc >= 7, every candidate lies within 3 cells
+// (< L/2) of the cell, so RUMD's minimum-image correction rounds to zero and
+// the plain FP32 difference is the same number.
+__device__ __forceinline__ bool stencil_wraps(int c, int nc) { return nc < 7 || c < 2 || c + 2 >= nc; }
 
 // Lists of one cell of a staged tile, built by one wave: lanes enumerate the
 // cell's stencil candidates (the 125 cells, or 25 rows of 5 consecutive halo
@@ -935,7 +986,11 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
       cx -= cx >= tg.nc0 ? tg.nc0 : 0;
       cy -= cy >= tg.nc1 ? tg.nc1 : 0;
       cz -= cz >= tg.nc2 ? tg.nc2 : 0;
-      if (h.wraps)
+      // a cell whose stencil stays inside the box on every axis (and cannot
+      // reach L/2: at least 7 cells per axis) needs no image correction
+      const bool cell_wraps =
+          h.wraps && (stencil_wraps(cx, tg.nc0) || stencil_wraps(cy, tg.nc1) || stencil_wraps(cz, tg.nc2));
+      if (cell_wraps)
         build_cell_wave<true>(h, L, tg, s_pos, s_cut, nt, bx, cx, cy, cz, s_wave[wave], s_key, nbl16, nvp, max_nbrs,
                               bytes);
       else
