@@ -81,6 +81,7 @@ extern "C" {
 #define RUMDX_FLAG_ERROR 6     /* nonzero: the last sorted build is unusable, RUMDX_BUILD_ERR_* bits */
 #define RUMDX_FLAG_EXCL_PENDING 7 /* internal: set by a completed build, cleared once its lists' exclusions are applied */
 #define RUMDX_FLAG_MAXTILE 8   /* most particles in one cell tile, last sorted build */
+#define RUMDX_FLAG_PRUNE 9     /* != 0: the next rumdx_nblist_prune rebuilds the inner lists */
 /* Bits of flags[RUMDX_FLAG_ERROR]. */
 #define RUMDX_BUILD_ERR_HALO 1u     /* a tile halo held more than 65535 particles */
 #define RUMDX_BUILD_ERR_MOVED 2u    /* rebuild_sorted: a particle moved more than one cell since the last build */
@@ -162,6 +163,20 @@ typedef struct rumdx_nblist {
    * available with exclusions (the reference switches to N2). */
   const uint32_t* excl_start_dev; /* n + 1 */
   const uint32_t* excl_ids_dev;
+  /* Inner lists of the sorted method (NULL = none; no counterpart in the
+   * reference).  The tile list holds every pair within rc_ij + skin at the
+   * last build; the inner list of slot i holds the entries of that list whose
+   * pair was within rc_ij + inner_skin at the last prune, in the same order
+   * and format (nbl16_in_dev like nbl16_dev, num_in_dev like num_nbrs_dev).
+   * It stays valid while every particle moves less than inner_skin/2 since
+   * the prune (the skin check raises flags[RUMDX_FLAG_PRUNE] at 0.49
+   * inner_skin), and the force kernels read it instead of the tile list, so
+   * pairs in the skin shell are not evaluated on every step. */
+  uint16_t* nbl16_in_dev;   /* like nbl16_dev */
+  uint32_t* num_in_dev;     /* nvp */
+  float* prune_r_dev;       /* nvp x 4: positions at the last prune */
+  const float* cut_in2_dev; /* ntypes^2: (rc_ij + inner_skin)^2 in FP32 */
+  float inner_skin;
 } rumdx_nblist;
 
 /* Cells per tile edge and the largest LDS halo a launch may request. */
@@ -216,7 +231,9 @@ uint32_t rumdx_halo_cap_estimate(const int32_t nc[3], uint32_t n);
 /* ---------------------------------------------------------------- neighbour list
  * NeighborList::CheckRebuildRequired / displacement_since_nb_build
  * (src/NeighborList.cu:569-581, 970-997): sets flags[REBUILD] if any particle
- * moved more than skin/2 (FP32 minimum image, strict >). */
+ * moved more than skin/2 (FP32 minimum image, strict >).  With inner lists it
+ * also sets flags[PRUNE] if any particle moved more than 0.49 inner_skin since
+ * the last prune. */
 int rumdx_nblist_check(const rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
 
 /* NeighborList::ResetNeighborList (src/NeighborList.cu:504-509): marks the list
@@ -250,6 +267,18 @@ int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, const float 
  * range access.  The build of NeighborList::UpdateNBlist for skin-triggered
  * rebuilds (src/NeighborList.cu:583-690, 717-781); no separate counterpart. */
 int rumdx_nblist_rebuild_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
+
+/* Prunes the tile lists into the inner lists (nb->nbl16_in_dev): for every
+ * particle of a tile whose halo the build staged, the entries whose pair is
+ * within rc_ij + inner_skin at the current positions, in list order (FP32
+ * distance; the displacement is exact in the tile's frame or RUMD's FP32
+ * minimum image), and records the positions in prune_r_dev.  Other tiles'
+ * particles and overflowed lists keep the tile list's count (the force
+ * kernels rescan them).  Runs on the device only when flags[RUMDX_FLAG_PRUNE]
+ * is set -- by every sorted build and by the skin check -- and clears it.
+ * Call after the build and before the force call.  No counterpart in the
+ * reference. */
+int rumdx_nblist_prune(const rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
 
 /* Writes the current list in RUMD's format -- u32 neighbour slots, column
  * major, out_dev[k*nvp + i] for k < num_nbrs[i] -- whatever the internal

@@ -420,8 +420,13 @@ int launch_pair(const PP& pp, const char* entry, const rumdx_particles* p, const
   if (nb != nullptr && nb->nbl16_dev != nullptr) {
     const CellFallback fb{nb->cell_rank_dev, nb->cell_start_dev, nb->slot_cell_dev, nb->nc[0], nb->nc[1], nb->nc[2],
                           Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev}};
+    // the pruned inner lists when the list has them (a subset in the same
+    // order and format: the pairs beyond them are outside rc)
+    const bool inner = nb->nbl16_in_dev != nullptr && nb->num_in_dev != nullptr;
+    const uint16_t* lists = inner ? nb->nbl16_in_dev : nb->nbl16_dev;
+    const uint32_t* counts = inner ? nb->num_in_dev : nb->num_nbrs_dev;
     hipLaunchKernelGGL((k_pair_tiles<PP, STR, INIT>), dim3(nb->ntiles), dim3(kBlock),
-                       tab_bytes + sizeof(float4) * nb->halo_cap, st, pp, r, f, w, sts, nb->nbl16_dev, nb->num_nbrs_dev,
+                       tab_bytes + sizeof(float4) * nb->halo_cap, st, pp, r, f, w, sts, lists, counts,
                        nb->lane_order_dev, params, nt, bx, reinterpret_cast<const uint4*>(nb->tile_info_dev),
                        nb->tile_halo_dev, nb->halo_cap, fb, p->nvp, nb->max_nbrs, lgp);
   } else if (nb != nullptr) {

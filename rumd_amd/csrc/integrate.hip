@@ -48,6 +48,8 @@ struct NvtArgs {
   const float4* last_r;  // skin check fused when non-null
   uint32_t* nb_flags;
   float skin2_over_4;
+  const float4* prune_r;  // inner-list check fused when non-null
+  float inner2;
   float dt;
   float omega2;
   uint32_t on_mask;
@@ -76,6 +78,8 @@ __global__ __launch_bounds__(kBlock) void k_nvt(NvtArgs a, Box bx) {
     if (HILO) a.rlo[i] = lo;
     if (SKIN) {
       if (min_image_d2_f32(r, a.last_r[i], bx) > a.skin2_over_4) a.nb_flags[RUMDX_FLAG_REBUILD] = 1u;
+      if (a.prune_r != nullptr && min_image_d2_f32(r, a.prune_r[i], bx) > a.inner2)
+        a.nb_flags[RUMDX_FLAG_PRUNE] = 1u;
     }
   }
   if (THERMO) {
@@ -241,6 +245,8 @@ extern "C" int rumdx_nvt_step(rumdx_particles* p, rumdx_nh_state* nh_dev, float 
     a.last_r = reinterpret_cast<const float4*>(nb->last_r_dev);
     a.nb_flags = nb->flags_dev;
     a.skin2_over_4 = static_cast<float>(0.25 * static_cast<double>(nb->skin) * static_cast<double>(nb->skin));
+    a.prune_r = reinterpret_cast<const float4*>(inner_prune_r(nb));
+    a.inner2 = inner_check2(nb);
   }
   const Box bx = make_box(box);
   const hipStream_t st = as_stream(stream);

@@ -96,17 +96,20 @@ __device__ __forceinline__ void finish_build(uint32_t* flags) {
     flags[RUMDX_FLAG_REBUILD] = 0u;
     flags[RUMDX_FLAG_BUILDS] += 1u;
     flags[RUMDX_FLAG_EXCL_PENDING] = 1u;
+    flags[RUMDX_FLAG_PRUNE] = 1u;
   }
 }
 
 // ------------------------------------------------------------------ skin check
 __global__ __launch_bounds__(kBlock) void k_skin_check(const float4* __restrict__ r, const float4* __restrict__ last_r,
                                                        uint32_t* __restrict__ flags, float skin2_over_4, Box bx,
-                                                       uint32_t n) {
+                                                       uint32_t n, const float4* __restrict__ prune_r, float inner2) {
   const uint32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
-  const float d2 = min_image_d2_f32(r[i], last_r[i], bx);
+  const float4 ri = r[i];
+  const float d2 = min_image_d2_f32(ri, last_r[i], bx);
   if (d2 > skin2_over_4) flags[RUMDX_FLAG_REBUILD] = 1u;
+  if (prune_r != nullptr && min_image_d2_f32(ri, prune_r[i], bx) > inner2) flags[RUMDX_FLAG_PRUNE] = 1u;
 }
 
 __global__ void k_request_rebuild(uint32_t* flags) {
@@ -627,6 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_build_finish(uint32_t* __restrict__ 
     flags[RUMDX_FLAG_REBUILD] = 0u;
     flags[RUMDX_FLAG_BUILDS] += 1u;
     flags[RUMDX_FLAG_EXCL_PENDING] = 1u;
+    flags[RUMDX_FLAG_PRUNE] = 1u;
   }
 }
 
@@ -1220,7 +1224,8 @@ extern "C" int rumdx_nblist_check(const rumdx_particles* p, rumdx_nblist* nb, co
   const float s24 = static_cast<float>(0.25 * static_cast<double>(nb->skin) * static_cast<double>(nb->skin));
   hipLaunchKernelGGL(k_skin_check, dim3(blocks_for(p->n)), dim3(kBlock), 0, as_stream(stream),
                      reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<const float4*>(nb->last_r_dev),
-                     nb->flags_dev, s24, make_box(box), p->n);
+                     nb->flags_dev, s24, make_box(box), p->n, reinterpret_cast<const float4*>(inner_prune_r(nb)),
+                     inner_check2(nb));
   return check_launch("rumdx_nblist_check");
 }
 
@@ -1244,6 +1249,151 @@ extern "C" int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb,
                      nb->nbl_dev, nb->num_nbrs_dev, nb->flags_dev, nb->cut_skin2_dev, nb->ntypes, make_box(box), p->n,
                      p->nvp, nb->max_nbrs, Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev});
   return check_launch("rumdx_nblist_build_n2");
+}
+
+// ------------------------------------------------------------------ inner lists
+// Prune of the tile lists (rumdx_nblist_prune).  Workgroups loop over the
+// tiles (a capped grid, so an idle launch costs little); each stages its
+// tile's halo in LDS as the force kernel does and gives one lane per
+// particle, in the build's lane order.  A lane walks its particle's list,
+// keeps the entries within rc + inner_skin and packs them into 16-byte
+// groups through a 4-word shift register (one funnel shift per word).
+namespace rumdx {
+namespace {
+
+constexpr uint32_t kPruneGrid = 1024;
+
+// Appends entry e (16 bits) to the group being packed in w: every entry moves
+// down one slot and e enters slot 7, so after eight appends slot k holds the
+// k-th entry.
+__device__ __forceinline__ void shift_in(u32x4& w, uint32_t e) {
+  w.x = __builtin_amdgcn_alignbit(w.y, w.x, 16);
+  w.y = __builtin_amdgcn_alignbit(w.z, w.y, 16);
+  w.z = __builtin_amdgcn_alignbit(w.w, w.z, 16);
+  w.w = __builtin_amdgcn_alignbit(e, w.w, 16);
+}
+
+template <bool FAST>
+__device__ __forceinline__ uint32_t prune_particle(uint32_t i, float4 ri, uint32_t nn, const float* crow,
+                                                   const float4* s_pos, const Box& bx,
+                                                   const uint16_t* __restrict__ nbl16, uint16_t* __restrict__ nbl16_in,
+                                                   uint32_t nvp) {
+  const u32x4* grp = reinterpret_cast<const u32x4*>(nbl16) + i;
+  u32x4* out = reinterpret_cast<u32x4*>(nbl16_in) + i;
+  u32x4 w = {0u, 0u, 0u, 0u};
+  uint32_t kept = 0;
+  const uint32_t ng = (nn + 7u) >> 3;
+  for (uint32_t g = 0; g < ng; ++g) {
+    const u32x4 q = __builtin_nontemporal_load(grp + static_cast<size_t>(g) * nvp);
+    const uint32_t ne = (nn - 8u * g) < 8u ? (nn - 8u * g) : 8u;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      if (static_cast<uint32_t>(e) < ne) {
+        const uint32_t ent = group_entry(q, e);
+        const float4 rj = halo_pos(s_pos, ent);
+        const float d2 = FAST ? plain_d2_f32(ri, rj) : min_image_d2_f32(ri, rj, bx);
+        if (d2 < crow[__float_as_uint(rj.w)]) {
+          shift_in(w, ent);
+          ++kept;
+          if ((kept & 7u) == 0u) out[static_cast<size_t>((kept >> 3) - 1u) * nvp] = w;
+        }
+      }
+    }
+  }
+  if (kept & 7u) {  // the partial last group: move its entries down to slots 0..
+    for (uint32_t k = kept & 7u; k < 8u; ++k) shift_in(w, 0u);
+    out[static_cast<size_t>(kept >> 3) * nvp] = w;
+  }
+  return kept;
+}
+
+__global__ __launch_bounds__(kBlock) void k_prune_tiles(const uint32_t* __restrict__ flags,
+                                                        const float4* __restrict__ r, float4* __restrict__ prune_r,
+                                                        const uint16_t* __restrict__ nbl16,
+                                                        const uint32_t* __restrict__ num_nbrs,
+                                                        const uint16_t* __restrict__ lane_order,
+                                                        uint16_t* __restrict__ nbl16_in, uint32_t* __restrict__ num_in,
+                                                        const float* __restrict__ cut_in2, uint32_t nt, Box bx,
+                                                        const uint4* __restrict__ tile_info,
+                                                        const uint32_t* __restrict__ halo_slots, uint32_t halo_cap,
+                                                        uint32_t ntiles, uint32_t nvp, uint32_t max_nbrs) {
+  if (__hip_atomic_load(flags + RUMDX_FLAG_PRUNE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  extern __shared__ float4 s_dyn[];
+  float4* s_pos = s_dyn;
+  float* s_cut = reinterpret_cast<float*>(s_dyn + halo_cap);
+  for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) s_cut[k] = cut_in2[k];
+  for (uint32_t t = xcd_block(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
+    const uint4 info = tile_info[t];
+    const bool staged = (info.y >> 16) == halo_cap && info.x <= halo_cap;
+    const bool fast = (info.y & kTileFast) != 0u;
+    const uint32_t shifts = fast ? info.y : 0u;
+    __syncthreads();  // the previous tile's halo has been read (and s_cut written)
+    if (staged) {
+      const uint32_t* hs = halo_slots + static_cast<size_t>(t) * halo_cap;
+      for (uint32_t q = threadIdx.x; q < info.x; q += kBlock) {
+        float4 pq = r[hs[q]];
+        pq.x = shift_coord(pq.x, shifts, 0, bx.L[0]);
+        pq.y = shift_coord(pq.y, shifts, 1, bx.L[1]);
+        pq.z = shift_coord(pq.z, shifts, 2, bx.L[2]);
+        s_pos[q] = pq;
+      }
+    }
+    __syncthreads();
+    const uint32_t np = info.w - info.z;
+    for (uint32_t k = threadIdx.x; k < np; k += kBlock) {
+      const uint32_t i = info.z + lane_order[info.z + k];
+      float4 ri = r[i];
+      prune_r[i] = ri;
+      const uint32_t nn = num_nbrs[i];
+      if (!staged || nn >= max_nbrs) {  // the force kernels rescan these particles
+        num_in[i] = nn;
+        continue;
+      }
+      const float* crow = s_cut + __float_as_uint(ri.w) * nt;
+      if (fast) {
+        ri.x = shift_coord(ri.x, shifts, 0, bx.L[0]);
+        ri.y = shift_coord(ri.y, shifts, 1, bx.L[1]);
+        ri.z = shift_coord(ri.z, shifts, 2, bx.L[2]);
+        num_in[i] = prune_particle<true>(i, ri, nn, crow, s_pos, bx, nbl16, nbl16_in, nvp);
+      } else {
+        num_in[i] = prune_particle<false>(i, ri, nn, crow, s_pos, bx, nbl16, nbl16_in, nvp);
+      }
+    }
+  }
+}
+
+__global__ void k_prune_done(uint32_t* flags) {
+  if (threadIdx.x == 0) flags[RUMDX_FLAG_PRUNE] = 0u;
+}
+
+}  // namespace
+}  // namespace rumdx
+
+extern "C" int rumdx_nblist_prune(const rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream) {
+  using namespace rumdx;
+  const char* entry = "rumdx_nblist_prune";
+  if (p == nullptr || nb == nullptr || box == nullptr) return fail(RUMDX_E_INVALID, std::string(entry) + ": null argument");
+  if (nb->nbl16_dev == nullptr || nb->tile_info_dev == nullptr || nb->tile_halo_dev == nullptr ||
+      nb->lane_order_dev == nullptr || nb->num_nbrs_dev == nullptr || nb->flags_dev == nullptr || nb->ntiles == 0)
+    return fail(RUMDX_E_INVALID, std::string(entry) + ": the sorted (tile) lists are not allocated");
+  if (nb->nbl16_in_dev == nullptr || nb->num_in_dev == nullptr || nb->prune_r_dev == nullptr ||
+      nb->cut_in2_dev == nullptr)
+    return fail(RUMDX_E_INVALID, std::string(entry) + ": inner lists not allocated");
+  if (!(nb->inner_skin > 0.f)) return fail(RUMDX_E_INVALID, std::string(entry) + ": inner_skin must be positive");
+  if (nb->halo_cap > RUMDX_HALO_CAP_MAX || nb->ntypes == 0 || nb->ntypes > RUMDX_MAX_TYPES)
+    return fail(RUMDX_E_INVALID, std::string(entry) + ": halo capacity or type count out of range");
+  if (p->r_dev == nullptr) return fail(RUMDX_E_INVALID, std::string(entry) + ": particle arrays not allocated");
+  if (p->n == 0) return RUMDX_OK;
+  const hipStream_t st = as_stream(stream);
+  const uint32_t grid = nb->ntiles < kPruneGrid ? nb->ntiles : kPruneGrid;
+  const size_t lds = sizeof(float4) * nb->halo_cap + sizeof(float) * nb->ntypes * nb->ntypes;
+  hipLaunchKernelGGL(k_prune_tiles, dim3(grid), dim3(kBlock), lds, st, nb->flags_dev,
+                     reinterpret_cast<const float4*>(p->r_dev), reinterpret_cast<float4*>(nb->prune_r_dev),
+                     nb->nbl16_dev, nb->num_nbrs_dev, nb->lane_order_dev, nb->nbl16_in_dev, nb->num_in_dev,
+                     nb->cut_in2_dev, nb->ntypes, make_box(box), reinterpret_cast<const uint4*>(nb->tile_info_dev),
+                     nb->tile_halo_dev, nb->halo_cap, nb->ntiles, p->nvp, nb->max_nbrs);
+  hipLaunchKernelGGL(k_prune_done, dim3(1), dim3(64), 0, st, nb->flags_dev);
+  return check_launch(entry);
 }
 
 namespace rumdx {

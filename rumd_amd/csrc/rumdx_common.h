@@ -196,4 +196,16 @@ struct Workspace {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Inner lists (sorted method): the positions of the last prune when the list
+// has inner lists (else null), and the squared displacement since that prune
+// above which the skin check asks for a new one: (0.49 inner_skin)^2, a 2 %
+// margin under the inner_skin/2 that keeps every pair within rc listed.
+inline const float* inner_prune_r(const rumdx_nblist* nb) {
+  return (nb->nbl16_in_dev != nullptr && nb->num_in_dev != nullptr) ? nb->prune_r_dev : nullptr;
+}
+inline float inner_check2(const rumdx_nblist* nb) {
+  const double h = 0.49 * static_cast<double>(nb->inner_skin);
+  return static_cast<float>(h * h);
+}
+
 }  // namespace rumdx

@@ -136,6 +136,10 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetNbHaloCapacity", [](PairPotential& p, unsigned cap) { p.GetNeighborList().SetHaloCapacity(cap); },
            py::arg("cap"))
       .def("GetNbHaloCapacity", [](PairPotential& p) { return p.GetNeighborList().GetHaloCapacity(); })
+      .def("SetNbListInnerSkin", [](PairPotential& p, float s) { p.GetNeighborList().SetInnerSkin(s); },
+           py::arg("inner_skin"))
+      .def("GetNbListInnerSkin", [](PairPotential& p) { return p.GetNeighborList().GetInnerSkin(); })
+      .def("HasNbInnerLists", [](PairPotential& p) { return p.GetNeighborList().HasInnerLists(); })
       .def("SetNbForceFullSort", [](PairPotential& p, bool on) {
         p.GetNeighborList().SetForceFullSort(on);
         if (p.GetSample()) p.GetSample()->InvalidateGraphs();

@@ -57,6 +57,20 @@ void NeighborList::SetSkin(float skin) {
   if (sample_) sample_->InvalidateGraphs();
 }
 
+// Inner-list skin of the sorted method (RUMDX inner lists): zero (the
+// default) switches the inner lists off, a negative value selects skin/3.
+void NeighborList::SetInnerSkin(float inner) {
+  if (!(inner == inner)) throw RUMD_Error("NeighborList", "SetInnerSkin", "inner skin is NaN");
+  inner_skin_ = inner;
+  allocated_ = false;
+  if (sample_) sample_->InvalidateGraphs();
+}
+
+float NeighborList::GetInnerSkin() const {
+  if (inner_skin_ < 0.f) return skin_ / 3.f;
+  return inner_skin_;
+}
+
 void NeighborList::SetNB_Method(const std::string& m) {
   if (m == "default") method_ = DEFAULT;
   else if (m == "none") method_ = NONE;
@@ -258,7 +272,35 @@ void NeighborList::Allocate() {
     desc_.tile_origin_dev = tile_origin_.get();
     desc_.tile_info_dev = tile_info_.get();
     SizeTileHalo();
+    // inner lists: the force kernels read the entries within rc + inner skin
+    const float inner = GetInnerSkin();
+    if (inner > 0.f && inner < skin_) {
+      nbl16_in_.resize(nbl16_.size());
+      num_in_.resize(nvp);
+      prune_r_.resize(4ull * nvp);
+      std::vector<float> ci2(ntypes_ * ntypes_);
+      for (size_t k = 0; k < ci2.size(); ++k) {
+        const float c = cutoffs_[k] + inner;
+        ci2[k] = c * c;
+      }
+      cut_in2_.resize(ci2.size());
+      cut_in2_.upload(ci2.data(), ci2.size(), st);
+      desc_.nbl16_in_dev = nbl16_in_.get();
+      desc_.num_in_dev = num_in_.get();
+      desc_.prune_r_dev = prune_r_.get();
+      desc_.cut_in2_dev = cut_in2_.get();
+      desc_.inner_skin = inner;
+    } else {
+      nbl16_in_.release();
+      num_in_.release();
+      prune_r_.release();
+      cut_in2_.release();
+    }
   } else {
+    nbl16_in_.release();
+    num_in_.release();
+    prune_r_.release();
+    cut_in2_.release();
     cell_rank_.release();
     cell_start_.release();
     cell_count_.release();
@@ -302,6 +344,7 @@ void NeighborList::UpdateNBlist() {
     sort_valid_ = true;
     sample_->BumpSlotEpoch();  // the build may have moved particles between slots
     if (!resort && !force_full_sort_) Settle();
+    Prune();
   }
   else
     rumdx_check(rumdx_nblist_build_n2(p, &desc_, box, st), "NeighborList", "UpdateNBlist");
@@ -313,6 +356,15 @@ void NeighborList::UpdateNBlist() {
 // settled at the next synchronisation) and any rebuild they ask for runs now.
 // Eager steps and graph blocks therefore see the same list history, whenever
 // the host next synchronises.
+// The inner lists' prune (rumdx_nblist_prune): on the device it runs only
+// after a build or when a particle moved 0.49 inner skin since the last one.
+void NeighborList::Prune() {
+  if (desc_.nbl16_in_dev == nullptr) return;
+  rumdx_check(rumdx_nblist_prune(sample_->GetParticleData().Descriptor(), &desc_,
+                                 sample_->GetSimulationBox()->GetBox8(), sample_->GetStream()),
+              "NeighborList", "UpdateNBlist");
+}
+
 void NeighborList::Settle() {
   const hipStream_t st = sample_->GetStream();
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;

@@ -157,6 +157,13 @@ class NeighborList {
   void SetCutoffs(const PairPotential* pot);
   void SetSkin(float skin);
   float GetSkin() const { return skin_; }
+  // Inner lists of the sorted method (no counterpart in the reference): the
+  // force kernels read the list entries within rc + inner skin, pruned from
+  // the list whenever a particle has moved 0.49 inner skin.  Zero (the
+  // default): off; negative: skin/3.  Takes effect at the next allocation.
+  void SetInnerSkin(float inner);
+  float GetInnerSkin() const;
+  bool HasInnerLists() const { return desc_.nbl16_in_dev != nullptr; }
   float GetMaxCutoff() const { return max_cutoff_; }
   void SetNB_Method(const std::string& method);
   std::string GetNB_Method() const;
@@ -215,6 +222,7 @@ class NeighborList {
  private:
   void Allocate();
   void Settle();
+  void Prune();
   Sample* sample_ = nullptr;
   Method method_ = DEFAULT;
   float skin_ = 0.5f;  // reference default (src/NeighborList.cu:50)
@@ -238,6 +246,10 @@ class NeighborList {
   bool force_full_sort_ = false;
   uint32_t last_max_tile_ = 0;  // most particles in one tile, last synchronised build
   DeviceArray<float> last_r_, cut_skin2_;
+  float inner_skin_ = 0.f;  // 0: off, < 0: skin/3
+  DeviceArray<uint16_t> nbl16_in_;
+  DeviceArray<uint32_t> num_in_;
+  DeviceArray<float> prune_r_, cut_in2_;
   DeviceArray<char> sort_scratch_;
   DeviceArray<uint16_t> lane_order_;
   DeviceArray<uint32_t> tile_halo_;  // ntiles x halo_cap

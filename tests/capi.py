@@ -61,6 +61,11 @@ class NbList(C.Structure):
         ("lanes_per_particle", C.c_uint32),
         ("excl_start_dev", C.c_void_p),
         ("excl_ids_dev", C.c_void_p),
+        ("nbl16_in_dev", C.c_void_p),
+        ("num_in_dev", C.c_void_p),
+        ("prune_r_dev", C.c_void_p),
+        ("cut_in2_dev", C.c_void_p),
+        ("inner_skin", C.c_float),
     ]
 
 
@@ -102,6 +107,7 @@ def lib() -> C.CDLL:
     L.rumdx_nblist_build_n2.argtypes = [PP, NP, f32a, vp]
     L.rumdx_nblist_build_sorted.argtypes = [PP, NP, f32a, vp]
     L.rumdx_nblist_rebuild_sorted.argtypes = [PP, NP, f32a, vp]
+    L.rumdx_nblist_prune.argtypes = [PP, NP, f32a, vp]
     L.rumdx_pair_lj.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
     L.rumdx_pair_lj_fp64.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
     L.rumdx_pair_form.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, i32, vp]
@@ -303,16 +309,32 @@ class GpuSystem:
         self.nb = nb
         check(lib().rumdx_nblist_request_rebuild(C.byref(self.nb), None))
 
+    def enable_inner(self, inner_skin):
+        """Inner lists (sorted method): the entries within rc + inner_skin,
+        pruned after builds and when the skin check asks."""
+        assert self.method == "sorted"
+        rc = self.params[:, 0]
+        ci2 = ((rc + np.float32(inner_skin)) * (rc + np.float32(inner_skin))).astype(np.float32)
+        self.d_cut_in2 = DeviceBuffer.from_array(ci2)
+        self.nbl16_in = DeviceBuffer(self.nbl16.nbytes)
+        self.num_in = DeviceBuffer(4 * self.nvp)
+        self.prune_r = DeviceBuffer(16 * self.nvp)
+        nb = self.nb
+        nb.nbl16_in_dev, nb.num_in_dev = self.nbl16_in.value, self.num_in.value
+        nb.prune_r_dev, nb.cut_in2_dev, nb.inner_skin = self.prune_r.value, self.d_cut_in2.value, inner_skin
+
     # ------------------------------------------------------------ operations
     def update_nblist(self, resort=False):
         """The skin check and the list build; with resort the sorted build is
         the tile re-sort (rumdx_nblist_rebuild_sorted), which needs a previous
-        sorted build of these particles."""
+        sorted build of these particles.  With inner lists, then the prune."""
         L = lib()
         check(L.rumdx_nblist_check(C.byref(self.p), C.byref(self.nb), self.box8, None))
         if self.method == "sorted":
             build = L.rumdx_nblist_rebuild_sorted if resort else L.rumdx_nblist_build_sorted
             check(build(C.byref(self.p), C.byref(self.nb), self.box8, None))
+            if self.nb.nbl16_in_dev:
+                check(L.rumdx_nblist_prune(C.byref(self.p), C.byref(self.nb), self.box8, None))
         else:
             check(L.rumdx_nblist_build_n2(C.byref(self.p), C.byref(self.nb), self.box8, None))
 
@@ -356,9 +378,17 @@ class GpuSystem:
     def flags_host(self):
         return self.flags.download(np.uint32, 16)
 
-    def neighbor_lists(self):
-        """(counts, slots) in RUMD's column-major u32 format, via rumdx_nblist_export."""
-        check(lib().rumdx_nblist_export(C.byref(self.p), C.byref(self.nb), self.export_buf.value, None))
-        cnt = self.num_nbrs.download(np.uint32, self.nvp)[: self.n]
+    def neighbor_lists(self, inner=False):
+        """(counts, slots) in RUMD's column-major u32 format, via
+        rumdx_nblist_export; with inner, the inner lists (exported through a
+        copy of the descriptor that names them as the tile lists)."""
+        nb = self.nb
+        counts = self.num_nbrs
+        if inner:
+            nb = NbList.from_buffer_copy(bytes(self.nb))
+            nb.nbl16_dev, nb.num_nbrs_dev = self.nbl16_in.value, self.num_in.value
+            counts = self.num_in
+        check(lib().rumdx_nblist_export(C.byref(self.p), C.byref(nb), self.export_buf.value, None))
+        cnt = counts.download(np.uint32, self.nvp)[: self.n]
         nbl = self.export_buf.download(np.uint32, (self.max_nbrs, self.nvp))
         return cnt, nbl
