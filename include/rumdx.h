@@ -177,6 +177,10 @@ typedef struct rumdx_nblist {
   float* prune_r_dev;       /* nvp x 4: positions at the last prune */
   const float* cut_in2_dev; /* ntypes^2: (rc_ij + inner_skin)^2 in FP32 */
   float inner_skin;
+  /* Occupancy the 12-6 tile force kernel is compiled for, in waves per SIMD:
+   * 0 = the default (5), or 4 or 6 (compiled variants of the stepping
+   * kernel, for the autotuner).  Results are bit-identical. */
+  uint32_t tile_waves_per_eu;
 } rumdx_nblist;
 
 /* Cells per tile edge and the largest LDS halo a launch may request. */

@@ -6,6 +6,10 @@ Technical parameters, with their ranges:
 
     tp         [1, 2, 4, 8]  lanes sharing one particle's neighbour list in the
                               force kernels (RUMD's threads per particle)
+    waves      [4, 5, 6]     waves per SIMD the cell-tile force kernel is
+                              compiled for (sorted lists; the tile mapping's
+                              occupancy: more waves hide more latency, fewer
+                              leave more registers per lane)
     skin       non-negative float; the neighbour-list skin
     NB_method  ["none", "n2", "sort"]
     pb         kept for RUMD scripts; the kernels here use fixed 256-lane
@@ -41,6 +45,7 @@ from . import rumd
 
 _METHODS = ("none", "n2", "sort")
 _TP = (1, 2, 4, 8)
+_WAVES = (4, 5, 6)
 
 
 class Autotune:
@@ -56,7 +61,7 @@ class Autotune:
         self.size_stop_n2 = int(kwargs.pop("size_stop_n2", 20000))
         self.verbose = bool(kwargs.pop("verbose", False))
         self.fixed = {}
-        for key in ("pb", "tp", "skin", "NB_method"):
+        for key in ("pb", "tp", "skin", "NB_method", "waves"):
             if key in kwargs:
                 self.fixed[key] = kwargs.pop(key)
         if kwargs:
@@ -104,7 +109,8 @@ class Autotune:
     def GetCurrentTechnicalParameters(self, sim) -> dict:
         pot = sim.potentialList[0]
         return {"pb": sim.sample.GetParticlesPerBlock(), "tp": sim.sample.GetThreadsPerParticle(),
-                "skin": pot.GetNbListSkin(), "NB_method": pot.GetNB_Method()}
+                "skin": pot.GetNbListSkin(), "NB_method": pot.GetNB_Method(),
+                "waves": pot.GetNbTileWavesPerEU() or 5}
 
     def CompareSavedParameters(self, user: dict):
         """The saved technical parameters of an entry whose user parameters
@@ -120,6 +126,7 @@ class Autotune:
             sim.SetPB_TP(int(tech["pb"]), int(tech["tp"]))
         pot.SetNB_Method(tech["NB_method"])
         pot.SetNbListSkin(float(tech["skin"]))
+        pot.SetNbTileWavesPerEU(int(tech.get("waves", 5)))
 
     def WriteParametersToFile(self, user: dict, tech: dict) -> None:
         entries = [e for e in self._read_file() if not _same_user(e.get("user", {}), user)]
@@ -221,6 +228,9 @@ class Autotune:
             if "tp" not in self.fixed:
                 scored = [(trial(dict(best, tp=tp)), dict(best, tp=tp)) for tp in tps]
                 best = min(scored, key=lambda x: x[0])[1]
+        if "waves" not in self.fixed and best["NB_method"] == "sort":
+            scored = [(trial(dict(best, waves=wv)), dict(best, waves=wv)) for wv in _WAVES]
+            best = min(scored, key=lambda x: x[0])[1]
         self._restore(sim, saved)
         return best
 

@@ -52,10 +52,11 @@ __device__ __forceinline__ uint32_t rep_cell(float4 p, const Box& bx, int nc) {
 // Fixed-point positions for the pair forces: per axis round(x/L 2^32) as a
 // 32-bit integer (x in [-L/2, L/2) maps into int32; the conversion wraps
 // modulo 2^32), and the type bits.  The difference of two such integers,
-// taken modulo 2^32, is the minimum-image displacement in units of L 2^-32:
-// converted to float and scaled by the exact power-of-two factor it has two
-// roundings (~1e-7 relative), where the FP64 route costs three times the
-// VALU work.  The quantisation, L 2^-33, is far below the positions' float ulp.
+// taken modulo 2^32, is the minimum-image displacement in units of L 2^-32,
+// exactly (no float rounding of a coordinate near L/2, unlike the FP32 minimum
+// image); scaled in FP64 and rounded once it costs four VALU per axis, where
+// the FP64 coordinate route costs about three times as much.  The
+// quantisation, L 2^-33, is far below the positions' float ulp.
 __device__ __forceinline__ int32_t fixed_axis(float x, double scale) {
   return static_cast<int32_t>(static_cast<uint32_t>(static_cast<long long>(rint(static_cast<double>(x) * scale))));
 }
@@ -383,26 +384,35 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
 
 // ------------------------------------------------------------------ forces
 // FP32 pair arithmetic with the minimum-image displacement taken exactly on
-// the fixed-point positions (to_fixed above).
+// the fixed-point positions (to_fixed above) and rounded once to FP32.
 struct Acc {
   float fx = 0.f, fy = 0.f, fz = 0.f, u = 0.f, vir = 0.f;
 };
 
+// One axis of the displacement: the wrapping difference of the fixed-point
+// coordinates (the minimum image, exactly) scaled by L 2^-32 in FP64 (exact:
+// a 32-bit integer times a float over a power of two) and rounded once to FP32,
+// so the displacement is the correctly rounded one.
+__device__ __forceinline__ float fixed_delta(int32_t a, int32_t b, double unit) {
+  return static_cast<float>(
+      static_cast<double>(static_cast<int32_t>(static_cast<uint32_t>(a) - static_cast<uint32_t>(b))) * unit);
+}
+
 template <int CUT>
-__device__ __forceinline__ void pair_b(int4 fi, int4 fj, float unit, const float4* prow, Acc& a) {
-  const float dx = static_cast<float>(static_cast<int32_t>(static_cast<uint32_t>(fi.x) - static_cast<uint32_t>(fj.x))) * unit;
-  const float dy = static_cast<float>(static_cast<int32_t>(static_cast<uint32_t>(fi.y) - static_cast<uint32_t>(fj.y))) * unit;
-  const float dz = static_cast<float>(static_cast<int32_t>(static_cast<uint32_t>(fi.z) - static_cast<uint32_t>(fj.z))) * unit;
+__device__ __forceinline__ void pair_b(int4 fi, int4 fj, double unit, const float4* prow, Acc& a) {
+  const float dx = fixed_delta(fi.x, fj.x, unit);
+  const float dy = fixed_delta(fi.y, fj.y, unit);
+  const float dz = fixed_delta(fi.z, fj.z, unit);
   const float d2 = dx * dx + dy * dy + dz * dz;
   const float4 pa = prow[2 * fj.w];
   const float4 pb = prow[2 * fj.w + 1];
-  float rinv = 0.f, inv2;
-  if (CUT == RUMDX_CUTOFF_SF) {
-    rinv = __builtin_amdgcn_rsqf(d2);
-    inv2 = rinv * rinv;
-  } else {
-    inv2 = __builtin_amdgcn_rcpf(d2);
-  }
+  // 1/r^2 from its own reciprocal (1 ulp) rather than the square of 1/r
+  // (2.5 ulp): the r^-14 force term multiplies that error by seven.  The
+  // batch kernel waits on gathers more than on VALU, so the second
+  // transcendental costs little here.
+  float rinv = 0.f;
+  const float inv2 = __builtin_amdgcn_rcpf(d2);
+  if (CUT == RUMDX_CUTOFF_SF) rinv = __builtin_amdgcn_rsqf(d2);
   const float inv6 = inv2 * inv2 * inv2;
   float s = (inv6 * inv2) * fmaf(pb.x, inv6, pb.y);
   float u;
@@ -458,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t*
   if (live) {
     const int4 fi = a.fixed[i];
     const float4* prow = s_tab + 2 * fi.w * nt;
-    const float unit = rp.L * 2.3283064365386963e-10f;  // L 2^-32, exact
+    const double unit = static_cast<double>(rp.L) * 2.3283064365386963e-10;  // L 2^-32, exact
     const uint32_t nn = num_nbrs[i];
     if (nn >= max_nbrs) {
       if (part == 0) {

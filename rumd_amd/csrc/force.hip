@@ -21,6 +21,7 @@ template <int CUT>
 struct LjPair {
   static constexpr uint32_t kRow = 2;
   static constexpr int kWavesPerEU = RUMDX_TILE_WAVES_PER_EU;
+  static constexpr bool kTunedWaves = true;
 
   __device__ __forceinline__ void load_table(float4* s_tab, const float* __restrict__ params, uint32_t nt) const {
     for (uint32_t k = threadIdx.x; k < nt * nt; k += blockDim.x) {

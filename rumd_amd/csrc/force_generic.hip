@@ -20,6 +20,7 @@ struct FormPair {
   int form;
   static constexpr uint32_t kRow = 3;
   static constexpr int kWavesPerEU = 3;
+  static constexpr bool kTunedWaves = false;
 
   __device__ __forceinline__ void load_table(float4* s_tab, const float* __restrict__ params, uint32_t nt) const {
     for (uint32_t k = threadIdx.x; k < nt * nt; k += blockDim.x) {

@@ -17,7 +17,8 @@
 // reproducible bit for bit.
 //
 // A pair policy PP supplies the potential: kRow float4 per type pair in LDS,
-// kWavesPerEU (the tile kernel's occupancy target),
+// kWavesPerEU (the tile kernel's occupancy target; kTunedWaves: the
+// stepping variant also exists for 4 and 6 waves, rumdx_nblist.tile_waves_per_eu),
 // load_table (RUMD's 12-float rows -> that LDS table) and core<STR> (the
 // masked pair term for a displacement, accumulated into Acc).  The 12-6
 // policy lives in force.hip, the other potential forms in force_generic.hip.
@@ -371,8 +372,8 @@ __device__ __forceinline__ void tile_particles(const PP& pp, uint32_t slot0, uin
 // offsets, so neighbour positions come from LDS.  The kernel uses no static
 // LDS: its dynamic LDS is the halo capacity and the parameter table, which
 // sets how many workgroups share a CU.
-template <class PP, bool STR, bool INIT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PP::kWavesPerEU))) void k_pair_tiles(
+template <class PP, bool STR, bool INIT, int WPE = PP::kWavesPerEU>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pair_tiles(
     PP pp, const float4* __restrict__ r, float4* __restrict__ f, float4* __restrict__ w, float4* __restrict__ sts,
     const uint16_t* __restrict__ nbl16, const uint32_t* __restrict__ num_nbrs,
     const uint16_t* __restrict__ lane_order, const float* __restrict__ params, uint32_t nt, Box bx,
@@ -425,10 +426,25 @@ int launch_pair(const PP& pp, const char* entry, const rumdx_particles* p, const
     const bool inner = nb->nbl16_in_dev != nullptr && nb->num_in_dev != nullptr;
     const uint16_t* lists = inner ? nb->nbl16_in_dev : nb->nbl16_dev;
     const uint32_t* counts = inner ? nb->num_in_dev : nb->num_nbrs_dev;
-    hipLaunchKernelGGL((k_pair_tiles<PP, STR, INIT>), dim3(nb->ntiles), dim3(kBlock),
-                       tab_bytes + sizeof(float4) * nb->halo_cap, st, pp, r, f, w, sts, lists, counts,
-                       nb->lane_order_dev, params, nt, bx, reinterpret_cast<const uint4*>(nb->tile_info_dev),
-                       nb->tile_halo_dev, nb->halo_cap, fb, p->nvp, nb->max_nbrs, lgp);
+    const auto launch = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(nb->ntiles), dim3(kBlock), tab_bytes + sizeof(float4) * nb->halo_cap, st, pp, r,
+                         f, w, sts, lists, counts, nb->lane_order_dev, params, nt, bx,
+                         reinterpret_cast<const uint4*>(nb->tile_info_dev), nb->tile_halo_dev, nb->halo_cap, fb, p->nvp,
+                         nb->max_nbrs, lgp);
+    };
+    // the step's variant (no stresses, initialising) of a policy whose
+    // occupancy target is tunable also comes compiled for 4 and 6 waves/SIMD
+    if constexpr (PP::kTunedWaves && !STR && INIT) {
+      if (nb->tile_waves_per_eu == 4u) {
+        launch(k_pair_tiles<PP, STR, INIT, 4>);
+        return check_launch(entry);
+      }
+      if (nb->tile_waves_per_eu == 6u) {
+        launch(k_pair_tiles<PP, STR, INIT, 6>);
+        return check_launch(entry);
+      }
+    }
+    launch(k_pair_tiles<PP, STR, INIT>);
   } else if (nb != nullptr) {
     CellFallback fb{};
     fb.ex = Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev};

@@ -140,6 +140,9 @@ PYBIND11_MODULE(_rumd, m) {
            py::arg("inner_skin"))
       .def("GetNbListInnerSkin", [](PairPotential& p) { return p.GetNeighborList().GetInnerSkin(); })
       .def("HasNbInnerLists", [](PairPotential& p) { return p.GetNeighborList().HasInnerLists(); })
+      .def("SetNbTileWavesPerEU", [](PairPotential& p, unsigned w) { p.GetNeighborList().SetTileWavesPerEU(w); },
+           py::arg("waves"))
+      .def("GetNbTileWavesPerEU", [](PairPotential& p) { return p.GetNeighborList().GetTileWavesPerEU(); })
       .def("SetNbForceFullSort", [](PairPotential& p, bool on) {
         p.GetNeighborList().SetForceFullSort(on);
         if (p.GetSample()) p.GetSample()->InvalidateGraphs();

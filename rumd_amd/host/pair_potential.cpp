@@ -66,6 +66,15 @@ void NeighborList::SetInnerSkin(float inner) {
   if (sample_) sample_->InvalidateGraphs();
 }
 
+// Waves per SIMD of the 12-6 tile force kernel (0 = its default, or 4, 5, 6).
+void NeighborList::SetTileWavesPerEU(unsigned w) {
+  if (w != 0 && (w < 4 || w > 6))
+    throw RUMD_Error("NeighborList", "SetTileWavesPerEU", "waves per SIMD must be 0 (default), 4, 5 or 6");
+  tile_waves_ = w;
+  desc_.tile_waves_per_eu = w;
+  if (sample_) sample_->InvalidateGraphs();
+}
+
 float NeighborList::GetInnerSkin() const {
   if (inner_skin_ < 0.f) return skin_ / 3.f;
   return inner_skin_;
@@ -226,6 +235,7 @@ void NeighborList::Allocate() {
   // the sample's threads per particle (Sample::SetPB_TP), as a power of two <= 8
   const unsigned tp = sample_->GetThreadsPerParticle();
   desc_.lanes_per_particle = tp >= 8 ? 8u : tp >= 4 ? 4u : tp >= 2 ? 2u : 1u;
+  desc_.tile_waves_per_eu = tile_waves_;
   if (method == SORT) {
     int pair_pots = 0;
     for (Potential* p : sample_->GetPotentials())

@@ -164,6 +164,10 @@ class NeighborList {
   void SetInnerSkin(float inner);
   float GetInnerSkin() const;
   bool HasInnerLists() const { return desc_.nbl16_in_dev != nullptr; }
+  // Occupancy variant of the 12-6 tile force kernel: 0 (default, 5), 4, 5 or 6
+  // waves per SIMD; the autotuner's tile mapping parameter.
+  void SetTileWavesPerEU(unsigned w);
+  unsigned GetTileWavesPerEU() const { return tile_waves_; }
   float GetMaxCutoff() const { return max_cutoff_; }
   void SetNB_Method(const std::string& method);
   std::string GetNB_Method() const;
@@ -247,6 +251,7 @@ class NeighborList {
   uint32_t last_max_tile_ = 0;  // most particles in one tile, last synchronised build
   DeviceArray<float> last_r_, cut_skin2_;
   float inner_skin_ = 0.f;  // 0: off, < 0: skin/3
+  unsigned tile_waves_ = 0;
   DeviceArray<uint16_t> nbl16_in_;
   DeviceArray<uint32_t> num_in_;
   DeviceArray<float> prune_r_, cut_in2_;

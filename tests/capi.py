@@ -66,6 +66,7 @@ class NbList(C.Structure):
         ("prune_r_dev", C.c_void_p),
         ("cut_in2_dev", C.c_void_p),
         ("inner_skin", C.c_float),
+        ("tile_waves_per_eu", C.c_uint32),
     ]
 
 

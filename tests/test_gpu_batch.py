@@ -9,7 +9,7 @@ import pytest
 
 import oracle
 from capi import gpu_available
-from helpers import FORCE_TOL_GUARDED, force_errors, ka_table
+from helpers import FORCE_TOL_GUARDED, force_errors, ka_table, sorted_rows
 from rumd_amd import configs
 
 pytestmark = pytest.mark.gpu
@@ -247,3 +247,49 @@ def test_batch_build_per_lane_path():
         assert np.array_equal(off, o2), r
         for i in range(s.n):
             assert set(ids[off[i]:off[i + 1]].tolist()) == set(i2[o2[i]:o2[i + 1]].tolist()), (r, i)
+
+
+def test_batch_full_replica_set_sampled_against_oracle():
+    """The benchmark's own replica set (512 KA replicas, N ~ U[256, 8192],
+    T ~ U[0.4, 2.0], 2.15 M particles) after 100 NVT steps in graph blocks
+    (per-replica rebuilds of hot and cold replicas): forces, energies and
+    neighbour sets of a sample of replicas -- the smallest, the largest, the
+    hottest, the coldest and every 64th -- against the oracle at their
+    current positions."""
+    systems, temps = configs.ka_replica_set()
+    assert len(systems) == 512
+    b = make_batch(systems, temps)
+    print(f"[case] 512-replica set, {sum(s.n for s in systems)} particles: 100 steps", flush=True)
+    b.RunSteps(100, 10)
+    b.CalcF()
+    ns = np.array([s.n for s in systems])
+    sample = sorted({int(np.argmin(ns)), int(np.argmax(ns)), int(np.argmax(temps)), int(np.argmin(temps)),
+                     *range(0, 512, 64)})
+    builds = np.asarray(b.GetNumberOfBuilds())
+    # hot replicas rebuild more often than cold ones
+    assert builds[int(np.argmax(temps))] > builds[int(np.argmin(temps))]
+    e = b.GetEnergies()
+    cut2 = configs.cut_skin2_table(ka_table(), 0.3)
+    for r in sample:
+        s = systems[r]
+        pos = b.GetPositions(r)
+        pos4 = configs.pack_pos4(pos, s.types)
+        print(f"[case] replica {r}: N={s.n} T={temps[r]:.2f} builds={builds[r]}", flush=True)
+        f = b.GetForcesAndEnergies(r)
+        f_ref, vir_ref = (oracle.forces_brute if s.n <= 2000 else oracle.forces_cells)(pos4, s.box, 2, ka_table(), SF)
+        err = force_errors(f[:, :3], f_ref[:, :3])
+        assert err["guarded_max"] <= FORCE_TOL_GUARDED, (r, err)
+        assert e[r, 0] == pytest.approx(float(f_ref[:, 3].sum()), rel=1e-5, abs=1e-3)
+    # every list stale (RestoreState), rebuilt at the current positions: the
+    # sampled replicas' sets are bit-exact against the FP32 restatement
+    b.SaveState()
+    b.RestoreState()
+    b.CalcF()
+    for r in sample:
+        s = systems[r]
+        pos4 = configs.pack_pos4(b.GetPositions(r), s.types)
+        b8 = configs.box8(s.box)
+        off, ids = b.GetNeighborListCSR(r)
+        o2, i2 = oracle.nblist_cells_f32(pos4, b8, 2, cut2, oracle.cell_grid(b8, 2.5, 0.3))
+        assert np.array_equal(off, o2), r
+        assert np.array_equal(sorted_rows(off, ids), sorted_rows(o2, i2)), r

@@ -562,3 +562,23 @@ def test_tile_resort_reports_a_particle_moved_beyond_a_cell():
     a.update_nblist(resort=True)
     f = a.flags_host()
     assert f[6] & 2, f
+
+
+def test_tile_force_occupancy_variants_are_bit_identical():
+    """The 12-6 tile force kernel compiled for 4, 5 (default) and 6 waves per
+    SIMD (the autotuner's tile mapping parameter) gives the same bits."""
+    sysm = configs.ka_random_lattice(30000, density=1.2, temperature=0.45, seed=83)
+    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
+    g = GpuSystem(pos4, configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses), configs.box8(sysm.box), 2,
+                  ka_table(), method="sorted")
+    out = {}
+    for wpe in (0, 4, 6):
+        print(f"[case] KA N=30000 SF tile waves {wpe}", flush=True)
+        g.nb.tile_waves_per_eu = wpe
+        g.calc_f(SF)
+        out[wpe] = (g.get("f").copy(), g.get("w").copy())
+    for wpe in (4, 6):
+        assert np.array_equal(out[wpe][0], out[0][0]) and np.array_equal(out[wpe][1], out[0][1])
+    f_ref, _ = oracle.forces_cells(pos4, sysm.box, 2, ka_table(), SF)
+    e = force_errors(unsort(out[0][0], g.unsorted_index())[:, :3], f_ref[:, :3])
+    assert e["guarded_max"] <= FORCE_TOL_GUARDED, e
