@@ -413,6 +413,13 @@ constexpr uint32_t kDilCellsMax = static_cast<uint32_t>((kTileEdge + 2) * (kTile
 static_assert(kDilCellsMax <= kBlock && kTileCells <= 64, "one thread per dilated cell, one lane per tile cell");
 static_assert(kResortCap <= 65536, "member indices are 16 bit");
 
+// Sum of a u32 over the 64 lanes of a wave (every lane gets it).
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 __device__ __forceinline__ unsigned long long lb_pack(unsigned long long state, uint32_t epoch, uint32_t count) {
   return (state << 62) | (static_cast<unsigned long long>(epoch & kLbEpochMask) << 32) | count;
 }
@@ -552,15 +559,23 @@ __global__ __launch_bounds__(kBlock) void k_tile_resort(uint32_t* __restrict__ f
       const uint32_t c = L.lr[k];
       L.order[L.coff[c] + atomicAdd(&L.cur[c], 1u)] = static_cast<uint16_t>(k);
     }
-  if (tid == 0) {
-    // this tile's first slot: the particles of every tile before it
+  if (tid < 64) {
+    // this tile's first slot: the particles of every tile before it.  The
+    // first wave reads 64 predecessors' words at a time, nearest first, until
+    // one holds an inclusive count; the sum runs up to that lane.
+    const uint32_t lane = tid;
     uint32_t acc = 0;
     bool ok = true;
-    for (uint32_t j = t; j-- > 0;) {
+    for (int32_t base = static_cast<int32_t>(t); base > 0; base -= 64) {
+      const int32_t j = base - 1 - static_cast<int32_t>(lane);
       unsigned long long w = 0;
+      bool ready = j < 0;
       for (uint32_t spin = 0;; ++spin) {
-        w = __hip_atomic_load(s.lookback + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((w >> 62) != 0ull && (static_cast<uint32_t>(w >> 32) & kLbEpochMask) == (epoch & kLbEpochMask)) break;
+        if (!ready) {
+          w = __hip_atomic_load(s.lookback + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ready = (w >> 62) != 0ull && (static_cast<uint32_t>(w >> 32) & kLbEpochMask) == (epoch & kLbEpochMask);
+        }
+        if (__builtin_amdgcn_ballot_w64(!ready) == 0ull) break;
         if (spin == kLookbackSpin) {
           ok = false;
           break;
@@ -568,16 +583,21 @@ __global__ __launch_bounds__(kBlock) void k_tile_resort(uint32_t* __restrict__ f
         __builtin_amdgcn_s_sleep(1);
       }
       if (!ok) break;
-      acc += static_cast<uint32_t>(w);
-      if ((w >> 62) == kLbInclusive) break;
+      const uint32_t count = j >= 0 ? static_cast<uint32_t>(w) : 0u;
+      const uint64_t incl = __builtin_amdgcn_ballot_w64(j >= 0 && (w >> 62) == kLbInclusive);
+      const uint32_t upto = incl != 0ull ? static_cast<uint32_t>(__builtin_ctzll(incl)) : 63u;
+      acc += wave_sum_u32(lane <= upto ? count : 0u);
+      if (incl != 0ull) break;
     }
-    if (!ok) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_WAIT);
-    __hip_atomic_store(s.lookback + t, lb_pack(kLbInclusive, epoch, acc + nmem), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    L.prefix = acc;
-    if (!fits) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_TILE);
-    // every particle is a member of exactly one tile unless one moved more than a cell
-    if (t == tg.ntiles - 1 && acc + nmem != n) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_MOVED);
+    if (lane == 0) {
+      if (!ok) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_WAIT);
+      __hip_atomic_store(s.lookback + t, lb_pack(kLbInclusive, epoch, acc + nmem), __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      L.prefix = acc;
+      if (!fits) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_TILE);
+      // every particle is a member of exactly one tile unless one moved more than a cell
+      if (t == tg.ntiles - 1 && acc + nmem != n) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_MOVED);
+    }
   }
   __syncthreads();
   const uint32_t S = L.prefix;
