@@ -388,14 +388,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
   const bool staged = (info.y >> 16) == halo_cap && info.x <= halo_cap;
   const bool fast = (info.y & kTileFast) != 0u;
   if (staged) {
+    // four halo particles per thread and pass: their slot and position loads
+    // are issued together, so a pass waits for two round trips, not eight
     const uint32_t shifts = fast ? info.y : 0u;
     const uint32_t* hs = halo_slots + static_cast<size_t>(t) * halo_cap;
-    for (uint32_t q = threadIdx.x; q < info.x; q += kBlock) {
-      float4 p = r[hs[q]];
-      p.x = shift_coord(p.x, shifts, 0, bx.L[0]);
-      p.y = shift_coord(p.y, shifts, 1, bx.L[1]);
-      p.z = shift_coord(p.z, shifts, 2, bx.L[2]);
-      s_pos[q] = p;
+    const uint32_t nh = info.x;
+    for (uint32_t q0 = threadIdx.x; q0 < nh; q0 += 4 * kBlock) {
+      uint32_t slot[4];
+      float4 p[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t q = q0 + u * kBlock;
+        slot[u] = hs[q < nh ? q : q0];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) p[u] = r[slot[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const uint32_t q = q0 + u * kBlock;
+        if (q < nh) {
+          p[u].x = shift_coord(p[u].x, shifts, 0, bx.L[0]);
+          p[u].y = shift_coord(p[u].y, shifts, 1, bx.L[1]);
+          p[u].z = shift_coord(p[u].z, shifts, 2, bx.L[2]);
+          s_pos[q] = p[u];
+        }
+      }
     }
   }
   __syncthreads();
