@@ -9,7 +9,8 @@ forces; FP64 energy/virial/kinetic sums, thermostat state and position
 bookkeeping ("mixed").
 
 A step is CalcF + Integrate (with the device-side skin check and, when it
-fires, the sort + list rebuild), run as HIP-graph-captured blocks of 10 steps
+fires, the sort + list rebuild; the forces and the NVT update run as one
+fused kernel, rumdx_pair_lj_nvt), run as HIP-graph-captured blocks of 10 steps
 that start at the first timed step; momentum is reset every 100 steps (at
 absolute step multiples of 100) as RUMD's Simulation.Run does.  Set-up runs
 E equilibration steps from the lattice (the liquid the timed steps measure),
@@ -18,9 +19,12 @@ instantiated after the warm-up, before the timer starts; the K timed steps
 are bracketed by stream synchronisation.
 
 Also reported:
-  roofline     the force kernel's algorithmic HBM bytes per launch
-               (16 + 4 + 4 nbar + 16 + 16 bytes per particle) over its average
-               duration, timed with HIP events on the sample's stream;
+  roofline     the fused step kernel's algorithmic HBM bytes per launch
+               (forces in RUMD's layout, 16 + 4 + 4 nbar + 16 + 16 bytes per
+               particle, plus the update's 96: v, low position parts, last
+               build positions, new positions) over its average duration,
+               timed with HIP events on the sample's stream; the force kernel
+               alone under roofline.force_only;
   cpu_baseline the FP64 CPU MD of the oracle (same algorithm: Verlet list with
                skin, cells, Nose-Hoover leapfrog) on the same start, on all host
                cores and on one core, over bounded samples.
@@ -124,11 +128,11 @@ def cpu_baseline(system, budget_s: float, max_steps: int):
     }
 
 
-def read_traffic(profile_dir: Path):
-    """HBM bytes per force-kernel launch from the committed rocprofv3 PMC
+def read_traffic(profile_dir: Path, name: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE + WRITE_SIZE, each calibrated
     against kernels of known traffic); None when absent."""
-    f = profile_dir / "force_kernel_ka1m_pmc.json"
+    f = profile_dir / name
     if not f.exists():
         return None, None
     try:
@@ -473,30 +477,56 @@ def main() -> int:
     captured, eager = run["captured"], run["eager"]
 
     nbar, nmax = pot.GetNeighborListStats()
+    fused = sample.UsesFusedSteps()
     force_ms = pot.TimeForceKernel(args.force_reps)
+    fused_ms = sample.TimeFusedStepKernel(args.force_reps) if fused else None
     build_ms = pot.TimeNbListBuild(10)
-    bytes_per_atom = 16 + 4 + 4 * nbar + 16 + 16
-    achieved = bytes_per_atom * n / (force_ms * 1e-3) / 1e9
-    traffic, traffic_kernel = read_traffic(ROOT / "profiles")
-    roofline = {
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "traffic_source": "committed rocprofv3 PMC summary of this workload (profiles/force_kernel_ka1m_pmc.md: "
-                          "FETCH_SIZE + WRITE_SIZE per launch, calibrated); not measured in this run",
-        "kernel": "k_pair_tiles<LjPair<SF>> (LJ force over the cell-tile lists)",
-        "kernel_ms": round(force_ms, 5),
-        "bytes_per_atom": round(bytes_per_atom, 2),
-    }
+    # RUMD's layout: r_i, the list length, u32 entries, f_i and w_i written
+    force_bytes = 16 + 4 + 4 * nbar + 16 + 16
+    # the fused step's update: v and (mixed) the low position parts read and
+    # written, the last build's positions read, the new positions written
+    nvt_bytes = 16 * (2 + 2 + 1 + 1) if args.precision == "mixed" else 16 * (2 + 1 + 1)
+
+    def hbm_roofline(bytes_per_atom, ms, pmc_file, kernel):
+        achieved = bytes_per_atom * n / (ms * 1e-3) / 1e9
+        traffic, _ = read_traffic(ROOT / "profiles", pmc_file)
+        return {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "traffic_source": f"committed rocprofv3 PMC summary of this workload (profiles/{pmc_file}: "
+                              "FETCH_SIZE + WRITE_SIZE per launch, calibrated); not measured in this run",
+            "kernel": kernel,
+            "kernel_ms": round(ms, 5),
+            "bytes_per_atom": round(bytes_per_atom, 2),
+        }
+
+    force_roofline = hbm_roofline(force_bytes, force_ms, "force_kernel_ka1m_pmc.json",
+                                  "k_pair_tiles<LjPair<SF>> (LJ force over the cell-tile lists)")
+    if fused:
+        # the step's dominant kernel: forces and the NVT update in one launch
+        # (timed with the thermostat's kinetic-energy sum off)
+        roofline = hbm_roofline(force_bytes + nvt_bytes, fused_ms, "fused_step_ka1m_pmc.json",
+                                "k_pair_tiles<LjPair<SF>, fused NVT step> (forces and the leapfrog/Nose-Hoover "
+                                "update over the cell-tile lists)")
+        roofline["force_only"] = force_roofline
+    else:
+        roofline = force_roofline
     # steps per rebuild: over the timed steps, and over the second half of the
     # equilibration (a longer window; the byte model uses it)
     kbar = args.steps / builds if builds else None
     kbar_model = run["kbar_equilibration"] or kbar
     rebuild_bytes = 4 * nbar + 260
-    step_bytes = bytes_per_atom + (64 + 48 + 4) + 32 + (rebuild_bytes / kbar_model if kbar_model else 0.0)
+    if fused:
+        step_bytes = force_bytes + nvt_bytes + (rebuild_bytes / kbar_model if kbar_model else 0.0)
+        model_terms = (f"fused force+NVT 16+4+4nbar+16+16 + {nvt_bytes} (skin check included), "
+                       "rebuild (4nbar+260)/kbar")
+    else:
+        step_bytes = force_bytes + (64 + 48 + 4) + 32 + (rebuild_bytes / kbar_model if kbar_model else 0.0)
+        model_terms = "force 16+4+4nbar+16+16, NVT 64+48+4, skin check 32, rebuild (4nbar+260)/kbar"
     pe = sample.GetPotentialEnergy() / n
     ke = itg.GetKineticEnergy() / n
 
@@ -558,7 +588,8 @@ def main() -> int:
             "timed_steps_captured": captured,
             "timed_steps_eager": eager,
             "model_bytes_per_atom_step": round(step_bytes, 1),
-            "model_terms": "force 16+4+4nbar+16+16, NVT 64+48+4, skin check 32, rebuild (4nbar+260)/kbar",
+            "model_terms": model_terms,
+            "fused_steps": fused,
             "model_hbm_fraction_step": round(value * step_bytes / 1e9 / HBM_PEAK_GBS, 4),
             "pe_per_particle": round(pe, 6),
             "ke_per_particle": round(ke, 6),

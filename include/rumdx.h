@@ -407,6 +407,24 @@ int rumdx_nvt_step(rumdx_particles* p, rumdx_nh_state* nh_dev, float dt, uint32_
                    float omega2, uint32_t dof, const float box[8], rumdx_nblist* nb,
                    void* workspace_dev, void* stream);
 
+/* One step of the run loop (Simulation.Run, Python/rumd/Simulation.py:583-597:
+ * sample.CalcF then itg.Integrate) for a sample whose only force term is a
+ * 12-6 pair potential over a list of the sorted method: rumdx_pair_lj with
+ * initialize = 1 and no stresses, and rumdx_nvt_step with the skin check, in
+ * one kernel.  A cell tile's particles are advanced as soon as the tile's
+ * forces are complete; their new positions go to r_out_dev (nvp x 4, not p->r_dev: other particles'
+ * forces still read the old positions), while v, im and rlo are updated in
+ * place and f, w hold the forces, energies and virials at the old positions.
+ * The caller swaps p->r_dev and r_out_dev afterwards.  With the thermostat on,
+ * the kinetic-energy sum is taken per cell tile (one double per tile in the
+ * workspace of rumdx_workspace_bytes(n), which must hold nb->ntiles of them:
+ * RUMDX_E_INVALID otherwise) and then in tile order, so Ps agrees with
+ * rumdx_nvt_step's to rounding; forces, velocities and positions of a step
+ * are bit-identical to the two calls'. */
+int rumdx_pair_lj_nvt(const rumdx_particles* p, const rumdx_nblist* nb, const float* params_dev, uint32_t ntypes,
+                      const float box[8], int cutoff_method, float* r_out_dev, rumdx_nh_state* nh_dev, float dt,
+                      uint32_t thermostat_on_mask, float omega2, uint32_t dof, void* workspace_dev, void* stream);
+
 /* IntegratorNVT::SetMomentumToZero (src/IntegratorNVT.cu:176-183, 444-472):
  * subtracts the centre-of-mass velocity, with the momentum sum in FP64. */
 int rumdx_zero_momentum(rumdx_particles* p, void* workspace_dev, void* stream);

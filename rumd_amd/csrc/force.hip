@@ -94,3 +94,69 @@ extern "C" int rumdx_pair_lj(const rumdx_particles* p, const rumdx_nblist* nb, c
     default: return fail(RUMDX_E_INVALID, "rumdx_pair_lj: invalid cutoff method");
   }
 }
+
+// One NVT step of a sample whose only force term is this 12-6 potential over a
+// sorted list: the forces at the positions p->r_dev, and in the same kernel
+// each particle's leapfrog / Nose-Hoover update with the skin check, the new
+// positions going to r_out_dev; then the thermostat update from the per-tile
+// kinetic-energy sums.  Per particle the arithmetic is that of rumdx_pair_lj
+// followed by rumdx_nvt_step.
+extern "C" int rumdx_pair_lj_nvt(const rumdx_particles* p, const rumdx_nblist* nb, const float* params_dev,
+                                 uint32_t ntypes, const float box[8], int cutoff_method, float* r_out_dev,
+                                 rumdx_nh_state* nh_dev, float dt, uint32_t thermostat_on_mask, float omega2,
+                                 uint32_t dof, void* workspace_dev, void* stream) {
+  using namespace rumdx;
+  const char* entry = "rumdx_pair_lj_nvt";
+  const int rc = check_pair_args(entry, p, nb, params_dev, ntypes, box, 0);
+  if (rc != RUMDX_OK) return rc;
+  if (nb == nullptr || nb->nbl16_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj_nvt: needs a list of the sorted method");
+  if (nb->flags_dev == nullptr || nb->last_r_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj_nvt: neighbour list not allocated");
+  if (p->v_dev == nullptr || p->im_dev == nullptr || r_out_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj_nvt: particle arrays not allocated");
+  if (r_out_dev == p->r_dev)
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj_nvt: r_out_dev must not alias the positions the forces read");
+  if (cutoff_method != RUMDX_CUTOFF_NS && cutoff_method != RUMDX_CUTOFF_SP && cutoff_method != RUMDX_CUTOFF_SF)
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj_nvt: invalid cutoff method");
+  const bool thermo = thermostat_on_mask != 0u;
+  if (thermo && (nh_dev == nullptr || workspace_dev == nullptr))
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj_nvt: thermostat state and workspace required");
+  if (thermo && dof == 0) return fail(RUMDX_E_INVALID, "rumdx_pair_lj_nvt: zero degrees of freedom");
+  // one kinetic-energy partial per tile in the workspace's partials
+  if (thermo && static_cast<size_t>(nb->ntiles) > 4u * static_cast<size_t>(blocks_for(p->n > 0 ? p->n : 1)))
+    return fail(RUMDX_E_INVALID, "rumdx_pair_lj_nvt: more tiles than the workspace holds partials (4 per 256 particles)");
+  if (p->n == 0) return RUMDX_OK;
+  NvtFuse fz{};
+  fz.r_out = reinterpret_cast<float4*>(r_out_dev);
+  fz.v = reinterpret_cast<float4*>(p->v_dev);
+  fz.im = reinterpret_cast<float4*>(p->im_dev);
+  fz.rlo = reinterpret_cast<float4*>(p->rlo_dev);
+  fz.nh = thermo ? nh_dev : nullptr;
+  fz.partials = thermo ? Workspace::partials(workspace_dev) : nullptr;
+  fz.last_r = reinterpret_cast<const float4*>(nb->last_r_dev);
+  fz.nb_flags = nb->flags_dev;
+  fz.skin2_over_4 = static_cast<float>(0.25 * static_cast<double>(nb->skin) * static_cast<double>(nb->skin));
+  fz.prune_r = reinterpret_cast<const float4*>(inner_prune_r(nb));
+  fz.inner2 = inner_check2(nb);
+  fz.dt = dt;
+  fz.on_mask = thermostat_on_mask;
+  const Box bx = make_box(box);
+  const hipStream_t st = as_stream(stream);
+  const uint32_t lpp = nb->lanes_per_particle;
+  const uint32_t lgp = lpp >= 8u ? 3u : lpp >= 4u ? 2u : lpp >= 2u ? 1u : 0u;
+  switch (cutoff_method) {
+    case RUMDX_CUTOFF_NS:
+      launch_tiles<LjPair<RUMDX_CUTOFF_NS>, false, true, true>({}, p, nb, params_dev, ntypes, bx, lgp, fz, st);
+      break;
+    case RUMDX_CUTOFF_SP:
+      launch_tiles<LjPair<RUMDX_CUTOFF_SP>, false, true, true>({}, p, nb, params_dev, ntypes, bx, lgp, fz, st);
+      break;
+    default:
+      launch_tiles<LjPair<RUMDX_CUTOFF_SF>, false, true, true>({}, p, nb, params_dev, ntypes, bx, lgp, fz, st);
+      break;
+  }
+  if (thermo)
+    hipLaunchKernelGGL(k_nvt_finish, dim3(1), dim3(kBlock), 0, st, fz.partials, nb->ntiles, nh_dev, dt, omega2, dof);
+  return check_launch(entry);
+}

@@ -26,6 +26,7 @@
 
 #include <string>
 
+#include "leapfrog.h"
 #include "rumdx_common.h"
 #include "tiles.h"
 
@@ -372,13 +373,20 @@ __device__ __forceinline__ void tile_particles(const PP& pp, uint32_t slot0, uin
 // offsets, so neighbour positions come from LDS.  The kernel uses no static
 // LDS: its dynamic LDS is the halo capacity and the parameter table, which
 // sets how many workgroups share a CU.
-template <class PP, bool STR, bool INIT, int WPE = PP::kWavesPerEU>
+//
+// With FUSE (initialising, no stresses) the kernel also takes the NVT step:
+// once the tile's forces are stored, its particles are advanced in slot order
+// (coalesced; the force loop visits them by list length), the positions into
+// fz.r_out, and the tile's sum of m v^2 goes to fz.partials[tile] for
+// k_nvt_finish.
+template <class PP, bool STR, bool INIT, int WPE = PP::kWavesPerEU, bool FUSE = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_pair_tiles(
     PP pp, const float4* __restrict__ r, float4* __restrict__ f, float4* __restrict__ w, float4* __restrict__ sts,
     const uint16_t* __restrict__ nbl16, const uint32_t* __restrict__ num_nbrs,
     const uint16_t* __restrict__ lane_order, const float* __restrict__ params, uint32_t nt, Box bx,
     const uint4* __restrict__ tile_info, const uint32_t* __restrict__ halo_slots, uint32_t halo_cap,
-    CellFallback fb, uint32_t nvp, uint32_t max_nbrs, uint32_t lgp) {
+    CellFallback fb, uint32_t nvp, uint32_t max_nbrs, uint32_t lgp, NvtFuse fz) {
+  static_assert(!FUSE || (INIT && !STR), "the fused step initialises forces and takes no stresses");
   extern __shared__ float4 s_dyn[];
   float4* s_pos = s_dyn;
   float4* s_tab = s_dyn + halo_cap;
@@ -387,6 +395,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
   pp.load_table(s_tab, params, nt);
   const bool staged = (info.y >> 16) == halo_cap && info.x <= halo_cap;
   const bool fast = (info.y & kTileFast) != 0u;
+  // the fused step's state of this thread's first particle, read now so that
+  // the reads overlap the staging and the pair loop
+  const uint32_t i_own = info.z + threadIdx.x;
+  NvtIn own{};
+  if (FUSE && i_own < info.w) own = nvt_fused_load(i_own, fz);
   if (staged) {
     // four halo particles per thread and pass: their slot and position loads
     // are issued together, so a pass waits for two round trips, not eight
@@ -422,6 +435,50 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
   else
     tile_particles<PP, STR, INIT, false>(pp, info.z, info.w, staged, 0u, r, f, w, sts, nbl16, num_nbrs, lane_order,
                                           s_pos, s_tab, nt, bx, fb, nvp, max_nbrs, lgp);
+  if (FUSE) {
+    // the tile's forces are stored (and read back from L2 in slot order);
+    // the halo is no longer read, so its LDS holds the reduction
+    __syncthreads();
+    const float ps = fz.nh != nullptr ? static_cast<float>(fz.nh->ps) : 0.f;
+    double ke2 = 0.0;
+    for (uint32_t i = i_own; i < info.w; i += kBlock)
+      ke2 += static_cast<double>(nvt_fused(i, r[i], f[i], i == i_own ? own : nvt_fused_load(i, fz), fz, ps, bx));
+    if (fz.nh != nullptr) {
+      const double sum = block_sum(ke2, reinterpret_cast<double*>(s_dyn));
+      if (threadIdx.x == 0) fz.partials[t] = sum;
+    }
+  }
+}
+
+// Launches the tile kernel over lists of the sorted method: the pruned inner
+// lists when the list has them (a subset in the same order and format: the
+// pairs beyond them are outside rc), and for the stepping variants (no
+// stresses, initialising) of a policy whose occupancy target is tunable, the
+// build compiled for rumdx_nblist.tile_waves_per_eu.
+template <class PP, bool STR, bool INIT, bool FUSE>
+void launch_tiles(const PP& pp, const rumdx_particles* p, const rumdx_nblist* nb, const float* params, uint32_t nt,
+                  const Box& bx, uint32_t lgp, const NvtFuse& fz, hipStream_t st) {
+  float4* r = reinterpret_cast<float4*>(p->r_dev);
+  float4* f = reinterpret_cast<float4*>(p->f_dev);
+  float4* w = reinterpret_cast<float4*>(p->w_dev);
+  float4* sts = reinterpret_cast<float4*>(p->sts_dev);
+  const size_t tab_bytes = sizeof(float4) * PP::kRow * nt * nt;
+  const CellFallback fb{nb->cell_rank_dev, nb->cell_start_dev, nb->slot_cell_dev, nb->nc[0], nb->nc[1], nb->nc[2],
+                        Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev}};
+  const bool inner = nb->nbl16_in_dev != nullptr && nb->num_in_dev != nullptr;
+  const uint16_t* lists = inner ? nb->nbl16_in_dev : nb->nbl16_dev;
+  const uint32_t* counts = inner ? nb->num_in_dev : nb->num_nbrs_dev;
+  const auto launch = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(nb->ntiles), dim3(kBlock), tab_bytes + sizeof(float4) * nb->halo_cap, st, pp, r, f,
+                       w, sts, lists, counts, nb->lane_order_dev, params, nt, bx,
+                       reinterpret_cast<const uint4*>(nb->tile_info_dev), nb->tile_halo_dev, nb->halo_cap, fb, p->nvp,
+                       nb->max_nbrs, lgp, fz);
+  };
+  if constexpr (PP::kTunedWaves && !STR && INIT) {
+    if (nb->tile_waves_per_eu == 4u) return launch(k_pair_tiles<PP, STR, INIT, 4, FUSE>);
+    if (nb->tile_waves_per_eu == 6u) return launch(k_pair_tiles<PP, STR, INIT, 6, FUSE>);
+  }
+  launch(k_pair_tiles<PP, STR, INIT, PP::kWavesPerEU, FUSE>);
 }
 
 template <class PP, bool STR, bool INIT>
@@ -436,32 +493,7 @@ int launch_pair(const PP& pp, const char* entry, const rumdx_particles* p, const
   const uint32_t lpp = nb != nullptr ? nb->lanes_per_particle : 1u;
   const uint32_t lgp = lpp >= 8u ? 3u : lpp >= 4u ? 2u : lpp >= 2u ? 1u : 0u;
   if (nb != nullptr && nb->nbl16_dev != nullptr) {
-    const CellFallback fb{nb->cell_rank_dev, nb->cell_start_dev, nb->slot_cell_dev, nb->nc[0], nb->nc[1], nb->nc[2],
-                          Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev}};
-    // the pruned inner lists when the list has them (a subset in the same
-    // order and format: the pairs beyond them are outside rc)
-    const bool inner = nb->nbl16_in_dev != nullptr && nb->num_in_dev != nullptr;
-    const uint16_t* lists = inner ? nb->nbl16_in_dev : nb->nbl16_dev;
-    const uint32_t* counts = inner ? nb->num_in_dev : nb->num_nbrs_dev;
-    const auto launch = [&](auto kernel) {
-      hipLaunchKernelGGL(kernel, dim3(nb->ntiles), dim3(kBlock), tab_bytes + sizeof(float4) * nb->halo_cap, st, pp, r,
-                         f, w, sts, lists, counts, nb->lane_order_dev, params, nt, bx,
-                         reinterpret_cast<const uint4*>(nb->tile_info_dev), nb->tile_halo_dev, nb->halo_cap, fb, p->nvp,
-                         nb->max_nbrs, lgp);
-    };
-    // the step's variant (no stresses, initialising) of a policy whose
-    // occupancy target is tunable also comes compiled for 4 and 6 waves/SIMD
-    if constexpr (PP::kTunedWaves && !STR && INIT) {
-      if (nb->tile_waves_per_eu == 4u) {
-        launch(k_pair_tiles<PP, STR, INIT, 4>);
-        return check_launch(entry);
-      }
-      if (nb->tile_waves_per_eu == 6u) {
-        launch(k_pair_tiles<PP, STR, INIT, 6>);
-        return check_launch(entry);
-      }
-    }
-    launch(k_pair_tiles<PP, STR, INIT>);
+    launch_tiles<PP, STR, INIT, false>(pp, p, nb, params, nt, bx, lgp, NvtFuse{}, st);
   } else if (nb != nullptr) {
     CellFallback fb{};
     fb.ex = Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev};

@@ -88,22 +88,6 @@ __global__ __launch_bounds__(kBlock) void k_nvt(NvtArgs a, Box bx) {
   }
 }
 
-// UpdateThermostatState in FP64: fixed-order sum of the block partials of
-// k_nvt, then Ps += dt omega^2 min(KE2 / (dof T) - 1, 10) and the T ramp.
-// One workgroup; the kernel boundary makes the partials visible.
-__global__ __launch_bounds__(kBlock) void k_nvt_finish(const double* __restrict__ partials, uint32_t np,
-                                                       rumdx_nh_state* nh, float dt, float omega2, uint32_t dof) {
-  __shared__ double s_red[kBlock / 64];
-  const double tot = sum_partials(partials, np, 1, s_red);
-  if (threadIdx.x == 0) {
-    double dev = tot / (static_cast<double>(dof) * nh->target_T) - 1.0;
-    if (dev > 10.0) dev = 10.0;  // runaway clamp (src/IntegratorNVT.cu:432-434)
-    nh->ps += static_cast<double>(dt) * static_cast<double>(omega2) * dev;
-    nh->ke2_sum = tot;
-    nh->target_T += nh->dT_per_step;
-  }
-}
-
 // Fixed-order sums of `ncomp`-strided block partials into out[c] * scale.
 __global__ __launch_bounds__(kBlock) void k_finish_sums(const double* __restrict__ partials, uint32_t np,
                                                         uint32_t ncomp, uint32_t mask, double scale,

@@ -67,4 +67,80 @@ __device__ __forceinline__ void add_image_shift(float4* __restrict__ im, uint32_
   }
 }
 
+// The NVT step fused into a force kernel (rumdx_pair_lj_nvt): a tile's
+// particles are advanced as soon as the tile's forces are complete.  Positions
+// go to a second array (other workgroups still read the old ones);
+// velocities, low position parts and image counts are the particle's own and
+// are updated in place.
+struct NvtFuse {
+  float4* r_out;
+  float4* v;
+  float4* im;
+  float4* rlo;            // null: plain FP32 positions
+  const rumdx_nh_state* nh;  // null: thermostat off
+  double* partials;       // one kinetic-energy partial per tile
+  const float4* last_r;   // skin check against the last build
+  uint32_t* nb_flags;
+  float skin2_over_4;
+  const float4* prune_r;  // inner-list check when non-null
+  float inner2;
+  float dt;
+  uint32_t on_mask;
+};
+
+// The particle state the fused step reads besides position and force; a
+// kernel may load it early (nvt_fused_load) to overlap the reads with work.
+struct NvtIn {
+  float4 v, lo, last_r;
+};
+
+__device__ __forceinline__ NvtIn nvt_fused_load(uint32_t i, const NvtFuse& z) {
+  NvtIn in;
+  in.v = z.v[i];
+  in.lo = z.rlo != nullptr ? z.rlo[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  in.last_r = z.last_r[i];
+  return in;
+}
+
+// One particle of the fused step: the update of k_nvt for slot i from its
+// position ri, force fi and state `in`; returns m v^2 when the particle is
+// thermostatted, else 0.
+__device__ __forceinline__ float nvt_fused(uint32_t i, float4 ri, float4 fi, NvtIn in, const NvtFuse& z, float ps,
+                                           const Box& bx) {
+  const bool apply = z.nh != nullptr && ((z.on_mask >> (__float_as_uint(ri.w) & 31u)) & 1u);
+  float3 shift;
+  float ke;
+  if (z.rlo != nullptr) {
+    ke = nvt_particle<true>(ri, in.v, fi, shift, in.lo, apply ? ps : 0.f, z.dt, bx);
+    z.rlo[i] = in.lo;
+  } else {
+    ke = nvt_particle<false>(ri, in.v, fi, shift, in.lo, apply ? ps : 0.f, z.dt, bx);
+  }
+  z.r_out[i] = ri;
+  z.v[i] = in.v;
+  add_image_shift(z.im, i, shift);
+  if (min_image_d2_f32(ri, in.last_r, bx) > z.skin2_over_4) z.nb_flags[RUMDX_FLAG_REBUILD] = 1u;
+  if (z.prune_r != nullptr && min_image_d2_f32(ri, z.prune_r[i], bx) > z.inner2) z.nb_flags[RUMDX_FLAG_PRUNE] = 1u;
+  return apply ? ke : 0.f;
+}
+
+namespace {
+
+// UpdateThermostatState in FP64: fixed-order sum of the block partials of
+// the integrator, then Ps += dt omega^2 min(KE2 / (dof T) - 1, 10) and the T
+// ramp.  One workgroup; the kernel boundary makes the partials visible.
+__global__ __launch_bounds__(kBlock) void k_nvt_finish(const double* __restrict__ partials, uint32_t np,
+                                                       rumdx_nh_state* nh, float dt, float omega2, uint32_t dof) {
+  __shared__ double s_red[kBlock / 64];
+  const double tot = sum_partials(partials, np, 1, s_red);
+  if (threadIdx.x == 0) {
+    double dev = tot / (static_cast<double>(dof) * nh->target_T) - 1.0;
+    if (dev > 10.0) dev = 10.0;  // runaway clamp (src/IntegratorNVT.cu:432-434)
+    nh->ps += static_cast<double>(dt) * static_cast<double>(omega2) * dev;
+    nh->ke2_sum = tot;
+    nh->target_T += nh->dT_per_step;
+  }
+}
+
+}  // namespace
 }  // namespace rumdx

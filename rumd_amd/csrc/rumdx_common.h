@@ -176,10 +176,21 @@ __device__ __forceinline__ bool last_to_arrive(uint32_t* counter, uint32_t nbloc
 }
 
 // Fixed-order sum of partials[0..np) by one workgroup; valid in thread 0.
+// Each thread adds its partials k, k + kBlock, ... in that order; the loads of
+// eight of them are issued together, so one workgroup does not wait for one
+// round trip per partial.
 __device__ __forceinline__ double sum_partials(const double* partials, uint32_t np, uint32_t stride,
                                                double* lds4) {
   double v = 0.0;
-  for (uint32_t k = threadIdx.x; k < np; k += kBlock) v += partials[static_cast<size_t>(k) * stride];
+  uint32_t k = threadIdx.x;
+  for (; k + 7u * kBlock < np; k += 8u * kBlock) {
+    double a[8];
+#pragma unroll
+    for (uint32_t u = 0; u < 8u; ++u) a[u] = partials[static_cast<size_t>(k + u * kBlock) * stride];
+#pragma unroll
+    for (uint32_t u = 0; u < 8u; ++u) v += a[u];
+  }
+  for (; k < np; k += kBlock) v += partials[static_cast<size_t>(k) * stride];
   return block_sum(v, lds4);
 }
 

@@ -75,6 +75,14 @@ void IntegratorNVT::Integrate() {
   if (nl) nl->NoteFusedCheck();
 }
 
+void IntegratorNVT::IntegrateFused(PairPotential& pp) {
+  if (!S || !S->GetNumberOfParticles()) throw RUMD_Error("IntegratorNVT", "Integrate", kNoSample);
+  if (!S->GetSimulationBox()) throw RUMD_Error("IntegratorNVT", "Integrate", "no simulation box");
+  const float omega2 = static_cast<float>(4. * M_PI * M_PI / (static_cast<double>(relaxation_time_) * relaxation_time_));
+  pp.CalcFIntegrateNVT(d_state_.get(), timeStep, thermostat_on_, omega2, S->GetNumberOfDOFs());
+  pp.GetNeighborList().NoteFusedCheck();
+}
+
 std::string IntegratorNVT::GetInfoString(unsigned precision) const {
   std::ostringstream s;
   if (thermostat_on_) {

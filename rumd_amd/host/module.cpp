@@ -361,6 +361,11 @@ PYBIND11_MODULE(_rumd, m) {
       .def("GetThreadsPerParticle", &Sample::GetThreadsPerParticle)
       .def("SetMixedPrecisionPositions", &Sample::SetMixedPrecisionPositions, py::arg("on"))
       .def("GetMixedPrecisionPositions", &Sample::GetMixedPrecisionPositions)
+      .def("SetFusedSteps", &Sample::SetFusedSteps, py::arg("on"))
+      .def("GetFusedSteps", &Sample::GetFusedSteps)
+      .def("UsesFusedSteps", [](Sample& s) { return s.FusedStepPotential() != nullptr; })
+      .def("TimeFusedStepKernel", &Sample::TimeFusedStepKernel, py::arg("reps"),
+           py::call_guard<py::gil_scoped_release>())
       .def("CalcF", &Sample::CalcF, py::arg("calc_stresses") = false)
       .def("GetPotentialEnergy", &Sample::GetPotentialEnergy)
       .def("GetVirial", &Sample::GetVirial)

@@ -607,6 +607,31 @@ void PairPotential::LaunchForce(const rumdx_particles* p, bool initialize, bool 
   rumdx_check(rc, GetID_String().c_str(), method);
 }
 
+bool PairPotential::CanFuseStep() {
+  if (!sample || GetPotentialForm() != 0 || fp64_forces_) return false;
+  if (neighborList.GetResolvedMethod() != NeighborList::SORT) return false;
+  const rumdx_nblist* nb = neighborList.Descriptor();
+  if (nb->nbl16_dev == nullptr || nb->ntiles == 0) return false;
+  // the workspace of rumdx_workspace_bytes(n) holds 4 partials per 256 particles
+  const size_t n = sample->GetNumberOfParticles();
+  return static_cast<size_t>(nb->ntiles) <= 4 * ((n + 255) / 256);
+}
+
+void PairPotential::CalcFIntegrateNVT(rumdx_nh_state* nh_dev, float dt, unsigned thermostat_on_mask, float omega2,
+                                      unsigned dof) {
+  if (!sample) throw RUMD_Error(GetID_String(), "CalcF", "no sample");
+  if (table_types_ != sample->GetNumberOfTypes()) Initialize();
+  neighborList.UpdateNBlist();
+  ParticleData& pd = sample->GetParticleData();
+  if (!pd.HasNextPositions())
+    throw RUMD_Error(GetID_String(), "CalcF", "the fused step needs the sample's second position array");
+  rumdx_check(rumdx_pair_lj_nvt(pd.Descriptor(), neighborList.Descriptor(), d_params_.get(), table_types_,
+                                sample->GetSimulationBox()->GetBox8(), cutoff_method_, pd.NextPositions(), nh_dev, dt,
+                                thermostat_on_mask, omega2, dof, sample->Workspace(), sample->GetStream()),
+              GetID_String().c_str(), "CalcF");
+  pd.SwapPositions();
+}
+
 void PairPotential::SetFP64Forces(bool on) {
   if (on && GetPotentialForm() != 0)
     throw RUMD_Error(GetID_String(), "SetFP64Forces", "the all-FP64 comparison path serves the 12-6 potentials only");
@@ -661,6 +686,49 @@ float PairPotential::TimeForceKernel(unsigned reps) {
   hip_check(hipEventSynchronize(b), "PairPotential", "TimeForceKernel");
   float ms = 0.f;
   hip_check(hipEventElapsedTime(&ms, a, b), "PairPotential", "TimeForceKernel");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms / reps;
+}
+
+float PairPotential::TimeFusedStepKernel(unsigned reps, float dt) {
+  if (!sample) throw RUMD_Error(GetID_String(), "TimeFusedStepKernel", "no sample");
+  if (!CanFuseStep()) throw RUMD_Error(GetID_String(), "TimeFusedStepKernel", "this potential takes no fused step");
+  if (reps == 0) reps = 1;
+  neighborList.UpdateNBlist();
+  const hipStream_t st = sample->GetStream();
+  ParticleData& pd = sample->GetParticleData();
+  const size_t f4 = 4ull * sample->GetNumberOfVirtualParticles();
+  DeviceArray<float> v(f4), im(f4), rlo, r_out(f4);
+  DeviceArray<uint32_t> flags(RUMDX_NB_FLAGS);
+  rumdx_particles p = *pd.Descriptor();
+  rumdx_nblist nb = *neighborList.Descriptor();
+  const auto copy = [&](void* dst, const void* src, size_t bytes) {
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st), "PairPotential", "TimeFusedStepKernel");
+  };
+  copy(v.get(), p.v_dev, sizeof(float) * f4);
+  copy(im.get(), p.im_dev, sizeof(float) * f4);
+  if (p.rlo_dev != nullptr) {
+    rlo.resize(f4);
+    copy(rlo.get(), p.rlo_dev, sizeof(float) * f4);
+    p.rlo_dev = rlo.get();
+  }
+  copy(flags.get(), nb.flags_dev, sizeof(uint32_t) * RUMDX_NB_FLAGS);
+  p.v_dev = v.get();
+  p.im_dev = im.get();
+  nb.flags_dev = flags.get();
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "PairPotential", "TimeFusedStepKernel");
+  hip_check(hipEventCreate(&b), "PairPotential", "TimeFusedStepKernel");
+  hip_check(hipEventRecord(a, st), "PairPotential", "TimeFusedStepKernel");
+  for (unsigned k = 0; k < reps; ++k)
+    rumdx_check(rumdx_pair_lj_nvt(&p, &nb, d_params_.get(), table_types_, sample->GetSimulationBox()->GetBox8(),
+                                  cutoff_method_, r_out.get(), nullptr, dt, 0u, 0.f, 0u, nullptr, st),
+                GetID_String().c_str(), "TimeFusedStepKernel");
+  hip_check(hipEventRecord(b, st), "PairPotential", "TimeFusedStepKernel");
+  hip_check(hipEventSynchronize(b), "PairPotential", "TimeFusedStepKernel");
+  float ms = 0.f;
+  hip_check(hipEventElapsedTime(&ms, a, b), "PairPotential", "TimeFusedStepKernel");
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   return ms / reps;

@@ -19,6 +19,7 @@ void ParticleData::SetNumberOfParticles(unsigned n, unsigned pb, bool mixed_posi
   d_w.resize(f4);
   d_im.resize(f4);
   d_sts.resize(f4);
+  d_r_next.release();
   if (mixed_) d_rlo.resize(f4); else d_rlo.release();
   d_unsorted_index.resize(nvp_);
   std::vector<uint32_t> id(nvp_);
@@ -40,6 +41,24 @@ void ParticleData::SetNumberOfParticles(unsigned n, unsigned pb, bool mixed_posi
   desc_.unsorted_index_dev = d_unsorted_index.get();
   desc_.n = n_;
   desc_.nvp = nvp_;
+}
+
+void ParticleData::EnsureNextPositions() {
+  if (!HasNextPositions()) d_r_next.resize(d_r.size());  // zero-filled like d_r's padding slots
+}
+
+void ParticleData::SwapPositions() {
+  std::swap(d_r, d_r_next);
+  desc_.r_dev = d_r.get();
+}
+
+void ParticleData::MovePositionsTo(float* home, hipStream_t st) {
+  if (d_r.get() == home) return;
+  if (home == nullptr || d_r_next.get() != home)
+    throw RUMD_Error("ParticleData", "MovePositionsTo", "not a position array of this sample");
+  hip_check(hipMemcpyAsync(home, d_r.get(), sizeof(float) * d_r.size(), hipMemcpyDeviceToDevice, st), "ParticleData",
+            "MovePositionsTo");
+  SwapPositions();
 }
 
 unsigned ParticleData::GetNumberThisType(unsigned t) const {

@@ -309,6 +309,22 @@ class PairPotential : public Potential {
   // comparison path; FP32 pair arithmetic otherwise.
   void SetFP64Forces(bool on);
   bool GetFP64Forces() const { return fp64_forces_; }
+  // True when a step of a sample whose only force term is this potential can
+  // run as one kernel (rumdx_pair_lj_nvt): the 12-6 form in FP32 over an
+  // allocated list of the sorted method whose tiles fit the sample's
+  // workspace.
+  bool CanFuseStep();
+  // CalcF(true, false) and the NVT step in one kernel; the new positions go to
+  // the sample's second position array, which then becomes the current one.
+  void CalcFIntegrateNVT(rumdx_nh_state* nh_dev, float dt, unsigned thermostat_on_mask, float omega2, unsigned dof);
+  // Average duration (ms) of the fused step kernel alone over `reps`
+  // launches, bracketed by HIP events like TimeForceKernel.  The launches
+  // advance copies of the velocities, images and low position parts, write
+  // their positions to a scratch array and raise their rebuild requests in a
+  // scratch copy of the list flags, so the sample's trajectory is untouched;
+  // the thermostat is off in them, which leaves out only the per-workgroup
+  // kinetic-energy sum and the one-workgroup thermostat update after it.
+  float TimeFusedStepKernel(unsigned reps, float dt);
   // Mean and maximum current list length (synchronises).
   std::vector<double> GetNeighborListStats();
   NeighborList& GetNeighborList() { return neighborList; }
@@ -724,6 +740,9 @@ class IntegratorNVT : public Integrator {
   void SetThermostatOnParticleType(unsigned type, bool on);
   unsigned GetThermostatOnMask() const { return thermostat_on_; }
   void SetTemperatureChangePerTimeStep(double dT);
+  // The step with the forces of `pp` computed in the same kernel
+  // (PairPotential::CalcFIntegrateNVT): CalcF and Integrate of Simulation.Run.
+  void IntegrateFused(PairPotential& pp);
 
  private:
   void PushState();
@@ -762,8 +781,18 @@ class ParticleData {
   void ResetSorting(hipStream_t st);
   rumdx_particles* Descriptor() { return &desc_; }
   std::vector<uint32_t> UnsortedIndex(hipStream_t st);
+  // The second position array of fused steps (rumdx_pair_lj_nvt writes the
+  // new positions there): allocated on demand, outside graph capture.
+  void EnsureNextPositions();
+  bool HasNextPositions() const { return d_r_next.size() == d_r.size() && d_r.size() != 0; }
+  float* NextPositions() const { return d_r_next.get(); }
+  // Makes the second array the current one (d_r and the descriptor follow).
+  void SwapPositions();
+  // Brings the positions back into the array `home` (one of the two) by a
+  // device copy on `st` when they are in the other one.
+  void MovePositionsTo(float* home, hipStream_t st);
 
-  DeviceArray<float> d_r, d_v, d_f, d_w, d_im, d_sts, d_rlo;
+  DeviceArray<float> d_r, d_v, d_f, d_w, d_im, d_sts, d_rlo, d_r_next;
   DeviceArray<uint32_t> d_unsorted_index;
 
  private:
@@ -832,6 +861,16 @@ class Sample {
   // (graphs stay valid until InvalidateGraphs).  Returns the number captured.
   unsigned PrepareRunSteps(unsigned long first_step, unsigned nsteps, unsigned block, unsigned momentum_interval);
   void InvalidateGraphs();
+  // Steps of a sample whose only force term is a 12-6 pair potential over a
+  // sorted list, integrated by IntegratorNVT, run the force and the
+  // integrator in one kernel (on by default; no counterpart in the
+  // reference, whose steps always launch them apart).
+  void SetFusedSteps(bool on);
+  bool GetFusedSteps() const { return fused_steps_; }
+  // The potential whose kernel takes the whole step, or null.
+  PairPotential* FusedStepPotential();
+  // PairPotential::TimeFusedStepKernel with the integrator's time step.
+  float TimeFusedStepKernel(unsigned reps);
 
   ParticleData& GetParticleData() { return pd_; }
   hipStream_t GetStream() const { return stream_; }
@@ -851,9 +890,16 @@ class Sample {
   void Synchronize();
 
  private:
+  // A captured step block.  It reads the positions from the array that was
+  // current when it was recorded (fused steps alternate two arrays) and ends
+  // in the same one.
+  struct Block {
+    hipGraphExec_t exec;
+    float* r_home;
+  };
   void PrepareRun();
   void StepOnce(bool zero_momentum);
-  hipGraphExec_t CaptureBlock(unsigned nsteps, bool reset_first);
+  Block CaptureBlock(unsigned nsteps, bool reset_first);
   unsigned pb_ = 0, tp_ = 0;
   unsigned user_dofs_ = 0;
   bool verbose_ = true;
@@ -866,7 +912,8 @@ class Sample {
   DeviceArray<char> workspace_;
   DeviceArray<double> scalars_;
   // captured step blocks: key = (block size, momentum reset at first step)
-  std::map<std::pair<unsigned, bool>, hipGraphExec_t> graphs_;
+  std::map<std::pair<unsigned, bool>, Block> graphs_;
+  bool fused_steps_ = true;
   std::unique_ptr<MoleculeData> molecules_;
   uint64_t slot_epoch_ = 1;
 };

@@ -208,8 +208,27 @@ void Sample::SortParticles() {
 }
 
 void Sample::InvalidateGraphs() {
-  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second.exec);
   graphs_.clear();
+}
+
+void Sample::SetFusedSteps(bool on) {
+  if (on == fused_steps_) return;
+  fused_steps_ = on;
+  InvalidateGraphs();
+}
+
+float Sample::TimeFusedStepKernel(unsigned reps) {
+  PrepareRun();
+  PairPotential* pp = FusedStepPotential();
+  if (pp == nullptr) throw RUMD_Error("Sample", "TimeFusedStepKernel", "the steps of this sample take no fused kernel");
+  return pp->TimeFusedStepKernel(reps, itg_->GetTimeStep());
+}
+
+PairPotential* Sample::FusedStepPotential() {
+  if (!fused_steps_ || potentials_.size() != 1 || dynamic_cast<IntegratorNVT*>(itg_) == nullptr) return nullptr;
+  auto* pp = dynamic_cast<PairPotential*>(potentials_.front());
+  return (pp != nullptr && pp->CanFuseStep()) ? pp : nullptr;
 }
 
 void Sample::PrepareRun() {
@@ -226,10 +245,17 @@ void Sample::PrepareRun() {
       // re-sort of skin-triggered rebuilds (the build settles its capacities)
       if (nl.NeedsFullSort()) nl.UpdateNBlist();
     }
+  if (FusedStepPotential() != nullptr) pd_.EnsureNextPositions();
 }
 
 void Sample::StepOnce(bool zero_momentum) {
   if (zero_momentum) itg_->SetMomentumToZero();
+  PairPotential* pp = FusedStepPotential();
+  if (pp != nullptr && pd_.HasNextPositions()) {
+    BumpSlotEpoch();
+    static_cast<IntegratorNVT*>(itg_)->IntegrateFused(*pp);
+    return;
+  }
   CalcF(false);
   itg_->Integrate();
 }
@@ -255,8 +281,9 @@ std::vector<std::pair<unsigned, bool>> Sample::RunSegments(unsigned long first_s
 // Records nsteps steps into a graph (nothing runs); the host-side state that
 // recording changes (the fused skin check's pending mark) is restored, so the
 // graph can be launched later as if recorded then.
-hipGraphExec_t Sample::CaptureBlock(unsigned nsteps, bool reset_first) {
+Sample::Block Sample::CaptureBlock(unsigned nsteps, bool reset_first) {
   NeighborList* nl = FusedCheckList();
+  float* home = pd_.d_r.get();
   hipGraph_t g = nullptr;
   hip_check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "Sample", "RunSteps");
   try {
@@ -264,9 +291,13 @@ hipGraphExec_t Sample::CaptureBlock(unsigned nsteps, bool reset_first) {
     // check fused into the previous integrator step
     if (nl) nl->ResetFusedCheck();
     for (unsigned k = 0; k < nsteps; ++k) StepOnce(k == 0 && reset_first);
+    // fused steps alternate the two position arrays: the graph ends in the
+    // one it starts from (a copy after an odd number of them)
+    pd_.MovePositionsTo(home, stream_);
   } catch (...) {
     (void)hipStreamEndCapture(stream_, &g);
     if (g) (void)hipGraphDestroy(g);
+    if (pd_.d_r.get() != home) pd_.SwapPositions();  // nothing ran
     throw;
   }
   hip_check(hipStreamEndCapture(stream_, &g), "Sample", "RunSteps");
@@ -274,7 +305,7 @@ hipGraphExec_t Sample::CaptureBlock(unsigned nsteps, bool reset_first) {
   const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   hip_check(e, "Sample", "RunSteps");
-  return exec;
+  return Block{exec, home};
 }
 
 unsigned Sample::PrepareRunSteps(unsigned long first_step, unsigned nsteps, unsigned block,
@@ -305,7 +336,8 @@ void Sample::RunSteps(unsigned long first_step, unsigned nsteps, unsigned block,
     if (it == graphs_.end()) it = graphs_.emplace(key, CaptureBlock(key.first, key.second)).first;
     // the graph's first step checks the skin itself (see CaptureBlock)
     if (NeighborList* nl = FusedCheckList()) nl->ResetFusedCheck();
-    hip_check(hipGraphLaunch(it->second, stream_), "Sample", "RunSteps");
+    pd_.MovePositionsTo(it->second.r_home, stream_);
+    hip_check(hipGraphLaunch(it->second.exec, stream_), "Sample", "RunSteps");
     if (NeighborList* nl = FusedCheckList()) nl->NoteFusedCheck();
   }
   Synchronize();

@@ -113,6 +113,7 @@ def lib() -> C.CDLL:
     L.rumdx_pair_lj_fp64.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
     L.rumdx_pair_form.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, i32, vp]
     L.rumdx_nvt_step.argtypes = [PP, NHP, f32, u32, f32, u32, f32a, NP, vp, vp]
+    L.rumdx_pair_lj_nvt.argtypes = [PP, NP, vp, u32, f32a, i32, vp, NHP, f32, u32, f32, u32, vp, vp]
     L.rumdx_zero_momentum.argtypes = [PP, vp, vp]
     L.rumdx_kinetic_energy.argtypes = [PP, NHP, f32, u32, vp, vp, vp]
     L.rumdx_reduce_f4.argtypes = [vp, u32, u32, vp, vp, vp]
@@ -363,6 +364,17 @@ class GpuSystem:
         nbp = C.byref(self.nb) if (fuse_skin and self.nb is not None) else None
         check(lib().rumdx_nvt_step(C.byref(self.p), C.cast(self.nh.value, C.POINTER(NhState)), dt, on_mask,
                                    omega2, dof, self.box8, nbp, self.ws.value, None))
+
+    def fused_step(self, cutoff, dt, on_mask=0, omega2=0.0, dof=None, r_out=None):
+        """rumdx_pair_lj_nvt: the list update, then forces and the NVT step in
+        one kernel with the new positions in r_out (a buffer of nvp x 4
+        floats); returns the entry point's code."""
+        dof = 3 * self.n - 3 if dof is None else dof
+        self.update_nblist()
+        return lib().rumdx_pair_lj_nvt(C.byref(self.p), C.byref(self.nb), self.d_params.value, self.nt, self.box8,
+                                       cutoff, r_out.value if r_out is not None else None,
+                                       C.cast(self.nh.value, C.POINTER(NhState)), dt, on_mask, omega2, dof,
+                                       self.ws.value, None)
 
     def reduce_f4(self, buf, comp_mask=15):
         check(lib().rumdx_reduce_f4(buf.value, self.n, comp_mask, self.out.value, self.ws.value, None))

@@ -118,6 +118,7 @@ def test_bonded_graph_blocks_equal_eager_and_conserve_energy(golden_dir):
         print(f"[case] pentamer200 NVE graph={graph}", flush=True)
         if graph:
             s.RunSteps(0, 200, 10, 0)
+            assert not s.UsesFusedSteps()  # more than one force term: the two-kernel step
         else:
             for _ in range(200):
                 s.CalcF()

@@ -93,18 +93,23 @@ def _run_api_loop(s, itg, n):
     s.Synchronize()
 
 
+@pytest.mark.parametrize("fused", [False, True])
 @pytest.mark.parametrize("mixed", [False, True])
-def test_graph_blocks_equal_eager_loop(mixed):
+def test_graph_blocks_equal_eager_loop(mixed, fused):
     """RunSteps (HIP-graph blocks of 10 steps) gives bit-identical results to
-    the eager CalcF/Integrate loop, replay after replay."""
+    eager steps, replay after replay: the CalcF/Integrate loop for the
+    two-kernel step, RunSteps with blocks of one step for the fused one."""
     sysm = configs.ka_random_lattice(12000, density=1.2, temperature=0.45, seed=31)
     conf = conf_from_system(sysm)
     out = []
     for mode in ("eager", "graph"):
         s, pot = ka_sample(conf, cutoff=rumd.ShiftedForce, mixed=mixed)
+        s.SetFusedSteps(fused)
         itg = rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.45, thermostatRelaxationTime=0.2)
         s.SetIntegrator(itg)
-        if mode == "eager":
+        if mode == "eager" and fused:
+            s.RunSteps(0, 60, 1, 100)
+        elif mode == "eager":
             for i in range(60):
                 if i % 100 == 0:
                     itg.SetMomentumToZero()
@@ -312,6 +317,7 @@ def test_run_segments_start_at_the_first_step_and_end_at_resets():
     instantiates their graphs without running a step."""
     sysm = configs.ka_random_lattice(2000, density=1.2, temperature=0.45, seed=79)
     s, pot = ka_sample(conf_from_system(sysm), cutoff=rumd.ShiftedForce)
+    s.SetFusedSteps(False)  # compared below with the CalcF/Integrate loop
     itg = rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.45, thermostatRelaxationTime=0.2)
     s.SetIntegrator(itg)
     assert s.RunSegments(5, 20, 10, 100) == [(10, False), (10, False)]

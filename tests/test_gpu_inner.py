@@ -215,6 +215,7 @@ def test_host_api_inner_lists_in_graph_blocks():
         pot.SetNbListSkin(0.3)
         pot.SetNbListInnerSkin(0.1)
         s.SetPotential(pot)
+        s.SetFusedSteps(False)  # the graph runs the kernels of the CalcF/Integrate loop
         itg = rumd.IntegratorNVT(timeStep=0.0025, targetTemperature=0.45, thermostatRelaxationTime=0.2)
         s.SetIntegrator(itg)
         print(f"[case] host API inner lists, {mode}", flush=True)
