@@ -524,6 +524,15 @@ typedef struct rumdx_batch {
   float* t_v_dev;
   float* t_im_dev;
   uint32_t* t_idx_dev;
+  /* Replica-staged forces (optional; NULL: neighbours gathered through L2).
+   * Work item k is (replica, first slot, end slot, 0), a slot range of one
+   * replica: rumdx_batch_forces runs one 1024-thread workgroup per item, which
+   * copies the replica's fixed-point positions into LDS (max_rep_n x 16 bytes
+   * plus the parameter table: at most the device's per-workgroup opt-in LDS)
+   * and reads every neighbour from there.  The forces are the same bits. */
+  const uint32_t* force_work_dev; /* force_work_n x 4 */
+  uint32_t force_work_n;
+  uint32_t max_rep_n;            /* particles of the largest replica */
 } rumdx_batch;
 
 /* Bytes of device workspace rumdx_batch_nvt_step / rumdx_batch_energies need. */

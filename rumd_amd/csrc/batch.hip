@@ -441,23 +441,105 @@ struct CellArrays {
   const uint32_t* cell_n;
 };
 
-// One particle's list over 2^lgp lanes (lane part takes entries part,
-// part + 2^lgp, ...), combined by a fixed butterfly: the workgroup covers
-// kBlock >> lgp slots, all of one replica (its 256-slot block).  An
-// overflowed list rescans the build-time stencil in its first lane.
-template <int CUT>
-__global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t* __restrict__ nbl,
-                                                    const uint32_t* __restrict__ num_nbrs, uint32_t max_nbrs,
-                                                    CellArrays cells, const float* __restrict__ params,
-                                                    uint32_t nt, uint32_t lgp) {
-  extern __shared__ float4 s_tab[];
-  for (uint32_t k = threadIdx.x; k < nt * nt; k += kBlock) {
+// The LDS table of the LJ pair term: per type pair {rc^2, A12, A6, s(rc) rc}
+// and {12 A12, 6 A6, -u(rc), -(u(rc) + s(rc) rc^2)}.
+__device__ __forceinline__ void load_lj_table(float4* s_tab, const float* __restrict__ params, uint32_t nt) {
+  for (uint32_t k = threadIdx.x; k < nt * nt; k += blockDim.x) {
     const float* q = params + static_cast<size_t>(k) * RUMDX_NUM_PARAM;
     const float rc = q[0], a12 = q[1], a6 = q[2];
     const float src = q[RUMDX_NUM_PARAM - 2], ucut = q[RUMDX_NUM_PARAM - 1];
     s_tab[2 * k] = make_float4(rc * rc, a12, a6, src);
     s_tab[2 * k + 1] = make_float4(12.f * a12, 6.f * a6, -ucut, -fmaf(src, rc, ucut));
   }
+}
+
+// Neighbour positions read from the batch's fixed-point array.
+struct GlobalFixed {
+  const int4* fixed;
+  __device__ __forceinline__ int4 operator()(uint32_t j) const { return fixed[j]; }
+};
+
+// Neighbour positions read from a replica staged in LDS (slot j at s[j - slot0]).
+struct StagedFixed {
+  const int4* s;
+  uint32_t slot0;
+  __device__ __forceinline__ int4 operator()(uint32_t j) const { return s[j - slot0]; }
+};
+
+// One lane's share of slot i's pair terms: entries part, part + 2^lgp, ... of
+// its list, four loads in flight, positions through `pos`.  An overflowed
+// list rescans the build-time stencil (from global memory) in its first lane.
+template <int CUT, class Pos>
+__device__ __forceinline__ void slot_terms(const BatchArgs& a, const rumdx_replica& rp, uint32_t i, int4 fi,
+                                           uint32_t part, uint32_t lgp, const uint32_t* __restrict__ nbl,
+                                           const uint32_t* __restrict__ num_nbrs, uint32_t max_nbrs,
+                                           const CellArrays& cells, const float4* prow, const Pos& pos, Acc& acc) {
+  const uint32_t step = 1u << lgp;
+  const double unit = static_cast<double>(rp.L) * 2.3283064365386963e-10;  // L 2^-32, exact
+  const uint32_t nn = num_nbrs[i];
+  if (nn >= max_nbrs) {
+    if (part == 0) {
+      // overflowed list: rescan the build-time stencil (every pair now
+      // inside rc was inside rc + skin then)
+      const int nc = rp.nc;
+      const uint32_t lc = cells.slot_cell[i] - rp.cell0;
+      const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc),
+                cz = static_cast<int>(lc / (nc * nc));
+      for (int dz = -2; dz <= 2; ++dz)
+        for (int dy = -2; dy <= 2; ++dy)
+          for (int dx = -2; dx <= 2; ++dx) {
+            const uint32_t c = rp.cell0 + static_cast<uint32_t>((cx + dx + nc) % nc +
+                                                                nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc)));
+            const uint32_t b0 = cells.cell_start[c], m = cells.cell_n[c];
+            for (uint32_t j = b0; j < b0 + m; ++j)
+              if (j != i) pair_b<CUT>(fi, a.fixed[j], unit, prow, acc);
+          }
+    }
+    return;
+  }
+  const size_t stride = static_cast<size_t>(a.nslots) << lgp;
+  const uint32_t* col = nbl + static_cast<size_t>(part) * a.nslots + i;
+  uint32_t k = part;
+  for (; k + 3 * step < nn; k += 4 * step) {
+    const uint32_t j0 = col[0], j1 = col[stride], j2 = col[2 * stride], j3 = col[3 * stride];
+    col += 4 * stride;
+    const int4 f0 = pos(j0), f1 = pos(j1), f2 = pos(j2), f3 = pos(j3);
+    pair_b<CUT>(fi, f0, unit, prow, acc);
+    pair_b<CUT>(fi, f1, unit, prow, acc);
+    pair_b<CUT>(fi, f2, unit, prow, acc);
+    pair_b<CUT>(fi, f3, unit, prow, acc);
+  }
+  for (; k < nn; k += step, col += stride) pair_b<CUT>(fi, pos(*col), unit, prow, acc);
+}
+
+// Sums the 2^lgp lanes of each slot (a fixed butterfly); every lane of the
+// wave takes part.
+__device__ __forceinline__ void combine_slot_lanes(Acc& acc, uint32_t lgp) {
+  for (uint32_t o = 1; o < (1u << lgp); o <<= 1) {
+    acc.fx += __shfl_xor(acc.fx, o, 64);
+    acc.fy += __shfl_xor(acc.fy, o, 64);
+    acc.fz += __shfl_xor(acc.fz, o, 64);
+    acc.u += __shfl_xor(acc.u, o, 64);
+    acc.vir += __shfl_xor(acc.vir, o, 64);
+  }
+}
+
+__device__ __forceinline__ void store_slot(const BatchArgs& a, uint32_t i, const Acc& acc) {
+  a.f[i] = make_float4(acc.fx, acc.fy, acc.fz, 0.5f * acc.u);
+  a.w[i] = make_float4(0.f, 0.f, 0.f, acc.vir);
+}
+
+// One particle's list over 2^lgp lanes (lane part takes entries part,
+// part + 2^lgp, ...), combined by a fixed butterfly: the workgroup covers
+// kBlock >> lgp slots, all of one replica (its 256-slot block); neighbour
+// positions are gathered through L2.
+template <int CUT>
+__global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t* __restrict__ nbl,
+                                                    const uint32_t* __restrict__ num_nbrs, uint32_t max_nbrs,
+                                                    CellArrays cells, const float* __restrict__ params,
+                                                    uint32_t nt, uint32_t lgp) {
+  extern __shared__ float4 s_tab[];
+  load_lj_table(s_tab, params, nt);
   __syncthreads();
   const uint32_t step = 1u << lgp;
   const uint32_t i = (blockIdx.x * kBlock + threadIdx.x) >> lgp, part = threadIdx.x & (step - 1u);
@@ -467,53 +549,46 @@ __global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t*
   Acc acc;
   if (live) {
     const int4 fi = a.fixed[i];
-    const float4* prow = s_tab + 2 * fi.w * nt;
-    const double unit = static_cast<double>(rp.L) * 2.3283064365386963e-10;  // L 2^-32, exact
-    const uint32_t nn = num_nbrs[i];
-    if (nn >= max_nbrs) {
-      if (part == 0) {
-        // overflowed list: rescan the build-time stencil (every pair now
-        // inside rc was inside rc + skin then)
-        const int nc = rp.nc;
-        const uint32_t lc = cells.slot_cell[i] - rp.cell0;
-        const int cx = static_cast<int>(lc % nc), cy = static_cast<int>((lc / nc) % nc),
-                  cz = static_cast<int>(lc / (nc * nc));
-        for (int dz = -2; dz <= 2; ++dz)
-          for (int dy = -2; dy <= 2; ++dy)
-            for (int dx = -2; dx <= 2; ++dx) {
-              const uint32_t c = rp.cell0 + static_cast<uint32_t>((cx + dx + nc) % nc +
-                                                                  nc * ((cy + dy + nc) % nc + nc * ((cz + dz + nc) % nc)));
-              const uint32_t b0 = cells.cell_start[c], m = cells.cell_n[c];
-              for (uint32_t j = b0; j < b0 + m; ++j)
-                if (j != i) pair_b<CUT>(fi, a.fixed[j], unit, prow, acc);
-            }
-      }
-    } else {
-      const size_t stride = static_cast<size_t>(a.nslots) << lgp;
-      const uint32_t* col = nbl + static_cast<size_t>(part) * a.nslots + i;
-      uint32_t k = part;
-      for (; k + 3 * step < nn; k += 4 * step) {
-        const uint32_t j0 = col[0], j1 = col[stride], j2 = col[2 * stride], j3 = col[3 * stride];
-        col += 4 * stride;
-        const int4 f0 = a.fixed[j0], f1 = a.fixed[j1], f2 = a.fixed[j2], f3 = a.fixed[j3];
-        pair_b<CUT>(fi, f0, unit, prow, acc);
-        pair_b<CUT>(fi, f1, unit, prow, acc);
-        pair_b<CUT>(fi, f2, unit, prow, acc);
-        pair_b<CUT>(fi, f3, unit, prow, acc);
-      }
-      for (; k < nn; k += step, col += stride) pair_b<CUT>(fi, a.fixed[*col], unit, prow, acc);
+    slot_terms<CUT>(a, rp, i, fi, part, lgp, nbl, num_nbrs, max_nbrs, cells, s_tab + 2 * fi.w * nt,
+                    GlobalFixed{a.fixed}, acc);
+  }
+  combine_slot_lanes(acc, lgp);
+  if (live && part == 0) store_slot(a, i, acc);
+}
+
+// The same pair terms with the replica staged: a workgroup takes one work
+// item (replica, first slot, end slot), copies the replica's fixed-point
+// positions into LDS, and walks the item's slots kStagedBlock >> lgp at a
+// time, reading every neighbour from LDS.  Lane mapping and summation order
+// are those of k_b_force, so the forces are the same bits.
+constexpr uint32_t kStagedBlock = 1024;
+static_assert(kStagedBlock <= kMaxWorkgroup, "staged force workgroup");
+template <int CUT>
+__global__ __launch_bounds__(kStagedBlock) void k_b_force_staged(BatchArgs a, const uint32_t* __restrict__ nbl,
+                                                                 const uint32_t* __restrict__ num_nbrs,
+                                                                 uint32_t max_nbrs, CellArrays cells,
+                                                                 const float* __restrict__ params, uint32_t nt,
+                                                                 uint32_t lgp, const uint4* __restrict__ work) {
+  extern __shared__ float4 s_dyn[];
+  float4* s_tab = s_dyn;
+  int4* s_fix = reinterpret_cast<int4*>(s_dyn + 2 * nt * nt);
+  load_lj_table(s_tab, params, nt);
+  const uint4 wk = work[blockIdx.x];
+  const rumdx_replica rp = a.rep[wk.x];
+  for (uint32_t k = threadIdx.x; k < rp.n; k += kStagedBlock) s_fix[k] = a.fixed[rp.slot0 + k];
+  __syncthreads();
+  const StagedFixed pos{s_fix, rp.slot0};
+  const uint32_t step = 1u << lgp, part = threadIdx.x & (step - 1u);
+  for (uint32_t base = wk.y; base < wk.z; base += kStagedBlock >> lgp) {
+    const uint32_t i = base + (threadIdx.x >> lgp);
+    const bool live = i < wk.z;
+    Acc acc;
+    if (live) {
+      const int4 fi = pos(i);
+      slot_terms<CUT>(a, rp, i, fi, part, lgp, nbl, num_nbrs, max_nbrs, cells, s_tab + 2 * fi.w * nt, pos, acc);
     }
-  }
-  for (uint32_t o = 1; o < step; o <<= 1) {
-    acc.fx += __shfl_xor(acc.fx, o, 64);
-    acc.fy += __shfl_xor(acc.fy, o, 64);
-    acc.fz += __shfl_xor(acc.fz, o, 64);
-    acc.u += __shfl_xor(acc.u, o, 64);
-    acc.vir += __shfl_xor(acc.vir, o, 64);
-  }
-  if (live && part == 0) {
-    a.f[i] = make_float4(acc.fx, acc.fy, acc.fz, 0.5f * acc.u);
-    a.w[i] = make_float4(0.f, 0.f, 0.f, acc.vir);
+    combine_slot_lanes(acc, lgp);
+    if (live && part == 0) store_slot(a, i, acc);
   }
 }
 
@@ -691,6 +766,29 @@ extern "C" int rumdx_batch_forces(rumdx_batch* b, const float* params_dev, int c
   const uint32_t nblk = (b->nslots / kBlock) << lgp;
   const size_t lds = sizeof(float4) * 2 * b->ntypes * b->ntypes;
   const CellArrays cells{b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev};
+  if (b->force_work_dev != nullptr) {
+    if (b->force_work_n == 0 || b->max_rep_n == 0)
+      return fail(RUMDX_E_INVALID, "rumdx_batch_forces: empty staged work list");
+    const size_t lds_staged = lds + sizeof(int4) * b->max_rep_n;
+    const auto* work = reinterpret_cast<const uint4*>(b->force_work_dev);
+    const auto launch = [&](auto kernel) -> int {
+      // above 64 KiB a workgroup's dynamic LDS must be opted into
+      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_staged));
+      if (e != hipSuccess)
+        return fail(RUMDX_E_INVALID, std::string("rumdx_batch_forces: staged LDS of ") + std::to_string(lds_staged) +
+                                         " bytes refused: " + hipGetErrorString(e));
+      hipLaunchKernelGGL(kernel, dim3(b->force_work_n), dim3(kStagedBlock), lds_staged, st, a, b->nbl_dev,
+                         b->num_nbrs_dev, b->max_nbrs, cells, params_dev, b->ntypes, lgp, work);
+      return check_launch("rumdx_batch_forces");
+    };
+    switch (cutoff_method) {
+      case RUMDX_CUTOFF_NS: return launch(k_b_force_staged<RUMDX_CUTOFF_NS>);
+      case RUMDX_CUTOFF_SP: return launch(k_b_force_staged<RUMDX_CUTOFF_SP>);
+      case RUMDX_CUTOFF_SF: return launch(k_b_force_staged<RUMDX_CUTOFF_SF>);
+      default: return fail(RUMDX_E_INVALID, "rumdx_batch_forces: invalid cutoff method");
+    }
+  }
   switch (cutoff_method) {
     case RUMDX_CUTOFF_NS:
       hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_NS>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,

@@ -399,7 +399,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
   // the reads overlap the staging and the pair loop
   const uint32_t i_own = info.z + threadIdx.x;
   NvtIn own{};
-  if (FUSE && i_own < info.w) own = nvt_fused_load(i_own, fz);
+  if (FUSE && i_own < info.w) own = nvt_fused_load(i_own, r, fz);
   if (staged) {
     // four halo particles per thread and pass: their slot and position loads
     // are issued together, so a pass waits for two round trips, not eight
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WPE))) v
     const float ps = fz.nh != nullptr ? static_cast<float>(fz.nh->ps) : 0.f;
     double ke2 = 0.0;
     for (uint32_t i = i_own; i < info.w; i += kBlock)
-      ke2 += static_cast<double>(nvt_fused(i, r[i], f[i], i == i_own ? own : nvt_fused_load(i, fz), fz, ps, bx));
+      ke2 += static_cast<double>(nvt_fused(i, f[i], i == i_own ? own : nvt_fused_load(i, r, fz), fz, ps, bx));
     if (fz.nh != nullptr) {
       const double sum = block_sum(ke2, reinterpret_cast<double*>(s_dyn));
       if (threadIdx.x == 0) fz.partials[t] = sum;
@@ -476,7 +476,10 @@ void launch_tiles(const PP& pp, const rumdx_particles* p, const rumdx_nblist* nb
   };
   if constexpr (PP::kTunedWaves && !STR && INIT) {
     if (nb->tile_waves_per_eu == 4u) return launch(k_pair_tiles<PP, STR, INIT, 4, FUSE>);
-    if (nb->tile_waves_per_eu == 6u) return launch(k_pair_tiles<PP, STR, INIT, 6, FUSE>);
+    // the fused step's update state does not fit six waves' registers (it
+    // would spill), and the tile's LDS allows five workgroups per CU anyway
+    if constexpr (!FUSE)
+      if (nb->tile_waves_per_eu == 6u) return launch(k_pair_tiles<PP, STR, INIT, 6, FUSE>);
   }
   launch(k_pair_tiles<PP, STR, INIT, PP::kWavesPerEU, FUSE>);
 }

@@ -88,14 +88,15 @@ struct NvtFuse {
   uint32_t on_mask;
 };
 
-// The particle state the fused step reads besides position and force; a
-// kernel may load it early (nvt_fused_load) to overlap the reads with work.
+// The particle state the fused step reads besides the force; a kernel may
+// load it early (nvt_fused_load) to overlap the reads with work.
 struct NvtIn {
-  float4 v, lo, last_r;
+  float4 r, v, lo, last_r;
 };
 
-__device__ __forceinline__ NvtIn nvt_fused_load(uint32_t i, const NvtFuse& z) {
+__device__ __forceinline__ NvtIn nvt_fused_load(uint32_t i, const float4* __restrict__ r, const NvtFuse& z) {
   NvtIn in;
+  in.r = r[i];
   in.v = z.v[i];
   in.lo = z.rlo != nullptr ? z.rlo[i] : make_float4(0.f, 0.f, 0.f, 0.f);
   in.last_r = z.last_r[i];
@@ -103,10 +104,11 @@ __device__ __forceinline__ NvtIn nvt_fused_load(uint32_t i, const NvtFuse& z) {
 }
 
 // One particle of the fused step: the update of k_nvt for slot i from its
-// position ri, force fi and state `in`; returns m v^2 when the particle is
-// thermostatted, else 0.
-__device__ __forceinline__ float nvt_fused(uint32_t i, float4 ri, float4 fi, NvtIn in, const NvtFuse& z, float ps,
+// force fi and state `in`; returns m v^2 when the particle is thermostatted,
+// else 0.
+__device__ __forceinline__ float nvt_fused(uint32_t i, float4 fi, NvtIn in, const NvtFuse& z, float ps,
                                            const Box& bx) {
+  float4 ri = in.r;
   const bool apply = z.nh != nullptr && ((z.on_mask >> (__float_as_uint(ri.w) & 31u)) & 1u);
   float3 shift;
   float ke;

@@ -466,6 +466,9 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetMaxNumNbrs", &ReplicaBatch::SetMaxNumNbrs, py::arg("max_nbrs"))
       .def("GetMaxNumNbrs", &ReplicaBatch::GetMaxNumNbrs)
       .def("SetLanesPerParticle", &ReplicaBatch::SetLanesPerParticle, py::arg("tp"))
+      .def("SetStagedForces", &ReplicaBatch::SetStagedForces, py::arg("on"))
+      .def("GetStagedForces", &ReplicaBatch::GetStagedForces)
+      .def("UsesStagedForces", &ReplicaBatch::UsesStagedForces)
       .def("GetLanesPerParticle", &ReplicaBatch::GetLanesPerParticle)
       .def("GetBoxLength", &ReplicaBatch::GetBoxLength, py::arg("replica"))
       .def("GetThermostat", &ReplicaBatch::GetThermostat, py::arg("replica"))

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 
 namespace {
 
@@ -104,6 +105,13 @@ void ReplicaBatch::SetMaxNumNbrs(unsigned m) {
   Relayout();
   user_max_nbrs_ = m;
   max_nbrs_ = 0;
+}
+
+void ReplicaBatch::SetStagedForces(bool on) {
+  staged_forces_ = on;
+  if (!allocated_) return;
+  desc_.force_work_dev = (on && force_work_.size()) ? force_work_.get() : nullptr;
+  InvalidateGraphs();
 }
 
 void ReplicaBatch::SetLanesPerParticle(unsigned tp) {
@@ -346,6 +354,33 @@ void ReplicaBatch::Allocate() {
   desc_.t_idx_dev = t_idx_.get();
   desc_.cell_tmp_dev = cell_tmp_.get();
   desc_.slot_cell_dev = slot_cell_.get();
+  // staged forces: work items of at most kForceChunk slots of one replica,
+  // the largest replicas first, when the largest fits one workgroup's LDS
+  max_rep_n_ = max_n;
+  force_work_.release();
+  int dev = 0, optin = 0;
+  hip_check(hipGetDevice(&dev), "ReplicaBatch", "Allocate");
+  hip_check(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev), "ReplicaBatch", "Allocate");
+  const size_t staged_lds = 16ull * max_n + 32ull * ntypes_ * ntypes_;
+  if (staged_lds <= static_cast<size_t>(optin)) {
+    constexpr uint32_t kForceChunk = 2048;
+    std::vector<uint32_t> order(nrep);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return reps_[x].types.size() > reps_[y].types.size(); });
+    std::vector<uint32_t> hw;
+    for (uint32_t r : order) {
+      const uint32_t n = static_cast<uint32_t>(reps_[r].types.size());
+      for (uint32_t c = 0; c < n; c += kForceChunk) {
+        hw.insert(hw.end(), {r, slot0_[r] + c, slot0_[r] + std::min(n, c + kForceChunk), 0u});
+      }
+    }
+    force_work_.resize(hw.size());
+    force_work_.upload(hw.data(), hw.size(), stream_);
+    desc_.force_work_n = static_cast<uint32_t>(hw.size() / 4);
+    desc_.max_rep_n = max_n;
+  }
+  desc_.force_work_dev = (staged_forces_ && force_work_.size()) ? force_work_.get() : nullptr;
   allocated_ = true;
 }
 

@@ -32,6 +32,12 @@ class ReplicaBatch {
   // Lanes that share one particle's list in the force kernel (1, 2, 4, 8).
   void SetLanesPerParticle(unsigned tp);
   unsigned GetLanesPerParticle() const { return lanes_per_particle_; }
+  // Force kernel with each replica staged whole in LDS (on by default where
+  // the largest replica fits the device's per-workgroup LDS), or neighbours
+  // gathered through L2; the forces are the same bits.
+  void SetStagedForces(bool on);
+  bool GetStagedForces() const { return staged_forces_; }
+  bool UsesStagedForces() const { return desc_.force_work_dev != nullptr; }
   // Device copies of the dynamic state (positions, velocities, images, slot
   // ids, forces, thermostats, flags); RestoreState puts them back and marks
   // every list stale, so trial runs leave no trace in the trajectory.
@@ -98,6 +104,8 @@ class ReplicaBatch {
   unsigned user_max_nbrs_ = 0, max_nbrs_ = 0, lanes_per_particle_ = 1;
   bool saved_ = false;
   bool allocated_ = false;
+  bool staged_forces_ = true;
+  uint32_t max_rep_n_ = 0;
   hipStream_t stream_ = nullptr;
 
   std::vector<uint32_t> slot0_, cell0_;
@@ -105,7 +113,7 @@ class ReplicaBatch {
   uint32_t nslots_ = 0, ncells_ = 0;
   DeviceArray<float> r_, v_, f_, w_, im_, last_r_, params_, cut2_, t_r_, t_v_, t_im_;
   DeviceArray<uint32_t> block_rep_, flags_, nbl_, num_nbrs_, cell_count_, cell_start_, cell_n_, cell_tmp_,
-      slot_cell_, idx_, t_idx_, fixed_, cellblock_rep_;
+      slot_cell_, idx_, t_idx_, fixed_, cellblock_rep_, force_work_;
   DeviceArray<rumdx_replica> rep_;
   DeviceArray<rumdx_nh_state> nh_, save_nh_;
   DeviceArray<float> save_r_, save_v_, save_f_, save_w_, save_im_;

@@ -187,6 +187,41 @@ def test_batch_lanes_per_particle(tp):
     assert force_errors(o.GetForcesAndEnergies(0)[:, :3], f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
 
 
+@pytest.mark.parametrize("tp", [1, 4])
+def test_batch_staged_forces_equal_gathered(tp):
+    """The replica-staged force kernel (each replica copied whole into LDS,
+    work items of at most 2048 slots, largest replicas first) gives the gather
+    kernel's forces bit for bit, over 40 steps with rebuilds; also with an
+    overflowed list."""
+    sizes = [256, 1500, 2049, 5000, 8192]
+    systems = _systems(sizes, seed0=870)
+    temps = [0.5, 1.0, 2.0, 0.6, 0.9]
+    out = []
+    for staged in (False, True):
+        b = make_batch(systems, temps)
+        b.SetLanesPerParticle(tp)
+        b.SetStagedForces(staged)
+        print(f"[case] batch staged={staged} lanes={tp}", flush=True)
+        b.RunSteps(40, 10)
+        assert b.UsesStagedForces() == staged
+        out.append([(b.GetPositions(r), b.GetForcesAndEnergies(r)) for r in range(len(sizes))])
+    for r in range(len(sizes)):
+        np.testing.assert_array_equal(out[0][r][0], out[1][r][0], err_msg=f"positions {r}")
+        np.testing.assert_array_equal(out[0][r][1], out[1][r][1], err_msg=f"forces {r}")
+    res = []
+    for staged in (False, True):
+        o = make_batch(systems[1:3], temps[1:3], max_nbrs=24)
+        o.SetLanesPerParticle(tp)
+        o.SetStagedForces(staged)
+        o.CalcF()
+        res.append([o.GetForcesAndEnergies(r) for r in range(2)])
+    for r in range(2):
+        np.testing.assert_array_equal(res[0][r], res[1][r])
+    f_ref, _ = oracle.forces_brute(configs.pack_pos4(systems[1].positions, systems[1].types), systems[1].box, 2,
+                                   ka_table(), SF)
+    assert force_errors(res[1][0][:, :3], f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
+
+
 def test_batch_save_restore_leaves_no_trace():
     """Trial steps between SaveState and RestoreState do not change the
     trajectory: the run continues bit for bit as if they never happened."""
