@@ -228,9 +228,55 @@ struct BatchWave {
   uint32_t seg_first[128];
   uint16_t seg_pre[128];
   uint16_t seg_len[128];
-  uint32_t list[kBatchStage];
+  uint32_t list[2][kBatchStage];  // the staged lists of the two particles tested together
 };
 static_assert(RUMDX_BATCH_CELL_BLOCK % (kBlock / 64) == 0, "whole cells per wave");
+
+// Appends the accepted candidates of one 64-lane chunk to a staged list of
+// cnt entries (ballot order = candidate order); returns the new count.
+__device__ __forceinline__ uint32_t b_append(bool acc, uint32_t j, uint32_t cnt, uint32_t* list, uint32_t max_nbrs) {
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
+  const uint32_t k = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+  if (acc && k < max_nbrs) list[k] = j;
+  return cnt + static_cast<uint32_t>(__builtin_popcountll(mask));
+}
+
+// Tests the cell's window against particle a (slot ia) and, when TWO,
+// particle b: each pass reads four 64-candidate chunks' slots and positions
+// once for both.  Without WRAPS (a stencil that does not wrap, nc >= 7) the
+// plain FP32 difference is the minimum-image one.
+template <bool WRAPS, bool TWO>
+__device__ __forceinline__ void b_scan_window(const BatchWave& wb, uint32_t* list_a, uint32_t* list_b,
+                                              const float4* __restrict__ sr, uint32_t slot0, uint32_t total,
+                                              float4 ra, float4 rb, const float* crow_a, const float* crow_b,
+                                              uint32_t ia, uint32_t ib, const Box& bx, uint32_t max_nbrs,
+                                              uint32_t& cnt_a, uint32_t& cnt_b) {
+  const uint32_t lane = threadIdx.x & 63u;
+  for (uint32_t q0 = 0; q0 < total; q0 += 4 * 64) {
+    uint32_t j[4];
+    bool in[4];  // lanes past the window end hold slot ia and test nothing
+    float4 rj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t q = q0 + 64u * u + lane;
+      in[u] = q < total;
+      j[u] = in[u] ? slot0 + wb.cand[q] : ia;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rj[u] = sr[j[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t tj = __float_as_uint(rj[u].w);
+      const float da = WRAPS ? min_image_d2_f32(ra, rj[u], bx) : plain_d2_f32(ra, rj[u]);
+      cnt_a = b_append(j[u] != ia && da < crow_a[tj], j[u], cnt_a, list_a, max_nbrs);
+      if (TWO) {
+        const float db = WRAPS ? min_image_d2_f32(rb, rj[u], bx) : plain_d2_f32(rb, rj[u]);
+        cnt_b = b_append(in[u] && j[u] != ib && db < crow_b[tj], j[u], cnt_b, list_b, max_nbrs);
+      }
+    }
+  }
+}
 
 // The lists of a rebuilding replica from its sorted staging, one wave per
 // cell (RUMDX_BATCH_CELL_BLOCK cells per workgroup, all of one replica): the
@@ -302,40 +348,41 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
         for (uint32_t q = 0; q < len; ++q) wb.cand[pre + q] = static_cast<uint16_t>(first + q - rp.slot0);
       }
       wave_sync_lds();
-      for (uint32_t p = 0; p < m; ++p) {
-        const uint32_t i = cs + p;
-        const float4 ri = st.r[i];
-        const float* crow = s_cut + __float_as_uint(ri.w) * nt;
-        uint32_t cnt = 0;
-        for (uint32_t q0 = 0; q0 < total; q0 += 4 * 64) {
-          uint32_t j[4];
-          float4 rj[4];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const uint32_t q = q0 + 64u * u + lane;
-            j[u] = q < total ? rp.slot0 + wb.cand[q] : i;
-          }
-#pragma unroll
-          for (int u = 0; u < 4; ++u) rj[u] = st.r[j[u]];
-#pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            const bool acc = j[u] != i && min_image_d2_f32(ri, rj[u], bx) < crow[__float_as_uint(rj[u].w)];
-            const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
-            const uint32_t k = cnt + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-            if (acc && k < max_nbrs) wb.list[k] = j[u];
-            cnt += static_cast<uint32_t>(__builtin_popcountll(mask));
-          }
+      const bool wraps = stencil_wraps(cx, nc) || stencil_wraps(cy, nc) || stencil_wraps(cz, nc);
+      for (uint32_t p = 0; p < m; p += 2) {
+        const bool two = p + 1 < m;
+        const uint32_t ia = cs + p, ib = two ? ia + 1 : ia;
+        const float4 ra = st.r[ia], rb = st.r[ib];
+        const float* crow_a = s_cut + __float_as_uint(ra.w) * nt;
+        const float* crow_b = s_cut + __float_as_uint(rb.w) * nt;
+        uint32_t cnt_a = 0, cnt_b = 0;
+        if (wraps) {
+          if (two)
+            b_scan_window<true, true>(wb, wb.list[0], wb.list[1], st.r, rp.slot0, total, ra, rb, crow_a, crow_b, ia,
+                                      ib, bx, max_nbrs, cnt_a, cnt_b);
+          else
+            b_scan_window<true, false>(wb, wb.list[0], wb.list[1], st.r, rp.slot0, total, ra, rb, crow_a, crow_b, ia,
+                                       ib, bx, max_nbrs, cnt_a, cnt_b);
+        } else {
+          if (two)
+            b_scan_window<false, true>(wb, wb.list[0], wb.list[1], st.r, rp.slot0, total, ra, rb, crow_a, crow_b, ia,
+                                       ib, bx, max_nbrs, cnt_a, cnt_b);
+          else
+            b_scan_window<false, false>(wb, wb.list[0], wb.list[1], st.r, rp.slot0, total, ra, rb, crow_a, crow_b,
+                                        ia, ib, bx, max_nbrs, cnt_a, cnt_b);
         }
         wave_sync_lds();
-        const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
-        for (uint32_t k = lane; k < stored; k += 64) nbl[static_cast<size_t>(k) * a.nslots + i] = wb.list[k];
-        if (lane == 0) {
-          num_nbrs[i] = stored;
-          if (cnt >= max_nbrs) fl[kOverflow] = 1u;
+        for (int t = 0; t < (two ? 2 : 1); ++t) {
+          const uint32_t i = t ? ib : ia, cnt = t ? cnt_b : cnt_a;
+          const uint32_t stored = cnt < max_nbrs ? cnt : max_nbrs;
+          for (uint32_t k = lane; k < stored; k += 64) nbl[static_cast<size_t>(k) * a.nslots + i] = wb.list[t][k];
+          if (lane == 0) {
+            num_nbrs[i] = stored;
+            if (cnt >= max_nbrs) fl[kOverflow] = 1u;
+          }
+          longest = cnt > longest ? cnt : longest;
         }
-        longest = cnt > longest ? cnt : longest;
-        wave_sync_lds();  // the flush has read the list
+        wave_sync_lds();  // the flushes have read the lists
       }
     } else {
       // one lane per particle over the segments (any stencil size)

@@ -807,11 +807,6 @@ __device__ __forceinline__ void build_cell_staged(const float4* s_pos, const flo
   }
 }
 
-// Whether the 5-cell stencil of cell c wraps around the box on an axis of nc
-// cells.  When it does not and nc >= 7, every candidate lies within 3 cells
-// (< L/2) of the cell, so RUMD's minimum-image correction rounds to zero and
-// the plain FP32 difference is the same number.
-__device__ __forceinline__ bool stencil_wraps(int c, int nc) { return nc < 7 || c < 2 || c + 2 >= nc; }
 
 // Lists of one cell of a staged tile, built by one wave: lanes enumerate the
 // cell's stencil candidates (the 125 cells, or 25 rows of 5 consecutive halo

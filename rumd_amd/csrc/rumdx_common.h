@@ -91,6 +91,24 @@ struct Exclusions {
   }
 };
 
+// FP32 squared distance without the minimum-image correction: identical to
+// min_image_d2_f32 when the pair cannot wrap (|dx| < L/2 so rint gives 0),
+// which holds for every pair of a tile whose halo does not wrap and for the
+// stencil of a cell that does not wrap (stencil_wraps).
+__device__ __forceinline__ float plain_d2_f32(float4 a, float4 b) {
+#pragma clang fp contract(off)
+  const float dx = a.x - b.x;
+  const float dy = a.y - b.y;
+  const float dz = a.z - b.z;
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// Whether the 5-cell stencil of cell c wraps around the box on an axis of nc
+// cells.  When it does not and nc >= 7, every candidate lies within 3 cells
+// (< L/2) of the cell, so RUMD's minimum-image correction rounds to zero and
+// the plain FP32 difference is the same number.
+__device__ __forceinline__ bool stencil_wraps(int c, int nc) { return nc < 7 || c < 2 || c + 2 >= nc; }
+
 // RUMD's FP32 minimum-image squared distance (RectangularSimulationBox::
 // calculateDistance, include/rumd/SimulationBox.h:111-125), evaluated in source
 // order without FMA contraction so that it is reproducible bit for bit on the

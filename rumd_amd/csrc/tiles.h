@@ -254,15 +254,4 @@ __device__ __forceinline__ uint32_t halo_cell(const Halo& h, const TileGeom& g, 
   return static_cast<uint32_t>(hx + h.ext[0] * (hy + h.ext[1] * hz));
 }
 
-// FP32 squared distance without the minimum-image correction: identical to
-// min_image_d2_f32 when the pair cannot wrap (|dx| < L/2 so rint gives 0),
-// which holds for every pair of a tile whose halo does not wrap.
-__device__ __forceinline__ float plain_d2_f32(float4 a, float4 b) {
-#pragma clang fp contract(off)
-  const float dx = a.x - b.x;
-  const float dy = a.y - b.y;
-  const float dz = a.z - b.z;
-  return dx * dx + dy * dy + dz * dz;
-}
-
 }  // namespace rumdx
