@@ -277,7 +277,9 @@ def batched_leg(nrep: int, warmup: int, steps: int, sample_reps: int, single_ste
     bpp = 16 + 4 + 4 * nbar_b + 16 + 16
     ach = bpp * ntot / (fms * 1e-3) / 1e9
     batch_roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": round(ach / HBM_PEAK_GBS, 4), "kernel": "k_b_force (batched LJ force)",
+                      "frac": round(ach / HBM_PEAK_GBS, 4),
+                      "kernel": ("k_b_force_staged (batched LJ force, replicas staged in LDS)" if b.UsesStagedForces()
+                                 else "k_b_force (batched LJ force, neighbours gathered through L2)"),
                       "kernel_ms": round(fms, 5), "bytes_per_atom": round(bpp, 2), "nbar": round(nbar_b, 2)}
     print(f"[bench] batch force kernel {fms * 1e3:.1f} us, {ach:.0f} GB/s", file=sys.stderr, flush=True)
     picks = list(range(0, nrep, max(1, nrep // sample_reps)))[:sample_reps]
