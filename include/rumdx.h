@@ -533,6 +533,12 @@ typedef struct rumdx_batch {
   const uint32_t* force_work_dev; /* force_work_n x 4 */
   uint32_t force_work_n;
   uint32_t max_rep_n;            /* particles of the largest replica */
+  /* Nonzero: rumdx_batch_update_lists sorts each rebuilding replica with one
+   * workgroup in LDS (its cell counts, their scan and the stable placement),
+   * which needs 4 (max_rep_n + 2 max_rep_cells) bytes of LDS, at most 64 KiB;
+   * the largest nc^3 of any replica.  0: the four-kernel sort over the batch.
+   * Both give the same order. */
+  uint32_t max_rep_cells;
 } rumdx_batch;
 
 /* Bytes of device workspace rumdx_batch_nvt_step / rumdx_batch_energies need. */

@@ -168,6 +168,87 @@ struct SortStage {
   uint32_t* idx;
 };
 
+// The same sort by one 1024-thread workgroup per rebuilding replica, in LDS:
+// cell counts (then running cursors), their exclusive scan, the particles
+// placed by cell, and each particle's rank among its cell's members by old
+// slot (stable).  cell_start, cell_n, slot_cell and the staging arrays come
+// out as from k_b_hist, k_b_scan, k_b_scatter and k_b_gather; cell_count is
+// not touched (it stays zero).
+constexpr uint32_t kSortBlock = 1024;
+__global__ __launch_bounds__(kSortBlock) void k_b_sort(BatchArgs a, uint32_t* __restrict__ slot_cell,
+                                                       uint32_t* __restrict__ cell_start,
+                                                       uint32_t* __restrict__ cell_n,
+                                                       const uint32_t* __restrict__ idx, SortStage st,
+                                                       uint32_t max_cells) {
+  const uint32_t rep = blockIdx.x;
+  if (!rebuild_pending(a.flags, rep)) return;
+  extern __shared__ uint32_t s_sort[];
+  __shared__ uint32_t s_w[kSortBlock / 64];
+  const rumdx_replica rp = a.rep[rep];
+  const uint32_t ncell = static_cast<uint32_t>(rp.nc) * rp.nc * rp.nc;
+  uint32_t* s_cnt = s_sort;                  // counts, then cursors (max_cells)
+  uint32_t* s_start = s_sort + max_cells;    // replica-local cell starts (max_cells)
+  uint32_t* s_tmp = s_sort + 2 * max_cells;  // old slots placed by cell (n)
+  for (uint32_t c = threadIdx.x; c < ncell; c += kSortBlock) s_cnt[c] = 0u;
+  __syncthreads();
+  const Box bx = rep_box(rp);
+  for (uint32_t k = threadIdx.x; k < rp.n; k += kSortBlock) {
+    const uint32_t i = rp.slot0 + k;
+    const uint32_t lc = rep_cell(a.r[i], bx, rp.nc);
+    slot_cell[i] = rp.cell0 + lc;
+    atomicAdd(s_cnt + lc, 1u);
+  }
+  __syncthreads();
+  // exclusive scan of the counts, kSortBlock cells at a time
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < ncell; base += kSortBlock) {
+    const uint32_t c = base + threadIdx.x;
+    const uint32_t v = c < ncell ? s_cnt[c] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= static_cast<uint32_t>(o)) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t off = carry, tot = 0;
+    for (uint32_t k = 0; k < kSortBlock / 64; ++k) {
+      off += k < wave ? s_w[k] : 0u;
+      tot += s_w[k];
+    }
+    if (c < ncell) {
+      const uint32_t start = off + x - v;
+      s_start[c] = start;
+      cell_start[rp.cell0 + c] = rp.slot0 + start;
+      cell_n[rp.cell0 + c] = v;
+      s_cnt[c] = start;  // the cell's placement cursor
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  // place the old slots by cell (any order inside a cell), then rank
+  for (uint32_t k = threadIdx.x; k < rp.n; k += kSortBlock) {
+    const uint32_t i = rp.slot0 + k;
+    const uint32_t lc = slot_cell[i] - rp.cell0;
+    s_tmp[atomicAdd(s_cnt + lc, 1u)] = i;
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < rp.n; k += kSortBlock) {
+    const uint32_t x = s_tmp[k];
+    const uint32_t lc = slot_cell[x] - rp.cell0;
+    const uint32_t b0 = s_start[lc], m = (lc + 1 < ncell ? s_start[lc + 1] : rp.n) - b0;
+    uint32_t d = b0;
+    for (uint32_t j = b0; j < b0 + m; ++j) d += s_tmp[j] < x ? 1u : 0u;
+    d += rp.slot0;
+    st.r[d] = a.r[x];
+    st.v[d] = a.v[x];
+    st.im[d] = a.im[x];
+    st.idx[d] = idx[x];
+  }
+}
+
 // Gathers every particle of a rebuilding replica into its sorted slot: the
 // cell's first slot plus the number of the cell's particles with a smaller
 // old slot (a stable counting sort), and zeroes the counts for the next
@@ -780,15 +861,23 @@ extern "C" int rumdx_batch_update_lists(rumdx_batch* b, void* stream) {
   const BatchArgs a = args_of(b);
   const hipStream_t st = as_stream(stream);
   const uint32_t nblk = b->nslots / kBlock;
-  hipLaunchKernelGGL(k_b_hist, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_count_dev);
-  hipLaunchKernelGGL(k_b_scan, dim3(b->nrep), dim3(kBlock), 0, st, a, b->cell_count_dev, b->cell_start_dev,
-                     b->cell_n_dev);
-  hipLaunchKernelGGL(k_b_scatter, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
-                     b->cell_count_dev, b->cell_tmp_dev);
   const SortStage stage{reinterpret_cast<float4*>(b->t_r_dev), reinterpret_cast<float4*>(b->t_v_dev),
                         reinterpret_cast<float4*>(b->t_im_dev), b->t_idx_dev};
-  hipLaunchKernelGGL(k_b_gather, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
-                     b->cell_n_dev, b->cell_count_dev, b->cell_tmp_dev, b->idx_dev, stage);
+  const size_t sort_lds = sizeof(uint32_t) * (static_cast<size_t>(b->max_rep_n) + 2ull * b->max_rep_cells);
+  if (b->max_rep_cells != 0) {
+    if (b->max_rep_n == 0 || sort_lds > 65536u)
+      return fail(RUMDX_E_INVALID, "rumdx_batch_update_lists: the one-workgroup sort needs at most 64 KiB of LDS");
+    hipLaunchKernelGGL(k_b_sort, dim3(b->nrep), dim3(kSortBlock), sort_lds, st, a, b->slot_cell_dev,
+                       b->cell_start_dev, b->cell_n_dev, b->idx_dev, stage, b->max_rep_cells);
+  } else {
+    hipLaunchKernelGGL(k_b_hist, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_count_dev);
+    hipLaunchKernelGGL(k_b_scan, dim3(b->nrep), dim3(kBlock), 0, st, a, b->cell_count_dev, b->cell_start_dev,
+                       b->cell_n_dev);
+    hipLaunchKernelGGL(k_b_scatter, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
+                       b->cell_count_dev, b->cell_tmp_dev);
+    hipLaunchKernelGGL(k_b_gather, dim3(nblk), dim3(kBlock), 0, st, a, b->slot_cell_dev, b->cell_start_dev,
+                       b->cell_n_dev, b->cell_count_dev, b->cell_tmp_dev, b->idx_dev, stage);
+  }
   hipLaunchKernelGGL(k_b_build, dim3(b->ncells / RUMDX_BATCH_CELL_BLOCK), dim3(kBlock), 0, st, a,
                      b->cellblock_rep_dev, b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev, stage, b->idx_dev,
                      b->cut_skin2_dev, b->ntypes, b->nbl_dev, b->num_nbrs_dev,

@@ -469,6 +469,9 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetStagedForces", &ReplicaBatch::SetStagedForces, py::arg("on"))
       .def("GetStagedForces", &ReplicaBatch::GetStagedForces)
       .def("UsesStagedForces", &ReplicaBatch::UsesStagedForces)
+      .def("SetWorkgroupSort", &ReplicaBatch::SetWorkgroupSort, py::arg("on"))
+      .def("GetWorkgroupSort", &ReplicaBatch::GetWorkgroupSort)
+      .def("UsesWorkgroupSort", &ReplicaBatch::UsesWorkgroupSort)
       .def("GetLanesPerParticle", &ReplicaBatch::GetLanesPerParticle)
       .def("GetBoxLength", &ReplicaBatch::GetBoxLength, py::arg("replica"))
       .def("GetThermostat", &ReplicaBatch::GetThermostat, py::arg("replica"))

@@ -114,6 +114,13 @@ void ReplicaBatch::SetStagedForces(bool on) {
   InvalidateGraphs();
 }
 
+void ReplicaBatch::SetWorkgroupSort(bool on) {
+  workgroup_sort_ = on;
+  if (!allocated_) return;
+  desc_.max_rep_cells = on ? max_rep_cells_ : 0u;
+  InvalidateGraphs();
+}
+
 void ReplicaBatch::SetLanesPerParticle(unsigned tp) {
   if (tp != 1 && tp != 2 && tp != 4 && tp != 8)
     throw RUMD_Error("ReplicaBatch", "SetLanesPerParticle", "lanes per particle must be 1, 2, 4 or 8");
@@ -381,6 +388,13 @@ void ReplicaBatch::Allocate() {
     desc_.max_rep_n = max_n;
   }
   desc_.force_work_dev = (staged_forces_ && force_work_.size()) ? force_work_.get() : nullptr;
+  // the one-workgroup rebuild sort when a replica's counts, starts and slots
+  // fit 64 KiB of LDS
+  uint64_t max_cells = 0;
+  for (unsigned r = 0; r < nrep; ++r)
+    max_cells = std::max<uint64_t>(max_cells, static_cast<uint64_t>(nc_[r]) * nc_[r] * nc_[r]);
+  max_rep_cells_ = (4ull * (max_n + 2 * max_cells) <= 65536ull) ? static_cast<uint32_t>(max_cells) : 0u;
+  desc_.max_rep_cells = workgroup_sort_ ? max_rep_cells_ : 0u;
   allocated_ = true;
 }
 

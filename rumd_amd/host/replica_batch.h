@@ -38,6 +38,12 @@ class ReplicaBatch {
   void SetStagedForces(bool on);
   bool GetStagedForces() const { return staged_forces_; }
   bool UsesStagedForces() const { return desc_.force_work_dev != nullptr; }
+  // Rebuild sort: one workgroup per rebuilding replica in LDS (on by default
+  // where the largest replica fits), or four kernels over the whole batch;
+  // the same order either way.
+  void SetWorkgroupSort(bool on);
+  bool GetWorkgroupSort() const { return workgroup_sort_; }
+  bool UsesWorkgroupSort() const { return desc_.max_rep_cells != 0; }
   // Device copies of the dynamic state (positions, velocities, images, slot
   // ids, forces, thermostats, flags); RestoreState puts them back and marks
   // every list stale, so trial runs leave no trace in the trajectory.
@@ -105,7 +111,8 @@ class ReplicaBatch {
   bool saved_ = false;
   bool allocated_ = false;
   bool staged_forces_ = true;
-  uint32_t max_rep_n_ = 0;
+  bool workgroup_sort_ = true;
+  uint32_t max_rep_n_ = 0, max_rep_cells_ = 0;
   hipStream_t stream_ = nullptr;
 
   std::vector<uint32_t> slot0_, cell0_;

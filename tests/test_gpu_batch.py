@@ -222,6 +222,30 @@ def test_batch_staged_forces_equal_gathered(tp):
     assert force_errors(res[1][0][:, :3], f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
 
 
+def test_batch_workgroup_sort_equals_batch_sort():
+    """Rebuild sorts by one workgroup per replica in LDS give the four-kernel
+    sort's particle order, cell starts and lists: 60 steps with rebuilds of
+    hot and cold replicas produce the same bits."""
+    sizes = [256, 1500, 4100, 8192]
+    systems = _systems(sizes, seed0=910)
+    temps = [2.0, 0.5, 1.2, 0.8]
+    out = []
+    for wg in (False, True):
+        b = make_batch(systems, temps)
+        b.SetWorkgroupSort(wg)
+        print(f"[case] batch workgroup sort={wg}", flush=True)
+        b.RunSteps(60, 10)
+        assert b.UsesWorkgroupSort() == wg
+        st = []
+        for r in range(len(sizes)):
+            off, ids = b.GetNeighborListCSR(r)
+            st.append((b.GetPositions(r), b.GetVelocities(r), b.GetForcesAndEnergies(r), off, ids))
+        out.append(st)
+    for r in range(len(sizes)):
+        for k, name in enumerate(("positions", "velocities", "forces", "list offsets", "list ids")):
+            np.testing.assert_array_equal(out[0][r][k], out[1][r][k], err_msg=f"{name} {r}")
+
+
 def test_batch_save_restore_leaves_no_trace():
     """Trial steps between SaveState and RestoreState do not change the
     trajectory: the run continues bit for bit as if they never happened."""
