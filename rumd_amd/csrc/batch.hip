@@ -359,6 +359,36 @@ __device__ __forceinline__ void b_scan_window(const BatchWave& wb, uint32_t* lis
   }
 }
 
+// Window chunks a lane holds in registers while the cell's particles are
+// tested (windows of up to 512 candidates): the candidates' slots and
+// positions are read once per cell, not once per pair of particles.
+constexpr uint32_t kRegChunks = 8;
+
+// The test of particles a (and b when TWO) against a window held in
+// registers (chunk u: candidates 64u + lane, `nch` chunks, `total`
+// candidates); the same order and criterion as b_scan_window.
+template <bool WRAPS, bool TWO>
+__device__ __forceinline__ void b_test_held(const uint32_t (&j)[kRegChunks], const float4 (&rj)[kRegChunks],
+                                            uint32_t nch, uint32_t total, uint32_t* list_a, uint32_t* list_b,
+                                            float4 ra, float4 rb, const float* crow_a, const float* crow_b,
+                                            uint32_t ia, uint32_t ib, const Box& bx, uint32_t max_nbrs,
+                                            uint32_t& cnt_a, uint32_t& cnt_b) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (uint32_t u = 0; u < kRegChunks; ++u) {
+    if (u < nch) {
+      const bool in = 64u * u + lane < total;
+      const uint32_t tj = __float_as_uint(rj[u].w);
+      const float da = WRAPS ? min_image_d2_f32(ra, rj[u], bx) : plain_d2_f32(ra, rj[u]);
+      cnt_a = b_append(in && j[u] != ia && da < crow_a[tj], j[u], cnt_a, list_a, max_nbrs);
+      if (TWO) {
+        const float db = WRAPS ? min_image_d2_f32(rb, rj[u], bx) : plain_d2_f32(rb, rj[u]);
+        cnt_b = b_append(in && j[u] != ib && db < crow_b[tj], j[u], cnt_b, list_b, max_nbrs);
+      }
+    }
+  }
+}
+
 // The lists of a rebuilding replica from its sorted staging, one wave per
 // cell (RUMDX_BATCH_CELL_BLOCK cells per workgroup, all of one replica): the
 // cell's stencil goes into an LDS window once, and each of its particles
@@ -430,6 +460,22 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
       }
       wave_sync_lds();
       const bool wraps = stencil_wraps(cx, nc) || stencil_wraps(cy, nc) || stencil_wraps(cz, nc);
+      // a window of at most kRegChunks chunks stays in registers for all of
+      // the cell's particles (lanes past its end hold the cell's first slot)
+      const bool held = total <= 64u * kRegChunks;
+      const uint32_t nch = (total + 63u) >> 6;
+      uint32_t hj[kRegChunks];
+      float4 hr[kRegChunks];
+#pragma unroll
+      for (uint32_t u = 0; u < kRegChunks; ++u) {
+        hj[u] = cs;
+        if (held && u < nch) {
+          const uint32_t q = 64u * u + lane;
+          hj[u] = q < total ? rp.slot0 + wb.cand[q] : cs;
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kRegChunks; ++u) hr[u] = (held && u < nch) ? st.r[hj[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
       for (uint32_t p = 0; p < m; p += 2) {
         const bool two = p + 1 < m;
         const uint32_t ia = cs + p, ib = two ? ia + 1 : ia;
@@ -437,7 +483,23 @@ __global__ __launch_bounds__(kBlock) void k_b_build(BatchArgs a, const uint32_t*
         const float* crow_a = s_cut + __float_as_uint(ra.w) * nt;
         const float* crow_b = s_cut + __float_as_uint(rb.w) * nt;
         uint32_t cnt_a = 0, cnt_b = 0;
-        if (wraps) {
+        if (held) {
+          if (wraps) {
+            if (two)
+              b_test_held<true, true>(hj, hr, nch, total, wb.list[0], wb.list[1], ra, rb, crow_a, crow_b, ia, ib, bx,
+                                      max_nbrs, cnt_a, cnt_b);
+            else
+              b_test_held<true, false>(hj, hr, nch, total, wb.list[0], wb.list[1], ra, rb, crow_a, crow_b, ia, ib, bx,
+                                       max_nbrs, cnt_a, cnt_b);
+          } else {
+            if (two)
+              b_test_held<false, true>(hj, hr, nch, total, wb.list[0], wb.list[1], ra, rb, crow_a, crow_b, ia, ib, bx,
+                                       max_nbrs, cnt_a, cnt_b);
+            else
+              b_test_held<false, false>(hj, hr, nch, total, wb.list[0], wb.list[1], ra, rb, crow_a, crow_b, ia, ib,
+                                        bx, max_nbrs, cnt_a, cnt_b);
+          }
+        } else if (wraps) {
           if (two)
             b_scan_window<true, true>(wb, wb.list[0], wb.list[1], st.r, rp.slot0, total, ra, rb, crow_a, crow_b, ia,
                                       ib, bx, max_nbrs, cnt_a, cnt_b);
