@@ -687,17 +687,22 @@ __device__ __forceinline__ void slot_terms(const BatchArgs& a, const rumdx_repli
     }
     return;
   }
+  // eight list entries (and their positions) in flight per pass: the list
+  // reads come from HBM, and fewer waves than that would leave their latency
+  // exposed at one staged workgroup per CU
   const size_t stride = static_cast<size_t>(a.nslots) << lgp;
   const uint32_t* col = nbl + static_cast<size_t>(part) * a.nslots + i;
   uint32_t k = part;
-  for (; k + 3 * step < nn; k += 4 * step) {
-    const uint32_t j0 = col[0], j1 = col[stride], j2 = col[2 * stride], j3 = col[3 * stride];
-    col += 4 * stride;
-    const int4 f0 = pos(j0), f1 = pos(j1), f2 = pos(j2), f3 = pos(j3);
-    pair_b<CUT>(fi, f0, unit, prow, acc);
-    pair_b<CUT>(fi, f1, unit, prow, acc);
-    pair_b<CUT>(fi, f2, unit, prow, acc);
-    pair_b<CUT>(fi, f3, unit, prow, acc);
+  for (; k + 7 * step < nn; k += 8 * step) {
+    uint32_t j[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) j[u] = col[u * stride];
+    col += 8 * stride;
+    int4 fj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) fj[u] = pos(j[u]);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pair_b<CUT>(fi, fj[u], unit, prow, acc);
   }
   for (; k < nn; k += step, col += stride) pair_b<CUT>(fi, pos(*col), unit, prow, acc);
 }
