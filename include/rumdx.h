@@ -533,6 +533,7 @@ typedef struct rumdx_batch {
   const uint32_t* force_work_dev; /* force_work_n x 4 */
   uint32_t force_work_n;
   uint32_t max_rep_n;            /* particles of the largest replica */
+  uint32_t force_work_whole;     /* nonzero: every work item is one whole replica (rumdx_batch_forces_nvt needs it) */
   /* Nonzero: rumdx_batch_update_lists sorts each rebuilding replica with one
    * workgroup in LDS (its cell counts, their scan and the stable placement),
    * which needs 4 (max_rep_n + 2 max_rep_cells) bytes of LDS, at most 64 KiB;
@@ -541,6 +542,14 @@ typedef struct rumdx_batch {
   uint32_t max_rep_cells;
 } rumdx_batch;
 
+/* rumdx_batch_forces followed by rumdx_batch_nvt_step in one launch (and the
+ * thermostat's finishing kernel): with a staged work list of whole replicas
+ * (force_work_whole), each workgroup computes its replica's forces and then
+ * advances it in place, with the same per-256-slot kinetic-energy partials as
+ * rumdx_batch_nvt_step; the results are the same bits as the two calls'.  No
+ * counterpart in the reference. */
+int rumdx_batch_forces_nvt(rumdx_batch* b, const float* params_dev, int cutoff_method, float dt,
+                           void* workspace_dev, void* stream);
 /* Bytes of device workspace rumdx_batch_nvt_step / rumdx_batch_energies need. */
 size_t rumdx_batch_workspace_bytes(uint32_t nslots);
 /* Marks every replica's list stale. */

@@ -20,6 +20,8 @@
 // memory.  Lists are u32 slots, column-major over the whole batch (entry k
 // of slot i at k*nslots + i).  Only replicas whose rebuild flag is set run
 // the build kernels; the others exit at once.
+#include <algorithm>
+
 #include "leapfrog.h"
 #include "rumdx_common.h"
 
@@ -756,14 +758,31 @@ __global__ __launch_bounds__(kBlock) void k_b_force(BatchArgs a, const uint32_t*
 // positions into LDS, and walks the item's slots kStagedBlock >> lgp at a
 // time, reading every neighbour from LDS.  Lane mapping and summation order
 // are those of k_b_force, so the forces are the same bits.
+//
+// With FUSE every work item is a whole replica, and the workgroup then also
+// takes the replica's NVT step (k_b_nvt's per-particle update and its
+// per-256-slot kinetic-energy partials, in the same order): nothing else
+// reads this replica's positions during the launch, so they are updated in
+// place.
 constexpr uint32_t kStagedBlock = 1024;
 static_assert(kStagedBlock <= kMaxWorkgroup, "staged force workgroup");
-template <int CUT>
+static_assert(kStagedBlock % kBlock == 0, "whole integrator blocks per pass");
+
+struct BatchNvt {
+  const rumdx_nh_state* nh;
+  const float4* last_r;
+  double* partials;  // one per 256-slot block, as k_b_nvt writes them
+  float dt;
+  float skin2_over_4;
+};
+
+template <int CUT, bool FUSE = false>
 __global__ __launch_bounds__(kStagedBlock) void k_b_force_staged(BatchArgs a, const uint32_t* __restrict__ nbl,
                                                                  const uint32_t* __restrict__ num_nbrs,
                                                                  uint32_t max_nbrs, CellArrays cells,
                                                                  const float* __restrict__ params, uint32_t nt,
-                                                                 uint32_t lgp, const uint4* __restrict__ work) {
+                                                                 uint32_t lgp, const uint4* __restrict__ work,
+                                                                 BatchNvt nv) {
   extern __shared__ float4 s_dyn[];
   float4* s_tab = s_dyn;
   int4* s_fix = reinterpret_cast<int4*>(s_dyn + 2 * nt * nt);
@@ -784,6 +803,42 @@ __global__ __launch_bounds__(kStagedBlock) void k_b_force_staged(BatchArgs a, co
     }
     combine_slot_lanes(acc, lgp);
     if (live && part == 0) store_slot(a, i, acc);
+  }
+  if (FUSE) {
+    // the replica's forces are stored; its slots in 256-slot blocks, each
+    // block's m v^2 summed as k_b_nvt's block_sum does (its four waves' sums
+    // in order)
+    __syncthreads();
+    double* s_w = reinterpret_cast<double*>(s_dyn);  // the staging is no longer read
+    const Box bx = rep_box(rp);
+    const float ps = rp.omega2 > 0.f ? static_cast<float>(nv.nh[wk.x].ps) : 0.f;
+    const uint32_t nb = (rp.n + kBlock - 1) / kBlock;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kStagedBlock / kBlock) {
+      const uint32_t i = rp.slot0 + b0 * kBlock + threadIdx.x;
+      float ke2 = 0.f;
+      if (i - rp.slot0 < rp.n) {
+        float4 r = a.r[i], v = a.v[i];
+        float4 lo = make_float4(0.f, 0.f, 0.f, 0.f);
+        float3 shift;
+        ke2 = nvt_particle<false>(r, v, a.f[i], shift, lo, ps, nv.dt, bx);
+        a.r[i] = r;
+        a.fixed[i] = to_fixed(r, fixed_scale(rp));
+        a.v[i] = v;
+        add_image_shift(a.im, i, shift);
+        if (min_image_d2_f32(r, nv.last_r[i], bx) > nv.skin2_over_4) a.flags[kRepFlags * wk.x + kRebuild] = 1u;
+      }
+      const double wsum = wave_sum(static_cast<double>(ke2));
+      if (lane == 0) s_w[wave] = wsum;
+      __syncthreads();
+      const uint32_t q = threadIdx.x;  // block b0 + q, q < 4
+      if (q < kStagedBlock / kBlock && b0 + q < nb) {
+        double sum = 0.0;
+        for (uint32_t k = 0; k < kBlock / 64; ++k) sum += s_w[q * (kBlock / 64) + k];
+        nv.partials[rp.slot0 / kBlock + b0 + q] = sum;
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -952,46 +1007,74 @@ extern "C" int rumdx_batch_update_lists(rumdx_batch* b, void* stream) {
   return check_launch("rumdx_batch_update_lists");
 }
 
-extern "C" int rumdx_batch_forces(rumdx_batch* b, const float* params_dev, int cutoff_method, void* stream) {
-  int rc = check_batch(b, "rumdx_batch_forces");
+namespace rumdx {
+namespace {
+
+// Argument checks shared by the batch force entry points; RUMDX_OK or the
+// failure code (with the message set).
+int check_batch_forces(const rumdx_batch* b, const float* params_dev, const char* entry) {
+  const std::string e(entry);
+  int rc = check_batch(b, entry);
   if (rc != RUMDX_OK) return rc;
   if (params_dev == nullptr || b->nbl_dev == nullptr || b->num_nbrs_dev == nullptr || b->fixed_dev == nullptr)
-    return fail(RUMDX_E_INVALID, "rumdx_batch_forces: null params, list or fixed-point positions");
-  if (b->ntypes == 0 || b->ntypes > RUMDX_MAX_TYPES) return fail(RUMDX_E_INVALID, "rumdx_batch_forces: ntypes out of range");
+    return fail(RUMDX_E_INVALID, e + ": null params, list or fixed-point positions");
+  if (b->ntypes == 0 || b->ntypes > RUMDX_MAX_TYPES) return fail(RUMDX_E_INVALID, e + ": ntypes out of range");
   if (b->slot_cell_dev == nullptr || b->cell_start_dev == nullptr || b->cell_n_dev == nullptr || b->max_nbrs == 0)
-    return fail(RUMDX_E_INVALID, "rumdx_batch_forces: cell arrays not allocated");
+    return fail(RUMDX_E_INVALID, e + ": cell arrays not allocated");
   const uint32_t lpp = b->lanes_per_particle;
   if (lpp > 8u || (lpp & (lpp - 1u)) != 0u)
-    return fail(RUMDX_E_INVALID, "rumdx_batch_forces: lanes_per_particle must be 0, 1, 2, 4 or 8");
+    return fail(RUMDX_E_INVALID, e + ": lanes_per_particle must be 0, 1, 2, 4 or 8");
+  if (b->force_work_dev != nullptr && (b->force_work_n == 0 || b->max_rep_n == 0))
+    return fail(RUMDX_E_INVALID, e + ": empty staged work list");
+  return RUMDX_OK;
+}
+
+// The replica-staged force kernel over the batch's work list (with FUSE, the
+// NVT update too).
+template <bool FUSE>
+int launch_staged(const rumdx_batch* b, const float* params_dev, int cutoff_method, const BatchNvt& nv,
+                  hipStream_t st, const char* entry) {
+  const uint32_t lpp = b->lanes_per_particle;
+  const uint32_t lgp = lpp >= 8u ? 3u : lpp >= 4u ? 2u : lpp >= 2u ? 1u : 0u;
+  const BatchArgs a = args_of(b);
+  const CellArrays cells{b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev};
+  size_t lds_staged = sizeof(float4) * 2 * b->ntypes * b->ntypes + sizeof(int4) * b->max_rep_n;
+  if (FUSE) lds_staged = std::max(lds_staged, sizeof(double) * (kStagedBlock / 64));  // the update's wave sums
+  const auto* work = reinterpret_cast<const uint4*>(b->force_work_dev);
+  const auto launch = [&](auto kernel) -> int {
+    // above 64 KiB a workgroup's dynamic LDS must be opted into
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_staged));
+    if (e != hipSuccess)
+      return fail(RUMDX_E_INVALID, std::string(entry) + ": staged LDS of " + std::to_string(lds_staged) +
+                                       " bytes refused: " + hipGetErrorString(e));
+    hipLaunchKernelGGL(kernel, dim3(b->force_work_n), dim3(kStagedBlock), lds_staged, st, a, b->nbl_dev,
+                       b->num_nbrs_dev, b->max_nbrs, cells, params_dev, b->ntypes, lgp, work, nv);
+    return check_launch(entry);
+  };
+  switch (cutoff_method) {
+    case RUMDX_CUTOFF_NS: return launch(k_b_force_staged<RUMDX_CUTOFF_NS, FUSE>);
+    case RUMDX_CUTOFF_SP: return launch(k_b_force_staged<RUMDX_CUTOFF_SP, FUSE>);
+    case RUMDX_CUTOFF_SF: return launch(k_b_force_staged<RUMDX_CUTOFF_SF, FUSE>);
+    default: return fail(RUMDX_E_INVALID, std::string(entry) + ": invalid cutoff method");
+  }
+}
+
+}  // namespace
+}  // namespace rumdx
+
+extern "C" int rumdx_batch_forces(rumdx_batch* b, const float* params_dev, int cutoff_method, void* stream) {
+  int rc = check_batch_forces(b, params_dev, "rumdx_batch_forces");
+  if (rc != RUMDX_OK) return rc;
+  const uint32_t lpp = b->lanes_per_particle;
   const uint32_t lgp = lpp >= 8u ? 3u : lpp >= 4u ? 2u : lpp >= 2u ? 1u : 0u;
   const BatchArgs a = args_of(b);
   const hipStream_t st = as_stream(stream);
   const uint32_t nblk = (b->nslots / kBlock) << lgp;
   const size_t lds = sizeof(float4) * 2 * b->ntypes * b->ntypes;
   const CellArrays cells{b->slot_cell_dev, b->cell_start_dev, b->cell_n_dev};
-  if (b->force_work_dev != nullptr) {
-    if (b->force_work_n == 0 || b->max_rep_n == 0)
-      return fail(RUMDX_E_INVALID, "rumdx_batch_forces: empty staged work list");
-    const size_t lds_staged = lds + sizeof(int4) * b->max_rep_n;
-    const auto* work = reinterpret_cast<const uint4*>(b->force_work_dev);
-    const auto launch = [&](auto kernel) -> int {
-      // above 64 KiB a workgroup's dynamic LDS must be opted into
-      const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_staged));
-      if (e != hipSuccess)
-        return fail(RUMDX_E_INVALID, std::string("rumdx_batch_forces: staged LDS of ") + std::to_string(lds_staged) +
-                                         " bytes refused: " + hipGetErrorString(e));
-      hipLaunchKernelGGL(kernel, dim3(b->force_work_n), dim3(kStagedBlock), lds_staged, st, a, b->nbl_dev,
-                         b->num_nbrs_dev, b->max_nbrs, cells, params_dev, b->ntypes, lgp, work);
-      return check_launch("rumdx_batch_forces");
-    };
-    switch (cutoff_method) {
-      case RUMDX_CUTOFF_NS: return launch(k_b_force_staged<RUMDX_CUTOFF_NS>);
-      case RUMDX_CUTOFF_SP: return launch(k_b_force_staged<RUMDX_CUTOFF_SP>);
-      case RUMDX_CUTOFF_SF: return launch(k_b_force_staged<RUMDX_CUTOFF_SF>);
-      default: return fail(RUMDX_E_INVALID, "rumdx_batch_forces: invalid cutoff method");
-    }
-  }
+  if (b->force_work_dev != nullptr)
+    return launch_staged<false>(b, params_dev, cutoff_method, BatchNvt{}, st, "rumdx_batch_forces");
   switch (cutoff_method) {
     case RUMDX_CUTOFF_NS:
       hipLaunchKernelGGL(k_b_force<RUMDX_CUTOFF_NS>, dim3(nblk), dim3(kBlock), lds, st, a, b->nbl_dev, b->num_nbrs_dev,
@@ -1022,6 +1105,24 @@ extern "C" int rumdx_batch_nvt_step(rumdx_batch* b, float dt, void* workspace_de
                      reinterpret_cast<const float4*>(b->last_r_dev), dt, 0.25f * b->skin * b->skin, partials);
   hipLaunchKernelGGL(k_b_nvt_finish, dim3(b->nrep), dim3(kBlock), 0, st, b->rep_dev, partials, b->nh_dev, dt);
   return check_launch("rumdx_batch_nvt_step");
+}
+
+extern "C" int rumdx_batch_forces_nvt(rumdx_batch* b, const float* params_dev, int cutoff_method, float dt,
+                                      void* workspace_dev, void* stream) {
+  int rc = check_batch_forces(b, params_dev, "rumdx_batch_forces_nvt");
+  if (rc != RUMDX_OK) return rc;
+  if (b->force_work_dev == nullptr || b->force_work_whole == 0)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_forces_nvt: needs a staged work list of whole replicas");
+  if (workspace_dev == nullptr || b->nh_dev == nullptr || b->last_r_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_batch_forces_nvt: null workspace, thermostat state or last_r");
+  const hipStream_t st = as_stream(stream);
+  double* partials = static_cast<double*>(workspace_dev);
+  const BatchNvt nv{b->nh_dev, reinterpret_cast<const float4*>(b->last_r_dev), partials, dt,
+                    0.25f * b->skin * b->skin};
+  rc = launch_staged<true>(b, params_dev, cutoff_method, nv, st, "rumdx_batch_forces_nvt");
+  if (rc != RUMDX_OK) return rc;
+  hipLaunchKernelGGL(k_b_nvt_finish, dim3(b->nrep), dim3(kBlock), 0, st, b->rep_dev, partials, b->nh_dev, dt);
+  return check_launch("rumdx_batch_forces_nvt");
 }
 
 extern "C" int rumdx_batch_energies(rumdx_batch* b, float dt, double* out_dev, void* workspace_dev, void* stream) {

@@ -472,6 +472,9 @@ PYBIND11_MODULE(_rumd, m) {
       .def("SetWorkgroupSort", &ReplicaBatch::SetWorkgroupSort, py::arg("on"))
       .def("GetWorkgroupSort", &ReplicaBatch::GetWorkgroupSort)
       .def("UsesWorkgroupSort", &ReplicaBatch::UsesWorkgroupSort)
+      .def("SetFusedSteps", &ReplicaBatch::SetFusedSteps, py::arg("on"))
+      .def("GetFusedSteps", &ReplicaBatch::GetFusedSteps)
+      .def("UsesFusedSteps", &ReplicaBatch::UsesFusedSteps)
       .def("GetLanesPerParticle", &ReplicaBatch::GetLanesPerParticle)
       .def("GetBoxLength", &ReplicaBatch::GetBoxLength, py::arg("replica"))
       .def("GetThermostat", &ReplicaBatch::GetThermostat, py::arg("replica"))

@@ -110,7 +110,48 @@ void ReplicaBatch::SetMaxNumNbrs(unsigned m) {
 void ReplicaBatch::SetStagedForces(bool on) {
   staged_forces_ = on;
   if (!allocated_) return;
-  desc_.force_work_dev = (on && force_work_.size()) ? force_work_.get() : nullptr;
+  BuildForceWork();
+  InvalidateGraphs();
+}
+
+// Staged forces: work items of one replica each, the largest replicas first,
+// when the largest fits one workgroup's LDS: whole replicas for fused steps,
+// else slot ranges of at most 2048.
+void ReplicaBatch::BuildForceWork() {
+  force_work_.release();
+  desc_.force_work_dev = nullptr;
+  desc_.force_work_n = 0;
+  desc_.force_work_whole = 0;
+  desc_.max_rep_n = max_rep_n_;
+  int dev = 0, optin = 0;
+  hip_check(hipGetDevice(&dev), "ReplicaBatch", "Allocate");
+  hip_check(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev), "ReplicaBatch", "Allocate");
+  const size_t staged_lds = 16ull * max_rep_n_ + 32ull * ntypes_ * ntypes_;
+  if (!staged_forces_ || staged_lds > static_cast<size_t>(optin)) return;
+  const bool whole = fused_steps_;
+  const uint32_t chunk = whole ? 0xffffffffu : 2048u;
+  const unsigned nrep = static_cast<unsigned>(reps_.size());
+  std::vector<uint32_t> order(nrep);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t x, uint32_t y) { return reps_[x].types.size() > reps_[y].types.size(); });
+  std::vector<uint32_t> hw;
+  for (uint32_t r : order) {
+    const uint32_t n = static_cast<uint32_t>(reps_[r].types.size());
+    const uint32_t len = std::max(1u, std::min(chunk, n));
+    for (uint32_t c = 0; c < n; c += len) hw.insert(hw.end(), {r, slot0_[r] + c, slot0_[r] + std::min(n, c + len), 0u});
+  }
+  force_work_.resize(hw.size());
+  force_work_.upload(hw.data(), hw.size(), stream_);
+  desc_.force_work_dev = force_work_.get();
+  desc_.force_work_n = static_cast<uint32_t>(hw.size() / 4);
+  desc_.force_work_whole = whole ? 1u : 0u;
+}
+
+void ReplicaBatch::SetFusedSteps(bool on) {
+  fused_steps_ = on;
+  if (!allocated_) return;
+  BuildForceWork();
   InvalidateGraphs();
 }
 
@@ -361,33 +402,8 @@ void ReplicaBatch::Allocate() {
   desc_.t_idx_dev = t_idx_.get();
   desc_.cell_tmp_dev = cell_tmp_.get();
   desc_.slot_cell_dev = slot_cell_.get();
-  // staged forces: work items of at most kForceChunk slots of one replica,
-  // the largest replicas first, when the largest fits one workgroup's LDS
   max_rep_n_ = max_n;
-  force_work_.release();
-  int dev = 0, optin = 0;
-  hip_check(hipGetDevice(&dev), "ReplicaBatch", "Allocate");
-  hip_check(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev), "ReplicaBatch", "Allocate");
-  const size_t staged_lds = 16ull * max_n + 32ull * ntypes_ * ntypes_;
-  if (staged_lds <= static_cast<size_t>(optin)) {
-    constexpr uint32_t kForceChunk = 2048;
-    std::vector<uint32_t> order(nrep);
-    std::iota(order.begin(), order.end(), 0u);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t x, uint32_t y) { return reps_[x].types.size() > reps_[y].types.size(); });
-    std::vector<uint32_t> hw;
-    for (uint32_t r : order) {
-      const uint32_t n = static_cast<uint32_t>(reps_[r].types.size());
-      for (uint32_t c = 0; c < n; c += kForceChunk) {
-        hw.insert(hw.end(), {r, slot0_[r] + c, slot0_[r] + std::min(n, c + kForceChunk), 0u});
-      }
-    }
-    force_work_.resize(hw.size());
-    force_work_.upload(hw.data(), hw.size(), stream_);
-    desc_.force_work_n = static_cast<uint32_t>(hw.size() / 4);
-    desc_.max_rep_n = max_n;
-  }
-  desc_.force_work_dev = (staged_forces_ && force_work_.size()) ? force_work_.get() : nullptr;
+  BuildForceWork();
   // the one-workgroup rebuild sort when a replica's counts, starts and slots
   // fit 64 KiB of LDS
   uint64_t max_cells = 0;
@@ -405,6 +421,13 @@ void ReplicaBatch::CalcF() {
 }
 
 void ReplicaBatch::Step() {
+  if (!allocated_) Allocate();
+  if (UsesFusedSteps()) {
+    rumdx_check(rumdx_batch_update_lists(&desc_, stream_), "ReplicaBatch", "RunSteps");
+    rumdx_check(rumdx_batch_forces_nvt(&desc_, params_.get(), cutoff_method_, dt_, workspace_.get(), stream_),
+                "ReplicaBatch", "RunSteps");
+    return;
+  }
   CalcF();
   rumdx_check(rumdx_batch_nvt_step(&desc_, dt_, workspace_.get(), stream_), "ReplicaBatch", "RunSteps");
 }

@@ -44,6 +44,12 @@ class ReplicaBatch {
   void SetWorkgroupSort(bool on);
   bool GetWorkgroupSort() const { return workgroup_sort_; }
   bool UsesWorkgroupSort() const { return desc_.max_rep_cells != 0; }
+  // Steps with the forces and the NVT update in one launch
+  // (rumdx_batch_forces_nvt; on by default with staged forces, whose work
+  // items are then whole replicas).  The same bits either way.
+  void SetFusedSteps(bool on);
+  bool GetFusedSteps() const { return fused_steps_; }
+  bool UsesFusedSteps() const { return desc_.force_work_dev != nullptr && desc_.force_work_whole != 0; }
   // Device copies of the dynamic state (positions, velocities, images, slot
   // ids, forces, thermostats, flags); RestoreState puts them back and marks
   // every list stale, so trial runs leave no trace in the trajectory.
@@ -112,6 +118,8 @@ class ReplicaBatch {
   bool allocated_ = false;
   bool staged_forces_ = true;
   bool workgroup_sort_ = true;
+  bool fused_steps_ = true;
+  void BuildForceWork();
   uint32_t max_rep_n_ = 0, max_rep_cells_ = 0;
   hipStream_t stream_ = nullptr;
 

@@ -222,6 +222,36 @@ def test_batch_staged_forces_equal_gathered(tp):
     assert force_errors(res[1][0][:, :3], f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
 
 
+@pytest.mark.parametrize("tp", [1, 2])
+def test_batch_fused_steps_equal_two_launches(tp):
+    """Forces and the NVT update in one launch per step (whole-replica work
+    items, rumdx_batch_forces_nvt) against the staged forces and the separate
+    integrator (items of at most 2048 slots): the same bits for positions,
+    velocities, forces and thermostat states over 60 steps with rebuilds,
+    thermostatted and NVE replicas mixed."""
+    sizes = [256, 1500, 2049, 5000, 8192]
+    systems = _systems(sizes, seed0=950)
+    temps = [2.0, 0.5, 1.0, 0.6, 0.9]
+    out = []
+    for fused in (False, True):
+        b = make_batch(systems, temps)
+        b.SetThermostat(2, temps[2], 0.0)  # NVE
+        b.SetLanesPerParticle(tp)
+        b.SetFusedSteps(fused)
+        print(f"[case] batch fused steps={fused} lanes={tp}", flush=True)
+        b.RunSteps(60, 10)
+        b.RunSteps(3, 1)  # eager steps too
+        assert b.UsesFusedSteps() == fused and b.UsesStagedForces()
+        out.append(([b.GetPositions(r) for r in range(len(sizes))], [b.GetVelocities(r) for r in range(len(sizes))],
+                    [b.GetForcesAndEnergies(r) for r in range(len(sizes))], b.GetThermostatStates(),
+                    b.GetEnergies()))
+    for k, name in enumerate(("positions", "velocities", "forces")):
+        for r in range(len(sizes)):
+            np.testing.assert_array_equal(out[0][k][r], out[1][k][r], err_msg=f"{name} {r}")
+    np.testing.assert_array_equal(np.asarray(out[0][3]), np.asarray(out[1][3]), err_msg="thermostat states")
+    np.testing.assert_array_equal(np.asarray(out[0][4]), np.asarray(out[1][4]), err_msg="energies")
+
+
 def test_batch_workgroup_sort_equals_batch_sort():
     """Rebuild sorts by one workgroup per replica in LDS give the four-kernel
     sort's particle order, cell starts and lists: 60 steps with rebuilds of
