@@ -75,13 +75,14 @@ extern "C" {
 #define RUMDX_FLAG_REBUILD 0   /* != 0: the next build call rebuilds (updateRequired[0]) */
 #define RUMDX_FLAG_OVERFLOW 1  /* sticky: some list hit max_nbrs (updateRequired[1]) */
 #define RUMDX_FLAG_BUILDS 2    /* number of completed builds */
-#define RUMDX_FLAG_MAXLEN 3    /* longest list seen since the flag was cleared */
+#define RUMDX_FLAG_MAXLEN 3    /* longest list seen since the flag was cleared (sorted builds: see rumdx_nblist_build_stats) */
 #define RUMDX_FLAG_DONE 4      /* internal completion counter, zero between calls */
-#define RUMDX_FLAG_MAXHALO 5   /* largest tile halo (particles) of the last sorted build */
+#define RUMDX_FLAG_MAXHALO 5   /* largest tile halo (particles) of the last sorted build, after rumdx_nblist_build_stats */
 #define RUMDX_FLAG_ERROR 6     /* nonzero: the last sorted build is unusable, RUMDX_BUILD_ERR_* bits */
 #define RUMDX_FLAG_EXCL_PENDING 7 /* internal: set by a completed build, cleared once its lists' exclusions are applied */
-#define RUMDX_FLAG_MAXTILE 8   /* most particles in one cell tile, last sorted build */
+#define RUMDX_FLAG_MAXTILE 8   /* most particles in one cell tile, last sorted build, after rumdx_nblist_build_stats */
 #define RUMDX_FLAG_PRUNE 9     /* != 0: the next rumdx_nblist_prune rebuilds the inner lists */
+#define RUMDX_FLAG_STATS 10    /* internal: set by a completed sorted build, cleared by rumdx_nblist_build_stats */
 /* Bits of flags[RUMDX_FLAG_ERROR]. */
 #define RUMDX_BUILD_ERR_HALO 1u     /* a tile halo held more than 65535 particles */
 #define RUMDX_BUILD_ERR_MOVED 2u    /* rebuild_sorted: a particle moved more than one cell since the last build */
@@ -257,6 +258,17 @@ int rumdx_nblist_build_n2(const rumdx_particles* p, rumdx_nblist* nb, const floa
  * the cells and by original slot inside a cell (a stable sort).  Does nothing
  * on the device unless flags[REBUILD] is set. */
 int rumdx_nblist_build_sorted(rumdx_particles* p, rumdx_nblist* nb, const float box[8], void* stream);
+
+/* The host's read of the build results before it grows the list
+ * (src/NeighborList.cu:660-670): for the sorted method, gathers the last
+ * build's longest list into flags[MAXLEN] (the maximum with its old value),
+ * its largest tile halo into flags[MAXHALO] and its most crowded tile into
+ * flags[MAXTILE] from the per-tile records the build left.  A one-workgroup
+ * kernel that does nothing unless a sorted build completed since the last
+ * call; the build itself leaves this out, so that the steps between host reads
+ * carry no launch for it.  Call it before reading those flags.  Returns
+ * RUMDX_OK without a launch when the list has no sorted build arrays. */
+int rumdx_nblist_build_stats(const rumdx_particles* p, rumdx_nblist* nb, void* stream);
 
 /* The same result as rumdx_nblist_build_sorted -- particle order, cells and
  * lists bit for bit -- for particles that a previous sorted build of this list

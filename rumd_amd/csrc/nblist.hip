@@ -624,14 +624,14 @@ __global__ __launch_bounds__(kBlock) void k_tile_resort(uint32_t* __restrict__ f
   }
 }
 
-// Completes a sorted build (after the tile build): longest list, largest halo
-// and largest tile from the per-tile records, the build count, and the
-// rebuild request cleared.  One workgroup; the kernel boundary orders it after
-// every tile.
-__global__ __launch_bounds__(kBlock) void k_build_finish(uint32_t* __restrict__ flags,
-                                                         const uint4* __restrict__ tile_info,
-                                                         const uint32_t* __restrict__ tile_max, uint32_t ntiles) {
-  if (!rebuild_pending(flags)) return;
+// The statistics of the last sorted build from its per-tile records: longest
+// list (kept as the maximum with the old value), largest halo and largest
+// tile.  One workgroup, launched on request between steps rather than after
+// every build, so captured steps carry no launch for it.
+__global__ __launch_bounds__(kBlock) void k_build_stats(uint32_t* __restrict__ flags,
+                                                        const uint4* __restrict__ tile_info,
+                                                        const uint32_t* __restrict__ tile_max, uint32_t ntiles) {
+  if (flags[RUMDX_FLAG_STATS] == 0u) return;
   __shared__ uint32_t s_max[kBlock / 64];
   uint32_t ml = 0, mh = 0, mt = 0;
   for (uint32_t t = threadIdx.x; t < ntiles; t += kBlock) {
@@ -647,10 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_build_finish(uint32_t* __restrict__ 
     flags[RUMDX_FLAG_MAXLEN] = ml > flags[RUMDX_FLAG_MAXLEN] ? ml : flags[RUMDX_FLAG_MAXLEN];
     flags[RUMDX_FLAG_MAXHALO] = mh;
     flags[RUMDX_FLAG_MAXTILE] = mt;
-    flags[RUMDX_FLAG_REBUILD] = 0u;
-    flags[RUMDX_FLAG_BUILDS] += 1u;
-    flags[RUMDX_FLAG_EXCL_PENDING] = 1u;
-    flags[RUMDX_FLAG_PRUNE] = 1u;
+    flags[RUMDX_FLAG_STATS] = 0u;
   }
 }
 
@@ -687,18 +684,19 @@ __device__ __forceinline__ uint32_t append_chunk(bool acc, uint32_t entry, uint3
 
 // One particle's test of one chunk of candidates: the accepted ones go to its
 // staged list in candidate order (ballot + lane count); returns the new count.
-// The particle's own entry is rejected through an infinite distance, so the
-// acceptance is one compare whose lane mask is the ballot.
+// The loop is short of scalar issue slots as much as of vector ones: the two
+// compares give their lane masks directly (one scalar AND, no vector select)
+// and the append is a plain exec-masked store.
 template <bool WRAPS>
 __device__ __forceinline__ uint32_t test_chunk(float4 pi, float4 rj, uint32_t e, float cut, uint32_t self,
                                                const Box& bx, uint16_t* list, uint32_t cnt) {
   const float d2 = WRAPS ? min_image_d2_f32(pi, rj, bx) : plain_d2_f32(pi, rj);
-  const bool acc = (e != self ? d2 : __builtin_inff()) < cut;
-  const uint64_t mask = __builtin_amdgcn_ballot_w64(acc);
+  const bool near = d2 < cut, other = e != self;
+  const uint64_t mask = __builtin_amdgcn_ballot_w64(near) & __builtin_amdgcn_ballot_w64(other);
   const uint32_t k = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
   uint16_t* at = list + cnt;  // cnt is uniform: the base stays scalar
-  if (acc) at[k] = static_cast<uint16_t>(e);
+  if (near && other) at[k] = static_cast<uint16_t>(e);
   return cnt + static_cast<uint32_t>(__builtin_popcountll(mask));
 }
 
@@ -972,6 +970,12 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
                                                              uint32_t* __restrict__ num_nbrs, uint32_t nvp,
                                                              uint32_t max_nbrs) {
   if (!rebuild_pending(flags)) return;
+  // Arrival at the start: its round trip overlaps the halo loads, and the
+  // block that arrives last (every block has read the request by then)
+  // completes the build at its end.
+  uint32_t arrival = 0;
+  if (threadIdx.x == 0) arrival = __hip_atomic_fetch_add(flags + RUMDX_FLAG_DONE, 1u, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
   __shared__ HaloLds L;
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
   __shared__ __align__(16) uint32_t s_key[kOrderMax];  // list lengths, then sort keys (same index, same thread)
@@ -1075,7 +1079,18 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
     o.lane_order[h.slot0 + rk] = static_cast<uint16_t>(k);
   }
   if (threadIdx.x == 0) {
-    s.tile_max[t] = longest;  // k_build_finish reduces these
+    // per-tile records (longest list here, halo and slot range in tile_info),
+    // reduced by k_build_stats when the host asks; plain stores, so no block
+    // waits on a same-address atomic
+    s.tile_max[t] = longest;
+    if (arrival == gridDim.x - 1) {
+      flags[RUMDX_FLAG_DONE] = 0u;
+      flags[RUMDX_FLAG_REBUILD] = 0u;
+      flags[RUMDX_FLAG_BUILDS] += 1u;
+      flags[RUMDX_FLAG_EXCL_PENDING] = 1u;
+      flags[RUMDX_FLAG_PRUNE] = 1u;
+      flags[RUMDX_FLAG_STATS] = 1u;
+    }
     if (h.npart > 65535u) atomicOr(flags + RUMDX_FLAG_ERROR, RUMDX_BUILD_ERR_HALO);
     // wrapping axes, and whether the shifted frame serves them
     const int nc[3] = {tg.nc0, tg.nc1, tg.nc2};
@@ -1453,8 +1468,6 @@ void launch_tile_build(rumdx_particles* p, rumdx_nblist* nb, const Box& bx, cons
   hipLaunchKernelGGL(k_build_tiles, dim3(nb->ntiles), dim3(kBuildBlock), sizeof(float4) * (nb->halo_cap + 1u), st,
                      nb->flags_dev, s, tg, nb->cut_skin2_dev, nb->ntypes, bx, home, nb->nbl16_dev, nb->num_nbrs_dev,
                      p->nvp, nb->max_nbrs);
-  hipLaunchKernelGGL(k_build_finish, dim3(1), dim3(kBlock), 0, st, nb->flags_dev,
-                     reinterpret_cast<const uint4*>(nb->tile_info_dev), s.tile_max, nb->ntiles);
   if (nb->excl_start_dev != nullptr)
     hipLaunchKernelGGL(k_apply_exclusions, dim3(nb->ntiles), dim3(kBlock), 0, st, nb->flags_dev,
                        reinterpret_cast<const uint4*>(nb->tile_info_dev), nb->tile_halo_dev, nb->halo_cap,
@@ -1518,6 +1531,19 @@ extern "C" int rumdx_nblist_rebuild_sorted(rumdx_particles* p, rumdx_nblist* nb,
   hipLaunchKernelGGL(k_tile_resort, dim3(nb->ntiles), dim3(kBlock), 0, st, nb->flags_dev, s, tg, bx, old, p->n);
   launch_tile_build(p, nb, bx, s, st);
   return check_launch("rumdx_nblist_rebuild_sorted");
+}
+
+extern "C" int rumdx_nblist_build_stats(const rumdx_particles* p, rumdx_nblist* nb, void* stream) {
+  if (p == nullptr || nb == nullptr || nb->flags_dev == nullptr)
+    return fail(RUMDX_E_INVALID, "rumdx_nblist_build_stats: null argument");
+  if (nb->nbl16_dev == nullptr || nb->tile_info_dev == nullptr || nb->sort_scratch_dev == nullptr || nb->ntiles == 0)
+    return RUMDX_OK;  // no sorted build: the N2 build keeps the flags itself
+  const uint32_t ncells =
+      static_cast<uint32_t>(nb->nc[0]) * static_cast<uint32_t>(nb->nc[1]) * static_cast<uint32_t>(nb->nc[2]);
+  const SortScratch s = carve(nb->sort_scratch_dev, p->nvp, ncells);
+  hipLaunchKernelGGL(k_build_stats, dim3(1), dim3(kBlock), 0, as_stream(stream), nb->flags_dev,
+                     reinterpret_cast<const uint4*>(nb->tile_info_dev), s.tile_max, nb->ntiles);
+  return check_launch("rumdx_nblist_build_stats");
 }
 
 extern "C" int rumdx_tile_table(const int32_t nc[3], const uint32_t* rank, uint32_t* first_rank_out,

@@ -222,11 +222,12 @@ __device__ __forceinline__ void stage_halo(uint32_t t, const TileGeom& g, const 
       for (int u = 0; u < 4; ++u) {
         const uint32_t q = q0 + u * blockDim.x + threadIdx.x;
         const uint32_t c = q < tot ? cellof[q] : 0u;
-        src[u] = L.start[c] + (q - L.off[c]);
+        // past the halo, slot 0 stands in: every load is unconditional, so
+        // the four stay in registers and in flight together
+        src[u] = q < tot ? L.start[c] + (q - L.off[c]) : 0u;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (q0 + u * blockDim.x + threadIdx.x < tot) v[u] = pos[src[u]];
+      for (int u = 0; u < 4; ++u) v[u] = pos[src[u]];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const uint32_t q = q0 + u * blockDim.x + threadIdx.x;

@@ -392,10 +392,17 @@ void NeighborList::Settle() {
   }
 }
 
+void NeighborList::ReadFlags(uint32_t* f) {
+  if (desc_.nbl16_dev != nullptr)
+    rumdx_check(rumdx_nblist_build_stats(sample_->GetParticleData().Descriptor(), &desc_, sample_->GetStream()),
+                "NeighborList", "UpdateNBlist");
+  flags_.download(f, RUMDX_NB_FLAGS, sample_->GetStream());
+}
+
 bool NeighborList::HandleOverflow() {
   if (!allocated_) return false;
   uint32_t f[RUMDX_NB_FLAGS];
-  flags_.download(f, RUMDX_NB_FLAGS, sample_->GetStream());
+  ReadFlags(f);
   if (f[RUMDX_FLAG_ERROR]) {
     std::string why;
     if (f[RUMDX_FLAG_ERROR] & RUMDX_BUILD_ERR_HALO) why += " a cell tile's halo holds more than 65535 particles;";
@@ -487,7 +494,7 @@ void NeighborList::DownloadCounts(std::vector<uint32_t>& out) {
 unsigned NeighborList::GetActualMaxNumNbrs() {
   if (!allocated_) return 0;
   uint32_t f[RUMDX_NB_FLAGS];
-  flags_.download(f, RUMDX_NB_FLAGS, sample_->GetStream());
+  ReadFlags(f);
   return f[RUMDX_FLAG_MAXLEN];
 }
 
@@ -509,8 +516,8 @@ std::vector<uint32_t> NeighborList::TileStats() {
   if (!allocated_ || desc_.nbl16_dev == nullptr) return {};
   const hipStream_t st = sample_->GetStream();
   std::vector<uint32_t> info(4ull * desc_.ntiles), f(RUMDX_NB_FLAGS);
+  ReadFlags(f.data());
   tile_info_.download(info.data(), info.size(), st);
-  flags_.download(f.data(), RUMDX_NB_FLAGS, st);
   // tile_info = {halo particles, flags | halo capacity << 16, slot0, slot1};
   // flag bit 0: fast, bits 1-6: per-axis shift below / above (RUMDX_TILE_*)
   uint32_t staged = 0, fast = 0, wrapping = 0, shifted = 0;

@@ -224,6 +224,8 @@ class NeighborList {
   rumdx_nblist* Descriptor() { return &desc_; }
 
  private:
+  // Downloads the flags after gathering the last sorted build's statistics.
+  void ReadFlags(uint32_t* f);
   void Allocate();
   void Settle();
   void Prune();

@@ -108,6 +108,7 @@ def lib() -> C.CDLL:
     L.rumdx_nblist_build_n2.argtypes = [PP, NP, f32a, vp]
     L.rumdx_nblist_build_sorted.argtypes = [PP, NP, f32a, vp]
     L.rumdx_nblist_rebuild_sorted.argtypes = [PP, NP, f32a, vp]
+    L.rumdx_nblist_build_stats.argtypes = [PP, NP, vp]
     L.rumdx_nblist_prune.argtypes = [PP, NP, f32a, vp]
     L.rumdx_pair_lj.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
     L.rumdx_pair_lj_fp64.argtypes = [PP, NP, vp, u32, f32a, i32, i32, i32, vp]
@@ -389,6 +390,10 @@ class GpuSystem:
         return self.idx.download(np.uint32, self.nvp)[: self.n]
 
     def flags_host(self):
+        """The list flags, after rumdx_nblist_build_stats has gathered the last
+        sorted build's statistics (as the host does before it grows a list)."""
+        if self.nb is not None and self.method == "sorted":
+            check(lib().rumdx_nblist_build_stats(C.byref(self.p), C.byref(self.nb), None))
         return self.flags.download(np.uint32, 16)
 
     def neighbor_lists(self, inner=False):
