@@ -338,13 +338,13 @@ def test_batch_build_per_lane_path():
             assert set(ids[off[i]:off[i + 1]].tolist()) == set(i2[o2[i]:o2[i + 1]].tolist()), (r, i)
 
 
-def test_batch_full_replica_set_sampled_against_oracle():
+def test_batch_full_replica_set_against_oracle():
     """The benchmark's own replica set (512 KA replicas, N ~ U[256, 8192],
     T ~ U[0.4, 2.0], 2.15 M particles) after 100 NVT steps in graph blocks
-    (per-replica rebuilds of hot and cold replicas): forces, energies and
-    neighbour sets of a sample of replicas -- the smallest, the largest, the
-    hottest, the coldest and every 64th -- against the oracle at their
-    current positions."""
+    (per-replica rebuilds of hot and cold replicas): the forces and energies
+    of every replica, and the neighbour sets of a sample -- the smallest, the
+    largest, the hottest, the coldest and every 64th -- against the oracle at
+    their current positions."""
     systems, temps = configs.ka_replica_set()
     assert len(systems) == 512
     b = make_batch(systems, temps)
@@ -359,11 +359,12 @@ def test_batch_full_replica_set_sampled_against_oracle():
     assert builds[int(np.argmax(temps))] > builds[int(np.argmin(temps))]
     e = b.GetEnergies()
     cut2 = configs.cut_skin2_table(ka_table(), 0.3)
-    for r in sample:
+    for r in range(len(systems)):
         s = systems[r]
         pos = b.GetPositions(r)
         pos4 = configs.pack_pos4(pos, s.types)
-        print(f"[case] replica {r}: N={s.n} T={temps[r]:.2f} builds={builds[r]}", flush=True)
+        if r in sample:
+            print(f"[case] replica {r}: N={s.n} T={temps[r]:.2f} builds={builds[r]}", flush=True)
         f = b.GetForcesAndEnergies(r)
         f_ref, vir_ref = (oracle.forces_brute if s.n <= 2000 else oracle.forces_cells)(pos4, s.box, 2, ka_table(), SF)
         err = force_errors(f[:, :3], f_ref[:, :3])
