@@ -213,23 +213,6 @@ def test_nblist_overflow_falls_back_exactly():
     assert force_errors(f[:, :3], f_ref[:, :3])["guarded_max"] <= FORCE_TOL_GUARDED
 
 
-def test_skin_trigger_and_no_rebuild():
-    sysm = configs.ka_random_lattice(10000, density=1.2, seed=9)
-    table = ka_table()
-    pos4 = configs.pack_pos4(sysm.positions, sysm.types)
-    g = GpuSystem(pos4, configs.pack_vel4(sysm.velocities, sysm.types, sysm.masses), configs.box8(sysm.box), 2,
-                  table, method="sorted", max_nbrs=200)
-    g.update_nblist()
-    assert g.flags_host()[2] == 1
-    g.update_nblist()  # nothing moved: no rebuild
-    assert g.flags_host()[2] == 1
-    r = g.get("r").copy()
-    r[17, 0] += np.float32(0.151)  # beyond skin/2 = 0.15
-    g.r.upload(np.vstack([r, np.zeros((g.nvp - g.n, 4), np.float32)]))
-    g.update_nblist()
-    assert g.flags_host()[2] == 2
-
-
 def _skin_flip(base, direction, box8, skin, s_lo, s_hi):
     """Bisects the displacement s of one particle from `base` along
     `direction` (positions rounded to float32, then wrapped into the box in
