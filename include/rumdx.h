@@ -144,7 +144,7 @@ typedef struct rumdx_nblist {
   uint32_t* cell_start_dev; /* ncells + 1, by rank: first slot of each cell */
   uint32_t* cell_count_dev; /* ncells, by rank: zero between builds */
   uint32_t* slot_cell_dev;  /* nvp: linear cell index of each slot at the last build */
-  void* sort_scratch_dev;   /* rumdx_sort_scratch_bytes(nvp, ncells) bytes */
+  void* sort_scratch_dev;   /* rumdx_sort_scratch_bytes(nvp, ncells) bytes, zero-filled before the first build */
   uint16_t* nbl16_dev;      /* sorted method: (max_nbrs rounded up to 8) x nvp halo offsets */
   uint32_t ntiles;          /* from rumdx_tile_table() */
   const uint32_t* tile_first_rank_dev; /* ntiles + 1 */

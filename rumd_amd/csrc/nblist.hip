@@ -43,7 +43,16 @@ struct SortScratch {
   uint32_t* start_next; // ncells + 1: cell starts of the build in progress (the build copies them home)
   uint32_t* tile_max;   // per tile: longest list of the last build
   unsigned long long* lookback;  // per tile: the re-sort's published counts
+  uint32_t* arrive;     // the tile build's arrival counters: kArriveGroups, then the total, one 128-byte line each
 };
+
+// The tile build counts its blocks' arrivals on kArriveGroups counters, block
+// b on counter b % kArriveGroups, each on its own 128-byte line, and the last
+// arrival of each group on a total: thousands of blocks on one address would
+// serialise at the memory side (a short build of many small tiles waited on
+// it).  The counters live in the zero-filled sort scratch and are left at zero.
+constexpr uint32_t kArriveGroups = 8;
+constexpr uint32_t kArriveStride = 32;  // u32 per 128-byte line
 
 // Grid cap of the particle-parallel rebuild kernels (they loop over the rest).
 constexpr uint32_t kChainGridCap = 1024;
@@ -73,7 +82,8 @@ SortScratch carve(void* base, uint32_t nvp, uint32_t ncells) {
   s.chunk_sum = reinterpret_cast<uint32_t*>(p); p += align256(sizeof(uint32_t) * scan_chunks(ncells));
   s.start_next = reinterpret_cast<uint32_t*>(p); p += align256(sizeof(uint32_t) * (ncells + 1ull));
   s.tile_max = reinterpret_cast<uint32_t*>(p); p += align256(sizeof(uint32_t) * ncells);  // ntiles <= ncells
-  s.lookback = reinterpret_cast<unsigned long long*>(p);
+  s.lookback = reinterpret_cast<unsigned long long*>(p); p += align256(sizeof(unsigned long long) * ncells);
+  s.arrive = reinterpret_cast<uint32_t*>(p);
   return s;
 }
 
@@ -82,7 +92,7 @@ size_t scratch_bytes(uint32_t nvp, uint32_t ncells) {
   const size_t f4n = align256(sizeof(float4) * nvp);
   return 3 * u32n + 6 * f4n + align256(sizeof(uint32_t) * scan_chunks(ncells)) +
          align256(sizeof(uint32_t) * (ncells + 1ull)) + align256(sizeof(uint32_t) * ncells) +
-         align256(sizeof(unsigned long long) * ncells);
+         align256(sizeof(unsigned long long) * ncells) + align256(sizeof(uint32_t) * kArriveStride * (kArriveGroups + 1));
 }
 
 __device__ __forceinline__ bool rebuild_pending(const uint32_t* flags) {
@@ -970,12 +980,22 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
                                                              uint32_t* __restrict__ num_nbrs, uint32_t nvp,
                                                              uint32_t max_nbrs) {
   if (!rebuild_pending(flags)) return;
-  // Arrival at the start: its round trip overlaps the halo loads, and the
-  // block that arrives last (every block has read the request by then)
-  // completes the build at its end.
-  uint32_t arrival = 0;
-  if (threadIdx.x == 0) arrival = __hip_atomic_fetch_add(flags + RUMDX_FLAG_DONE, 1u, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
+  // Arrival at the start (grouped counters, kArriveGroups): its round trip
+  // overlaps the halo loads, and the block that arrives last (every block has
+  // read the request by then) completes the build at its end.
+  bool last = false;
+  if (threadIdx.x == 0) {
+    const uint32_t g = blockIdx.x % kArriveGroups;
+    const uint32_t members = (gridDim.x - g + kArriveGroups - 1) / kArriveGroups;
+    const uint32_t groups = gridDim.x < kArriveGroups ? gridDim.x : kArriveGroups;
+    uint32_t* ctr = s.arrive + g * kArriveStride;
+    if (__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
+      __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      uint32_t* total = s.arrive + kArriveGroups * kArriveStride;
+      last = __hip_atomic_fetch_add(total, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == groups - 1;
+      if (last) __hip_atomic_store(total, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
   __shared__ HaloLds L;
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
   __shared__ __align__(16) uint32_t s_key[kOrderMax];  // list lengths, then sort keys (same index, same thread)
@@ -1083,8 +1103,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
     // reduced by k_build_stats when the host asks; plain stores, so no block
     // waits on a same-address atomic
     s.tile_max[t] = longest;
-    if (arrival == gridDim.x - 1) {
-      flags[RUMDX_FLAG_DONE] = 0u;
+    if (last) {
       flags[RUMDX_FLAG_REBUILD] = 0u;
       flags[RUMDX_FLAG_BUILDS] += 1u;
       flags[RUMDX_FLAG_EXCL_PENDING] = 1u;
