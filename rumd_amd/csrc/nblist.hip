@@ -54,6 +54,22 @@ struct SortScratch {
 constexpr uint32_t kArriveGroups = 8;
 constexpr uint32_t kArriveStride = 32;  // u32 per 128-byte line
 
+// Counts the calling block's arrival on the grouped counters `arrive`
+// (thread 0 only); true for the block that arrives last in the launch.  The
+// counters are left at zero for the next launch that uses them.
+__device__ __forceinline__ bool grouped_arrival_last(uint32_t* arrive) {
+  const uint32_t g = blockIdx.x % kArriveGroups;
+  const uint32_t members = (gridDim.x - g + kArriveGroups - 1) / kArriveGroups;
+  const uint32_t groups = gridDim.x < kArriveGroups ? gridDim.x : kArriveGroups;
+  uint32_t* ctr = arrive + g * kArriveStride;
+  if (__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != members - 1) return false;
+  __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t* total = arrive + kArriveGroups * kArriveStride;
+  if (__hip_atomic_fetch_add(total, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != groups - 1) return false;
+  __hip_atomic_store(total, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
 // Grid cap of the particle-parallel rebuild kernels (they loop over the rest).
 constexpr uint32_t kChainGridCap = 1024;
 
@@ -983,19 +999,7 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
   // Arrival at the start (grouped counters, kArriveGroups): its round trip
   // overlaps the halo loads, and the block that arrives last (every block has
   // read the request by then) completes the build at its end.
-  bool last = false;
-  if (threadIdx.x == 0) {
-    const uint32_t g = blockIdx.x % kArriveGroups;
-    const uint32_t members = (gridDim.x - g + kArriveGroups - 1) / kArriveGroups;
-    const uint32_t groups = gridDim.x < kArriveGroups ? gridDim.x : kArriveGroups;
-    uint32_t* ctr = s.arrive + g * kArriveStride;
-    if (__hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == members - 1) {
-      __hip_atomic_store(ctr, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      uint32_t* total = s.arrive + kArriveGroups * kArriveStride;
-      last = __hip_atomic_fetch_add(total, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == groups - 1;
-      if (last) __hip_atomic_store(total, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  const bool last = threadIdx.x == 0 && grouped_arrival_last(s.arrive);
   __shared__ HaloLds L;
   __shared__ float s_cut[RUMDX_MAX_TYPES * RUMDX_MAX_TYPES];
   __shared__ __align__(16) uint32_t s_key[kOrderMax];  // list lengths, then sort keys (same index, same thread)
@@ -1139,15 +1143,18 @@ __global__ __launch_bounds__(kBuildBlock) void k_build_tiles(uint32_t* __restric
 // halo was not staged (their force kernel rescans cells and skips exclusions
 // there) and overflowed lists are left alone.  A completed build raises
 // flags[EXCL_PENDING]; without it every block returns at once (no atomics on
-// the many idle steps).  With it, every block arrives at the completion
-// counter and the last one clears the flag, so no block can miss it.
+// the many idle steps).  With it, every block counts its arrival on the
+// build's grouped counters and the last one clears the flag, so no block can
+// miss it.
 __global__ __launch_bounds__(kBlock) void k_apply_exclusions(uint32_t* __restrict__ flags,
                                                              const uint4* __restrict__ tile_info,
                                                              const uint32_t* __restrict__ tile_halo, uint32_t halo_cap,
                                                              Exclusions ex, uint16_t* __restrict__ nbl16,
                                                              uint32_t* __restrict__ num_nbrs, uint32_t nvp,
-                                                             uint32_t max_nbrs) {
+                                                             uint32_t max_nbrs, uint32_t* __restrict__ arrive) {
   if (__hip_atomic_load(flags + RUMDX_FLAG_EXCL_PENDING, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) return;
+  // arrival at the start, on the grouped counters (the build left them at zero)
+  const bool last = threadIdx.x == 0 && grouped_arrival_last(arrive);
   {
     const uint32_t t = blockIdx.x;
     const uint4 info = tile_info[t];
@@ -1171,7 +1178,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_exclusions(uint32_t* __restric
       num_nbrs[i] = kept;
     }
   }
-  if (last_to_arrive(flags + RUMDX_FLAG_DONE, gridDim.x) && threadIdx.x == 0) flags[RUMDX_FLAG_EXCL_PENDING] = 0u;
+  if (last) flags[RUMDX_FLAG_EXCL_PENDING] = 0u;
 }
 
 // Converts tile-format entries (halo offsets) back to slots.
@@ -1491,7 +1498,7 @@ void launch_tile_build(rumdx_particles* p, rumdx_nblist* nb, const Box& bx, cons
     hipLaunchKernelGGL(k_apply_exclusions, dim3(nb->ntiles), dim3(kBlock), 0, st, nb->flags_dev,
                        reinterpret_cast<const uint4*>(nb->tile_info_dev), nb->tile_halo_dev, nb->halo_cap,
                        Exclusions{p->unsorted_index_dev, nb->excl_start_dev, nb->excl_ids_dev}, nb->nbl16_dev,
-                       nb->num_nbrs_dev, p->nvp, nb->max_nbrs);
+                       nb->num_nbrs_dev, p->nvp, nb->max_nbrs, s.arrive);
 }
 
 }  // namespace
